@@ -1,0 +1,167 @@
+/*
+ * otg.h -- C ABI of libotg, the MI355X (gfx950) static linear ortho-tree.
+ *
+ * Plain C types only: pointers, sizes, status codes.  No exceptions cross this
+ * boundary.  Every entry point returns an otg_status (0 = OK).
+ *
+ * The reference (attcs/Octree, header-only C++20) has no FFI; its seam is the
+ * Core template parameter of OrthoTreeQueryBase (core/ot_query.h:98-99) and the
+ * static-tree aliases (core/ot_aliases.h:82-91, :237, :240).  Each entry point
+ * below names the reference member it replaces.  The C++ drop-in
+ * (include/otg/orthotree_gpu.hpp) maps these onto the reference's class and
+ * method names.
+ *
+ * Geometry layouts (all FP64, row-major "AoS", identical to a
+ * std::span<const std::array<double, D>> / BoundingBoxND<D> span):
+ *   point   : D doubles
+ *   box     : D min doubles, then D max doubles
+ *   ray     : D origin doubles, then D direction doubles
+ * Entity ids are the positions in the caller's span (ContiguousIndex strategy,
+ * core/entity_adapter.h:170-174); in a forest they are local to each set.
+ */
+#ifndef OTG_OTG_H
+#define OTG_OTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define OTG_API __attribute__((visibility("default")))
+#else
+#define OTG_API
+#endif
+
+typedef int otg_status;
+#define OTG_OK 0
+#define OTG_ERR_INVALID_ARG 1   /* bad dimension, cap == 0, depth out of range, NULL handle */
+#define OTG_ERR_HIP 2           /* a HIP runtime call failed */
+#define OTG_ERR_OOM 3           /* device allocation failed (reference: std::bad_alloc) */
+#define OTG_ERR_UNSUPPORTED 4   /* input the GPU path does not mirror (see otg_status_string) */
+#define OTG_ERR_OUT_OF_SPACE 5  /* a point outside an explicit box: the reference Build never ends */
+#define OTG_ERR_KEY_OVERFLOW 6  /* set-id + location bits of a forest exceed 64 */
+#define OTG_ERR_NO_DEVICE 7     /* no HIP device visible */
+
+#define OTG_MEM_HOST 0   /* pointer is host memory */
+#define OTG_MEM_DEVICE 1 /* pointer is device memory of the context's device */
+
+#define OTG_KIND_POINT 0 /* StaticOrthoTreePointND: loose 1.0, MinPoint geometry */
+#define OTG_KIND_BOX 1   /* StaticOrthoTreeBoxND<D, true>: loose 2.0, MBR geometry */
+
+#define OTG_RANGE_INSIDE 0  /* RangeSearchMode::Inside  (core/types.h) */
+#define OTG_RANGE_OVERLAP 1 /* RangeSearchMode::Overlap */
+
+typedef struct otg_ctx otg_ctx;
+typedef struct otg_tree otg_tree;
+typedef struct otg_result otg_result;
+
+typedef struct {
+  int32_t kind;        /* OTG_KIND_* */
+  int32_t dim;         /* D */
+  uint32_t nsets;      /* 1 for a tree built with otg_tree_build */
+  uint32_t reserved;
+  uint64_t entity_count;
+  uint64_t node_count; /* over all sets */
+  uint64_t device_bytes;
+} otg_tree_info;
+
+typedef struct {
+  uint64_t entity_offset; /* first entity of the set in the forest input */
+  uint64_t entity_count;
+  uint64_t node_offset;   /* first node of the set in the forest node table */
+  uint64_t node_count;    /* GetNodeCount() of the set's tree */
+  uint32_t max_depth;     /* GetMaxDepthID() */
+  uint32_t max_elements;  /* GetMaxElementNum() */
+  double space_min[16];   /* tree box (GridSpaceIndexing box space) */
+  double space_max[16];
+} otg_set_info;
+
+/* ---- context ------------------------------------------------------------ */
+/* One HIP device + one default-priority stream + a caching device allocator. */
+OTG_API otg_status otg_ctx_create(int device, otg_ctx **out);
+OTG_API void otg_ctx_destroy(otg_ctx *ctx);
+OTG_API otg_status otg_ctx_synchronize(otg_ctx *ctx);
+OTG_API const char *otg_status_string(otg_status status);
+/* Last error detail text of this context (empty if none). */
+OTG_API const char *otg_ctx_last_error(const otg_ctx *ctx);
+/* Device memory held by the context's allocator cache, and a cache release. */
+OTG_API otg_status otg_ctx_mem_stats(const otg_ctx *ctx, uint64_t *bytes_in_use, uint64_t *bytes_cached);
+OTG_API otg_status otg_ctx_release_cache(otg_ctx *ctx);
+/* Per-kernel timing with HIP events on the context stream (for benchmarks). */
+OTG_API otg_status otg_ctx_set_profiling(otg_ctx *ctx, int enable);
+/* Copies up to cap entries; returns the number of distinct kernels seen. */
+OTG_API otg_status otg_ctx_kernel_times(otg_ctx *ctx, uint32_t cap, char *names /* cap x 64 */, double *total_ms,
+                                        uint64_t *launches, uint32_t *count);
+OTG_API otg_status otg_ctx_reset_kernel_times(otg_ctx *ctx);
+
+/* ---- build -------------------------------------------------------------- */
+/* StaticLinearOrthoTreeCore::Create<ParExec> (core/ot_static_linear_core.h:523-599)
+ * and the PAR_EXEC constructor (:118-129).  box_or_null: 2*dim doubles (host) or
+ * NULL for the entities' bounding box (internal_geometry_module.h:342-370);
+ * max_depth 0 = EstimateMaxDepth (detail/utils.h:613-622). */
+OTG_API otg_status otg_tree_build(otg_ctx *ctx, int kind, int dim, uint64_t n, const double *geom, int mem,
+                                  const double *box_or_null, uint32_t max_depth, uint32_t max_elements, otg_tree **out);
+
+/* Batched build: nsets independent trees of uneven sizes built together.
+ * set_offsets: nsets+1 host values (entity ranges in geom); boxes_or_null:
+ * nsets x 2*dim host doubles; max_depths_or_null: nsets (0 = estimate);
+ * max_elements: nsets.  Each set is the tree otg_tree_build would make. */
+OTG_API otg_status otg_forest_build(otg_ctx *ctx, int kind, int dim, uint32_t nsets, const uint64_t *set_offsets,
+                                    const double *geom, int mem, const double *boxes_or_null,
+                                    const uint32_t *max_depths_or_null, const uint32_t *max_elements, otg_tree **out);
+
+OTG_API void otg_tree_free(otg_tree *tree);
+OTG_API otg_status otg_tree_get_info(const otg_tree *tree, otg_tree_info *out);
+OTG_API otg_status otg_tree_get_set_info(const otg_tree *tree, uint32_t set, otg_set_info *out);
+
+/* Node table of one set (for parity and the node API, :178-268).  Any pointer
+ * may be NULL.  Node ids are BFS ranks local to the set; children of node i are
+ * [child_begin[i], child_begin[i] + child_count[i]) (GetNodeChildren :191-202);
+ * entity segment [entity_begin, +entity_count) indexes entity_ids
+ * (GetNodeEntities :204-212, m_entityStorage).  node_min / node_size: node_count
+ * x dim (GetNodeMinPoint / GetNodeSize).  node_key: Morton node key with
+ * sentinel bit (si_morton.h:216-226).  sorted_keys: location codes of the
+ * entity storage order (code << dbits | depth, dbits = 0 for points). */
+OTG_API otg_status otg_tree_export(const otg_tree *tree, uint32_t set, uint8_t *node_depth, uint32_t *entity_begin,
+                                   uint32_t *entity_count, uint32_t *child_begin, uint32_t *child_count,
+                                   uint64_t *node_key, double *node_min, double *node_size, uint32_t *entity_ids,
+                                   uint64_t *sorted_keys);
+
+/* ---- queries ------------------------------------------------------------ */
+/* Results are CSR: offsets[nq+1] (uint64) and ids (uint32, local entity ids).
+ * set_ids: per-query set of a forest (NULL = set 0); same memory space as the
+ * queries. */
+
+/* OrthoTreeQueryBase::GetNearestNeighbors(point, k, maxDistanceWithin, entities,
+ * tolerance) (core/ot_query.h:1208-1270).  max_within = DBL_MAX for the
+ * overload without a distance limit.  Ids sorted by ascending distance. */
+OTG_API otg_status otg_knn(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
+                           uint32_t k, double max_within, double tolerance, otg_result **out);
+
+/* RangeSearch(range, entities, mode, tolerance) (core/ot_query.h:476-583). */
+OTG_API otg_status otg_range(const otg_tree *tree, uint64_t nq, const double *ranges, const uint32_t *set_ids, int mem,
+                             int mode, double tolerance, otg_result **out);
+
+/* RayIntersectedFirst(origin, direction, entities, tolerance, toleranceIncrement,
+ * maxDistance) (core/ot_query.h:2035-2146).  Ids in descending enter distance. */
+OTG_API otg_status otg_ray_first(const otg_tree *tree, uint64_t nq, const double *rays, const uint32_t *set_ids, int mem,
+                                 double tolerance, double tolerance_increment, double max_distance, otg_result **out);
+
+/* CollisionDetection<ParExec>(entities, tolerance) of one set of a box tree
+ * (core/ot_query.h:1901-1906).  Result: nq = pair count, ids = 2*pairs
+ * (i < j), sorted by (i, j). */
+OTG_API otg_status otg_collide(const otg_tree *tree, uint32_t set, double tolerance, otg_result **out);
+
+OTG_API otg_status otg_result_get_info(const otg_result *res, uint64_t *nq, uint64_t *nids);
+OTG_API otg_status otg_result_copy(const otg_result *res, uint64_t *offsets, uint32_t *ids);
+/* Device-resident CSR (valid until otg_result_free). */
+OTG_API otg_status otg_result_device(const otg_result *res, const uint64_t **offsets, const uint32_t **ids);
+OTG_API void otg_result_free(otg_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OTG_OTG_H */

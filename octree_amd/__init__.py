@@ -1,0 +1,359 @@
+"""octree_amd -- Python host mirror of OrthoTree's static-tree Core API over libotg.
+
+The product is the C ABI in include/otg/otg.h (and the C++20 drop-in
+include/otg/orthotree_gpu.hpp).  This module mirrors the reference's class and
+method names (core/ot_aliases.h:82-91, :225-245; core/ot_query.h) for Python
+callers, tests and the benchmark.  Every call runs on the GPU through libotg;
+there is no CPU fallback.
+
+Geometry is numpy float64: points (n, D); boxes (n, 2, D) or (n, 2*D) as
+[min..., max...]; rays (n, 2*D) as [origin..., direction...].
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import OtgError
+
+BASE_TOLERANCE = float(np.finfo(np.float64).eps * 100.0)  # adapters/general.h:57
+DBL_MAX = float(np.finfo(np.float64).max)
+
+PAR_EXEC = "PAR_EXEC"  # types.h:117-119 execution tags (the GPU path is the parallel one)
+SEQ_EXEC = "SEQ_EXEC"
+GPU_EXEC = PAR_EXEC
+
+
+def _check(status: int, what: str, ctx: Optional["Context"] = None) -> None:
+    if status != _abi.OTG_OK:
+        detail = ""
+        if ctx is not None and ctx.handle:
+            detail = (_abi.lib().otg_ctx_last_error(ctx.handle) or b"").decode()
+        raise OtgError(status, what, detail)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One HIP device + stream (otg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        L = _abi.lib()
+        h = C.c_void_p()
+        st = L.otg_ctx_create(device, C.byref(h))
+        if st != _abi.OTG_OK:
+            raise OtgError(st, "otg_ctx_create")
+        self.handle = h
+
+    def synchronize(self) -> None:
+        _check(_abi.lib().otg_ctx_synchronize(self.handle), "otg_ctx_synchronize", self)
+
+    def set_profiling(self, on: bool) -> None:
+        _check(_abi.lib().otg_ctx_set_profiling(self.handle, 1 if on else 0), "set_profiling", self)
+
+    def kernel_times(self) -> dict:
+        L = _abi.lib()
+        cap = 128
+        names = C.create_string_buffer(cap * 64)
+        ms = (C.c_double * cap)()
+        nl = (C.c_uint64 * cap)()
+        cnt = C.c_uint32()
+        _check(L.otg_ctx_kernel_times(self.handle, cap, names, ms, nl, C.byref(cnt)), "kernel_times", self)
+        out = {}
+        for i in range(min(cnt.value, cap)):
+            nm = names.raw[i * 64:(i + 1) * 64].split(b"\0", 1)[0].decode()
+            out[nm] = (ms[i], nl[i])
+        return out
+
+    def reset_kernel_times(self) -> None:
+        _check(_abi.lib().otg_ctx_reset_kernel_times(self.handle), "reset_kernel_times", self)
+
+    def mem_stats(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(_abi.lib().otg_ctx_mem_stats(self.handle, C.byref(a), C.byref(b)), "mem_stats", self)
+        return a.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _abi.lib().otg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Result:
+    """CSR result (otg_result): offsets (nq+1,) uint64, ids uint32."""
+
+    def __init__(self, handle, ctx: Context):
+        self.handle = handle
+        self.ctx = ctx
+
+    def csr(self):
+        L = _abi.lib()
+        nq, nids = C.c_uint64(), C.c_uint64()
+        _check(L.otg_result_get_info(self.handle, C.byref(nq), C.byref(nids)), "result_get_info", self.ctx)
+        offsets = np.empty(nq.value + 1, dtype=np.uint64)
+        ids = np.empty(max(nids.value, 1), dtype=np.uint32)
+        _check(L.otg_result_copy(self.handle, _ptr(offsets), _ptr(ids)), "result_copy", self.ctx)
+        return offsets, ids[: nids.value]
+
+    def lists(self):
+        off, ids = self.csr()
+        return [ids[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
+
+    def free(self):
+        if self.handle:
+            _abi.lib().otg_result_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _f64(a, cols: int) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    return a.reshape(-1, cols) if a.size else a.reshape(0, cols)
+
+
+class StaticTree:
+    """Static linear ortho-tree resident in HBM (one set) or a forest of sets.
+
+    Mirrors StaticLinearOrthoTreeCore + OrthoTreeQueryBase (core/ot_static_linear_core.h,
+    core/ot_query.h).  kind: 'point' or 'box'.
+    """
+
+    KIND = "point"
+    DIM = 3
+
+    def __init__(self, exec_tag=None, entities=None, max_depth: Optional[int] = None,
+                 box_space=None, max_elements: int = 16, *, ctx: Optional[Context] = None,
+                 dim: Optional[int] = None, kind: Optional[str] = None):
+        self.ctx = ctx or default_context()
+        self.kind = kind or self.KIND
+        self.dim = dim or self.DIM
+        self.handle = None
+        if isinstance(exec_tag, str) or exec_tag is None:
+            ents = entities
+        else:  # constructor without an exec tag: (entities, depth, box, maxElem)
+            ents = exec_tag
+        if ents is not None:
+            ok = self.Create(ents, max_depth, box_space, max_elements)
+            assert ok
+
+    # -- build ------------------------------------------------------------
+    def _cols(self) -> int:
+        return self.dim if self.kind == "point" else 2 * self.dim
+
+    def Create(self, entities, max_depth: Optional[int] = None, box_space=None,
+               max_elements: int = 16, exec_tag=PAR_EXEC) -> bool:
+        """StaticLinearOrthoTreeCore::Create<ParExec> (ot_static_linear_core.h:523-599)."""
+        self.free()
+        geom = _f64(entities, self._cols())
+        box = None if box_space is None else _f64(box_space, 2 * self.dim).reshape(-1)
+        h = C.c_void_p()
+        L = _abi.lib()
+        kind = _abi.OTG_KIND_POINT if self.kind == "point" else _abi.OTG_KIND_BOX
+        st = L.otg_tree_build(self.ctx.handle, kind, self.dim, geom.shape[0], _ptr(geom), _abi.OTG_MEM_HOST,
+                              _ptr(box), int(max_depth or 0), int(max_elements), C.byref(h))
+        _check(st, "otg_tree_build", self.ctx)
+        self.handle = h
+        self.n = geom.shape[0]
+        return True
+
+    @classmethod
+    def forest(cls, sets: Sequence, max_elements, max_depths=None, boxes=None, *, ctx=None, dim=None, kind=None,
+               device_geom=None):
+        """Batched build of many independent trees (otg_forest_build)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        self.kind = kind or cls.KIND
+        self.dim = dim or cls.DIM
+        self.handle = None
+        cols = self._cols()
+        arrs = [_f64(s, cols) for s in sets]
+        ns = len(arrs)
+        offs = np.zeros(ns + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([a.shape[0] for a in arrs])
+        geom = np.concatenate(arrs) if arrs else np.zeros((0, cols))
+        caps = np.ascontiguousarray(np.broadcast_to(np.asarray(max_elements, dtype=np.uint32), (ns,)))
+        deps = None if max_depths is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(max_depths, dtype=np.uint32), (ns,)))
+        bxs = None if boxes is None else _f64(boxes, 2 * self.dim)
+        h = C.c_void_p()
+        kindc = _abi.OTG_KIND_POINT if self.kind == "point" else _abi.OTG_KIND_BOX
+        st = _abi.lib().otg_forest_build(self.ctx.handle, kindc, self.dim, ns,
+                                         offs.ctypes.data_as(_abi.U64P), _ptr(geom), _abi.OTG_MEM_HOST, _ptr(bxs),
+                                         _ptr(deps), caps.ctypes.data_as(_abi.U32P), C.byref(h))
+        _check(st, "otg_forest_build", self.ctx)
+        self.handle = h
+        self.n = geom.shape[0]
+        return self
+
+    def free(self):
+        if getattr(self, "handle", None):
+            _abi.lib().otg_tree_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # -- node API / export ------------------------------------------------
+    def info(self) -> _abi.TreeInfo:
+        ti = _abi.TreeInfo()
+        _check(_abi.lib().otg_tree_get_info(self.handle, C.byref(ti)), "tree_get_info", self.ctx)
+        return ti
+
+    def set_info(self, s: int = 0) -> _abi.SetInfo:
+        si = _abi.SetInfo()
+        _check(_abi.lib().otg_tree_get_set_info(self.handle, s, C.byref(si)), "tree_get_set_info", self.ctx)
+        return si
+
+    def GetNodeCount(self, s: int = 0) -> int:
+        return int(self.set_info(s).node_count)
+
+    def GetMaxDepthID(self, s: int = 0) -> int:
+        return int(self.set_info(s).max_depth)
+
+    def export(self, s: int = 0) -> dict:
+        si = self.set_info(s)
+        nn, ne, D = int(si.node_count), int(si.entity_count), self.dim
+        out = {
+            "depth": np.empty(nn, np.uint8), "ebegin": np.empty(nn, np.uint32), "ecount": np.empty(nn, np.uint32),
+            "cbegin": np.empty(nn, np.uint32), "ccount": np.empty(nn, np.uint32), "key": np.empty(nn, np.uint64),
+            "min": np.empty((nn, D), np.float64), "size": np.empty((nn, D), np.float64),
+            "ids": np.empty(ne, np.uint32), "sorted_keys": np.empty(ne, np.uint64),
+        }
+        st = _abi.lib().otg_tree_export(self.handle, s, *[_ptr(out[k]) for k in
+                                                          ("depth", "ebegin", "ecount", "cbegin", "ccount", "key",
+                                                           "min", "size", "ids", "sorted_keys")])
+        _check(st, "otg_tree_export", self.ctx)
+        out["max_depth"] = int(si.max_depth)
+        out["space_min"] = np.array(si.space_min[:D])
+        out["space_max"] = np.array(si.space_max[:D])
+        return out
+
+    # -- batched queries (CSR) -------------------------------------------------
+    def _sets_arg(self, set_ids, nq):
+        if set_ids is None:
+            return None
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(set_ids, dtype=np.uint32), (nq,)))
+
+    def knn_batch(self, points, k: int, max_within: float = DBL_MAX, tolerance: float = BASE_TOLERANCE,
+                  set_ids=None) -> Result:
+        q = _f64(points, self.dim)
+        sids = self._sets_arg(set_ids, q.shape[0])
+        h = C.c_void_p()
+        st = _abi.lib().otg_knn(self.handle, q.shape[0], _ptr(q), _ptr(sids), _abi.OTG_MEM_HOST, int(k),
+                                float(max_within), float(tolerance), C.byref(h))
+        _check(st, "otg_knn", self.ctx)
+        return Result(h, self.ctx)
+
+    def range_batch(self, ranges, mode: int = _abi.OTG_RANGE_INSIDE, tolerance: float = BASE_TOLERANCE,
+                    set_ids=None) -> Result:
+        q = _f64(ranges, 2 * self.dim)
+        sids = self._sets_arg(set_ids, q.shape[0])
+        h = C.c_void_p()
+        st = _abi.lib().otg_range(self.handle, q.shape[0], _ptr(q), _ptr(sids), _abi.OTG_MEM_HOST, int(mode),
+                                  float(tolerance), C.byref(h))
+        _check(st, "otg_range", self.ctx)
+        return Result(h, self.ctx)
+
+    def ray_first_batch(self, rays, tolerance: float = BASE_TOLERANCE, tolerance_increment: float = 0.0,
+                        max_distance: float = DBL_MAX, set_ids=None) -> Result:
+        q = _f64(rays, 2 * self.dim)
+        sids = self._sets_arg(set_ids, q.shape[0])
+        h = C.c_void_p()
+        st = _abi.lib().otg_ray_first(self.handle, q.shape[0], _ptr(q), _ptr(sids), _abi.OTG_MEM_HOST,
+                                      float(tolerance), float(tolerance_increment), float(max_distance), C.byref(h))
+        _check(st, "otg_ray_first", self.ctx)
+        return Result(h, self.ctx)
+
+    def collide(self, tolerance: float = BASE_TOLERANCE, s: int = 0) -> np.ndarray:
+        h = C.c_void_p()
+        _check(_abi.lib().otg_collide(self.handle, s, float(tolerance), C.byref(h)), "otg_collide", self.ctx)
+        r = Result(h, self.ctx)
+        _, ids = r.csr()
+        r.free()
+        return ids.reshape(-1, 2)
+
+    # -- reference-named single queries (ot_query.h) ----------------------
+    def GetNearestNeighbors(self, point, k: int, *args, tolerance: float = BASE_TOLERANCE):
+        """GetNearestNeighbors(point, k, entities, tol) or (point, k, maxDistanceWithin, entities, tol)."""
+        max_within = DBL_MAX
+        rest = list(args)
+        if rest and np.isscalar(rest[0]):
+            max_within = float(rest.pop(0))
+        if rest:
+            rest.pop(0)  # entities span (resident on the device already)
+        if rest:
+            tolerance = float(rest.pop(0))
+        return self.knn_batch([point], k, max_within, tolerance).lists()[0]
+
+    def RangeSearch(self, box, entities=None, mode: int = _abi.OTG_RANGE_INSIDE, tolerance: float = BASE_TOLERANCE):
+        return self.range_batch([np.asarray(box, dtype=np.float64).reshape(-1)], mode, tolerance).lists()[0]
+
+    def RayIntersectedFirst(self, origin, direction, entities=None, tolerance: float = BASE_TOLERANCE,
+                            tolerance_increment: float = 0.0, max_distance: float = DBL_MAX):
+        ray = np.concatenate([np.asarray(origin, np.float64).reshape(-1), np.asarray(direction, np.float64).reshape(-1)])
+        return self.ray_first_batch([ray], tolerance, tolerance_increment, max_distance).lists()[0]
+
+    def CollisionDetection(self, entities=None, tolerance: float = BASE_TOLERANCE, exec_tag=PAR_EXEC):
+        return [tuple(p) for p in self.collide(tolerance).tolist()]
+
+
+def _make(kind: str, dim: int, name: str):
+    return type(name, (StaticTree,), {"KIND": kind, "DIM": dim})
+
+
+# core/ot_aliases.h:225-245
+StaticQuadtreePoint = _make("point", 2, "StaticQuadtreePoint")
+StaticQuadtreeBox = _make("box", 2, "StaticQuadtreeBox")
+StaticOctreePoint = _make("point", 3, "StaticOctreePoint")
+StaticOctreeBox = _make("box", 3, "StaticOctreeBox")
+StaticHexatreePoint = _make("point", 4, "StaticHexatreePoint")
+StaticHexatreeBox = _make("box", 4, "StaticHexatreeBox")
+
+
+def StaticOrthoTreePointND(dim: int):
+    """core/ot_aliases.h:82-85"""
+    return _make("point", dim, f"StaticOrthoTreePointND_{dim}")
+
+
+def StaticOrthoTreeBoxND(dim: int, loose: bool = True):
+    """core/ot_aliases.h:88-91 (loose trees only: LOOSE_FACTOR 2.0)"""
+    if not loose:
+        raise NotImplementedError("non-loose box trees are outside the GPU path")
+    return _make("box", dim, f"StaticOrthoTreeBoxND_{dim}")
+
+
+__all__ = [
+    "Context", "Result", "StaticTree", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
+    "StaticQuadtreePoint", "StaticQuadtreeBox", "StaticOctreePoint", "StaticOctreeBox", "StaticHexatreePoint",
+    "StaticHexatreeBox", "StaticOrthoTreePointND", "StaticOrthoTreeBoxND", "default_context",
+]

@@ -1,0 +1,448 @@
+// otg_abi.hip -- the extern "C" boundary of libotg: contexts, trees, results.
+// No C++ exception crosses this boundary; every failure becomes a status code.
+#include <cfloat>
+#include <cstdio>
+
+#include "otg_internal.hpp"
+
+namespace otg {
+
+// ---- caching allocator ----------------------------------------------------
+static size_t round_size(size_t b) {
+  if (b <= 256) return 256;
+  // round to 1/8 of the next power of two to bound waste
+  size_t p = 1;
+  while (p < b) p <<= 1;
+  size_t step = p >> 3;
+  return (b + step - 1) / step * step;
+}
+
+void *Ctx::alloc(size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  size_t sz = round_size(bytes);
+  std::lock_guard<std::mutex> g(mu);
+  auto it = free_blocks.lower_bound(sz);
+  if (it != free_blocks.end() && it->first <= sz + sz / 4) {
+    void *p = it->second;
+    size_t have = it->first;
+    free_blocks.erase(it);
+    live_blocks[p] = have;
+    bytes_cached -= have;
+    bytes_in_use += have;
+    return p;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    // release the cache and retry once
+    for (auto &kv : free_blocks) (void)hipFree(kv.second);
+    free_blocks.clear();
+    bytes_cached = 0;
+    e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error{OTG_ERR_OOM, "hipMalloc of " + std::to_string(sz) + " bytes failed"};
+    }
+  }
+  live_blocks[p] = sz;
+  bytes_in_use += sz;
+  return p;
+}
+
+void Ctx::free(void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = live_blocks.find(p);
+  if (it == live_blocks.end()) return;
+  size_t sz = it->second;
+  live_blocks.erase(it);
+  bytes_in_use -= sz;
+  // blocks are reused by later work on the same stream (stream order keeps it safe)
+  free_blocks.emplace(sz, p);
+  bytes_cached += sz;
+}
+
+void Ctx::release_cache() {
+  std::lock_guard<std::mutex> g(mu);
+  (void)hipStreamSynchronize(stream);
+  for (auto &kv : free_blocks) (void)hipFree(kv.second);
+  free_blocks.clear();
+  bytes_cached = 0;
+}
+
+hipEvent_t Ctx::get_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  OTG_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::begin_kernel(const char *name, hipEvent_t *start) {
+  *start = get_event();
+  OTG_HIP_CHECK(hipEventRecord(*start, stream));
+}
+
+void Ctx::end_kernel(const char *name, hipEvent_t start) {
+  hipEvent_t stop = get_event();
+  OTG_HIP_CHECK(hipEventRecord(stop, stream));
+  pending_events.push_back({name, {start, stop}});
+}
+
+void Ctx::collect_times() {
+  if (pending_events.empty()) return;
+  OTG_HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto &pe : pending_events) {
+    float ms = 0.f;
+    OTG_HIP_CHECK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    KTime &kt = ktimes[pe.first];
+    kt.ms += ms;
+    kt.launches += 1;
+    event_pool.push_back(pe.second.first);
+    event_pool.push_back(pe.second.second);
+  }
+  pending_events.clear();
+}
+
+Tree::~Tree() {
+  if (!ctx) return;
+  ctx->free(d_sets);
+  ctx->free(d_ids);
+  ctx->free(d_keys);
+  ctx->free(d_geom);
+  ctx->free(d_info);
+  ctx->free(d_rend);
+  ctx->free(d_depth);
+  ctx->free(d_nkey);
+  ctx->free(d_nmin);
+  ctx->free(d_nsize);
+}
+
+}  // namespace otg
+
+using otg::Ctx;
+using otg::Error;
+
+struct otg_ctx : public Ctx {};
+struct otg_tree : public otg::Tree {};
+struct otg_result : public otg::Result {};
+
+namespace {
+
+template <typename F>
+otg_status guarded(Ctx *ctx, F &&f) {
+  try {
+    f();
+    if (ctx) ctx->last_error.clear();
+    return OTG_OK;
+  } catch (const Error &e) {
+    if (ctx) ctx->last_error = e.msg;
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    if (ctx) ctx->last_error = "host allocation failed";
+    return OTG_ERR_OOM;
+  } catch (...) {
+    if (ctx) ctx->last_error = "unknown error";
+    return OTG_ERR_HIP;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+OTG_API const char *otg_status_string(otg_status s) {
+  switch (s) {
+    case OTG_OK: return "ok";
+    case OTG_ERR_INVALID_ARG: return "invalid argument";
+    case OTG_ERR_HIP: return "HIP runtime error";
+    case OTG_ERR_OOM: return "out of device memory";
+    case OTG_ERR_UNSUPPORTED: return "unsupported input";
+    case OTG_ERR_OUT_OF_SPACE: return "point outside the given tree box";
+    case OTG_ERR_KEY_OVERFLOW: return "key bits exceed 64";
+    case OTG_ERR_NO_DEVICE: return "no HIP device";
+  }
+  return "unknown status";
+}
+
+OTG_API otg_status otg_ctx_create(int device, otg_ctx **out) {
+  if (!out) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return OTG_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) return OTG_ERR_INVALID_ARG;
+  otg_ctx *c = new (std::nothrow) otg_ctx();
+  if (!c) return OTG_ERR_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return OTG_ERR_HIP;
+  }
+  *out = c;
+  return OTG_OK;
+}
+
+OTG_API void otg_ctx_destroy(otg_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &pe : ctx->pending_events) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
+  for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+OTG_API otg_status otg_ctx_synchronize(otg_ctx *ctx) {
+  if (!ctx) return OTG_ERR_INVALID_ARG;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+OTG_API const char *otg_ctx_last_error(const otg_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+OTG_API otg_status otg_ctx_mem_stats(const otg_ctx *ctx, uint64_t *in_use, uint64_t *cached) {
+  if (!ctx) return OTG_ERR_INVALID_ARG;
+  if (in_use) *in_use = ctx->bytes_in_use;
+  if (cached) *cached = ctx->bytes_cached;
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_ctx_release_cache(otg_ctx *ctx) {
+  if (!ctx) return OTG_ERR_INVALID_ARG;
+  return guarded(ctx, [&] { ctx->release_cache(); });
+}
+
+OTG_API otg_status otg_ctx_set_profiling(otg_ctx *ctx, int enable) {
+  if (!ctx) return OTG_ERR_INVALID_ARG;
+  ctx->profiling = enable != 0;
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_ctx_kernel_times(otg_ctx *ctx, uint32_t cap, char *names, double *total_ms, uint64_t *launches,
+                                        uint32_t *count) {
+  if (!ctx) return OTG_ERR_INVALID_ARG;
+  return guarded(ctx, [&] {
+    ctx->collect_times();
+    uint32_t i = 0;
+    for (auto &kv : ctx->ktimes) {
+      if (i < cap) {
+        if (names) {
+          std::snprintf(names + (size_t)i * 64, 64, "%s", kv.first.c_str());
+        }
+        if (total_ms) total_ms[i] = kv.second.ms;
+        if (launches) launches[i] = kv.second.launches;
+      }
+      ++i;
+    }
+    if (count) *count = i;
+  });
+}
+
+OTG_API otg_status otg_ctx_reset_kernel_times(otg_ctx *ctx) {
+  if (!ctx) return OTG_ERR_INVALID_ARG;
+  return guarded(ctx, [&] {
+    ctx->collect_times();
+    ctx->ktimes.clear();
+  });
+}
+
+OTG_API otg_status otg_forest_build(otg_ctx *ctx, int kind, int dim, uint32_t nsets, const uint64_t *set_offsets,
+                                    const double *geom, int mem, const double *boxes_or_null,
+                                    const uint32_t *max_depths_or_null, const uint32_t *max_elements, otg_tree **out) {
+  if (!ctx || !out || !set_offsets || !max_elements || nsets == 0) return OTG_ERR_INVALID_ARG;
+  if (kind != OTG_KIND_POINT && kind != OTG_KIND_BOX) return OTG_ERR_INVALID_ARG;
+  if (dim < 2 || dim > otg::kMaxDim) return OTG_ERR_INVALID_ARG;
+  if (mem != OTG_MEM_HOST && mem != OTG_MEM_DEVICE) return OTG_ERR_INVALID_ARG;
+  for (uint32_t s = 0; s < nsets; ++s)
+    if (set_offsets[s + 1] < set_offsets[s]) return OTG_ERR_INVALID_ARG;
+  if (set_offsets[nsets] && !geom) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  otg_tree *t = new (std::nothrow) otg_tree();
+  if (!t) return OTG_ERR_OOM;
+  t->ctx = ctx;
+  t->kind = kind;
+  t->dim = dim;
+  otg_status st = guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::build_forest(t, geom, mem, set_offsets, nsets, boxes_or_null, max_depths_or_null, max_elements);
+  });
+  if (st != OTG_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    delete t;
+    return st;
+  }
+  *out = t;
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_tree_build(otg_ctx *ctx, int kind, int dim, uint64_t n, const double *geom, int mem,
+                                  const double *box_or_null, uint32_t max_depth, uint32_t max_elements, otg_tree **out) {
+  uint64_t offs[2] = {0, n};
+  return otg_forest_build(ctx, kind, dim, 1, offs, geom, mem, box_or_null, max_depth ? &max_depth : nullptr,
+                          &max_elements, out);
+}
+
+OTG_API void otg_tree_free(otg_tree *t) {
+  if (!t) return;
+  if (t->ctx) (void)hipStreamSynchronize(t->ctx->stream);
+  delete t;
+}
+
+OTG_API otg_status otg_tree_get_info(const otg_tree *t, otg_tree_info *out) {
+  if (!t || !out) return OTG_ERR_INVALID_ARG;
+  out->kind = t->kind;
+  out->dim = t->dim;
+  out->nsets = (uint32_t)t->sets.size();
+  out->reserved = 0;
+  out->entity_count = t->n;
+  out->node_count = t->node_count;
+  out->device_bytes = t->device_bytes;
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_tree_get_set_info(const otg_tree *t, uint32_t set, otg_set_info *out) {
+  if (!t || !out || set >= t->sets.size()) return OTG_ERR_INVALID_ARG;
+  const otg::SetParams &sp = t->sets[set];
+  out->entity_offset = sp.ent_off;
+  out->entity_count = sp.ent_cnt;
+  out->node_offset = sp.node_off;
+  out->node_count = sp.node_cnt;
+  out->max_depth = sp.max_depth;
+  out->max_elements = sp.cap;
+  for (int d = 0; d < 16; ++d) {
+    out->space_min[d] = d < t->dim ? sp.space_min[d] : 0.0;
+    out->space_max[d] = d < t->dim ? sp.space_max[d] : 0.0;
+  }
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_tree_export(const otg_tree *t, uint32_t set, uint8_t *node_depth, uint32_t *entity_begin,
+                                   uint32_t *entity_count, uint32_t *child_begin, uint32_t *child_count,
+                                   uint64_t *node_key, double *node_min, double *node_size, uint32_t *entity_ids,
+                                   uint64_t *sorted_keys) {
+  if (!t || set >= t->sets.size()) return OTG_ERR_INVALID_ARG;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    const otg::SetParams &sp = t->sets[set];
+    const uint64_t nn = sp.node_cnt, no = sp.node_off, ne = sp.ent_cnt, eo = sp.ent_off;
+    const int D = t->dim;
+    hipStream_t st = ctx->stream;
+    std::vector<otg::NodeInfo> info(nn);
+    if (nn) OTG_HIP_CHECK(hipMemcpyAsync(info.data(), t->d_info + no, nn * sizeof(otg::NodeInfo), hipMemcpyDeviceToHost, st));
+    if (node_depth && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_depth, t->d_depth + no, nn, hipMemcpyDeviceToHost, st));
+    if (node_key && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_key, t->d_nkey + no, nn * 8, hipMemcpyDeviceToHost, st));
+    if (node_min && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_min, t->d_nmin + no * D, nn * D * 8, hipMemcpyDeviceToHost, st));
+    if (node_size && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_size, t->d_nsize + no * D, nn * D * 8, hipMemcpyDeviceToHost, st));
+    if (entity_ids && ne) OTG_HIP_CHECK(hipMemcpyAsync(entity_ids, t->d_ids + eo, ne * 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint64_t> keys;
+    if (sorted_keys && ne) {
+      keys.resize(ne);
+      OTG_HIP_CHECK(hipMemcpyAsync(keys.data(), t->d_keys + eo, ne * 8, hipMemcpyDeviceToHost, st));
+    }
+    OTG_HIP_CHECK(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < nn; ++i) {
+      if (entity_begin) entity_begin[i] = (uint32_t)(info[i].ebegin - eo);
+      if (entity_count) entity_count[i] = info[i].ecount;
+      if (child_begin) child_begin[i] = info[i].ccount ? (uint32_t)(info[i].cbegin - no) : 0u;
+      if (child_count) child_count[i] = info[i].ccount;
+    }
+    if (sorted_keys) {
+      const otg::KeyLayout &L = t->layout;
+      const uint64_t mask = (L.cbits + L.dbits >= 64) ? ~0ull : ((1ull << (L.cbits + L.dbits)) - 1);
+      for (uint64_t i = 0; i < ne; ++i) sorted_keys[i] = keys[i] & mask;
+    }
+  });
+}
+
+OTG_API otg_status otg_knn(const otg_tree *t, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
+                           uint32_t k, double max_within, double tolerance, otg_result **out) {
+  if (!t || !out || (nq && !points)) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(otg::query_knn(t, nq, points, set_ids, mem, k, max_within, tolerance));
+  });
+}
+
+OTG_API otg_status otg_range(const otg_tree *t, uint64_t nq, const double *ranges, const uint32_t *set_ids, int mem,
+                             int mode, double tolerance, otg_result **out) {
+  if (!t || !out || (nq && !ranges)) return OTG_ERR_INVALID_ARG;
+  if (mode != OTG_RANGE_INSIDE && mode != OTG_RANGE_OVERLAP) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(otg::query_range(t, nq, ranges, set_ids, mem, mode, tolerance));
+  });
+}
+
+OTG_API otg_status otg_ray_first(const otg_tree *t, uint64_t nq, const double *rays, const uint32_t *set_ids, int mem,
+                                 double tolerance, double tolerance_increment, double max_distance, otg_result **out) {
+  if (!t || !out || (nq && !rays)) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(
+        otg::query_ray(t, nq, rays, set_ids, mem, tolerance, tolerance_increment, max_distance));
+  });
+}
+
+OTG_API otg_status otg_collide(const otg_tree *t, uint32_t set, double tolerance, otg_result **out) {
+  if (!t || !out) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(otg::query_collide(t, set, tolerance));
+  });
+}
+
+OTG_API otg_status otg_result_get_info(const otg_result *r, uint64_t *nq, uint64_t *nids) {
+  if (!r) return OTG_ERR_INVALID_ARG;
+  if (nq) *nq = r->nq;
+  if (nids) *nids = r->nids;
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_result_copy(const otg_result *r, uint64_t *offsets, uint32_t *ids) {
+  if (!r) return OTG_ERR_INVALID_ARG;
+  Ctx *ctx = r->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    if (offsets) OTG_HIP_CHECK(hipMemcpyAsync(offsets, r->d_offsets, (r->nq + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (ids && r->nids) OTG_HIP_CHECK(hipMemcpyAsync(ids, r->d_ids, r->nids * 4, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+OTG_API otg_status otg_result_device(const otg_result *r, const uint64_t **offsets, const uint32_t **ids) {
+  if (!r) return OTG_ERR_INVALID_ARG;
+  if (offsets) *offsets = r->d_offsets;
+  if (ids) *ids = r->d_ids;
+  return OTG_OK;
+}
+
+OTG_API void otg_result_free(otg_result *r) {
+  if (!r) return;
+  delete r;
+}
+
+}  // extern "C"

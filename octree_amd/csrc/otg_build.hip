@@ -1,0 +1,931 @@
+// otg_build.hip -- one-launch-per-stage build of OrthoTree's static linear
+// tree (core/ot_static_linear_core.h:523-599) for one tree or a forest of many
+// independent trees.
+//
+// The reference builds its LCA-compressed node table with a serial BFS queue
+// (Build<true> :417-519).  Here the same table is derived from boundary levels
+// of the sorted location codes, with data-parallel passes only:
+//
+//  1. encode     location key per entity (set | code | depth), id payload
+//  2. sort       LDS-ranked LSD radix sort over the key bits in use
+//  3. levels     v[i] = 1 + L_i for the boundary between sorted entities i-1, i,
+//                where L_i = min(common Morton digits, depth(i-1), depth(i));
+//                v = 0 at set boundaries.  The node that boundary i splits is
+//                the maximal run around it with interior levels >= L_i, of
+//                depth L_i.
+//  4. active     boundary i is a split of the final tree iff that run has more
+//                than `cap` entities, L_i < maxDepth and (boxes) it is not made
+//                of identical keys only (the "all non-refinable" leaf, :465-472).
+//                Decided with a bounded window walk over v.
+//  5. scans      running-minimum level sets to both sides of every boundary
+//                give each node's number of ancestors = its BFS generation.
+//  6. nodes      internal node = leftmost active boundary of its run; leaf =
+//                segment between consecutive active boundaries, unless it is a
+//                single non-refinable box of its parent.  BFS id = rank by
+//                (set, generation, position): a radix sort of node keys.
+//  7. finalize   entity segments, children ranges (binary search in the sorted
+//                node keys), MinPoint geometry (:346-368) / MBR (:280-327).
+#include <algorithm>
+#include <cmath>
+
+#include "otg_internal.hpp"
+
+namespace otg {
+
+constexpr int kB = 256;
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dmax_std(double a, double b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ double dmin_std(double a, double b) { return (b < a) ? b : a; }  // std::min
+
+__device__ __forceinline__ uint32_t set_of(const uint64_t *set_off, uint32_t nsets, uint64_t i) {
+  if (nsets == 1) return 0;
+  uint32_t lo = 0, hi = nsets;  // find last s with set_off[s] <= i
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (set_off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// si_morton.h:498-574: bit i of g[d] -> bit i*D + d
+template <int D>
+__device__ __forceinline__ uint64_t morton_encode(const uint32_t (&g)[D]) {
+  uint64_t code = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    uint64_t x = g[d];
+    for (int i = 0; i < 32 && (i * D + d) < 64; ++i) code |= ((x >> i) & 1ull) << (i * D + d);
+  }
+  return code;
+}
+
+__device__ __forceinline__ uint32_t bw64(uint64_t x) { return x ? 64u - (uint32_t)__clzll(x) : 0u; }
+
+// common leading Morton digits of two codes of a depth-M tree (LCA depth)
+template <int D>
+__device__ __forceinline__ uint32_t lcp_digits(uint64_t a, uint64_t b, uint32_t M) {
+  uint64_t x = a ^ b;
+  if (!x) return M;
+  return M - (bw64(x) + D - 1) / D;
+}
+
+struct KeyView {
+  uint32_t dbits, cbits;
+  __device__ uint64_t code(uint64_t k) const { return (k >> dbits) & ((cbits >= 64) ? ~0ull : ((1ull << cbits) - 1)); }
+  __device__ uint32_t depth(uint64_t k, uint32_t M) const {
+    return dbits ? (uint32_t)(k & ((1ull << dbits) - 1)) : M;
+  }
+  __device__ uint64_t set_base(uint32_t s) const { return (cbits + dbits >= 64) ? 0ull : ((uint64_t)s << (cbits + dbits)); }
+};
+
+// ---------------------------------------------------------------------------
+// 0. bounding box (internal_geometry_module.h:342-370), exact min/max through
+// order-preserving integer images of doubles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ord_bits(double x) {
+  uint64_t u = (uint64_t)__double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_bbox(const double *__restrict__ geom, uint64_t begin, uint64_t end,
+                                              unsigned long long *__restrict__ out /* 2D: min ords, max ords */) {
+  constexpr int S = BOX ? 2 * D : D;
+  uint64_t lo[D], hi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = ~0ull;
+    hi[d] = 0ull;
+  }
+  for (uint64_t i = begin + (uint64_t)blockIdx.x * kB + threadIdx.x; i < end; i += (uint64_t)gridDim.x * kB) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      uint64_t a = ord_bits(geom[i * S + d]);
+      uint64_t b = ord_bits(geom[i * S + (BOX ? D + d : d)]);
+      lo[d] = a < lo[d] ? a : lo[d];
+      hi[d] = b > hi[d] ? b : hi[d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    for (int off = 32; off > 0; off >>= 1) {
+      uint64_t a = __shfl_xor(lo[d], off, 64);
+      uint64_t b = __shfl_xor(hi[d], off, 64);
+      lo[d] = a < lo[d] ? a : lo[d];
+      hi[d] = b > hi[d] ? b : hi[d];
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      atomicMin(&out[d], (unsigned long long)lo[d]);
+      atomicMax(&out[D + d], (unsigned long long)hi[d]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1. encode
+// ---------------------------------------------------------------------------
+// error bits
+constexpr uint32_t kErrOutOfSpace = 1u;
+constexpr uint32_t kErrUnsupported = 2u;
+
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, uint64_t n,
+                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                const SetParams *__restrict__ sets, KeyView kv,
+                                                uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                uint32_t *__restrict__ err) {
+  uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = set_of(set_off, nsets, i);
+  const SetParams &sp = sets[s];
+  const uint32_t M = sp.max_depth;
+  uint32_t g[D];
+  uint64_t code = 0;
+  uint32_t depth = M;
+  if (!BOX) {
+    // si_grid.h:145-175 GetPointGridID
+    const uint32_t max_raster_id = (uint32_t)((1ull << M) - 1);
+    bool out = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double c = geom[i * D + d] - sp.space_min[d];
+      if (c < -kBaseTol || c > sp.size[d] + kBaseTol) out = true;
+      uint32_t r = 0;
+      if (c > 0) {
+        double x = c * sp.raster[d];
+        r = (x >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)x;
+        r = r < max_raster_id ? r : max_raster_id;
+      }
+      g[d] = r;
+    }
+    if (out) {
+      atomicOr(err, kErrOutOfSpace);
+      code = 0;
+      depth = 0;
+    } else {
+      code = morton_encode<D>(g);
+    }
+  } else {
+    // si_grid.h:221-285 GetLooseBoxGridData<2.0>, si_mortongrid.h:78-97
+    const double *b = &geom[i * 2 * D];
+    bool inside = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double rmin = sp.space_min[d], rmax = sp.space_max[d];
+      double bmin = b[d], bmax = b[D + d];
+      if (rmin - kBaseTol > bmin || bmin > rmax + kBaseTol || rmin - kBaseTol > bmax || bmax > rmax + kBaseTol)
+        inside = false;
+    }
+    if (!inside) {
+      code = 0;
+      depth = 0;
+    } else {
+      double max_rel = 0.0;
+      bool bad = false;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double center = (b[d] + b[D + d]) * 0.5;
+        double x = (center - sp.space_min[d]) * sp.raster[d];
+        uint32_t gc = (x <= 0.0) ? 0u : ((x >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)x);
+        if (gc >> M) bad = true;
+        g[d] = gc;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double sz = b[D + d] - b[d];
+        max_rel = dmax_std(max_rel, sz * sp.raster[d]);
+      }
+      double cx = ceil(max_rel);
+      uint32_t mrg = (cx >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)cx;
+      uint32_t level = mrg ? bw64((uint64_t)(mrg - 1)) : 0u;
+      if (level > M) bad = true;
+      if (bad) {
+        atomicOr(err, kErrUnsupported);
+        level = 0;
+      }
+      code = morton_encode<D>(g);
+      uint32_t sh = level * D;
+      code = (code >> sh) << sh;
+      depth = M - level;
+    }
+  }
+  keys[i] = kv.set_base(s) | (code << kv.dbits) | (BOX ? (uint64_t)depth : 0ull);
+  vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// 3. boundary levels
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kB) k_levels(const uint64_t *__restrict__ keys, uint64_t n,
+                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                SetParams *__restrict__ sets, KeyView kv, uint8_t *__restrict__ v) {
+  uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i > n) return;
+  uint32_t val = 0;
+  uint32_t s = 0;
+  bool interior = false;
+  if (i > 0 && i < n) {
+    s = set_of(set_off, nsets, i);
+    if (sets[s].ent_off != i) {
+      interior = true;
+      const uint32_t M = sets[s].max_depth;
+      uint64_t ka = keys[i - 1], kb = keys[i];
+      uint32_t L = lcp_digits<D>(kv.code(ka), kv.code(kb), M);
+      uint32_t da = kv.depth(ka, M), db = kv.depth(kb, M);
+      L = L < da ? L : da;
+      L = L < db ? L : db;
+      val = L + 1;
+    }
+  }
+  v[i] = (uint8_t)val;
+  // per-set minimum interior level (coded), wave-aggregated
+  const uint64_t act = __ballot(interior);
+  if (!act) return;
+  const int lane = threadIdx.x & 63;
+  uint32_t s0 = __shfl(s, __ffsll((long long)act) - 1, 64);
+  bool same = __all(!interior || s == s0);
+  if (same) {
+    uint32_t m = interior ? val : 255u;
+    for (int off = 32; off > 0; off >>= 1) {
+      uint32_t o = __shfl_xor(m, off, 64);
+      m = o < m ? o : m;
+    }
+    if (lane == __ffsll((long long)act) - 1) atomicMin(&sets[s0].min_lcp, m);
+  } else if (interior) {
+    atomicMin(&sets[s].min_lcp, val);
+  }
+}
+
+// per-set: does the root split?  (count > cap, and for boxes not every entity at
+// depth 0 with one key: ot_static_linear_core.h:441-472)
+template <bool BOX>
+__global__ void k_root_flags(const uint64_t *__restrict__ keys, SetParams *__restrict__ sets, uint32_t nsets, KeyView kv) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsets) return;
+  SetParams &sp = sets[s];
+  uint32_t splits = 0;
+  if (sp.ent_cnt > sp.cap) {
+    splits = 1;
+    if (BOX) {
+      uint64_t kf = keys[sp.ent_off], kl = keys[sp.ent_off + sp.ent_cnt - 1];
+      if (kf == kl && kv.depth(kf, sp.max_depth) == 0) splits = 0;
+    }
+  }
+  sp.root_splits = splits;
+}
+
+// ---------------------------------------------------------------------------
+// 4. active boundaries
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t *a, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (lo < hi) {
+    uint64_t mid = lo + ((hi - lo) >> 1);
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint64_t upper_bound_u64(const uint64_t *a, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (lo < hi) {
+    uint64_t mid = lo + ((hi - lo) >> 1);
+    if (a[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool BOX>
+__global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ v,
+                                                uint64_t n, const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                const SetParams *__restrict__ sets, uint8_t *__restrict__ active) {
+  uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i > n) return;
+  uint8_t a = 0;
+  const uint32_t vi = (i > 0 && i < n) ? v[i] : 0u;
+  if (vi) {
+    const uint32_t s = set_of(set_off, nsets, i);
+    const SetParams &sp = sets[s];
+    const uint32_t L = vi - 1;
+    if (L < sp.max_depth) {
+      const uint64_t cap = sp.cap;
+      uint64_t cnt = 2;
+      // walk left over boundaries i-1, i-2, ... and right over i+1, ...
+      uint64_t j = i - 1;
+      while (cnt <= cap && v[j] >= vi) {
+        ++cnt;
+        --j;
+      }
+      j = i + 1;
+      while (cnt <= cap && v[j] >= vi) {
+        ++cnt;
+        ++j;
+      }
+      if (cnt > cap) {
+        a = 1;
+        if (BOX && keys[i - 1] == keys[i]) {
+          // run of one repeated key at depth L: a leaf holding only
+          // non-refinable entities unless the run extends beyond the key block
+          const uint64_t sb = sp.ent_off, se = sp.ent_off + sp.ent_cnt;
+          uint64_t lo = lower_bound_u64(keys, sb, i, keys[i]);
+          uint64_t hi = upper_bound_u64(keys, i, se, keys[i]);
+          if (v[lo] < vi && v[hi] < vi) a = 0;
+        }
+      }
+    }
+  }
+  active[i] = a;
+}
+
+// ---------------------------------------------------------------------------
+// 5/6. node events
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t internal_gen(const uint32_t *lm, const uint32_t *rm, const uint8_t *v, uint64_t p) {
+  const uint32_t vp = v[p];
+  if (vp <= 1) return 0;  // the root (level 0)
+  uint32_t anc = (lm[p - 1] | rm[p + 1]) & ((1u << vp) - 1u) & ~3u;
+  return (uint32_t)__popc(anc) + 1u;
+}
+
+// flags per position p in [0, n): seg_flag (segment start), int_count
+// (extra root + internal representative)
+__global__ void __launch_bounds__(kB) k_event_flags(const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
+                                                     const uint32_t *__restrict__ lm, uint64_t n,
+                                                     const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                     const SetParams *__restrict__ sets, uint32_t *__restrict__ seg_flag,
+                                                     uint32_t *__restrict__ int_flag) {
+  uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t s = set_of(set_off, nsets, p);
+  const SetParams &sp = sets[s];
+  const bool set_start = (sp.ent_off == p);
+  uint32_t ic = 0;
+  if (set_start && sp.root_splits && sp.min_lcp > 1u) ic = 1;  // extra root above a deeper single child
+  if (!set_start && active[p]) {
+    const uint32_t vp = v[p];
+    if (!((lm[p - 1] >> vp) & 1u)) ++ic;  // leftmost boundary of its node
+  }
+  seg_flag[p] = (set_start || active[p]) ? 1u : 0u;
+  int_flag[p] = ic;
+}
+
+__global__ void __launch_bounds__(kB) k_seg_compact(const uint32_t *__restrict__ seg_flag, const uint32_t *__restrict__ seg_idx,
+                                                     uint64_t n, uint32_t *__restrict__ seg_list, uint32_t nseg) {
+  uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (p < n && seg_flag[p]) seg_list[seg_idx[p]] = (uint32_t)p;
+  if (p == 0) seg_list[nseg] = (uint32_t)n;
+}
+
+// Which segments are nodes (leaves): all except single non-refinable boxes.
+template <bool BOX>
+__global__ void __launch_bounds__(kB) k_seg_nodes(const uint32_t *__restrict__ seg_list, uint32_t nseg,
+                                                   const uint8_t *__restrict__ v, const uint64_t *__restrict__ keys,
+                                                   const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                   const SetParams *__restrict__ sets, KeyView kv,
+                                                   uint32_t *__restrict__ seg_node) {
+  uint32_t k = blockIdx.x * kB + threadIdx.x;
+  if (k >= nseg) return;
+  uint32_t node = 1;
+  if (BOX) {
+    uint64_t s = seg_list[k], e = seg_list[k + 1];
+    if (e - s == 1) {
+      uint32_t st = set_of(set_off, nsets, s);
+      uint32_t M = sets[st].max_depth;
+      uint32_t vs = (sets[st].ent_off == s) ? 0u : v[s];
+      uint32_t ve = v[e];
+      uint32_t pv = vs > ve ? vs : ve;
+      if (pv >= 1 && kv.depth(keys[s], M) + 1 == pv) node = 0;
+    }
+  }
+  seg_node[k] = node;
+}
+
+struct NodeKeyLayout {
+  uint32_t cobits;  // coordinate bits
+  uint32_t gbits;   // generation bits
+  __device__ uint64_t make(uint32_t s, uint32_t gen, uint64_t coord) const {
+    return ((uint64_t)s << (gbits + cobits)) | ((uint64_t)gen << cobits) | coord;
+  }
+};
+
+constexpr uint32_t kTypeRoot = 0u, kTypeInternal = 1u, kTypeLeaf = 2u;
+
+__global__ void __launch_bounds__(kB) k_emit_internal(const uint32_t *__restrict__ int_flag, const uint32_t *__restrict__ int_idx,
+                                                       const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
+                                                       const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
+                                                       uint64_t n, const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                       const SetParams *__restrict__ sets, NodeKeyLayout nk,
+                                                       uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
+  uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (p >= n || !int_flag[p]) return;
+  const uint32_t s = set_of(set_off, nsets, p);
+  const SetParams &sp = sets[s];
+  uint32_t out = int_idx[p];
+  const bool set_start = (sp.ent_off == p);
+  if (set_start && sp.root_splits && sp.min_lcp > 1u) {
+    nkeys[out] = nk.make(s, 0, 0);
+    ndesc[out] = (kTypeRoot << 30) | (uint32_t)p;
+    ++out;
+  }
+  if (!set_start && active[p] && !((lm[p - 1] >> v[p]) & 1u)) {
+    uint32_t gen = internal_gen(lm, rm, v, p);
+    nkeys[out] = nk.make(s, gen, 2 * (p - sp.ent_off) - 1);
+    ndesc[out] = (kTypeInternal << 30) | (uint32_t)p;
+  }
+}
+
+__global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__ seg_list, uint32_t nseg,
+                                                     const uint32_t *__restrict__ seg_node, const uint32_t *__restrict__ seg_node_idx,
+                                                     uint32_t out_base, const uint8_t *__restrict__ v,
+                                                     const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
+                                                     const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                     const SetParams *__restrict__ sets, NodeKeyLayout nk,
+                                                     uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
+  uint32_t k = blockIdx.x * kB + threadIdx.x;
+  if (k >= nseg || !seg_node[k]) return;
+  uint64_t s0 = seg_list[k], e = seg_list[k + 1];
+  const uint32_t s = set_of(set_off, nsets, s0);
+  const SetParams &sp = sets[s];
+  const bool starts = (sp.ent_off == s0);
+  const bool ends = (sp.ent_off + sp.ent_cnt == e);
+  uint32_t gen;
+  if (starts && ends) {
+    gen = sp.root_splits ? 1u : 0u;
+  } else {
+    uint32_t vs = starts ? 0u : v[s0];
+    uint32_t ve = ends ? 0u : v[e];
+    uint64_t pb = (vs >= ve) ? s0 : e;
+    gen = internal_gen(lm, rm, v, pb) + 1u;
+  }
+  uint32_t out = out_base + seg_node_idx[k];
+  nkeys[out] = nk.make(s, gen, 2 * (s0 - sp.ent_off));
+  ndesc[out] = (kTypeLeaf << 30) | k;
+}
+
+// node offsets per set: first node whose key has set id >= s
+__global__ void k_set_node_offsets(const uint64_t *__restrict__ nkeys, uint64_t nnodes, SetParams *__restrict__ sets,
+                                   uint32_t nsets, NodeKeyLayout nk) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsets) return;
+  uint64_t lo = lower_bound_u64(nkeys, 0, nnodes, nk.make(s, 0, 0));
+  uint64_t hi = (s + 1 == nsets) ? nnodes : lower_bound_u64(nkeys, 0, nnodes, nk.make(s + 1, 0, 0));
+  sets[s].node_off = lo;
+  sets[s].node_cnt = hi - lo;
+}
+
+// ---------------------------------------------------------------------------
+// 7. finalize
+// ---------------------------------------------------------------------------
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nkeys, const uint32_t *__restrict__ ndesc,
+                                                  uint64_t nnodes, const uint64_t *__restrict__ keys,
+                                                  const uint8_t *__restrict__ v, const uint32_t *__restrict__ seg_list,
+                                                  const SetParams *__restrict__ sets, KeyView kv, NodeKeyLayout nk,
+                                                  NodeInfo *__restrict__ info, uint32_t *__restrict__ rend,
+                                                  uint8_t *__restrict__ ndepth, uint64_t *__restrict__ nodekey,
+                                                  double *__restrict__ nmin, double *__restrict__ nsize) {
+  uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nnodes) return;
+  const uint64_t key = nkeys[x];
+  const uint32_t s = (uint32_t)(key >> (nk.gbits + nk.cobits));
+  const uint32_t gen = (uint32_t)((key >> nk.cobits) & ((1u << nk.gbits) - 1));
+  const SetParams &sp = sets[s];
+  const uint32_t M = sp.max_depth;
+  const uint32_t desc = ndesc[x];
+  const uint32_t type = desc >> 30;
+  const uint32_t pos = desc & 0x3FFFFFFFu;
+  const uint64_t sb = sp.ent_off, se = sp.ent_off + sp.ent_cnt;
+  uint64_t a, b;
+  uint32_t nr = 0, depth;
+  uint64_t pivot_code;
+  bool internal = true;
+  if (type == kTypeRoot) {
+    a = sb;
+    b = se;
+    depth = 0;
+    pivot_code = 0;
+  } else if (type == kTypeInternal) {
+    const uint64_t p = pos;
+    const uint32_t L = (uint32_t)v[p] - 1u;
+    const uint32_t r = (M - L) * D;  // bits below the node's digits
+    const uint64_t code = kv.code(keys[p]);
+    const uint64_t P = (r >= 64) ? 0ull : (code >> r);
+    const uint64_t lo_code = (r >= 64) ? 0ull : (P << r);
+    const uint64_t hi_code = (P + 1) << r;
+    const uint64_t lo_key = kv.set_base(s) | (lo_code << kv.dbits) | (BOX ? (uint64_t)L : 0ull);
+    a = lower_bound_u64(keys, sb, p, lo_key);
+    if (hi_code == (1ull << (D * M))) {
+      b = se;  // the prefix is the last one of the set
+    } else {
+      const uint64_t hi_key = kv.set_base(s) + (hi_code << kv.dbits);
+      b = lower_bound_u64(keys, p, se, hi_key);
+    }
+    if (BOX && keys[a] == lo_key) nr = (uint32_t)(upper_bound_u64(keys, a, b, lo_key) - a);
+    depth = L;
+    pivot_code = kv.code(keys[a]);
+  } else {
+    internal = false;
+    a = seg_list[pos];
+    b = seg_list[pos + 1];
+    const uint64_t ka = keys[a];
+    pivot_code = kv.code(ka);
+    if (a == sb && b == se && !sp.root_splits) {
+      depth = 0;  // the root is a leaf
+    } else {
+      uint32_t L = lcp_digits<D>(pivot_code, kv.code(keys[b - 1]), M);
+      uint32_t da = kv.depth(ka, M);
+      depth = L < da ? L : da;
+    }
+  }
+  NodeInfo ni;
+  ni.ebegin = (uint32_t)a;
+  ni.ecount = internal ? nr : (uint32_t)(b - a);
+  if (internal) {
+    const uint64_t lo = nk.make(s, gen + 1, 2 * (a - sb));
+    const uint64_t hi = nk.make(s, gen + 1, 2 * (b - sb));
+    const uint64_t c0 = lower_bound_u64(nkeys, x + 1, nnodes, lo);
+    const uint64_t c1 = lower_bound_u64(nkeys, c0, nnodes, hi);
+    ni.cbegin = (uint32_t)c0;
+    ni.ccount = (uint32_t)(c1 - c0);
+  } else {
+    ni.cbegin = 0;
+    ni.ccount = 0;
+  }
+  info[x] = ni;
+  rend[x] = (uint32_t)b;
+  ndepth[x] = (uint8_t)depth;
+  // si_morton.h:216-226 node key of the LCA location (pivot code, depth)
+  const uint32_t sh = (M - depth) * D;
+  const uint64_t nkey = (1ull << (depth * D)) | ((sh >= 64) ? 0ull : (pivot_code >> sh));
+  nodekey[x] = nkey;
+#pragma unroll
+  for (int d = 0; d < D; ++d) nsize[x * D + d] = sp.depth_size[(BOX ? 0 : depth) * D + d];
+  if (!BOX) {
+    // InitNodeGeometry (MinPoint): decode the key, cell centre minus half size
+    uint32_t g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = 0;
+    uint32_t shift = 0;
+    for (uint32_t lvl = M - depth; lvl < M; ++lvl)
+#pragma unroll
+      for (int d = 0; d < D; ++d, ++shift) g[d] |= (uint32_t)((nkey >> shift) & 1ull) << lvl;
+    const double half_grid = (double)(1ull << (M - depth)) * 0.5;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double center = ((double)g[d] + half_grid) * sp.deraster[d] + sp.space_min[d];
+      nmin[x * D + d] = center - sp.depth_size[(depth + 1) * D + d];
+      nsize[x * D + d] = sp.depth_size[depth * D + d];
+    }
+  }
+}
+
+// sorted SoA geometry and local ids
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_gather(const double *__restrict__ geom, const uint32_t *__restrict__ vals, uint64_t n,
+                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                double *__restrict__ sorted, uint32_t *__restrict__ ids) {
+  uint64_t j = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (j >= n) return;
+  constexpr int S = BOX ? 2 * D : D;
+  uint32_t id = vals[j];
+#pragma unroll
+  for (int c = 0; c < S; ++c) sorted[(uint64_t)c * n + j] = geom[(uint64_t)id * S + c];
+  uint32_t s = set_of(set_off, nsets, id);
+  ids[j] = id - (uint32_t)set_off[s];
+}
+
+// ---------------------------------------------------------------------------
+// MBR, bottom-up with arrival counters (InitializeMinimalNodeGeometry :280-327)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kB) k_parents(const NodeInfo *__restrict__ info, uint64_t nnodes,
+                                                 uint32_t *__restrict__ parent, uint32_t *__restrict__ pending) {
+  uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nnodes) return;
+  NodeInfo ni = info[x];
+  pending[x] = ni.ccount;
+  for (uint32_t c = 0; c < ni.ccount; ++c) parent[ni.cbegin + c] = (uint32_t)x;
+}
+
+template <int D>
+__device__ void mbr_of_node(uint64_t x, const NodeInfo *info, const double *sorted, uint64_t n, double *nmin,
+                            double *nsize) {
+  NodeInfo ni = info[x];
+  double mn[D], mx[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    mn[d] = 1.7976931348623157e308;
+    mx[d] = -1.7976931348623157e308;
+  }
+  for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      mn[d] = dmin_std(mn[d], sorted[(uint64_t)d * n + j]);
+      mx[d] = dmax_std(mx[d], sorted[(uint64_t)(D + d) * n + j]);
+    }
+  }
+  for (uint32_t c = ni.cbegin; c < ni.cbegin + ni.ccount; ++c) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double cmin = nmin[(uint64_t)c * D + d];
+      double csz = nsize[(uint64_t)c * D + d];
+      mn[d] = dmin_std(mn[d], cmin);
+      mx[d] = dmax_std(mx[d], cmin + csz);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    nmin[x * D + d] = mn[d];
+    nsize[x * D + d] = mx[d] - mn[d];
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kB) k_mbr(const NodeInfo *__restrict__ info, uint64_t nnodes, const uint32_t *__restrict__ parent,
+                                             uint32_t *__restrict__ pending, const uint8_t *__restrict__ is_root,
+                                             const double *__restrict__ sorted, uint64_t n, double *nmin, double *nsize) {
+  uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nnodes) return;
+  if (info[x].ccount != 0) return;  // start at leaves
+  for (;;) {
+    mbr_of_node<D>(x, info, sorted, n, nmin, nsize);
+    if (is_root[x]) break;
+    __threadfence();
+    uint32_t p = parent[x];
+    uint32_t before = __hip_atomic_fetch_add(&pending[p], 0xFFFFFFFFu, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (before != 1u) break;
+    __threadfence();
+    x = p;
+  }
+}
+
+__global__ void k_mark_roots(const SetParams *__restrict__ sets, uint32_t nsets, uint8_t *__restrict__ is_root) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsets) return;
+  if (sets[s].node_cnt) is_root[sets[s].node_off] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+uint32_t max_theoretical_depth(int dim) { return dim <= 2 ? (uint32_t)(31 / dim) : (uint32_t)(63 / dim); }
+
+uint32_t estimate_max_depth(uint64_t n, uint64_t cap, int dim) {
+  // detail/utils.h:613-622
+  if (n <= cap) return 2;
+  uint64_t nleaf = n / cap;
+  uint32_t est = (uint32_t)(std::log2((double)nleaf) / (double)dim);
+  uint32_t hi = max_theoretical_depth(dim);
+  return std::clamp<uint32_t>(est, 2u, hi);
+}
+
+static void init_set_space(SetParams &sp, int D, bool box, const double *bmin, const double *bmax) {
+  // si_grid.h:65-103 and ot_base.h:121-134 (host FP64, IEEE, no contraction)
+  volatile double res = (double)(1ull << sp.max_depth);
+  for (int d = 0; d < D; ++d) {
+    sp.space_min[d] = bmin[d];
+    sp.space_max[d] = bmax[d];
+    volatile double sz = bmax[d] - bmin[d];
+    sp.size[d] = sz;
+    sp.deraster[d] = sz / res;
+    sp.raster[d] = (sz == 0) ? 1.0 : (res / sz);
+  }
+  double loose = box ? 2.0 : 1.0;
+  for (uint32_t k = 0; k < sp.max_depth + 2; ++k, loose *= 0.5)
+    for (int d = 0; d < D; ++d) sp.depth_size[k * D + d] = sp.size[d] * loose;
+}
+
+static double ord_to_double(uint64_t u) {
+  uint64_t b = (u >> 63) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u;
+  double x;
+  std::memcpy(&x, &b, 8);
+  return x;
+}
+
+template <int D, bool BOX>
+static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *offsets, uint32_t nsets,
+                       const double *boxes, const uint32_t *depths, const uint32_t *caps) {
+  Ctx *ctx = t->ctx;
+  constexpr int S = BOX ? 2 * D : D;
+  const uint64_t n = offsets[nsets];
+  t->n = n;
+  if (n >= (1ull << 30)) throw Error{OTG_ERR_UNSUPPORTED, "more than 2^30 entities"};
+
+  // entity geometry on the device (input order)
+  Buf<double> dgeom_own;
+  const double *dgeom = geom_in;
+  if (mem == OTG_MEM_HOST) {
+    dgeom_own = Buf<double>(ctx, n * S);
+    if (n) OTG_HIP_CHECK(hipMemcpyAsync(dgeom_own.p, geom_in, n * S * 8, hipMemcpyHostToDevice, ctx->stream));
+    dgeom = dgeom_own.p;
+  }
+
+  // per-set parameters
+  t->sets.assign(nsets, SetParams{});
+  std::vector<uint32_t> need_bbox;
+  uint32_t maxM = 1;
+  uint64_t max_n = 1;
+  for (uint32_t s = 0; s < nsets; ++s) {
+    SetParams &sp = t->sets[s];
+    sp.ent_off = offsets[s];
+    sp.ent_cnt = offsets[s + 1] - offsets[s];
+    sp.cap = caps[s];
+    if (sp.cap == 0) throw Error{OTG_ERR_INVALID_ARG, "max elements per node must be > 0"};
+    uint32_t M = (depths && depths[s]) ? depths[s] : estimate_max_depth(sp.ent_cnt, sp.cap, D);
+    if (M < 1 || M > max_theoretical_depth(D) || M > (uint32_t)kMaxDepth)
+      throw Error{OTG_ERR_INVALID_ARG, "max depth out of range"};
+    sp.max_depth = M;
+    sp.min_lcp = 255;
+    maxM = std::max(maxM, M);
+    max_n = std::max<uint64_t>(max_n, sp.ent_cnt);
+    if (!boxes) need_bbox.push_back(s);
+  }
+  if (!need_bbox.empty()) {
+    Buf<unsigned long long> bb(ctx, (size_t)need_bbox.size() * 2 * D);
+    std::vector<unsigned long long> init(need_bbox.size() * 2 * D);
+    for (size_t q = 0; q < need_bbox.size(); ++q)
+      for (int d = 0; d < D; ++d) {
+        init[q * 2 * D + d] = ~0ull;
+        init[q * 2 * D + D + d] = 0ull;
+      }
+    OTG_HIP_CHECK(hipMemcpyAsync(bb.p, init.data(), init.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    for (size_t q = 0; q < need_bbox.size(); ++q) {
+      const SetParams &sp = t->sets[need_bbox[q]];
+      if (!sp.ent_cnt) continue;
+      uint32_t blocks = std::min<uint64_t>(div_up(sp.ent_cnt, kB), 2048);
+      launch(ctx, "bbox", k_bbox<D, BOX>, dim3(blocks), dim3(kB), 0, dgeom, sp.ent_off, sp.ent_off + sp.ent_cnt,
+             bb.p + q * 2 * D);
+    }
+    OTG_HIP_CHECK(hipMemcpyAsync(init.data(), bb.p, init.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q < need_bbox.size(); ++q) {
+      double bmin[kMaxDim], bmax[kMaxDim];
+      for (int d = 0; d < D; ++d) {
+        bmin[d] = ord_to_double(init[q * 2 * D + d]);
+        bmax[d] = ord_to_double(init[q * 2 * D + D + d]);
+        if (t->sets[need_bbox[q]].ent_cnt == 0) {
+          bmin[d] = 1.7976931348623157e308;
+          bmax[d] = -1.7976931348623157e308;
+        }
+      }
+      init_set_space(t->sets[need_bbox[q]], D, BOX, bmin, bmax);
+    }
+  }
+  if (boxes)
+    for (uint32_t s = 0; s < nsets; ++s) init_set_space(t->sets[s], D, BOX, &boxes[(size_t)s * 2 * D], &boxes[(size_t)s * 2 * D + D]);
+
+  KeyLayout &L = t->layout;
+  L.dbits = BOX ? (32 - __builtin_clz(maxM)) : 0;
+  L.cbits = D * maxM;
+  L.sbits = nsets > 1 ? (32 - __builtin_clz(nsets - 1)) : 0;
+  L.total = L.sbits + L.cbits + L.dbits;
+  if (L.total > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "set id and location bits exceed 64"};
+  KeyView kv{L.dbits, L.cbits};
+
+  Buf<uint64_t> d_off(ctx, nsets + 1);
+  OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, offsets, (nsets + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  t->d_sets = (SetParams *)ctx->alloc(sizeof(SetParams) * nsets);
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_sets, t->sets.data(), sizeof(SetParams) * nsets, hipMemcpyHostToDevice, ctx->stream));
+
+  if (n == 0) {
+    t->node_count = 0;
+    return;
+  }
+
+  // 1. encode
+  Buf<uint32_t> err(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, ctx->stream));
+  t->d_keys = (uint64_t *)ctx->alloc(n * 8);
+  Buf<uint32_t> vals(ctx, n);
+  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, (const uint64_t *)d_off.p, nsets,
+         (const SetParams *)t->d_sets, kv, t->d_keys, vals.p, err.p);
+  uint32_t herr = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+
+  // 2. sort
+  radix_sort_pairs(ctx, t->d_keys, vals.p, n, L.total);
+
+  // 3. levels (+ per-set minimum level), root flags
+  Buf<uint8_t> v(ctx, n + 1);
+  launch(ctx, "levels", k_levels<D>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys, n,
+         (const uint64_t *)d_off.p, nsets, t->d_sets, kv, v.p);
+  launch(ctx, "root_flags", k_root_flags<BOX>, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)t->d_keys,
+         t->d_sets, nsets, kv);
+
+  // 4. active boundaries
+  Buf<uint8_t> active(ctx, n + 1);
+  launch(ctx, "active", k_active<BOX>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys,
+         (const uint8_t *)v.p, n, (const uint64_t *)d_off.p, nsets, (const SetParams *)t->d_sets, active.p);
+
+  // 5. level scans
+  Buf<uint32_t> lm(ctx, n + 1), rm(ctx, n + 1);
+  level_scans(ctx, v.p, n + 1, lm.p, rm.p);
+
+  // 6. node events
+  Buf<uint32_t> seg_flag(ctx, n), int_flag(ctx, n), seg_idx(ctx, n), int_idx(ctx, n);
+  launch(ctx, "event_flags", k_event_flags, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)v.p,
+         (const uint8_t *)active.p, (const uint32_t *)lm.p, n, (const uint64_t *)d_off.p, nsets,
+         (const SetParams *)t->d_sets, seg_flag.p, int_flag.p);
+  Buf<uint32_t> totals(ctx, 4);
+  exclusive_scan_u32(ctx, seg_flag.p, seg_idx.p, n, totals.p + 0);
+  exclusive_scan_u32(ctx, int_flag.p, int_idx.p, n, totals.p + 1);
+  uint32_t ht[2];
+  OTG_HIP_CHECK(hipMemcpyAsync(ht, totals.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (herr & kErrOutOfSpace) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
+  if (herr & kErrUnsupported) throw Error{OTG_ERR_UNSUPPORTED, "box level or grid id beyond the tree depth"};
+  const uint32_t nseg = ht[0], nint = ht[1];
+
+  Buf<uint32_t> seg_list(ctx, (size_t)nseg + 1), seg_node(ctx, nseg), seg_node_idx(ctx, nseg);
+  launch(ctx, "seg_compact", k_seg_compact, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)seg_flag.p,
+         (const uint32_t *)seg_idx.p, n, seg_list.p, nseg);
+  launch(ctx, "seg_nodes", k_seg_nodes<BOX>, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
+         (const uint8_t *)v.p, (const uint64_t *)t->d_keys, (const uint64_t *)d_off.p, nsets,
+         (const SetParams *)t->d_sets, kv, seg_node.p);
+  exclusive_scan_u32(ctx, seg_node.p, seg_node_idx.p, nseg, totals.p + 2);
+  uint32_t nleaf = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&nleaf, totals.p + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  const uint64_t nnodes = (uint64_t)nint + nleaf;
+  t->node_count = nnodes;
+
+  NodeKeyLayout nk;
+  nk.cobits = 64 - __builtin_clzll(2 * max_n + 1);
+  nk.gbits = 5;
+  if (nk.cobits + nk.gbits + L.sbits > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "node key overflow"};
+  Buf<uint64_t> nkeys(ctx, nnodes);
+  Buf<uint32_t> ndesc(ctx, nnodes);
+  launch(ctx, "emit_internal", k_emit_internal, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)int_flag.p,
+         (const uint32_t *)int_idx.p, (const uint8_t *)v.p, (const uint8_t *)active.p, (const uint32_t *)lm.p,
+         (const uint32_t *)rm.p, n, (const uint64_t *)d_off.p, nsets, (const SetParams *)t->d_sets, nk, nkeys.p,
+         ndesc.p);
+  launch(ctx, "emit_leaves", k_emit_leaves, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
+         (const uint32_t *)seg_node.p, (const uint32_t *)seg_node_idx.p, nint, (const uint8_t *)v.p,
+         (const uint32_t *)lm.p, (const uint32_t *)rm.p, (const uint64_t *)d_off.p, nsets,
+         (const SetParams *)t->d_sets, nk, nkeys.p, ndesc.p);
+  radix_sort_pairs(ctx, nkeys.p, ndesc.p, nnodes, nk.cobits + nk.gbits + L.sbits);
+  launch(ctx, "set_node_offsets", k_set_node_offsets, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)nkeys.p,
+         nnodes, t->d_sets, nsets, nk);
+
+  // 7. finalize
+  t->d_info = (NodeInfo *)ctx->alloc(nnodes * sizeof(NodeInfo));
+  t->d_rend = (uint32_t *)ctx->alloc(nnodes * 4);
+  t->d_depth = (uint8_t *)ctx->alloc(nnodes);
+  t->d_nkey = (uint64_t *)ctx->alloc(nnodes * 8);
+  t->d_nmin = (double *)ctx->alloc(nnodes * D * 8);
+  t->d_nsize = (double *)ctx->alloc(nnodes * D * 8);
+  launch(ctx, "finalize", k_finalize<D, BOX>, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const uint64_t *)nkeys.p,
+         (const uint32_t *)ndesc.p, nnodes, (const uint64_t *)t->d_keys, (const uint8_t *)v.p,
+         (const uint32_t *)seg_list.p, (const SetParams *)t->d_sets, kv, nk, t->d_info, t->d_rend, t->d_depth,
+         t->d_nkey, t->d_nmin, t->d_nsize);
+
+  // sorted geometry + local ids
+  t->d_geom = (double *)ctx->alloc(n * S * 8);
+  t->d_ids = (uint32_t *)ctx->alloc(n * 4);
+  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, (const uint32_t *)vals.p, n,
+         (const uint64_t *)d_off.p, nsets, t->d_geom, t->d_ids);
+
+  if (BOX) {
+    Buf<uint32_t> parent(ctx, nnodes), pending(ctx, nnodes);
+    Buf<uint8_t> is_root(ctx, nnodes);
+    OTG_HIP_CHECK(hipMemsetAsync(is_root.p, 0, nnodes, ctx->stream));
+    launch(ctx, "parents", k_parents, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
+           parent.p, pending.p);
+    launch(ctx, "mark_roots", k_mark_roots, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)t->d_sets, nsets,
+           is_root.p);
+    launch(ctx, "mbr", k_mbr<D>, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
+           (const uint32_t *)parent.p, pending.p, (const uint8_t *)is_root.p, (const double *)t->d_geom, n, t->d_nmin,
+           t->d_nsize);
+  }
+  OTG_HIP_CHECK(hipMemcpyAsync(t->sets.data(), t->d_sets, sizeof(SetParams) * nsets, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  t->device_bytes = n * (S * 8 + 4 + 8) + nnodes * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+}
+
+void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets, const double *boxes,
+                  const uint32_t *depths, const uint32_t *caps) {
+  const bool box = t->kind == OTG_KIND_BOX;
+  switch (t->dim) {
+#define OTG_CASE(DD)                                                                    \
+  case DD:                                                                              \
+    if (box) build_impl<DD, true>(t, geom, mem, offsets, nsets, boxes, depths, caps);  \
+    else build_impl<DD, false>(t, geom, mem, offsets, nsets, boxes, depths, caps);     \
+    break;
+    OTG_CASE(2)
+    OTG_CASE(3)
+    OTG_CASE(4)
+    OTG_CASE(5)
+    OTG_CASE(6)
+#undef OTG_CASE
+    default:
+      throw Error{OTG_ERR_INVALID_ARG, "dimension must be 2..6"};
+  }
+}
+
+}  // namespace otg

@@ -1,0 +1,206 @@
+// otg_internal.hpp -- shared host/device declarations of libotg.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/otg/otg.h"
+
+namespace otg {
+
+constexpr int kMaxDim = 6;      // dimensions instantiated on the device
+constexpr int kMaxDepth = 19;   // configuration.h:78 MAX_ALLOWED_DEPTH_ID
+constexpr double kBaseTol = 2.220446049250313080847e-16 * 100.0;  // general.h:57
+
+// Per-set parameters of a forest (a plain tree is a forest of one set).
+struct SetParams {
+  uint64_t ent_off;    // first entity (storage position == input position)
+  uint64_t ent_cnt;
+  uint64_t node_off;   // first node in the forest node table
+  uint64_t node_cnt;
+  uint32_t max_depth;  // M
+  uint32_t cap;        // max elements per node
+  uint32_t root_splits;  // filled by the build
+  uint32_t min_lcp;      // smallest boundary level inside the set (+1 coded)
+  double space_min[kMaxDim];
+  double space_max[kMaxDim];
+  double size[kMaxDim];
+  double raster[kMaxDim];
+  double deraster[kMaxDim];
+  double depth_size[(kMaxDepth + 2) * kMaxDim];  // m_nodeSize per depth
+};
+
+// Key layout of the forest sort: set << (cbits + dbits) | code << dbits | depth
+struct KeyLayout {
+  uint32_t dbits;  // depth bits (boxes) or 0 (points)
+  uint32_t cbits;  // D * max(M)
+  uint32_t sbits;  // bit width of (nsets - 1)
+  uint32_t total;  // sbits + cbits + dbits
+};
+
+// Node record used by every traversal.
+struct NodeInfo {
+  uint32_t ebegin;  // own entity segment (storage positions)
+  uint32_t ecount;
+  uint32_t cbegin;  // children (global node ids)
+  uint32_t ccount;
+};
+
+class Ctx;
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+class Ctx {
+ public:
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  bool profiling = false;
+  struct KTime {
+    double ms = 0;
+    uint64_t launches = 0;
+  };
+  std::map<std::string, KTime> ktimes;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_events;
+  std::vector<hipEvent_t> event_pool;
+
+  // caching allocator: size class -> free blocks
+  std::multimap<size_t, void *> free_blocks;
+  std::map<void *, size_t> live_blocks;
+  uint64_t bytes_in_use = 0, bytes_cached = 0;
+  std::mutex mu;
+
+  void *alloc(size_t bytes);
+  void free(void *p);
+  void release_cache();
+
+  hipEvent_t get_event();
+  void begin_kernel(const char *name, hipEvent_t *start);
+  void end_kernel(const char *name, hipEvent_t start);
+  void collect_times();  // synchronises the stream
+};
+
+// Scoped device buffer from the context allocator.
+template <typename T>
+struct Buf {
+  Ctx *ctx = nullptr;
+  T *p = nullptr;
+  size_t n = 0;
+  Buf() = default;
+  Buf(Ctx *c, size_t count) : ctx(c), n(count) { p = count ? (T *)c->alloc(count * sizeof(T)) : nullptr; }
+  Buf(const Buf &) = delete;
+  Buf &operator=(const Buf &) = delete;
+  Buf(Buf &&o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  Buf &operator=(Buf &&o) noexcept {
+    reset();
+    ctx = o.ctx; p = o.p; n = o.n;
+    o.p = nullptr; o.n = 0;
+    return *this;
+  }
+  ~Buf() { reset(); }
+  void reset() {
+    if (p && ctx) ctx->free(p);
+    p = nullptr;
+    n = 0;
+  }
+  T *release() { T *q = p; p = nullptr; n = 0; return q; }
+};
+
+// Error carrying a status code across the host orchestration (never crosses the ABI).
+struct Error {
+  otg_status status;
+  std::string msg;
+};
+
+#define OTG_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t e__ = (expr);                                                             \
+    if (e__ != hipSuccess)                                                               \
+      throw ::otg::Error{e__ == hipErrorOutOfMemory ? OTG_ERR_OOM : OTG_ERR_HIP,         \
+                         std::string(#expr) + ": " + hipGetErrorString(e__)};            \
+  } while (0)
+
+// Launch with optional event timing.
+template <typename Kernel, typename... Args>
+inline void launch(Ctx *ctx, const char *name, Kernel k, dim3 grid, dim3 block, size_t shmem, Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipEvent_t ev0 = nullptr;
+  if (ctx->profiling) ctx->begin_kernel(name, &ev0);
+  hipLaunchKernelGGL(k, grid, block, shmem, ctx->stream, args...);
+  OTG_HIP_CHECK(hipGetLastError());
+  if (ctx->profiling) ctx->end_kernel(name, ev0);
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---- primitives (otg_primitives.hip) ----
+// Exclusive prefix sum of n uint32 values into out (may alias in); returns total on host if total != nullptr.
+void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total);
+// Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
+// result is in keys/vals on return.
+void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits);
+// Boundary-level scans: for v[0..n) (uint8 levels, 0 = hard boundary) computes
+// left[i] = set of running minima of v[i], v[i-1], ... (inclusive, stops at level 0) and
+// right[i] = same going right, as bit masks.
+void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask);
+
+// ---- tree ----
+struct Tree {
+  Ctx *ctx = nullptr;
+  int kind = 0;
+  int dim = 0;
+  uint64_t n = 0;
+  uint64_t node_count = 0;
+  std::vector<SetParams> sets;
+  KeyLayout layout{};
+  SetParams *d_sets = nullptr;
+  // entity data in storage (sorted) order
+  uint32_t *d_ids = nullptr;      // local entity ids
+  uint64_t *d_keys = nullptr;     // sorted keys
+  double *d_geom = nullptr;       // SoA sorted geometry: comp*n + j; boxes: 2*dim comps (min..., max...)
+  // nodes
+  NodeInfo *d_info = nullptr;
+  uint32_t *d_rend = nullptr;     // subtree entity range end
+  uint8_t *d_depth = nullptr;
+  uint64_t *d_nkey = nullptr;     // node Morton key
+  double *d_nmin = nullptr;       // node_count x dim
+  double *d_nsize = nullptr;      // node_count x dim
+  uint64_t device_bytes = 0;
+
+  ~Tree();
+};
+
+void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets,
+                  const double *boxes, const uint32_t *depths, const uint32_t *caps);
+
+struct Result {
+  Ctx *ctx = nullptr;
+  uint64_t nq = 0;
+  uint64_t nids = 0;
+  uint64_t *d_offsets = nullptr;
+  uint32_t *d_ids = nullptr;
+  ~Result();
+};
+
+Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
+                  double max_within, double tol);
+Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
+                    double tol);
+Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
+                  double inc, double max_dist);
+Result *query_collide(const Tree *t, uint32_t set, double tol);
+
+// host-side mirrors of reference helpers
+uint32_t max_theoretical_depth(int dim);
+uint32_t estimate_max_depth(uint64_t n, uint64_t cap, int dim);
+
+}  // namespace otg

@@ -1,0 +1,319 @@
+// otg_primitives.hip -- scans and the LDS-ranked LSD radix sort used by the build.
+//
+// gfx950 specifics: 64-wide wavefronts; ranks within a wavefront come from
+// ballot-based digit matching (8 ballots per round), per-wavefront digit
+// counters live in LDS, and a 256-thread block (4 wavefronts) owns a tile of
+// 2048 keys.
+#include "otg_internal.hpp"
+
+namespace otg {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;
+
+// ---------------------------------------------------------------------------
+// Generic block scan
+// ---------------------------------------------------------------------------
+struct OpAdd {
+  using T = uint32_t;
+  __device__ static T identity() { return 0u; }
+  __device__ static T combine(T a, T b) { return a + b; }
+  __device__ static T shfl_up(T x, int off) { return __shfl_up(x, off, 64); }
+};
+
+// State of the "running minima" monoid over boundary levels.  For a segment,
+// m = minimum level, mask = set of values taken by the running minimum when the
+// segment is walked from its right end to its left end.  combine(left, right):
+// walk right first, then continue into left where only values below right.m
+// are new minima.
+struct LevelState {
+  uint32_t m;
+  uint32_t mask;
+};
+struct OpLevel {
+  using T = LevelState;
+  __device__ static T identity() { return LevelState{255u, 0u}; }
+  __device__ static uint32_t low_bits(uint32_t m) { return m >= 32u ? 0xFFFFFFFFu : ((1u << m) - 1u); }
+  __device__ static T combine(T a, T b) {
+    T r;
+    r.m = a.m < b.m ? a.m : b.m;
+    r.mask = b.mask | (a.mask & low_bits(b.m));
+    return r;
+  }
+  __device__ static T shfl_up(T x, int off) {
+    T r;
+    r.m = __shfl_up(x.m, off, 64);
+    r.mask = __shfl_up(x.mask, off, 64);
+    return r;
+  }
+};
+
+// Inclusive scan of one value per thread across the block.  Returns the
+// inclusive value; *excl receives the exclusive value; *total the block total.
+template <class Op>
+__device__ typename Op::T block_scan(typename Op::T x, typename Op::T *excl, typename Op::T *total,
+                                     typename Op::T *lds /* kBlock/64 */) {
+  using T = typename Op::T;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  T inc = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T y = Op::shfl_up(inc, off);
+    if (lane >= off) inc = Op::combine(y, inc);
+  }
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  T wpre = Op::identity();
+  T tot = Op::identity();
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) wpre = Op::combine(wpre, lds[w]);
+    tot = Op::combine(tot, lds[w]);
+  }
+  __syncthreads();
+  T prev = Op::shfl_up(inc, 1);
+  T ex = lane == 0 ? wpre : Op::combine(wpre, prev);
+  *excl = ex;
+  *total = tot;
+  return Op::combine(wpre, inc);
+}
+
+// ---- loaders / storers ----
+struct U32Source {
+  const uint32_t *p;
+  __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
+};
+struct LevelSource {
+  const uint8_t *v;
+  uint64_t n;
+  bool reverse;
+  __device__ LevelState operator()(uint64_t i) const {
+    uint32_t x = v[reverse ? (n - 1 - i) : i];
+    return LevelState{x, 1u << x};
+  }
+};
+
+template <class Op, class Src>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(Src src, uint64_t n, typename Op::T *partials) {
+  using T = typename Op::T;
+  __shared__ T lds[kBlock / 64];
+  uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  T acc = Op::identity();
+#pragma unroll
+  for (int i = 0; i < kItems; ++i)
+    if (base + i < n) acc = Op::combine(acc, src(base + i));
+  T ex, tot;
+  block_scan<Op>(acc, &ex, &tot, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Exclusive scan of up to any number of partials with a single block (carry loop).
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_scan_partials(typename Op::T *partials, uint64_t n) {
+  using T = typename Op::T;
+  __shared__ T lds[kBlock / 64];
+  T carry = Op::identity();
+  for (uint64_t base = 0; base < n; base += kTile) {
+    T items[kItems];
+    uint64_t b = base + (uint64_t)threadIdx.x * kItems;
+    T acc = Op::identity();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      items[i] = (b + i < n) ? partials[b + i] : Op::identity();
+      acc = Op::combine(acc, items[i]);
+    }
+    T ex, tot;
+    block_scan<Op>(acc, &ex, &tot, lds);
+    T run = Op::combine(carry, ex);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if (b + i < n) partials[b + i] = run;  // exclusive
+      run = Op::combine(run, items[i]);
+    }
+    carry = Op::combine(carry, tot);
+    __syncthreads();
+  }
+}
+
+// Final pass: exclusive u32 output.
+__global__ void __launch_bounds__(kBlock) k_scan_final_u32(const uint32_t *in, uint32_t *out, uint64_t n,
+                                                            const uint32_t *partials, uint32_t *d_total,
+                                                            uint32_t nblocks) {
+  __shared__ uint32_t lds[kBlock / 64];
+  uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  uint32_t items[kItems];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    items[i] = (base + i < n) ? in[base + i] : 0u;
+    acc += items[i];
+  }
+  uint32_t ex, tot;
+  block_scan<OpAdd>(acc, &ex, &tot, lds);
+  uint32_t run = partials[blockIdx.x] + ex;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += items[i];
+  }
+  if (d_total && blockIdx.x == nblocks - 1 && threadIdx.x == kBlock - 1) *d_total = run;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_final_level(LevelSource src, uint64_t n,
+                                                              const LevelState *partials, uint32_t *mask_out) {
+  __shared__ LevelState lds[kBlock / 64];
+  uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  LevelState items[kItems];
+  LevelState acc = OpLevel::identity();
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    items[i] = (base + i < n) ? src(base + i) : OpLevel::identity();
+    acc = OpLevel::combine(acc, items[i]);
+  }
+  LevelState ex, tot;
+  block_scan<OpLevel>(acc, &ex, &tot, lds);
+  LevelState run = OpLevel::combine(partials[blockIdx.x], ex);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    run = OpLevel::combine(run, items[i]);  // inclusive
+    uint64_t idx = base + i;
+    if (idx < n) mask_out[src.reverse ? (n - 1 - idx) : idx] = run.mask;
+  }
+}
+
+void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total) {
+  if (n == 0) {
+    if (d_total) OTG_HIP_CHECK(hipMemsetAsync(d_total, 0, 4, ctx->stream));
+    return;
+  }
+  uint32_t nb = div_up(n, kTile);
+  Buf<uint32_t> partials(ctx, nb);
+  launch(ctx, "scan_reduce_u32", k_scan_reduce<OpAdd, U32Source>, dim3(nb), dim3(kBlock), 0, U32Source{in}, n,
+         partials.p);
+  launch(ctx, "scan_partials_u32", k_scan_partials<OpAdd>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
+  launch(ctx, "scan_final_u32", k_scan_final_u32, dim3(nb), dim3(kBlock), 0, in, out, n, (const uint32_t *)partials.p,
+         d_total, nb);
+}
+
+void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask) {
+  if (n == 0) return;
+  uint32_t nb = div_up(n, kTile);
+  Buf<LevelState> partials(ctx, nb);
+  for (int dir = 0; dir < 2; ++dir) {
+    LevelSource src{v, n, dir == 1};
+    launch(ctx, "scan_reduce_level", k_scan_reduce<OpLevel, LevelSource>, dim3(nb), dim3(kBlock), 0, src, n,
+           partials.p);
+    launch(ctx, "scan_partials_level", k_scan_partials<OpLevel>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
+    launch(ctx, "scan_final_level", k_scan_final_level, dim3(nb), dim3(kBlock), 0, src, n,
+           (const LevelState *)partials.p, dir == 0 ? left_mask : right_mask);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_rs_hist(const uint64_t *__restrict__ keys, uint64_t n, uint32_t shift,
+                                                     uint32_t *__restrict__ hist, uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                        uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                        uint64_t n, uint32_t shift,
+                                                        const uint32_t *__restrict__ offs, uint32_t nblocks) {
+  __shared__ uint32_t wcnt[kBlock / 64][256];
+  __shared__ uint32_t base_off[256];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
+  base_off[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+  __syncthreads();
+
+  const uint64_t tile = (uint64_t)blockIdx.x * kTile + (uint64_t)wave * (64 * kItems);
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint64_t key[kItems];
+  uint32_t val[kItems];
+  uint32_t dig[kItems];
+  uint32_t off[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    uint64_t i = tile + (uint64_t)r * 64 + lane;
+    bool valid = i < n;
+    key[r] = valid ? kin[i] : 0ull;
+    val[r] = valid ? vin[i] : 0u;
+    uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+    dig[r] = d;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bal = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    uint32_t before = wcnt[wave][d];
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    off[r] = before + rank;
+    // the lowest peer publishes the group's count
+    if (valid && (peers & lt) == 0ull) wcnt[wave][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  {
+    uint32_t t = threadIdx.x;
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+      uint32_t c = wcnt[w][t];
+      wcnt[w][t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    uint64_t i = tile + (uint64_t)r * 64 + lane;
+    if (i < n) {
+      uint32_t pos = base_off[dig[r]] + wcnt[wave][dig[r]] + off[r];
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+    }
+  }
+}
+
+void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits) {
+  if (n <= 1 || bits == 0) return;
+  uint32_t passes = (bits + 7) / 8;
+  uint32_t nb = div_up(n, kTile);
+  Buf<uint64_t> k2(ctx, n);
+  Buf<uint32_t> v2(ctx, n);
+  Buf<uint32_t> hist(ctx, (size_t)256 * nb);
+  uint64_t *ka = keys, *kb = k2.p;
+  uint32_t *va = vals, *vb = v2.p;
+  for (uint32_t p = 0; p < passes; ++p) {
+    uint32_t shift = p * 8;
+    launch(ctx, "radix_hist", k_rs_hist, dim3(nb), dim3(kBlock), 0, (const uint64_t *)ka, n, shift, hist.p, nb);
+    exclusive_scan_u32(ctx, hist.p, hist.p, (uint64_t)256 * nb, nullptr);
+    launch(ctx, "radix_scatter", k_rs_scatter, dim3(nb), dim3(kBlock), 0, (const uint64_t *)ka, (const uint32_t *)va,
+           kb, vb, n, shift, (const uint32_t *)hist.p, nb);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    OTG_HIP_CHECK(hipMemcpyAsync(keys, ka, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    OTG_HIP_CHECK(hipMemcpyAsync(vals, va, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+}
+
+}  // namespace otg

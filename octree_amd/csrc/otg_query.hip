@@ -1,0 +1,874 @@
+// otg_query.hip -- batched queries over resident static trees.
+//
+// One lane per query with a short private stack (depth-first, children pushed
+// so that the nearest one is popped first where the query has a distance
+// order).  Every predicate that decides an id is the reference's FP64
+// expression with the same operation order (built with -ffp-contract=off):
+//   kNN        ot_query.h:1208-1270, AddEntityDistance :1058-1164,
+//              GetBoxWallDistanceAD internal_geometry_module.h:424-458
+//   range      ot_query.h:476-583
+//   ray first  ot_query.h:2035-2122, RayHitTester igm:536-723
+//   collision  ot_query.h:1901-1906 (pairs with GetBoxRelation == Overlapped)
+// The result sets equal the reference's; only traversal order differs, which
+// the reference leaves unobservable except for exact ties at the final kNN /
+// ray bound (counted as measure-zero divergences by the tests).
+#include <algorithm>
+#include <cfloat>
+
+#include "otg_internal.hpp"
+
+namespace otg {
+
+constexpr int kQB = 128;  // lanes per block for traversals (2 wavefronts)
+
+struct TreeView {
+  const NodeInfo *info;
+  const uint32_t *rend;
+  const double *nmin;
+  const double *nsize;
+  const double *geom;  // SoA, sorted order
+  uint64_t n;
+  const uint32_t *ids;
+  const SetParams *sets;
+};
+
+static TreeView view_of(const Tree *t) {
+  return TreeView{t->d_info, t->d_rend, t->d_nmin, t->d_nsize, t->d_geom, t->n, t->d_ids, t->d_sets};
+}
+
+template <int D>
+struct StackCap {
+  // root + maxDepth * (2^D - 1) pending siblings
+  static constexpr int kDepth = D == 2 ? 15 : (D == 3 ? 19 : (D == 4 ? 15 : (D == 5 ? 12 : 10)));
+  static constexpr int value = 1 + kDepth * ((1 << D) - 1);
+};
+
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+
+// internal_geometry_module.h:424-458 GetBoxWallDistanceAD(.., isInsideConsideredAsZero = true)
+template <int D>
+__device__ __forceinline__ double wall_distance(const double (&q)[D], const double *mn, const double *sz) {
+  double cd[D], hs[D];
+  bool inside = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    hs[d] = sz[d] * 0.5;
+    cd[d] = fabs(mn[d] + hs[d] - q[d]);
+    inside &= cd[d] <= hs[d];
+  }
+  if (inside) return 0.0;
+  double s2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    double v = smax(0.0, cd[d] - hs[d]);
+    s2 += v * v;
+  }
+  return sqrt(s2);
+}
+
+// ot_query.h:1033-1036 GetValueWithToleranceUpper
+__device__ __forceinline__ double tol_upper(double v, double tol) { return fmax(tol, v * (1.0 + tol)); }
+
+// ---------------------------------------------------------------------------
+// kNN (points)
+// ---------------------------------------------------------------------------
+template <int D, int KCAP>
+__global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
+                                              uint64_t nq, uint32_t k, double F0, double tol,
+                                              uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                              uint32_t *__restrict__ overflow) {
+  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (qi >= nq) return;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  uint32_t cnt = 0;
+  double bd[KCAP];
+  uint32_t bid[KCAP];
+  double q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+  if (sp.node_cnt != 0 && k != 0) {
+    double F = F0;
+    bool heapified = false;
+    double dropped = DBL_MAX;  // smallest distance pushed out of the bounded list
+    uint32_t stack[StackCap<D>::value];
+    int top = 0;
+    stack[top++] = (uint32_t)sp.node_off;
+    while (top > 0) {
+      const uint32_t node = stack[--top];
+      const double w = wall_distance<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D]);
+      if (w >= F) continue;
+      const NodeInfo ni = tv.info[node];
+      for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+          d2 += v * v;
+        }
+        const double dist = sqrt(d2);
+        if (dist >= F) continue;
+        const uint32_t id = tv.ids[j];
+        // insert (dist, id) into the ascending list
+        uint32_t pos = 0;
+#pragma unroll
+        for (int t = 0; t < KCAP; ++t)
+          if ((uint32_t)t < cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
+        if (cnt == KCAP) dropped = smin(dropped, bd[KCAP - 1]);
+#pragma unroll
+        for (int t = KCAP - 1; t >= 1; --t)
+          if ((uint32_t)t > pos && (uint32_t)t <= cnt) {
+            bd[t] = bd[t - 1];
+            bid[t] = bid[t - 1];
+          }
+#pragma unroll
+        for (int t = 0; t < KCAP; ++t)
+          if ((uint32_t)t == pos) {
+            bd[t] = dist;
+            bid[t] = id;
+          }
+        if (cnt < KCAP) ++cnt;
+        if (cnt >= k) {
+          double dk = 0.0;
+#pragma unroll
+          for (int t = 0; t < KCAP; ++t)
+            if ((uint32_t)t == k - 1) dk = bd[t];
+          const double lim = tol_upper(dk, tol);
+          if (!heapified) {
+            heapified = true;
+            F = lim;  // make_heap branch assigns unconditionally (:1124-1129)
+          } else if (lim < F) {
+            F = lim;
+            uint32_t keep = 0;
+#pragma unroll
+            for (int t = 0; t < KCAP; ++t)
+              if ((uint32_t)t < cnt && !(bd[t] > F)) ++keep;
+            cnt = keep;
+          }
+        }
+      }
+      if (ni.ccount) {
+        // push children with wall < F, farthest first so the nearest pops first
+        constexpr int CMAX = 1 << D;
+        double cw[CMAX];
+        uint32_t cid[CMAX];
+        int nc = 0;
+        for (uint32_t c = 0; c < ni.ccount; ++c) {
+          const uint32_t ch = ni.cbegin + c;
+          const double wc = wall_distance<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D]);
+          if (wc >= F) continue;
+          // insertion by descending wall distance
+          int p = nc;
+          while (p > 0 && cw[p - 1] < wc) {
+            cw[p] = cw[p - 1];
+            cid[p] = cid[p - 1];
+            --p;
+          }
+          cw[p] = wc;
+          cid[p] = ch;
+          ++nc;
+        }
+        for (int c = 0; c < nc; ++c) stack[top++] = cid[c];
+      }
+    }
+    if (dropped <= F) atomicOr(overflow, 1u);
+  }
+  out_cnt[qi] = cnt;
+#pragma unroll
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t < cnt) out_ids[qi * KCAP + t] = bid[t];
+}
+
+// ---------------------------------------------------------------------------
+// range search
+// ---------------------------------------------------------------------------
+// internal_geometry_module.h:190-198 DoesRangeContainBox
+__device__ __forceinline__ bool range_contains(double rmin, double rmax, double bmin, double bmax, double tol) {
+  if (rmin - tol > bmin || bmin > rmax + tol) return false;
+  if (rmin - tol > bmax || bmax > rmax + tol) return false;
+  return true;
+}
+
+// adapters/general.h:262-289 GetBoxRelation(e1, e2, tol) == Overlapped
+template <int D>
+__device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const double *mx1, const double *mn2,
+                                                     const double *mx2, double tol) {
+  bool adjacent = false;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double a0 = mn1[d], a1 = mx1[d], b0 = mn2[d], b1 = mx2[d];
+    if (a0 + tol < b1 && b0 + tol < a1) continue;
+    if (fabs(a0 - b1) <= tol || fabs(a1 - b0) <= tol) {
+      adjacent = true;
+      continue;
+    }
+    if (b1 + tol < a0 || a1 + tol < b0) return false;
+  }
+  return !adjacent;
+}
+
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_range(TreeView tv, const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
+                                                uint64_t nq, int overlap_mode, double tol, uint32_t *__restrict__ counts,
+                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
+  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (qi >= nq) return;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  double rmin[D], rmax[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    rmin[d] = ranges[qi * 2 * D + d];
+    rmax[d] = ranges[qi * 2 * D + D + d];
+  }
+  uint64_t o = FILL ? offsets[qi] : 0;
+  uint32_t cnt = 0;
+  if (sp.node_cnt != 0) {
+    const uint32_t root = (uint32_t)sp.node_off;
+    bool root_in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double mn = tv.nmin[(uint64_t)root * D + d];
+      const double bmax = mn + tv.nsize[(uint64_t)root * D + d];
+      root_in &= range_contains(rmin[d], rmax[d], mn, bmax, tol);
+    }
+    double vol = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vol *= rmax[d] - rmin[d];
+    if (root_in) {
+      cnt = (uint32_t)sp.ent_cnt;  // iota over the entity span (:488-512)
+      if (FILL)
+        for (uint32_t i = 0; i < cnt; ++i) out[o + i] = i;
+    } else if (vol > 0.0) {
+      double rsz[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) rsz[d] = rmax[d] - rmin[d];
+      uint32_t stack[StackCap<D>::value];
+      int top = 0;
+      stack[top++] = root;
+      while (top > 0) {
+        const uint32_t node = stack[--top];
+        const double *nm = &tv.nmin[(uint64_t)node * D];
+        const double *ns = &tv.nsize[(uint64_t)node * D];
+        bool overlap = true, contained = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          // AreBoxesOverlappingByMinPoint(range min/size, node min/size) igm:148-161
+          const double maxL = rmin[d] + rsz[d];
+          const double maxR = nm[d] + ns[d];
+          if (maxL <= nm[d] - tol || maxR <= rmin[d] - tol) overlap = false;
+          contained &= range_contains(rmin[d], rmax[d], nm[d], nm[d] + ns[d], tol);
+        }
+        if (!overlap) continue;
+        const NodeInfo ni = tv.info[node];
+        if (contained) {
+          // whole subtree: its entities are the contiguous storage range
+          const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
+          if (FILL)
+            for (uint32_t j = e0; j < e1; ++j) out[o + cnt + (j - e0)] = tv.ids[j];
+          cnt += e1 - e0;
+          continue;
+        }
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          bool ok = true;
+          if (!BOX) {
+            if (tol != 0.0) {
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const double p = tv.geom[(uint64_t)d * tv.n + j];
+                ok &= (rmin[d] - tol < p && p < rmax[d] + tol);
+              }
+            } else {
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const double p = tv.geom[(uint64_t)d * tv.n + j];
+                ok &= (rmin[d] <= p && p <= rmax[d]);
+              }
+            }
+          } else {
+            double emn[D], emx[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              emn[d] = tv.geom[(uint64_t)d * tv.n + j];
+              emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
+            }
+            if (overlap_mode) {
+              ok = boxes_overlap_strict<D>(rmin, rmax, emn, emx, tol);
+            } else {
+              // AreBoxesOverlapped(range, e, true, ..) general.h:296-308
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                if (rmin[d] > emn[d] || emn[d] > rmax[d]) ok = false;
+                if (rmin[d] > emx[d] || emx[d] > rmax[d]) ok = false;
+              }
+            }
+          }
+          if (ok) {
+            if (FILL) out[o + cnt] = tv.ids[j];
+            ++cnt;
+          }
+        }
+        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+      }
+    }
+  }
+  if (!FILL) counts[qi] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// ray first hit
+// ---------------------------------------------------------------------------
+template <int D>
+struct RayTester {
+  double origin[D], inv[D], dir[D];
+  double min_tol, inc;
+  uint32_t sign, zero;  // bit masks
+
+  // internal_geometry_module.h:539-568 Make
+  __device__ bool make(const double *o, const double *dv, double tol, double tinc) {
+    double s2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) s2 += dv[d] * dv[d];
+    if (!(fabs(s2 - 1.0) < 0.000001)) return false;
+    min_tol = tol;
+    inc = tinc;
+    sign = 0;
+    zero = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      origin[d] = o[d];
+      dir[d] = dv[d];
+      const bool z = fabs(dv[d]) < 2.0 * DBL_EPSILON;
+      inv[d] = z ? __builtin_nan("") : (1.0 / dv[d]);
+      if (dv[d] < 0) sign |= 1u << d;
+      if (z) zero |= 1u << d;
+    }
+    return true;
+  }
+
+  // BoxHitTest<isConeToleranceConsidered = true> igm:631-712
+  __device__ bool box_hit(const double (&mind)[D], const double (&maxd)[D], double *enter, double *exit) const {
+    double en = -DBL_MAX, ex = DBL_MAX;
+    double pd0[D], pd1[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) pd0[d] = mind[d] * inv[d];
+#pragma unroll
+    for (int d = 0; d < D; ++d) pd1[d] = maxd[d] * inv[d];
+    if (zero) {
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        if (!((zero >> d) & 1u)) en = fmax(en, ((sign >> d) & 1u) ? pd1[d] : pd0[d]);
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        if (!((zero >> d) & 1u)) ex = fmin(ex, ((sign >> d) & 1u) ? pd0[d] : pd1[d]);
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) en = smax(en, ((sign >> d) & 1u) ? pd1[d] : pd0[d]);
+#pragma unroll
+      for (int d = 0; d < D; ++d) ex = smin(ex, ((sign >> d) & 1u) ? pd0[d] : pd1[d]);
+    }
+    const double exit_tol = smax(0.0, ex) * inc + min_tol;
+    en -= 0.5 * (smax(0.0, en) * inc + min_tol);
+    ex += 0.5 * (smax(0.0, ex) * inc + min_tol);
+    if (en >= ex || ex <= 0.0) return false;
+    en = smax(0.0, en);
+    if (zero) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (!((zero >> d) & 1u)) continue;
+        if (mind[d] > exit_tol || maxd[d] < -exit_tol) return false;
+      }
+    }
+    *enter = en;
+    *exit = ex;
+    return true;
+  }
+
+  // Hit(minPoint, size) igm:575-585
+  __device__ bool hit_node(const double *mn, const double *sz, double *en, double *ex) const {
+    double a[D], b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) a[d] = mn[d] - origin[d];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = mn[d] + sz[d] - origin[d];
+    return box_hit(a, b, en, ex);
+  }
+
+  // Hit(point) igm:599-626
+  __device__ bool hit_point(const double (&p)[D], double *en, double *ex) const {
+    double v[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d] = p[d] - origin[d];
+    double t = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) t += v[d] * dir[d];
+    double len2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) len2 += v[d] * v[d];
+    const double dist2 = smax(0.0, len2 - t * t);
+    const double ct = smax(0.0, t) * inc + min_tol;
+    if (dist2 <= ct * ct && t >= -ct) {
+      *en = smax(0.0, t - ct);
+      *ex = t + ct;
+      return true;
+    }
+    return false;
+  }
+};
+
+template <int D, bool BOX, int RCAP>
+__global__ void __launch_bounds__(kQB) k_ray(TreeView tv, const double *__restrict__ rays, const uint32_t *__restrict__ qset,
+                                              uint64_t nq, double tol, double inc, double max_dist,
+                                              uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                              uint32_t *__restrict__ overflow) {
+  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (qi >= nq) return;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  uint32_t cnt = 0;
+  double ce[RCAP];
+  uint32_t cid[RCAP];
+  RayTester<D> rt;
+  if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
+    const double applied = (tol == 0) ? DBL_EPSILON : tol;
+    double M = max_dist;
+    struct Item {
+      uint32_t node;
+      double enter;
+    };
+    Item stack[StackCap<D>::value];
+    int top = 0;
+    double en, ex;
+    const uint32_t root = (uint32_t)sp.node_off;
+    if (rt.hit_node(&tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &en, &ex)) stack[top++] = Item{root, en};
+    while (top > 0) {
+      const Item it = stack[--top];
+      if (it.enter > M) continue;
+      const NodeInfo ni = tv.info[it.node];
+      for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+        bool hit;
+        if (BOX) {
+          double a[D], b[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) a[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
+#pragma unroll
+          for (int d = 0; d < D; ++d) b[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
+          hit = rt.box_hit(a, b, &en, &ex);
+        } else {
+          double p[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) p[d] = tv.geom[(uint64_t)d * tv.n + j];
+          hit = rt.hit_point(p, &en, &ex);
+        }
+        if (!hit || en > M) continue;
+        if (cnt == RCAP) {
+          atomicOr(overflow, 1u);
+        } else {
+#pragma unroll
+          for (int t = 0; t < RCAP; ++t)
+            if ((uint32_t)t == cnt) {
+              ce[t] = en;
+              cid[t] = tv.ids[j];
+            }
+          ++cnt;
+        }
+        if (ex >= M) continue;
+        M = (1.0 + inc) * ex + applied;
+        uint32_t w = 0;
+#pragma unroll
+        for (int t = 0; t < RCAP; ++t) {
+          if ((uint32_t)t < cnt && !(ce[t] > M)) {
+            // compact in place (w <= t)
+#pragma unroll
+            for (int u = 0; u < RCAP; ++u)
+              if ((uint32_t)u == w) {
+                ce[u] = ce[t];
+                cid[u] = cid[t];
+              }
+            ++w;
+          }
+        }
+        cnt = w;
+      }
+      if (ni.ccount) {
+        constexpr int CMAX = 1 << D;
+        Item ch[CMAX];
+        int nc = 0;
+        for (uint32_t c = 0; c < ni.ccount; ++c) {
+          const uint32_t x = ni.cbegin + c;
+          if (!rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &en, &ex)) continue;
+          if (en > M) continue;
+          int p = nc;
+          while (p > 0 && ch[p - 1].enter < en) {
+            ch[p] = ch[p - 1];
+            --p;
+          }
+          ch[p] = Item{x, en};
+          ++nc;
+        }
+        for (int c = 0; c < nc; ++c) stack[top++] = ch[c];
+      }
+    }
+    // descending enter distance, ties by id (selection into output order)
+  }
+  out_cnt[qi] = cnt;
+  // simple in-register sort (cnt <= RCAP, small)
+  for (uint32_t a = 0; a < cnt; ++a) {
+    uint32_t best = a;
+    for (uint32_t b = a + 1; b < cnt; ++b) {
+      double eb = 0, ebest = 0;
+      uint32_t ib = 0, ibest = 0;
+#pragma unroll
+      for (int t = 0; t < RCAP; ++t) {
+        if ((uint32_t)t == b) { eb = ce[t]; ib = cid[t]; }
+        if ((uint32_t)t == best) { ebest = ce[t]; ibest = cid[t]; }
+      }
+      if (eb > ebest || (eb == ebest && ib < ibest)) best = b;
+    }
+    double ea = 0, ebest = 0;
+    uint32_t ia = 0, ibest = 0;
+#pragma unroll
+    for (int t = 0; t < RCAP; ++t) {
+      if ((uint32_t)t == a) { ea = ce[t]; ia = cid[t]; }
+      if ((uint32_t)t == best) { ebest = ce[t]; ibest = cid[t]; }
+    }
+#pragma unroll
+    for (int t = 0; t < RCAP; ++t) {
+      if ((uint32_t)t == a) { ce[t] = ebest; cid[t] = ibest; }
+      if ((uint32_t)t == best && best != a) { ce[t] = ea; cid[t] = ia; }
+    }
+    out_ids[qi * RCAP + a] = ibest;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// self collision (boxes): every entity queries the tree with its own box and
+// keeps partners with a larger id.
+// ---------------------------------------------------------------------------
+template <int D, bool FILL>
+__global__ void __launch_bounds__(kQB) k_collide(TreeView tv, uint32_t set, double tol, uint32_t *__restrict__ counts,
+                                                  const uint64_t *__restrict__ offsets, uint64_t *__restrict__ pairs) {
+  const SetParams &sp = tv.sets[set];
+  const uint64_t li = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (li >= sp.ent_cnt) return;
+  const uint64_t j0 = sp.ent_off + li;  // storage position
+  const uint32_t id0 = tv.ids[j0];
+  double emn[D], emx[D], esz[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    emn[d] = tv.geom[(uint64_t)d * tv.n + j0];
+    emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j0];
+    esz[d] = emx[d] - emn[d];
+  }
+  uint64_t o = FILL ? offsets[li] : 0;
+  uint32_t cnt = 0;
+  uint32_t stack[StackCap<D>::value];
+  int top = 0;
+  stack[top++] = (uint32_t)sp.node_off;
+  while (top > 0) {
+    const uint32_t node = stack[--top];
+    const double *nm = &tv.nmin[(uint64_t)node * D];
+    const double *ns = &tv.nsize[(uint64_t)node * D];
+    bool overlap = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double maxL = nm[d] + ns[d];
+      const double maxR = emn[d] + esz[d];
+      if (maxL <= emn[d] - tol || maxR <= nm[d] - tol) overlap = false;
+    }
+    if (!overlap) continue;
+    const NodeInfo ni = tv.info[node];
+    for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+      const uint32_t id1 = tv.ids[j];
+      if (id1 <= id0) continue;
+      double fmn[D], fmx[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        fmn[d] = tv.geom[(uint64_t)d * tv.n + j];
+        fmx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
+      }
+      if (boxes_overlap_strict<D>(emn, emx, fmn, fmx, tol)) {
+        if (FILL) pairs[o + cnt] = ((uint64_t)id0 << 32) | id1;
+        ++cnt;
+      }
+    }
+    for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+  }
+  if (!FILL) counts[li] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// CSR helpers
+// ---------------------------------------------------------------------------
+__global__ void k_offsets_u64(const uint32_t *__restrict__ scan, uint64_t n, uint32_t total, uint64_t *__restrict__ off) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) off[i] = scan[i];
+  if (i == n) off[n] = total;
+}
+
+__global__ void k_compact_fixed(const uint32_t *__restrict__ src, uint32_t stride, const uint64_t *__restrict__ off,
+                                uint64_t nq, uint32_t *__restrict__ dst) {
+  uint64_t qi = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (qi >= nq) return;
+  uint64_t o = off[qi];
+  uint32_t c = (uint32_t)(off[qi + 1] - o);
+  for (uint32_t t = 0; t < c; ++t) dst[o + t] = src[qi * stride + t];
+}
+
+__global__ void k_split_pairs(const uint64_t *__restrict__ keys, uint64_t np, uint32_t jbits, uint32_t *__restrict__ ids,
+                              uint64_t *__restrict__ off) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < np) {
+    uint64_t k = keys[i];
+    ids[2 * i] = (uint32_t)(k >> jbits);
+    ids[2 * i + 1] = (uint32_t)(k & ((1ull << jbits) - 1));
+  }
+  if (i <= np) off[i] = 2 * i;
+}
+
+__global__ void k_pack_pairs(uint64_t *__restrict__ pairs, uint64_t np, uint32_t jbits) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= np) return;
+  uint64_t k = pairs[i];
+  pairs[i] = ((k >> 32) << jbits) | (k & 0xFFFFFFFFull);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+Result::~Result() {
+  if (ctx) {
+    if (d_offsets) ctx->free(d_offsets);
+    if (d_ids) ctx->free(d_ids);
+  }
+}
+
+struct DevInput {
+  Buf<double> own;
+  Buf<uint32_t> own_sets;
+  const double *q = nullptr;
+  const uint32_t *sets = nullptr;
+};
+
+static void stage_inputs(Ctx *ctx, DevInput &in, const double *q, const uint32_t *sets, uint64_t count_doubles,
+                         uint64_t nq, int mem) {
+  if (mem == OTG_MEM_DEVICE) {
+    in.q = q;
+    in.sets = sets;
+    return;
+  }
+  in.own = Buf<double>(ctx, count_doubles);
+  if (count_doubles)
+    OTG_HIP_CHECK(hipMemcpyAsync(in.own.p, q, count_doubles * 8, hipMemcpyHostToDevice, ctx->stream));
+  in.q = in.own.p;
+  if (sets) {
+    in.own_sets = Buf<uint32_t>(ctx, nq);
+    OTG_HIP_CHECK(hipMemcpyAsync(in.own_sets.p, sets, nq * 4, hipMemcpyHostToDevice, ctx->stream));
+    in.sets = in.own_sets.p;
+  }
+}
+
+// Fixed-stride per-query results -> CSR
+static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, uint32_t stride, uint64_t nq) {
+  Result *r = new Result();
+  r->ctx = ctx;
+  r->nq = nq;
+  Buf<uint32_t> scan(ctx, nq + 1);
+  Buf<uint32_t> total(ctx, 1);
+  exclusive_scan_u32(ctx, cnt, scan.p, nq, total.p);
+  uint32_t ht = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  r->nids = ht;
+  r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
+  r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
+  launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
+         r->d_offsets);
+  launch(ctx, "compact_fixed", k_compact_fixed, dim3(div_up(nq, 256)), dim3(256), 0, ids, stride,
+         (const uint64_t *)r->d_offsets, nq, r->d_ids);
+  return r;
+}
+
+template <int D, int KCAP>
+static Result *knn_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, uint32_t k, double F0, double tol) {
+  Ctx *ctx = t->ctx;
+  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
+  launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs, nq, k, F0, tol, ids.p,
+         cnt.p, ovf.p);
+  uint32_t hovf = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
+  if (hovf) {
+    delete r;
+    throw Error{OTG_ERR_UNSUPPORTED, "kNN tolerance band holds more ids than the per-query list"};
+  }
+  return r;
+}
+
+Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
+                  double max_within, double tol) {
+  if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
+  if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
+  Ctx *ctx = t->ctx;
+  DevInput in;
+  stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
+  const double F0 = (max_within == DBL_MAX) ? DBL_MAX : fmax(tol, max_within * (1.0 + tol));
+  if (nq == 0 || t->n == 0 || k == 0) {
+    Buf<uint32_t> z(ctx, std::max<uint64_t>(nq, 1));
+    OTG_HIP_CHECK(hipMemsetAsync(z.p, 0, std::max<uint64_t>(nq, 1) * 4, ctx->stream));
+    return fixed_to_csr(ctx, z.p, z.p, 0, nq);
+  }
+#define OTG_KNN(DD)                                                                     \
+  case DD:                                                                              \
+    if (k <= 8) return knn_impl<DD, 16>(t, nq, in.q, in.sets, k, F0, tol);               \
+    if (k <= 16) return knn_impl<DD, 24>(t, nq, in.q, in.sets, k, F0, tol);              \
+    if (k <= 32) return knn_impl<DD, 40>(t, nq, in.q, in.sets, k, F0, tol);              \
+    return knn_impl<DD, 72>(t, nq, in.q, in.sets, k, F0, tol);
+  switch (t->dim) {
+    OTG_KNN(2)
+    OTG_KNN(3)
+    OTG_KNN(4)
+    OTG_KNN(5)
+    OTG_KNN(6)
+  }
+#undef OTG_KNN
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
+template <int D, bool BOX>
+static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, int mode, double tol) {
+  Ctx *ctx = t->ctx;
+  TreeView tv = view_of(t);
+  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
+  launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs, nq, mode, tol,
+         cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
+  Result *r = new Result();
+  r->ctx = ctx;
+  r->nq = nq;
+  Buf<uint32_t> scan(ctx, nq + 1), total(ctx, 1);
+  exclusive_scan_u32(ctx, cnt.p, scan.p, nq, total.p);
+  uint32_t ht = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  r->nids = ht;
+  r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
+  r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
+  launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
+         r->d_offsets);
+  launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs, nq, mode, tol,
+         (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
+  return r;
+}
+
+Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
+                    double tol) {
+  Ctx *ctx = t->ctx;
+  DevInput in;
+  stage_inputs(ctx, in, ranges, sets, nq * 2 * t->dim, nq, mem);
+  const bool box = t->kind == OTG_KIND_BOX;
+#define OTG_RANGE(DD)                                                     \
+  case DD:                                                                \
+    return box ? range_impl<DD, true>(t, nq, in.q, in.sets, mode, tol)    \
+               : range_impl<DD, false>(t, nq, in.q, in.sets, mode, tol);
+  switch (t->dim) {
+    OTG_RANGE(2)
+    OTG_RANGE(3)
+    OTG_RANGE(4)
+    OTG_RANGE(5)
+    OTG_RANGE(6)
+  }
+#undef OTG_RANGE
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
+template <int D, bool BOX>
+static Result *ray_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, double tol, double inc,
+                        double max_dist) {
+  Ctx *ctx = t->ctx;
+  constexpr int RCAP = 16;
+  Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
+  launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs, nq, tol, inc,
+         max_dist, ids.p, cnt.p, ovf.p);
+  uint32_t hovf = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  Result *r = fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
+  if (hovf) {
+    delete r;
+    throw Error{OTG_ERR_UNSUPPORTED, "more first-hit candidates than the per-ray list"};
+  }
+  return r;
+}
+
+Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol, double inc,
+                  double max_dist) {
+  Ctx *ctx = t->ctx;
+  DevInput in;
+  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
+  const bool box = t->kind == OTG_KIND_BOX;
+#define OTG_RAY(DD)                                                                 \
+  case DD:                                                                          \
+    return box ? ray_impl<DD, true>(t, nq, in.q, in.sets, tol, inc, max_dist)        \
+               : ray_impl<DD, false>(t, nq, in.q, in.sets, tol, inc, max_dist);
+  switch (t->dim) {
+    OTG_RAY(2)
+    OTG_RAY(3)
+    OTG_RAY(4)
+    OTG_RAY(5)
+    OTG_RAY(6)
+  }
+#undef OTG_RAY
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
+template <int D>
+static Result *collide_impl(const Tree *t, uint32_t set, double tol) {
+  Ctx *ctx = t->ctx;
+  const SetParams &sp = t->sets[set];
+  const uint64_t ne = sp.ent_cnt;
+  TreeView tv = view_of(t);
+  Result *r = new Result();
+  r->ctx = ctx;
+  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(ne, 1));
+  launch(ctx, "collide_count", k_collide<D, false>, dim3(div_up(ne, kQB)), dim3(kQB), 0, tv, set, tol, cnt.p,
+         (const uint64_t *)nullptr, (uint64_t *)nullptr);
+  Buf<uint32_t> scan(ctx, ne + 1), total(ctx, 1);
+  exclusive_scan_u32(ctx, cnt.p, scan.p, ne, total.p);
+  uint32_t np = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&np, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  Buf<uint64_t> off(ctx, ne + 1);
+  launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(ne + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, ne, np,
+         off.p);
+  Buf<uint64_t> pairs(ctx, std::max<uint64_t>(np, 1));
+  launch(ctx, "collide_fill", k_collide<D, true>, dim3(div_up(ne, kQB)), dim3(kQB), 0, tv, set, tol, (uint32_t *)nullptr,
+         (const uint64_t *)off.p, pairs.p);
+  const uint32_t jbits = ne > 1 ? (64 - __builtin_clzll(ne - 1)) : 1;
+  Buf<uint32_t> dummy(ctx, std::max<uint64_t>(np, 1));
+  launch(ctx, "pack_pairs", k_pack_pairs, dim3(div_up(np, 256)), dim3(256), 0, pairs.p, (uint64_t)np, jbits);
+  radix_sort_pairs(ctx, pairs.p, dummy.p, np, 2 * jbits);
+  r->nq = np;
+  r->nids = 2ull * np;
+  r->d_offsets = (uint64_t *)ctx->alloc((np + 1) * 8);
+  r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(2ull * np, 1) * 4);
+  launch(ctx, "split_pairs", k_split_pairs, dim3(div_up(np + 1, 256)), dim3(256), 0, (const uint64_t *)pairs.p,
+         (uint64_t)np, jbits, r->d_ids, r->d_offsets);
+  return r;
+}
+
+Result *query_collide(const Tree *t, uint32_t set, double tol) {
+  if (t->kind != OTG_KIND_BOX) throw Error{OTG_ERR_UNSUPPORTED, "collision detection of point trees"};
+  if (set >= t->sets.size()) throw Error{OTG_ERR_INVALID_ARG, "set index"};
+  switch (t->dim) {
+    case 2: return collide_impl<2>(t, set, tol);
+    case 3: return collide_impl<3>(t, set, tol);
+    case 4: return collide_impl<4>(t, set, tol);
+    case 5: return collide_impl<5>(t, set, tol);
+    case 6: return collide_impl<6>(t, set, tol);
+  }
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
+}  // namespace otg

@@ -1,0 +1,289 @@
+"""Shared test helpers: seeded generators, std::mt19937, brute-force checkers,
+and the structural rule checker of the static linear tree's node table.
+
+The brute-force checkers restate each query's *result set* from its closed-form
+definition without any tree, so they pin the oracle's and the GPU's traversals;
+the rule checker validates a node table against the build rules of
+core/ot_static_linear_core.h:417-519 without running a BFS.
+"""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+EPS = float(np.finfo(np.float64).eps)
+BASE_TOL = EPS * 100.0
+DBL_MAX = float(np.finfo(np.float64).max)
+
+
+def kat():
+    return json.loads((GOLDEN / "kat_reference.json").read_text())
+
+
+# ---------------------------------------------------------------------------
+# std::mt19937 (C++ [rand.eng.mt], standardised output), for the reference's
+# seeded procedures (Issue36 / Issue38).
+# ---------------------------------------------------------------------------
+class MT19937:
+    def __init__(self, seed: int = 5489):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+        self.idx = 624
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(624):
+            y = (mt[i] & 0x80000000) | (mt[(i + 1) % 624] & 0x7FFFFFFF)
+            v = mt[(i + 397) % 624] ^ (y >> 1)
+            if y & 1:
+                v ^= 0x9908B0DF
+            mt[i] = v
+        self.idx = 0
+
+    def __call__(self) -> int:
+        if self.idx >= 624:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y & 0xFFFFFFFF
+
+
+def issue38_boxes(rng: MT19937, n=1000, space_max=8.0, size_max=2.0):
+    """general.tests.cpp:4728-4772 box generator (3D)."""
+    b = np.zeros((n, 6))
+    b[0, :3] = 0.0
+    b[0, 3:] = size_max
+    b[1, :3] = space_max - size_max
+    b[1, 3:] = space_max
+    for i in range(2, n):
+        for d in range(3):
+            mn = min(float(rng() % 10000) / 10000.0 * space_max, space_max - 0.1)
+            mx = min(mn + float(rng() % 10000) / 10000.0 * size_max, space_max)
+            b[i, d] = mn
+            b[i, 3 + d] = mx
+    return b
+
+
+# ---------------------------------------------------------------------------
+# seeded generators (exact doubles, shared by oracle and GPU)
+# ---------------------------------------------------------------------------
+def uniform_points(n, dim, lo=0.0, hi=100.0, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, (n, dim))
+
+
+def lognormal_boxes(n, dim, lo=0.0, hi=1000.0, mu=np.log(0.5), sigma=0.6, seed=0):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(lo, hi, (n, dim))
+    h = np.exp(rng.normal(mu, sigma, (n, dim)))
+    return np.concatenate([c - h, c + h], axis=1)
+
+
+def gaussian_mixture(n, dim, k, lo, hi, smin, smax, seed=0):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(lo, hi, (k, dim))
+    sig = rng.uniform(smin, smax, k)
+    lab = rng.integers(0, k, n)
+    pts = centers[lab] + rng.normal(0.0, 1.0, (n, dim)) * sig[lab, None]
+    bad = ((pts < lo) | (pts > hi)).any(axis=1)
+    while bad.any():
+        m = int(bad.sum())
+        lab2 = rng.integers(0, k, m)
+        pts[bad] = centers[lab2] + rng.normal(0.0, 1.0, (m, dim)) * sig[lab2, None]
+        bad = ((pts < lo) | (pts > hi)).any(axis=1)
+    return pts
+
+
+def unit_dirs(n, dim, seed=0):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0.0, 1.0, (n, dim))
+    return v / np.sqrt((v * v).sum(axis=1))[:, None]
+
+
+# ---------------------------------------------------------------------------
+# brute-force result sets
+# ---------------------------------------------------------------------------
+def distances(q, pts):
+    d2 = np.zeros(pts.shape[0])
+    for d in range(pts.shape[1]):
+        v = q[d] - pts[:, d]
+        d2 = d2 + v * v
+    return np.sqrt(d2)
+
+
+def knn_brute(q, pts, k, max_within=DBL_MAX, tol=BASE_TOL):
+    """Closed-form kNN set of ot_query.h:1208-1247: {d < F_final}, ascending by
+    (distance, id).  Returns (ids, ambiguous) where ambiguous flags a distance
+    exactly at the final bound (order-dependent in the reference)."""
+    dist = distances(q, pts)
+    F0 = DBL_MAX if max_within == DBL_MAX else max(tol, max_within * (1.0 + tol))
+    cand = np.nonzero(dist < F0)[0]
+    if len(cand) < k:
+        order = cand[np.lexsort((cand, dist[cand]))]
+        return order.tolist(), False
+    dk = np.sort(dist[cand])[k - 1]
+    F = max(tol, dk * (1.0 + tol))
+    lim = min(F, F0) if F0 != DBL_MAX else F
+    sel = np.nonzero(dist < lim)[0]
+    amb = bool(np.any(dist == lim)) or (F0 != DBL_MAX and F > F0 and bool(np.any((dist >= F0) & (dist < F))))
+    order = sel[np.lexsort((sel, dist[sel]))]
+    return order.tolist(), amb
+
+
+def range_brute_points(r, pts, tol=BASE_TOL):
+    D = pts.shape[1]
+    ok = np.ones(pts.shape[0], bool)
+    for d in range(D):
+        if tol != 0.0:
+            ok &= (r[d] - tol < pts[:, d]) & (pts[:, d] < r[D + d] + tol)
+        else:
+            ok &= (r[d] <= pts[:, d]) & (pts[:, d] <= r[D + d])
+    return np.nonzero(ok)[0].tolist()
+
+
+def boxes_overlap_strict(a, B, tol):
+    """general.h:262-289 GetBoxRelation(a, B[i]) == Overlapped, vectorised over B."""
+    D = a.shape[0] // 2
+    over = np.ones(B.shape[0], bool)
+    for d in range(D):
+        over &= (a[d] + tol < B[:, D + d]) & (B[:, d] + tol < a[D + d])
+    return over
+
+
+def collision_brute(boxes, tol=BASE_TOL):
+    n = boxes.shape[0]
+    pairs = []
+    for i in range(n - 1):
+        m = boxes_overlap_strict(boxes[i], boxes[i + 1:], tol)
+        for j in np.nonzero(m)[0]:
+            pairs.append((i, i + 1 + int(j)))
+    return np.array(pairs, dtype=np.int64).reshape(-1, 2)
+
+
+# ---------------------------------------------------------------------------
+# node table comparison and rule checks
+# ---------------------------------------------------------------------------
+def compare_tables(gpu: dict, ora: dict, is_box: bool):
+    """Exact equality of two node tables; entity storage compared within runs of
+    equal location keys (the reference's std::sort leaves their order open)."""
+    for name in ("depth", "ebegin", "ecount", "ccount", "key", "min", "size"):
+        a, b = gpu[name], ora[name]
+        assert a.shape == b.shape, (name, a.shape, b.shape)
+        if a.dtype.kind == "f":
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"{name} differs at " \
+                f"{np.argwhere(a.view(np.uint64) != b.view(np.uint64))[:5].tolist()}"
+        else:
+            assert np.array_equal(a, b), f"{name} differs at {np.nonzero(a != b)[0][:10].tolist()}"
+    m = ora["ccount"] > 0
+    assert np.array_equal(gpu["cbegin"][m], ora["cbegin"][m]), "cbegin differs"
+    # entity storage: equal multiset per equal-key run
+    if is_box:
+        db = max(1, int(ora["max_depth"]).bit_length())
+        okeys = (ora["sorted_code"].astype(np.uint64) << np.uint64(db)) | ora["sorted_depth"].astype(np.uint64)
+    else:
+        okeys = ora["sorted_code"]
+    gkeys = gpu["sorted_keys"]
+    assert np.array_equal(gkeys, okeys), "sorted location keys differ"
+    n = len(okeys)
+    if n:
+        starts = np.concatenate([[0], np.nonzero(okeys[1:] != okeys[:-1])[0] + 1, [n]])
+        ga, oa = gpu["ids"], ora["ids"]
+        for s, e in zip(starts[:-1], starts[1:]):
+            if e - s == 1:
+                assert ga[s] == oa[s], ("entity id", s)
+            else:
+                assert sorted(ga[s:e]) == sorted(oa[s:e]), ("entity run", s, e)
+
+
+def check_node_table_rules(t: dict, n: int, cap: int, D: int, is_box: bool):
+    """Structural rules of the static linear tree (ot_static_linear_core.h:417-519),
+    checked directly on a node table + sorted codes/depths:
+      * node 0 is the root (depth 0, key 1); children blocks are contiguous and
+        listed in parent (BFS) order: child blocks concatenated = 1..N-1;
+      * a node's subtree covers a contiguous entity range; internal nodes keep
+        the entities whose depth equals the node depth (boxes) at the front, the
+        children cover the rest in order, grouped by the Morton digit at
+        depth+1, one child per distinct digit;
+      * non-root node depth = LCA depth of its range = min(common digits of the
+        first and last code, depth of the first entity);
+      * leaf iff count <= cap, depth >= maxDepth, or (boxes) all entities stay
+        at the node depth; the node key is (1 << depth*D) | code >> (M-depth)*D."""
+    M = int(t["max_depth"])
+    codes = [int(c) for c in t["sorted_code"]]
+    deps = [int(x) for x in t["sorted_depth"]] if is_box else [M] * n
+    nn = len(t["depth"])
+    if n == 0:
+        assert nn == 0
+        return
+    assert int(t["depth"][0]) == 0 and int(t["key"][0]) == 1
+    # BFS contiguity
+    expect_next = 1
+    for i in range(nn):
+        cc = int(t["ccount"][i])
+        if cc:
+            assert int(t["cbegin"][i]) == expect_next, ("bfs order", i)
+            expect_next += cc
+    assert expect_next == nn
+    # ranges
+    rng_of = [None] * nn
+
+    def rng(i):
+        if rng_of[i] is None:
+            a = int(t["ebegin"][i])
+            if int(t["ccount"][i]) == 0:
+                rng_of[i] = (a, a + int(t["ecount"][i]))
+            else:
+                cb, cc = int(t["cbegin"][i]), int(t["ccount"][i])
+                rng_of[i] = (a, rng(cb + cc - 1)[1])
+        return rng_of[i]
+
+    for i in reversed(range(nn)):
+        rng(i)
+    assert rng_of[0] == (0, n)
+    for i in range(nn):
+        a, b = rng_of[i]
+        dep = int(t["depth"][i])
+        cnt = b - a
+        cc = int(t["ccount"][i])
+        if i > 0:
+            x = codes[a] ^ codes[b - 1]
+            lcp = M if x == 0 else M - (x.bit_length() + D - 1) // D
+            assert dep == min(lcp, deps[a]), ("lca depth", i, dep, lcp, deps[a])
+        sh = (M - dep) * D
+        assert int(t["key"][i]) == ((1 << (dep * D)) | (codes[a] >> sh)), ("node key", i)
+        all_nr = is_box and all(deps[j] == dep for j in range(a, b))
+        if cnt <= cap or dep >= M or all_nr:
+            assert cc == 0, ("should be leaf", i)
+            assert int(t["ecount"][i]) == cnt
+        else:
+            assert cc > 0, ("should split", i)
+            nr = int(t["ecount"][i])
+            for j in range(a, a + nr):
+                assert deps[j] == dep
+            if is_box and a + nr < b:
+                assert deps[a + nr] != dep
+            if not is_box:
+                assert nr == 0
+            pos = a + nr
+            prev_digit = -1
+            sh1 = (M - dep - 1) * D
+            for c in range(int(t["cbegin"][i]), int(t["cbegin"][i]) + cc):
+                ca, cb2 = rng_of[c]
+                assert ca == pos, ("children cover in order", i, c)
+                digs = {(codes[j] >> sh1) & ((1 << D) - 1) for j in range(ca, cb2)}
+                assert len(digs) == 1, ("one digit per child", i, c)
+                dg = digs.pop()
+                assert dg > prev_digit
+                prev_digit = dg
+                assert int(t["depth"][c]) > dep
+                pos = cb2
+            assert pos == b

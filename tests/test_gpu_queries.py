@@ -1,0 +1,181 @@
+"""GPU parity of the batched queries against the oracle and the reference's KATs:
+kNN (ot_query.h:1208-1270), RangeSearch (:476-583), RayIntersectedFirst
+(:2035-2122), CollisionDetection (:1901-1906).  Ids must be identical; kNN and
+ray lists are compared in order (distance ties ordered by id on both sides)."""
+import numpy as np
+import pytest
+
+from helpers import (BASE_TOL, DBL_MAX, MT19937, collision_brute, gaussian_mixture, issue38_boxes, kat, knn_brute,
+                     lognormal_boxes, uniform_points, unit_dirs)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _tree(ctx, kind, D, geom, box, M, cap):
+    import octree_amd
+
+    cls = octree_amd.StaticOrthoTreePointND(D) if kind == "point" else octree_amd.StaticOrthoTreeBoxND(D)
+    t = cls(ctx=ctx)
+    t.Create(geom, M or None, box, cap)
+    return t
+
+
+# ---- kNN ----------------------------------------------------------------------------
+@pytest.mark.parametrize("D,n,k,cap,M", [(2, 3000, 4, 5, 10), (3, 20000, 8, 16, 7), (3, 5000, 1, 2, 8),
+                                         (6, 20000, 16, 32, 5), (3, 4000, 32, 8, 6), (3, 500, 64, 4, 5)])
+def test_knn_vs_oracle(gpu_ctx, D, n, k, cap, M):
+    pts = uniform_points(n, D, seed=D + n + k)
+    pts[:100] = pts[100:200]
+    qs = uniform_points(2000, D, seed=7 + D)
+    qs[:50] = pts[:50]
+    g = _tree(gpu_ctx, "point", D, pts, None, M, cap)
+    o = O.OracleTree(D, "point", pts, None, M, cap)
+    goff, gids = g.knn_batch(qs, k).csr()
+    ooff, oids = o.knn(qs, k, threads=8)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(gids, oids)
+
+
+def test_knn_max_within_and_tolerance(gpu_ctx):
+    pts = uniform_points(5000, 3, seed=1)
+    qs = uniform_points(1000, 3, seed=2)
+    g = _tree(gpu_ctx, "point", 3, pts, None, 6, 8)
+    o = O.OracleTree(3, "point", pts, None, 6, 8)
+    for mw, tol in ((0.0, BASE_TOL), (1.5, BASE_TOL), (4.0, 0.0), (DBL_MAX, 1e-6)):
+        goff, gids = g.knn_batch(qs, 8, mw, tol).csr()
+        ooff, oids = o.knn(qs, 8, mw, tol, threads=8)
+        assert np.array_equal(goff, ooff) and np.array_equal(gids, oids), (mw, tol)
+
+
+def test_knn_kats(gpu_ctx):
+    k = kat()["knn_point_set_no1"]
+    g = _tree(gpu_ctx, "point", 2, np.array(k["points"]), None, k["max_depth"], k["cap"])
+    for c in k["cases"]:
+        if c["k"] > 64:
+            continue
+        got = g.GetNearestNeighbors(c["q"], c["k"], c.get("max_within", DBL_MAX), None, c.get("tol", BASE_TOL))
+        assert sorted(got) == sorted(c["expect"]), c["src"]
+    for c in kat()["knn_issue9"]:
+        pts = np.array(c["points"])
+        g = _tree(gpu_ctx, "point", c["dim"], pts, c["box"], c["max_depth"], c["cap"])
+        d2 = ((pts - np.array(c["q"])) ** 2).sum(axis=1)
+        assert g.GetNearestNeighbors(c["q"], 1)[0] == int(np.argmin(d2)), c["src"]
+
+
+def test_knn_c1_size_brute_sample(gpu_ctx):
+    """1M points, 10k queries k=8: GPU == oracle on all, == brute force on a sample."""
+    pts = uniform_points(1_000_000, 3, 0.0, 100.0, seed=2024)
+    qs = uniform_points(10_000, 3, 0.0, 100.0, seed=2025)
+    box = [0, 0, 0, 100, 100, 100]
+    g = _tree(gpu_ctx, "point", 3, pts, box, 10, 16)
+    o = O.OracleTree(3, "point", pts, box, 10, 16)
+    goff, gids = g.knn_batch(qs, 8).csr()
+    ooff, oids = o.knn(qs, 8, threads=16)
+    assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
+    for qi in range(0, 10_000, 500):
+        exp, amb = knn_brute(qs[qi], pts, 8)
+        assert not amb
+        assert gids[goff[qi]:goff[qi + 1]].tolist() == exp
+
+
+# ---- range ----------------------------------------------------------------------------
+@pytest.mark.parametrize("kind,D,mode", [("point", 2, 0), ("point", 3, 0), ("point", 6, 0), ("box", 3, 0), ("box", 3, 1),
+                                         ("box", 2, 1)])
+def test_range_vs_oracle(gpu_ctx, kind, D, mode):
+    n = 20000
+    if kind == "point":
+        geom = uniform_points(n, D, 0.0, 100.0, seed=D)
+    else:
+        geom = lognormal_boxes(n, D, 0.0, 100.0, np.log(0.5), 0.8, seed=D)
+    rng = np.random.default_rng(D + mode)
+    c = rng.uniform(0, 100, (3000, D))
+    e = rng.uniform(0.5, 15, (3000, 1))
+    ranges = np.concatenate([c - e, c + e], axis=1)
+    ranges[0] = np.concatenate([np.full(D, -5.0), np.full(D, 105.0)])
+    ranges[1] = np.concatenate([np.full(D, 5.0), np.full(D, 5.0)])
+    ranges[2] = np.concatenate([np.full(D, 0.0), np.full(D, 50.0)])
+    g = _tree(gpu_ctx, kind, D, geom, None, 6, 8)
+    o = O.OracleTree(D, kind, geom, None, 6, 8)
+    goff, gids = g.range_batch(ranges, mode).csr()
+    ooff, oids = o.range(ranges, overlap=bool(mode), threads=8)
+    assert np.array_equal(goff, ooff)
+    for qi in range(len(ranges)):
+        assert sorted(gids[goff[qi]:goff[qi + 1]].tolist()) == sorted(oids[ooff[qi]:ooff[qi + 1]].tolist()), qi
+
+
+# ---- ray first -----------------------------------------------------------------------
+def test_ray_kats(gpu_ctx):
+    for c in kat()["ray_first_2d"]:
+        g = _tree(gpu_ctx, "box", 2, np.array(c["boxes"]), None, c["depth"], c["cap"])
+        got = g.RayIntersectedFirst(c["o"], c["d"], None, c["tol"], c.get("inc", 0.0), c.get("max_dist", DBL_MAX))
+        assert got == c["expect"], c["src"]
+
+
+@pytest.mark.parametrize("kind,D", [("box", 3), ("box", 2), ("point", 3)])
+def test_ray_vs_oracle(gpu_ctx, kind, D):
+    n = 30000
+    if kind == "box":
+        geom = lognormal_boxes(n, D, 0.0, 1000.0, np.log(0.5 * 4), 0.6, seed=21)
+    else:
+        geom = uniform_points(n, D, 0.0, 1000.0, seed=21)
+    o_ = uniform_points(3000, D, 0.0, 1000.0, seed=22)
+    d_ = unit_dirs(3000, D, seed=23)
+    rays = np.concatenate([o_, d_], axis=1)
+    rays[0, D:] = 0.0
+    rays[0, D] = 1.0  # axis-aligned: zero components take the NaN path
+    rays[1, D:] = 2.0  # not normalised -> empty (igm:541-545)
+    tol = BASE_TOL if kind == "box" else 0.5
+    g = _tree(gpu_ctx, kind, D, geom, None, 7, 8)
+    o = O.OracleTree(D, kind, geom, None, 7, 8)
+    goff, gids = g.ray_first_batch(rays, tol).csr()
+    ooff, oids = o.ray_first(rays, tol, threads=8)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(gids, oids)
+    assert goff[2] - goff[1] == 0
+
+
+# ---- collision -----------------------------------------------------------------------
+def test_collision_issue38(gpu_ctx):
+    rng = MT19937(0)
+    for _ in range(10):
+        b = issue38_boxes(rng)
+        g = _tree(gpu_ctx, "box", 3, b, None, 10, 5)
+        assert np.array_equal(g.collide().astype(np.int64), collision_brute(b))
+
+
+def test_collision_vs_oracle(gpu_ctx):
+    b = lognormal_boxes(200_000, 3, 0.0, 1000.0, np.log(0.5), 0.6, seed=31)
+    b = np.concatenate([b, lognormal_boxes(2000, 3, 0.0, 1000.0, np.log(20.0), 0.5, seed=32)])
+    g = _tree(gpu_ctx, "box", 3, b, None, 8, 32)
+    o = O.OracleTree(3, "box", b, None, 8, 32)
+    assert np.array_equal(g.collide(), o.collide())
+
+
+# ---- forest queries -------------------------------------------------------------------
+def test_forest_queries_match_per_set_trees(gpu_ctx):
+    import octree_amd
+
+    D = 3
+    sizes = [5, 700, 3000, 40, 9000, 1]
+    sets = [gaussian_mixture(n, D, 8, 0.0, 1000.0, 2.0, 20.0, seed=40 + i) for i, n in enumerate(sizes)]
+    box = np.tile([[0.0] * D + [1000.0] * D], (len(sizes), 1))
+    f = octree_amd.StaticOctreePoint.forest(sets, 8, None, box, ctx=gpu_ctx)
+    rng = np.random.default_rng(9)
+    nq = 4000
+    sid = rng.integers(0, len(sizes), nq).astype(np.uint32)
+    qs = uniform_points(nq, D, 0.0, 1000.0, seed=10)
+    c = rng.uniform(0, 1000, (nq, D))
+    e = rng.uniform(1, 30, (nq, 1))
+    rq = np.concatenate([c - e, c + e], axis=1)
+    koff, kids = f.knn_batch(qs, 16, set_ids=sid).csr()
+    roff, rids = f.range_batch(rq, set_ids=sid).csr()
+    for s in range(len(sizes)):
+        o = O.OracleTree(D, "point", sets[s], box[s], 0, 8)
+        m = np.nonzero(sid == s)[0]
+        ooff, oids = o.knn(qs[m], 16)
+        for j, qi in enumerate(m):
+            assert kids[koff[qi]:koff[qi + 1]].tolist() == oids[ooff[j]:ooff[j + 1]].tolist()
+        ooff, oids = o.range(rq[m])
+        for j, qi in enumerate(m):
+            assert sorted(rids[roff[qi]:roff[qi + 1]].tolist()) == sorted(oids[ooff[j]:ooff[j + 1]].tolist())
