@@ -1,0 +1,297 @@
+"""Benchmark: batched vs plain GPU build + kNN / range over many uneven point sets.
+
+Workload (default, one MI355X): 256 independent 3-D point sets, sizes log-uniform
+in [1e3, 1e6] (~30M points), each a mixture of 8 isotropic Gaussians
+(sigma U[2,20]) clipped to [0,1000]^3, FP64; per set 50,000 kNN queries (k=16,
+drawn from the set's mixture) and 50,000 RangeSearch boxes (edge U[1,30]).
+Leaf capacity is picked by a short autotune over {8,16,32,64}; max depth is the
+reference's EstimateMaxDepth per set.
+
+One step = build all 256 trees (one forest build) + all kNN + all range queries,
+inputs resident in HBM.  The plain path builds and queries one tree at a time.
+Prints ONE JSON line (rank 0).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "kNN queries/s + build Mpts/s, batched vs plain GPU (bit-exact ids); p50/p99 ms"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# workload generation (seeded, exact doubles)
+# ---------------------------------------------------------------------------
+def make_workload(nsets=256, nq=50_000, seed=1234, lo=3.0, hi=6.0, size_scale=1.0):
+    rng = np.random.default_rng(seed)
+    sizes = np.round(10.0 ** rng.uniform(lo, hi, nsets) * size_scale).astype(np.int64)
+    sizes = np.maximum(sizes, 1)
+    pts, kq, rq = [], [], []
+    for s in range(nsets):
+        r = np.random.default_rng(seed * 1000 + s)
+        centers = r.uniform(0.0, 1000.0, (8, 3))
+        sig = r.uniform(2.0, 20.0, 8)
+
+        def draw(m):
+            lab = r.integers(0, 8, m)
+            p = centers[lab] + r.normal(0.0, 1.0, (m, 3)) * sig[lab, None]
+            bad = ((p < 0.0) | (p > 1000.0)).any(axis=1)
+            while bad.any():
+                k = int(bad.sum())
+                lab2 = r.integers(0, 8, k)
+                p[bad] = centers[lab2] + r.normal(0.0, 1.0, (k, 3)) * sig[lab2, None]
+                bad = ((p < 0.0) | (p > 1000.0)).any(axis=1)
+            return p
+
+        pts.append(draw(int(sizes[s])))
+        kq.append(draw(nq))
+        e = r.uniform(1.0, 30.0, (nq, 1))
+        c = r.uniform(0.0, 1000.0, (nq, 3))
+        c = np.clip(c, e / 2, 1000.0 - e / 2)
+        rq.append(np.concatenate([c - e / 2, c + e / 2], axis=1))
+    offs = np.zeros(nsets + 1, np.uint64)
+    offs[1:] = np.cumsum(sizes)
+    return {
+        "sizes": sizes, "offsets": offs, "points": np.concatenate(pts), "knn": np.concatenate(kq),
+        "range": np.concatenate(rq), "set_ids": np.repeat(np.arange(nsets, dtype=np.uint32), nq), "nq": nq,
+        "box": np.array([0.0, 0.0, 0.0, 1000.0, 1000.0, 1000.0]),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsets", type=int, default=256)
+    ap.add_argument("--nq", type=int, default=50_000)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--cap", type=int, default=0, help="0 = autotune over {8,16,32,64}")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-only", action="store_true", help="one batched step, no plain/CPU legs")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the forest shards by whole trees; see DESIGN.md); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import octree_amd
+
+    t_gen = time.time()
+    W = make_workload(args.nsets, args.nq)
+    log(f"workload: {args.nsets} sets, {int(W['offsets'][-1])} points, {len(W['knn'])} kNN + {len(W['range'])} range "
+        f"queries (generated in {time.time() - t_gen:.1f}s)")
+    dev = torch.device("cuda:0")
+    d_pts = torch.from_numpy(W["points"]).to(dev)
+    d_knn = torch.from_numpy(W["knn"]).to(dev)
+    d_rng = torch.from_numpy(W["range"]).to(dev)
+    d_sid = torch.from_numpy(W["set_ids"].view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    ctx = octree_amd.Context(0)
+    offs = W["offsets"]
+    nsets = args.nsets
+    nq = args.nq
+    N = int(offs[-1])
+    box = np.tile(W["box"], (nsets, 1))
+    Q = len(W["knn"])
+
+    def batched_step(cap, timings=None):
+        t0 = time.perf_counter()
+        f = octree_amd.StaticOctreePoint.forest(None, cap, None, box, ctx=ctx, device_geom=(d_pts, offs))
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        rk = f.knn_batch(d_knn, args.k, set_ids=d_sid)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        rr = f.range_batch(d_rng, set_ids=d_sid)
+        ctx.synchronize()
+        t3 = time.perf_counter()
+        if timings is not None:
+            timings.append((t1 - t0, t2 - t1, t3 - t2, t3 - t0))
+        return f, rk, rr
+
+    def plain_step(cap, timings=None, lat=None):
+        tb = tk = tr = 0.0
+        T0 = time.perf_counter()
+        for s in range(nsets):
+            a, b = int(offs[s]), int(offs[s + 1])
+            t0 = time.perf_counter()
+            t = octree_amd.StaticOctreePoint(ctx=ctx)
+            t.Create(d_pts[a:b], None, W["box"], cap)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            rk = t.knn_batch(d_knn[s * nq:(s + 1) * nq], args.k)
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            rr = t.range_batch(d_rng[s * nq:(s + 1) * nq])
+            ctx.synchronize()
+            t3 = time.perf_counter()
+            tb += t1 - t0
+            tk += t2 - t1
+            tr += t3 - t2
+            if lat is not None:
+                lat.append((t3 - t0) * 1e3)
+            rk.free()
+            rr.free()
+            t.free()
+        if timings is not None:
+            timings.append((tb, tk, tr, time.perf_counter() - T0))
+
+    # ---- autotune leaf capacity (batched step time) ----
+    caps = [args.cap] if args.cap else [8, 16, 32, 64]
+    tune = {}
+    if len(caps) > 1:
+        for c in caps:
+            tt = []
+            f, rk, rr = batched_step(c, tt)
+            rk.free(); rr.free(); f.free()
+            tt = []
+            f, rk, rr = batched_step(c, tt)
+            rk.free(); rr.free(); f.free()
+            tune[c] = tt[0][3]
+            log(f"autotune cap={c}: step {tt[0][3] * 1e3:.1f} ms (build {tt[0][0] * 1e3:.1f}, knn {tt[0][1] * 1e3:.1f}, "
+                f"range {tt[0][2] * 1e3:.1f})")
+        cap = min(tune, key=tune.get)
+    else:
+        cap = caps[0]
+    log(f"leaf capacity: {cap}")
+
+    # ---- batched timed steps ----
+    for _ in range(args.warmup):
+        f, rk, rr = batched_step(cap)
+        rk.free(); rr.free(); f.free()
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    bt = []
+    T0 = time.perf_counter()
+    for _ in range(args.steps):
+        f, rk, rr = batched_step(cap, bt)
+        nk = rk.csr()[1].shape[0] if _ == 0 else None
+        rk.free(); rr.free(); f.free()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    T1 = time.perf_counter()
+    ms_per_step = (T1 - T0) / args.steps * 1e3
+    b_build = float(np.median([x[0] for x in bt]))
+    b_knn = float(np.median([x[1] for x in bt]))
+    b_range = float(np.median([x[2] for x in bt]))
+    b_step = float(np.median([x[3] for x in bt]))
+
+    # ---- per-kernel times (HIP events on the library stream) for the roofline ----
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    f, rk, rr = batched_step(cap)
+    kt = ctx.kernel_times()
+    rk.free(); rr.free(); f.free()
+    ctx.set_profiling(False)
+    build_kernels = {k: v for k, v in kt.items()}
+    log("kernel times (ms, launches): " + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in sorted(kt.items(), key=lambda x: -x[1][0])))
+
+    # build roofline: algorithmic bytes per point = input 24 + key/id write 12 + 24 per radix pass
+    # + boundary scan 8 + id write 4 (+ node records), over the build kernels
+    bits = 5 * 3 + 8  # per-set max depth 5 (sets <= 1e6 points, cap >= 8) + 8 set bits
+    passes = math.ceil(bits / 8)
+    build_names = [k for k in kt if k not in ("knn", "range_count", "range_fill", "compact_fixed", "offsets_u64")]
+    build_ms = sum(kt[k][0] for k in build_names)
+
+    out = {
+        "metric": METRIC,
+        "value": Q / b_knn,
+        "unit": "kNN queries/s (batched)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian mixtures)",
+        "config": {"workload": "batched uneven build+query: 256 sets, sizes log-uniform [1e3,1e6], 8-Gaussian mixtures "
+                               "in [0,1000]^3, 50k kNN (k=16) + 50k RangeSearch (edge 1-30) per set",
+                   "total_points": N, "knn_queries": Q, "range_queries": Q, "k": args.k, "leaf_capacity": cap,
+                   "autotune_step_s": tune, "parallelism": "single GPU"},
+        "batched": {"build_mpts_s": N / b_build / 1e6, "knn_qps": Q / b_knn, "range_qps": Q / b_range,
+                    "step_ms": b_step * 1e3, "build_ms": b_build * 1e3, "knn_ms": b_knn * 1e3, "range_ms": b_range * 1e3,
+                    "p50_step_ms": float(np.percentile([x[3] for x in bt], 50)) * 1e3,
+                    "p99_step_ms": float(np.percentile([x[3] for x in bt], 99)) * 1e3},
+        "kernel_ms": {k: v[0] for k, v in kt.items()},
+    }
+
+    if not args.no_plain and not args.profile_only:
+        pt, lat = [], []
+        plain_step(cap)  # warm
+        plain_step(cap, pt, lat)
+        tb, tk, tr, tt = pt[0]
+        out["plain"] = {"build_mpts_s": N / tb / 1e6, "knn_qps": Q / tk, "range_qps": Q / tr, "step_ms": tt * 1e3,
+                        "p50_set_ms": float(np.percentile(lat, 50)), "p99_set_ms": float(np.percentile(lat, 99))}
+        out["speedup_batched_vs_plain"] = {"knn": (Q / b_knn) / (Q / tk), "range": (Q / b_range) / (Q / tr),
+                                           "build": (N / b_build) / (N / tb), "step": tt / b_step}
+
+    # roofline of the dominant kernel
+    dom = max(kt, key=lambda k: kt[k][0])
+    dom_ms, dom_n = kt[dom]
+    avg_ms = dom_ms / dom_n
+    if dom == "knn":
+        # compulsory bytes per query: point 24 + set id 4 + |out| ids x 4 + every returned entity's
+        # coordinates and id (24 + 4) read at least once
+        per_q = 24 + 4 + args.k * 4 + args.k * 28
+        algo = per_q * Q
+    elif dom in ("range_count", "range_fill"):
+        algo = (48 + 4) * Q
+    else:
+        algo = (24 + 12 + 24 * passes + 8 + 4) * N
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo,
+                       "avg_launch_ms": avg_ms}
+
+    # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
+    if not args.no_cpu and not args.profile_only:
+        from oracle import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        done_q, t_q, done_p, t_b = 0, 0.0, 0, 0.0
+        s = 0
+        t_start = time.perf_counter()
+        while s < nsets and time.perf_counter() - t_start < args.cpu_seconds:
+            a, b = int(offs[s]), int(offs[s + 1])
+            t0 = time.perf_counter()
+            tree = O.OracleTree(3, "point", W["points"][a:b], W["box"], 0, cap)
+            t1 = time.perf_counter()
+            tree.knn(W["knn"][s * nq:(s + 1) * nq], args.k, threads=threads)
+            t2 = time.perf_counter()
+            t_b += t1 - t0
+            t_q += t2 - t1
+            done_p += b - a
+            done_q += nq
+            s += 1
+        out["cpu_baseline"] = {"value": done_q / t_q, "unit": "kNN queries/s", "cores": threads, "kind": "port",
+                               "sample": f"first {s} of {nsets} sets ({done_p} points, {done_q} kNN k={args.k}); "
+                                         f"build single-threaded {done_p / t_b / 1e6:.2f} Mpts/s",
+                               "build_mpts_s": done_p / t_b / 1e6}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,366 @@
+// orthotree_gpu.hpp -- C++20 drop-in for OrthoTree's static linear trees, backed
+// by libotg (MI355X / gfx950) through the C ABI in otg.h.
+//
+// Same class names, constructor and method signatures as the reference's
+// static-tree aliases and query layer:
+//   StaticOctreePoint / StaticOctreeBox / StaticQuadtree* / StaticOrthoTreePointND<D> /
+//   StaticOrthoTreeBoxND<D, true>                         (core/ot_aliases.h:82-91, :225-245)
+//   ctor(PAR_EXEC, entities, maxDepth, boxSpace, maxElementNo) and Create(...)
+//                                                         (core/ot_static_linear_core.h:103-129, :523-613)
+//   GetNearestNeighbors, RangeSearch, RayIntersectedFirst, CollisionDetection
+//                                                         (core/ot_query.h:1208-1270, :476-583, :2035-2146, :1901-1906)
+//   GetNodeCount / GetNodeChildren / GetNodeEntities / GetNodeMinPoint / GetNodeSize / GetNodeBox
+//                                                         (core/ot_static_linear_core.h:178-268)
+// plus batch overloads over spans of queries returning CSR results.
+//
+// The tree copies the entities into HBM at Create and keeps them resident; later
+// calls take the entity span only for signature parity.  Entity testers and
+// distance callbacks are not accepted (a host callback cannot run on the GPU).
+// Errors: like the reference (noexcept, std::terminate on allocation failure),
+// a failed device call prints the libotg status and terminates; Create returns
+// false for points outside an explicit box (where the reference never returns).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstdio>
+#include <exception>
+#include <limits>
+#include <optional>
+#include <span>
+#include <utility>
+#include <vector>
+
+#include "otg.h"
+
+namespace otg {
+
+using depth_t = uint32_t;
+using index_t = uint32_t;
+
+namespace ExecutionTags {
+struct Sequential {};
+struct Parallel {};
+}  // namespace ExecutionTags
+using SeqExec = ExecutionTags::Sequential;
+using ParExec = ExecutionTags::Parallel;
+inline constexpr SeqExec SEQ_EXEC{};
+inline constexpr ParExec PAR_EXEC{};
+inline constexpr ParExec GPU_EXEC{};
+
+template <int D>
+using PointND = std::array<double, D>;
+template <int D>
+struct BoundingBoxND {
+  PointND<D> Min, Max;
+};
+template <int D>
+struct RayND {
+  PointND<D> Origin, Direction;
+};
+using Point2D = PointND<2>;
+using Point3D = PointND<3>;
+using BoundingBox2D = BoundingBoxND<2>;
+using BoundingBox3D = BoundingBoxND<3>;
+
+enum class RangeSearchMode { Inside, Overlap };
+
+inline constexpr double BASE_TOLERANCE = std::numeric_limits<double>::epsilon() * 100.0;
+
+namespace detail {
+[[noreturn]] inline void fail(otg_status st, const char *what, const otg_ctx *ctx) {
+  std::fprintf(stderr, "libotg: %s failed: %s (%s)\n", what, otg_status_string(st), ctx ? otg_ctx_last_error(ctx) : "");
+  std::terminate();
+}
+inline void check(otg_status st, const char *what, const otg_ctx *ctx) {
+  if (st != OTG_OK) fail(st, what, ctx);
+}
+}  // namespace detail
+
+// One device + stream.  Context::Default() is shared by trees built without an explicit context.
+class Context {
+ public:
+  explicit Context(int device = 0) { detail::check(otg_ctx_create(device, &ctx_), "otg_ctx_create", nullptr); }
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  ~Context() { otg_ctx_destroy(ctx_); }
+  otg_ctx *get() const noexcept { return ctx_; }
+  static Context &Default() {
+    static Context c(0);
+    return c;
+  }
+
+ private:
+  otg_ctx *ctx_ = nullptr;
+};
+
+// CSR batch result: ids of query i are ids[offsets[i] .. offsets[i+1]).
+struct BatchResult {
+  std::vector<uint64_t> offsets;
+  std::vector<index_t> ids;
+  std::span<const index_t> operator[](size_t i) const {
+    return std::span<const index_t>(ids.data() + offsets[i], offsets[i + 1] - offsets[i]);
+  }
+  size_t size() const { return offsets.empty() ? 0 : offsets.size() - 1; }
+};
+
+template <int DIM, bool IS_BOX>
+class StaticOrthoTreeGPU {
+ public:
+  static constexpr int DIMENSION_NO = DIM;
+  using TVector = PointND<DIM>;
+  using TBox = BoundingBoxND<DIM>;
+  using TRay = RayND<DIM>;
+  using Entity = std::conditional_t<IS_BOX, TBox, TVector>;
+  using EntityContainerView = std::span<const Entity>;
+  using NodeID = uint32_t;
+  using EntityID = index_t;
+
+  StaticOrthoTreeGPU() = default;
+  explicit StaticOrthoTreeGPU(Context &ctx) noexcept : ctx_(&ctx) {}
+
+  template <typename TExecMode>
+  StaticOrthoTreeGPU(TExecMode, EntityContainerView entities, std::optional<depth_t> maxDepthID = std::nullopt,
+                     std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16) noexcept {
+    [[maybe_unused]] bool ok = Create(entities, maxDepthID, boxSpace, maxElementNoInNode);
+  }
+  explicit StaticOrthoTreeGPU(EntityContainerView entities, std::optional<depth_t> maxDepthID = std::nullopt,
+                              std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16) noexcept {
+    [[maybe_unused]] bool ok = Create(entities, maxDepthID, boxSpace, maxElementNoInNode);
+  }
+  StaticOrthoTreeGPU(const StaticOrthoTreeGPU &) = delete;
+  StaticOrthoTreeGPU &operator=(const StaticOrthoTreeGPU &) = delete;
+  StaticOrthoTreeGPU(StaticOrthoTreeGPU &&o) noexcept { *this = std::move(o); }
+  StaticOrthoTreeGPU &operator=(StaticOrthoTreeGPU &&o) noexcept {
+    std::swap(ctx_, o.ctx_);
+    std::swap(tree_, o.tree_);
+    std::swap(nodes_, o.nodes_);
+    std::swap(entity_count_, o.entity_count_);
+    return *this;
+  }
+  ~StaticOrthoTreeGPU() { otg_tree_free(tree_); }
+
+  // ---- build ------------------------------------------------------------------
+  template <typename TExecMode = ParExec>
+  bool Create(EntityContainerView entities, std::optional<depth_t> maxDepthIn = std::nullopt,
+              std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16,
+              TExecMode = {}) noexcept {
+    otg_tree_free(tree_);
+    tree_ = nullptr;
+    nodes_ = {};
+    std::array<double, 2 * DIM> box{};
+    if (boxSpace)
+      for (int d = 0; d < DIM; ++d) {
+        box[d] = boxSpace->Min[d];
+        box[DIM + d] = boxSpace->Max[d];
+      }
+    otg_status st = otg_tree_build(ctx().get(), IS_BOX ? OTG_KIND_BOX : OTG_KIND_POINT, DIM, entities.size(),
+                                   reinterpret_cast<const double *>(entities.data()), OTG_MEM_HOST,
+                                   boxSpace ? box.data() : nullptr, maxDepthIn.value_or(0),
+                                   static_cast<uint32_t>(maxElementNoInNode), &tree_);
+    if (st == OTG_ERR_OUT_OF_SPACE) return false;
+    detail::check(st, "otg_tree_build", ctx().get());
+    entity_count_ = entities.size();
+    return true;
+  }
+
+  template <typename TExecMode = ParExec>
+  static bool Create(StaticOrthoTreeGPU &tree, EntityContainerView entities, std::optional<depth_t> maxDepthIn = std::nullopt,
+                     std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16,
+                     TExecMode execMode = {}) noexcept {
+    return tree.Create(entities, maxDepthIn, boxSpace, maxElementNoInNode, execMode);
+  }
+
+  // ---- queries ------------------------------------------------------------------
+  std::vector<EntityID> GetNearestNeighbors(TVector const &searchPoint, std::size_t neighborNo, EntityContainerView entities,
+                                            double tolerance = BASE_TOLERANCE) const noexcept
+    requires(!IS_BOX)
+  {
+    return GetNearestNeighbors(searchPoint, neighborNo, std::numeric_limits<double>::max(), entities, tolerance);
+  }
+  std::vector<EntityID> GetNearestNeighbors(TVector const &searchPoint, std::size_t neighborNo, double maxDistanceWithin,
+                                            EntityContainerView, double tolerance = BASE_TOLERANCE) const noexcept
+    requires(!IS_BOX)
+  {
+    auto r = GetNearestNeighbors(std::span<const TVector>(&searchPoint, 1), neighborNo, maxDistanceWithin, tolerance);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
+  }
+  // batch overload
+  BatchResult GetNearestNeighbors(std::span<const TVector> searchPoints, std::size_t neighborNo,
+                                  double maxDistanceWithin = std::numeric_limits<double>::max(),
+                                  double tolerance = BASE_TOLERANCE) const noexcept
+    requires(!IS_BOX)
+  {
+    otg_result *res = nullptr;
+    if (!tree_) return empty(searchPoints.size());
+    detail::check(otg_knn(tree_, searchPoints.size(), reinterpret_cast<const double *>(searchPoints.data()), nullptr,
+                          OTG_MEM_HOST, static_cast<uint32_t>(neighborNo), maxDistanceWithin, tolerance, &res),
+                  "otg_knn", ctx().get());
+    return take(res);
+  }
+
+  std::vector<EntityID> RangeSearch(TBox const &range, EntityContainerView, RangeSearchMode mode = RangeSearchMode::Inside,
+                                    double tolerance = BASE_TOLERANCE) const noexcept {
+    auto r = RangeSearch(std::span<const TBox>(&range, 1), mode, tolerance);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
+  }
+  BatchResult RangeSearch(std::span<const TBox> ranges, RangeSearchMode mode = RangeSearchMode::Inside,
+                          double tolerance = BASE_TOLERANCE) const noexcept {
+    otg_result *res = nullptr;
+    if (!tree_) return empty(ranges.size());
+    detail::check(otg_range(tree_, ranges.size(), reinterpret_cast<const double *>(ranges.data()), nullptr, OTG_MEM_HOST,
+                            mode == RangeSearchMode::Inside ? OTG_RANGE_INSIDE : OTG_RANGE_OVERLAP, tolerance, &res),
+                  "otg_range", ctx().get());
+    return take(res);
+  }
+
+  std::vector<EntityID> RayIntersectedFirst(TVector const &rayBasePoint, TVector const &rayHeading, EntityContainerView,
+                                            double tolerance = BASE_TOLERANCE, double toleranceIncrement = 0.0,
+                                            double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    TRay ray{rayBasePoint, rayHeading};
+    auto r = RayIntersectedFirst(std::span<const TRay>(&ray, 1), tolerance, toleranceIncrement, maxDistance);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
+  }
+  std::vector<EntityID> RayIntersectedFirst(TRay const &ray, EntityContainerView entities, double tolerance = BASE_TOLERANCE,
+                                            double toleranceIncrement = 0.0,
+                                            double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    return RayIntersectedFirst(ray.Origin, ray.Direction, entities, tolerance, toleranceIncrement, maxDistance);
+  }
+  BatchResult RayIntersectedFirst(std::span<const TRay> rays, double tolerance = BASE_TOLERANCE,
+                                  double toleranceIncrement = 0.0,
+                                  double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    otg_result *res = nullptr;
+    if (!tree_) return empty(rays.size());
+    detail::check(otg_ray_first(tree_, rays.size(), reinterpret_cast<const double *>(rays.data()), nullptr, OTG_MEM_HOST,
+                                tolerance, toleranceIncrement, maxDistance, &res),
+                  "otg_ray_first", ctx().get());
+    return take(res);
+  }
+
+  template <typename TExecMode = ParExec>
+  std::vector<std::pair<EntityID, EntityID>> CollisionDetection(EntityContainerView, double tolerance = BASE_TOLERANCE,
+                                                                TExecMode = {}) const noexcept
+    requires(IS_BOX)
+  {
+    std::vector<std::pair<EntityID, EntityID>> out;
+    if (!tree_) return out;
+    otg_result *res = nullptr;
+    detail::check(otg_collide(tree_, 0, tolerance, &res), "otg_collide", ctx().get());
+    BatchResult r = take(res);
+    out.resize(r.ids.size() / 2);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = {r.ids[2 * i], r.ids[2 * i + 1]};
+    return out;
+  }
+
+  // ---- node API (host copy of the device node table, made on first use) ----------------
+  std::size_t GetNodeCount() const noexcept { return nodes().depth.size(); }
+  depth_t GetMaxDepthID() const noexcept { return nodes().max_depth; }
+  static constexpr NodeID GetRootNodeID() noexcept { return 0; }
+  std::span<const EntityID> GetNodeEntities(NodeID node) const noexcept {
+    const auto &n = nodes();
+    return std::span<const EntityID>(n.ids.data() + n.ebegin[node], n.ecount[node]);
+  }
+  std::pair<NodeID, NodeID> GetNodeChildren(NodeID node) const noexcept {
+    const auto &n = nodes();
+    return {n.cbegin[node], n.cbegin[node] + n.ccount[node]};
+  }
+  bool IsNodeEntitiesEmpty(NodeID node) const noexcept { return nodes().ecount[node] == 0; }
+  TVector GetNodeMinPoint(NodeID node) const noexcept { return vec(nodes().nmin, node); }
+  TVector GetNodeSize(NodeID node) const noexcept { return vec(nodes().nsize, node); }
+  TBox GetNodeBox(NodeID node) const noexcept {
+    TBox b;
+    b.Min = GetNodeMinPoint(node);
+    TVector s = GetNodeSize(node);
+    for (int d = 0; d < DIM; ++d) b.Max[d] = b.Min[d] + s[d];
+    return b;
+  }
+  depth_t GetNodeDepthID(NodeID node) const noexcept { return nodes().depth[node]; }
+  std::span<const EntityID> GetEntityStorage() const noexcept { return nodes().ids; }
+  static constexpr bool AreChildNodesOverlapping() noexcept { return IS_BOX; }
+
+  otg_tree *native_handle() const noexcept { return tree_; }
+
+ private:
+  struct HostNodes {
+    bool loaded = false;
+    depth_t max_depth = 0;
+    std::vector<uint8_t> depth;
+    std::vector<uint32_t> ebegin, ecount, cbegin, ccount;
+    std::vector<double> nmin, nsize;
+    std::vector<EntityID> ids;
+  };
+
+  Context &ctx() const noexcept { return ctx_ ? *ctx_ : Context::Default(); }
+
+  const HostNodes &nodes() const noexcept {
+    if (!nodes_.loaded && tree_) {
+      otg_set_info si{};
+      detail::check(otg_tree_get_set_info(tree_, 0, &si), "otg_tree_get_set_info", ctx().get());
+      HostNodes &n = nodes_;
+      size_t nn = si.node_count;
+      n.max_depth = si.max_depth;
+      n.depth.resize(nn);
+      n.ebegin.resize(nn);
+      n.ecount.resize(nn);
+      n.cbegin.resize(nn);
+      n.ccount.resize(nn);
+      n.nmin.resize(nn * DIM);
+      n.nsize.resize(nn * DIM);
+      n.ids.resize(si.entity_count);
+      detail::check(otg_tree_export(tree_, 0, n.depth.data(), n.ebegin.data(), n.ecount.data(), n.cbegin.data(),
+                                    n.ccount.data(), nullptr, n.nmin.data(), n.nsize.data(), n.ids.data(), nullptr),
+                    "otg_tree_export", ctx().get());
+      n.loaded = true;
+    }
+    return nodes_;
+  }
+
+  static TVector vec(const std::vector<double> &a, NodeID node) {
+    TVector v;
+    for (int d = 0; d < DIM; ++d) v[d] = a[(size_t)node * DIM + d];
+    return v;
+  }
+
+  static BatchResult empty(size_t nq) {
+    BatchResult r;
+    r.offsets.assign(nq + 1, 0);
+    return r;
+  }
+
+  BatchResult take(otg_result *res) const noexcept {
+    BatchResult r;
+    uint64_t nq = 0, nids = 0;
+    detail::check(otg_result_get_info(res, &nq, &nids), "otg_result_get_info", ctx().get());
+    r.offsets.resize(nq + 1);
+    r.ids.resize(nids);
+    detail::check(otg_result_copy(res, r.offsets.data(), r.ids.data()), "otg_result_copy", ctx().get());
+    otg_result_free(res);
+    return r;
+  }
+
+  Context *ctx_ = nullptr;
+  otg_tree *tree_ = nullptr;
+  mutable HostNodes nodes_;
+  std::size_t entity_count_ = 0;
+};
+
+// core/ot_aliases.h:82-91, :225-245
+template <int DIM>
+using StaticOrthoTreePointND = StaticOrthoTreeGPU<DIM, false>;
+namespace detail {
+template <int DIM, bool IS_LOOSE_TREE>
+struct BoxTreeOf {
+  static_assert(IS_LOOSE_TREE, "the GPU path builds loose (LOOSE_FACTOR 2.0) box trees only");
+  using type = StaticOrthoTreeGPU<DIM, true>;
+};
+}  // namespace detail
+template <int DIM, bool IS_LOOSE_TREE = true>
+using StaticOrthoTreeBoxND = typename detail::BoxTreeOf<DIM, IS_LOOSE_TREE>::type;
+using StaticQuadtreePoint = StaticOrthoTreePointND<2>;
+using StaticQuadtreeBox = StaticOrthoTreeBoxND<2, true>;
+using StaticOctreePoint = StaticOrthoTreePointND<3>;
+using StaticOctreeBox = StaticOrthoTreeBoxND<3, true>;
+using StaticHexatreePoint = StaticOrthoTreePointND<4>;
+using StaticHexatreeBox = StaticOrthoTreeBoxND<4, true>;
+
+}  // namespace otg

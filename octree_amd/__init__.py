@@ -12,7 +12,7 @@ Geometry is numpy float64: points (n, D); boxes (n, 2, D) or (n, 2*D) as
 from __future__ import annotations
 
 import ctypes as C
-import sys
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -50,6 +50,7 @@ class Context:
         if st != _abi.OTG_OK:
             raise OtgError(st, "otg_ctx_create")
         self.handle = h
+        self._children = weakref.WeakSet()  # trees / results that must be freed before the context
 
     def synchronize(self) -> None:
         _check(_abi.lib().otg_ctx_synchronize(self.handle), "otg_ctx_synchronize", self)
@@ -81,6 +82,8 @@ class Context:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
+            for child in list(getattr(self, "_children", ())):
+                child.free()
             _abi.lib().otg_ctx_destroy(self.handle)
             self.handle = None
 
@@ -107,6 +110,7 @@ class Result:
     def __init__(self, handle, ctx: Context):
         self.handle = handle
         self.ctx = ctx
+        ctx._children.add(self)
 
     def csr(self):
         L = _abi.lib()
@@ -131,6 +135,19 @@ class Result:
             self.free()
         except Exception:
             pass
+
+
+def _is_device(a) -> bool:
+    """Objects exposing data_ptr() (e.g. torch device tensors) are passed as device memory."""
+    return hasattr(a, "data_ptr") and getattr(getattr(a, "device", None), "type", "cpu") != "cpu"
+
+
+def _dev(a, cols: int):
+    """(pointer, rows) of a contiguous device float64/int32 tensor."""
+    if not a.is_contiguous():
+        raise ValueError("device input must be contiguous")
+    rows = a.numel() // cols if cols else a.numel()
+    return C.c_void_p(a.data_ptr()), rows
 
 
 def _f64(a, cols: int) -> np.ndarray:
@@ -171,16 +188,22 @@ class StaticTree:
                max_elements: int = 16, exec_tag=PAR_EXEC) -> bool:
         """StaticLinearOrthoTreeCore::Create<ParExec> (ot_static_linear_core.h:523-599)."""
         self.free()
-        geom = _f64(entities, self._cols())
+        if _is_device(entities):
+            gptr, n = _dev(entities, self._cols())
+            mem = _abi.OTG_MEM_DEVICE
+        else:
+            geom = _f64(entities, self._cols())
+            gptr, n, mem = _ptr(geom), geom.shape[0], _abi.OTG_MEM_HOST
         box = None if box_space is None else _f64(box_space, 2 * self.dim).reshape(-1)
         h = C.c_void_p()
         L = _abi.lib()
         kind = _abi.OTG_KIND_POINT if self.kind == "point" else _abi.OTG_KIND_BOX
-        st = L.otg_tree_build(self.ctx.handle, kind, self.dim, geom.shape[0], _ptr(geom), _abi.OTG_MEM_HOST,
-                              _ptr(box), int(max_depth or 0), int(max_elements), C.byref(h))
+        st = L.otg_tree_build(self.ctx.handle, kind, self.dim, n, gptr, mem, _ptr(box), int(max_depth or 0),
+                              int(max_elements), C.byref(h))
         _check(st, "otg_tree_build", self.ctx)
         self.handle = h
-        self.n = geom.shape[0]
+        self.ctx._children.add(self)
+        self.n = n
         return True
 
     @classmethod
@@ -193,11 +216,18 @@ class StaticTree:
         self.dim = dim or cls.DIM
         self.handle = None
         cols = self._cols()
-        arrs = [_f64(s, cols) for s in sets]
-        ns = len(arrs)
-        offs = np.zeros(ns + 1, dtype=np.uint64)
-        offs[1:] = np.cumsum([a.shape[0] for a in arrs])
-        geom = np.concatenate(arrs) if arrs else np.zeros((0, cols))
+        if device_geom is not None:  # (device tensor of all sets, host offsets)
+            dgeom, offs = device_geom
+            offs = np.ascontiguousarray(np.asarray(offs, dtype=np.uint64))
+            ns = len(offs) - 1
+            gptr, mem = C.c_void_p(dgeom.data_ptr()), _abi.OTG_MEM_DEVICE
+        else:
+            arrs = [_f64(s, cols) for s in sets]
+            ns = len(arrs)
+            offs = np.zeros(ns + 1, dtype=np.uint64)
+            offs[1:] = np.cumsum([a.shape[0] for a in arrs])
+            geom = np.concatenate(arrs) if arrs else np.zeros((0, cols))
+            gptr, mem = _ptr(geom), _abi.OTG_MEM_HOST
         caps = np.ascontiguousarray(np.broadcast_to(np.asarray(max_elements, dtype=np.uint32), (ns,)))
         deps = None if max_depths is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(max_depths, dtype=np.uint32), (ns,)))
@@ -205,11 +235,12 @@ class StaticTree:
         h = C.c_void_p()
         kindc = _abi.OTG_KIND_POINT if self.kind == "point" else _abi.OTG_KIND_BOX
         st = _abi.lib().otg_forest_build(self.ctx.handle, kindc, self.dim, ns,
-                                         offs.ctypes.data_as(_abi.U64P), _ptr(geom), _abi.OTG_MEM_HOST, _ptr(bxs),
+                                         offs.ctypes.data_as(_abi.U64P), gptr, mem, _ptr(bxs),
                                          _ptr(deps), caps.ctypes.data_as(_abi.U32P), C.byref(h))
         _check(st, "otg_forest_build", self.ctx)
         self.handle = h
-        self.n = geom.shape[0]
+        self.ctx._children.add(self)
+        self.n = int(offs[-1])
         return self
 
     def free(self):
@@ -262,35 +293,42 @@ class StaticTree:
     def _sets_arg(self, set_ids, nq):
         if set_ids is None:
             return None
+        if _is_device(set_ids):
+            return set_ids
         return np.ascontiguousarray(np.broadcast_to(np.asarray(set_ids, dtype=np.uint32), (nq,)))
+
+    def _query_args(self, q, cols, set_ids):
+        """(pointer, nq, set pointer, mem, keepalive) for host arrays or device tensors."""
+        if _is_device(q):
+            qp, nq = _dev(q, cols)
+            sp = None if set_ids is None else C.c_void_p(set_ids.data_ptr())
+            return qp, nq, sp, _abi.OTG_MEM_DEVICE, (q, set_ids)
+        qa = _f64(q, cols)
+        sids = self._sets_arg(set_ids, qa.shape[0])
+        return _ptr(qa), qa.shape[0], _ptr(sids), _abi.OTG_MEM_HOST, (qa, sids)
 
     def knn_batch(self, points, k: int, max_within: float = DBL_MAX, tolerance: float = BASE_TOLERANCE,
                   set_ids=None) -> Result:
-        q = _f64(points, self.dim)
-        sids = self._sets_arg(set_ids, q.shape[0])
+        qp, nq, sp, mem, _keep = self._query_args(points, self.dim, set_ids)
         h = C.c_void_p()
-        st = _abi.lib().otg_knn(self.handle, q.shape[0], _ptr(q), _ptr(sids), _abi.OTG_MEM_HOST, int(k),
-                                float(max_within), float(tolerance), C.byref(h))
+        st = _abi.lib().otg_knn(self.handle, nq, qp, sp, mem, int(k), float(max_within), float(tolerance), C.byref(h))
         _check(st, "otg_knn", self.ctx)
         return Result(h, self.ctx)
 
     def range_batch(self, ranges, mode: int = _abi.OTG_RANGE_INSIDE, tolerance: float = BASE_TOLERANCE,
                     set_ids=None) -> Result:
-        q = _f64(ranges, 2 * self.dim)
-        sids = self._sets_arg(set_ids, q.shape[0])
+        qp, nq, sp, mem, _keep = self._query_args(ranges, 2 * self.dim, set_ids)
         h = C.c_void_p()
-        st = _abi.lib().otg_range(self.handle, q.shape[0], _ptr(q), _ptr(sids), _abi.OTG_MEM_HOST, int(mode),
-                                  float(tolerance), C.byref(h))
+        st = _abi.lib().otg_range(self.handle, nq, qp, sp, mem, int(mode), float(tolerance), C.byref(h))
         _check(st, "otg_range", self.ctx)
         return Result(h, self.ctx)
 
     def ray_first_batch(self, rays, tolerance: float = BASE_TOLERANCE, tolerance_increment: float = 0.0,
                         max_distance: float = DBL_MAX, set_ids=None) -> Result:
-        q = _f64(rays, 2 * self.dim)
-        sids = self._sets_arg(set_ids, q.shape[0])
+        qp, nq, sp, mem, _keep = self._query_args(rays, 2 * self.dim, set_ids)
         h = C.c_void_p()
-        st = _abi.lib().otg_ray_first(self.handle, q.shape[0], _ptr(q), _ptr(sids), _abi.OTG_MEM_HOST,
-                                      float(tolerance), float(tolerance_increment), float(max_distance), C.byref(h))
+        st = _abi.lib().otg_ray_first(self.handle, nq, qp, sp, mem, float(tolerance), float(tolerance_increment),
+                                      float(max_distance), C.byref(h))
         _check(st, "otg_ray_first", self.ctx)
         return Result(h, self.ctx)
 

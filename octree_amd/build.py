@@ -74,12 +74,27 @@ def build_oracle() -> Path:
     return root / "oracle" / "liboc.so"
 
 
+def build_dropin_test() -> Path:
+    """tests/cpp/dropin_test: the C++20 drop-in header compiled against libotg."""
+    root = HERE.parent
+    src = root / "tests" / "cpp" / "dropin_test.cpp"
+    out = src.with_suffix("")
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++20", "-O2", "-I", str(root / "include"), str(src), "-L", str(HERE), "-lotg",
+           "-Wl,-rpath,$ORIGIN/../../octree_amd", "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"drop-in test build failed:\n{res.stdout}\n{res.stderr}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
     args = ap.parse_args(argv)
     print(build(args.jobs))
     print(build_oracle())
+    print(build_dropin_test())
     return 0
 
 

@@ -91,7 +91,8 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
   if (sp.node_cnt != 0 && k != 0) {
     double F = F0;
     bool heapified = false;
-    double dropped = DBL_MAX;  // smallest distance pushed out of the bounded list
+    double dropped = 0.0;  // smallest distance pushed out of the bounded list
+    bool any_dropped = false;
     uint32_t stack[StackCap<D>::value];
     int top = 0;
     stack[top++] = (uint32_t)sp.node_off;
@@ -115,7 +116,11 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
 #pragma unroll
         for (int t = 0; t < KCAP; ++t)
           if ((uint32_t)t < cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
-        if (cnt == KCAP) dropped = smin(dropped, bd[KCAP - 1]);
+        if (cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
+          double out_d = (pos == (uint32_t)KCAP) ? dist : bd[KCAP - 1];
+          dropped = any_dropped ? smin(dropped, out_d) : out_d;
+          any_dropped = true;
+        }
 #pragma unroll
         for (int t = KCAP - 1; t >= 1; --t)
           if ((uint32_t)t > pos && (uint32_t)t <= cnt) {
@@ -172,7 +177,7 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
         for (int c = 0; c < nc; ++c) stack[top++] = cid[c];
       }
     }
-    if (dropped <= F) atomicOr(overflow, 1u);
+    if (any_dropped && dropped <= F) atomicOr(overflow, 1u);
   }
   out_cnt[qi] = cnt;
 #pragma unroll

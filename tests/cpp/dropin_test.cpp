@@ -1,0 +1,121 @@
+// Exercises include/otg/orthotree_gpu.hpp the way reference user code calls the
+// static trees (core/ot_aliases.h names, PAR_EXEC constructor, query methods),
+// on the reference's own known-answer data (tests/unittests/general.tests.cpp).
+// Prints "DROPIN OK" and exits 0 when every check passes.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "otg/orthotree_gpu.hpp"
+
+using namespace otg;
+
+static int failures = 0;
+#define STEP(msg)                   \
+  do {                              \
+    std::printf("step: %s\n", msg); \
+    std::fflush(stdout);            \
+  } while (0)
+#define EXPECT(cond)                                                  \
+  do {                                                                \
+    if (!(cond)) {                                                    \
+      std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);   \
+      ++failures;                                                     \
+    }                                                                 \
+  } while (0)
+
+static bool same_set(std::vector<index_t> a, std::vector<index_t> b) {
+  std::sort(a.begin(), a.end());
+  std::sort(b.begin(), b.end());
+  return a == b;
+}
+
+int main() {
+  // getPointSetNo1<2>() (general.tests.cpp:72-130), tree (points, 3, nullopt, 3)
+  std::vector<Point2D> pts = {{0.0, 0.0}, {3.0, 3.0}, {5.0, 3.0}, {5.2, 3.0}, {7.0, 1.0}, {0.0, 5.0}, {1.0, 5.0},
+                              {0.0, 7.0}, {1.0, 7.0}, {3.0, 7.0}, {3.0, 7.0}, {2.2, 4.5}, {2.5, 4.5}, {3.4, 4.5},
+                              {3.6, 4.6}, {2.4, 5.5}, {2.5, 5.5}, {3.5, 5.5}, {3.5, 5.5}, {5.0, 5.0}, {5.0, 5.0},
+                              {7.0, 5.0}, {7.0, 5.0}, {6.5, 6.5}, {6.5, 6.5}, {7.5, 6.5}, {7.5, 6.5}, {6.5, 7.5},
+                              {6.5, 7.5}, {7.5, 7.5}, {7.5, 7.5}, {8.0, 8.0}};
+  STEP("build quadtree points");
+  StaticQuadtreePoint qt(PAR_EXEC, pts, 3, std::nullopt, 3);
+  STEP("node count");
+  EXPECT(qt.GetNodeCount() > 1);
+  EXPECT(qt.GetMaxDepthID() == 3);
+  STEP("knn");
+  EXPECT(same_set(qt.GetNearestNeighbors({3.5, 5.5}, 3, pts), {17, 18, 14}));            // :2488
+  EXPECT(same_set(qt.GetNearestNeighbors({3.5, 5.5}, 4, 0.0, pts), {17, 18}));           // :2510
+  EXPECT(same_set(qt.GetNearestNeighbors({-1.0, -1.0}, 5, pts), {0, 1, 5, 6, 11}));      // :2593
+  EXPECT(same_set(qt.GetNearestNeighbors({2.0, 2.0}, 2, pts), {1, 11}));                 // :2558
+  STEP("node api");
+  // node API: children ranges cover every node once, entities cover every id once
+  {
+    std::vector<int> seen(qt.GetNodeCount(), 0), ent(pts.size(), 0);
+    seen[0] = 1;
+    for (uint32_t i = 0; i < qt.GetNodeCount(); ++i) {
+      auto [b, e] = qt.GetNodeChildren(i);
+      for (auto c = b; c < e; ++c) ++seen[c];
+      for (auto id : qt.GetNodeEntities(i)) ++ent[id];
+    }
+    EXPECT(std::all_of(seen.begin(), seen.end(), [](int x) { return x == 1; }));
+    EXPECT(std::all_of(ent.begin(), ent.end(), [](int x) { return x == 1; }));
+  }
+  STEP("range");
+  auto r = qt.RangeSearch(BoundingBox2D{{2.0, 4.0}, {4.0, 6.0}}, pts);
+  std::vector<index_t> expect_r;
+  for (index_t i = 0; i < pts.size(); ++i)
+    if (pts[i][0] > 2.0 - BASE_TOLERANCE && pts[i][0] < 4.0 + BASE_TOLERANCE && pts[i][1] > 4.0 - BASE_TOLERANCE &&
+        pts[i][1] < 6.0 + BASE_TOLERANCE)
+      expect_r.push_back(i);
+  EXPECT(same_set(r, expect_r));
+
+  // ray KATs on the 4-box diagonal (general.tests.cpp:1360-1453), QuadtreeBox(boxes, 3, nullopt, 2)
+  std::vector<BoundingBox2D> boxes = {{{0, 0}, {1, 1}}, {{1, 1}, {2, 2}}, {{2, 2}, {3, 3}}, {{3, 3}, {4, 4}}};
+  STEP("box tree + rays");
+  StaticQuadtreeBox qb(PAR_EXEC, boxes, 3, std::nullopt, 2);
+  EXPECT((qb.RayIntersectedFirst({0.0, 0.5}, {1.0, 0.0}, boxes, 0.0) == std::vector<index_t>{0}));
+  EXPECT((qb.RayIntersectedFirst({0.0, 2.0}, {1.0, 0.0}, boxes, 0.0) == std::vector<index_t>{2, 1}));
+  EXPECT((qb.RayIntersectedFirst({1.5, 1.0}, {-1.0, 0.0}, boxes, 0.0) == std::vector<index_t>{0, 1}));
+  EXPECT(qb.RayIntersectedFirst({0.0, -1.0}, {1.0, 0.0}, boxes, 0.0).empty());
+  EXPECT((qb.RayIntersectedFirst(RayND<2>{{3.5, 3.5}, {-1.0 / std::sqrt(5.0), -2.0 / std::sqrt(5.0)}}, boxes, 0.0) ==
+          std::vector<index_t>{3}));
+
+  // collision: overlapping 3D boxes (strict, with tolerance), compared with brute force
+  std::vector<BoundingBox3D> bx;
+  for (int i = 0; i < 200; ++i) {
+    double x = (i * 37 % 100) * 0.1, y = (i * 53 % 100) * 0.1, z = (i * 71 % 100) * 0.1;
+    bx.push_back({{x, y, z}, {x + 0.8, y + 0.6, z + 0.7}});
+  }
+  STEP("collision");
+  StaticOctreeBox ob(PAR_EXEC, bx, 5, std::nullopt, 4);
+  STEP("collision: built");
+  std::printf("nodes %zu\n", ob.GetNodeCount());
+  auto pairs = ob.CollisionDetection<ParExec>(bx);
+  STEP("collision: detected");
+  std::vector<std::pair<index_t, index_t>> brute;
+  for (index_t i = 0; i < bx.size(); ++i)
+    for (index_t j = i + 1; j < bx.size(); ++j) {
+      bool ov = true;
+      for (int d = 0; d < 3; ++d)
+        ov = ov && (bx[i].Min[d] + BASE_TOLERANCE < bx[j].Max[d]) && (bx[j].Min[d] + BASE_TOLERANCE < bx[i].Max[d]);
+      if (ov) brute.push_back({i, j});
+    }
+  EXPECT(pairs == brute);
+
+  // batch overload
+  std::vector<Point3D> p3;
+  for (int64_t i = 0; i < 5000; ++i)
+    p3.push_back({double(i * 7919 % 1000) * 0.1, double(i * 104729 % 1000) * 0.1, double(i * 1299709 % 1000) * 0.1});
+  STEP("batch knn");
+  StaticOctreePoint op(PAR_EXEC, p3, 6, BoundingBox3D{{0, 0, 0}, {100, 100, 100}}, 16);
+  std::vector<Point3D> q = {{50, 50, 50}, {1, 2, 3}, {99, 0, 42}};
+  auto br = op.GetNearestNeighbors(std::span<const Point3D>(q), 8);
+  EXPECT(br.size() == 3);
+  for (size_t i = 0; i < q.size(); ++i) {
+    auto one = op.GetNearestNeighbors(q[i], 8, p3);
+    EXPECT(std::vector<index_t>(br[i].begin(), br[i].end()) == one);
+  }
+  if (failures == 0) std::printf("DROPIN OK\n");
+  return failures ? 1 : 0;
+}
