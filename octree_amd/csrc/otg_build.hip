@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "otg_device.hpp"
 #include "otg_internal.hpp"
 
 namespace otg {
@@ -51,20 +52,6 @@ __device__ __forceinline__ uint32_t set_of(const uint64_t *set_off, uint32_t nse
   return lo;
 }
 
-// si_morton.h:498-574: bit i of g[d] -> bit i*D + d
-template <int D>
-__device__ __forceinline__ uint64_t morton_encode(const uint32_t (&g)[D]) {
-  uint64_t code = 0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    uint64_t x = g[d];
-    for (int i = 0; i < 32 && (i * D + d) < 64; ++i) code |= ((x >> i) & 1ull) << (i * D + d);
-  }
-  return code;
-}
-
-__device__ __forceinline__ uint32_t bw64(uint64_t x) { return x ? 64u - (uint32_t)__clzll(x) : 0u; }
-
 // common leading Morton digits of two codes of a depth-M tree (LCA depth)
 template <int D>
 __device__ __forceinline__ uint32_t lcp_digits(uint64_t a, uint64_t b, uint32_t M) {
@@ -80,6 +67,8 @@ struct KeyView {
     return dbits ? (uint32_t)(k & ((1ull << dbits) - 1)) : M;
   }
   __device__ uint64_t set_base(uint32_t s) const { return (cbits + dbits >= 64) ? 0ull : ((uint64_t)s << (cbits + dbits)); }
+  // set id of a sorted key (no search over set offsets needed after the sort)
+  __device__ uint32_t set(uint64_t k) const { return (cbits + dbits >= 64) ? 0u : (uint32_t)(k >> (cbits + dbits)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -233,11 +222,11 @@ __global__ void __launch_bounds__(kB) k_levels(const uint64_t *__restrict__ keys
   uint32_t s = 0;
   bool interior = false;
   if (i > 0 && i < n) {
-    s = set_of(set_off, nsets, i);
-    if (sets[s].ent_off != i) {
+    const uint64_t ka = keys[i - 1], kb = keys[i];
+    s = kv.set(kb);
+    if (kv.set(ka) == s) {
       interior = true;
       const uint32_t M = sets[s].max_depth;
-      uint64_t ka = keys[i - 1], kb = keys[i];
       uint32_t L = lcp_digits<D>(kv.code(ka), kv.code(kb), M);
       uint32_t da = kv.depth(ka, M), db = kv.depth(kb, M);
       L = L < da ? L : da;
@@ -246,22 +235,9 @@ __global__ void __launch_bounds__(kB) k_levels(const uint64_t *__restrict__ keys
     }
   }
   v[i] = (uint8_t)val;
-  // per-set minimum interior level (coded), wave-aggregated
-  const uint64_t act = __ballot(interior);
-  if (!act) return;
-  const int lane = threadIdx.x & 63;
-  uint32_t s0 = __shfl(s, __ffsll((long long)act) - 1, 64);
-  bool same = __all(!interior || s == s0);
-  if (same) {
-    uint32_t m = interior ? val : 255u;
-    for (int off = 32; off > 0; off >>= 1) {
-      uint32_t o = __shfl_xor(m, off, 64);
-      m = o < m ? o : m;
-    }
-    if (lane == __ffsll((long long)act) - 1) atomicMin(&sets[s0].min_lcp, m);
-  } else if (interior) {
-    atomicMin(&sets[s].min_lcp, val);
-  }
+  // a level-0 boundary means the root itself branches (no extra root above a
+  // single deeper child); min_lcp becomes 1 then, and stays 255 otherwise
+  if (interior && val == 1u) sets[s].min_lcp = 1u;
 }
 
 // per-set: does the root split?  (count > cap, and for boxes not every entity at
@@ -304,14 +280,14 @@ __device__ __forceinline__ uint64_t upper_bound_u64(const uint64_t *a, uint64_t 
 
 template <bool BOX>
 __global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ v,
-                                                uint64_t n, const uint64_t *__restrict__ set_off, uint32_t nsets,
-                                                const SetParams *__restrict__ sets, uint8_t *__restrict__ active) {
+                                                uint64_t n, KeyView kv, const SetParams *__restrict__ sets,
+                                                uint8_t *__restrict__ active) {
   uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (i > n) return;
   uint8_t a = 0;
   const uint32_t vi = (i > 0 && i < n) ? v[i] : 0u;
   if (vi) {
-    const uint32_t s = set_of(set_off, nsets, i);
+    const uint32_t s = kv.set(keys[i]);
     const SetParams &sp = sets[s];
     const uint32_t L = vi - 1;
     if (L < sp.max_depth) {
@@ -358,12 +334,12 @@ __device__ __forceinline__ uint32_t internal_gen(const uint32_t *lm, const uint3
 // (extra root + internal representative)
 __global__ void __launch_bounds__(kB) k_event_flags(const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
                                                      const uint32_t *__restrict__ lm, uint64_t n,
-                                                     const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                     const uint64_t *__restrict__ keys, KeyView kv,
                                                      const SetParams *__restrict__ sets, uint32_t *__restrict__ seg_flag,
                                                      uint32_t *__restrict__ int_flag) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (p >= n) return;
-  const uint32_t s = set_of(set_off, nsets, p);
+  const uint32_t s = kv.set(keys[p]);
   const SetParams &sp = sets[s];
   const bool set_start = (sp.ent_off == p);
   uint32_t ic = 0;
@@ -396,7 +372,7 @@ __global__ void __launch_bounds__(kB) k_seg_nodes(const uint32_t *__restrict__ s
   if (BOX) {
     uint64_t s = seg_list[k], e = seg_list[k + 1];
     if (e - s == 1) {
-      uint32_t st = set_of(set_off, nsets, s);
+      uint32_t st = kv.set(keys[s]);
       uint32_t M = sets[st].max_depth;
       uint32_t vs = (sets[st].ent_off == s) ? 0u : v[s];
       uint32_t ve = v[e];
@@ -420,12 +396,12 @@ constexpr uint32_t kTypeRoot = 0u, kTypeInternal = 1u, kTypeLeaf = 2u;
 __global__ void __launch_bounds__(kB) k_emit_internal(const uint32_t *__restrict__ int_flag, const uint32_t *__restrict__ int_idx,
                                                        const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
                                                        const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
-                                                       uint64_t n, const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                       uint64_t n, const uint64_t *__restrict__ keys, KeyView kv,
                                                        const SetParams *__restrict__ sets, NodeKeyLayout nk,
                                                        uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (p >= n || !int_flag[p]) return;
-  const uint32_t s = set_of(set_off, nsets, p);
+  const uint32_t s = kv.set(keys[p]);
   const SetParams &sp = sets[s];
   uint32_t out = int_idx[p];
   const bool set_start = (sp.ent_off == p);
@@ -445,13 +421,13 @@ __global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__
                                                      const uint32_t *__restrict__ seg_node, const uint32_t *__restrict__ seg_node_idx,
                                                      uint32_t out_base, const uint8_t *__restrict__ v,
                                                      const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
-                                                     const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                     const uint64_t *__restrict__ keys, KeyView kv,
                                                      const SetParams *__restrict__ sets, NodeKeyLayout nk,
                                                      uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
   uint32_t k = blockIdx.x * kB + threadIdx.x;
   if (k >= nseg || !seg_node[k]) return;
   uint64_t s0 = seg_list[k], e = seg_list[k + 1];
-  const uint32_t s = set_of(set_off, nsets, s0);
+  const uint32_t s = kv.set(keys[s0]);
   const SetParams &sp = sets[s];
   const bool starts = (sp.ent_off == s0);
   const bool ends = (sp.ent_off + sp.ent_cnt == e);
@@ -589,7 +565,8 @@ __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nk
 // sorted SoA geometry and local ids
 template <int D, bool BOX>
 __global__ void __launch_bounds__(kB) k_gather(const double *__restrict__ geom, const uint32_t *__restrict__ vals, uint64_t n,
-                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                const uint64_t *__restrict__ keys, KeyView kv,
+                                                const uint64_t *__restrict__ set_off,
                                                 double *__restrict__ sorted, uint32_t *__restrict__ ids) {
   uint64_t j = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (j >= n) return;
@@ -597,8 +574,7 @@ __global__ void __launch_bounds__(kB) k_gather(const double *__restrict__ geom, 
   uint32_t id = vals[j];
 #pragma unroll
   for (int c = 0; c < S; ++c) sorted[(uint64_t)c * n + j] = geom[(uint64_t)id * S + c];
-  uint32_t s = set_of(set_off, nsets, id);
-  ids[j] = id - (uint32_t)set_off[s];
+  ids[j] = id - (uint32_t)set_off[kv.set(keys[j])];
 }
 
 // ---------------------------------------------------------------------------
@@ -671,6 +647,23 @@ __global__ void k_mark_roots(const SetParams *__restrict__ sets, uint32_t nsets,
   if (sets[s].node_cnt) is_root[sets[s].node_off] = 1;
 }
 
+// leaves holding more than `cap` entities (they sit at max depth or are all non-refinable)
+__global__ void __launch_bounds__(kB) k_oversized(const NodeInfo *__restrict__ info, uint64_t nnodes,
+                                                   const SetParams *__restrict__ sets, uint32_t nsets,
+                                                   uint32_t *__restrict__ flag) {
+  uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nnodes) return;
+  NodeInfo ni = info[x];
+  if (ni.ccount) return;
+  uint32_t lo = 0, hi = nsets;  // last set with node_off <= x
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (sets[mid].node_off <= x) lo = mid;
+    else hi = mid;
+  }
+  if (ni.ecount > sets[lo].cap) atomicOr(flag, 1u);
+}
+
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -710,7 +703,7 @@ static double ord_to_double(uint64_t u) {
 
 template <int D, bool BOX>
 static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *offsets, uint32_t nsets,
-                       const double *boxes, const uint32_t *depths, const uint32_t *caps) {
+                       const double *boxes, const uint32_t *depths, const uint32_t *caps, bool allow_aux = true) {
   Ctx *ctx = t->ctx;
   constexpr int S = BOX ? 2 * D : D;
   const uint64_t n = offsets[nsets];
@@ -821,7 +814,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   // 4. active boundaries
   Buf<uint8_t> active(ctx, n + 1);
   launch(ctx, "active", k_active<BOX>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys,
-         (const uint8_t *)v.p, n, (const uint64_t *)d_off.p, nsets, (const SetParams *)t->d_sets, active.p);
+         (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
 
   // 5. level scans
   Buf<uint32_t> lm(ctx, n + 1), rm(ctx, n + 1);
@@ -830,7 +823,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   // 6. node events
   Buf<uint32_t> seg_flag(ctx, n), int_flag(ctx, n), seg_idx(ctx, n), int_idx(ctx, n);
   launch(ctx, "event_flags", k_event_flags, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)v.p,
-         (const uint8_t *)active.p, (const uint32_t *)lm.p, n, (const uint64_t *)d_off.p, nsets,
+         (const uint8_t *)active.p, (const uint32_t *)lm.p, n, (const uint64_t *)t->d_keys, kv,
          (const SetParams *)t->d_sets, seg_flag.p, int_flag.p);
   Buf<uint32_t> totals(ctx, 4);
   exclusive_scan_u32(ctx, seg_flag.p, seg_idx.p, n, totals.p + 0);
@@ -863,11 +856,11 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   Buf<uint32_t> ndesc(ctx, nnodes);
   launch(ctx, "emit_internal", k_emit_internal, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)int_flag.p,
          (const uint32_t *)int_idx.p, (const uint8_t *)v.p, (const uint8_t *)active.p, (const uint32_t *)lm.p,
-         (const uint32_t *)rm.p, n, (const uint64_t *)d_off.p, nsets, (const SetParams *)t->d_sets, nk, nkeys.p,
+         (const uint32_t *)rm.p, n, (const uint64_t *)t->d_keys, kv, (const SetParams *)t->d_sets, nk, nkeys.p,
          ndesc.p);
   launch(ctx, "emit_leaves", k_emit_leaves, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
          (const uint32_t *)seg_node.p, (const uint32_t *)seg_node_idx.p, nint, (const uint8_t *)v.p,
-         (const uint32_t *)lm.p, (const uint32_t *)rm.p, (const uint64_t *)d_off.p, nsets,
+         (const uint32_t *)lm.p, (const uint32_t *)rm.p, (const uint64_t *)t->d_keys, kv,
          (const SetParams *)t->d_sets, nk, nkeys.p, ndesc.p);
   radix_sort_pairs(ctx, nkeys.p, ndesc.p, nnodes, nk.cobits + nk.gbits + L.sbits);
   launch(ctx, "set_node_offsets", k_set_node_offsets, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)nkeys.p,
@@ -889,7 +882,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   t->d_geom = (double *)ctx->alloc(n * S * 8);
   t->d_ids = (uint32_t *)ctx->alloc(n * 4);
   launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, (const uint32_t *)vals.p, n,
-         (const uint64_t *)d_off.p, nsets, t->d_geom, t->d_ids);
+         (const uint64_t *)t->d_keys, kv, (const uint64_t *)d_off.p, t->d_geom, t->d_ids);
 
   if (BOX) {
     Buf<uint32_t> parent(ctx, nnodes), pending(ctx, nnodes);
@@ -906,6 +899,45 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   OTG_HIP_CHECK(hipMemcpyAsync(t->sets.data(), t->d_sets, sizeof(SetParams) * nsets, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   t->device_bytes = n * (S * 8 + 4 + 8) + nnodes * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+
+  if (!BOX && allow_aux) {
+    // kNN acceleration: a deeper tree over the same points when max-depth leaves overflow
+    const uint32_t fmax = std::min<uint32_t>({(uint32_t)kMaxDepth, max_theoretical_depth(D), (64 - L.sbits) / D});
+    bool deeper = false;
+    for (const SetParams &sp : t->sets) deeper |= sp.max_depth < fmax;
+    if (deeper) {
+      Buf<uint32_t> flag(ctx, 1);
+      OTG_HIP_CHECK(hipMemsetAsync(flag.p, 0, 4, ctx->stream));
+      launch(ctx, "oversized", k_oversized, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
+             (const SetParams *)t->d_sets, nsets, flag.p);
+      uint32_t hflag = 0;
+      OTG_HIP_CHECK(hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      if (hflag) {
+        std::vector<double> sboxes((size_t)nsets * 2 * D);
+        std::vector<uint32_t> fdepth(nsets);
+        for (uint32_t s = 0; s < nsets; ++s) {
+          for (int d = 0; d < D; ++d) {
+            sboxes[(size_t)s * 2 * D + d] = t->sets[s].space_min[d];
+            sboxes[(size_t)s * 2 * D + D + d] = t->sets[s].space_max[d];
+          }
+          fdepth[s] = std::max(t->sets[s].max_depth, fmax);
+        }
+        Tree *aux = new Tree();
+        aux->ctx = ctx;
+        aux->kind = t->kind;
+        aux->dim = t->dim;
+        try {
+          build_impl<D, false>(aux, dgeom, OTG_MEM_DEVICE, offsets, nsets, sboxes.data(), fdepth.data(), caps, false);
+        } catch (...) {
+          delete aux;
+          throw;
+        }
+        t->knn_tree = aux;
+        t->device_bytes += aux->device_bytes;
+      }
+    }
+  }
 }
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets, const double *boxes,

@@ -27,7 +27,7 @@ struct SetParams {
   uint32_t max_depth;  // M
   uint32_t cap;        // max elements per node
   uint32_t root_splits;  // filled by the build
-  uint32_t min_lcp;      // smallest boundary level inside the set (+1 coded)
+  uint32_t min_lcp;      // 1 if a boundary inside the set has level 0, else 255
   double space_min[kMaxDim];
   double space_max[kMaxDim];
   double size[kMaxDim];
@@ -175,6 +175,9 @@ struct Tree {
   double *d_nmin = nullptr;       // node_count x dim
   double *d_nsize = nullptr;      // node_count x dim
   uint64_t device_bytes = 0;
+  // Finer tree over the same points, used only by kNN when the reference tree has
+  // leaves over capacity at max depth (kNN ids depend on distances only).
+  Tree *knn_tree = nullptr;
 
   ~Tree();
 };

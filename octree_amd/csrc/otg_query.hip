@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cfloat>
 
+#include "otg_device.hpp"
 #include "otg_internal.hpp"
 
 namespace otg {
@@ -46,124 +47,163 @@ struct StackCap {
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 
-// internal_geometry_module.h:424-458 GetBoxWallDistanceAD(.., isInsideConsideredAsZero = true)
-template <int D>
-__device__ __forceinline__ double wall_distance(const double (&q)[D], const double *mn, const double *sz) {
-  double cd[D], hs[D];
-  bool inside = true;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    hs[d] = sz[d] * 0.5;
-    cd[d] = fabs(mn[d] + hs[d] - q[d]);
-    inside &= cd[d] <= hs[d];
-  }
-  if (inside) return 0.0;
-  double s2 = 0.0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    double v = smax(0.0, cd[d] - hs[d]);
-    s2 += v * v;
-  }
-  return sqrt(s2);
-}
-
 // ot_query.h:1033-1036 GetValueWithToleranceUpper
 __device__ __forceinline__ double tol_upper(double v, double tol) { return fmax(tol, v * (1.0 + tol)); }
+
+// A bound T >= (next double above F)^2, exact real arithmetic.  sqrt is
+// correctly rounded and monotone, so d2 >= T implies sqrt(d2) > F: such
+// candidates are rejected without taking the square root, and every other
+// decision is made on sqrt(d2) exactly as the reference makes it.
+__device__ __forceinline__ double sq_reject_bound(double F) {
+  // F >= 0 and finite here; the next double above it is one unit in the last place up
+  const double nf = __longlong_as_double(__double_as_longlong(F) + 1);
+  return (nf * nf) * (1.0 + 0x1p-50);
+}
+
+// GetBoxWallDistanceAD(.., true) as a squared distance (0 inside); the node is
+// pruned exactly when wall_distance() >= F, decided on sqrt when s2 < T.
+template <int D>
+__device__ __forceinline__ double wall_distance2(const double (&q)[D], const double *mn, const double *sz, bool *inside) {
+  double s2 = 0.0;
+  bool in = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double hs = sz[d] * 0.5;
+    const double cd = fabs(mn[d] + hs - q[d]);
+    in &= cd <= hs;
+    const double v = smax(0.0, cd - hs);
+    s2 += v * v;
+  }
+  *inside = in;
+  return in ? 0.0 : s2;
+}
 
 // ---------------------------------------------------------------------------
 // kNN (points)
 // ---------------------------------------------------------------------------
-template <int D, int KCAP>
-__global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
-                                              uint64_t nq, uint32_t k, double F0, double tol,
-                                              uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                              uint32_t *__restrict__ overflow) {
-  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (qi >= nq) return;
-  const uint32_t s = qset ? qset[qi] : 0u;
-  const SetParams &sp = tv.sets[s];
-  uint32_t cnt = 0;
+// Per-query candidate list with the reference's acceptance rule
+// (AddEntityDistance ot_query.h:1058-1164): an entity enters when its distance
+// is below F; once k entries exist F = tol_upper(k-th smallest) (assigned
+// unconditionally the first time, :1124-1129), afterwards only a smaller limit
+// replaces F and entries above it leave (:1138-1157).  The list keeps the KCAP
+// smallest entries; a dropped entry that would still be <= F at the end marks
+// the query as overflowed (reported, never silently truncated).
+template <int KCAP>
+struct KnnList {
   double bd[KCAP];
   uint32_t bid[KCAP];
+  uint32_t cnt;
+  double F, T, dropped;
+  bool heapified, any_dropped;
+
+  __device__ void init(double F0) {
+    cnt = 0;
+    F = F0;
+    T = sq_reject_bound(F0);
+    dropped = 0.0;
+    heapified = false;
+    any_dropped = false;
+  }
+
+  __device__ void offer(double dist, uint32_t id, uint32_t k, double tol) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t < cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
+    if (cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
+      const double out_d = (pos == (uint32_t)KCAP) ? dist : bd[KCAP - 1];
+      dropped = any_dropped ? smin(dropped, out_d) : out_d;
+      any_dropped = true;
+    }
+#pragma unroll
+    for (int t = KCAP - 1; t >= 1; --t)
+      if ((uint32_t)t > pos && (uint32_t)t <= cnt) {
+        bd[t] = bd[t - 1];
+        bid[t] = bid[t - 1];
+      }
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t == pos) {
+        bd[t] = dist;
+        bid[t] = id;
+      }
+    if (cnt < KCAP) ++cnt;
+    if (cnt >= k) {
+      double dk = 0.0;
+#pragma unroll
+      for (int t = 0; t < KCAP; ++t)
+        if ((uint32_t)t == k - 1) dk = bd[t];
+      const double lim = tol_upper(dk, tol);
+      if (!heapified) {
+        heapified = true;
+        F = lim;
+        T = sq_reject_bound(F);
+      } else if (lim < F) {
+        F = lim;
+        T = sq_reject_bound(F);
+        uint32_t keep = 0;
+#pragma unroll
+        for (int t = 0; t < KCAP; ++t)
+          if ((uint32_t)t < cnt && !(bd[t] > F)) ++keep;
+        cnt = keep;
+      }
+    }
+  }
+
+  // GetNodeWallDistance >= F  (the node would be skipped by the reference)
+  __device__ bool node_out(double w2, bool inside) const { return inside ? (0.0 >= F) : (w2 >= T || sqrt(w2) >= F); }
+};
+
+// One lane per query (queries sorted by (set, Morton code) so neighbouring lanes
+// walk nearby nodes); depth-first with the nearest child popped first.
+template <int D, int KCAP>
+__global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
+                                              const uint32_t *__restrict__ order, uint64_t nq, uint32_t k, double F0,
+                                              double tol, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                              uint32_t *__restrict__ overflow) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
   double q[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+  KnnList<KCAP> L;
+  L.init(F0);
   if (sp.node_cnt != 0 && k != 0) {
-    double F = F0;
-    bool heapified = false;
-    double dropped = 0.0;  // smallest distance pushed out of the bounded list
-    bool any_dropped = false;
     uint32_t stack[StackCap<D>::value];
     int top = 0;
     stack[top++] = (uint32_t)sp.node_off;
     while (top > 0) {
       const uint32_t node = stack[--top];
-      const double w = wall_distance<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D]);
-      if (w >= F) continue;
+      bool in;
+      const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+      if (L.node_out(w2, in)) continue;
       const NodeInfo ni = tv.info[node];
       for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
         double d2 = 0.0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+          const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
           d2 += v * v;
         }
+        if (d2 >= L.T) continue;
         const double dist = sqrt(d2);
-        if (dist >= F) continue;
-        const uint32_t id = tv.ids[j];
-        // insert (dist, id) into the ascending list
-        uint32_t pos = 0;
-#pragma unroll
-        for (int t = 0; t < KCAP; ++t)
-          if ((uint32_t)t < cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
-        if (cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
-          double out_d = (pos == (uint32_t)KCAP) ? dist : bd[KCAP - 1];
-          dropped = any_dropped ? smin(dropped, out_d) : out_d;
-          any_dropped = true;
-        }
-#pragma unroll
-        for (int t = KCAP - 1; t >= 1; --t)
-          if ((uint32_t)t > pos && (uint32_t)t <= cnt) {
-            bd[t] = bd[t - 1];
-            bid[t] = bid[t - 1];
-          }
-#pragma unroll
-        for (int t = 0; t < KCAP; ++t)
-          if ((uint32_t)t == pos) {
-            bd[t] = dist;
-            bid[t] = id;
-          }
-        if (cnt < KCAP) ++cnt;
-        if (cnt >= k) {
-          double dk = 0.0;
-#pragma unroll
-          for (int t = 0; t < KCAP; ++t)
-            if ((uint32_t)t == k - 1) dk = bd[t];
-          const double lim = tol_upper(dk, tol);
-          if (!heapified) {
-            heapified = true;
-            F = lim;  // make_heap branch assigns unconditionally (:1124-1129)
-          } else if (lim < F) {
-            F = lim;
-            uint32_t keep = 0;
-#pragma unroll
-            for (int t = 0; t < KCAP; ++t)
-              if ((uint32_t)t < cnt && !(bd[t] > F)) ++keep;
-            cnt = keep;
-          }
-        }
+        if (dist >= L.F) continue;
+        L.offer(dist, tv.ids[j], k, tol);
       }
       if (ni.ccount) {
-        // push children with wall < F, farthest first so the nearest pops first
+        // push children still within F, farthest first so the nearest pops first
         constexpr int CMAX = 1 << D;
         double cw[CMAX];
         uint32_t cid[CMAX];
         int nc = 0;
         for (uint32_t c = 0; c < ni.ccount; ++c) {
           const uint32_t ch = ni.cbegin + c;
-          const double wc = wall_distance<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D]);
-          if (wc >= F) continue;
-          // insertion by descending wall distance
+          bool cin;
+          const double wc = wall_distance2<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D], &cin);
+          if (L.node_out(wc, cin)) continue;
           int p = nc;
           while (p > 0 && cw[p - 1] < wc) {
             cw[p] = cw[p - 1];
@@ -177,12 +217,12 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
         for (int c = 0; c < nc; ++c) stack[top++] = cid[c];
       }
     }
-    if (any_dropped && dropped <= F) atomicOr(overflow, 1u);
+    if (L.any_dropped && L.dropped <= L.F) atomicOr(overflow, 1u);
   }
-  out_cnt[qi] = cnt;
+  out_cnt[qi] = L.cnt;
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t < cnt) out_ids[qi * KCAP + t] = bid[t];
+    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = L.bid[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -215,10 +255,12 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
 
 template <int D, bool BOX, bool FILL>
 __global__ void __launch_bounds__(kQB) k_range(TreeView tv, const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
-                                                uint64_t nq, int overlap_mode, double tol, uint32_t *__restrict__ counts,
-                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
-  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (qi >= nq) return;
+                                                const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode, double tol,
+                                                uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                                uint32_t *__restrict__ out) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double rmin[D], rmax[D];
@@ -424,11 +466,12 @@ struct RayTester {
 
 template <int D, bool BOX, int RCAP>
 __global__ void __launch_bounds__(kQB) k_ray(TreeView tv, const double *__restrict__ rays, const uint32_t *__restrict__ qset,
-                                              uint64_t nq, double tol, double inc, double max_dist,
-                                              uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                              uint32_t *__restrict__ overflow) {
-  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (qi >= nq) return;
+                                              const uint32_t *__restrict__ order, uint64_t nq, double tol, double inc,
+                                              double max_dist, uint32_t *__restrict__ out_ids,
+                                              uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   uint32_t cnt = 0;
@@ -604,6 +647,46 @@ __global__ void __launch_bounds__(kQB) k_collide(TreeView tv, uint32_t set, doub
 }
 
 // ---------------------------------------------------------------------------
+// query ordering: (set, Morton code of the query's anchor point) so that the
+// lanes of a wavefront walk the same nodes and read the same leaves.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q, uint32_t stride,
+                                                     const uint32_t *__restrict__ qset, uint64_t nq,
+                                                     const SetParams *__restrict__ sets, uint32_t cbits,
+                                                     uint64_t *__restrict__ keys, uint32_t *__restrict__ idx) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t s = qset ? qset[i] : 0u;
+  const SetParams &sp = sets[s];
+  const uint32_t M = sp.max_depth;
+  const double top = (double)((1ull << M) - 1);
+  uint32_t g[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double x = (q[i * stride + d] - sp.space_min[d]) * sp.raster[d];
+    g[d] = (x > 0.0) ? (uint32_t)(x < top ? x : top) : 0u;
+  }
+  keys[i] = ((uint64_t)s << cbits) | morton_encode<D>(g);
+  idx[i] = (uint32_t)i;
+}
+
+template <int D>
+static Buf<uint32_t> query_order(const Tree *t, const double *q, uint32_t stride, const uint32_t *qs, uint64_t nq) {
+  Ctx *ctx = t->ctx;
+  Buf<uint32_t> idx(ctx, nq);
+  if (nq < 2) {
+    if (nq) OTG_HIP_CHECK(hipMemsetAsync(idx.p, 0, 4, ctx->stream));
+    return idx;
+  }
+  Buf<uint64_t> keys(ctx, nq);
+  launch(ctx, "query_keys", k_query_keys<D>, dim3(div_up(nq, 256)), dim3(256), 0, q, stride, qs, nq,
+         (const SetParams *)t->d_sets, t->layout.cbits, keys.p, idx.p);
+  radix_sort_pairs(ctx, keys.p, idx.p, nq, t->layout.cbits + t->layout.sbits);
+  return idx;
+}
+
+// ---------------------------------------------------------------------------
 // CSR helpers
 // ---------------------------------------------------------------------------
 __global__ void k_offsets_u64(const uint32_t *__restrict__ scan, uint64_t n, uint32_t total, uint64_t *__restrict__ off) {
@@ -698,10 +781,11 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
 template <int D, int KCAP>
 static Result *knn_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, uint32_t k, double F0, double tol) {
   Ctx *ctx = t->ctx;
+  Buf<uint32_t> order = query_order<D>(t, q, D, qs, nq);
   Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
-  launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs, nq, k, F0, tol, ids.p,
-         cnt.p, ovf.p);
+  launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs, (const uint32_t *)order.p,
+         nq, k, F0, tol, ids.p, cnt.p, ovf.p);
   uint32_t hovf = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
@@ -717,6 +801,7 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
   if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
   if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
   Ctx *ctx = t->ctx;
+  if (t->knn_tree) t = t->knn_tree;  // same ids: the kNN set depends on distances only
   DevInput in;
   stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
   const double F0 = (max_within == DBL_MAX) ? DBL_MAX : fmax(tol, max_within * (1.0 + tol));
@@ -746,9 +831,10 @@ template <int D, bool BOX>
 static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, int mode, double tol) {
   Ctx *ctx = t->ctx;
   TreeView tv = view_of(t);
+  Buf<uint32_t> order = query_order<D>(t, q, 2 * D, qs, nq);
   Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
-  launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs, nq, mode, tol,
-         cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
+  launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs,
+         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
   Result *r = new Result();
   r->ctx = ctx;
   r->nq = nq;
@@ -762,14 +848,17 @@ static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uin
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
-  launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs, nq, mode, tol,
-         (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
+  launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs,
+         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
   return r;
 }
 
 Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
                     double tol) {
   Ctx *ctx = t->ctx;
+  // Point trees: the finer index answers the same set (each entity is tested,
+  // or included with a node box that the range contains, ot_query.h:543-580).
+  if (t->knn_tree && t->kind == OTG_KIND_POINT) t = t->knn_tree;
   DevInput in;
   stage_inputs(ctx, in, ranges, sets, nq * 2 * t->dim, nq, mem);
   const bool box = t->kind == OTG_KIND_BOX;
@@ -795,8 +884,9 @@ static Result *ray_impl(const Tree *t, uint64_t nq, const double *q, const uint3
   constexpr int RCAP = 16;
   Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
-  launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs, nq, tol, inc,
-         max_dist, ids.p, cnt.p, ovf.p);
+  Buf<uint32_t> order = query_order<D>(t, q, 2 * D, qs, nq);
+  launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs,
+         (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p);
   uint32_t hovf = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
