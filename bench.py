@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-only", action="store_true", help="one batched step, no plain/CPU legs")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "octree_point_1m", "octree_box_4m", "point6d_8m"],
+                    help="batched (default) or one of the single-tree configurations")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -96,6 +99,10 @@ def main():
     import torch
 
     import octree_amd
+
+    if args.workload != "batched":
+        print(json.dumps(run_single(args, octree_amd, torch)), flush=True)
+        return
 
     t_gen = time.time()
     W = make_workload(args.nsets, args.nq)
@@ -294,6 +301,133 @@ def main():
                                          f"build single-threaded {done_p / t_b / 1e6:.2f} Mpts/s",
                                "build_mpts_s": done_p / t_b / 1e6}
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# single-tree configurations (plain path), each with its own CPU baseline
+# ---------------------------------------------------------------------------
+def _timed(ctx, fn, reps):
+    fn()
+    ctx.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+        if hasattr(out, "free"):
+            out.free()
+    return float(np.median(ts)), ts
+
+
+def run_single(args, octree_amd, torch):
+    from oracle import oracle as O
+
+    dev = torch.device("cuda:0")
+    ctx = octree_amd.Context(0)
+    threads = min(16, os.cpu_count() or 1)
+    reps = max(1, args.steps)
+    out = {"metric": METRIC, "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)"}
+    if args.workload == "octree_point_1m":
+        rng = np.random.default_rng(11)
+        pts = rng.uniform(0.0, 100.0, (1_000_000, 3))
+        qs = rng.uniform(0.0, 100.0, (10_000, 3))
+        box = [0, 0, 0, 100, 100, 100]
+        d_pts, d_q = torch.from_numpy(pts).to(dev), torch.from_numpy(qs).to(dev)
+        torch.cuda.synchronize()
+        tree = octree_amd.StaticOctreePoint(ctx=ctx)
+        tb, _ = _timed(ctx, lambda: tree.Create(d_pts, 10, box, 16), reps)
+        tk, _ = _timed(ctx, lambda: tree.knn_batch(d_q, 8), reps)
+        t0 = time.perf_counter()
+        o = O.OracleTree(3, "point", pts, box, 10, 16)
+        cb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.knn(qs, 8, threads=threads)
+        ck = time.perf_counter() - t0
+        out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
+                    "config": {"workload": "OctreePoint 1,000,000 pts U[0,100)^3, depth 10, cap 16; 10,000 kNN k=8"},
+                    "build_ms": tb * 1e3, "build_mpts_s": 1.0 / tb, "knn_ms": tk * 1e3,
+                    "node_count": tree.GetNodeCount(),
+                    "cpu_baseline": {"value": len(qs) / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
+                                     "sample": "whole workload", "build_mpts_s": 1.0 / cb, "build_cores": 1}})
+    elif args.workload == "octree_box_4m":
+        rng = np.random.default_rng(12)
+        n = 4_000_000
+        c = rng.uniform(0.0, 1000.0, (n, 3))
+        h = np.exp(rng.normal(np.log(0.5), 0.6, (n, 3)))
+        boxes = np.concatenate([c - h, c + h], axis=1)
+        o_ = rng.uniform(0.0, 1000.0, (100_000, 3))
+        v = rng.normal(0.0, 1.0, (100_000, 3))
+        rays = np.concatenate([o_, v / np.sqrt((v * v).sum(axis=1))[:, None]], axis=1)
+        d_b, d_r = torch.from_numpy(boxes).to(dev), torch.from_numpy(rays).to(dev)
+        torch.cuda.synchronize()
+        tree = octree_amd.StaticOctreeBox(ctx=ctx)
+        tb, _ = _timed(ctx, lambda: tree.Create(d_b, 8, None, 32), reps)
+        tr, _ = _timed(ctx, lambda: tree.ray_first_batch(d_r), reps)
+        tcol, _ = _timed(ctx, lambda: tree.collide(), reps)
+        npairs = len(tree.collide())
+        # CPU: oracle build of a 400k-box sample, its collision, and 100k rays on it
+        m = 400_000
+        t0 = time.perf_counter()
+        o = O.OracleTree(3, "box", boxes[:m], None, 8, 32)
+        cb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.ray_first(rays, threads=threads)
+        cr = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.collide()
+        cc = time.perf_counter() - t0
+        out.update({"value": len(rays) / tr, "unit": "RayIntersectedFirst rays/s", "ms_per_step": (tb + tr + tcol) * 1e3,
+                    "config": {"workload": "OctreeBox 4,000,000 AABBs centres U[0,1000)^3, half-extent lognormal(ln0.5,0.6), "
+                                           "depth 8, cap 32; self CollisionDetection + 100,000 rays"},
+                    "build_ms": tb * 1e3, "build_mboxes_s": n / tb / 1e6, "ray_ms": tr * 1e3,
+                    "collision_ms": tcol * 1e3, "collision_pairs": npairs, "collision_boxes_s": n / tcol,
+                    "cpu_baseline": {"value": len(rays) / cr, "unit": "rays/s", "cores": threads, "kind": "port",
+                                     "sample": f"first {m} boxes; 100,000 rays", "build_mboxes_s": m / cb / 1e6,
+                                     "collision_boxes_s_single_thread": m / cc}})
+    else:  # point6d_8m
+        rng = np.random.default_rng(13)
+        k = 32
+        centers = rng.uniform(0.0, 1.0, (k, 6))
+        sig = rng.uniform(0.01, 0.05, k)
+
+        def draw(m):
+            lab = rng.integers(0, k, m)
+            p = centers[lab] + rng.normal(0.0, 1.0, (m, 6)) * sig[lab, None]
+            bad = ((p < 0.0) | (p > 1.0)).any(axis=1)
+            while bad.any():
+                kk = int(bad.sum())
+                lab2 = rng.integers(0, k, kk)
+                p[bad] = centers[lab2] + rng.normal(0.0, 1.0, (kk, 6)) * sig[lab2, None]
+                bad = ((p < 0.0) | (p > 1.0)).any(axis=1)
+            return p
+
+        pts = draw(8_000_000)
+        qs = draw(1_000_000)
+        box = [0.0] * 6 + [1.0] * 6
+        d_p, d_q = torch.from_numpy(pts).to(dev), torch.from_numpy(qs).to(dev)
+        torch.cuda.synchronize()
+        T = octree_amd.StaticOrthoTreePointND(6)
+        tree = T(ctx=ctx)
+        tb, _ = _timed(ctx, lambda: tree.Create(d_p, 8, box, 32), reps)
+        tk, _ = _timed(ctx, lambda: tree.knn_batch(d_q, 16), reps)
+        # CPU: oracle build of a 1M-point sample and 20k queries on it
+        m = 1_000_000
+        t0 = time.perf_counter()
+        o = O.OracleTree(6, "point", pts[:m], box, 8, 32)
+        cb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.knn(qs[:20_000], 16, threads=threads)
+        ck = time.perf_counter() - t0
+        out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
+                    "config": {"workload": "TreePointND<6> 8,000,000 pts 32-cluster mixture in [0,1]^6, depth 8, cap 32; "
+                                           "1,000,000 kNN k=16"},
+                    "build_ms": tb * 1e3, "build_mpts_s": 8.0 / tb, "knn_ms": tk * 1e3,
+                    "cpu_baseline": {"value": 20_000 / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
+                                     "sample": f"tree of the first {m} points; 20,000 queries", "build_mpts_s": 1.0 / cb}})
+    kt = None
+    return out
 
 
 if __name__ == "__main__":

@@ -161,6 +161,39 @@ OTG_API otg_status otg_result_copy(const otg_result *res, uint64_t *offsets, uin
 OTG_API otg_status otg_result_device(const otg_result *res, const uint64_t **offsets, const uint32_t **ids);
 OTG_API void otg_result_free(otg_result *res);
 
+/* ---- serving: resident trees and continuous batching ----
+ * No counterpart in the reference (a header-only library): the runtime for callers that keep
+ * many trees resident in HBM and send streams of query batches.  Trees are built once and
+ * stay resident under `hbm_budget_bytes`; adding one that does not fit evicts the least
+ * recently used idle trees.  `nworkers` worker threads, each with its own stream, take all
+ * pending requests, merge requests on the same tree with equal parameters into one launch and
+ * split the results back.  A request's result equals the direct call's (otg_knn / otg_range /
+ * otg_ray_first) on the same queries, whatever it was batched with. */
+typedef struct otg_server otg_server;
+OTG_API otg_status otg_server_create(int device, uint64_t hbm_budget_bytes, uint32_t nworkers, otg_server **out);
+/* Waits for running batches; frees every resident tree and unclaimed result. */
+OTG_API void otg_server_destroy(otg_server *server);
+/* Builds (as otg_tree_build) and registers the tree under tree_id, replacing an idle one. */
+OTG_API otg_status otg_server_add_tree(otg_server *server, uint32_t tree_id, int kind, int dim, uint64_t n,
+                                       const double *geom, int mem, const double *box_or_null, uint32_t max_depth,
+                                       uint32_t max_elements);
+OTG_API otg_status otg_server_remove_tree(otg_server *server, uint32_t tree_id);
+OTG_API otg_status otg_server_has_tree(otg_server *server, uint32_t tree_id, int *present);
+/* query 0 = kNN (k, p0 = max_within, p1 = tolerance); 1 = range (mode, p0 = tolerance);
+ * 2 = ray first (p0 = tolerance, p1 = tolerance increment, p2 = max distance).  The queries
+ * (host memory) are copied; returns at once with a request id. */
+OTG_API otg_status otg_server_submit(otg_server *server, uint32_t tree_id, int query, uint64_t nq,
+                                     const double *queries, uint32_t k, double p0, double p1, double p2, int mode,
+                                     uint64_t *request_id);
+/* Blocks until the request is served; *out is a host-resident result owned by the caller
+ * (otg_result_copy / otg_result_free; otg_result_device is not available for it).
+ * latency_ms = submit to done, queue_ms = submit to batch start (either may be NULL). */
+OTG_API otg_status otg_server_wait(otg_server *server, uint64_t request_id, otg_result **out, double *latency_ms,
+                                   double *queue_ms);
+/* stats[8] = batches, launches, requests served, queries served, evictions, resident bytes,
+ * resident trees, pending requests. */
+OTG_API otg_status otg_server_stats(otg_server *server, uint64_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

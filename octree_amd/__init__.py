@@ -107,10 +107,11 @@ def default_context() -> Context:
 class Result:
     """CSR result (otg_result): offsets (nq+1,) uint64, ids uint32."""
 
-    def __init__(self, handle, ctx: Context):
+    def __init__(self, handle, ctx: Optional[Context]):
         self.handle = handle
-        self.ctx = ctx
-        ctx._children.add(self)
+        self.ctx = ctx  # None for host-resident results (served requests)
+        if ctx is not None:
+            ctx._children.add(self)
 
     def csr(self):
         L = _abi.lib()
@@ -365,6 +366,88 @@ class StaticTree:
         return [tuple(p) for p in self.collide(tolerance).tolist()]
 
 
+class Server:
+    """Resident trees under an HBM budget with continuous batching (otg_server_*).
+
+    Requests are submitted without blocking and collected with wait(); a served
+    request's ids equal the direct StaticTree query's on the same queries.
+    """
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, workers: int = 2):
+        h = C.c_void_p()
+        st = _abi.lib().otg_server_create(device, int(hbm_budget), int(workers), C.byref(h))
+        if st != _abi.OTG_OK:
+            raise OtgError(st, "otg_server_create")
+        self.handle = h
+        self._dims = {}
+
+    def add_tree(self, tree_id: int, kind: str, dim: int, entities, max_depth: Optional[int] = None,
+                 box_space=None, max_elements: int = 16) -> None:
+        cols = dim if kind == "point" else 2 * dim
+        geom = _f64(entities, cols)
+        box = None if box_space is None else _f64(box_space, 2 * dim).reshape(-1)
+        kindc = _abi.OTG_KIND_POINT if kind == "point" else _abi.OTG_KIND_BOX
+        st = _abi.lib().otg_server_add_tree(self.handle, int(tree_id), kindc, dim, geom.shape[0], _ptr(geom),
+                                            _abi.OTG_MEM_HOST, _ptr(box), int(max_depth or 0), int(max_elements))
+        _check(st, "otg_server_add_tree")
+        self._dims[int(tree_id)] = (kind, dim)
+
+    def remove_tree(self, tree_id: int) -> None:
+        _check(_abi.lib().otg_server_remove_tree(self.handle, int(tree_id)), "otg_server_remove_tree")
+
+    def has_tree(self, tree_id: int) -> bool:
+        v = C.c_int()
+        _check(_abi.lib().otg_server_has_tree(self.handle, int(tree_id), C.byref(v)), "otg_server_has_tree")
+        return bool(v.value)
+
+    def _submit(self, tree_id, query, q, k=0, p0=0.0, p1=0.0, p2=0.0, mode=0) -> int:
+        rid = C.c_uint64()
+        st = _abi.lib().otg_server_submit(self.handle, int(tree_id), query, q.shape[0], _ptr(q), int(k), float(p0),
+                                          float(p1), float(p2), int(mode), C.byref(rid))
+        _check(st, "otg_server_submit")
+        return rid.value
+
+    def submit_knn(self, tree_id: int, points, k: int, max_within: float = DBL_MAX,
+                   tolerance: float = BASE_TOLERANCE) -> int:
+        return self._submit(tree_id, 0, _f64(points, self._dims[int(tree_id)][1]), k=k, p0=max_within, p1=tolerance)
+
+    def submit_range(self, tree_id: int, ranges, mode: int = _abi.OTG_RANGE_INSIDE,
+                     tolerance: float = BASE_TOLERANCE) -> int:
+        return self._submit(tree_id, 1, _f64(ranges, 2 * self._dims[int(tree_id)][1]), p0=tolerance, mode=mode)
+
+    def submit_ray(self, tree_id: int, rays, tolerance: float = BASE_TOLERANCE, tolerance_increment: float = 0.0,
+                   max_distance: float = DBL_MAX) -> int:
+        return self._submit(tree_id, 2, _f64(rays, 2 * self._dims[int(tree_id)][1]), p0=tolerance,
+                            p1=tolerance_increment, p2=max_distance)
+
+    def wait(self, request_id: int):
+        """-> (Result, latency_ms, queue_ms)."""
+        h, lat, qms = C.c_void_p(), C.c_double(), C.c_double()
+        st = _abi.lib().otg_server_wait(self.handle, int(request_id), C.byref(h), C.byref(lat), C.byref(qms))
+        if st != _abi.OTG_OK and h.value:
+            _abi.lib().otg_result_free(h)
+        _check(st, "otg_server_wait")
+        return Result(h, None), lat.value, qms.value
+
+    def stats(self) -> dict:
+        v = (C.c_uint64 * 8)()
+        _check(_abi.lib().otg_server_stats(self.handle, v), "otg_server_stats")
+        keys = ["batches", "launches", "requests", "queries", "evictions", "resident_bytes", "resident_trees",
+                "pending"]
+        return dict(zip(keys, [int(x) for x in v]))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _abi.lib().otg_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _make(kind: str, dim: int, name: str):
     return type(name, (StaticTree,), {"KIND": kind, "DIM": dim})
 
@@ -391,7 +474,7 @@ def StaticOrthoTreeBoxND(dim: int, loose: bool = True):
 
 
 __all__ = [
-    "Context", "Result", "StaticTree", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
+    "Context", "Result", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
     "StaticQuadtreePoint", "StaticQuadtreeBox", "StaticOctreePoint", "StaticOctreeBox", "StaticHexatreePoint",
     "StaticHexatreeBox", "StaticOrthoTreePointND", "StaticOrthoTreeBoxND", "default_context",
 ]

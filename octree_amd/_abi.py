@@ -36,6 +36,8 @@ EXPORTED = [
     "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_free", "otg_tree_get_info",
     "otg_tree_get_set_info", "otg_tree_export", "otg_knn", "otg_range", "otg_ray_first", "otg_collide",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
+    "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
+    "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_stats",
 ]
 
 
@@ -102,6 +104,16 @@ def lib() -> C.CDLL:
         "otg_result_copy": (C.c_int, [P, P, P]),
         "otg_result_device": (C.c_int, [P, PP, PP]),
         "otg_result_free": (None, [P]),
+        "otg_server_create": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, PP]),
+        "otg_server_destroy": (None, [P]),
+        "otg_server_add_tree": (C.c_int, [P, C.c_uint32, C.c_int, C.c_int, C.c_uint64, P, C.c_int, P, C.c_uint32,
+                                          C.c_uint32]),
+        "otg_server_remove_tree": (C.c_int, [P, C.c_uint32]),
+        "otg_server_has_tree": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_int)]),
+        "otg_server_submit": (C.c_int, [P, C.c_uint32, C.c_int, C.c_uint64, P, C.c_uint32, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, U64P]),
+        "otg_server_wait": (C.c_int, [P, C.c_uint64, PP, F64P, F64P]),
+        "otg_server_stats": (C.c_int, [P, U64P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -128,9 +128,6 @@ Tree::~Tree() {
 using otg::Ctx;
 using otg::Error;
 
-struct otg_ctx : public Ctx {};
-struct otg_tree : public otg::Tree {};
-struct otg_result : public otg::Result {};
 
 namespace {
 
@@ -425,6 +422,11 @@ OTG_API otg_status otg_result_get_info(const otg_result *r, uint64_t *nq, uint64
 
 OTG_API otg_status otg_result_copy(const otg_result *r, uint64_t *offsets, uint32_t *ids) {
   if (!r) return OTG_ERR_INVALID_ARG;
+  if (r->host_only) {
+    if (offsets) std::memcpy(offsets, r->h_offsets.data(), (r->nq + 1) * 8);
+    if (ids && r->nids) std::memcpy(ids, r->h_ids.data(), r->nids * 4);
+    return OTG_OK;
+  }
   Ctx *ctx = r->ctx;
   return guarded(ctx, [&] {
     OTG_HIP_CHECK(hipSetDevice(ctx->device));
@@ -435,7 +437,7 @@ OTG_API otg_status otg_result_copy(const otg_result *r, uint64_t *offsets, uint3
 }
 
 OTG_API otg_status otg_result_device(const otg_result *r, const uint64_t **offsets, const uint32_t **ids) {
-  if (!r) return OTG_ERR_INVALID_ARG;
+  if (!r || r->host_only) return OTG_ERR_INVALID_ARG;
   if (offsets) *offsets = r->d_offsets;
   if (ids) *ids = r->d_ids;
   return OTG_OK;

@@ -191,8 +191,16 @@ struct Result {
   uint64_t nids = 0;
   uint64_t *d_offsets = nullptr;
   uint32_t *d_ids = nullptr;
+  // host-resident results (served requests): used when host_only is set
+  bool host_only = false;
+  std::vector<uint64_t> h_offsets;
+  std::vector<uint32_t> h_ids;
   ~Result();
 };
+
+// Queries run on the calling thread's context when one is set (serving workers).
+Ctx *run_ctx(const Tree *t);
+void set_thread_run_ctx(Ctx *c);
 
 Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
                   double max_within, double tol);
@@ -207,3 +215,8 @@ uint32_t max_theoretical_depth(int dim);
 uint32_t estimate_max_depth(uint64_t n, uint64_t cap, int dim);
 
 }  // namespace otg
+
+// Opaque ABI handles are the internal objects (no extra members).
+struct otg_ctx : public otg::Ctx {};
+struct otg_tree : public otg::Tree {};
+struct otg_result : public otg::Result {};

@@ -33,6 +33,12 @@ struct TreeView {
   const SetParams *sets;
 };
 
+// Context whose stream and allocator run a query: the calling thread's
+// override (serving workers each own one) or the tree's context.
+thread_local Ctx *tl_run_ctx = nullptr;
+Ctx *run_ctx(const Tree *t) { return tl_run_ctx ? tl_run_ctx : t->ctx; }
+void set_thread_run_ctx(Ctx *c) { tl_run_ctx = c; }
+
 static TreeView view_of(const Tree *t) {
   return TreeView{t->d_info, t->d_rend, t->d_nmin, t->d_nsize, t->d_geom, t->n, t->d_ids, t->d_sets};
 }
@@ -81,7 +87,7 @@ __device__ __forceinline__ double wall_distance2(const double (&q)[D], const dou
 // ---------------------------------------------------------------------------
 // kNN (points)
 // ---------------------------------------------------------------------------
-// Per-query candidate list with the reference's acceptance rule
+// Per-query candidate list (bd/bid arrays + KnnState) with the reference's acceptance rule
 // (AddEntityDistance ot_query.h:1058-1164): an entity enters when its distance
 // is below F; once k entries exist F = tol_upper(k-th smallest) (assigned
 // unconditionally the first time, :1124-1129), afterwards only a smaller limit
@@ -89,73 +95,79 @@ __device__ __forceinline__ double wall_distance2(const double (&q)[D], const dou
 // smallest entries; a dropped entry that would still be <= F at the end marks
 // the query as overflowed (reported, never silently truncated).
 template <int KCAP>
-struct KnnList {
-  double bd[KCAP];
-  uint32_t bid[KCAP];
+struct KnnState {
   uint32_t cnt;
   double F, T, dropped;
   bool heapified, any_dropped;
-
-  __device__ void init(double F0) {
-    cnt = 0;
-    F = F0;
-    T = sq_reject_bound(F0);
-    dropped = 0.0;
-    heapified = false;
-    any_dropped = false;
-  }
-
-  __device__ void offer(double dist, uint32_t id, uint32_t k, double tol) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (int t = 0; t < KCAP; ++t)
-      if ((uint32_t)t < cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
-    if (cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
-      const double out_d = (pos == (uint32_t)KCAP) ? dist : bd[KCAP - 1];
-      dropped = any_dropped ? smin(dropped, out_d) : out_d;
-      any_dropped = true;
-    }
-#pragma unroll
-    for (int t = KCAP - 1; t >= 1; --t)
-      if ((uint32_t)t > pos && (uint32_t)t <= cnt) {
-        bd[t] = bd[t - 1];
-        bid[t] = bid[t - 1];
-      }
-#pragma unroll
-    for (int t = 0; t < KCAP; ++t)
-      if ((uint32_t)t == pos) {
-        bd[t] = dist;
-        bid[t] = id;
-      }
-    if (cnt < KCAP) ++cnt;
-    if (cnt >= k) {
-      double dk = 0.0;
-#pragma unroll
-      for (int t = 0; t < KCAP; ++t)
-        if ((uint32_t)t == k - 1) dk = bd[t];
-      const double lim = tol_upper(dk, tol);
-      if (!heapified) {
-        heapified = true;
-        F = lim;
-        T = sq_reject_bound(F);
-      } else if (lim < F) {
-        F = lim;
-        T = sq_reject_bound(F);
-        uint32_t keep = 0;
-#pragma unroll
-        for (int t = 0; t < KCAP; ++t)
-          if ((uint32_t)t < cnt && !(bd[t] > F)) ++keep;
-        cnt = keep;
-      }
-    }
-  }
-
-  // GetNodeWallDistance >= F  (the node would be skipped by the reference)
-  __device__ bool node_out(double w2, bool inside) const { return inside ? (0.0 >= F) : (w2 >= T || sqrt(w2) >= F); }
 };
 
+template <int KCAP>
+__device__ __forceinline__ void knn_init(KnnState<KCAP> &S, double F0) {
+  S.cnt = 0;
+  S.F = F0;
+  S.T = sq_reject_bound(F0);
+  S.dropped = 0.0;
+  S.heapified = false;
+  S.any_dropped = false;
+}
+
+// GetNodeWallDistance >= F: the node would be skipped by the reference
+template <int KCAP>
+__device__ __forceinline__ bool knn_node_out(const KnnState<KCAP> &S, double w2, bool inside) {
+  return inside ? (0.0 >= S.F) : (w2 >= S.T || sqrt(w2) >= S.F);
+}
+
+template <int KCAP>
+__device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &S, double dist,
+                                          uint32_t id, uint32_t k, double tol) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t < S.cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
+  if (S.cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
+    const double out_d = (pos == (uint32_t)KCAP) ? dist : bd[KCAP - 1];
+    S.dropped = S.any_dropped ? smin(S.dropped, out_d) : out_d;
+    S.any_dropped = true;
+  }
+#pragma unroll
+  for (int t = KCAP - 1; t >= 1; --t)
+    if ((uint32_t)t > pos && (uint32_t)t <= S.cnt) {
+      bd[t] = bd[t - 1];
+      bid[t] = bid[t - 1];
+    }
+#pragma unroll
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t == pos) {
+      bd[t] = dist;
+      bid[t] = id;
+    }
+  if (S.cnt < KCAP) ++S.cnt;
+  if (S.cnt >= k) {
+    double dk = 0.0;
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t == k - 1) dk = bd[t];
+    const double lim = tol_upper(dk, tol);
+    if (!S.heapified) {
+      S.heapified = true;
+      S.F = lim;  // the make_heap branch assigns unconditionally (:1124-1129)
+      S.T = sq_reject_bound(S.F);
+    } else if (lim < S.F) {
+      S.F = lim;
+      S.T = sq_reject_bound(S.F);
+      uint32_t keep = 0;
+#pragma unroll
+      for (int t = 0; t < KCAP; ++t)
+        if ((uint32_t)t < S.cnt && !(bd[t] > S.F)) ++keep;
+      S.cnt = keep;
+    }
+  }
+}
+
 // One lane per query (queries sorted by (set, Morton code) so neighbouring lanes
-// walk nearby nodes); depth-first with the nearest child popped first.
+// walk nearby nodes); depth-first with the nearest child popped first.  Stack
+// entries carry the child's squared wall distance computed when it was pushed,
+// so a pop re-checks it against the current F without reloading the node box.
 template <int D, int KCAP>
 __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                                               const uint32_t *__restrict__ order, uint64_t nq, uint32_t k, double F0,
@@ -169,17 +181,26 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
   double q[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
-  KnnList<KCAP> L;
-  L.init(F0);
+  double bd[KCAP];
+  uint32_t bid[KCAP];
+  KnnState<KCAP> L;
+  knn_init(L, F0);
   if (sp.node_cnt != 0 && k != 0) {
-    uint32_t stack[StackCap<D>::value];
+    uint32_t stack_n[StackCap<D>::value];
+    double stack_w[StackCap<D>::value];
     int top = 0;
-    stack[top++] = (uint32_t)sp.node_off;
-    while (top > 0) {
-      const uint32_t node = stack[--top];
+    {
+      const uint32_t root = (uint32_t)sp.node_off;
       bool in;
-      const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
-      if (L.node_out(w2, in)) continue;
+      stack_w[0] = wall_distance2<D>(q, &tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &in);
+      stack_n[0] = root;
+      top = 1;
+    }
+    while (top > 0) {
+      --top;
+      const uint32_t node = stack_n[top];
+      // wall2 is 0 inside the node box, and (0 >= F) is the reference's test there too
+      if (knn_node_out(L, stack_w[top], false)) continue;
       const NodeInfo ni = tv.info[node];
       for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
         double d2 = 0.0;
@@ -191,30 +212,26 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
         if (d2 >= L.T) continue;
         const double dist = sqrt(d2);
         if (dist >= L.F) continue;
-        L.offer(dist, tv.ids[j], k, tol);
+        knn_offer(bd, bid, L, dist, tv.ids[j], k, tol);
       }
       if (ni.ccount) {
         // push children still within F, farthest first so the nearest pops first
-        constexpr int CMAX = 1 << D;
-        double cw[CMAX];
-        uint32_t cid[CMAX];
-        int nc = 0;
+        const int base = top;
         for (uint32_t c = 0; c < ni.ccount; ++c) {
           const uint32_t ch = ni.cbegin + c;
           bool cin;
           const double wc = wall_distance2<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D], &cin);
-          if (L.node_out(wc, cin)) continue;
-          int p = nc;
-          while (p > 0 && cw[p - 1] < wc) {
-            cw[p] = cw[p - 1];
-            cid[p] = cid[p - 1];
+          if (knn_node_out(L, wc, false)) continue;
+          int p = top;
+          while (p > base && stack_w[p - 1] < wc) {
+            stack_w[p] = stack_w[p - 1];
+            stack_n[p] = stack_n[p - 1];
             --p;
           }
-          cw[p] = wc;
-          cid[p] = ch;
-          ++nc;
+          stack_w[p] = wc;
+          stack_n[p] = ch;
+          ++top;
         }
-        for (int c = 0; c < nc; ++c) stack[top++] = cid[c];
       }
     }
     if (L.any_dropped && L.dropped <= L.F) atomicOr(overflow, 1u);
@@ -222,7 +239,7 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
   out_cnt[qi] = L.cnt;
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = L.bid[t];
+    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -673,7 +690,7 @@ __global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q
 
 template <int D>
 static Buf<uint32_t> query_order(const Tree *t, const double *q, uint32_t stride, const uint32_t *qs, uint64_t nq) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   Buf<uint32_t> idx(ctx, nq);
   if (nq < 2) {
     if (nq) OTG_HIP_CHECK(hipMemsetAsync(idx.p, 0, 4, ctx->stream));
@@ -780,7 +797,7 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
 
 template <int D, int KCAP>
 static Result *knn_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, uint32_t k, double F0, double tol) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   Buf<uint32_t> order = query_order<D>(t, q, D, qs, nq);
   Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
@@ -800,7 +817,7 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
                   double max_within, double tol) {
   if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
   if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   if (t->knn_tree) t = t->knn_tree;  // same ids: the kNN set depends on distances only
   DevInput in;
   stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
@@ -813,8 +830,8 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
 #define OTG_KNN(DD)                                                                     \
   case DD:                                                                              \
     if (k <= 8) return knn_impl<DD, 16>(t, nq, in.q, in.sets, k, F0, tol);               \
-    if (k <= 16) return knn_impl<DD, 24>(t, nq, in.q, in.sets, k, F0, tol);              \
-    if (k <= 32) return knn_impl<DD, 40>(t, nq, in.q, in.sets, k, F0, tol);              \
+    if (k <= 16) return knn_impl<DD, 20>(t, nq, in.q, in.sets, k, F0, tol);              \
+    if (k <= 32) return knn_impl<DD, 36>(t, nq, in.q, in.sets, k, F0, tol);              \
     return knn_impl<DD, 72>(t, nq, in.q, in.sets, k, F0, tol);
   switch (t->dim) {
     OTG_KNN(2)
@@ -829,7 +846,7 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
 
 template <int D, bool BOX>
 static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, int mode, double tol) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   TreeView tv = view_of(t);
   Buf<uint32_t> order = query_order<D>(t, q, 2 * D, qs, nq);
   Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
@@ -855,7 +872,7 @@ static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uin
 
 Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
                     double tol) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   // Point trees: the finer index answers the same set (each entity is tested,
   // or included with a node box that the range contains, ot_query.h:543-580).
   if (t->knn_tree && t->kind == OTG_KIND_POINT) t = t->knn_tree;
@@ -880,7 +897,7 @@ Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint
 template <int D, bool BOX>
 static Result *ray_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, double tol, double inc,
                         double max_dist) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   constexpr int RCAP = 16;
   Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
@@ -899,7 +916,7 @@ static Result *ray_impl(const Tree *t, uint64_t nq, const double *q, const uint3
 
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol, double inc,
                   double max_dist) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   DevInput in;
   stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
   const bool box = t->kind == OTG_KIND_BOX;
@@ -920,7 +937,7 @@ Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t
 
 template <int D>
 static Result *collide_impl(const Tree *t, uint32_t set, double tol) {
-  Ctx *ctx = t->ctx;
+  Ctx *ctx = run_ctx(t);
   const SetParams &sp = t->sets[set];
   const uint64_t ne = sp.ent_cnt;
   TreeView tv = view_of(t);
