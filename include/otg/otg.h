@@ -173,11 +173,14 @@ typedef struct otg_server otg_server;
 OTG_API otg_status otg_server_create(int device, uint64_t hbm_budget_bytes, uint32_t nworkers, otg_server **out);
 /* Waits for running batches; frees every resident tree and unclaimed result. */
 OTG_API void otg_server_destroy(otg_server *server);
-/* Builds (as otg_tree_build) and registers the tree under tree_id, replacing an idle one. */
+/* Registers the tree under tree_id (keeping a host copy of the entities, so that an evicted
+ * tree is rebuilt when a request needs it) and builds it as otg_tree_build does; replaces a
+ * tree registered under the same id once nothing uses it. */
 OTG_API otg_status otg_server_add_tree(otg_server *server, uint32_t tree_id, int kind, int dim, uint64_t n,
                                        const double *geom, int mem, const double *box_or_null, uint32_t max_depth,
                                        uint32_t max_elements);
 OTG_API otg_status otg_server_remove_tree(otg_server *server, uint32_t tree_id);
+/* *present: 0 not registered, 1 resident, 2 registered but evicted. */
 OTG_API otg_status otg_server_has_tree(otg_server *server, uint32_t tree_id, int *present);
 /* query 0 = kNN (k, p0 = max_within, p1 = tolerance); 1 = range (mode, p0 = tolerance);
  * 2 = ray first (p0 = tolerance, p1 = tolerance increment, p2 = max distance).  The queries
@@ -190,8 +193,8 @@ OTG_API otg_status otg_server_submit(otg_server *server, uint32_t tree_id, int q
  * latency_ms = submit to done, queue_ms = submit to batch start (either may be NULL). */
 OTG_API otg_status otg_server_wait(otg_server *server, uint64_t request_id, otg_result **out, double *latency_ms,
                                    double *queue_ms);
-/* stats[8] = batches, launches, requests served, queries served, evictions, resident bytes,
- * resident trees, pending requests. */
+/* stats[10] = batches, launches, requests served, queries served, evictions, resident bytes,
+ * resident trees, pending requests, rebuilds of evicted trees, registered trees. */
 OTG_API otg_status otg_server_stats(otg_server *server, uint64_t *stats);
 
 #ifdef __cplusplus

@@ -395,10 +395,15 @@ class Server:
     def remove_tree(self, tree_id: int) -> None:
         _check(_abi.lib().otg_server_remove_tree(self.handle, int(tree_id)), "otg_server_remove_tree")
 
-    def has_tree(self, tree_id: int) -> bool:
+    def tree_state(self, tree_id: int) -> int:
+        """0 not registered, 1 resident, 2 registered but evicted (rebuilt on demand)."""
         v = C.c_int()
         _check(_abi.lib().otg_server_has_tree(self.handle, int(tree_id), C.byref(v)), "otg_server_has_tree")
-        return bool(v.value)
+        return v.value
+
+    def has_tree(self, tree_id: int) -> bool:
+        """True when the tree is resident in HBM."""
+        return self.tree_state(tree_id) == 1
 
     def _submit(self, tree_id, query, q, k=0, p0=0.0, p1=0.0, p2=0.0, mode=0) -> int:
         rid = C.c_uint64()
@@ -430,10 +435,10 @@ class Server:
         return Result(h, None), lat.value, qms.value
 
     def stats(self) -> dict:
-        v = (C.c_uint64 * 8)()
+        v = (C.c_uint64 * 10)()
         _check(_abi.lib().otg_server_stats(self.handle, v), "otg_server_stats")
         keys = ["batches", "launches", "requests", "queries", "evictions", "resident_bytes", "resident_trees",
-                "pending"]
+                "pending", "rebuilds", "registered_trees"]
         return dict(zip(keys, [int(x) for x in v]))
 
     def close(self) -> None:

@@ -111,6 +111,7 @@ void Ctx::collect_times() {
 Tree::~Tree() {
   delete knn_tree;
   if (!ctx) return;
+  ctx->free(d_view);
   ctx->free(d_sets);
   ctx->free(d_ids);
   ctx->free(d_keys);

@@ -958,6 +958,7 @@ void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets,
     default:
       throw Error{OTG_ERR_INVALID_ARG, "dimension must be 2..6"};
   }
+  publish_views(t);
 }
 
 }  // namespace otg

@@ -153,6 +153,19 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint
 // right[i] = same going right, as bit masks.
 void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask);
 
+// Device-side view of one tree, passed to traversal kernels (an array of them
+// lets one launch serve queries on many trees).
+struct TreeView {
+  const NodeInfo *info;
+  const uint32_t *rend;
+  const double *nmin;
+  const double *nsize;
+  const double *geom;  // SoA, sorted order
+  uint64_t n;
+  const uint32_t *ids;
+  const SetParams *sets;
+};
+
 // ---- tree ----
 struct Tree {
   Ctx *ctx = nullptr;
@@ -178,9 +191,14 @@ struct Tree {
   // Finer tree over the same points, used only by kNN when the reference tree has
   // leaves over capacity at max depth (kNN ids depend on distances only).
   Tree *knn_tree = nullptr;
+  TreeView *d_view = nullptr;  // device copy of this tree's TreeView
 
   ~Tree();
 };
+
+// Host TreeView of a built tree, and its device copy in t->d_view (and the finer index's).
+TreeView view_of(const Tree *t);
+void publish_views(Tree *t);
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets,
                   const double *boxes, const uint32_t *depths, const uint32_t *caps);
@@ -208,6 +226,11 @@ Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint
                     double tol);
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
                   double inc, double max_dist);
+// One launch over queries on several trees of one kind and dimension; host inputs.
+// query: 0 kNN (qk per-query k, p0 max_within, p1 tol), 1 range (mode, p0 tol),
+// 2 ray first (p0 tol, p1 increment, p2 max distance).
+Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees, uint64_t nq, const double *q,
+                    const uint32_t *qview, const uint32_t *qk, double p0, double p1, double p2, int mode);
 Result *query_collide(const Tree *t, uint32_t set, double tol);
 
 // host-side mirrors of reference helpers

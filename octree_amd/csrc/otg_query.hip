@@ -22,16 +22,6 @@ namespace otg {
 
 constexpr int kQB = 128;  // lanes per block for traversals (2 wavefronts)
 
-struct TreeView {
-  const NodeInfo *info;
-  const uint32_t *rend;
-  const double *nmin;
-  const double *nsize;
-  const double *geom;  // SoA, sorted order
-  uint64_t n;
-  const uint32_t *ids;
-  const SetParams *sets;
-};
 
 // Context whose stream and allocator run a query: the calling thread's
 // override (serving workers each own one) or the tree's context.
@@ -39,8 +29,35 @@ thread_local Ctx *tl_run_ctx = nullptr;
 Ctx *run_ctx(const Tree *t) { return tl_run_ctx ? tl_run_ctx : t->ctx; }
 void set_thread_run_ctx(Ctx *c) { tl_run_ctx = c; }
 
-static TreeView view_of(const Tree *t) {
+TreeView view_of(const Tree *t) {
   return TreeView{t->d_info, t->d_rend, t->d_nmin, t->d_nsize, t->d_geom, t->n, t->d_ids, t->d_sets};
+}
+
+void publish_views(Tree *t) {
+  for (Tree *x = t; x; x = x->knn_tree) {
+    if (!x->d_view) x->d_view = (TreeView *)x->ctx->alloc(sizeof(TreeView));
+    const TreeView hv = view_of(x);
+    OTG_HIP_CHECK(hipMemcpyAsync(x->d_view, &hv, sizeof(TreeView), hipMemcpyHostToDevice, x->ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(x->ctx->stream));
+  }
+}
+
+// Runs f(tree view) for this lane's tree.  The view index is made wave-uniform
+// (one tree per pass over the lanes of a wavefront), so the tree's pointers are
+// loaded once into scalar registers; queries are ordered by tree, so a wavefront
+// almost always finishes in one pass.
+template <typename F>
+__device__ __forceinline__ void with_tree(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                          uint64_t qi, F &&f) {
+  const uint32_t mine = qview ? qview[qi] : 0u;
+  for (;;) {
+    const uint32_t v = __builtin_amdgcn_readfirstlane(mine);
+    if (mine == v) {
+      const TreeView tv = views[v];
+      f(tv);
+      return;
+    }
+  }
 }
 
 template <int D>
@@ -169,13 +186,10 @@ __device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KC
 // entries carry the child's squared wall distance computed when it was pushed,
 // so a pop re-checks it against the current F without reloading the node box.
 template <int D, int KCAP>
-__global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
-                                              const uint32_t *__restrict__ order, uint64_t nq, uint32_t k, double F0,
-                                              double tol, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                              uint32_t *__restrict__ overflow) {
-  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (ti >= nq) return;
-  const uint64_t qi = order ? order[ti] : ti;
+__device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
+                                          const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
+                                          uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                          uint32_t *__restrict__ overflow) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double q[D];
@@ -242,6 +256,24 @@ __global__ void __launch_bounds__(kQB) k_knn(TreeView tv, const double *__restri
     if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
 }
 
+// views/qview: the trees of this launch and each query's tree (qview null: one tree);
+// qk: per-query k (null: k for all), every k <= KCAP - slack.
+template <int D, int KCAP>
+__global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                              const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
+                                              const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
+                                              uint64_t nq, uint32_t k, double F0, double tol,
+                                              uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                              uint32_t *__restrict__ overflow) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  const uint32_t kq = qk ? qk[qi] : k;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, out_ids, out_cnt, overflow);
+  });
+}
+
 // ---------------------------------------------------------------------------
 // range search
 // ---------------------------------------------------------------------------
@@ -271,13 +303,10 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
 }
 
 template <int D, bool BOX, bool FILL>
-__global__ void __launch_bounds__(kQB) k_range(TreeView tv, const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
-                                                const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode, double tol,
-                                                uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
-                                                uint32_t *__restrict__ out) {
-  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (ti >= nq) return;
-  const uint64_t qi = order ? order[ti] : ti;
+__device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
+                                            const uint32_t *__restrict__ qset, int overlap_mode, double tol,
+                                            uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                            uint32_t *__restrict__ out) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double rmin[D], rmax[D];
@@ -378,6 +407,20 @@ __global__ void __launch_bounds__(kQB) k_range(TreeView tv, const double *__rest
     }
   }
   if (!FILL) counts[qi] = cnt;
+}
+
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                                const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
+                                                const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode,
+                                                double tol, uint32_t *__restrict__ counts,
+                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, counts, offsets, out);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -482,13 +525,10 @@ struct RayTester {
 };
 
 template <int D, bool BOX, int RCAP>
-__global__ void __launch_bounds__(kQB) k_ray(TreeView tv, const double *__restrict__ rays, const uint32_t *__restrict__ qset,
-                                              const uint32_t *__restrict__ order, uint64_t nq, double tol, double inc,
-                                              double max_dist, uint32_t *__restrict__ out_ids,
-                                              uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
-  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (ti >= nq) return;
-  const uint64_t qi = order ? order[ti] : ti;
+__device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
+                                          const uint32_t *__restrict__ qset, double tol, double inc, double max_dist,
+                                          uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                          uint32_t *__restrict__ overflow) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   uint32_t cnt = 0;
@@ -607,6 +647,20 @@ __global__ void __launch_bounds__(kQB) k_ray(TreeView tv, const double *__restri
   }
 }
 
+template <int D, bool BOX, int RCAP>
+__global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                              const double *__restrict__ rays, const uint32_t *__restrict__ qset,
+                                              const uint32_t *__restrict__ order, uint64_t nq, double tol, double inc,
+                                              double max_dist, uint32_t *__restrict__ out_ids,
+                                              uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, out_ids, out_cnt, overflow);
+  });
+}
+
 // ---------------------------------------------------------------------------
 // self collision (boxes): every entity queries the tree with its own box and
 // keeps partners with a larger id.
@@ -670,12 +724,15 @@ __global__ void __launch_bounds__(kQB) k_collide(TreeView tv, uint32_t set, doub
 template <int D>
 __global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q, uint32_t stride,
                                                      const uint32_t *__restrict__ qset, uint64_t nq,
-                                                     const SetParams *__restrict__ sets, uint32_t cbits,
-                                                     uint64_t *__restrict__ keys, uint32_t *__restrict__ idx) {
+                                                     const TreeView *__restrict__ views,
+                                                     const uint32_t *__restrict__ qview, uint32_t cbits,
+                                                     uint32_t vshift, uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ idx) {
   uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= nq) return;
   const uint32_t s = qset ? qset[i] : 0u;
-  const SetParams &sp = sets[s];
+  const uint32_t v = qview ? qview[i] : 0u;
+  const SetParams &sp = views[v].sets[s];
   const uint32_t M = sp.max_depth;
   const double top = (double)((1ull << M) - 1);
   uint32_t g[D];
@@ -684,22 +741,36 @@ __global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q
     const double x = (q[i * stride + d] - sp.space_min[d]) * sp.raster[d];
     g[d] = (x > 0.0) ? (uint32_t)(x < top ? x : top) : 0u;
   }
-  keys[i] = ((uint64_t)s << cbits) | morton_encode<D>(g);
+  keys[i] = ((uint64_t)v << vshift) | ((uint64_t)s << cbits) | morton_encode<D>(g);
   idx[i] = (uint32_t)i;
 }
 
+// The trees one launch serves: device TreeViews, each query's view index
+// (null: a single tree) and the widest key layout among them.
+struct ViewSet {
+  const TreeView *views = nullptr;
+  const uint32_t *qview = nullptr;
+  uint32_t nviews = 1;
+  uint32_t cbits = 0, sbits = 0;
+};
+
+static uint32_t bit_width32(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+
 template <int D>
-static Buf<uint32_t> query_order(const Tree *t, const double *q, uint32_t stride, const uint32_t *qs, uint64_t nq) {
-  Ctx *ctx = run_ctx(t);
+static Buf<uint32_t> query_order(Ctx *ctx, const ViewSet &vs, const double *q, uint32_t stride, const uint32_t *qs,
+                                 uint64_t nq) {
   Buf<uint32_t> idx(ctx, nq);
   if (nq < 2) {
     if (nq) OTG_HIP_CHECK(hipMemsetAsync(idx.p, 0, 4, ctx->stream));
     return idx;
   }
+  const uint32_t vbits = bit_width32(vs.nviews - 1);
+  const uint32_t vshift = vs.cbits + vs.sbits;
+  if (vshift + vbits > 64) throw Error{OTG_ERR_UNSUPPORTED, "query ordering key exceeds 64 bits"};
   Buf<uint64_t> keys(ctx, nq);
-  launch(ctx, "query_keys", k_query_keys<D>, dim3(div_up(nq, 256)), dim3(256), 0, q, stride, qs, nq,
-         (const SetParams *)t->d_sets, t->layout.cbits, keys.p, idx.p);
-  radix_sort_pairs(ctx, keys.p, idx.p, nq, t->layout.cbits + t->layout.sbits);
+  launch(ctx, "query_keys", k_query_keys<D>, dim3(div_up(nq, 256)), dim3(256), 0, q, stride, qs, nq, vs.views,
+         vs.qview, vs.cbits, vshift, keys.p, idx.p);
+  radix_sort_pairs(ctx, keys.p, idx.p, nq, vshift + vbits);
   return idx;
 }
 
@@ -796,13 +867,13 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
 }
 
 template <int D, int KCAP>
-static Result *knn_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, uint32_t k, double F0, double tol) {
-  Ctx *ctx = run_ctx(t);
-  Buf<uint32_t> order = query_order<D>(t, q, D, qs, nq);
+static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
+                        const uint32_t *qk, uint32_t k, double F0, double tol) {
+  Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
   Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
-  launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs, (const uint32_t *)order.p,
-         nq, k, F0, tol, ids.p, cnt.p, ovf.p);
+  launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
+         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p);
   uint32_t hovf = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
@@ -813,27 +884,16 @@ static Result *knn_impl(const Tree *t, uint64_t nq, const double *q, const uint3
   return r;
 }
 
-Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
-                  double max_within, double tol) {
-  if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
-  if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
-  Ctx *ctx = run_ctx(t);
-  if (t->knn_tree) t = t->knn_tree;  // same ids: the kNN set depends on distances only
-  DevInput in;
-  stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
-  const double F0 = (max_within == DBL_MAX) ? DBL_MAX : fmax(tol, max_within * (1.0 + tol));
-  if (nq == 0 || t->n == 0 || k == 0) {
-    Buf<uint32_t> z(ctx, std::max<uint64_t>(nq, 1));
-    OTG_HIP_CHECK(hipMemsetAsync(z.p, 0, std::max<uint64_t>(nq, 1) * 4, ctx->stream));
-    return fixed_to_csr(ctx, z.p, z.p, 0, nq);
-  }
-#define OTG_KNN(DD)                                                                     \
-  case DD:                                                                              \
-    if (k <= 8) return knn_impl<DD, 16>(t, nq, in.q, in.sets, k, F0, tol);               \
-    if (k <= 16) return knn_impl<DD, 20>(t, nq, in.q, in.sets, k, F0, tol);              \
-    if (k <= 32) return knn_impl<DD, 36>(t, nq, in.q, in.sets, k, F0, tol);              \
-    return knn_impl<DD, 72>(t, nq, in.q, in.sets, k, F0, tol);
-  switch (t->dim) {
+// kmax = the largest k of the launch (per-query k in qk, or k for all)
+static Result *knn_dispatch(Ctx *ctx, int dim, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
+                            const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol) {
+#define OTG_KNN(DD)                                                                    \
+  case DD:                                                                             \
+    if (kmax <= 8) return knn_impl<DD, 16>(ctx, vs, nq, q, qs, qk, k, F0, tol);          \
+    if (kmax <= 16) return knn_impl<DD, 20>(ctx, vs, nq, q, qs, qk, k, F0, tol);         \
+    if (kmax <= 32) return knn_impl<DD, 36>(ctx, vs, nq, q, qs, qk, k, F0, tol);         \
+    return knn_impl<DD, 72>(ctx, vs, nq, q, qs, qk, k, F0, tol);
+  switch (dim) {
     OTG_KNN(2)
     OTG_KNN(3)
     OTG_KNN(4)
@@ -844,13 +904,50 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
   throw Error{OTG_ERR_INVALID_ARG, "dimension"};
 }
 
-template <int D, bool BOX>
-static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, int mode, double tol) {
+static Result *empty_result(Ctx *ctx, uint64_t nq) {
+  Buf<uint32_t> z(ctx, std::max<uint64_t>(nq, 1));
+  OTG_HIP_CHECK(hipMemsetAsync(z.p, 0, std::max<uint64_t>(nq, 1) * 4, ctx->stream));
+  return fixed_to_csr(ctx, z.p, z.p, 0, nq);
+}
+
+static ViewSet single_view(const Tree *t) {
+  ViewSet vs;
+  vs.views = t->d_view;
+  vs.cbits = t->layout.cbits;
+  vs.sbits = t->layout.sbits;
+  return vs;
+}
+
+// kNN answers come from the finer index when there is one (the kNN set depends on distances only).
+static const Tree *knn_index(const Tree *t) { return t->knn_tree ? t->knn_tree : t; }
+// Point range answers too: each entity is tested, or included with a node box
+// that the range contains (ot_query.h:543-580), whichever tree is walked.
+static const Tree *range_index(const Tree *t) {
+  return (t->knn_tree && t->kind == OTG_KIND_POINT) ? t->knn_tree : t;
+}
+
+static double knn_F0(double max_within, double tol) {
+  return (max_within == DBL_MAX) ? DBL_MAX : fmax(tol, max_within * (1.0 + tol));
+}
+
+Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
+                  double max_within, double tol) {
+  if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
+  if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
   Ctx *ctx = run_ctx(t);
-  TreeView tv = view_of(t);
-  Buf<uint32_t> order = query_order<D>(t, q, 2 * D, qs, nq);
+  t = knn_index(t);
+  DevInput in;
+  stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
+  if (nq == 0 || t->n == 0 || k == 0) return empty_result(ctx, nq);
+  return knn_dispatch(ctx, t->dim, single_view(t), nq, in.q, in.sets, nullptr, k, k, knn_F0(max_within, tol), tol);
+}
+
+template <int D, bool BOX>
+static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, int mode,
+                          double tol) {
+  Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
   Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
-  launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs,
+  launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
          (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
   Result *r = new Result();
   r->ctx = ctx;
@@ -865,25 +962,18 @@ static Result *range_impl(const Tree *t, uint64_t nq, const double *q, const uin
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
-  launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, tv, q, qs,
+  launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
          (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
   return r;
 }
 
-Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
-                    double tol) {
-  Ctx *ctx = run_ctx(t);
-  // Point trees: the finer index answers the same set (each entity is tested,
-  // or included with a node box that the range contains, ot_query.h:543-580).
-  if (t->knn_tree && t->kind == OTG_KIND_POINT) t = t->knn_tree;
-  DevInput in;
-  stage_inputs(ctx, in, ranges, sets, nq * 2 * t->dim, nq, mem);
-  const bool box = t->kind == OTG_KIND_BOX;
-#define OTG_RANGE(DD)                                                     \
-  case DD:                                                                \
-    return box ? range_impl<DD, true>(t, nq, in.q, in.sets, mode, tol)    \
-               : range_impl<DD, false>(t, nq, in.q, in.sets, mode, tol);
-  switch (t->dim) {
+static Result *range_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
+                              const uint32_t *qs, int mode, double tol) {
+#define OTG_RANGE(DD)                                                  \
+  case DD:                                                             \
+    return box ? range_impl<DD, true>(ctx, vs, nq, q, qs, mode, tol)   \
+               : range_impl<DD, false>(ctx, vs, nq, q, qs, mode, tol);
+  switch (dim) {
     OTG_RANGE(2)
     OTG_RANGE(3)
     OTG_RANGE(4)
@@ -894,15 +984,23 @@ Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint
   throw Error{OTG_ERR_INVALID_ARG, "dimension"};
 }
 
-template <int D, bool BOX>
-static Result *ray_impl(const Tree *t, uint64_t nq, const double *q, const uint32_t *qs, double tol, double inc,
-                        double max_dist) {
+Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
+                    double tol) {
   Ctx *ctx = run_ctx(t);
+  t = range_index(t);
+  DevInput in;
+  stage_inputs(ctx, in, ranges, sets, nq * 2 * t->dim, nq, mem);
+  return range_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, mode, tol);
+}
+
+template <int D, bool BOX>
+static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol,
+                        double inc, double max_dist) {
   constexpr int RCAP = 16;
   Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
-  Buf<uint32_t> order = query_order<D>(t, q, 2 * D, qs, nq);
-  launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, view_of(t), q, qs,
+  Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
+  launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
          (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p);
   uint32_t hovf = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -914,17 +1012,13 @@ static Result *ray_impl(const Tree *t, uint64_t nq, const double *q, const uint3
   return r;
 }
 
-Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol, double inc,
-                  double max_dist) {
-  Ctx *ctx = run_ctx(t);
-  DevInput in;
-  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
-  const bool box = t->kind == OTG_KIND_BOX;
-#define OTG_RAY(DD)                                                                 \
-  case DD:                                                                          \
-    return box ? ray_impl<DD, true>(t, nq, in.q, in.sets, tol, inc, max_dist)        \
-               : ray_impl<DD, false>(t, nq, in.q, in.sets, tol, inc, max_dist);
-  switch (t->dim) {
+static Result *ray_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
+                            const uint32_t *qs, double tol, double inc, double max_dist) {
+#define OTG_RAY(DD)                                                           \
+  case DD:                                                                    \
+    return box ? ray_impl<DD, true>(ctx, vs, nq, q, qs, tol, inc, max_dist)   \
+               : ray_impl<DD, false>(ctx, vs, nq, q, qs, tol, inc, max_dist);
+  switch (dim) {
     OTG_RAY(2)
     OTG_RAY(3)
     OTG_RAY(4)
@@ -933,6 +1027,61 @@ Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t
   }
 #undef OTG_RAY
   throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
+Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol, double inc,
+                  double max_dist) {
+  Ctx *ctx = run_ctx(t);
+  DevInput in;
+  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
+  return ray_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, tol, inc, max_dist);
+}
+
+// One launch over queries on several trees of the same kind and dimension
+// (serving).  qview[i] indexes `trees`; qk[i] is query i's k (kNN).  Host inputs.
+Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees, uint64_t nq, const double *q,
+                    const uint32_t *qview, const uint32_t *qk, double p0, double p1, double p2, int mode) {
+  if (trees.empty()) throw Error{OTG_ERR_INVALID_ARG, "no trees"};
+  const int kind = trees[0]->kind, dim = trees[0]->dim;
+  std::vector<TreeView> hv;
+  ViewSet vs;
+  vs.nviews = (uint32_t)trees.size();
+  uint64_t total_n = 0;
+  for (const Tree *t : trees) {
+    if (t->kind != kind || t->dim != dim) throw Error{OTG_ERR_INVALID_ARG, "trees of one launch differ in kind or dim"};
+    const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : t);
+    hv.push_back(view_of(x));
+    vs.cbits = std::max(vs.cbits, x->layout.cbits);
+    vs.sbits = std::max(vs.sbits, x->layout.sbits);
+    total_n += x->n;
+  }
+  if (query == 0 && kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
+  uint32_t kmax = 0;
+  if (query == 0)
+    for (uint64_t i = 0; i < nq; ++i) kmax = std::max(kmax, qk[i]);
+  if (kmax > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
+  for (uint64_t i = 0; i < nq; ++i)
+    if (qview[i] >= vs.nviews) throw Error{OTG_ERR_INVALID_ARG, "query tree index"};
+  const uint64_t stride = query == 0 ? dim : 2 * dim;
+  Buf<TreeView> dv(ctx, hv.size());
+  Buf<double> dq(ctx, std::max<uint64_t>(nq * stride, 1));
+  Buf<uint32_t> dqv(ctx, std::max<uint64_t>(nq, 1)), dqk(ctx, std::max<uint64_t>(nq, 1));
+  OTG_HIP_CHECK(hipMemcpyAsync(dv.p, hv.data(), hv.size() * sizeof(TreeView), hipMemcpyHostToDevice, ctx->stream));
+  if (nq) {
+    OTG_HIP_CHECK(hipMemcpyAsync(dq.p, q, nq * stride * 8, hipMemcpyHostToDevice, ctx->stream));
+    OTG_HIP_CHECK(hipMemcpyAsync(dqv.p, qview, nq * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (query == 0) OTG_HIP_CHECK(hipMemcpyAsync(dqk.p, qk, nq * 4, hipMemcpyHostToDevice, ctx->stream));
+  }
+  vs.views = dv.p;
+  vs.qview = dqv.p;
+  Result *r;
+  if (nq == 0 || total_n == 0 || (query == 0 && kmax == 0)) r = empty_result(ctx, nq);
+  else if (query == 0) r = knn_dispatch(ctx, dim, vs, nq, dq.p, nullptr, dqk.p, 0, kmax, knn_F0(p0, p1), p1);
+  else if (query == 1) r = range_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, mode, p0);
+  else r = ray_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, p0, p1, p2);
+  // the staged host arrays must outlive the copies: wait before returning
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return r;
 }
 
 template <int D>

@@ -2,26 +2,28 @@
 // batching of query requests.
 //
 // The reference has no serving layer (it is a header-only library); this is the
-// MI355X-side runtime the batched path needs:
-//  * trees are built into HBM and stay resident; adding a tree that does not
-//    fit evicts least-recently-used trees no request is using, and waits for
-//    in-flight work to release memory before giving up;
-//  * requests (a batch of kNN / range / ray queries on one tree) are queued
-//    and admitted by worker slots as they free up; a worker takes everything
-//    pending (up to a query budget), coalesces requests on the same tree with
-//    the same query parameters into one launch, and splits the CSR result back
-//    per request;
-//  * two or more workers, each with its own context (stream + allocator),
-//    take turns, so a batch is built and launched while the previous one is
-//    still on the GPU.
-// A request's ids do not depend on what it was batched with: each query's
-// result is computed independently by the same kernels.
+// runtime the batched path needs on the device side:
+//  * trees are registered with a host copy of their entities and built into HBM;
+//    admission is by memory accounting against the budget: a tree that does not
+//    fit evicts the least-recently-used trees that no running batch uses, and an
+//    evicted tree is rebuilt when a request for it arrives;
+//  * requests (a batch of kNN / range / ray queries on one tree) are queued and
+//    taken by worker slots as they free up; a worker takes everything pending (up
+//    to a query budget) and serves it with one launch per (query kind, tree kind,
+//    dimension, parameters): queries on different trees share the launch (each
+//    query carries its tree index; kNN queries carry their own k);
+//  * two or more workers, each with its own stream and allocator, take turns, so
+//    one batch is prepared and launched while the previous one is on the GPU.
+// A query's ids do not depend on what it was batched with: every query is
+// computed independently by the same traversal.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 
 #include "otg_internal.hpp"
@@ -33,7 +35,7 @@ struct ServerRequest {
   uint32_t tree_id = 0;
   int query = 0;  // 0 kNN, 1 range, 2 ray
   uint32_t k = 0;
-  double p0 = 0, p1 = 0, p2 = 0;  // kNN: max_within, tol ; range: tol ; ray: tol, inc, max_dist
+  double p0 = 0, p1 = 0, p2 = 0;
   int mode = 0;
   uint64_t nq = 0;
   std::vector<double> data;
@@ -49,11 +51,19 @@ struct TreeFree {
 };
 
 struct ServedTree {
+  // registration (kept while evicted, to rebuild on demand)
+  int kind = 0, dim = 0;
+  uint64_t n = 0;
+  std::vector<double> geom;
+  std::vector<double> box;  // empty: bounding box of the entities
+  uint32_t max_depth = 0, cap = 0;
+  // residency
   std::unique_ptr<otg_tree, TreeFree> tree;
   uint64_t bytes = 0;
   uint64_t last_use = 0;
   uint32_t inflight = 0;
-  bool evict_pending = false;
+  bool building = false;
+  uint64_t builds = 0;
 };
 
 static double now_ms() {
@@ -61,14 +71,21 @@ static double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Admission estimate of a tree's device bytes (entities, keys, nodes and a finer
+// index for point trees), used before the build reports the real figure.
+static uint64_t tree_estimate(int kind, int dim, uint64_t n) {
+  const uint64_t per = (kind == OTG_KIND_BOX ? 16ull * dim : 8ull * dim) + 12;
+  return n * per * (kind == OTG_KIND_BOX ? 2 : 3) + (64ull << 20);
+}
+
 class Server {
  public:
   Ctx *build_ctx = nullptr;  // builds and owns tree memory
   std::vector<Ctx *> worker_ctx;
   uint64_t budget = 0;
-  uint64_t resident = 0;
+  uint64_t resident = 0;  // bytes of resident trees + reservations of builds in progress
   uint64_t max_batch_queries = 1ull << 21;
-  std::mutex mu;
+  std::mutex mu, build_mu;
   std::condition_variable cv_work, cv_done, cv_mem;
   std::deque<ServerRequest *> pending;
   std::unordered_map<uint64_t, ServerRequest *> requests;
@@ -77,20 +94,105 @@ class Server {
   bool stop = false;
   uint64_t next_id = 1, tick = 0;
   // statistics
-  uint64_t batches = 0, launches = 0, served_requests = 0, served_queries = 0, evictions = 0;
+  uint64_t batches = 0, launches = 0, served_requests = 0, served_queries = 0, evictions = 0, rebuilds = 0;
 
-  ~Server() {
+  void shutdown() {
     {
       std::lock_guard<std::mutex> g(mu);
       stop = true;
     }
     cv_work.notify_all();
+    cv_done.notify_all();
     for (auto &w : workers) w.join();
+    workers.clear();
     for (auto &kv : requests) {
-      delete kv.second->result;
+      otg_result_free(kv.second->result);
       delete kv.second;
     }
+    requests.clear();
+    pending.clear();
     trees.clear();
+  }
+
+  // Evict least-recently-used idle trees until `need` more bytes fit (mu held;
+  // `self` is the tree being admitted).  A caller that holds pinned trees waits a
+  // bounded time (`bounded`), so two workers never wait on each other's pins.
+  bool make_room(std::unique_lock<std::mutex> &lk, uint64_t need, const ServedTree *self, bool bounded) {
+    if (need > budget) return false;
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(20);
+    for (;;) {
+      if (resident + need <= budget) return true;
+      ServedTree *victim = nullptr;
+      for (auto &kv : trees) {
+        ServedTree &e = kv.second;
+        if (e.tree && e.inflight == 0 && !e.building && (!victim || e.last_use < victim->last_use)) victim = &e;
+      }
+      if (victim) {
+        resident -= victim->bytes;
+        victim->bytes = 0;
+        victim->tree.reset();
+        ++evictions;
+        continue;
+      }
+      // memory held by running batches or other builds comes back when they end
+      bool busy = false;
+      for (auto &kv : trees)
+        if (&kv.second != self) busy |= kv.second.inflight > 0 || kv.second.building;
+      if (!busy) return false;
+      if (!bounded) {
+        cv_mem.wait(lk);
+      } else if (cv_mem.wait_until(lk, deadline) == std::cv_status::timeout) {
+        return resident + need <= budget;
+      }
+    }
+  }
+
+  // Build e's tree into HBM (mu held on entry and exit; released while building).
+  otg_status make_resident(std::unique_lock<std::mutex> &lk, ServedTree &e, bool bounded = false) {
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(20);
+    for (;;) {  // another thread may be building this tree: wait for it
+      if (e.tree) return OTG_OK;
+      if (!e.building) break;
+      if (!bounded) {
+        cv_mem.wait(lk);
+      } else if (cv_mem.wait_until(lk, deadline) == std::cv_status::timeout && e.building) {
+        return OTG_ERR_OOM;
+      }
+    }
+    const uint64_t est = tree_estimate(e.kind, e.dim, e.n);
+    e.building = true;  // claims the build (others wait above)
+    if (!make_room(lk, est, &e, bounded)) {
+      e.building = false;
+      cv_mem.notify_all();
+      return OTG_ERR_OOM;
+    }
+    resident += est;
+    lk.unlock();
+    otg_tree *t = nullptr;
+    otg_status st;
+    {
+      std::lock_guard<std::mutex> bg(build_mu);
+      otg_ctx *bc = static_cast<otg_ctx *>(build_ctx);
+      st = otg_tree_build(bc, e.kind, e.dim, e.n, e.geom.data(), OTG_MEM_HOST, e.box.empty() ? nullptr : e.box.data(),
+                          e.max_depth, e.cap, &t);
+      if (st == OTG_OK) st = otg_ctx_synchronize(bc);  // workers read the tree from their own streams
+    }
+    lk.lock();
+    resident -= est;
+    if (st == OTG_OK) {
+      otg_tree_info ti{};
+      otg_tree_get_info(t, &ti);
+      e.tree.reset(t);
+      e.bytes = ti.device_bytes;
+      resident += e.bytes;
+      e.builds++;
+      if (e.builds > 1) ++rebuilds;
+    } else if (t) {
+      otg_tree_free(t);
+    }
+    e.building = false;
+    cv_mem.notify_all();
+    return st;
   }
 
   void worker_loop(int w) {
@@ -102,7 +204,7 @@ class Server {
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] { return stop || !pending.empty(); });
-        if (stop && pending.empty()) return;
+        if (stop) return;
         uint64_t q = 0;
         while (!pending.empty() && (batch.empty() || q + pending.front()->nq <= max_batch_queries)) {
           ServerRequest *r = pending.front();
@@ -110,20 +212,13 @@ class Server {
           q += r->nq;
           r->t_start = now_ms();
           batch.push_back(r);
-          auto it = trees.find(r->tree_id);
-          if (it != trees.end()) {
-            it->second.inflight++;
-            it->second.last_use = ++tick;
-          }
         }
         ++batches;
       }
-      run_batch(batch);
+      run_batch(ctx, batch);
       {
         std::lock_guard<std::mutex> g(mu);
         for (ServerRequest *r : batch) {
-          auto it = trees.find(r->tree_id);
-          if (it != trees.end() && it->second.inflight) it->second.inflight--;
           r->t_done = now_ms();
           r->done = true;
           served_requests++;
@@ -131,70 +226,117 @@ class Server {
         }
       }
       cv_done.notify_all();
+    }
+  }
+
+  // Group a batch by launch: (query, tree kind, dim, k class, parameters).
+  void run_batch(Ctx *ctx, std::vector<ServerRequest *> &batch) {
+    using Key = std::tuple<int, int, int, int, double, double, double, int>;
+    std::map<Key, std::vector<ServerRequest *>> groups;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      for (ServerRequest *r : batch) {
+        auto it = trees.find(r->tree_id);
+        if (it == trees.end()) {
+          r->status = OTG_ERR_INVALID_ARG;
+          r->error = "tree not registered";
+          continue;
+        }
+        const ServedTree &e = it->second;
+        if (r->query == 0 && e.kind != OTG_KIND_POINT) {
+          r->status = OTG_ERR_UNSUPPORTED;
+          r->error = "kNN on box trees is not built yet";
+          continue;
+        }
+        if (r->query == 0 && r->k > 64) {
+          r->status = OTG_ERR_UNSUPPORTED;
+          r->error = "k > 64";
+          continue;
+        }
+        const int kcls = r->query == 0 ? (r->k <= 8 ? 0 : r->k <= 16 ? 1 : r->k <= 32 ? 2 : 3) : 0;
+        groups[Key{r->query, e.kind, e.dim, kcls, r->p0, r->p1, r->p2, r->mode}].push_back(r);
+      }
+    }
+    for (auto &kv : groups) run_group(ctx, kv.second);
+  }
+
+  void run_group(Ctx *ctx, std::vector<ServerRequest *> &group) {
+    size_t i = 0;
+    while (i < group.size()) {
+      // make the trees resident and pin them for the launch; when memory for the
+      // next tree is held by this launch's own pins, launch what is pinned first
+      std::vector<const Tree *> views;
+      std::vector<uint32_t> pinned;
+      std::unordered_map<uint32_t, uint32_t> view_of_tree;
+      std::vector<ServerRequest *> live;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        for (; i < group.size(); ++i) {
+          ServerRequest *r = group[i];
+          if (!view_of_tree.count(r->tree_id)) {
+            auto it = trees.find(r->tree_id);
+            otg_status st = it == trees.end() ? OTG_ERR_INVALID_ARG : make_resident(lk, it->second, !pinned.empty());
+            it = trees.find(r->tree_id);  // the map may have changed while unlocked
+            if (st != OTG_OK || it == trees.end() || !it->second.tree) {
+              if (st == OTG_ERR_OOM && !pinned.empty()) break;  // retried with no pins held
+              r->status = st != OTG_OK ? st : OTG_ERR_INVALID_ARG;
+              r->error = "tree could not be made resident";
+              continue;
+            }
+            ServedTree &e = it->second;
+            e.inflight++;
+            e.last_use = ++tick;
+            pinned.push_back(r->tree_id);
+            view_of_tree.emplace(r->tree_id, (uint32_t)views.size());
+            views.push_back(e.tree.get());
+          }
+          live.push_back(r);
+        }
+      }
+      if (!live.empty()) launch_group(ctx, live, views, view_of_tree);
+      {
+        std::lock_guard<std::mutex> g(mu);
+        for (uint32_t id : pinned) {
+          auto it = trees.find(id);
+          if (it != trees.end() && it->second.inflight) it->second.inflight--;
+        }
+      }
       cv_mem.notify_all();
     }
   }
 
-  // Coalesce requests with the same (tree, query, parameters) into one launch.
-  void run_batch(std::vector<ServerRequest *> &batch) {
-    std::vector<bool> used(batch.size(), false);
-    for (size_t i = 0; i < batch.size(); ++i) {
-      if (used[i]) continue;
-      ServerRequest *a = batch[i];
-      std::vector<ServerRequest *> group;
-      for (size_t j = i; j < batch.size(); ++j) {
-        ServerRequest *b = batch[j];
-        if (used[j]) continue;
-        if (b->tree_id == a->tree_id && b->query == a->query && b->k == a->k && b->p0 == a->p0 && b->p1 == a->p1 &&
-            b->p2 == a->p2 && b->mode == a->mode) {
-          group.push_back(b);
-          used[j] = true;
-        }
-      }
-      run_group(group);
-    }
-  }
-
-  void run_group(std::vector<ServerRequest *> &group) {
-    Tree *t = nullptr;
-    {
-      std::lock_guard<std::mutex> g(mu);
-      auto it = trees.find(group[0]->tree_id);
-      if (it != trees.end()) t = it->second.tree.get();
-    }
-    if (!t) {
-      for (auto *r : group) {
-        r->status = OTG_ERR_INVALID_ARG;
-        r->error = "tree not resident";
-      }
-      return;
-    }
-    const int stride = (group[0]->query == 0) ? t->dim : 2 * t->dim;
+  void launch_group(Ctx *ctx, std::vector<ServerRequest *> &group, const std::vector<const Tree *> &views,
+                    const std::unordered_map<uint32_t, uint32_t> &view_of_tree) {
+    const ServerRequest *a = group[0];
+    const int dim = views[0]->dim;
+    const int stride = a->query == 0 ? dim : 2 * dim;
     uint64_t total = 0;
     for (auto *r : group) total += r->nq;
-    std::vector<double> cat;
-    const double *data = nullptr;
-    if (group.size() == 1) {
-      data = group[0]->data.data();
-    } else {
-      cat.reserve(total * stride);
-      for (auto *r : group) cat.insert(cat.end(), r->data.begin(), r->data.end());
-      data = cat.data();
-    }
     try {
-      ServerRequest *a = group[0];
-      std::unique_ptr<Result> res;
-      if (a->query == 0) res.reset(query_knn(t, total, data, nullptr, OTG_MEM_HOST, a->k, a->p0, a->p1));
-      else if (a->query == 1) res.reset(query_range(t, total, data, nullptr, OTG_MEM_HOST, a->mode, a->p0));
-      else res.reset(query_ray(t, total, data, nullptr, OTG_MEM_HOST, a->p0, a->p1, a->p2));
-      launches++;
-      Ctx *ctx = res->ctx;
+      std::vector<double> q;
+      std::vector<uint32_t> qv(total), qk;
+      q.reserve(total * stride);
+      if (a->query == 0) qk.resize(total);
+      uint64_t at = 0;
+      for (auto *r : group) {
+        q.insert(q.end(), r->data.begin(), r->data.end());
+        const uint32_t v = view_of_tree.at(r->tree_id);
+        std::fill(qv.begin() + at, qv.begin() + at + r->nq, v);
+        if (a->query == 0) std::fill(qk.begin() + at, qk.begin() + at + r->nq, r->k);
+        at += r->nq;
+      }
+      std::unique_ptr<Result> res(query_multi(ctx, a->query, views, total, q.data(), qv.data(),
+                                              qk.empty() ? nullptr : qk.data(), a->p0, a->p1, a->p2, a->mode));
       std::vector<uint64_t> off(total + 1);
       std::vector<uint32_t> ids(res->nids);
       OTG_HIP_CHECK(hipMemcpyAsync(off.data(), res->d_offsets, (total + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
       if (res->nids)
         OTG_HIP_CHECK(hipMemcpyAsync(ids.data(), res->d_ids, res->nids * 4, hipMemcpyDeviceToHost, ctx->stream));
       OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      {
+        std::lock_guard<std::mutex> g(mu);
+        launches++;
+      }
       uint64_t q0 = 0;
       for (auto *r : group) {
         otg_result *out = new otg_result();
@@ -213,33 +355,10 @@ class Server {
         r->status = e.status;
         r->error = e.msg;
       }
+    } catch (const std::bad_alloc &) {
+      for (auto *r : group) r->status = OTG_ERR_OOM;
     } catch (...) {
       for (auto *r : group) r->status = OTG_ERR_HIP;
-    }
-  }
-
-  // Evict least-recently-used idle trees until `need` more bytes fit.
-  bool make_room(std::unique_lock<std::mutex> &lk, uint64_t need) {
-    if (need > budget) return false;
-    for (;;) {
-      if (resident + need <= budget) return true;
-      uint32_t victim = 0;
-      uint64_t best = UINT64_MAX;
-      bool found = false;
-      for (auto &kv : trees)
-        if (kv.second.inflight == 0 && kv.second.last_use < best) {
-          best = kv.second.last_use;
-          victim = kv.first;
-          found = true;
-        }
-      if (found) {
-        resident -= trees[victim].bytes;
-        trees.erase(victim);
-        ++evictions;
-        continue;
-      }
-      if (trees.empty()) return resident + need <= budget;
-      cv_mem.wait(lk);  // everything resident is in use: wait for a batch to finish
     }
   }
 };
@@ -247,8 +366,6 @@ class Server {
 }  // namespace otg
 
 struct otg_server : public otg::Server {};
-
-using otg::Error;
 
 extern "C" {
 
@@ -285,19 +402,7 @@ OTG_API void otg_server_destroy(otg_server *s) {
   if (!s) return;
   otg_ctx *bc = static_cast<otg_ctx *>(s->build_ctx);
   std::vector<otg::Ctx *> wcs = s->worker_ctx;
-  {
-    std::lock_guard<std::mutex> g(s->mu);
-    s->stop = true;
-  }
-  s->cv_work.notify_all();
-  for (auto &w : s->workers) w.join();
-  s->workers.clear();
-  for (auto &kv : s->requests) {
-    delete kv.second->result;
-    delete kv.second;
-  }
-  s->requests.clear();
-  s->trees.clear();
+  s->shutdown();
   delete s;
   for (auto *w : wcs) otg_ctx_destroy(static_cast<otg_ctx *>(w));
   otg_ctx_destroy(bc);
@@ -305,66 +410,78 @@ OTG_API void otg_server_destroy(otg_server *s) {
 
 OTG_API otg_status otg_server_add_tree(otg_server *s, uint32_t tree_id, int kind, int dim, uint64_t n, const double *geom,
                                        int mem, const double *box_or_null, uint32_t max_depth, uint32_t max_elements) {
-  if (!s) return OTG_ERR_INVALID_ARG;
-  otg_ctx *bc = static_cast<otg_ctx *>(s->build_ctx);
-  // size estimate for admission: entities + nodes (~n/2 nodes worst case) + a finer kNN index for points
-  const uint64_t per = (kind == OTG_KIND_BOX ? 16ull * dim : 8ull * dim) + 12;
-  const uint64_t est = n * per * (kind == OTG_KIND_BOX ? 2 : 3) + (64ull << 20);
-  {
-    std::unique_lock<std::mutex> lk(s->mu);
+  if (!s || (n && !geom) || (kind != OTG_KIND_POINT && kind != OTG_KIND_BOX) || dim < 2 || dim > otg::kMaxDim ||
+      (mem != OTG_MEM_HOST && mem != OTG_MEM_DEVICE))
+    return OTG_ERR_INVALID_ARG;
+  otg::ServedTree e;
+  e.kind = kind;
+  e.dim = dim;
+  e.n = n;
+  e.max_depth = max_depth;
+  e.cap = max_elements;
+  const uint64_t cols = kind == OTG_KIND_BOX ? 2ull * dim : (uint64_t)dim;
+  try {
+    e.geom.resize(n * cols);
+    if (box_or_null) e.box.assign(box_or_null, box_or_null + 2 * dim);
+  } catch (...) {
+    return OTG_ERR_OOM;
+  }
+  if (n) {
+    if (mem == OTG_MEM_HOST) {
+      std::copy(geom, geom + n * cols, e.geom.begin());
+    } else {
+      (void)hipSetDevice(s->build_ctx->device);
+      if (hipMemcpy(e.geom.data(), geom, n * cols * 8, hipMemcpyDeviceToHost) != hipSuccess) return OTG_ERR_HIP;
+    }
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->trees.count(tree_id)) {  // replace: wait until nothing uses the old one
+    s->cv_mem.wait(lk, [&] {
+      auto i2 = s->trees.find(tree_id);
+      return i2 == s->trees.end() || (i2->second.inflight == 0 && !i2->second.building);
+    });
     auto it = s->trees.find(tree_id);
     if (it != s->trees.end()) {
-      if (it->second.inflight) return OTG_ERR_INVALID_ARG;
       s->resident -= it->second.bytes;
       s->trees.erase(it);
     }
-    if (!s->make_room(lk, est)) return OTG_ERR_OOM;
-    s->resident += est;  // reserved while building
   }
-  otg_tree *t = nullptr;
-  otg_status st = otg_tree_build(bc, kind, dim, n, geom, mem, box_or_null, max_depth, max_elements, &t);
-  if (st == OTG_OK) st = otg_ctx_synchronize(bc);  // workers read the tree from their own streams
-  if (st != OTG_OK && t) {
-    otg_tree_free(t);
-    t = nullptr;
+  otg::ServedTree &slot = s->trees[tree_id];
+  slot = std::move(e);
+  otg_status st = s->make_resident(lk, slot);
+  if (st != OTG_OK) {
+    auto i3 = s->trees.find(tree_id);
+    if (i3 != s->trees.end() && !i3->second.tree && !i3->second.building && i3->second.inflight == 0)
+      s->trees.erase(i3);
   }
-  std::lock_guard<std::mutex> g(s->mu);
-  s->resident -= est;
-  if (st != OTG_OK) return st;
-  otg_tree_info ti{};
-  otg_tree_get_info(t, &ti);
-  otg::ServedTree e;
-  e.tree.reset(t);
-  e.bytes = ti.device_bytes;
-  e.last_use = ++s->tick;
-  s->resident += e.bytes;
-  s->trees[tree_id] = std::move(e);
-  return OTG_OK;
+  return st;
 }
 
 OTG_API otg_status otg_server_remove_tree(otg_server *s, uint32_t tree_id) {
   if (!s) return OTG_ERR_INVALID_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
+  if (!s->trees.count(tree_id)) return OTG_ERR_INVALID_ARG;
+  s->cv_mem.wait(lk, [&] {
+    auto i2 = s->trees.find(tree_id);
+    return i2 == s->trees.end() || (i2->second.inflight == 0 && !i2->second.building);
+  });
   auto it = s->trees.find(tree_id);
-  if (it == s->trees.end()) return OTG_ERR_INVALID_ARG;
-  s->cv_mem.wait(lk, [&] { return s->trees.count(tree_id) == 0 || s->trees[tree_id].inflight == 0; });
-  it = s->trees.find(tree_id);
   if (it != s->trees.end()) {
     s->resident -= it->second.bytes;
     s->trees.erase(it);
   }
+  s->cv_mem.notify_all();
   return OTG_OK;
 }
 
 OTG_API otg_status otg_server_has_tree(otg_server *s, uint32_t tree_id, int *present) {
   if (!s || !present) return OTG_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> g(s->mu);
-  *present = s->trees.count(tree_id) ? 1 : 0;
+  auto it = s->trees.find(tree_id);
+  *present = it == s->trees.end() ? 0 : (it->second.tree ? 1 : 2);
   return OTG_OK;
 }
 
-// query: 0 kNN (k, p0 = max_within, p1 = tolerance), 1 range (mode, p0 = tolerance),
-// 2 ray first (p0 = tolerance, p1 = tolerance increment, p2 = max distance).
 OTG_API otg_status otg_server_submit(otg_server *s, uint32_t tree_id, int query, uint64_t nq, const double *queries,
                                      uint32_t k, double p0, double p1, double p2, int mode, uint64_t *request_id) {
   if (!s || !request_id || (nq && !queries) || query < 0 || query > 2) return OTG_ERR_INVALID_ARG;
@@ -373,7 +490,7 @@ OTG_API otg_status otg_server_submit(otg_server *s, uint32_t tree_id, int query,
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->trees.find(tree_id);
     if (it == s->trees.end()) return OTG_ERR_INVALID_ARG;
-    dim = it->second.tree->dim;
+    dim = it->second.dim;
   }
   auto *r = new (std::nothrow) otg::ServerRequest();
   if (!r) return OTG_ERR_OOM;
@@ -404,7 +521,6 @@ OTG_API otg_status otg_server_submit(otg_server *s, uint32_t tree_id, int query,
   return OTG_OK;
 }
 
-// Blocks until the request is done; the caller owns *out (free with otg_result_free).
 OTG_API otg_status otg_server_wait(otg_server *s, uint64_t request_id, otg_result **out, double *latency_ms,
                                    double *queue_ms) {
   if (!s || !out) return OTG_ERR_INVALID_ARG;
@@ -413,8 +529,9 @@ OTG_API otg_status otg_server_wait(otg_server *s, uint64_t request_id, otg_resul
   auto it = s->requests.find(request_id);
   if (it == s->requests.end()) return OTG_ERR_INVALID_ARG;
   otg::ServerRequest *r = it->second;
-  s->cv_done.wait(lk, [&] { return r->done; });
-  s->requests.erase(it);
+  s->cv_done.wait(lk, [&] { return r->done || s->stop; });
+  if (!r->done) return OTG_ERR_INVALID_ARG;
+  s->requests.erase(r->id);
   lk.unlock();
   if (latency_ms) *latency_ms = r->t_done - r->t_submit;
   if (queue_ms) *queue_ms = r->t_start - r->t_submit;
@@ -425,17 +542,21 @@ OTG_API otg_status otg_server_wait(otg_server *s, uint64_t request_id, otg_resul
   return st;
 }
 
-OTG_API otg_status otg_server_stats(otg_server *s, uint64_t *stats /* 8 values */) {
+OTG_API otg_status otg_server_stats(otg_server *s, uint64_t *stats) {
   if (!s || !stats) return OTG_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> g(s->mu);
+  uint64_t res_trees = 0;
+  for (auto &kv : s->trees) res_trees += kv.second.tree ? 1 : 0;
   stats[0] = s->batches;
   stats[1] = s->launches;
   stats[2] = s->served_requests;
   stats[3] = s->served_queries;
   stats[4] = s->evictions;
   stats[5] = s->resident;
-  stats[6] = s->trees.size();
+  stats[6] = res_trees;
   stats[7] = s->pending.size();
+  stats[8] = s->rebuilds;
+  stats[9] = s->trees.size();
   return OTG_OK;
 }
 

@@ -141,9 +141,10 @@ def test_lru_eviction_under_budget():
     srv.add_tree(1, "point", 3, pts[0], 8, None, 16)
     one = srv.stats()["resident_bytes"]
     assert one > 0
-    # a budget that holds two trees of this size (plus the build reservation) but not three
+    # admission reserves an estimate (36 B x 3 per point + 64 MiB) on top of what is resident:
+    # this budget admits a second tree next to one resident tree, but not a third next to two
     srv.close()
-    srv = octree_amd.Server(0, hbm_budget=int(one * 2.5) + (200000 * 36 * 3 + (64 << 20)), workers=1)
+    srv = octree_amd.Server(0, hbm_budget=int(one * 1.5) + (200000 * 36 * 3 + (64 << 20)), workers=1)
     srv.add_tree(1, "point", 3, pts[0], 8, None, 16)
     srv.add_tree(2, "point", 3, pts[1], 8, None, 16)
     r, _, _ = srv.wait(srv.submit_knn(1, pts[0][:10], 1))  # touch tree 1: tree 2 becomes LRU
@@ -153,9 +154,50 @@ def test_lru_eviction_under_budget():
     st = srv.stats()
     assert st["evictions"] >= 1
     assert srv.has_tree(3) and srv.has_tree(1) and not srv.has_tree(2)
+    assert srv.tree_state(2) == 2  # registered, evicted
+    # a request on the evicted tree rebuilds it (evicting the least recently used one)
+    r, _, _ = srv.wait(srv.submit_knn(2, pts[1][:5], 1))
+    assert np.array_equal(r.csr()[1], np.arange(5, dtype=np.uint32))
+    st = srv.stats()
+    assert st["rebuilds"] == 1 and srv.has_tree(2) and st["resident_trees"] == 2
     with pytest.raises(octree_amd.OtgError):
-        srv.submit_knn(2, pts[1][:4], 1)
+        srv.submit_knn(99, pts[1][:4], 1)  # never registered
     srv.remove_tree(1)
-    assert not srv.has_tree(1)
-    assert srv.stats()["resident_trees"] == 1
+    assert srv.tree_state(1) == 0
+    assert srv.stats()["registered_trees"] == 2
+    srv.close()
+
+
+def test_served_mix_under_eviction(gpu_ctx):
+    """A budget that holds about half of the trees: requests keep evicting and
+    rebuilding trees while others run; every result still equals the direct query."""
+    import octree_amd
+
+    specs = {}
+    for i in range(6):
+        if i % 2 == 0:
+            specs[i] = ("point", 3, gaussian_mixture(120000, 3, 6, 0.0, 100.0, 0.5, 5.0, seed=40 + i), 8, 16)
+        else:
+            specs[i] = ("box", 3, lognormal_boxes(60000, 3, 0.0, 100.0, seed=40 + i), 6, 16)
+    est = sum((120000 * 36 * 3 if k == "point" else 60000 * 60 * 2) + (64 << 20) for k, *_ in specs.values())
+    srv = octree_amd.Server(0, hbm_budget=est // 2, workers=2)
+    direct = {}
+    for tid, (kind, D, geom, M, cap) in specs.items():
+        srv.add_tree(tid, kind, D, geom, M, None, cap)
+        direct[tid] = _direct(gpu_ctx, kind, D, geom, M, cap)
+    work = []
+    for tid, (kind, D, _, _, _) in specs.items():
+        work += [(tid,) + r for r in _requests(100 + tid, D, kind, 10)]
+    order = np.random.default_rng(9).permutation(len(work))
+    work = [work[i] for i in order]
+    ids = [_submit(srv, tid, q, payload, params) for tid, q, payload, params in work]
+    for i, (tid, q, payload, params) in enumerate(work):
+        res, _, _ = srv.wait(ids[i])
+        off, got = res.csr()
+        eoff, exp = _direct_result(direct[tid], q, payload, params)
+        assert np.array_equal(off, eoff) and np.array_equal(got, exp), (tid, q, params, len(payload))
+    st = srv.stats()
+    assert st["evictions"] > 0 and st["rebuilds"] > 0 and st["resident_bytes"] <= est // 2
+    for t in direct.values():
+        t.free()
     srv.close()
