@@ -34,6 +34,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known size);
+    None when no summary covers this kernel and workload."""
+    path = ROOT / "profiles" / "pmc_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if not line.startswith("traffic "):
+            continue
+        kv = dict(x.split("=", 1) for x in line.split()[1:])
+        if kv.get("kernel") == kernel and kv.get("workload", "").replace("_", " ") == workload:
+            return float(kv["bytes"])
+    return None
+
+
 # ---------------------------------------------------------------------------
 # workload generation (seeded, exact doubles)
 # ---------------------------------------------------------------------------
@@ -86,6 +102,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--requests", type=int, default=1500, help="served workload: request count")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 1 GiB device copy (the known-size kernel that scales PMC byte counters)")
     ap.add_argument("--load", type=float, default=0.7, help="served workload: offered load / measured capacity")
     ap.add_argument("--profile-only", action="store_true", help="one batched step, no plain/CPU legs")
     ap.add_argument("--workload", default="batched",
@@ -102,6 +120,14 @@ def main():
 
     import octree_amd
 
+    if args.calibrate:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration: 3 copies of 1 GiB (elementwise copy kernel)")
+        del a, b
     if args.workload == "served":
         print(json.dumps(run_served(args, octree_amd)), flush=True)
         return
@@ -264,17 +290,24 @@ def main():
     dom_ms, dom_n = kt[dom]
     avg_ms = dom_ms / dom_n
     if dom == "knn":
-        # compulsory bytes per query: point 24 + set id 4 + |out| ids x 4 + every returned entity's
-        # coordinates and id (24 + 4) read at least once
-        per_q = 24 + 4 + args.k * 4 + args.k * 28
-        algo = per_q * Q
+        # minimal touched set of each query (computed on the built tree, independent of the
+        # traversal order): query point 24 B + 4 B per returned id + a node record of 37 B
+        # (entity segment 8, child begin 4, depth 1, MinPoint 24) for every node whose wall
+        # distance is below the final radius + 28 B (coordinates, id) per entity those nodes hold
+        f, rk, rr = batched_step(cap)
+        rk.free(); rr.free()
+        fp = f.knn_footprint(d_knn, args.k, set_ids=d_sid)
+        f.free()
+        algo = 24 * Q + 4 * fp["ids"] + 37 * fp["nodes"] + 28 * fp["entities"]
+        out["knn_footprint"] = dict(fp, per_query_bytes=algo / Q)
     elif dom in ("range_count", "range_fill"):
         algo = (48 + 4) * Q
     else:
         algo = (24 + 12 + 24 * passes + 8 + 4) * N
     achieved = algo / (avg_ms * 1e-3) / 1e9
+    traffic = pmc_traffic(dom, f"batched cap={cap}")
     out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo,
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": algo,
                        "avg_launch_ms": avg_ms}
 
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----

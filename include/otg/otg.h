@@ -141,6 +141,12 @@ OTG_API otg_status otg_tree_export(const otg_tree *tree, uint32_t set, uint8_t *
 OTG_API otg_status otg_knn(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
                            uint32_t k, double max_within, double tolerance, otg_result **out);
 
+/* Measurement helper, no counterpart in the reference: runs the kNN batch and sums the
+ * minimal set it must touch -- out[0] = visits of nodes whose wall distance is below the
+ * query's final radius, out[1] = entities held by those nodes, out[2] = result ids. */
+OTG_API otg_status otg_knn_footprint(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids,
+                                     int mem, uint32_t k, double max_within, double tolerance, uint64_t *out);
+
 /* RangeSearch(range, entities, mode, tolerance) (core/ot_query.h:476-583). */
 OTG_API otg_status otg_range(const otg_tree *tree, uint64_t nq, const double *ranges, const uint32_t *set_ids, int mem,
                              int mode, double tolerance, otg_result **out);

@@ -316,6 +316,17 @@ class StaticTree:
         _check(st, "otg_knn", self.ctx)
         return Result(h, self.ctx)
 
+    def knn_footprint(self, points, k: int, max_within: float = DBL_MAX, tolerance: float = BASE_TOLERANCE,
+                      set_ids=None) -> dict:
+        """Minimal set the kNN batch must touch (otg_knn_footprint): node visits with wall
+        distance below each query's final radius, the entities they hold, and result ids."""
+        qp, nq, sp, mem, _keep = self._query_args(points, self.dim, set_ids)
+        out = (C.c_uint64 * 3)()
+        st = _abi.lib().otg_knn_footprint(self.handle, nq, qp, sp, mem, int(k), float(max_within), float(tolerance),
+                                          out)
+        _check(st, "otg_knn_footprint", self.ctx)
+        return {"nodes": int(out[0]), "entities": int(out[1]), "ids": int(out[2])}
+
     def range_batch(self, ranges, mode: int = _abi.OTG_RANGE_INSIDE, tolerance: float = BASE_TOLERANCE,
                     set_ids=None) -> Result:
         qp, nq, sp, mem, _keep = self._query_args(ranges, 2 * self.dim, set_ids)
@@ -394,6 +405,7 @@ class Server:
 
     def remove_tree(self, tree_id: int) -> None:
         _check(_abi.lib().otg_server_remove_tree(self.handle, int(tree_id)), "otg_server_remove_tree")
+        self._dims.pop(int(tree_id), None)
 
     def tree_state(self, tree_id: int) -> int:
         """0 not registered, 1 resident, 2 registered but evicted (rebuilt on demand)."""
@@ -405,6 +417,11 @@ class Server:
         """True when the tree is resident in HBM."""
         return self.tree_state(tree_id) == 1
 
+    def _dim(self, tree_id) -> int:
+        if int(tree_id) not in self._dims:
+            raise OtgError(1, "otg_server_submit", f"tree {tree_id} is not registered")
+        return self._dims[int(tree_id)][1]
+
     def _submit(self, tree_id, query, q, k=0, p0=0.0, p1=0.0, p2=0.0, mode=0) -> int:
         rid = C.c_uint64()
         st = _abi.lib().otg_server_submit(self.handle, int(tree_id), query, q.shape[0], _ptr(q), int(k), float(p0),
@@ -414,15 +431,15 @@ class Server:
 
     def submit_knn(self, tree_id: int, points, k: int, max_within: float = DBL_MAX,
                    tolerance: float = BASE_TOLERANCE) -> int:
-        return self._submit(tree_id, 0, _f64(points, self._dims[int(tree_id)][1]), k=k, p0=max_within, p1=tolerance)
+        return self._submit(tree_id, 0, _f64(points, self._dim(tree_id)), k=k, p0=max_within, p1=tolerance)
 
     def submit_range(self, tree_id: int, ranges, mode: int = _abi.OTG_RANGE_INSIDE,
                      tolerance: float = BASE_TOLERANCE) -> int:
-        return self._submit(tree_id, 1, _f64(ranges, 2 * self._dims[int(tree_id)][1]), p0=tolerance, mode=mode)
+        return self._submit(tree_id, 1, _f64(ranges, 2 * self._dim(tree_id)), p0=tolerance, mode=mode)
 
     def submit_ray(self, tree_id: int, rays, tolerance: float = BASE_TOLERANCE, tolerance_increment: float = 0.0,
                    max_distance: float = DBL_MAX) -> int:
-        return self._submit(tree_id, 2, _f64(rays, 2 * self._dims[int(tree_id)][1]), p0=tolerance,
+        return self._submit(tree_id, 2, _f64(rays, 2 * self._dim(tree_id)), p0=tolerance,
                             p1=tolerance_increment, p2=max_distance)
 
     def wait(self, request_id: int):

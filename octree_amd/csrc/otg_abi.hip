@@ -2,6 +2,7 @@
 // No C++ exception crosses this boundary; every failure becomes a status code.
 #include <cfloat>
 #include <cstdio>
+#include <memory>
 
 #include "otg_internal.hpp"
 
@@ -377,6 +378,20 @@ OTG_API otg_status otg_knn(const otg_tree *t, uint64_t nq, const double *points,
   return guarded(ctx, [&] {
     OTG_HIP_CHECK(hipSetDevice(ctx->device));
     *out = static_cast<otg_result *>(otg::query_knn(t, nq, points, set_ids, mem, k, max_within, tolerance));
+  });
+}
+
+OTG_API otg_status otg_knn_footprint(const otg_tree *t, uint64_t nq, const double *points, const uint32_t *set_ids,
+                                     int mem, uint32_t k, double max_within, double tolerance, uint64_t *out) {
+  if (!t || !out || (nq && !points)) return OTG_ERR_INVALID_ARG;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    uint64_t fp[2] = {0, 0};
+    std::unique_ptr<otg::Result> r(otg::query_knn(t, nq, points, set_ids, mem, k, max_within, tolerance, fp));
+    out[0] = fp[0];
+    out[1] = fp[1];
+    out[2] = r->nids;
   });
 }
 

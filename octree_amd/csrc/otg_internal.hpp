@@ -221,7 +221,8 @@ Ctx *run_ctx(const Tree *t);
 void set_thread_run_ctx(Ctx *c);
 
 Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
-                  double max_within, double tol);
+                  double max_within, double tol,
+                  uint64_t *footprint = nullptr);
 Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
                     double tol);
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,

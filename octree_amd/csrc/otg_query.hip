@@ -189,7 +189,7 @@ template <int D, int KCAP>
 __device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                           const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
                                           uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                          uint32_t *__restrict__ overflow) {
+                                          uint32_t *__restrict__ overflow, double *__restrict__ out_F) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double q[D];
@@ -251,6 +251,7 @@ __device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const
     if (L.any_dropped && L.dropped <= L.F) atomicOr(overflow, 1u);
   }
   out_cnt[qi] = L.cnt;
+  if (out_F) out_F[qi] = L.F;
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
     if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
@@ -264,14 +265,60 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
                                               const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
                                               uint64_t nq, uint32_t k, double F0, double tol,
                                               uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                              uint32_t *__restrict__ overflow) {
+                                              uint32_t *__restrict__ overflow, double *__restrict__ out_F) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
   const uint32_t kq = qk ? qk[qi] : k;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, out_ids, out_cnt, overflow);
+    knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, out_ids, out_cnt, overflow, out_F);
   });
+}
+
+// Measurement helper: the nodes a kNN query must visit (wall distance below the
+// query's final radius F) and the entities they hold, summed over the launch.
+template <int D>
+__global__ void __launch_bounds__(kQB) k_knn_footprint(const TreeView *__restrict__ views,
+                                                        const uint32_t *__restrict__ qview,
+                                                        const double *__restrict__ qpts,
+                                                        const uint32_t *__restrict__ qset,
+                                                        const uint32_t *__restrict__ order, uint64_t nq,
+                                                        const double *__restrict__ Fq,
+                                                        unsigned long long *__restrict__ acc) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  unsigned long long nodes = 0, ents = 0;
+  if (ti < nq) {
+    const uint64_t qi = order ? order[ti] : ti;
+    with_tree(views, qview, qi, [&](const TreeView &tv) {
+      const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+      if (sp.node_cnt == 0) return;
+      const double F = Fq[qi];
+      double q[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+      uint32_t stack[StackCap<D>::value];
+      int top = 0;
+      stack[top++] = (uint32_t)sp.node_off;
+      while (top > 0) {
+        const uint32_t node = stack[--top];
+        bool in;
+        const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+        if (sqrt(w2) >= F) continue;
+        const NodeInfo ni = tv.info[node];
+        ++nodes;
+        ents += ni.ecount;
+        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+      }
+    });
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    nodes += __shfl_down(nodes, o);
+    ents += __shfl_down(ents, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&acc[0], nodes);
+    atomicAdd(&acc[1], ents);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -866,14 +913,28 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
   return r;
 }
 
+// footprint (optional, measurement): out[0..1] += node and entity visits of the minimal walk
 template <int D, int KCAP>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
-                        const uint32_t *qk, uint32_t k, double F0, double tol) {
+                        const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
   Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1);
+  Buf<double> Fq;
+  if (footprint) Fq = Buf<double>(ctx, nq);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
   launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
-         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p);
+         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p);
+  if (footprint) {
+    Buf<unsigned long long> acc(ctx, 2);
+    OTG_HIP_CHECK(hipMemsetAsync(acc.p, 0, 16, ctx->stream));
+    launch(ctx, "knn_footprint", k_knn_footprint<D>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
+           (const uint32_t *)order.p, nq, (const double *)Fq.p, acc.p);
+    unsigned long long h[2] = {0, 0};
+    OTG_HIP_CHECK(hipMemcpyAsync(h, acc.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    footprint[0] += h[0];
+    footprint[1] += h[1];
+  }
   uint32_t hovf = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
@@ -886,13 +947,14 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
 
 // kmax = the largest k of the launch (per-query k in qk, or k for all)
 static Result *knn_dispatch(Ctx *ctx, int dim, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
-                            const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol) {
-#define OTG_KNN(DD)                                                                    \
-  case DD:                                                                             \
-    if (kmax <= 8) return knn_impl<DD, 16>(ctx, vs, nq, q, qs, qk, k, F0, tol);          \
-    if (kmax <= 16) return knn_impl<DD, 20>(ctx, vs, nq, q, qs, qk, k, F0, tol);         \
-    if (kmax <= 32) return knn_impl<DD, 36>(ctx, vs, nq, q, qs, qk, k, F0, tol);         \
-    return knn_impl<DD, 72>(ctx, vs, nq, q, qs, qk, k, F0, tol);
+                            const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol,
+                            uint64_t *footprint = nullptr) {
+#define OTG_KNN(DD)                                                                               \
+  case DD:                                                                                        \
+    if (kmax <= 8) return knn_impl<DD, 16>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);          \
+    if (kmax <= 16) return knn_impl<DD, 20>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);         \
+    if (kmax <= 32) return knn_impl<DD, 36>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);         \
+    return knn_impl<DD, 72>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);
   switch (dim) {
     OTG_KNN(2)
     OTG_KNN(3)
@@ -931,7 +993,7 @@ static double knn_F0(double max_within, double tol) {
 }
 
 Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
-                  double max_within, double tol) {
+                  double max_within, double tol, uint64_t *footprint) {
   if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
   if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
   Ctx *ctx = run_ctx(t);
@@ -939,7 +1001,8 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
   DevInput in;
   stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
   if (nq == 0 || t->n == 0 || k == 0) return empty_result(ctx, nq);
-  return knn_dispatch(ctx, t->dim, single_view(t), nq, in.q, in.sets, nullptr, k, k, knn_F0(max_within, tol), tol);
+  return knn_dispatch(ctx, t->dim, single_view(t), nq, in.q, in.sets, nullptr, k, k, knn_F0(max_within, tol), tol,
+                      footprint);
 }
 
 template <int D, bool BOX>

@@ -229,10 +229,14 @@ class Server {
     }
   }
 
-  // Group a batch by launch: (query, tree kind, dim, k class, parameters).
+  // Serve a batch: its trees are taken in chunks -- the resident ones first, then
+  // the others in chunks that fit the budget -- so each evicted tree is rebuilt at
+  // most once per batch; within a chunk, one launch per (query, tree kind, dim,
+  // k class, parameters).
   void run_batch(Ctx *ctx, std::vector<ServerRequest *> &batch) {
-    using Key = std::tuple<int, int, int, int, double, double, double, int>;
-    std::map<Key, std::vector<ServerRequest *>> groups;
+    std::unordered_map<uint32_t, std::vector<ServerRequest *>> by_tree;
+    std::vector<uint32_t> tree_order;
+    std::vector<std::vector<uint32_t>> chunks;
     {
       std::lock_guard<std::mutex> g(mu);
       for (ServerRequest *r : batch) {
@@ -253,11 +257,43 @@ class Server {
           r->error = "k > 64";
           continue;
         }
-        const int kcls = r->query == 0 ? (r->k <= 8 ? 0 : r->k <= 16 ? 1 : r->k <= 32 ? 2 : 3) : 0;
-        groups[Key{r->query, e.kind, e.dim, kcls, r->p0, r->p1, r->p2, r->mode}].push_back(r);
+        auto &v = by_tree[r->tree_id];
+        if (v.empty()) tree_order.push_back(r->tree_id);
+        v.push_back(r);
       }
+      std::vector<uint32_t> resident_now, others;
+      for (uint32_t id : tree_order) (trees[id].tree ? resident_now : others).push_back(id);
+      if (!resident_now.empty()) chunks.push_back(resident_now);
+      std::vector<uint32_t> cur;
+      uint64_t acc = 0;
+      for (uint32_t id : others) {
+        const ServedTree &e = trees[id];
+        const uint64_t est = tree_estimate(e.kind, e.dim, e.n);
+        if (!cur.empty() && acc + est > budget / 2) {
+          chunks.push_back(cur);
+          cur.clear();
+          acc = 0;
+        }
+        cur.push_back(id);
+        acc += est;
+      }
+      if (!cur.empty()) chunks.push_back(cur);
     }
-    for (auto &kv : groups) run_group(ctx, kv.second);
+    using Key = std::tuple<int, int, int, int, double, double, double, int>;
+    for (const auto &chunk : chunks) {
+      std::map<Key, std::vector<ServerRequest *>> groups;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        for (uint32_t id : chunk) {
+          const ServedTree &e = trees[id];
+          for (ServerRequest *r : by_tree[id]) {
+            const int kcls = r->query == 0 ? (r->k <= 8 ? 0 : r->k <= 16 ? 1 : r->k <= 32 ? 2 : 3) : 0;
+            groups[Key{r->query, e.kind, e.dim, kcls, r->p0, r->p1, r->p2, r->mode}].push_back(r);
+          }
+        }
+      }
+      for (auto &kv : groups) run_group(ctx, kv.second);
+    }
   }
 
   void run_group(Ctx *ctx, std::vector<ServerRequest *> &group) {
@@ -491,6 +527,7 @@ OTG_API otg_status otg_server_submit(otg_server *s, uint32_t tree_id, int query,
     auto it = s->trees.find(tree_id);
     if (it == s->trees.end()) return OTG_ERR_INVALID_ARG;
     dim = it->second.dim;
+    it->second.last_use = ++s->tick;  // recency counts requests, not only launches
   }
   auto *r = new (std::nothrow) otg::ServerRequest();
   if (!r) return OTG_ERR_OOM;

@@ -179,8 +179,14 @@ def test_served_mix_under_eviction(gpu_ctx):
             specs[i] = ("point", 3, gaussian_mixture(120000, 3, 6, 0.0, 100.0, 0.5, 5.0, seed=40 + i), 8, 16)
         else:
             specs[i] = ("box", 3, lognormal_boxes(60000, 3, 0.0, 100.0, seed=40 + i), 6, 16)
-    est = sum((120000 * 36 * 3 if k == "point" else 60000 * 60 * 2) + (64 << 20) for k, *_ in specs.values())
-    srv = octree_amd.Server(0, hbm_budget=est // 2, workers=2)
+    probe = octree_amd.Server(0, hbm_budget=8 << 30, workers=1)
+    for tid, (kind, D, geom, M, cap) in specs.items():
+        probe.add_tree(tid, kind, D, geom, M, None, cap)
+    working = probe.stats()["resident_bytes"]
+    probe.close()
+    # room for about half of the trees plus one admission reservation (the largest estimate)
+    budget = working // 2 + max(120000 * 36 * 3, 60000 * 60 * 2) + (64 << 20)
+    srv = octree_amd.Server(0, hbm_budget=budget, workers=2)
     direct = {}
     for tid, (kind, D, geom, M, cap) in specs.items():
         srv.add_tree(tid, kind, D, geom, M, None, cap)
@@ -197,7 +203,7 @@ def test_served_mix_under_eviction(gpu_ctx):
         eoff, exp = _direct_result(direct[tid], q, payload, params)
         assert np.array_equal(off, eoff) and np.array_equal(got, exp), (tid, q, params, len(payload))
     st = srv.stats()
-    assert st["evictions"] > 0 and st["rebuilds"] > 0 and st["resident_bytes"] <= est // 2
+    assert st["evictions"] > 0 and st["rebuilds"] > 0 and st["resident_bytes"] <= budget
     for t in direct.values():
         t.free()
     srv.close()
