@@ -181,14 +181,48 @@ __device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KC
   }
 }
 
+// Traversal stacks.  Entries are (node, squared wall distance at push time).
+// ArrayStack: private memory with the worst-case depth (root + maxDepth x
+// (2^D - 1) pending siblings).  LdsStack: a short stack in LDS, lane-interleaved
+// so a wavefront's accesses to one depth fall in distinct banks; the wall distance
+// is kept as a float rounded down, so a pop prunes only nodes the exact value
+// would prune too (an unpruned extra node only costs its distance tests).
+template <int D>
+struct ArrayStack {
+  static constexpr int cap = StackCap<D>::value;
+  uint32_t n[cap];
+  double w[cap];
+  __device__ __forceinline__ uint32_t node(int i) const { return n[i]; }
+  __device__ __forceinline__ double wall(int i) const { return w[i]; }
+  __device__ __forceinline__ void set(int i, uint32_t nd, double wl) {
+    n[i] = nd;
+    w[i] = wl;
+  }
+};
+
+constexpr int kLdsStack = 24;  // entries per lane: 24 x 8 B x 128 lanes = 24 KiB per block
+
+struct LdsStack {
+  static constexpr int cap = kLdsStack;
+  uint32_t *n;
+  float *w;
+  __device__ __forceinline__ uint32_t node(int i) const { return n[i * kQB]; }
+  __device__ __forceinline__ double wall(int i) const { return (double)w[i * kQB]; }
+  __device__ __forceinline__ void set(int i, uint32_t nd, double wl) {
+    n[i * kQB] = nd;
+    w[i * kQB] = __double2float_rd(wl);
+  }
+};
+
 // One lane per query (queries sorted by (set, Morton code) so neighbouring lanes
 // walk nearby nodes); depth-first with the nearest child popped first.  Stack
 // entries carry the child's squared wall distance computed when it was pushed,
 // so a pop re-checks it against the current F without reloading the node box.
-template <int D, int KCAP>
-__device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
+// Returns false (writing nothing) when the stack would overflow.
+template <int D, int KCAP, typename Stack>
+__device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                           const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
-                                          uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                          Stack &S, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
                                           uint32_t *__restrict__ overflow, double *__restrict__ out_F) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
@@ -200,21 +234,18 @@ __device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const
   KnnState<KCAP> L;
   knn_init(L, F0);
   if (sp.node_cnt != 0 && k != 0) {
-    uint32_t stack_n[StackCap<D>::value];
-    double stack_w[StackCap<D>::value];
     int top = 0;
     {
       const uint32_t root = (uint32_t)sp.node_off;
       bool in;
-      stack_w[0] = wall_distance2<D>(q, &tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &in);
-      stack_n[0] = root;
+      S.set(0, root, wall_distance2<D>(q, &tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &in));
       top = 1;
     }
     while (top > 0) {
       --top;
-      const uint32_t node = stack_n[top];
+      const uint32_t node = S.node(top);
       // wall2 is 0 inside the node box, and (0 >= F) is the reference's test there too
-      if (knn_node_out(L, stack_w[top], false)) continue;
+      if (knn_node_out(L, S.wall(top), false)) continue;
       const NodeInfo ni = tv.info[node];
       for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
         double d2 = 0.0;
@@ -236,14 +267,13 @@ __device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const
           bool cin;
           const double wc = wall_distance2<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D], &cin);
           if (knn_node_out(L, wc, false)) continue;
+          if (top == Stack::cap) return false;
           int p = top;
-          while (p > base && stack_w[p - 1] < wc) {
-            stack_w[p] = stack_w[p - 1];
-            stack_n[p] = stack_n[p - 1];
+          while (p > base && S.wall(p - 1) < wc) {
+            S.set(p, S.node(p - 1), S.wall(p - 1));
             --p;
           }
-          stack_w[p] = wc;
-          stack_n[p] = ch;
+          S.set(p, ch, wc);
           ++top;
         }
       }
@@ -255,23 +285,49 @@ __device__ __forceinline__ void knn_query(const TreeView &tv, uint64_t qi, const
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
     if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
+  return true;
 }
 
 // views/qview: the trees of this launch and each query's tree (qview null: one tree);
-// qk: per-query k (null: k for all), every k <= KCAP - slack.
+// qk: per-query k (null: k for all).  LDS short stack; a query whose stack would
+// overflow is appended to redo[] for k_knn_deep.
 template <int D, int KCAP>
 __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                               const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                                               const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
                                               uint64_t nq, uint32_t k, double F0, double tol,
                                               uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                              uint32_t *__restrict__ overflow, double *__restrict__ out_F) {
+                                              uint32_t *__restrict__ overflow, double *__restrict__ out_F,
+                                              uint32_t *__restrict__ redo, uint32_t *__restrict__ redo_n) {
+  __shared__ uint32_t s_n[kLdsStack * kQB];
+  __shared__ float s_w[kLdsStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
   const uint32_t kq = qk ? qk[qi] : k;
+  LdsStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, out_ids, out_cnt, overflow, out_F);
+    if (!knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F))
+      redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
+  });
+}
+
+// The queries whose short stack overflowed, with the worst-case private stack.
+template <int D, int KCAP>
+__global__ void __launch_bounds__(kQB) k_knn_deep(const TreeView *__restrict__ views,
+                                                   const uint32_t *__restrict__ qview, const double *__restrict__ qpts,
+                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ qk,
+                                                   const uint32_t *__restrict__ list, uint32_t nlist, uint32_t k,
+                                                   double F0, double tol, uint32_t *__restrict__ out_ids,
+                                                   uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
+                                                   double *__restrict__ out_F) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nlist) return;
+  const uint64_t qi = list[ti];
+  const uint32_t kq = qk ? qk[qi] : k;
+  ArrayStack<D> S;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
   });
 }
 
@@ -349,11 +405,27 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
   return !adjacent;
 }
 
-template <int D, bool BOX, bool FILL>
-__device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
+// Node-only stacks for the range walk (same layouts as the kNN stacks).
+template <int D>
+struct ArrayNodeStack {
+  static constexpr int cap = StackCap<D>::value;
+  uint32_t n[cap];
+  __device__ __forceinline__ uint32_t &at(int i) { return n[i]; }
+};
+
+struct LdsNodeStack {
+  static constexpr int cap = kLdsStack;
+  uint32_t *n;
+  __device__ __forceinline__ uint32_t &at(int i) { return n[i * kQB]; }
+};
+
+// Returns false (the count not written; fill output possibly partial, rewritten
+// by the deep pass at the same offsets) when the stack would overflow.
+template <int D, bool BOX, bool FILL, typename Stack>
+__device__ __forceinline__ bool range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
                                             const uint32_t *__restrict__ qset, int overlap_mode, double tol,
-                                            uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
-                                            uint32_t *__restrict__ out) {
+                                            Stack &stack, uint32_t *__restrict__ counts,
+                                            const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double rmin[D], rmax[D];
@@ -384,11 +456,10 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
       double rsz[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) rsz[d] = rmax[d] - rmin[d];
-      uint32_t stack[StackCap<D>::value];
       int top = 0;
-      stack[top++] = root;
+      stack.at(top++) = root;
       while (top > 0) {
-        const uint32_t node = stack[--top];
+        const uint32_t node = stack.at(--top);
         const double *nm = &tv.nmin[(uint64_t)node * D];
         const double *ns = &tv.nsize[(uint64_t)node * D];
         bool overlap = true, contained = true;
@@ -449,24 +520,50 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
             ++cnt;
           }
         }
-        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+        if (top + (int)ni.ccount > Stack::cap) return false;
+        for (uint32_t c = 0; c < ni.ccount; ++c) stack.at(top++) = ni.cbegin + c;
       }
     }
   }
   if (!FILL) counts[qi] = cnt;
+  return true;
 }
 
+// LDS short stack; a query whose stack would overflow is appended to redo[]
+// (count pass) for k_range_deep, which both passes then run over that list.
 template <int D, bool BOX, bool FILL>
 __global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                                 const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
                                                 const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode,
                                                 double tol, uint32_t *__restrict__ counts,
-                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
+                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
+                                                uint32_t *__restrict__ redo, uint32_t *__restrict__ redo_n) {
+  __shared__ uint32_t s_n[kLdsStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
+  LdsNodeStack S{&s_n[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, counts, offsets, out);
+    if (!range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out) && !FILL)
+      redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
+  });
+}
+
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_range_deep(const TreeView *__restrict__ views,
+                                                     const uint32_t *__restrict__ qview,
+                                                     const double *__restrict__ ranges,
+                                                     const uint32_t *__restrict__ qset,
+                                                     const uint32_t *__restrict__ list, uint32_t nlist,
+                                                     int overlap_mode, double tol, uint32_t *__restrict__ counts,
+                                                     const uint64_t *__restrict__ offsets,
+                                                     uint32_t *__restrict__ out) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nlist) return;
+  const uint64_t qi = list[ti];
+  ArrayNodeStack<D> S;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out);
   });
 }
 
@@ -918,12 +1015,18 @@ template <int D, int KCAP>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
                         const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
-  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1);
+  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 2), redo(ctx, nq);
   Buf<double> Fq;
   if (footprint) Fq = Buf<double>(ctx, nq);
-  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 8, ctx->stream));  // [0] list overflow, [1] redo count
   launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
-         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p);
+         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p, redo.p, ovf.p + 1);
+  uint32_t hredo = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hredo, ovf.p + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (hredo)
+    launch(ctx, "knn_deep", k_knn_deep<D, KCAP>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
+           qk, (const uint32_t *)redo.p, hredo, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p);
   if (footprint) {
     Buf<unsigned long long> acc(ctx, 2);
     OTG_HIP_CHECK(hipMemsetAsync(acc.p, 0, 16, ctx->stream));
@@ -1009,9 +1112,18 @@ template <int D, bool BOX>
 static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, int mode,
                           double tol) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
-  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
+  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1)), redo(ctx, std::max<uint64_t>(nq, 1)), redo_n(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(redo_n.p, 0, 4, ctx->stream));
   launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
+         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, redo.p,
+         redo_n.p);
+  uint32_t hredo = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hredo, redo_n.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (hredo)
+    launch(ctx, "range_count_deep", k_range_deep<D, BOX, false>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views,
+           vs.qview, q, qs, (const uint32_t *)redo.p, hredo, mode, tol, cnt.p, (const uint64_t *)nullptr,
+           (uint32_t *)nullptr);
   Result *r = new Result();
   r->ctx = ctx;
   r->nq = nq;
@@ -1026,7 +1138,12 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
   launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
+         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
+         (uint32_t *)nullptr, (uint32_t *)nullptr);
+  if (hredo)  // the same queries overflow in the fill walk: redo them in full
+    launch(ctx, "range_fill_deep", k_range_deep<D, BOX, true>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views,
+           vs.qview, q, qs, (const uint32_t *)redo.p, hredo, mode, tol, (uint32_t *)nullptr,
+           (const uint64_t *)r->d_offsets, r->d_ids);
   return r;
 }
 

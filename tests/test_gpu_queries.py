@@ -179,3 +179,36 @@ def test_forest_queries_match_per_set_trees(gpu_ctx):
         ooff, oids = o.range(rq[m])
         for j, qi in enumerate(m):
             assert sorted(rids[roff[qi]:roff[qi + 1]].tolist()) == sorted(oids[ooff[j]:ooff[j + 1]].tolist())
+
+
+# ---- short-stack overflow: the deep fallback passes ----------------------------------
+def test_short_stack_overflow_falls_back(gpu_ctx):
+    """Queries whose LDS short stack would overflow are redone with the worst-case
+    private stack; the ids are the oracle's either way, and the deep passes ran."""
+    import octree_amd
+
+    ctx = octree_amd.Context(0)
+    pts = uniform_points(20000, 3, seed=91)
+    g = _tree(ctx, "point", 3, pts, None, 9, 1)
+    o = O.OracleTree(3, "point", pts, None, 9, 1)
+    # far-away kNN with a large k keeps many siblings within the radius
+    qs = np.concatenate([uniform_points(200, 3, seed=92) + np.array([400.0, 0.0, 0.0]), uniform_points(300, 3, seed=93)])
+    # ranges covering most of the space cut through nodes at every level
+    rg = np.array([[1.0, 1.0, 1.0, 99.0, 99.0, 99.0], [0.5, 2.0, 3.0, 97.0, 98.0, 99.5],
+                   [10.0, 10.0, 10.0, 60.0, 60.0, 60.0]])
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    got_k = g.knn_batch(qs, 64).csr()
+    got_r = g.range_batch(rg).csr()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    exp_k = o.knn(qs, 64, threads=8)
+    exp_r = o.range(rg, threads=8)
+    assert np.array_equal(got_k[0], exp_k[0]) and np.array_equal(got_k[1], exp_k[1])
+    assert np.array_equal(got_r[0], exp_r[0])
+    for qi in range(len(rg)):  # range output order is the walk's: compared as sets
+        a, b = got_r[0][qi], got_r[0][qi + 1]
+        assert sorted(got_r[1][a:b].tolist()) == sorted(exp_r[1][a:b].tolist()), qi
+    assert "knn_deep" in kt and "range_count_deep" in kt and "range_fill_deep" in kt, sorted(kt)
+    g.free()
+    ctx.close()
