@@ -288,6 +288,142 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
   return true;
 }
 
+// Compare-exchange on (key, index) pairs held in registers (static indices).
+__device__ __forceinline__ void cas(double &wa, uint32_t &ia, double &wb, uint32_t &ib) {
+  if (wb < wa) {
+    const double tw = wa;
+    wa = wb;
+    wb = tw;
+    const uint32_t ti = ia;
+    ia = ib;
+    ib = ti;
+  }
+}
+
+// Stack of tree levels for D <= 3: one entry per node on the current path whose
+// children are still pending -- the index of its first child and the pending
+// children's offsets in nearest-first order, packed D bits each, with their count
+// in the top 4 bits.  The depth is bounded by the tree depth (<= 19 levels).
+struct LdsLevelStack {
+  static constexpr int cap = kLdsStack;
+  uint32_t *cb;
+  uint32_t *pk;
+};
+
+// kNN walk with level entries: a pop takes the nearest pending child, recomputes
+// its wall distance and, when the current radius excludes it, drops the level
+// (its remaining children are farther).  Same visits and acceptance rule as
+// knn_query; only the order of equally distant siblings may differ.
+template <int D, int KCAP>
+__device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
+                                                 const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
+                                                 LdsLevelStack &S, uint32_t *__restrict__ out_ids,
+                                                 uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
+                                                 double *__restrict__ out_F) {
+  static_assert(D <= 3, "packed child order needs D * 2^D <= 28 bits");
+  constexpr int NC = 1 << D;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  double q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+  double bd[KCAP];
+  uint32_t bid[KCAP];
+  KnnState<KCAP> L;
+  knn_init(L, F0);
+  if (sp.node_cnt != 0 && k != 0) {
+    int top = 0;
+    uint32_t node = (uint32_t)sp.node_off;
+    bool have = true;  // `node` is to be visited
+    {
+      bool in;
+      const double w = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+      have = !knn_node_out(L, w, false);
+    }
+    for (;;) {
+      if (have) {
+        const NodeInfo ni = tv.info[node];
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          double d2 = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+            d2 += v * v;
+          }
+          if (d2 >= L.T) continue;
+          const double dist = sqrt(d2);
+          if (dist >= L.F) continue;
+          knn_offer(bd, bid, L, dist, tv.ids[j], k, tol);
+        }
+        if (ni.ccount) {
+          double w[NC];
+          uint32_t ix[NC];
+          uint32_t cnt = 0;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            ix[c] = (uint32_t)c;
+            w[c] = __builtin_inf();
+            if ((uint32_t)c < ni.ccount) {
+              const uint32_t ch = ni.cbegin + c;
+              bool cin;
+              const double wc =
+                  wall_distance2<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D], &cin);
+              if (!knn_node_out(L, wc, false)) {
+                w[c] = fmin(wc, DBL_MAX);
+                ++cnt;
+              }
+            }
+          }
+          if (NC == 4) {
+            cas(w[0], ix[0], w[1], ix[1]); cas(w[2], ix[2], w[3], ix[3]);
+            cas(w[0], ix[0], w[2], ix[2]); cas(w[1], ix[1], w[3], ix[3]);
+            cas(w[1], ix[1], w[2], ix[2]);
+          } else {  // Batcher odd-even merge sort of 8
+            cas(w[0], ix[0], w[1], ix[1]); cas(w[2], ix[2], w[3], ix[3]);
+            cas(w[4], ix[4], w[5], ix[5]); cas(w[6], ix[6], w[7], ix[7]);
+            cas(w[0], ix[0], w[2], ix[2]); cas(w[1], ix[1], w[3], ix[3]);
+            cas(w[4], ix[4], w[6], ix[6]); cas(w[5], ix[5], w[7], ix[7]);
+            cas(w[1], ix[1], w[2], ix[2]); cas(w[5], ix[5], w[6], ix[6]);
+            cas(w[0], ix[0], w[4], ix[4]); cas(w[1], ix[1], w[5], ix[5]);
+            cas(w[2], ix[2], w[6], ix[6]); cas(w[3], ix[3], w[7], ix[7]);
+            cas(w[2], ix[2], w[4], ix[4]); cas(w[3], ix[3], w[5], ix[5]);
+            cas(w[1], ix[1], w[2], ix[2]); cas(w[3], ix[3], w[4], ix[4]);
+            cas(w[5], ix[5], w[6], ix[6]);
+          }
+          if (cnt) {
+            uint32_t packed = cnt << 28;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if ((uint32_t)c < cnt) packed |= ix[c] << (D * c);
+            if (top == LdsLevelStack::cap) return false;
+            S.cb[top * kQB] = ni.cbegin;
+            S.pk[top * kQB] = packed;
+            ++top;
+          }
+        }
+      }
+      if (top == 0) break;
+      // nearest pending child of the innermost level
+      const uint32_t packed = S.pk[(top - 1) * kQB];
+      const uint32_t cnt = packed >> 28;
+      node = S.cb[(top - 1) * kQB] + (packed & (NC - 1));
+      if (cnt == 1) --top;
+      else S.pk[(top - 1) * kQB] = ((cnt - 1) << 28) | ((packed & 0x0FFFFFFFu) >> D);
+      bool in;
+      const double w = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+      have = !knn_node_out(L, w, false);
+      if (!have && cnt > 1) --top;  // the rest of this level is farther still
+    }
+    if (L.any_dropped && L.dropped <= L.F) atomicOr(overflow, 1u);
+  }
+  out_cnt[qi] = L.cnt;
+  if (out_F) out_F[qi] = L.F;
+#pragma unroll
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
+  return true;
+}
+
 // views/qview: the trees of this launch and each query's tree (qview null: one tree);
 // qk: per-query k (null: k for all).  LDS short stack; a query whose stack would
 // overflow is appended to redo[] for k_knn_deep.
@@ -300,15 +436,21 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
                                               uint32_t *__restrict__ overflow, double *__restrict__ out_F,
                                               uint32_t *__restrict__ redo, uint32_t *__restrict__ redo_n) {
   __shared__ uint32_t s_n[kLdsStack * kQB];
-  __shared__ float s_w[kLdsStack * kQB];
+  __shared__ uint32_t s_w[kLdsStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
   const uint32_t kq = qk ? qk[qi] : k;
-  LdsStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    if (!knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F))
-      redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
+    bool ok;
+    if constexpr (D <= 3) {
+      LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
+      ok = knn_query_levels<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+    } else {
+      LdsStack S{&s_n[threadIdx.x], reinterpret_cast<float *>(&s_w[threadIdx.x])};
+      ok = knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+    }
+    if (!ok) redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
   });
 }
 
@@ -405,26 +547,20 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
   return !adjacent;
 }
 
-// Node-only stacks for the range walk (same layouts as the kNN stacks).
-template <int D>
-struct ArrayNodeStack {
-  static constexpr int cap = StackCap<D>::value;
-  uint32_t n[cap];
-  __device__ __forceinline__ uint32_t &at(int i) { return n[i]; }
+// Level stack in LDS for the range walk: per level with pending children, the
+// next child and the end of the node's children.  A path holds at most one entry
+// per tree depth, so the short stack never overflows.
+struct LdsLevelIter {
+  uint32_t *a, *b;
+  __device__ __forceinline__ uint32_t &nx(int i) { return a[i * kQB]; }
+  __device__ __forceinline__ uint32_t &en(int i) { return b[i * kQB]; }
 };
+static_assert(kLdsStack >= kMaxDepth + 1, "a level stack holds one entry per depth");
 
-struct LdsNodeStack {
-  static constexpr int cap = kLdsStack;
-  uint32_t *n;
-  __device__ __forceinline__ uint32_t &at(int i) { return n[i * kQB]; }
-};
-
-// Returns false (the count not written; fill output possibly partial, rewritten
-// by the deep pass at the same offsets) when the stack would overflow.
-template <int D, bool BOX, bool FILL, typename Stack>
-__device__ __forceinline__ bool range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
+template <int D, bool BOX, bool FILL>
+__device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
                                             const uint32_t *__restrict__ qset, int overlap_mode, double tol,
-                                            Stack &stack, uint32_t *__restrict__ counts,
+                                            LdsLevelIter &stack, uint32_t *__restrict__ counts,
                                             const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
@@ -456,10 +592,11 @@ __device__ __forceinline__ bool range_query(const TreeView &tv, uint64_t qi, con
       double rsz[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) rsz[d] = rmax[d] - rmin[d];
+      // walk: one stack entry per level with pending children (next, end), so the
+      // depth never exceeds the tree's
       int top = 0;
-      stack.at(top++) = root;
-      while (top > 0) {
-        const uint32_t node = stack.at(--top);
+      uint32_t node = root;
+      for (;;) {
         const double *nm = &tv.nmin[(uint64_t)node * D];
         const double *ns = &tv.nsize[(uint64_t)node * D];
         bool overlap = true, contained = true;
@@ -471,97 +608,80 @@ __device__ __forceinline__ bool range_query(const TreeView &tv, uint64_t qi, con
           if (maxL <= nm[d] - tol || maxR <= rmin[d] - tol) overlap = false;
           contained &= range_contains(rmin[d], rmax[d], nm[d], nm[d] + ns[d], tol);
         }
-        if (!overlap) continue;
-        const NodeInfo ni = tv.info[node];
-        if (contained) {
+        const NodeInfo ni = overlap ? tv.info[node] : NodeInfo{0, 0, 0, 0};
+        if (overlap && contained) {
           // whole subtree: its entities are the contiguous storage range
           const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
           if (FILL)
             for (uint32_t j = e0; j < e1; ++j) out[o + cnt + (j - e0)] = tv.ids[j];
           cnt += e1 - e0;
-          continue;
-        }
-        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-          bool ok = true;
-          if (!BOX) {
-            if (tol != 0.0) {
+        } else if (overlap) {
+          for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+            bool ok = true;
+            if (!BOX) {
+              if (tol != 0.0) {
 #pragma unroll
-              for (int d = 0; d < D; ++d) {
-                const double p = tv.geom[(uint64_t)d * tv.n + j];
-                ok &= (rmin[d] - tol < p && p < rmax[d] + tol);
+                for (int d = 0; d < D; ++d) {
+                  const double p = tv.geom[(uint64_t)d * tv.n + j];
+                  ok &= (rmin[d] - tol < p && p < rmax[d] + tol);
+                }
+              } else {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                  const double p = tv.geom[(uint64_t)d * tv.n + j];
+                  ok &= (rmin[d] <= p && p <= rmax[d]);
+                }
               }
             } else {
+              double emn[D], emx[D];
 #pragma unroll
               for (int d = 0; d < D; ++d) {
-                const double p = tv.geom[(uint64_t)d * tv.n + j];
-                ok &= (rmin[d] <= p && p <= rmax[d]);
+                emn[d] = tv.geom[(uint64_t)d * tv.n + j];
+                emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
+              }
+              if (overlap_mode) {
+                ok = boxes_overlap_strict<D>(rmin, rmax, emn, emx, tol);
+              } else {
+                // AreBoxesOverlapped(range, e, true, ..) general.h:296-308
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                  if (rmin[d] > emn[d] || emn[d] > rmax[d]) ok = false;
+                  if (rmin[d] > emx[d] || emx[d] > rmax[d]) ok = false;
+                }
               }
             }
-          } else {
-            double emn[D], emx[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              emn[d] = tv.geom[(uint64_t)d * tv.n + j];
-              emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
-            }
-            if (overlap_mode) {
-              ok = boxes_overlap_strict<D>(rmin, rmax, emn, emx, tol);
-            } else {
-              // AreBoxesOverlapped(range, e, true, ..) general.h:296-308
-#pragma unroll
-              for (int d = 0; d < D; ++d) {
-                if (rmin[d] > emn[d] || emn[d] > rmax[d]) ok = false;
-                if (rmin[d] > emx[d] || emx[d] > rmax[d]) ok = false;
-              }
+            if (ok) {
+              if (FILL) out[o + cnt] = tv.ids[j];
+              ++cnt;
             }
           }
-          if (ok) {
-            if (FILL) out[o + cnt] = tv.ids[j];
-            ++cnt;
+          if (ni.ccount) {
+            stack.nx(top) = ni.cbegin;
+            stack.en(top) = ni.cbegin + ni.ccount;
+            ++top;
           }
         }
-        if (top + (int)ni.ccount > Stack::cap) return false;
-        for (uint32_t c = 0; c < ni.ccount; ++c) stack.at(top++) = ni.cbegin + c;
+        if (top == 0) break;
+        node = stack.nx(top - 1)++;
+        if (stack.nx(top - 1) == stack.en(top - 1)) --top;
       }
     }
   }
   if (!FILL) counts[qi] = cnt;
-  return true;
 }
 
-// LDS short stack; a query whose stack would overflow is appended to redo[]
-// (count pass) for k_range_deep, which both passes then run over that list.
 template <int D, bool BOX, bool FILL>
 __global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                                 const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
                                                 const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode,
                                                 double tol, uint32_t *__restrict__ counts,
-                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
-                                                uint32_t *__restrict__ redo, uint32_t *__restrict__ redo_n) {
-  __shared__ uint32_t s_n[kLdsStack * kQB];
+                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
+  __shared__ uint32_t s_a[kLdsStack * kQB];
+  __shared__ uint32_t s_b[kLdsStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
-  LdsNodeStack S{&s_n[threadIdx.x]};
-  with_tree(views, qview, qi, [&](const TreeView &tv) {
-    if (!range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out) && !FILL)
-      redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
-  });
-}
-
-template <int D, bool BOX, bool FILL>
-__global__ void __launch_bounds__(kQB) k_range_deep(const TreeView *__restrict__ views,
-                                                     const uint32_t *__restrict__ qview,
-                                                     const double *__restrict__ ranges,
-                                                     const uint32_t *__restrict__ qset,
-                                                     const uint32_t *__restrict__ list, uint32_t nlist,
-                                                     int overlap_mode, double tol, uint32_t *__restrict__ counts,
-                                                     const uint64_t *__restrict__ offsets,
-                                                     uint32_t *__restrict__ out) {
-  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (ti >= nlist) return;
-  const uint64_t qi = list[ti];
-  ArrayNodeStack<D> S;
+  LdsLevelIter S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out);
   });
@@ -1112,18 +1232,9 @@ template <int D, bool BOX>
 static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, int mode,
                           double tol) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
-  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1)), redo(ctx, std::max<uint64_t>(nq, 1)), redo_n(ctx, 1);
-  OTG_HIP_CHECK(hipMemsetAsync(redo_n.p, 0, 4, ctx->stream));
+  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
   launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, redo.p,
-         redo_n.p);
-  uint32_t hredo = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&hredo, redo_n.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  if (hredo)
-    launch(ctx, "range_count_deep", k_range_deep<D, BOX, false>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views,
-           vs.qview, q, qs, (const uint32_t *)redo.p, hredo, mode, tol, cnt.p, (const uint64_t *)nullptr,
-           (uint32_t *)nullptr);
+         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
   Result *r = new Result();
   r->ctx = ctx;
   r->nq = nq;
@@ -1138,12 +1249,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
   launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
-         (uint32_t *)nullptr, (uint32_t *)nullptr);
-  if (hredo)  // the same queries overflow in the fill walk: redo them in full
-    launch(ctx, "range_fill_deep", k_range_deep<D, BOX, true>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views,
-           vs.qview, q, qs, (const uint32_t *)redo.p, hredo, mode, tol, (uint32_t *)nullptr,
-           (const uint64_t *)r->d_offsets, r->d_ids);
+         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
   return r;
 }
 

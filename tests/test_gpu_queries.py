@@ -181,10 +181,11 @@ def test_forest_queries_match_per_set_trees(gpu_ctx):
             assert sorted(rids[roff[qi]:roff[qi + 1]].tolist()) == sorted(oids[ooff[j]:ooff[j + 1]].tolist())
 
 
-# ---- short-stack overflow: the deep fallback passes ----------------------------------
-def test_short_stack_overflow_falls_back(gpu_ctx):
-    """Queries whose LDS short stack would overflow are redone with the worst-case
-    private stack; the ids are the oracle's either way, and the deep passes ran."""
+# ---- short stacks: level entries bound the 2-D/3-D walks; 6-D kNN falls back ------
+def test_short_stack_paths(gpu_ctx):
+    """3-D kNN and all range walks keep one stack entry per tree level (never
+    overflow); a 6-D kNN walk whose LDS short stack overflows is redone with the
+    worst-case private stack.  The ids are the oracle's either way."""
     import octree_amd
 
     ctx = octree_amd.Context(0)
@@ -196,12 +197,8 @@ def test_short_stack_overflow_falls_back(gpu_ctx):
     # ranges covering most of the space cut through nodes at every level
     rg = np.array([[1.0, 1.0, 1.0, 99.0, 99.0, 99.0], [0.5, 2.0, 3.0, 97.0, 98.0, 99.5],
                    [10.0, 10.0, 10.0, 60.0, 60.0, 60.0]])
-    ctx.set_profiling(True)
-    ctx.reset_kernel_times()
     got_k = g.knn_batch(qs, 64).csr()
     got_r = g.range_batch(rg).csr()
-    kt = ctx.kernel_times()
-    ctx.set_profiling(False)
     exp_k = o.knn(qs, 64, threads=8)
     exp_r = o.range(rg, threads=8)
     assert np.array_equal(got_k[0], exp_k[0]) and np.array_equal(got_k[1], exp_k[1])
@@ -209,6 +206,19 @@ def test_short_stack_overflow_falls_back(gpu_ctx):
     for qi in range(len(rg)):  # range output order is the walk's: compared as sets
         a, b = got_r[0][qi], got_r[0][qi + 1]
         assert sorted(got_r[1][a:b].tolist()) == sorted(exp_r[1][a:b].tolist()), qi
-    assert "knn_deep" in kt and "range_count_deep" in kt and "range_fill_deep" in kt, sorted(kt)
     g.free()
+    # 6-D: up to 63 pending siblings per level overflow the short stack
+    p6 = uniform_points(30000, 6, seed=94)
+    g6 = _tree(ctx, "point", 6, p6, None, 4, 2)
+    o6 = O.OracleTree(6, "point", p6, None, 4, 2)
+    q6 = np.concatenate([uniform_points(100, 6, seed=95) + 300.0, uniform_points(200, 6, seed=96)])
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    got6 = g6.knn_batch(q6, 32).csr()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    exp6 = o6.knn(q6, 32, threads=8)
+    assert np.array_equal(got6[0], exp6[0]) and np.array_equal(got6[1], exp6[1])
+    assert "knn_deep" in kt, sorted(kt)
+    g6.free()
     ctx.close()
