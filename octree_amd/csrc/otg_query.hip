@@ -550,12 +550,14 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
 // Level stack in LDS for the range walk: per level with pending children, the
 // next child and the end of the node's children.  A path holds at most one entry
 // per tree depth, so the short stack never overflows.
+constexpr int kRangeStack = kMaxDepth + 1;  // 20 x 5 B x 128 lanes = 12.5 KiB per block
+
 struct LdsLevelIter {
-  uint32_t *a, *b;
+  uint32_t *a;  // next pending child
+  uint8_t *b;   // children left at this level (<= 2^D)
   __device__ __forceinline__ uint32_t &nx(int i) { return a[i * kQB]; }
-  __device__ __forceinline__ uint32_t &en(int i) { return b[i * kQB]; }
+  __device__ __forceinline__ uint8_t &left(int i) { return b[i * kQB]; }
 };
-static_assert(kLdsStack >= kMaxDepth + 1, "a level stack holds one entry per depth");
 
 template <int D, bool BOX, bool FILL>
 __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
@@ -657,13 +659,13 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
           }
           if (ni.ccount) {
             stack.nx(top) = ni.cbegin;
-            stack.en(top) = ni.cbegin + ni.ccount;
+            stack.left(top) = (uint8_t)ni.ccount;
             ++top;
           }
         }
         if (top == 0) break;
         node = stack.nx(top - 1)++;
-        if (stack.nx(top - 1) == stack.en(top - 1)) --top;
+        if (--stack.left(top - 1) == 0) --top;
       }
     }
   }
@@ -676,8 +678,8 @@ __global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ view
                                                 const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode,
                                                 double tol, uint32_t *__restrict__ counts,
                                                 const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
-  __shared__ uint32_t s_a[kLdsStack * kQB];
-  __shared__ uint32_t s_b[kLdsStack * kQB];
+  __shared__ uint32_t s_a[kRangeStack * kQB];
+  __shared__ uint8_t s_b[kRangeStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
