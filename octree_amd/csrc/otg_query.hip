@@ -553,7 +553,7 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
 constexpr int kRangeStack = kMaxDepth + 1;  // 20 x 5 B x 128 lanes = 12.5 KiB per block
 
 struct LdsLevelIter {
-  uint32_t *a;  // next pending child
+  uint32_t *a;  // next pending child (children are taken last to first)
   uint8_t *b;   // children left at this level (<= 2^D)
   __device__ __forceinline__ uint32_t &nx(int i) { return a[i * kQB]; }
   __device__ __forceinline__ uint8_t &left(int i) { return b[i * kQB]; }
@@ -657,14 +657,14 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
               ++cnt;
             }
           }
-          if (ni.ccount) {
-            stack.nx(top) = ni.cbegin;
+          if (ni.ccount) {  // last child first, the reference's depth-first output order
+            stack.nx(top) = ni.cbegin + ni.ccount - 1;
             stack.left(top) = (uint8_t)ni.ccount;
             ++top;
           }
         }
         if (top == 0) break;
-        node = stack.nx(top - 1)++;
+        node = stack.nx(top - 1)--;
         if (--stack.left(top - 1) == 0) --top;
       }
     }
