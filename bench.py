@@ -121,12 +121,14 @@ def main():
     import octree_amd
 
     if args.calibrate:
+        # known-size kernels for the PMC byte counters: the fill writes 1 GiB, each
+        # add reads 1 GiB and writes 1 GiB
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            torch.add(a, 1.0, out=b)
         torch.cuda.synchronize()
-        log("calibration: 3 copies of 1 GiB (elementwise copy kernel)")
+        log("calibration: fill of 1 GiB, then 3 elementwise adds reading and writing 1 GiB each")
         del a, b
     if args.workload == "served":
         print(json.dumps(run_served(args, octree_amd)), flush=True)

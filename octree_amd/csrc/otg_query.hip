@@ -288,6 +288,51 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
   return true;
 }
 
+// Distance tests of entities [j, je) against the candidate list.  Coordinates of
+// four entities are loaded before any is tested, so a lane keeps several memory
+// requests in flight; the acceptance order is the entity order, as in a plain loop.
+template <int D, int KCAP>
+__device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D], uint32_t j, uint32_t je,
+                                         double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &L, uint32_t k,
+                                         double tol) {
+  constexpr int U = 4;
+  for (; j + U <= je; j += U) {
+    double d2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) d2[u] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double c[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) c[u] = tv.geom[(uint64_t)d * tv.n + j + u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double v = q[d] - c[u];
+        d2[u] += v * v;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (d2[u] >= L.T) continue;
+      const double dist = sqrt(d2[u]);
+      if (dist >= L.F) continue;
+      knn_offer(bd, bid, L, dist, tv.ids[j + u], k, tol);
+    }
+  }
+  for (; j < je; ++j) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+      d2 += v * v;
+    }
+    if (d2 >= L.T) continue;
+    const double dist = sqrt(d2);
+    if (dist >= L.F) continue;
+    knn_offer(bd, bid, L, dist, tv.ids[j], k, tol);
+  }
+}
+
 // Compare-exchange on (key, index) pairs held in registers (static indices).
 __device__ __forceinline__ void cas(double &wa, uint32_t &ia, double &wb, uint32_t &ib) {
   if (wb < wa) {
@@ -343,18 +388,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
     for (;;) {
       if (have) {
         const NodeInfo ni = tv.info[node];
-        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-          double d2 = 0.0;
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
-            d2 += v * v;
-          }
-          if (d2 >= L.T) continue;
-          const double dist = sqrt(d2);
-          if (dist >= L.F) continue;
-          knn_offer(bd, bid, L, dist, tv.ids[j], k, tol);
-        }
+        knn_scan<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, L, k, tol);
         if (ni.ccount) {
           double w[NC];
           uint32_t ix[NC];
