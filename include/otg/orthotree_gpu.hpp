@@ -174,13 +174,11 @@ class StaticOrthoTreeGPU {
   // ---- queries ------------------------------------------------------------------
   std::vector<EntityID> GetNearestNeighbors(TVector const &searchPoint, std::size_t neighborNo, EntityContainerView entities,
                                             double tolerance = BASE_TOLERANCE) const noexcept
-    requires(!IS_BOX)
   {
     return GetNearestNeighbors(searchPoint, neighborNo, std::numeric_limits<double>::max(), entities, tolerance);
   }
   std::vector<EntityID> GetNearestNeighbors(TVector const &searchPoint, std::size_t neighborNo, double maxDistanceWithin,
                                             EntityContainerView, double tolerance = BASE_TOLERANCE) const noexcept
-    requires(!IS_BOX)
   {
     auto r = GetNearestNeighbors(std::span<const TVector>(&searchPoint, 1), neighborNo, maxDistanceWithin, tolerance);
     return std::vector<EntityID>(r.ids.begin(), r.ids.end());
@@ -189,7 +187,6 @@ class StaticOrthoTreeGPU {
   BatchResult GetNearestNeighbors(std::span<const TVector> searchPoints, std::size_t neighborNo,
                                   double maxDistanceWithin = std::numeric_limits<double>::max(),
                                   double tolerance = BASE_TOLERANCE) const noexcept
-    requires(!IS_BOX)
   {
     otg_result *res = nullptr;
     if (!tree_) return empty(searchPoints.size());

@@ -52,6 +52,18 @@ __device__ __forceinline__ uint32_t set_of(const uint64_t *set_off, uint32_t nse
   return lo;
 }
 
+// set_of for consecutive indices of a wavefront: the first active lane's set is
+// found by a wave-uniform search (scalar loads), the others step over the few set
+// boundaries inside the wavefront (indices of later lanes are larger).
+__device__ __forceinline__ uint32_t set_of_wave(const uint64_t *set_off, uint32_t nsets, uint64_t i) {
+  if (nsets == 1) return 0;
+  const uint32_t lo32 = __builtin_amdgcn_readfirstlane((uint32_t)i);
+  const uint32_t hi32 = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 32));
+  uint32_t s = set_of(set_off, nsets, ((uint64_t)hi32 << 32) | lo32);
+  while (s + 1 < nsets && set_off[s + 1] <= i) ++s;
+  return s;
+}
+
 // common leading Morton digits of two codes of a depth-M tree (LCA depth)
 template <int D>
 __device__ __forceinline__ uint32_t lcp_digits(uint64_t a, uint64_t b, uint32_t M) {
@@ -132,7 +144,7 @@ __global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, 
                                                 uint32_t *__restrict__ err) {
   uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;
-  const uint32_t s = set_of(set_off, nsets, i);
+  const uint32_t s = set_of_wave(set_off, nsets, i);
   const SetParams &sp = sets[s];
   const uint32_t M = sp.max_depth;
   uint32_t g[D];

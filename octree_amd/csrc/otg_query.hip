@@ -134,9 +134,11 @@ __device__ __forceinline__ bool knn_node_out(const KnnState<KCAP> &S, double w2,
   return inside ? (0.0 >= S.F) : (w2 >= S.T || sqrt(w2) >= S.F);
 }
 
+// Inserts (dist, id) into the optimistic list (ascending; ties by id), keeping
+// the KCAP smallest and remembering the smallest dropped distance.
 template <int KCAP>
-__device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &S, double dist,
-                                          uint32_t id, uint32_t k, double tol) {
+__device__ __forceinline__ void knn_insert(double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &S, double dist,
+                                           uint32_t id) {
   uint32_t pos = 0;
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
@@ -159,25 +161,115 @@ __device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KC
       bid[t] = id;
     }
   if (S.cnt < KCAP) ++S.cnt;
+}
+
+// F from the pessimistic k-th distance dk (:1120-1157): assigned at the first
+// heapify, afterwards only a smaller limit replaces it and optimistic entries
+// above it leave.
+template <int KCAP>
+__device__ __forceinline__ void knn_limit(const double (&bd)[KCAP], KnnState<KCAP> &S, double dk, double tol) {
+  const double lim = tol_upper(dk, tol);
+  if (!S.heapified) {
+    S.heapified = true;
+    S.F = lim;  // the make_heap branch assigns unconditionally (:1124-1129)
+    S.T = sq_reject_bound(S.F);
+  } else if (lim < S.F) {
+    S.F = lim;
+    S.T = sq_reject_bound(S.F);
+    uint32_t keep = 0;
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t < S.cnt && !(bd[t] > S.F)) ++keep;
+    S.cnt = keep;
+  }
+}
+
+// Points: the optimistic and pessimistic distances coincide, so the pessimistic
+// k-th is the optimistic list's k-th entry.
+template <int KCAP>
+__device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &S, double dist,
+                                          uint32_t id, uint32_t k, double tol) {
+  knn_insert(bd, bid, S, dist, id);
   if (S.cnt >= k) {
     double dk = 0.0;
 #pragma unroll
     for (int t = 0; t < KCAP; ++t)
       if ((uint32_t)t == k - 1) dk = bd[t];
-    const double lim = tol_upper(dk, tol);
-    if (!S.heapified) {
-      S.heapified = true;
-      S.F = lim;  // the make_heap branch assigns unconditionally (:1124-1129)
-      S.T = sq_reject_bound(S.F);
-    } else if (lim < S.F) {
-      S.F = lim;
-      S.T = sq_reject_bound(S.F);
-      uint32_t keep = 0;
+    knn_limit(bd, S, dk, tol);
+  }
+}
+
+// Boxes: optimistic distance into the list, pessimistic distance into the k
+// smallest pessimistic values (push_heap + pop_heap of :1131-1134), F from their
+// largest.
+template <int KCAP>
+__device__ __forceinline__ void knn_offer_box(double (&bd)[KCAP], uint32_t (&bid)[KCAP], double (&pes)[KCAP],
+                                              uint32_t &npes, KnnState<KCAP> &S, double dmin, double dminmax,
+                                              uint32_t id, uint32_t k, double tol) {
+  knn_insert(bd, bid, S, dmin, id);
+  uint32_t pos = 0;
 #pragma unroll
-      for (int t = 0; t < KCAP; ++t)
-        if ((uint32_t)t < S.cnt && !(bd[t] > S.F)) ++keep;
-      S.cnt = keep;
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t < npes && pes[t] <= dminmax) ++pos;
+  if (pos < k) {
+#pragma unroll
+    for (int t = KCAP - 1; t >= 1; --t)
+      if ((uint32_t)t > pos && (uint32_t)t <= npes && (uint32_t)t < k) pes[t] = pes[t - 1];
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t == pos) pes[t] = dminmax;
+    if (npes < k) ++npes;
+  }
+  if (npes >= k) {
+    double dk = 0.0;
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t == k - 1) dk = pes[t];
+    knn_limit(bd, S, dk, tol);
+  }
+}
+
+// MinMaxDistance2 (adapters/general.h:159-210) in its operation order: squared
+// optimistic (0 inside the box within tol per axis) and pessimistic distances.
+template <int D>
+__device__ __forceinline__ void box_minmax2(const double (&q)[D], const TreeView &tv, uint32_t j, double tol,
+                                            double *min2_out, double *mm2_out) {
+  double min2 = 0.0, mm2 = 0.0, far_in = DBL_MAX, largest = 0.0;
+  bool inside = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double minDist = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+    const double maxDist = tv.geom[(uint64_t)(D + d) * tv.n + j] - q[d];
+    const bool in_c = (-tol <= minDist && -tol <= maxDist);
+    inside &= in_c;
+    double a = minDist * minDist, b = maxDist * maxDist;
+    if (b < a) {
+      const double t = a;
+      a = b;
+      b = t;
     }
+    if (inside) far_in = (b < far_in) ? b : far_in;
+    if (!in_c) min2 += a;
+    largest = (largest < b - a) ? b - a : largest;
+    mm2 += b;
+  }
+  *min2_out = inside ? 0.0 : min2;
+  *mm2_out = inside ? far_in : mm2 - largest;
+}
+
+// Box entities [j, je): MinMaxDistance per entity (ot_query.h:1079-1082), an
+// entity enters when its optimistic distance is below F.
+template <int D, int KCAP>
+__device__ __forceinline__ void knn_scan_box(const TreeView &tv, const double (&q)[D], uint32_t j, uint32_t je,
+                                             double (&bd)[KCAP], uint32_t (&bid)[KCAP], double (&pes)[KCAP],
+                                             uint32_t &npes, KnnState<KCAP> &L, uint32_t k, double tol) {
+  for (; j < je; ++j) {
+    double min2, mm2;
+    box_minmax2<D>(q, tv, j, tol, &min2, &mm2);
+    if (min2 >= L.T) continue;
+    const double dmin = sqrt(min2);
+    if (dmin >= L.F) continue;
+    knn_offer_box(bd, bid, pes, npes, L, dmin, sqrt(mm2), tv.ids[j], k, tol);
   }
 }
 
@@ -219,7 +311,7 @@ struct LdsStack {
 // entries carry the child's squared wall distance computed when it was pushed,
 // so a pop re-checks it against the current F without reloading the node box.
 // Returns false (writing nothing) when the stack would overflow.
-template <int D, int KCAP, typename Stack>
+template <int D, int KCAP, bool BOX, typename Stack>
 __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                           const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
                                           Stack &S, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
@@ -233,6 +325,8 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
   uint32_t bid[KCAP];
   KnnState<KCAP> L;
   knn_init(L, F0);
+  double pes[KCAP];
+  uint32_t npes = 0;
   if (sp.node_cnt != 0 && k != 0) {
     int top = 0;
     {
@@ -247,18 +341,10 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
       // wall2 is 0 inside the node box, and (0 >= F) is the reference's test there too
       if (knn_node_out(L, S.wall(top), false)) continue;
       const NodeInfo ni = tv.info[node];
-      for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-        double d2 = 0.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
-          d2 += v * v;
-        }
-        if (d2 >= L.T) continue;
-        const double dist = sqrt(d2);
-        if (dist >= L.F) continue;
-        knn_offer(bd, bid, L, dist, tv.ids[j], k, tol);
-      }
+      if constexpr (BOX)
+        knn_scan_box<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, pes, npes, L, k, tol);
+      else
+        knn_scan<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, L, k, tol);
       if (ni.ccount) {
         // push children still within F, farthest first so the nearest pops first
         const int base = top;
@@ -359,7 +445,7 @@ struct LdsLevelStack {
 // its wall distance and, when the current radius excludes it, drops the level
 // (its remaining children are farther).  Same visits and acceptance rule as
 // knn_query; only the order of equally distant siblings may differ.
-template <int D, int KCAP>
+template <int D, int KCAP, bool BOX>
 __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                                  const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
                                                  LdsLevelStack &S, uint32_t *__restrict__ out_ids,
@@ -376,9 +462,13 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
   uint32_t bid[KCAP];
   KnnState<KCAP> L;
   knn_init(L, F0);
+  double pes[KCAP];
+  uint32_t npes = 0;
   if (sp.node_cnt != 0 && k != 0) {
     int top = 0;
     uint32_t node = (uint32_t)sp.node_off;
+    // the node record is loaded together with the box that decides the visit
+    NodeInfo ni = tv.info[node];
     bool have = true;  // `node` is to be visited
     {
       bool in;
@@ -387,8 +477,10 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
     }
     for (;;) {
       if (have) {
-        const NodeInfo ni = tv.info[node];
-        knn_scan<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, L, k, tol);
+        if constexpr (BOX)
+          knn_scan_box<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, pes, npes, L, k, tol);
+        else
+          knn_scan<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, L, k, tol);
         if (ni.ccount) {
           double w[NC];
           uint32_t ix[NC];
@@ -441,6 +533,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
       const uint32_t packed = S.pk[(top - 1) * kQB];
       const uint32_t cnt = packed >> 28;
       node = S.cb[(top - 1) * kQB] + (packed & (NC - 1));
+      ni = tv.info[node];
       if (cnt == 1) --top;
       else S.pk[(top - 1) * kQB] = ((cnt - 1) << 28) | ((packed & 0x0FFFFFFFu) >> D);
       bool in;
@@ -461,7 +554,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
 // views/qview: the trees of this launch and each query's tree (qview null: one tree);
 // qk: per-query k (null: k for all).  LDS short stack; a query whose stack would
 // overflow is appended to redo[] for k_knn_deep.
-template <int D, int KCAP>
+template <int D, int KCAP, bool BOX>
 __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                               const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                                               const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
@@ -479,17 +572,17 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
     bool ok;
     if constexpr (D <= 3) {
       LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
-      ok = knn_query_levels<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+      ok = knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
     } else {
       LdsStack S{&s_n[threadIdx.x], reinterpret_cast<float *>(&s_w[threadIdx.x])};
-      ok = knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+      ok = knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
     }
     if (!ok) redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
   });
 }
 
 // The queries whose short stack overflowed, with the worst-case private stack.
-template <int D, int KCAP>
+template <int D, int KCAP, bool BOX>
 __global__ void __launch_bounds__(kQB) k_knn_deep(const TreeView *__restrict__ views,
                                                    const uint32_t *__restrict__ qview, const double *__restrict__ qpts,
                                                    const uint32_t *__restrict__ qset, const uint32_t *__restrict__ qk,
@@ -503,7 +596,7 @@ __global__ void __launch_bounds__(kQB) k_knn_deep(const TreeView *__restrict__ v
   const uint32_t kq = qk ? qk[qi] : k;
   ArrayStack<D> S;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    knn_query<D, KCAP>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+    knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
   });
 }
 
@@ -635,6 +728,7 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
       for (;;) {
         const double *nm = &tv.nmin[(uint64_t)node * D];
         const double *ns = &tv.nsize[(uint64_t)node * D];
+        const NodeInfo ni = tv.info[node];  // loaded with the box that decides the visit
         bool overlap = true, contained = true;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -644,7 +738,6 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
           if (maxL <= nm[d] - tol || maxR <= rmin[d] - tol) overlap = false;
           contained &= range_contains(rmin[d], rmax[d], nm[d], nm[d] + ns[d], tol);
         }
-        const NodeInfo ni = overlap ? tv.info[node] : NodeInfo{0, 0, 0, 0};
         if (overlap && contained) {
           // whole subtree: its entities are the contiguous storage range
           const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
@@ -1167,7 +1260,7 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
 }
 
 // footprint (optional, measurement): out[0..1] += node and entity visits of the minimal walk
-template <int D, int KCAP>
+template <int D, int KCAP, bool BOX>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
                         const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
@@ -1175,13 +1268,13 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   Buf<double> Fq;
   if (footprint) Fq = Buf<double>(ctx, nq);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 8, ctx->stream));  // [0] list overflow, [1] redo count
-  launch(ctx, "knn", k_knn<D, KCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
+  launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
          (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p, redo.p, ovf.p + 1);
   uint32_t hredo = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hredo, ovf.p + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   if (hredo)
-    launch(ctx, "knn_deep", k_knn_deep<D, KCAP>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
+    launch(ctx, "knn_deep", k_knn_deep<D, KCAP, BOX>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
            qk, (const uint32_t *)redo.p, hredo, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p);
   if (footprint) {
     Buf<unsigned long long> acc(ctx, 2);
@@ -1205,15 +1298,21 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
 }
 
 // kmax = the largest k of the launch (per-query k in qk, or k for all)
-static Result *knn_dispatch(Ctx *ctx, int dim, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
-                            const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol,
+static Result *knn_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
+                            const uint32_t *qs, const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol,
                             uint64_t *footprint = nullptr) {
-#define OTG_KNN(DD)                                                                               \
-  case DD:                                                                                        \
-    if (kmax <= 8) return knn_impl<DD, 16>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);          \
-    if (kmax <= 16) return knn_impl<DD, 20>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);         \
-    if (kmax <= 32) return knn_impl<DD, 36>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);         \
-    return knn_impl<DD, 72>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);
+#define OTG_KNN_K(DD, BB)                                                                              \
+  if (kmax <= 8) return knn_impl<DD, 16, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);              \
+  if (kmax <= 16) return knn_impl<DD, 20, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
+  if (kmax <= 32) return knn_impl<DD, 36, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
+  return knn_impl<DD, 72, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);
+#define OTG_KNN(DD)          \
+  case DD:                   \
+    if (box) {               \
+      OTG_KNN_K(DD, true)    \
+    } else {                 \
+      OTG_KNN_K(DD, false)   \
+    }
   switch (dim) {
     OTG_KNN(2)
     OTG_KNN(3)
@@ -1222,6 +1321,7 @@ static Result *knn_dispatch(Ctx *ctx, int dim, const ViewSet &vs, uint64_t nq, c
     OTG_KNN(6)
   }
 #undef OTG_KNN
+#undef OTG_KNN_K
   throw Error{OTG_ERR_INVALID_ARG, "dimension"};
 }
 
@@ -1253,15 +1353,14 @@ static double knn_F0(double max_within, double tol) {
 
 Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
                   double max_within, double tol, uint64_t *footprint) {
-  if (t->kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
   if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
   Ctx *ctx = run_ctx(t);
   t = knn_index(t);
   DevInput in;
   stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
   if (nq == 0 || t->n == 0 || k == 0) return empty_result(ctx, nq);
-  return knn_dispatch(ctx, t->dim, single_view(t), nq, in.q, in.sets, nullptr, k, k, knn_F0(max_within, tol), tol,
-                      footprint);
+  return knn_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, nullptr, k, k,
+                      knn_F0(max_within, tol), tol, footprint);
 }
 
 template <int D, bool BOX>
@@ -1377,7 +1476,6 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
     vs.sbits = std::max(vs.sbits, x->layout.sbits);
     total_n += x->n;
   }
-  if (query == 0 && kind != OTG_KIND_POINT) throw Error{OTG_ERR_UNSUPPORTED, "kNN on box trees is not built yet"};
   uint32_t kmax = 0;
   if (query == 0)
     for (uint64_t i = 0; i < nq; ++i) kmax = std::max(kmax, qk[i]);
@@ -1398,7 +1496,8 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
   vs.qview = dqv.p;
   Result *r;
   if (nq == 0 || total_n == 0 || (query == 0 && kmax == 0)) r = empty_result(ctx, nq);
-  else if (query == 0) r = knn_dispatch(ctx, dim, vs, nq, dq.p, nullptr, dqk.p, 0, kmax, knn_F0(p0, p1), p1);
+  else if (query == 0)
+    r = knn_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, dqk.p, 0, kmax, knn_F0(p0, p1), p1);
   else if (query == 1) r = range_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, mode, p0);
   else r = ray_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, p0, p1, p2);
   // the staged host arrays must outlive the copies: wait before returning

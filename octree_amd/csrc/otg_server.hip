@@ -247,11 +247,6 @@ class Server {
           continue;
         }
         const ServedTree &e = it->second;
-        if (r->query == 0 && e.kind != OTG_KIND_POINT) {
-          r->status = OTG_ERR_UNSUPPORTED;
-          r->error = "kNN on box trees is not built yet";
-          continue;
-        }
         if (r->query == 0 && r->k > 64) {
           r->status = OTG_ERR_UNSUPPORTED;
           r->error = "k > 64";

@@ -723,10 +723,47 @@ static int ed_cmp_asc(const void *a, const void *b) {
 /* ot_query.h:1033-1036 GetValueWithToleranceUpper */
 static inline double tol_upper(double v, double tol) { return fmax(tol, v * (1.0 + tol)); }
 
-/* ot_query.h:1208-1247 GetNearestNeighbors (points) with AddEntityDistance
- * :1058-1164 (optimistic/pessimistic heaps, eviction of entries > F) and the
- * priority traversal :235-291 on GetBoxWallDistanceAD(.., true).  Writes at
- * most out_cap ids; returns the full result size. */
+/* adapters/general.h:159-210 MinMaxDistance2 + :212-218 MinMaxDistance
+ * (Roussopoulos MINMAXDIST): optimistic distance (0 inside the box, within
+ * tol per axis) and pessimistic distance of a point to a box, same op order. */
+static void oc_minmax_distance(const double *pt, const double *box, int D, double tol, double *dmin,
+                               double *dminmax) {
+  double min2 = 0.0, minmax2 = 0.0;
+  double farthest_inside2 = DBL_MAX, largest_diff = 0.0;
+  int inside = 1;
+  for (int d = 0; d < D; ++d) {
+    const double v = pt[d], minC = box[d], maxC = box[D + d];
+    const double minDist = v - minC, maxDist = maxC - v;
+    const int in_c = (-tol <= minDist && -tol <= maxDist);
+    inside &= in_c;
+    double a = minDist * minDist, b = maxDist * maxDist;
+    if (b < a) {
+      double tmp = a;
+      a = b;
+      b = tmp;
+    }
+    if (inside) farthest_inside2 = (b < farthest_inside2) ? b : farthest_inside2; /* std::min */
+    if (!in_c) min2 += a;
+    largest_diff = (largest_diff < b - a) ? b - a : largest_diff; /* std::max */
+    minmax2 += b;
+  }
+  if (inside) {
+    min2 = 0.0;
+    minmax2 = farthest_inside2;
+  } else {
+    minmax2 -= largest_diff;
+  }
+  *dmin = sqrt(min2);
+  *dminmax = sqrt(minmax2);
+}
+
+/* ot_query.h:1208-1247 GetNearestNeighbors with AddEntityDistance :1058-1164
+ * (optimistic heap by the optimistic distance, pessimistic heap of the k
+ * smallest pessimistic distances, F from the pessimistic k-th, eviction of
+ * optimistic entries > F together with their pessimistic entries) and the
+ * priority traversal :235-291 on GetBoxWallDistanceAD(.., true).  Points use
+ * Distance for both; boxes MinMaxDistance (:1079-1082).  Writes at most out_cap
+ * ids; returns the full result size. */
 static size_t oc_knn_one(const oc_tree *t, const double *pts, const double *q, uint32_t k, double max_within, double tol,
                          uint32_t *out, size_t out_cap, oc_pn **pqbuf, size_t *pqcap, oc_ed **ebuf, size_t *ecap) {
   int D = t->dim;
@@ -750,28 +787,43 @@ static size_t oc_knn_one(const oc_tree *t, const double *pts, const double *q, u
     uint32_t node = top.node;
     for (uint32_t j = t->node_ebegin[node]; j < t->node_ebegin[node] + t->node_ecount[node]; ++j) {
       uint32_t id = t->entity_ids[j];
-      double dist = oc_distance(q, &pts[(size_t)id * D], D);
-      if (dist >= F) continue;
-      int should_heapify = (nopt == (size_t)k - 1);
-      oc_ed e = {id, dist};
+      double dmin, dminmax;
+      if (t->is_box) {
+        oc_minmax_distance(q, &pts[(size_t)id * 2 * D], D, tol, &dmin, &dminmax);
+      } else {
+        dmin = dminmax = oc_distance(q, &pts[(size_t)id * D], D);
+      }
+      if (dmin >= F) continue;
+      oc_ed e = {id, dmin}, ep = {id, dminmax};
       if (!heapified) {
         opt[nopt++] = e;
-        pes[npes++] = e;
+        pes[npes++] = ep;
         if (npes < k) continue;
-        /* should_heapify holds here: make_heap both, F from the k-th */
+        /* the optimistic size was k-1 before this push: make_heap both, F from the k-th */
         for (size_t i = 0; i < nopt; ++i) heap_sift_up(opt, i);
         for (size_t i = 0; i < npes; ++i) heap_sift_up(pes, i);
         heapified = 1;
-        (void)should_heapify;
         F = tol_upper(pes[0].dist, tol);
       } else {
         heap_push(opt, &nopt, e);
-        heap_push(pes, &npes, e);
+        heap_push(pes, &npes, ep);
         heap_pop(pes, &npes);
         double lim = tol_upper(pes[0].dist, tol);
         if (lim < F) {
           F = lim;
-          while (nopt && opt[0].dist > F) heap_pop(opt, &nopt);
+          int pes_removed = 0;
+          while (nopt && opt[0].dist > F) {
+            uint32_t rid = opt[0].id;
+            for (size_t i = 0; i < npes; ++i)
+              if (pes[i].id == rid) {
+                pes[i] = pes[--npes];
+                pes_removed = 1;
+                break;
+              }
+            heap_pop(opt, &nopt);
+          }
+          if (pes_removed)
+            for (size_t i = 0; i < npes; ++i) heap_sift_up(pes, i);
         }
       }
     }

@@ -99,6 +99,16 @@ fixtures = {
         {"boxes": [[0.0, 0.0, 3.0, 1.0], [1.0, 0.0, 2.0, 1.0], [10.0, 0.0, 11.0, 1.0]], "depth": 3, "cap": 2, "o": [-1.0, 0.5],
          "d": [1.0, 0.0], "tol": 0.0, "inc": 0.0, "max_dist": 5.0, "expect": [1, 0], "src": "general.tests.cpp:1500"},
     ],
+    "knn_box_2d": [  # box-tree kNN (MinMaxDistance) general.tests.cpp; expected sets (order not asserted there)
+        {"boxes": [[0.20, 0.00, 2.31, 2.31], [1.00, 0.00, 1.01, 1.41], [0.40, 0.00, 0.89, 0.89], [2.10, 0.00, 3.66, 3.66],
+                   [-2.21, 0.00, -0.20, 2.21], [-2.50, 0.00, -0.90, 2.47]],
+         "depth": 3, "cap": 2, "q": [0.0, 0.0], "k": 1, "tol": 2.220446049250313e-16, "expect": [0, 2, 4],
+         "src": "general.tests.cpp:2648"},
+        {"boxes": [[0.0, 0.0, 1.0, 1.0], [1.0, 1.0, 3.0, 3.0], [4.0, 4.0, 5.0, 5.0]], "depth": 3, "cap": 2,
+         "q": [2.0, 2.0], "k": 1, "expect": [1], "src": "general.tests.cpp:2772"},
+        {"boxes": [[0.0, 0.0, 1.0, 1.0], [1.0, 1.0, 3.0, 3.0], [4.0, 4.0, 5.0, 5.0]], "depth": 3, "cap": 2,
+         "q": [-1.0, 1.0], "k": 1, "expect": [0], "src": "general.tests.cpp:2788"},
+    ],
     "issue38_collision": {  # general.tests.cpp:4728-4772: 10 rounds x 1000 loose 3D boxes, mt19937(0); vs brute force
         "rounds": 10, "box_no": 1000, "space_max": 8.0, "size_max": 2.0, "depth": 10, "cap": 5,
         "src": "general.tests.cpp:4728",

@@ -139,6 +139,57 @@ def knn_brute(q, pts, k, max_within=DBL_MAX, tol=BASE_TOL):
     return order.tolist(), amb
 
 
+def box_minmax(q, boxes, tol):
+    """Optimistic and pessimistic point-box distances in the operation order of
+    MinMaxDistance2 (adapters/general.h:159-210), vectorised over the boxes."""
+    D = len(q)
+    q = np.asarray(q, np.float64)
+    n = boxes.shape[0]
+    min2 = np.zeros(n)
+    mm2 = np.zeros(n)
+    far_in = np.full(n, DBL_MAX)
+    largest = np.zeros(n)
+    inside = np.ones(n, bool)
+    for d in range(D):
+        min_dist = q[d] - boxes[:, d]
+        max_dist = boxes[:, D + d] - q[d]
+        in_c = (-tol <= min_dist) & (-tol <= max_dist)
+        inside &= in_c
+        a, b = min_dist * min_dist, max_dist * max_dist
+        lo, hi = np.where(b < a, b, a), np.where(b < a, a, b)
+        far_in = np.where(inside & (hi < far_in), hi, far_in)
+        min2 = np.where(in_c, min2, min2 + lo)
+        largest = np.where(largest < hi - lo, hi - lo, largest)
+        mm2 = mm2 + hi
+    min2 = np.where(inside, 0.0, min2)
+    mm2 = np.where(inside, far_in, mm2 - largest)
+    return np.sqrt(min2), np.sqrt(mm2)
+
+
+def minmaxdist_textbook(q, boxes):
+    """MINMAXDIST of Roussopoulos, Kelley, Vincent (1995) for points outside the
+    box: min over axes k of (near-face distance_k^2 + sum of far-face distances_i^2, i != k)."""
+    D = len(q)
+    lo = np.minimum((q - boxes[:, :D]) ** 2, (boxes[:, D:] - q) ** 2)
+    hi = np.maximum((q - boxes[:, :D]) ** 2, (boxes[:, D:] - q) ** 2)
+    tot = hi.sum(axis=1)
+    return np.sqrt(np.min(tot[:, None] - hi + lo, axis=1))
+
+
+def knn_box_brute(q, boxes, k, tol=BASE_TOL):
+    """Closed-form box kNN set (ot_query.h:1058-1164 with MinMaxDistance): F_final =
+    tol_upper(k-th smallest pessimistic distance); result = {optimistic < F_final}
+    ascending by (optimistic distance, id).  Returns (ids, ambiguous)."""
+    dmin, dmm = box_minmax(q, boxes, tol)
+    F = DBL_MAX
+    if len(boxes) >= k:
+        x = np.sort(dmm)[k - 1]
+        F = max(tol, x * (1.0 + tol))
+    sel = np.nonzero(dmin < F)[0]
+    order = sel[np.lexsort((sel, dmin[sel]))]
+    return order.tolist(), bool(np.any(dmin == F))
+
+
 def range_brute_points(r, pts, tol=BASE_TOL):
     D = pts.shape[1]
     ok = np.ones(pts.shape[0], bool)

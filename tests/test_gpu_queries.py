@@ -79,6 +79,28 @@ def test_knn_c1_size_brute_sample(gpu_ctx):
         assert gids[goff[qi]:goff[qi + 1]].tolist() == exp
 
 
+# ---- box-tree kNN (MinMaxDistance) -----------------------------------------------------
+def test_box_knn_kats(gpu_ctx):
+    for c in kat()["knn_box_2d"]:
+        g = _tree(gpu_ctx, "box", 2, np.array(c["boxes"]), None, c["depth"], c["cap"])
+        off, ids = g.knn_batch([c["q"]], c["k"], DBL_MAX, c.get("tol", BASE_TOL)).csr()
+        assert sorted(ids.tolist()) == sorted(c["expect"]), c["src"]
+
+
+@pytest.mark.parametrize("D,n,k,cap,M", [(2, 5000, 1, 4, 6), (3, 20000, 8, 16, 6), (3, 8000, 16, 8, 5),
+                                         (3, 3000, 40, 8, 5), (4, 6000, 4, 8, 4), (6, 4000, 4, 16, 3)])
+def test_box_knn_vs_oracle(gpu_ctx, D, n, k, cap, M):
+    boxes = lognormal_boxes(n, D, 0.0, 100.0, np.log(0.8), 0.7, seed=D * 3 + k)
+    qs = uniform_points(1500, D, seed=5 + D + k)
+    qs[:10] = boxes[:10, :D]  # on box corners
+    g = _tree(gpu_ctx, "box", D, boxes, None, M, cap)
+    o = O.OracleTree(D, "box", boxes, None, M, cap)
+    goff, gids = g.knn_batch(qs, k).csr()
+    ooff, oids = o.knn(qs, k, threads=8)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(gids, oids)
+
+
 # ---- range ----------------------------------------------------------------------------
 @pytest.mark.parametrize("kind,D,mode", [("point", 2, 0), ("point", 3, 0), ("point", 6, 0), ("box", 3, 0), ("box", 3, 1),
                                          ("box", 2, 1)])

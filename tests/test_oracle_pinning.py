@@ -5,8 +5,9 @@ import math
 import numpy as np
 import pytest
 
-from helpers import (BASE_TOL, DBL_MAX, MT19937, check_node_table_rules, collision_brute, issue38_boxes, kat,
-                     knn_brute, lognormal_boxes, range_brute_points, uniform_points, unit_dirs)
+from helpers import (BASE_TOL, DBL_MAX, MT19937, box_minmax, check_node_table_rules, collision_brute, issue38_boxes,
+                     kat, knn_box_brute, knn_brute, lognormal_boxes, minmaxdist_textbook, range_brute_points,
+                     uniform_points, unit_dirs)
 from oracle import oracle as O
 
 
@@ -160,6 +161,73 @@ def test_knn_oracle_max_within():
             exp, a = knn_brute(qs[qi], pts, 8, mw)
             if not a:
                 assert got[qi] == exp
+
+
+# ---- box-tree kNN (MinMaxDistance) ---------------------------------------------------
+def test_box_minmax_closed_forms():
+    """The restated MinMaxDistance against the textbook MINMAXDIST for points
+    outside the box, the plain point-box distance for the optimistic value, and
+    the inside rule (0 and the smallest far-face distance)."""
+    rng = np.random.default_rng(5)
+    for D in (2, 3, 6):
+        boxes = lognormal_boxes(400, D, 0.0, 10.0, seed=D)
+        for _ in range(20):
+            q = rng.uniform(-2.0, 12.0, D)
+            dmin, dmm = box_minmax(q, boxes, 0.0)
+            lo, hi = boxes[:, :D], boxes[:, D:]
+            outside = np.any((q < lo) | (q > hi), axis=1)
+            plain = np.sqrt((np.maximum(0.0, np.maximum(lo - q, q - hi)) ** 2).sum(axis=1))
+            np.testing.assert_allclose(dmin, plain, rtol=1e-12, atol=1e-12)
+            np.testing.assert_allclose(dmm[outside], minmaxdist_textbook(q, boxes[outside]), rtol=1e-9, atol=1e-12)
+            far = np.sqrt(np.min(np.maximum((q - lo) ** 2, (hi - q) ** 2), axis=1))
+            np.testing.assert_array_equal(dmin[~outside], 0.0)
+            np.testing.assert_allclose(dmm[~outside], far[~outside], rtol=1e-12)
+            assert np.all(dmin <= dmm * (1 + 1e-12) + 1e-12)
+
+
+def test_box_knn_kats():
+    for c in kat()["knn_box_2d"]:
+        boxes = np.array(c["boxes"])
+        t = O.OracleTree(2, "box", boxes, None, c["depth"], c["cap"])
+        off, ids = t.knn([c["q"]], c["k"], DBL_MAX, c.get("tol", BASE_TOL))
+        assert sorted(ids.tolist()) == sorted(c["expect"]), c["src"]
+        exp, _ = knn_box_brute(np.array(c["q"]), boxes, c["k"], c.get("tol", BASE_TOL))
+        assert sorted(exp) == sorted(c["expect"]), c["src"]
+
+
+@pytest.mark.parametrize("D,k,cap,M", [(2, 1, 4, 6), (3, 4, 8, 5), (3, 16, 16, 6), (4, 8, 8, 4)])
+def test_box_knn_oracle_vs_brute(D, k, cap, M):
+    boxes = lognormal_boxes(2500, D, 0.0, 100.0, np.log(0.8), 0.7, seed=D * 11 + k)
+    qs = uniform_points(200, D, seed=77 + D)
+    t = O.OracleTree(D, "box", boxes, None, M, cap)
+    off, ids = t.knn(qs, k)
+    got = O.lists(off, ids)
+    amb = 0
+    for qi in range(len(qs)):
+        exp, a = knn_box_brute(qs[qi], boxes, k)
+        if a:
+            amb += 1
+            continue
+        assert got[qi] == exp, qi
+    assert amb == 0
+
+
+def test_box_knn_planted_mistake_is_caught():
+    """A pessimistic rule that drops the largest-difference term (plain sum of
+    far-face distances) gives a different result set on these queries."""
+    boxes = lognormal_boxes(1500, 3, 0.0, 50.0, np.log(1.0), 0.6, seed=3)
+    qs = uniform_points(100, 3, 0.0, 50.0, seed=4)
+    t = O.OracleTree(3, "box", boxes, None, 5, 8)
+    off, ids = t.knn(qs, 4)
+    got = O.lists(off, ids)
+    differ = 0
+    for qi, q in enumerate(qs):
+        dmin, _ = box_minmax(q, boxes, BASE_TOL)
+        wrong = np.sqrt((np.maximum((q - boxes[:, :3]) ** 2, (boxes[:, 3:] - q) ** 2)).sum(axis=1))
+        F = max(BASE_TOL, np.sort(wrong)[3] * (1 + BASE_TOL))
+        sel = np.nonzero(dmin < F)[0]
+        differ += int(sel[np.lexsort((sel, dmin[sel]))].tolist() != got[qi])
+    assert differ > 0
 
 
 # ---- range --------------------------------------------------------------------------
