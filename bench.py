@@ -493,7 +493,7 @@ def make_served(nreq, seed=77):
             c = r.uniform(0.0, 1000.0, (n, 3))
             h = np.exp(r.normal(np.log(0.5), 0.6, (n, 3)))
             trees.append(("box", np.concatenate([c - h, c + h], axis=1), 32))
-    # Zipf(1) popularity over a random ranking of the trees
+    # Zipf(1) popularity over a random ranking of the trees; kNN goes to point and box trees
     ranks = rng.permutation(64)
     pop = 1.0 / (1.0 + ranks)
     pop /= pop.sum()
@@ -503,10 +503,11 @@ def make_served(nreq, seed=77):
         kind, g, _ = trees[tid]
         nq = int(np.exp(rng.uniform(0.0, np.log(65536.0))))
         u = rng.random()
-        if kind == "point" and u < 0.5:
-            base = g[rng.integers(0, len(g), nq)] + rng.normal(0.0, 1.0, (nq, 3))
+        if (kind == "point" and u < 0.5) or (kind == "box" and u < 0.34):
+            pick = g[rng.integers(0, len(g), nq)]
+            base = (pick if kind == "point" else 0.5 * (pick[:, :3] + pick[:, 3:])) + rng.normal(0.0, 1.0, (nq, 3))
             reqs.append((tid, "knn", base, int(rng.integers(1, 33))))
-        elif (kind == "point" and u < 0.75) or (kind == "box" and u < 0.5):
+        elif (kind == "point" and u < 0.75) or (kind == "box" and u < 0.67):
             c = rng.uniform(0.0, 1000.0, (nq, 3))
             e = rng.uniform(1.0, 30.0, (nq, 1))
             reqs.append((tid, "range", np.concatenate([c - e / 2, c + e / 2], axis=1), 0))
@@ -659,7 +660,7 @@ def run_served(args, octree_amd):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",
         "config": {"workload": "served stream: 64 trees (32 point sets 2e5-4e6 pts uniform/clustered, 32 box sets "
                                "1e5-2e6 AABBs log-normal extents), Zipf(1) tree popularity, batches log-uniform "
-                               "1-65,536 mixing kNN (k 1-32, point trees), range and rays; HBM budget 60% of "
+                               "1-65,536 mixing kNN (k 1-32), range and rays; HBM budget 60% of "
                                "the working set", "requests": len(reqs), "queries": total_q,
                    "working_set_bytes": working, "budget_bytes": budget, "workers": 2},
         "saturation": sat, "open_loop": dict(op, offered_requests_per_s=offered),

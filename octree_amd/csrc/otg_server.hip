@@ -61,6 +61,7 @@ struct ServedTree {
   std::unique_ptr<otg_tree, TreeFree> tree;
   uint64_t bytes = 0;
   uint64_t last_use = 0;
+  double heat = 0.0;  // requests, decayed per batch: eviction keeps the frequently used trees
   uint32_t inflight = 0;
   bool building = false;
   uint64_t builds = 0;
@@ -114,7 +115,8 @@ class Server {
     trees.clear();
   }
 
-  // Evict least-recently-used idle trees until `need` more bytes fit (mu held;
+  // Evict the coldest idle trees (least requested, then least recently used)
+  // until `need` more bytes fit (mu held;
   // `self` is the tree being admitted).  A caller that holds pinned trees waits a
   // bounded time (`bounded`), so two workers never wait on each other's pins.
   bool make_room(std::unique_lock<std::mutex> &lk, uint64_t need, const ServedTree *self, bool bounded) {
@@ -125,7 +127,9 @@ class Server {
       ServedTree *victim = nullptr;
       for (auto &kv : trees) {
         ServedTree &e = kv.second;
-        if (e.tree && e.inflight == 0 && !e.building && (!victim || e.last_use < victim->last_use)) victim = &e;
+        if (e.tree && e.inflight == 0 && !e.building &&
+            (!victim || e.heat < victim->heat || (e.heat == victim->heat && e.last_use < victim->last_use)))
+          victim = &e;
       }
       if (victim) {
         resident -= victim->bytes;
@@ -214,6 +218,7 @@ class Server {
           batch.push_back(r);
         }
         ++batches;
+        for (auto &kv : trees) kv.second.heat *= 0.9;
       }
       run_batch(ctx, batch);
       {
@@ -522,7 +527,8 @@ OTG_API otg_status otg_server_submit(otg_server *s, uint32_t tree_id, int query,
     auto it = s->trees.find(tree_id);
     if (it == s->trees.end()) return OTG_ERR_INVALID_ARG;
     dim = it->second.dim;
-    it->second.last_use = ++s->tick;  // recency counts requests, not only launches
+    it->second.last_use = ++s->tick;  // recency and heat count requests, not only launches
+    it->second.heat += 1.0;
   }
   auto *r = new (std::nothrow) otg::ServerRequest();
   if (!r) return OTG_ERR_OOM;
