@@ -315,7 +315,8 @@ template <int D, int KCAP, bool BOX, typename Stack>
 __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                           const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
                                           Stack &S, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                          uint32_t *__restrict__ overflow, double *__restrict__ out_F) {
+                                          uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
+                                          double *__restrict__ out_F) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double q[D];
@@ -364,7 +365,8 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
         }
       }
     }
-    if (L.any_dropped && L.dropped <= L.F) atomicOr(overflow, 1u);
+    // the band below F holds more ids than the list: the query is redone by k_knn_band
+    if (L.any_dropped && L.dropped <= L.F) band[atomicAdd(overflow, 1u)] = (uint32_t)qi;
   }
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
@@ -450,7 +452,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
                                                  const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
                                                  LdsLevelStack &S, uint32_t *__restrict__ out_ids,
                                                  uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
-                                                 double *__restrict__ out_F) {
+                                                 uint32_t *__restrict__ band, double *__restrict__ out_F) {
   static_assert(D <= 3, "packed child order needs D * 2^D <= 28 bits");
   constexpr int NC = 1 << D;
   const uint32_t s = qset ? qset[qi] : 0u;
@@ -541,7 +543,8 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
       have = !knn_node_out(L, w, false);
       if (!have && cnt > 1) --top;  // the rest of this level is farther still
     }
-    if (L.any_dropped && L.dropped <= L.F) atomicOr(overflow, 1u);
+    // the band below F holds more ids than the list: the query is redone by k_knn_band
+    if (L.any_dropped && L.dropped <= L.F) band[atomicAdd(overflow, 1u)] = (uint32_t)qi;
   }
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
@@ -560,8 +563,9 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
                                               const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
                                               uint64_t nq, uint32_t k, double F0, double tol,
                                               uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                              uint32_t *__restrict__ overflow, double *__restrict__ out_F,
-                                              uint32_t *__restrict__ redo, uint32_t *__restrict__ redo_n) {
+                                              uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
+                                              double *__restrict__ out_F, uint32_t *__restrict__ redo,
+                                              uint32_t *__restrict__ redo_n) {
   __shared__ uint32_t s_n[kLdsStack * kQB];
   __shared__ uint32_t s_w[kLdsStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
@@ -572,10 +576,11 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
     bool ok;
     if constexpr (D <= 3) {
       LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
-      ok = knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+      ok = knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band,
+                                          out_F);
     } else {
       LdsStack S{&s_n[threadIdx.x], reinterpret_cast<float *>(&s_w[threadIdx.x])};
-      ok = knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+      ok = knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
     }
     if (!ok) redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
   });
@@ -589,15 +594,129 @@ __global__ void __launch_bounds__(kQB) k_knn_deep(const TreeView *__restrict__ v
                                                    const uint32_t *__restrict__ list, uint32_t nlist, uint32_t k,
                                                    double F0, double tol, uint32_t *__restrict__ out_ids,
                                                    uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
-                                                   double *__restrict__ out_F) {
+                                                   uint32_t *__restrict__ band, double *__restrict__ out_F) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nlist) return;
   const uint64_t qi = list[ti];
   const uint32_t kq = qk ? qk[qi] : k;
   ArrayStack<D> S;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, out_F);
+    knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
   });
+}
+
+// Queries whose band below the final F held more ids than the register list: a
+// fixed-radius walk collects every entity with (optimistic) distance below F --
+// the same arithmetic as the list pass -- counted first, then written as
+// (distance, id) pairs and sorted per query.
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_knn_band(const TreeView *__restrict__ views,
+                                                   const uint32_t *__restrict__ qview, const double *__restrict__ qpts,
+                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ band,
+                                                   uint32_t nband, const double *__restrict__ Fq, double tol,
+                                                   uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
+                                                   double *__restrict__ out_d, uint32_t *__restrict__ out_id) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nband) return;
+  const uint64_t qi = band[ti];
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+    const double F = Fq[qi], T = sq_reject_bound(F);
+    double q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+    uint32_t cnt = 0;
+    const uint64_t o = FILL ? offsets[ti] : 0;
+    if (sp.node_cnt != 0) {
+      uint32_t stack[StackCap<D>::value];
+      int top = 0;
+      stack[top++] = (uint32_t)sp.node_off;
+      while (top > 0) {
+        const uint32_t node = stack[--top];
+        bool in;
+        const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+        if (w2 >= T || sqrt(w2) >= F) continue;
+        const NodeInfo ni = tv.info[node];
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          double x2, mm2 = 0.0;
+          if (BOX) {
+            box_minmax2<D>(q, tv, j, tol, &x2, &mm2);
+          } else {
+            x2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+              x2 += v * v;
+            }
+          }
+          if (x2 >= T) continue;
+          const double dist = sqrt(x2);
+          if (!(dist < F)) continue;
+          if (FILL) {
+            out_d[o + cnt] = dist;
+            out_id[o + cnt] = tv.ids[j];
+          }
+          ++cnt;
+        }
+        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+      }
+    }
+    if (!FILL) counts[ti] = cnt;
+  });
+}
+
+// Heap sort of each band query's (distance, id) pairs, ascending, ties by id.
+__device__ __forceinline__ bool pair_less(double da, uint32_t ia, double db, uint32_t ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+__global__ void __launch_bounds__(kQB) k_sort_band(const uint32_t *__restrict__ offsets, uint32_t nband,
+                                                    double *__restrict__ d, uint32_t *__restrict__ id) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nband) return;
+  double *a = d + offsets[ti];
+  uint32_t *b = id + offsets[ti];
+  const uint32_t n = offsets[ti + 1] - offsets[ti];
+  auto sift = [&](uint32_t i, uint32_t m) {
+    for (;;) {
+      uint32_t l = 2 * i + 1, r = l + 1, x = i;
+      if (l < m && pair_less(a[x], b[x], a[l], b[l])) x = l;
+      if (r < m && pair_less(a[x], b[x], a[r], b[r])) x = r;
+      if (x == i) return;
+      const double td = a[i];
+      a[i] = a[x];
+      a[x] = td;
+      const uint32_t tid = b[i];
+      b[i] = b[x];
+      b[x] = tid;
+      i = x;
+    }
+  };
+  for (uint32_t i = n / 2; i-- > 0;) sift(i, n);
+  for (uint32_t m = n; m > 1; --m) {
+    const double td = a[0];
+    a[0] = a[m - 1];
+    a[m - 1] = td;
+    const uint32_t tid = b[0];
+    b[0] = b[m - 1];
+    b[m - 1] = tid;
+    sift(0, m - 1);
+  }
+}
+
+__global__ void k_band_counts(const uint32_t *__restrict__ band, uint32_t nband, const uint32_t *__restrict__ bcnt,
+                              uint32_t *__restrict__ cnt) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < nband) cnt[band[i]] = bcnt[i];
+}
+
+__global__ void k_band_copy(const uint32_t *__restrict__ band, uint32_t nband, const uint32_t *__restrict__ boff,
+                            const uint32_t *__restrict__ bid, const uint64_t *__restrict__ off,
+                            uint32_t *__restrict__ dst) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nband) return;
+  const uint64_t o = off[band[i]];
+  for (uint32_t t = boff[i]; t < boff[i + 1]; ++t) dst[o + (t - boff[i])] = bid[t];
 }
 
 // Measurement helper: the nodes a kNN query must visit (wall distance below the
@@ -1182,6 +1301,7 @@ __global__ void k_compact_fixed(const uint32_t *__restrict__ src, uint32_t strid
   if (qi >= nq) return;
   uint64_t o = off[qi];
   uint32_t c = (uint32_t)(off[qi + 1] - o);
+  if (c > stride) c = stride;  // longer lists (kNN bands) are written by their own pass
   for (uint32_t t = 0; t < c; ++t) dst[o + t] = src[qi * stride + t];
 }
 
@@ -1264,18 +1384,20 @@ template <int D, int KCAP, bool BOX>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
                         const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
-  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 2), redo(ctx, nq);
-  Buf<double> Fq;
-  if (footprint) Fq = Buf<double>(ctx, nq);
-  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 8, ctx->stream));  // [0] list overflow, [1] redo count
+  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 2), redo(ctx, nq), band(ctx, nq);
+  Buf<double> Fq(ctx, nq);
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 8, ctx->stream));  // [0] band count, [1] redo count
   launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
-         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p, redo.p, ovf.p + 1);
-  uint32_t hredo = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&hredo, ovf.p + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p, redo.p, ovf.p + 1);
+  uint32_t hovf[2] = {0, 0};
+  OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 8, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  if (hredo)
-    launch(ctx, "knn_deep", k_knn_deep<D, KCAP, BOX>, dim3(div_up(hredo, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-           qk, (const uint32_t *)redo.p, hredo, k, F0, tol, ids.p, cnt.p, ovf.p, Fq.p);
+  if (hovf[1]) {
+    launch(ctx, "knn_deep", k_knn_deep<D, KCAP, BOX>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+           qs, qk, (const uint32_t *)redo.p, hovf[1], k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
+    OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
   if (footprint) {
     Buf<unsigned long long> acc(ctx, 2);
     OTG_HIP_CHECK(hipMemsetAsync(acc.p, 0, 16, ctx->stream));
@@ -1287,13 +1409,31 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
     footprint[0] += h[0];
     footprint[1] += h[1];
   }
-  uint32_t hovf = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  const uint32_t nb = hovf[0];
+  if (nb == 0) return fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
+  // band queries: count, fill, sort, then splice into the CSR
+  Buf<uint32_t> bcnt(ctx, nb), boff(ctx, nb + 1), btot(ctx, 1);
+  launch(ctx, "knn_band_count", k_knn_band<D, BOX, false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+         qs, (const uint32_t *)band.p, nb, (const double *)Fq.p, tol, bcnt.p, (const uint32_t *)nullptr,
+         (double *)nullptr, (uint32_t *)nullptr);
+  exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p);
+  uint32_t hb = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
+  Buf<double> bd(ctx, std::max<uint32_t>(hb, 1));
+  Buf<uint32_t> bi(ctx, std::max<uint32_t>(hb, 1));
+  launch(ctx, "knn_band_fill", k_knn_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+         qs, (const uint32_t *)band.p, nb, (const double *)Fq.p, tol, (uint32_t *)nullptr, (const uint32_t *)boff.p,
+         bd.p, bi.p);
+  launch(ctx, "knn_band_sort", k_sort_band, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, nb, bd.p,
+         bi.p);
+  launch(ctx, "knn_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
+         (const uint32_t *)bcnt.p, cnt.p);
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
-  if (hovf) {
-    delete r;
-    throw Error{OTG_ERR_UNSUPPORTED, "kNN tolerance band holds more ids than the per-query list"};
-  }
+  launch(ctx, "knn_band_copy", k_band_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
+         (const uint32_t *)boff.p, (const uint32_t *)bi.p, (const uint64_t *)r->d_offsets, r->d_ids);
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the staged count above lives on this stack frame
   return r;
 }
 

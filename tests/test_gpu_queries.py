@@ -79,6 +79,29 @@ def test_knn_c1_size_brute_sample(gpu_ctx):
         assert gids[goff[qi]:goff[qi + 1]].tolist() == exp
 
 
+def test_knn_band_longer_than_list(gpu_ctx):
+    """Many entities inside the tolerance band (duplicates at the query): the list
+    pass overflows and the band pass returns every id below F, sorted."""
+    import octree_amd
+
+    pts = uniform_points(6000, 3, seed=31)
+    pts[:300] = pts[5000]  # 300 + 1 identical points
+    qs = np.concatenate([pts[5000:5001], uniform_points(500, 3, seed=32), pts[:3] + 1e-15])
+    g = _tree(gpu_ctx, "point", 3, pts, None, 6, 8)
+    o = O.OracleTree(3, "point", pts, None, 6, 8)
+    ctx = gpu_ctx
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    for k in (1, 5, 40):
+        goff, gids = g.knn_batch(qs, k).csr()
+        ooff, oids = o.knn(qs, k, threads=8)
+        assert np.array_equal(goff, ooff) and np.array_equal(gids, oids), k
+        assert goff[1] - goff[0] == 301
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    assert "knn_band_fill" in kt, sorted(kt)
+
+
 # ---- box-tree kNN (MinMaxDistance) -----------------------------------------------------
 def test_box_knn_kats(gpu_ctx):
     for c in kat()["knn_box_2d"]:
