@@ -50,11 +50,53 @@ struct TreeFree {
   void operator()(otg_tree *t) const { otg_tree_free(t); }
 };
 
+// Host copy of a registered tree's entities in pinned memory, so a rebuild after
+// eviction uploads at full link speed.
+struct PinnedGeom {
+  double *p = nullptr;
+  uint64_t count = 0;
+  PinnedGeom() = default;
+  PinnedGeom(const PinnedGeom &) = delete;
+  PinnedGeom &operator=(const PinnedGeom &) = delete;
+  PinnedGeom(PinnedGeom &&o) noexcept : p(o.p), count(o.count) {
+    o.p = nullptr;
+    o.count = 0;
+  }
+  PinnedGeom &operator=(PinnedGeom &&o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      count = o.count;
+      o.p = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
+  ~PinnedGeom() { reset(); }
+  bool allocate(uint64_t n) {
+    reset();
+    if (n == 0) return true;
+    if (hipHostMalloc((void **)&p, n * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return false;
+    }
+    count = n;
+    return true;
+  }
+  void reset() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  double *data() { return p; }
+};
+
 struct ServedTree {
   // registration (kept while evicted, to rebuild on demand)
   int kind = 0, dim = 0;
   uint64_t n = 0;
-  std::vector<double> geom;
+  PinnedGeom geom;
   std::vector<double> box;  // empty: bounding box of the entities
   uint32_t max_depth = 0, cap = 0;
   // residency
@@ -86,6 +128,7 @@ class Server {
   uint64_t budget = 0;
   uint64_t resident = 0;  // bytes of resident trees + reservations of builds in progress
   uint64_t max_batch_queries = 1ull << 21;
+  double defer_ms = 5.0;
   std::mutex mu, build_mu;
   std::condition_variable cv_work, cv_done, cv_mem;
   std::deque<ServerRequest *> pending;
@@ -207,15 +250,31 @@ class Server {
       std::vector<ServerRequest *> batch;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stop || !pending.empty(); });
-        if (stop) return;
-        uint64_t q = 0;
-        while (!pending.empty() && (batch.empty() || q + pending.front()->nq <= max_batch_queries)) {
-          ServerRequest *r = pending.front();
-          pending.pop_front();
-          q += r->nq;
-          r->t_start = now_ms();
-          batch.push_back(r);
+        // Requests for evicted trees wait up to defer_ms while other work exists, so
+        // one rebuild serves every request that arrived for the tree meanwhile.
+        for (;;) {
+          cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+          if (stop) return;
+          const double now = now_ms();
+          uint64_t q = 0;
+          double next_due = 1e300;
+          for (auto it = pending.begin(); it != pending.end();) {
+            ServerRequest *r = *it;
+            if (!batch.empty() && q + r->nq > max_batch_queries) break;
+            auto tr = trees.find(r->tree_id);
+            const bool cold = tr != trees.end() && !tr->second.tree;
+            if (cold && now - r->t_submit < defer_ms) {
+              next_due = std::min(next_due, r->t_submit + defer_ms);
+              ++it;
+              continue;
+            }
+            q += r->nq;
+            r->t_start = now;
+            batch.push_back(r);
+            it = pending.erase(it);
+          }
+          if (!batch.empty()) break;
+          cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(std::max(0.05, next_due - now)));
         }
         ++batches;
         for (auto &kv : trees) kv.second.heat *= 0.9;
@@ -456,18 +515,18 @@ OTG_API otg_status otg_server_add_tree(otg_server *s, uint32_t tree_id, int kind
   e.max_depth = max_depth;
   e.cap = max_elements;
   const uint64_t cols = kind == OTG_KIND_BOX ? 2ull * dim : (uint64_t)dim;
+  (void)hipSetDevice(s->build_ctx->device);
   try {
-    e.geom.resize(n * cols);
     if (box_or_null) e.box.assign(box_or_null, box_or_null + 2 * dim);
   } catch (...) {
     return OTG_ERR_OOM;
   }
+  if (!e.geom.allocate(n * cols)) return OTG_ERR_OOM;
   if (n) {
     if (mem == OTG_MEM_HOST) {
-      std::copy(geom, geom + n * cols, e.geom.begin());
-    } else {
-      (void)hipSetDevice(s->build_ctx->device);
-      if (hipMemcpy(e.geom.data(), geom, n * cols * 8, hipMemcpyDeviceToHost) != hipSuccess) return OTG_ERR_HIP;
+      std::copy(geom, geom + n * cols, e.geom.data());
+    } else if (hipMemcpy(e.geom.data(), geom, n * cols * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      return OTG_ERR_HIP;
     }
   }
   std::unique_lock<std::mutex> lk(s->mu);
