@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (the known-size kernel that scales PMC byte counters)")
     ap.add_argument("--load", type=float, default=0.7, help="served workload: offered load / measured capacity")
+    ap.add_argument("--budget-frac", type=float, default=0.6,
+                    help="served workload: HBM budget as a fraction of the all-resident working set")
     ap.add_argument("--profile-only", action="store_true", help="one batched step, no plain/CPU legs")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "octree_point_1m", "octree_box_4m", "point6d_8m", "served"],
@@ -544,7 +546,7 @@ def run_served(args, octree_amd):
     t_add = time.perf_counter() - tb
     working = srv.stats()["resident_bytes"]
     srv.close()
-    budget = int(working * 0.6)
+    budget = int(working * args.budget_frac)
     log(f"working set {working / 2**30:.2f} GiB (all 64 trees built in {t_add:.1f}s); budget {budget / 2**30:.2f} GiB")
 
     def serve(load_qps=None):
@@ -590,7 +592,8 @@ def run_served(args, octree_amd):
                 "requests_per_s": len(reqs) / (T1 - T0), "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "launches": st["launches"] - st0["launches"],
                 "batches": st["batches"] - st0["batches"], "evictions": st["evictions"] - st0["evictions"],
-                "rebuilds": st["rebuilds"] - st0["rebuilds"]}
+                "rebuilds": st["rebuilds"] - st0["rebuilds"],
+                "build_ms": (st["build_us"] - st0["build_us"]) / 1e3, "query_ms": (st["query_us"] - st0["query_us"]) / 1e3}
 
     sat = serve(None)
     log(f"saturation: {sat}")
@@ -660,8 +663,8 @@ def run_served(args, octree_amd):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",
         "config": {"workload": "served stream: 64 trees (32 point sets 2e5-4e6 pts uniform/clustered, 32 box sets "
                                "1e5-2e6 AABBs log-normal extents), Zipf(1) tree popularity, batches log-uniform "
-                               "1-65,536 mixing kNN (k 1-32), range and rays; HBM budget 60% of "
-                               "the working set", "requests": len(reqs), "queries": total_q,
+                               "1-65,536 mixing kNN (k 1-32), range and rays; HBM budget "
+                               f"{args.budget_frac:.0%} of the working set", "requests": len(reqs), "queries": total_q,
                    "working_set_bytes": working, "budget_bytes": budget, "workers": 2},
         "saturation": sat, "open_loop": dict(op, offered_requests_per_s=offered),
         "plain": plain, "speedup_served_vs_plain": sat["queries_per_s"] / plain["queries_per_s"],

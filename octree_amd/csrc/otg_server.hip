@@ -139,6 +139,7 @@ class Server {
   uint64_t next_id = 1, tick = 0;
   // statistics
   uint64_t batches = 0, launches = 0, served_requests = 0, served_queries = 0, evictions = 0, rebuilds = 0;
+  double build_ms = 0.0, query_ms = 0.0;  // wall time in builds and in query launches (incl. copies)
 
   void shutdown() {
     {
@@ -217,6 +218,7 @@ class Server {
     lk.unlock();
     otg_tree *t = nullptr;
     otg_status st;
+    const double tb0 = now_ms();
     {
       std::lock_guard<std::mutex> bg(build_mu);
       otg_ctx *bc = static_cast<otg_ctx *>(build_ctx);
@@ -225,6 +227,7 @@ class Server {
       if (st == OTG_OK) st = otg_ctx_synchronize(bc);  // workers read the tree from their own streams
     }
     lk.lock();
+    build_ms += now_ms() - tb0;
     resident -= est;
     if (st == OTG_OK) {
       otg_tree_info ti{};
@@ -402,6 +405,15 @@ class Server {
 
   void launch_group(Ctx *ctx, std::vector<ServerRequest *> &group, const std::vector<const Tree *> &views,
                     const std::unordered_map<uint32_t, uint32_t> &view_of_tree) {
+    const double t0 = now_ms();
+    struct Timer {
+      Server *s;
+      double t0;
+      ~Timer() {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->query_ms += now_ms() - t0;
+      }
+    } timer{this, t0};
     const ServerRequest *a = group[0];
     const int dim = views[0]->dim;
     const int stride = a->query == 0 ? dim : 2 * dim;
@@ -654,6 +666,8 @@ OTG_API otg_status otg_server_stats(otg_server *s, uint64_t *stats) {
   stats[7] = s->pending.size();
   stats[8] = s->rebuilds;
   stats[9] = s->trees.size();
+  stats[10] = (uint64_t)(s->build_ms * 1000.0);
+  stats[11] = (uint64_t)(s->query_ms * 1000.0);
   return OTG_OK;
 }
 
