@@ -92,6 +92,36 @@ struct PinnedGeom {
   double *data() { return p; }
 };
 
+// Per-worker pinned staging for query inputs and results (grows, never shrinks).
+struct PinnedBytes {
+  char *p = nullptr;
+  uint64_t cap = 0;
+  PinnedBytes() = default;
+  PinnedBytes(const PinnedBytes &) = delete;
+  PinnedBytes &operator=(const PinnedBytes &) = delete;
+  ~PinnedBytes() {
+    if (p) (void)hipHostFree(p);
+  }
+  char *ensure(uint64_t bytes) {
+    if (bytes <= cap) return p;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const uint64_t want = std::max<uint64_t>(bytes + bytes / 4, 1ull << 20);
+    if (hipHostMalloc((void **)&p, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw std::bad_alloc();
+    }
+    cap = want;
+    return p;
+  }
+};
+
+struct Staging {
+  PinnedBytes in, out;
+};
+
 struct ServedTree {
   // registration (kept while evicted, to rebuild on demand)
   int kind = 0, dim = 0;
@@ -125,6 +155,7 @@ class Server {
  public:
   Ctx *build_ctx = nullptr;  // builds and owns tree memory
   std::vector<Ctx *> worker_ctx;
+  std::vector<std::unique_ptr<Staging>> staging;  // one per worker
   uint64_t budget = 0;
   uint64_t resident = 0;  // bytes of resident trees + reservations of builds in progress
   uint64_t max_batch_queries = 1ull << 21;
@@ -282,7 +313,7 @@ class Server {
         ++batches;
         for (auto &kv : trees) kv.second.heat *= 0.9;
       }
-      run_batch(ctx, batch);
+      run_batch(ctx, *staging[w], batch);
       {
         std::lock_guard<std::mutex> g(mu);
         for (ServerRequest *r : batch) {
@@ -300,7 +331,7 @@ class Server {
   // the others in chunks that fit the budget -- so each evicted tree is rebuilt at
   // most once per batch; within a chunk, one launch per (query, tree kind, dim,
   // k class, parameters).
-  void run_batch(Ctx *ctx, std::vector<ServerRequest *> &batch) {
+  void run_batch(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &batch) {
     std::unordered_map<uint32_t, std::vector<ServerRequest *>> by_tree;
     std::vector<uint32_t> tree_order;
     std::vector<std::vector<uint32_t>> chunks;
@@ -354,11 +385,11 @@ class Server {
           }
         }
       }
-      for (auto &kv : groups) run_group(ctx, kv.second);
+      for (auto &kv : groups) run_group(ctx, stg, kv.second);
     }
   }
 
-  void run_group(Ctx *ctx, std::vector<ServerRequest *> &group) {
+  void run_group(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &group) {
     size_t i = 0;
     while (i < group.size()) {
       // make the trees resident and pin them for the launch; when memory for the
@@ -391,7 +422,7 @@ class Server {
           live.push_back(r);
         }
       }
-      if (!live.empty()) launch_group(ctx, live, views, view_of_tree);
+      if (!live.empty()) launch_group(ctx, stg, live, views, view_of_tree);
       {
         std::lock_guard<std::mutex> g(mu);
         for (uint32_t id : pinned) {
@@ -403,7 +434,8 @@ class Server {
     }
   }
 
-  void launch_group(Ctx *ctx, std::vector<ServerRequest *> &group, const std::vector<const Tree *> &views,
+  void launch_group(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &group,
+                    const std::vector<const Tree *> &views,
                     const std::unordered_map<uint32_t, uint32_t> &view_of_tree) {
     const double t0 = now_ms();
     struct Timer {
@@ -420,25 +452,27 @@ class Server {
     uint64_t total = 0;
     for (auto *r : group) total += r->nq;
     try {
-      std::vector<double> q;
-      std::vector<uint32_t> qv(total), qk;
-      q.reserve(total * stride);
-      if (a->query == 0) qk.resize(total);
+      // inputs straight into pinned memory: queries, per-query tree index, per-query k
+      const uint64_t qbytes = total * stride * 8, vbytes = total * 4;
+      char *in = stg.in.ensure(qbytes + 2 * vbytes);
+      double *q = reinterpret_cast<double *>(in);
+      uint32_t *qv = reinterpret_cast<uint32_t *>(in + qbytes);
+      uint32_t *qk = a->query == 0 ? qv + total : nullptr;
       uint64_t at = 0;
       for (auto *r : group) {
-        q.insert(q.end(), r->data.begin(), r->data.end());
+        std::copy(r->data.begin(), r->data.end(), q + at * stride);
         const uint32_t v = view_of_tree.at(r->tree_id);
-        std::fill(qv.begin() + at, qv.begin() + at + r->nq, v);
-        if (a->query == 0) std::fill(qk.begin() + at, qk.begin() + at + r->nq, r->k);
+        std::fill(qv + at, qv + at + r->nq, v);
+        if (qk) std::fill(qk + at, qk + at + r->nq, r->k);
         at += r->nq;
       }
-      std::unique_ptr<Result> res(query_multi(ctx, a->query, views, total, q.data(), qv.data(),
-                                              qk.empty() ? nullptr : qk.data(), a->p0, a->p1, a->p2, a->mode));
-      std::vector<uint64_t> off(total + 1);
-      std::vector<uint32_t> ids(res->nids);
-      OTG_HIP_CHECK(hipMemcpyAsync(off.data(), res->d_offsets, (total + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
-      if (res->nids)
-        OTG_HIP_CHECK(hipMemcpyAsync(ids.data(), res->d_ids, res->nids * 4, hipMemcpyDeviceToHost, ctx->stream));
+      std::unique_ptr<Result> res(
+          query_multi(ctx, a->query, views, total, q, qv, qk, a->p0, a->p1, a->p2, a->mode));
+      char *outb = stg.out.ensure((total + 1) * 8 + res->nids * 4);
+      uint64_t *off = reinterpret_cast<uint64_t *>(outb);
+      uint32_t *ids = reinterpret_cast<uint32_t *>(outb + (total + 1) * 8);
+      OTG_HIP_CHECK(hipMemcpyAsync(off, res->d_offsets, (total + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+      if (res->nids) OTG_HIP_CHECK(hipMemcpyAsync(ids, res->d_ids, res->nids * 4, hipMemcpyDeviceToHost, ctx->stream));
       OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
       {
         std::lock_guard<std::mutex> g(mu);
@@ -453,7 +487,7 @@ class Server {
         out->nids = e - b;
         out->h_offsets.resize(r->nq + 1);
         for (uint64_t i = 0; i <= r->nq; ++i) out->h_offsets[i] = off[q0 + i] - b;
-        out->h_ids.assign(ids.begin() + b, ids.begin() + e);
+        out->h_ids.assign(ids + b, ids + e);
         r->result = out;
         q0 += r->nq;
       }
@@ -500,6 +534,7 @@ OTG_API otg_status otg_server_create(int device, uint64_t hbm_budget_bytes, uint
     }
     s->worker_ctx.push_back(wc);
   }
+  for (uint32_t w = 0; w < nworkers; ++w) s->staging.emplace_back(new otg::Staging());
   for (uint32_t w = 0; w < nworkers; ++w) s->workers.emplace_back([s, w] { s->worker_loop((int)w); });
   *out = s;
   return OTG_OK;
