@@ -593,7 +593,8 @@ def run_served(args, octree_amd):
                 "p99_ms": float(np.percentile(lat, 99)), "launches": st["launches"] - st0["launches"],
                 "batches": st["batches"] - st0["batches"], "evictions": st["evictions"] - st0["evictions"],
                 "rebuilds": st["rebuilds"] - st0["rebuilds"],
-                "build_ms": (st["build_us"] - st0["build_us"]) / 1e3, "query_ms": (st["query_us"] - st0["query_us"]) / 1e3}
+                **{k[:-3] + "_ms": (st[k] - st0[k]) / 1e3 for k in ("build_us", "query_us", "stage_us", "launch_us",
+                                                                     "split_us")}}
 
     sat = serve(None)
     log(f"saturation: {sat}")
@@ -601,6 +602,9 @@ def run_served(args, octree_amd):
     op = serve(offered)
     log(f"open loop at {offered:.0f} req/s: {op}")
 
+    if args.no_plain:
+        return {"metric": METRIC, "value": sat["queries_per_s"], "unit": "served queries/s (kNN+range+ray mix)",
+                "saturation": sat, "open_loop": dict(op, offered_requests_per_s=offered)}
     # plain: one request at a time, direct single-tree calls, every tree resident
     ctx = octree_amd.Context(0)
     direct = []

@@ -199,9 +199,10 @@ OTG_API otg_status otg_server_submit(otg_server *server, uint32_t tree_id, int q
  * latency_ms = submit to done, queue_ms = submit to batch start (either may be NULL). */
 OTG_API otg_status otg_server_wait(otg_server *server, uint64_t request_id, otg_result **out, double *latency_ms,
                                    double *queue_ms);
-/* stats[12] = batches, launches, requests served, queries served, evictions, resident bytes,
- * resident trees, pending requests, rebuilds of evicted trees, registered trees, microseconds
- * spent building trees, microseconds spent in query launches (staging and copies included). */
+/* stats[15] = batches, launches, requests served, queries served, evictions, resident bytes,
+ * resident trees, pending requests, rebuilds of evicted trees, registered trees, and
+ * microseconds spent building trees, in query launches overall, and within those staging
+ * the inputs, in the launch itself, and splitting the results per request. */
 OTG_API otg_status otg_server_stats(otg_server *server, uint64_t *stats);
 
 #ifdef __cplusplus

@@ -452,10 +452,11 @@ class Server:
         return Result(h, None), lat.value, qms.value
 
     def stats(self) -> dict:
-        v = (C.c_uint64 * 12)()
+        v = (C.c_uint64 * 15)()
         _check(_abi.lib().otg_server_stats(self.handle, v), "otg_server_stats")
         keys = ["batches", "launches", "requests", "queries", "evictions", "resident_bytes", "resident_trees",
-                "pending", "rebuilds", "registered_trees", "build_us", "query_us"]
+                "pending", "rebuilds", "registered_trees", "build_us", "query_us", "stage_us", "launch_us",
+                "split_us"]
         return dict(zip(keys, [int(x) for x in v]))
 
     def close(self) -> None:

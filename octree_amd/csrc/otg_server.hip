@@ -171,6 +171,7 @@ class Server {
   // statistics
   uint64_t batches = 0, launches = 0, served_requests = 0, served_queries = 0, evictions = 0, rebuilds = 0;
   double build_ms = 0.0, query_ms = 0.0;  // wall time in builds and in query launches (incl. copies)
+  double stage_ms = 0.0, launch_ms = 0.0, split_ms = 0.0;  // query_ms split: inputs, launch, results
 
   void shutdown() {
     {
@@ -466,8 +467,10 @@ class Server {
         if (qk) std::fill(qk + at, qk + at + r->nq, r->k);
         at += r->nq;
       }
+      const double t1 = now_ms();
       std::unique_ptr<Result> res(
           query_multi(ctx, a->query, views, total, q, qv, qk, a->p0, a->p1, a->p2, a->mode));
+      const double t2 = now_ms();
       char *outb = stg.out.ensure((total + 1) * 8 + res->nids * 4);
       uint64_t *off = reinterpret_cast<uint64_t *>(outb);
       uint32_t *ids = reinterpret_cast<uint32_t *>(outb + (total + 1) * 8);
@@ -490,6 +493,12 @@ class Server {
         out->h_ids.assign(ids + b, ids + e);
         r->result = out;
         q0 += r->nq;
+      }
+      {
+        std::lock_guard<std::mutex> g(mu);
+        stage_ms += t1 - t0;
+        launch_ms += t2 - t1;
+        split_ms += now_ms() - t2;
       }
     } catch (const Error &e) {
       for (auto *r : group) {
@@ -703,6 +712,9 @@ OTG_API otg_status otg_server_stats(otg_server *s, uint64_t *stats) {
   stats[9] = s->trees.size();
   stats[10] = (uint64_t)(s->build_ms * 1000.0);
   stats[11] = (uint64_t)(s->query_ms * 1000.0);
+  stats[12] = (uint64_t)(s->stage_ms * 1000.0);
+  stats[13] = (uint64_t)(s->launch_ms * 1000.0);
+  stats[14] = (uint64_t)(s->split_ms * 1000.0);
   return OTG_OK;
 }
 
