@@ -1036,6 +1036,39 @@ struct RayTester {
   }
 };
 
+// Descending enter distance, ties by id (selection into output order).
+template <int RCAP>
+__device__ __forceinline__ void ray_output(uint64_t qi, uint32_t cnt, double (&ce)[RCAP], uint32_t (&cid)[RCAP],
+                                           uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt) {
+  out_cnt[qi] = cnt;
+  for (uint32_t a = 0; a < cnt; ++a) {
+    uint32_t best = a;
+    for (uint32_t b = a + 1; b < cnt; ++b) {
+      double eb = 0, ebest = 0;
+      uint32_t ib = 0, ibest = 0;
+#pragma unroll
+      for (int t = 0; t < RCAP; ++t) {
+        if ((uint32_t)t == b) { eb = ce[t]; ib = cid[t]; }
+        if ((uint32_t)t == best) { ebest = ce[t]; ibest = cid[t]; }
+      }
+      if (eb > ebest || (eb == ebest && ib < ibest)) best = b;
+    }
+    double ea = 0, ebest = 0;
+    uint32_t ia = 0, ibest = 0;
+#pragma unroll
+    for (int t = 0; t < RCAP; ++t) {
+      if ((uint32_t)t == a) { ea = ce[t]; ia = cid[t]; }
+      if ((uint32_t)t == best) { ebest = ce[t]; ibest = cid[t]; }
+    }
+#pragma unroll
+    for (int t = 0; t < RCAP; ++t) {
+      if ((uint32_t)t == a) { ce[t] = ebest; cid[t] = ibest; }
+      if ((uint32_t)t == best && best != a) { ce[t] = ea; cid[t] = ia; }
+    }
+    out_ids[qi * RCAP + a] = ibest;
+  }
+}
+
 template <int D, bool BOX, int RCAP>
 __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
                                           const uint32_t *__restrict__ qset, double tol, double inc, double max_dist,
@@ -1129,34 +1162,135 @@ __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const
     }
     // descending enter distance, ties by id (selection into output order)
   }
-  out_cnt[qi] = cnt;
-  // simple in-register sort (cnt <= RCAP, small)
-  for (uint32_t a = 0; a < cnt; ++a) {
-    uint32_t best = a;
-    for (uint32_t b = a + 1; b < cnt; ++b) {
-      double eb = 0, ebest = 0;
-      uint32_t ib = 0, ibest = 0;
+  ray_output<RCAP>(qi, cnt, ce, cid, out_ids, out_cnt);
+}
+
+// Ray walk for D <= 3 with level entries (as the kNN walk): per level, the
+// pending children packed in ascending enter distance; a pop recomputes the
+// child's hit and, when its enter exceeds the current bound M, drops the level
+// (the rest enter later still).  Same visits and candidates as ray_query.
+template <int D, bool BOX, int RCAP>
+__device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
+                                                 const uint32_t *__restrict__ qset, double tol, double inc,
+                                                 double max_dist, LdsLevelStack &S, uint32_t *__restrict__ out_ids,
+                                                 uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
+  constexpr int NC = 1 << D;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  uint32_t cnt = 0;
+  double ce[RCAP];
+  uint32_t cid[RCAP];
+  RayTester<D> rt;
+  if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
+    const double applied = (tol == 0) ? DBL_EPSILON : tol;
+    double M = max_dist;
+    double en, ex;
+    int top = 0;
+    uint32_t node = (uint32_t)sp.node_off;
+    NodeInfo ni = tv.info[node];
+    bool have = rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex) && !(en > M);
+    for (;;) {
+      if (have) {
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          bool hit;
+          if (BOX) {
+            double lo[D], hi[D];
 #pragma unroll
-      for (int t = 0; t < RCAP; ++t) {
-        if ((uint32_t)t == b) { eb = ce[t]; ib = cid[t]; }
-        if ((uint32_t)t == best) { ebest = ce[t]; ibest = cid[t]; }
+            for (int d = 0; d < D; ++d) lo[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
+#pragma unroll
+            for (int d = 0; d < D; ++d) hi[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
+            hit = rt.box_hit(lo, hi, &en, &ex);
+          } else {
+            double pnt[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) pnt[d] = tv.geom[(uint64_t)d * tv.n + j];
+            hit = rt.hit_point(pnt, &en, &ex);
+          }
+          if (!hit || en > M) continue;
+          if (cnt == RCAP) {
+            atomicOr(overflow, 1u);
+          } else {
+#pragma unroll
+            for (int t = 0; t < RCAP; ++t)
+              if ((uint32_t)t == cnt) {
+                ce[t] = en;
+                cid[t] = tv.ids[j];
+              }
+            ++cnt;
+          }
+          if (ex >= M) continue;
+          M = (1.0 + inc) * ex + applied;
+          uint32_t w = 0;
+#pragma unroll
+          for (int t = 0; t < RCAP; ++t) {
+            if ((uint32_t)t < cnt && !(ce[t] > M)) {
+#pragma unroll
+              for (int u = 0; u < RCAP; ++u)
+                if ((uint32_t)u == w) {
+                  ce[u] = ce[t];
+                  cid[u] = cid[t];
+                }
+              ++w;
+            }
+          }
+          cnt = w;
+        }
+        if (ni.ccount) {
+          double w[NC];
+          uint32_t ix[NC];
+          uint32_t nc = 0;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            ix[c] = (uint32_t)c;
+            w[c] = __builtin_inf();
+            if ((uint32_t)c < ni.ccount) {
+              const uint32_t x = ni.cbegin + c;
+              double cen, cex;
+              if (rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &cen, &cex) && !(cen > M)) {
+                w[c] = fmax(cen, -DBL_MAX);
+                ++nc;
+              }
+            }
+          }
+          if (NC == 4) {
+            cas(w[0], ix[0], w[1], ix[1]); cas(w[2], ix[2], w[3], ix[3]);
+            cas(w[0], ix[0], w[2], ix[2]); cas(w[1], ix[1], w[3], ix[3]);
+            cas(w[1], ix[1], w[2], ix[2]);
+          } else {
+            cas(w[0], ix[0], w[1], ix[1]); cas(w[2], ix[2], w[3], ix[3]);
+            cas(w[4], ix[4], w[5], ix[5]); cas(w[6], ix[6], w[7], ix[7]);
+            cas(w[0], ix[0], w[2], ix[2]); cas(w[1], ix[1], w[3], ix[3]);
+            cas(w[4], ix[4], w[6], ix[6]); cas(w[5], ix[5], w[7], ix[7]);
+            cas(w[1], ix[1], w[2], ix[2]); cas(w[5], ix[5], w[6], ix[6]);
+            cas(w[0], ix[0], w[4], ix[4]); cas(w[1], ix[1], w[5], ix[5]);
+            cas(w[2], ix[2], w[6], ix[6]); cas(w[3], ix[3], w[7], ix[7]);
+            cas(w[2], ix[2], w[4], ix[4]); cas(w[3], ix[3], w[5], ix[5]);
+            cas(w[1], ix[1], w[2], ix[2]); cas(w[3], ix[3], w[4], ix[4]);
+            cas(w[5], ix[5], w[6], ix[6]);
+          }
+          if (nc) {
+            uint32_t packed = nc << 28;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if ((uint32_t)c < nc) packed |= ix[c] << (D * c);
+            S.cb[top * kQB] = ni.cbegin;
+            S.pk[top * kQB] = packed;
+            ++top;
+          }
+        }
       }
-      if (eb > ebest || (eb == ebest && ib < ibest)) best = b;
+      if (top == 0) break;
+      const uint32_t packed = S.pk[(top - 1) * kQB];
+      const uint32_t pc = packed >> 28;
+      node = S.cb[(top - 1) * kQB] + (packed & (NC - 1));
+      ni = tv.info[node];
+      if (pc == 1) --top;
+      else S.pk[(top - 1) * kQB] = ((pc - 1) << 28) | ((packed & 0x0FFFFFFFu) >> D);
+      have = rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex) && !(en > M);
+      if (!have && pc > 1) --top;  // the rest of this level enters later still
     }
-    double ea = 0, ebest = 0;
-    uint32_t ia = 0, ibest = 0;
-#pragma unroll
-    for (int t = 0; t < RCAP; ++t) {
-      if ((uint32_t)t == a) { ea = ce[t]; ia = cid[t]; }
-      if ((uint32_t)t == best) { ebest = ce[t]; ibest = cid[t]; }
-    }
-#pragma unroll
-    for (int t = 0; t < RCAP; ++t) {
-      if ((uint32_t)t == a) { ce[t] = ebest; cid[t] = ibest; }
-      if ((uint32_t)t == best && best != a) { ce[t] = ea; cid[t] = ia; }
-    }
-    out_ids[qi * RCAP + a] = ibest;
   }
+  ray_output<RCAP>(qi, cnt, ce, cid, out_ids, out_cnt);
 }
 
 template <int D, bool BOX, int RCAP>
@@ -1165,11 +1299,18 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
                                               const uint32_t *__restrict__ order, uint64_t nq, double tol, double inc,
                                               double max_dist, uint32_t *__restrict__ out_ids,
                                               uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
+  __shared__ uint32_t s_a[(D <= 3 ? kLdsStack : 1) * kQB];
+  __shared__ uint32_t s_b[(D <= 3 ? kLdsStack : 1) * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, out_ids, out_cnt, overflow);
+    if constexpr (D <= 3) {
+      LdsLevelStack S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
+      ray_query_levels<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, S, out_ids, out_cnt, overflow);
+    } else {
+      ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, out_ids, out_cnt, overflow);
+    }
   });
 }
 
@@ -1487,6 +1628,12 @@ static const Tree *range_index(const Tree *t) {
   return (t->knn_tree && t->kind == OTG_KIND_POINT) ? t->knn_tree : t;
 }
 
+// Point rays too: the first-hit set is the hits whose enter distance is within
+// the bound set by the nearest exits (ot_query.h:2085-2099) -- distances only.
+static const Tree *ray_index(const Tree *t) {
+  return (t->knn_tree && t->kind == OTG_KIND_POINT) ? t->knn_tree : t;
+}
+
 static double knn_F0(double max_within, double tol) {
   return (max_within == DBL_MAX) ? DBL_MAX : fmax(tol, max_within * (1.0 + tol));
 }
@@ -1593,6 +1740,7 @@ static Result *ray_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol, double inc,
                   double max_dist) {
   Ctx *ctx = run_ctx(t);
+  t = ray_index(t);
   DevInput in;
   stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
   return ray_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, tol, inc, max_dist);
@@ -1610,7 +1758,7 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
   uint64_t total_n = 0;
   for (const Tree *t : trees) {
     if (t->kind != kind || t->dim != dim) throw Error{OTG_ERR_INVALID_ARG, "trees of one launch differ in kind or dim"};
-    const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : t);
+    const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : ray_index(t));
     hv.push_back(view_of(x));
     vs.cbits = std::max(vs.cbits, x->layout.cbits);
     vs.sbits = std::max(vs.sbits, x->layout.sbits);
