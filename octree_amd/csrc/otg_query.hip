@@ -665,11 +665,14 @@ __global__ void __launch_bounds__(kQB) k_knn_band(const TreeView *__restrict__ v
   });
 }
 
-// Heap sort of each band query's (distance, id) pairs, ascending, ties by id.
+// Heap sort of each band query's (distance, id) pairs: ascending distance for
+// kNN, descending enter distance for rays; ties by id either way.
+template <bool DESC>
 __device__ __forceinline__ bool pair_less(double da, uint32_t ia, double db, uint32_t ib) {
-  return da < db || (da == db && ia < ib);
+  return (DESC ? da > db : da < db) || (da == db && ia < ib);
 }
 
+template <bool DESC>
 __global__ void __launch_bounds__(kQB) k_sort_band(const uint32_t *__restrict__ offsets, uint32_t nband,
                                                     double *__restrict__ d, uint32_t *__restrict__ id) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
@@ -680,8 +683,8 @@ __global__ void __launch_bounds__(kQB) k_sort_band(const uint32_t *__restrict__ 
   auto sift = [&](uint32_t i, uint32_t m) {
     for (;;) {
       uint32_t l = 2 * i + 1, r = l + 1, x = i;
-      if (l < m && pair_less(a[x], b[x], a[l], b[l])) x = l;
-      if (r < m && pair_less(a[x], b[x], a[r], b[r])) x = r;
+      if (l < m && pair_less<DESC>(a[x], b[x], a[l], b[l])) x = l;
+      if (r < m && pair_less<DESC>(a[x], b[x], a[r], b[r])) x = r;
       if (x == i) return;
       const double td = a[i];
       a[i] = a[x];
@@ -1073,16 +1076,18 @@ template <int D, bool BOX, int RCAP>
 __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
                                           const uint32_t *__restrict__ qset, double tol, double inc, double max_dist,
                                           uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                          uint32_t *__restrict__ overflow) {
+                                          uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
+                                          double *__restrict__ out_M) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   uint32_t cnt = 0;
   double ce[RCAP];
   uint32_t cid[RCAP];
   RayTester<D> rt;
+  double M = max_dist;
+  bool over = false;  // more candidates than the list: redone by k_ray_band
   if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
     const double applied = (tol == 0) ? DBL_EPSILON : tol;
-    double M = max_dist;
     struct Item {
       uint32_t node;
       double enter;
@@ -1113,7 +1118,7 @@ __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const
         }
         if (!hit || en > M) continue;
         if (cnt == RCAP) {
-          atomicOr(overflow, 1u);
+          over = true;
         } else {
 #pragma unroll
           for (int t = 0; t < RCAP; ++t)
@@ -1162,6 +1167,10 @@ __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const
     }
     // descending enter distance, ties by id (selection into output order)
   }
+  if (over) {
+    band[atomicAdd(overflow, 1u)] = (uint32_t)qi;
+    out_M[qi] = M;
+  }
   ray_output<RCAP>(qi, cnt, ce, cid, out_ids, out_cnt);
 }
 
@@ -1173,7 +1182,8 @@ template <int D, bool BOX, int RCAP>
 __device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
                                                  const uint32_t *__restrict__ qset, double tol, double inc,
                                                  double max_dist, LdsLevelStack &S, uint32_t *__restrict__ out_ids,
-                                                 uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
+                                                 uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
+                                                 uint32_t *__restrict__ band, double *__restrict__ out_M) {
   constexpr int NC = 1 << D;
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
@@ -1181,9 +1191,10 @@ __device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi
   double ce[RCAP];
   uint32_t cid[RCAP];
   RayTester<D> rt;
+  double M = max_dist;
+  bool over = false;  // more candidates than the list: redone by k_ray_band
   if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
     const double applied = (tol == 0) ? DBL_EPSILON : tol;
-    double M = max_dist;
     double en, ex;
     int top = 0;
     uint32_t node = (uint32_t)sp.node_off;
@@ -1208,7 +1219,7 @@ __device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi
           }
           if (!hit || en > M) continue;
           if (cnt == RCAP) {
-            atomicOr(overflow, 1u);
+            over = true;
           } else {
 #pragma unroll
             for (int t = 0; t < RCAP; ++t)
@@ -1290,6 +1301,10 @@ __device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi
       if (!have && pc > 1) --top;  // the rest of this level enters later still
     }
   }
+  if (over) {
+    band[atomicAdd(overflow, 1u)] = (uint32_t)qi;
+    out_M[qi] = M;
+  }
   ray_output<RCAP>(qi, cnt, ce, cid, out_ids, out_cnt);
 }
 
@@ -1298,7 +1313,8 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
                                               const double *__restrict__ rays, const uint32_t *__restrict__ qset,
                                               const uint32_t *__restrict__ order, uint64_t nq, double tol, double inc,
                                               double max_dist, uint32_t *__restrict__ out_ids,
-                                              uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow) {
+                                              uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
+                                              uint32_t *__restrict__ band, double *__restrict__ out_M) {
   __shared__ uint32_t s_a[(D <= 3 ? kLdsStack : 1) * kQB];
   __shared__ uint32_t s_b[(D <= 3 ? kLdsStack : 1) * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
@@ -1307,10 +1323,68 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     if constexpr (D <= 3) {
       LdsLevelStack S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
-      ray_query_levels<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, S, out_ids, out_cnt, overflow);
+      ray_query_levels<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, S, out_ids, out_cnt, overflow, band,
+                                     out_M);
     } else {
-      ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, out_ids, out_cnt, overflow);
+      ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, out_ids, out_cnt, overflow, band, out_M);
     }
+  });
+}
+
+// Rays with more candidates than the list: a walk with the final bound M
+// collects every hit entering within it, as (enter, id), counted then written.
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_ray_band(const TreeView *__restrict__ views,
+                                                   const uint32_t *__restrict__ qview, const double *__restrict__ rays,
+                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ band,
+                                                   uint32_t nband, const double *__restrict__ Mq, double tol,
+                                                   double inc, uint32_t *__restrict__ counts,
+                                                   const uint32_t *__restrict__ offsets, double *__restrict__ out_e,
+                                                   uint32_t *__restrict__ out_id) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nband) return;
+  const uint64_t qi = band[ti];
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+    const double M = Mq[qi];
+    uint32_t cnt = 0;
+    const uint64_t o = FILL ? offsets[ti] : 0;
+    RayTester<D> rt;
+    if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
+      uint32_t stack[StackCap<D>::value];
+      int top = 0;
+      stack[top++] = (uint32_t)sp.node_off;
+      double en, ex;
+      while (top > 0) {
+        const uint32_t node = stack[--top];
+        if (!rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex) || en > M) continue;
+        const NodeInfo ni = tv.info[node];
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          bool hit;
+          if (BOX) {
+            double lo[D], hi[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) lo[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
+#pragma unroll
+            for (int d = 0; d < D; ++d) hi[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
+            hit = rt.box_hit(lo, hi, &en, &ex);
+          } else {
+            double pnt[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) pnt[d] = tv.geom[(uint64_t)d * tv.n + j];
+            hit = rt.hit_point(pnt, &en, &ex);
+          }
+          if (!hit || en > M) continue;
+          if (FILL) {
+            out_e[o + cnt] = en;
+            out_id[o + cnt] = tv.ids[j];
+          }
+          ++cnt;
+        }
+        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+      }
+    }
+    if (!FILL) counts[ti] = cnt;
   });
 }
 
@@ -1567,8 +1641,8 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   launch(ctx, "knn_band_fill", k_knn_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
          qs, (const uint32_t *)band.p, nb, (const double *)Fq.p, tol, (uint32_t *)nullptr, (const uint32_t *)boff.p,
          bd.p, bi.p);
-  launch(ctx, "knn_band_sort", k_sort_band, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, nb, bd.p,
-         bi.p);
+  launch(ctx, "knn_band_sort", k_sort_band<false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, nb,
+         bd.p, bi.p);
   launch(ctx, "knn_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
          (const uint32_t *)bcnt.p, cnt.p);
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
@@ -1705,18 +1779,38 @@ template <int D, bool BOX>
 static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol,
                         double inc, double max_dist) {
   constexpr int RCAP = 16;
-  Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1);
+  Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1), band(ctx, std::max<uint64_t>(nq, 1));
+  Buf<double> Mq(ctx, std::max<uint64_t>(nq, 1));
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
   launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p);
-  uint32_t hovf = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+         (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p);
+  uint32_t nb = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&nb, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (nb == 0) return fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
+  Buf<uint32_t> bcnt(ctx, nb), boff(ctx, nb + 1), btot(ctx, 1);
+  launch(ctx, "ray_band_count", k_ray_band<D, BOX, false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+         qs, (const uint32_t *)band.p, nb, (const double *)Mq.p, tol, inc, bcnt.p, (const uint32_t *)nullptr,
+         (double *)nullptr, (uint32_t *)nullptr);
+  exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p);
+  uint32_t hb = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
+  Buf<double> be(ctx, std::max<uint32_t>(hb, 1));
+  Buf<uint32_t> bi(ctx, std::max<uint32_t>(hb, 1));
+  launch(ctx, "ray_band_fill", k_ray_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+         qs, (const uint32_t *)band.p, nb, (const double *)Mq.p, tol, inc, (uint32_t *)nullptr,
+         (const uint32_t *)boff.p, be.p, bi.p);
+  launch(ctx, "ray_band_sort", k_sort_band<true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, nb,
+         be.p, bi.p);
+  launch(ctx, "ray_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
+         (const uint32_t *)bcnt.p, cnt.p);
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
-  if (hovf) {
-    delete r;
-    throw Error{OTG_ERR_UNSUPPORTED, "more first-hit candidates than the per-ray list"};
-  }
+  launch(ctx, "ray_band_copy", k_band_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
+         (const uint32_t *)boff.p, (const uint32_t *)bi.p, (const uint64_t *)r->d_offsets, r->d_ids);
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the staged count above lives on this stack frame
   return r;
 }
 

@@ -180,6 +180,42 @@ def test_ray_vs_oracle(gpu_ctx, kind, D):
     assert goff[2] - goff[1] == 0
 
 
+def test_ray_points_on_finer_index(gpu_ctx):
+    """Clustered points under a shallow max depth: leaves overflow, the build adds
+    the finer index, and point rays walk it; hits equal the oracle's on the
+    reference tree (2-D and 3-D level walks)."""
+    for D in (2, 3):
+        pts = gaussian_mixture(40000, D, 5, 0.0, 100.0, 0.5, 3.0, seed=50 + D)
+        g = _tree(gpu_ctx, "point", D, pts, None, 3, 8)
+        o = O.OracleTree(D, "point", pts, None, 3, 8)
+        org = pts[np.random.default_rng(D).integers(0, len(pts), 2000)] + 2.0
+        rays = np.concatenate([org, unit_dirs(2000, D, seed=60 + D)], axis=1)
+        for tol in (0.05, 0.3):
+            goff, gids = g.ray_first_batch(rays, tol).csr()
+            ooff, oids = o.ray_first(rays, tol, threads=8)
+            assert np.array_equal(goff, ooff) and np.array_equal(gids, oids), (D, tol)
+            assert len(gids) > 100
+
+
+def test_ray_band_longer_than_list(gpu_ctx):
+    """Points strung along the ray inside its tolerance cone: more first-hit
+    candidates than the per-ray list; the band pass returns them all, ordered."""
+    line = np.zeros((200, 3))
+    line[:, 0] = 10.0 + 0.01 * np.arange(200)
+    pts = np.concatenate([uniform_points(5000, 3, seed=71), line])
+    g = _tree(gpu_ctx, "point", 3, pts, None, 6, 8)
+    o = O.OracleTree(3, "point", pts, None, 6, 8)
+    rays = np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0, 1.0, 0.0]])
+    gpu_ctx.set_profiling(True)
+    gpu_ctx.reset_kernel_times()
+    goff, gids = g.ray_first_batch(rays, 0.5).csr()
+    kt = gpu_ctx.kernel_times()
+    gpu_ctx.set_profiling(False)
+    ooff, oids = o.ray_first(rays, 0.5, threads=8)
+    assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
+    assert goff[1] - goff[0] > 16 and "ray_band_fill" in kt, (goff, sorted(kt))
+
+
 # ---- collision -----------------------------------------------------------------------
 def test_collision_issue38(gpu_ctx):
     rng = MT19937(0)
