@@ -1510,14 +1510,17 @@ __global__ void k_offsets_u64(const uint32_t *__restrict__ scan, uint64_t n, uin
   if (i == n) off[n] = total;
 }
 
+// One thread per (query, slot): consecutive threads read consecutive slots and
+// write consecutive CSR positions, so both sides coalesce.
 __global__ void k_compact_fixed(const uint32_t *__restrict__ src, uint32_t stride, const uint64_t *__restrict__ off,
                                 uint64_t nq, uint32_t *__restrict__ dst) {
-  uint64_t qi = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t qi = x / stride;
+  const uint32_t t = (uint32_t)(x - qi * stride);
   if (qi >= nq) return;
-  uint64_t o = off[qi];
-  uint32_t c = (uint32_t)(off[qi + 1] - o);
-  if (c > stride) c = stride;  // longer lists (kNN bands) are written by their own pass
-  for (uint32_t t = 0; t < c; ++t) dst[o + t] = src[qi * stride + t];
+  const uint64_t o = off[qi];
+  // longer lists (kNN and ray bands) are written by their own pass
+  if (t < off[qi + 1] - o) dst[o + t] = src[x];
 }
 
 __global__ void k_split_pairs(const uint64_t *__restrict__ keys, uint64_t np, uint32_t jbits, uint32_t *__restrict__ ids,
@@ -1589,8 +1592,9 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
-  launch(ctx, "compact_fixed", k_compact_fixed, dim3(div_up(nq, 256)), dim3(256), 0, ids, stride,
-         (const uint64_t *)r->d_offsets, nq, r->d_ids);
+  if (stride)
+    launch(ctx, "compact_fixed", k_compact_fixed, dim3(div_up(nq * stride, 256)), dim3(256), 0, ids, stride,
+           (const uint64_t *)r->d_offsets, nq, r->d_ids);
   return r;
 }
 
