@@ -495,7 +495,8 @@ def make_served(nreq, seed=77):
             c = r.uniform(0.0, 1000.0, (n, 3))
             h = np.exp(r.normal(np.log(0.5), 0.6, (n, 3)))
             trees.append(("box", np.concatenate([c - h, c + h], axis=1), 32))
-    # Zipf(1) popularity over a random ranking of the trees; kNN goes to point and box trees
+    # Zipf(1) popularity over a random ranking of the trees; kNN and range go to point and box
+    # trees, rays to the box scenes (a ray meets a point only inside its tolerance cone)
     ranks = rng.permutation(64)
     pop = 1.0 / (1.0 + ranks)
     pop /= pop.sum()
@@ -509,7 +510,7 @@ def make_served(nreq, seed=77):
             pick = g[rng.integers(0, len(g), nq)]
             base = (pick if kind == "point" else 0.5 * (pick[:, :3] + pick[:, 3:])) + rng.normal(0.0, 1.0, (nq, 3))
             reqs.append((tid, "knn", base, int(rng.integers(1, 33))))
-        elif (kind == "point" and u < 0.75) or (kind == "box" and u < 0.67):
+        elif kind == "point" or (kind == "box" and u < 0.67):
             c = rng.uniform(0.0, 1000.0, (nq, 3))
             e = rng.uniform(1.0, 30.0, (nq, 1))
             reqs.append((tid, "range", np.concatenate([c - e / 2, c + e / 2], axis=1), 0))
@@ -667,7 +668,7 @@ def run_served(args, octree_amd):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)",
         "config": {"workload": "served stream: 64 trees (32 point sets 2e5-4e6 pts uniform/clustered, 32 box sets "
                                "1e5-2e6 AABBs log-normal extents), Zipf(1) tree popularity, batches log-uniform "
-                               "1-65,536 mixing kNN (k 1-32), range and rays; HBM budget "
+                               "1-65,536 mixing kNN (k 1-32) and range on all trees, rays on the box trees; HBM budget "
                                f"{args.budget_frac:.0%} of the working set", "requests": len(reqs), "queries": total_q,
                    "working_set_bytes": working, "budget_bytes": budget, "workers": 2},
         "saturation": sat, "open_loop": dict(op, offered_requests_per_s=offered),

@@ -1706,12 +1706,6 @@ static const Tree *range_index(const Tree *t) {
   return (t->knn_tree && t->kind == OTG_KIND_POINT) ? t->knn_tree : t;
 }
 
-// Point rays too: the first-hit set is the hits whose enter distance is within
-// the bound set by the nearest exits (ot_query.h:2085-2099) -- distances only.
-static const Tree *ray_index(const Tree *t) {
-  return (t->knn_tree && t->kind == OTG_KIND_POINT) ? t->knn_tree : t;
-}
-
 static double knn_F0(double max_within, double tol) {
   return (max_within == DBL_MAX) ? DBL_MAX : fmax(tol, max_within * (1.0 + tol));
 }
@@ -1837,8 +1831,10 @@ static Result *ray_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint
 
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol, double inc,
                   double max_dist) {
+  // Rays always walk the reference tree: each hit whose exit is below the bound
+  // assigns M = (1 + inc) exit + tol (ot_query.h:2093-2096), which can raise M, so the
+  // result depends on the order hits are met -- the reference tree's order.
   Ctx *ctx = run_ctx(t);
-  t = ray_index(t);
   DevInput in;
   stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
   return ray_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, tol, inc, max_dist);
@@ -1856,7 +1852,7 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
   uint64_t total_n = 0;
   for (const Tree *t : trees) {
     if (t->kind != kind || t->dim != dim) throw Error{OTG_ERR_INVALID_ARG, "trees of one launch differ in kind or dim"};
-    const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : ray_index(t));
+    const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : t);
     hv.push_back(view_of(x));
     vs.cbits = std::max(vs.cbits, x->layout.cbits);
     vs.sbits = std::max(vs.sbits, x->layout.sbits);

@@ -180,10 +180,11 @@ def test_ray_vs_oracle(gpu_ctx, kind, D):
     assert goff[2] - goff[1] == 0
 
 
-def test_ray_points_on_finer_index(gpu_ctx):
-    """Clustered points under a shallow max depth: leaves overflow, the build adds
-    the finer index, and point rays walk it; hits equal the oracle's on the
-    reference tree (2-D and 3-D level walks)."""
+def test_ray_points_clustered_wide_tolerance(gpu_ctx):
+    """Clustered points under a shallow max depth (over-full leaves) with wide
+    tolerance cones: each hit can raise the bound M, so the result depends on the
+    order hits are met; rays walk the reference tree nearest cell first, which is
+    the reference's best-first order for non-overlapping cells (2-D and 3-D)."""
     for D in (2, 3):
         pts = gaussian_mixture(40000, D, 5, 0.0, 100.0, 0.5, 3.0, seed=50 + D)
         g = _tree(gpu_ctx, "point", D, pts, None, 3, 8)
