@@ -1072,10 +1072,82 @@ __device__ __forceinline__ void ray_output(uint64_t qi, uint32_t cnt, double (&c
   }
 }
 
-template <int D, bool BOX, int RCAP>
-__device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
+// Node heaps for the ray walk: a binary min-heap on node enter distance with the
+// same push (sift up while smaller than the parent) and pop (last to root, sift
+// down towards the smaller child) as the oracle's, so equal enter distances come
+// out in the same order.  LdsHeap: lane-interleaved in LDS; ArrayHeap: private.
+constexpr int kRayHeap = 16;  // 16 x 12 B x 128 lanes = 24 KiB per block
+
+struct LdsHeap {
+  static constexpr int cap = kRayHeap;
+  uint32_t *n;
+  double *p;
+  __device__ __forceinline__ uint32_t &node(int i) { return n[i * kQB]; }
+  __device__ __forceinline__ double &prio(int i) { return p[i * kQB]; }
+};
+
+template <int D>
+struct ArrayHeap {
+  static constexpr int cap = 2 * StackCap<D>::value;
+  uint32_t nd[cap];
+  double pr[cap];
+  __device__ __forceinline__ uint32_t &node(int i) { return nd[i]; }
+  __device__ __forceinline__ double &prio(int i) { return pr[i]; }
+};
+
+template <typename Heap>
+__device__ __forceinline__ void heap_push(Heap &H, int &n, uint32_t node, double prio) {
+  int i = n++;
+  H.node(i) = node;
+  H.prio(i) = prio;
+  while (i > 0) {
+    const int par = (i - 1) / 2;
+    if (!(H.prio(i) < H.prio(par))) break;
+    const uint32_t tn = H.node(par);
+    const double tp = H.prio(par);
+    H.node(par) = H.node(i);
+    H.prio(par) = H.prio(i);
+    H.node(i) = tn;
+    H.prio(i) = tp;
+    i = par;
+  }
+}
+
+template <typename Heap>
+__device__ __forceinline__ void heap_pop(Heap &H, int &n, uint32_t *node, double *prio) {
+  *node = H.node(0);
+  *prio = H.prio(0);
+  --n;
+  H.node(0) = H.node(n);
+  H.prio(0) = H.prio(n);
+  int i = 0;
+  for (;;) {
+    const int l = 2 * i + 1, r = l + 1;
+    int m = i;
+    if (l < n && H.prio(l) < H.prio(m)) m = l;
+    if (r < n && H.prio(r) < H.prio(m)) m = r;
+    if (m == i) break;
+    const uint32_t tn = H.node(m);
+    const double tp = H.prio(m);
+    H.node(m) = H.node(i);
+    H.prio(m) = H.prio(i);
+    H.node(i) = tn;
+    H.prio(i) = tp;
+    i = m;
+  }
+}
+
+// RayIntersectedFirst (ot_query.h:2035-2122): best-first on node enter distance
+// (TraverseNodesByPriority :235-291), stopping at the first node entering beyond
+// the bound M; every hit child is queued (M may still rise before it is popped).
+// Each entity hit entering within M is a candidate; a hit whose exit is below M
+// sets M = (1 + inc) exit + tol, which can raise or lower it, so the result depends
+// on this visiting order -- the reference's.  Returns false (nothing written) when
+// the heap would overflow.
+template <int D, bool BOX, int RCAP, typename Heap>
+__device__ __forceinline__ bool ray_query(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
                                           const uint32_t *__restrict__ qset, double tol, double inc, double max_dist,
-                                          uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                          Heap &H, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
                                           uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                                           double *__restrict__ out_M) {
   const uint32_t s = qset ? qset[qi] : 0u;
@@ -1088,33 +1160,30 @@ __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const
   bool over = false;  // more candidates than the list: redone by k_ray_band
   if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
     const double applied = (tol == 0) ? DBL_EPSILON : tol;
-    struct Item {
-      uint32_t node;
-      double enter;
-    };
-    Item stack[StackCap<D>::value];
-    int top = 0;
+    int n = 0;
     double en, ex;
     const uint32_t root = (uint32_t)sp.node_off;
-    if (rt.hit_node(&tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &en, &ex)) stack[top++] = Item{root, en};
-    while (top > 0) {
-      const Item it = stack[--top];
-      if (it.enter > M) continue;
-      const NodeInfo ni = tv.info[it.node];
+    if (rt.hit_node(&tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &en, &ex)) heap_push(H, n, root, en);
+    while (n > 0) {
+      uint32_t node;
+      double prio;
+      heap_pop(H, n, &node, &prio);
+      if (prio > M) break;
+      const NodeInfo ni = tv.info[node];
       for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
         bool hit;
         if (BOX) {
-          double a[D], b[D];
+          double lo[D], hi[D];
 #pragma unroll
-          for (int d = 0; d < D; ++d) a[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
+          for (int d = 0; d < D; ++d) lo[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
 #pragma unroll
-          for (int d = 0; d < D; ++d) b[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
-          hit = rt.box_hit(a, b, &en, &ex);
+          for (int d = 0; d < D; ++d) hi[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
+          hit = rt.box_hit(lo, hi, &en, &ex);
         } else {
-          double p[D];
+          double pnt[D];
 #pragma unroll
-          for (int d = 0; d < D; ++d) p[d] = tv.geom[(uint64_t)d * tv.n + j];
-          hit = rt.hit_point(p, &en, &ex);
+          for (int d = 0; d < D; ++d) pnt[d] = tv.geom[(uint64_t)d * tv.n + j];
+          hit = rt.hit_point(pnt, &en, &ex);
         }
         if (!hit || en > M) continue;
         if (cnt == RCAP) {
@@ -1146,159 +1215,12 @@ __device__ __forceinline__ void ray_query(const TreeView &tv, uint64_t qi, const
         }
         cnt = w;
       }
-      if (ni.ccount) {
-        constexpr int CMAX = 1 << D;
-        Item ch[CMAX];
-        int nc = 0;
-        for (uint32_t c = 0; c < ni.ccount; ++c) {
-          const uint32_t x = ni.cbegin + c;
-          if (!rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &en, &ex)) continue;
-          if (en > M) continue;
-          int p = nc;
-          while (p > 0 && ch[p - 1].enter < en) {
-            ch[p] = ch[p - 1];
-            --p;
-          }
-          ch[p] = Item{x, en};
-          ++nc;
-        }
-        for (int c = 0; c < nc; ++c) stack[top++] = ch[c];
+      for (uint32_t c = 0; c < ni.ccount; ++c) {
+        const uint32_t x = ni.cbegin + c;
+        if (!rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &en, &ex)) continue;
+        if (n == Heap::cap) return false;
+        heap_push(H, n, x, en);
       }
-    }
-    // descending enter distance, ties by id (selection into output order)
-  }
-  if (over) {
-    band[atomicAdd(overflow, 1u)] = (uint32_t)qi;
-    out_M[qi] = M;
-  }
-  ray_output<RCAP>(qi, cnt, ce, cid, out_ids, out_cnt);
-}
-
-// Ray walk for D <= 3 with level entries (as the kNN walk): per level, the
-// pending children packed in ascending enter distance; a pop recomputes the
-// child's hit and, when its enter exceeds the current bound M, drops the level
-// (the rest enter later still).  Same visits and candidates as ray_query.
-template <int D, bool BOX, int RCAP>
-__device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
-                                                 const uint32_t *__restrict__ qset, double tol, double inc,
-                                                 double max_dist, LdsLevelStack &S, uint32_t *__restrict__ out_ids,
-                                                 uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
-                                                 uint32_t *__restrict__ band, double *__restrict__ out_M) {
-  constexpr int NC = 1 << D;
-  const uint32_t s = qset ? qset[qi] : 0u;
-  const SetParams &sp = tv.sets[s];
-  uint32_t cnt = 0;
-  double ce[RCAP];
-  uint32_t cid[RCAP];
-  RayTester<D> rt;
-  double M = max_dist;
-  bool over = false;  // more candidates than the list: redone by k_ray_band
-  if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
-    const double applied = (tol == 0) ? DBL_EPSILON : tol;
-    double en, ex;
-    int top = 0;
-    uint32_t node = (uint32_t)sp.node_off;
-    NodeInfo ni = tv.info[node];
-    bool have = rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex) && !(en > M);
-    for (;;) {
-      if (have) {
-        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-          bool hit;
-          if (BOX) {
-            double lo[D], hi[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) lo[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
-#pragma unroll
-            for (int d = 0; d < D; ++d) hi[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
-            hit = rt.box_hit(lo, hi, &en, &ex);
-          } else {
-            double pnt[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) pnt[d] = tv.geom[(uint64_t)d * tv.n + j];
-            hit = rt.hit_point(pnt, &en, &ex);
-          }
-          if (!hit || en > M) continue;
-          if (cnt == RCAP) {
-            over = true;
-          } else {
-#pragma unroll
-            for (int t = 0; t < RCAP; ++t)
-              if ((uint32_t)t == cnt) {
-                ce[t] = en;
-                cid[t] = tv.ids[j];
-              }
-            ++cnt;
-          }
-          if (ex >= M) continue;
-          M = (1.0 + inc) * ex + applied;
-          uint32_t w = 0;
-#pragma unroll
-          for (int t = 0; t < RCAP; ++t) {
-            if ((uint32_t)t < cnt && !(ce[t] > M)) {
-#pragma unroll
-              for (int u = 0; u < RCAP; ++u)
-                if ((uint32_t)u == w) {
-                  ce[u] = ce[t];
-                  cid[u] = cid[t];
-                }
-              ++w;
-            }
-          }
-          cnt = w;
-        }
-        if (ni.ccount) {
-          double w[NC];
-          uint32_t ix[NC];
-          uint32_t nc = 0;
-#pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            ix[c] = (uint32_t)c;
-            w[c] = __builtin_inf();
-            if ((uint32_t)c < ni.ccount) {
-              const uint32_t x = ni.cbegin + c;
-              double cen, cex;
-              if (rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &cen, &cex) && !(cen > M)) {
-                w[c] = fmax(cen, -DBL_MAX);
-                ++nc;
-              }
-            }
-          }
-          if (NC == 4) {
-            cas(w[0], ix[0], w[1], ix[1]); cas(w[2], ix[2], w[3], ix[3]);
-            cas(w[0], ix[0], w[2], ix[2]); cas(w[1], ix[1], w[3], ix[3]);
-            cas(w[1], ix[1], w[2], ix[2]);
-          } else {
-            cas(w[0], ix[0], w[1], ix[1]); cas(w[2], ix[2], w[3], ix[3]);
-            cas(w[4], ix[4], w[5], ix[5]); cas(w[6], ix[6], w[7], ix[7]);
-            cas(w[0], ix[0], w[2], ix[2]); cas(w[1], ix[1], w[3], ix[3]);
-            cas(w[4], ix[4], w[6], ix[6]); cas(w[5], ix[5], w[7], ix[7]);
-            cas(w[1], ix[1], w[2], ix[2]); cas(w[5], ix[5], w[6], ix[6]);
-            cas(w[0], ix[0], w[4], ix[4]); cas(w[1], ix[1], w[5], ix[5]);
-            cas(w[2], ix[2], w[6], ix[6]); cas(w[3], ix[3], w[7], ix[7]);
-            cas(w[2], ix[2], w[4], ix[4]); cas(w[3], ix[3], w[5], ix[5]);
-            cas(w[1], ix[1], w[2], ix[2]); cas(w[3], ix[3], w[4], ix[4]);
-            cas(w[5], ix[5], w[6], ix[6]);
-          }
-          if (nc) {
-            uint32_t packed = nc << 28;
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-              if ((uint32_t)c < nc) packed |= ix[c] << (D * c);
-            S.cb[top * kQB] = ni.cbegin;
-            S.pk[top * kQB] = packed;
-            ++top;
-          }
-        }
-      }
-      if (top == 0) break;
-      const uint32_t packed = S.pk[(top - 1) * kQB];
-      const uint32_t pc = packed >> 28;
-      node = S.cb[(top - 1) * kQB] + (packed & (NC - 1));
-      ni = tv.info[node];
-      if (pc == 1) --top;
-      else S.pk[(top - 1) * kQB] = ((pc - 1) << 28) | ((packed & 0x0FFFFFFFu) >> D);
-      have = rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex) && !(en > M);
-      if (!have && pc > 1) --top;  // the rest of this level enters later still
     }
   }
   if (over) {
@@ -1306,6 +1228,7 @@ __device__ __forceinline__ void ray_query_levels(const TreeView &tv, uint64_t qi
     out_M[qi] = M;
   }
   ray_output<RCAP>(qi, cnt, ce, cid, out_ids, out_cnt);
+  return true;
 }
 
 template <int D, bool BOX, int RCAP>
@@ -1314,20 +1237,36 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
                                               const uint32_t *__restrict__ order, uint64_t nq, double tol, double inc,
                                               double max_dist, uint32_t *__restrict__ out_ids,
                                               uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
-                                              uint32_t *__restrict__ band, double *__restrict__ out_M) {
-  __shared__ uint32_t s_a[(D <= 3 ? kLdsStack : 1) * kQB];
-  __shared__ uint32_t s_b[(D <= 3 ? kLdsStack : 1) * kQB];
+                                              uint32_t *__restrict__ band, double *__restrict__ out_M,
+                                              uint32_t *__restrict__ redo, uint32_t *__restrict__ redo_n) {
+  __shared__ uint32_t s_n[kRayHeap * kQB];
+  __shared__ double s_p[kRayHeap * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
+  LdsHeap H{&s_n[threadIdx.x], &s_p[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    if constexpr (D <= 3) {
-      LdsLevelStack S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
-      ray_query_levels<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, S, out_ids, out_cnt, overflow, band,
-                                     out_M);
-    } else {
-      ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, out_ids, out_cnt, overflow, band, out_M);
-    }
+    if (!ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M))
+      redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
+  });
+}
+
+// Rays whose LDS heap would overflow, with the large private heap.
+template <int D, bool BOX, int RCAP>
+__global__ void __launch_bounds__(kQB) k_ray_deep(const TreeView *__restrict__ views,
+                                                   const uint32_t *__restrict__ qview, const double *__restrict__ rays,
+                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ list,
+                                                   uint32_t nlist, double tol, double inc, double max_dist,
+                                                   uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                                   uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
+                                                   double *__restrict__ out_M, uint32_t *__restrict__ fail) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nlist) return;
+  const uint64_t qi = list[ti];
+  ArrayHeap<D> H;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    if (!ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M))
+      atomicOr(fail, 1u);
   });
 }
 
@@ -1777,15 +1716,24 @@ template <int D, bool BOX>
 static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol,
                         double inc, double max_dist) {
   constexpr int RCAP = 16;
-  Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, std::max<uint64_t>(nq, 1)), ovf(ctx, 1), band(ctx, std::max<uint64_t>(nq, 1));
-  Buf<double> Mq(ctx, std::max<uint64_t>(nq, 1));
-  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));
+  const uint64_t nq1 = std::max<uint64_t>(nq, 1);
+  Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, nq1), ovf(ctx, 3), band(ctx, nq1), redo(ctx, nq1);
+  Buf<double> Mq(ctx, nq1);
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 12, ctx->stream));  // [0] band count, [1] redo count, [2] deep failure
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
   launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p);
-  uint32_t nb = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&nb, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+         (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p, redo.p, ovf.p + 1);
+  uint32_t hovf[3] = {0, 0, 0};
+  OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (hovf[1]) {
+    launch(ctx, "ray_deep", k_ray_deep<D, BOX, RCAP>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+           qs, (const uint32_t *)redo.p, hovf[1], tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p, ovf.p + 2);
+    OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (hovf[2]) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
+  }
+  const uint32_t nb = hovf[0];
   if (nb == 0) return fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
   Buf<uint32_t> bcnt(ctx, nb), boff(ctx, nb + 1), btot(ctx, 1);
   launch(ctx, "ray_band_count", k_ray_band<D, BOX, false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
