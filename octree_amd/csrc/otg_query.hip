@@ -1072,10 +1072,10 @@ __device__ __forceinline__ void ray_output(uint64_t qi, uint32_t cnt, double (&c
   }
 }
 
-// Node heaps for the ray walk: a binary min-heap on node enter distance with the
-// same push (sift up while smaller than the parent) and pop (last to root, sift
-// down towards the smaller child) as the oracle's, so equal enter distances come
-// out in the same order.  LdsHeap: lane-interleaved in LDS; ArrayHeap: private.
+// Node heaps for the ray walk: a binary min-heap on node enter distance; push
+// sifts up while smaller than the parent, pop moves the last entry to the root and
+// sifts it down towards the smaller child, so equal enter distances leave in a
+// fixed order.  LdsHeap: lane-interleaved in LDS; ArrayHeap: private.
 constexpr int kRayHeap = 16;  // 16 x 12 B x 128 lanes = 24 KiB per block
 
 struct LdsHeap {
@@ -1270,33 +1270,41 @@ __global__ void __launch_bounds__(kQB) k_ray_deep(const TreeView *__restrict__ v
   });
 }
 
-// Rays with more candidates than the list: a walk with the final bound M
-// collects every hit entering within it, as (enter, id), counted then written.
-template <int D, bool BOX, bool FILL>
-__global__ void __launch_bounds__(kQB) k_ray_band(const TreeView *__restrict__ views,
-                                                   const uint32_t *__restrict__ qview, const double *__restrict__ rays,
-                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ band,
-                                                   uint32_t nband, const double *__restrict__ Mq, double tol,
-                                                   double inc, uint32_t *__restrict__ counts,
-                                                   const uint32_t *__restrict__ offsets, double *__restrict__ out_e,
-                                                   uint32_t *__restrict__ out_id) {
+// Rays with more candidates than the register list are replayed exactly (same
+// best-first walk, same bound updates) with a candidate list of `cap` entries in
+// global memory; a replay that outgrows it sets its count to UINT32_MAX and is run
+// again with a larger list.  The candidates end sorted by descending enter, ties by id.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kQB) k_ray_replay(const TreeView *__restrict__ views,
+                                                     const uint32_t *__restrict__ qview,
+                                                     const double *__restrict__ rays, const uint32_t *__restrict__ qset,
+                                                     const uint32_t *__restrict__ list, uint32_t nlist, double tol,
+                                                     double inc, double max_dist, uint32_t cap,
+                                                     double *__restrict__ cand_e, uint32_t *__restrict__ cand_id,
+                                                     uint32_t *__restrict__ counts, uint32_t *__restrict__ fail) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (ti >= nband) return;
-  const uint64_t qi = band[ti];
+  if (ti >= nlist) return;
+  const uint64_t qi = list[ti];
+  double *ce = cand_e + ti * cap;
+  uint32_t *cid = cand_id + ti * cap;
+  ArrayHeap<D> H;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
-    const double M = Mq[qi];
     uint32_t cnt = 0;
-    const uint64_t o = FILL ? offsets[ti] : 0;
+    bool too_many = false;
     RayTester<D> rt;
     if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
-      uint32_t stack[StackCap<D>::value];
-      int top = 0;
-      stack[top++] = (uint32_t)sp.node_off;
+      const double applied = (tol == 0) ? DBL_EPSILON : tol;
+      double M = max_dist;
+      int n = 0;
       double en, ex;
-      while (top > 0) {
-        const uint32_t node = stack[--top];
-        if (!rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex) || en > M) continue;
+      const uint32_t root = (uint32_t)sp.node_off;
+      if (rt.hit_node(&tv.nmin[(uint64_t)root * D], &tv.nsize[(uint64_t)root * D], &en, &ex)) heap_push(H, n, root, en);
+      while (n > 0 && !too_many) {
+        uint32_t node;
+        double prio;
+        heap_pop(H, n, &node, &prio);
+        if (prio > M) break;
         const NodeInfo ni = tv.info[node];
         for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
           bool hit;
@@ -1314,17 +1322,78 @@ __global__ void __launch_bounds__(kQB) k_ray_band(const TreeView *__restrict__ v
             hit = rt.hit_point(pnt, &en, &ex);
           }
           if (!hit || en > M) continue;
-          if (FILL) {
-            out_e[o + cnt] = en;
-            out_id[o + cnt] = tv.ids[j];
+          if (cnt == cap) {
+            too_many = true;
+            break;
           }
+          ce[cnt] = en;
+          cid[cnt] = tv.ids[j];
           ++cnt;
+          if (ex >= M) continue;
+          M = (1.0 + inc) * ex + applied;
+          uint32_t w = 0;
+          for (uint32_t t = 0; t < cnt; ++t)
+            if (!(ce[t] > M)) {
+              ce[w] = ce[t];
+              cid[w] = cid[t];
+              ++w;
+            }
+          cnt = w;
         }
-        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+        for (uint32_t c = 0; c < ni.ccount && !too_many; ++c) {
+          const uint32_t x = ni.cbegin + c;
+          if (!rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &en, &ex)) continue;
+          if (n == ArrayHeap<D>::cap) {
+            atomicOr(fail, 1u);
+            too_many = true;
+            break;
+          }
+          heap_push(H, n, x, en);
+        }
       }
     }
-    if (!FILL) counts[ti] = cnt;
+    if (too_many) {
+      counts[ti] = 0xFFFFFFFFu;
+      return;
+    }
+    counts[ti] = cnt;
+    // heap sort: descending enter, ties by id
+    auto sift = [&](uint32_t i, uint32_t m) {
+      for (;;) {
+        uint32_t l = 2 * i + 1, r = l + 1, x = i;
+        if (l < m && pair_less<true>(ce[x], cid[x], ce[l], cid[l])) x = l;
+        if (r < m && pair_less<true>(ce[x], cid[x], ce[r], cid[r])) x = r;
+        if (x == i) return;
+        const double td = ce[i];
+        ce[i] = ce[x];
+        ce[x] = td;
+        const uint32_t tid = cid[i];
+        cid[i] = cid[x];
+        cid[x] = tid;
+        i = x;
+      }
+    };
+    for (uint32_t i = cnt / 2; i-- > 0;) sift(i, cnt);
+    for (uint32_t m = cnt; m > 1; --m) {
+      const double td = ce[0];
+      ce[0] = ce[m - 1];
+      ce[m - 1] = td;
+      const uint32_t tid = cid[0];
+      cid[0] = cid[m - 1];
+      cid[m - 1] = tid;
+      sift(0, m - 1);
+    }
   });
+}
+
+// Copies replayed candidate lists (stride cap) into their CSR slots.
+__global__ void k_replay_copy(const uint32_t *__restrict__ band, uint32_t nband, uint32_t cap,
+                              const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ cid,
+                              const uint64_t *__restrict__ off, uint32_t *__restrict__ dst) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nband) return;
+  const uint64_t o = off[band[i]];
+  for (uint32_t t = 0; t < bcnt[i]; ++t) dst[o + t] = cid[i * cap + t];
 }
 
 // ---------------------------------------------------------------------------
@@ -1735,28 +1804,34 @@ static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   }
   const uint32_t nb = hovf[0];
   if (nb == 0) return fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
-  Buf<uint32_t> bcnt(ctx, nb), boff(ctx, nb + 1), btot(ctx, 1);
-  launch(ctx, "ray_band_count", k_ray_band<D, BOX, false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
-         qs, (const uint32_t *)band.p, nb, (const double *)Mq.p, tol, inc, bcnt.p, (const uint32_t *)nullptr,
-         (double *)nullptr, (uint32_t *)nullptr);
-  exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p);
-  uint32_t hb = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
-  Buf<double> be(ctx, std::max<uint32_t>(hb, 1));
-  Buf<uint32_t> bi(ctx, std::max<uint32_t>(hb, 1));
-  launch(ctx, "ray_band_fill", k_ray_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
-         qs, (const uint32_t *)band.p, nb, (const double *)Mq.p, tol, inc, (uint32_t *)nullptr,
-         (const uint32_t *)boff.p, be.p, bi.p);
-  launch(ctx, "ray_band_sort", k_sort_band<true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, nb,
-         be.p, bi.p);
+  // replay the overflowing rays with a global candidate list, growing it until all fit
+  Buf<uint32_t> bcnt(ctx, nb), fail(ctx, 1);
+  uint32_t cap = 256;
+  Buf<double> ce;
+  Buf<uint32_t> ci;
+  for (;;) {
+    ce = Buf<double>(ctx, (uint64_t)nb * cap);
+    ci = Buf<uint32_t>(ctx, (uint64_t)nb * cap);
+    OTG_HIP_CHECK(hipMemsetAsync(fail.p, 0, 4, ctx->stream));
+    launch(ctx, "ray_replay", k_ray_replay<D, BOX>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
+           (const uint32_t *)band.p, nb, tol, inc, max_dist, cap, ce.p, ci.p, bcnt.p, fail.p);
+    std::vector<uint32_t> h(nb);
+    uint32_t hfail = 0;
+    OTG_HIP_CHECK(hipMemcpyAsync(h.data(), bcnt.p, (uint64_t)nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipMemcpyAsync(&hfail, fail.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (hfail) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
+    bool all = true;
+    for (uint32_t v : h) all &= v != 0xFFFFFFFFu;
+    if (all) break;
+    if ((uint64_t)cap * 8 > (1ull << 26)) throw Error{OTG_ERR_UNSUPPORTED, "more first-hit candidates than supported"};
+    cap *= 8;
+  }
   launch(ctx, "ray_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
          (const uint32_t *)bcnt.p, cnt.p);
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
-  launch(ctx, "ray_band_copy", k_band_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
-         (const uint32_t *)boff.p, (const uint32_t *)bi.p, (const uint64_t *)r->d_offsets, r->d_ids);
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the staged count above lives on this stack frame
+  launch(ctx, "ray_replay_copy", k_replay_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb, cap,
+         (const uint32_t *)bcnt.p, (const uint32_t *)ci.p, (const uint64_t *)r->d_offsets, r->d_ids);
   return r;
 }
 

@@ -200,7 +200,7 @@ def test_ray_points_clustered_wide_tolerance(gpu_ctx):
 
 def test_ray_band_longer_than_list(gpu_ctx):
     """Points strung along the ray inside its tolerance cone: more first-hit
-    candidates than the per-ray list; the band pass returns them all, ordered."""
+    candidates than the per-ray list; the exact replay with a larger list returns them."""
     line = np.zeros((200, 3))
     line[:, 0] = 10.0 + 0.01 * np.arange(200)
     pts = np.concatenate([uniform_points(5000, 3, seed=71), line])
@@ -214,7 +214,7 @@ def test_ray_band_longer_than_list(gpu_ctx):
     gpu_ctx.set_profiling(False)
     ooff, oids = o.ray_first(rays, 0.5, threads=8)
     assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
-    assert goff[1] - goff[0] > 16 and "ray_band_fill" in kt, (goff, sorted(kt))
+    assert goff[1] - goff[0] > 16 and "ray_replay" in kt, (goff, sorted(kt))
 
 
 # ---- collision -----------------------------------------------------------------------
