@@ -381,18 +381,19 @@ def run_single(args, octree_amd, torch):
         tree = octree_amd.StaticOctreePoint(ctx=ctx)
         tb, _ = _timed(ctx, lambda: tree.Create(d_pts, 10, box, 16), reps)
         tk, _ = _timed(ctx, lambda: tree.knn_batch(d_q, 8), reps)
-        t0 = time.perf_counter()
-        o = O.OracleTree(3, "point", pts, box, 10, 16)
-        cb = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        o.knn(qs, 8, threads=threads)
-        ck = time.perf_counter() - t0
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
                     "config": {"workload": "OctreePoint 1,000,000 pts U[0,100)^3, depth 10, cap 16; 10,000 kNN k=8"},
                     "build_ms": tb * 1e3, "build_mpts_s": 1.0 / tb, "knn_ms": tk * 1e3,
-                    "node_count": tree.GetNodeCount(),
-                    "cpu_baseline": {"value": len(qs) / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
-                                     "sample": "whole workload", "build_mpts_s": 1.0 / cb, "build_cores": 1}})
+                    "node_count": tree.GetNodeCount()})
+        if not args.no_cpu:
+            t0 = time.perf_counter()
+            o = O.OracleTree(3, "point", pts, box, 10, 16)
+            cb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.knn(qs, 8, threads=threads)
+            ck = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": len(qs) / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
+                                   "sample": "whole workload", "build_mpts_s": 1.0 / cb, "build_cores": 1}
     elif args.workload == "octree_box_4m":
         rng = np.random.default_rng(12)
         n = 4_000_000
@@ -411,23 +412,24 @@ def run_single(args, octree_amd, torch):
         npairs = len(tree.collide())
         # CPU: oracle build of a 400k-box sample, its collision, and 100k rays on it
         m = 400_000
-        t0 = time.perf_counter()
-        o = O.OracleTree(3, "box", boxes[:m], None, 8, 32)
-        cb = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        o.ray_first(rays, threads=threads)
-        cr = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        o.collide()
-        cc = time.perf_counter() - t0
         out.update({"value": len(rays) / tr, "unit": "RayIntersectedFirst rays/s", "ms_per_step": (tb + tr + tcol) * 1e3,
                     "config": {"workload": "OctreeBox 4,000,000 AABBs centres U[0,1000)^3, half-extent lognormal(ln0.5,0.6), "
                                            "depth 8, cap 32; self CollisionDetection + 100,000 rays"},
                     "build_ms": tb * 1e3, "build_mboxes_s": n / tb / 1e6, "ray_ms": tr * 1e3,
-                    "collision_ms": tcol * 1e3, "collision_pairs": npairs, "collision_boxes_s": n / tcol,
-                    "cpu_baseline": {"value": len(rays) / cr, "unit": "rays/s", "cores": threads, "kind": "port",
-                                     "sample": f"first {m} boxes; 100,000 rays", "build_mboxes_s": m / cb / 1e6,
-                                     "collision_boxes_s_single_thread": m / cc}})
+                    "collision_ms": tcol * 1e3, "collision_pairs": npairs, "collision_boxes_s": n / tcol})
+        if not args.no_cpu:
+            t0 = time.perf_counter()
+            o = O.OracleTree(3, "box", boxes[:m], None, 8, 32)
+            cb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.ray_first(rays, threads=threads)
+            cr = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.collide()
+            cc = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": len(rays) / cr, "unit": "rays/s", "cores": threads, "kind": "port",
+                                   "sample": f"first {m} boxes; 100,000 rays", "build_mboxes_s": m / cb / 1e6,
+                                   "collision_boxes_s_single_thread": m / cc}
     else:  # point6d_8m
         rng = np.random.default_rng(13)
         k = 32
@@ -456,18 +458,19 @@ def run_single(args, octree_amd, torch):
         tk, _ = _timed(ctx, lambda: tree.knn_batch(d_q, 16), reps)
         # CPU: oracle build of a 1M-point sample and 20k queries on it
         m = 1_000_000
-        t0 = time.perf_counter()
-        o = O.OracleTree(6, "point", pts[:m], box, 8, 32)
-        cb = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        o.knn(qs[:20_000], 16, threads=threads)
-        ck = time.perf_counter() - t0
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
                     "config": {"workload": "TreePointND<6> 8,000,000 pts 32-cluster mixture in [0,1]^6, depth 8, cap 32; "
                                            "1,000,000 kNN k=16"},
-                    "build_ms": tb * 1e3, "build_mpts_s": 8.0 / tb, "knn_ms": tk * 1e3,
-                    "cpu_baseline": {"value": 20_000 / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
-                                     "sample": f"tree of the first {m} points; 20,000 queries", "build_mpts_s": 1.0 / cb}})
+                    "build_ms": tb * 1e3, "build_mpts_s": 8.0 / tb, "knn_ms": tk * 1e3})
+        if not args.no_cpu:
+            t0 = time.perf_counter()
+            o = O.OracleTree(6, "point", pts[:m], box, 8, 32)
+            cb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.knn(qs[:20_000], 16, threads=threads)
+            ck = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": 20_000 / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
+                                   "sample": f"tree of the first {m} points; 20,000 queries", "build_mpts_s": 1.0 / cb}
     kt = None
     return out
 
