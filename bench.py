@@ -312,7 +312,8 @@ def main():
     traffic = pmc_traffic(dom, f"batched cap={cap}")
     out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": algo,
-                       "avg_launch_ms": avg_ms}
+                       "avg_launch_ms": avg_ms,
+                       "traffic_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9}
 
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     if not args.no_cpu and not args.profile_only:
