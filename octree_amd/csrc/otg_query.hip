@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
       LdsStack S{&s_n[threadIdx.x], reinterpret_cast<float *>(&s_w[threadIdx.x])};
       ok = knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
     }
-    if (!ok) redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
+    if (!ok) redo[atomicAdd(redo_n, 1u)] = (uint32_t)ti;  // position in the query order
   });
 }
 
@@ -591,13 +591,14 @@ template <int D, int KCAP, bool BOX>
 __global__ void __launch_bounds__(kQB) k_knn_deep(const TreeView *__restrict__ views,
                                                    const uint32_t *__restrict__ qview, const double *__restrict__ qpts,
                                                    const uint32_t *__restrict__ qset, const uint32_t *__restrict__ qk,
+                                                   const uint32_t *__restrict__ order,
                                                    const uint32_t *__restrict__ list, uint32_t nlist, uint32_t k,
                                                    double F0, double tol, uint32_t *__restrict__ out_ids,
                                                    uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
                                                    uint32_t *__restrict__ band, double *__restrict__ out_F) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nlist) return;
-  const uint64_t qi = list[ti];
+  const uint64_t qi = order ? order[list[ti]] : list[ti];
   const uint32_t kq = qk ? qk[qi] : k;
   ArrayStack<D> S;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
@@ -1247,7 +1248,7 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
   LdsHeap H{&s_n[threadIdx.x], &s_p[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     if (!ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M))
-      redo[atomicAdd(redo_n, 1u)] = (uint32_t)qi;
+      redo[atomicAdd(redo_n, 1u)] = (uint32_t)ti;  // position in the query order
   });
 }
 
@@ -1255,14 +1256,15 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
 template <int D, bool BOX, int RCAP>
 __global__ void __launch_bounds__(kQB) k_ray_deep(const TreeView *__restrict__ views,
                                                    const uint32_t *__restrict__ qview, const double *__restrict__ rays,
-                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ list,
-                                                   uint32_t nlist, double tol, double inc, double max_dist,
-                                                   uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
-                                                   uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
-                                                   double *__restrict__ out_M, uint32_t *__restrict__ fail) {
+                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ order,
+                                                   const uint32_t *__restrict__ list, uint32_t nlist, double tol,
+                                                   double inc, double max_dist, uint32_t *__restrict__ out_ids,
+                                                   uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
+                                                   uint32_t *__restrict__ band, double *__restrict__ out_M,
+                                                   uint32_t *__restrict__ fail) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nlist) return;
-  const uint64_t qi = list[ti];
+  const uint64_t qi = order ? order[list[ti]] : list[ti];
   ArrayHeap<D> H;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     if (!ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M))
@@ -1607,6 +1609,25 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
 }
 
 // footprint (optional, measurement): out[0..1] += node and entity visits of the minimal walk
+__global__ void k_widen_u32(const uint32_t *__restrict__ in, uint64_t n, uint64_t *__restrict__ keys,
+                            uint32_t *__restrict__ vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    keys[i] = in[i];
+    vals[i] = in[i];
+  }
+}
+
+// Sorts a list of query-order positions appended by atomics, so a redo pass walks
+// its queries in the (set, Morton) order again.
+static void sort_positions(Ctx *ctx, uint32_t *list, uint32_t n, uint64_t nq) {
+  if (n < 2) return;
+  Buf<uint64_t> keys(ctx, n);
+  launch(ctx, "redo_widen", k_widen_u32, dim3(div_up(n, 256)), dim3(256), 0, (const uint32_t *)list, (uint64_t)n,
+         keys.p, list);
+  radix_sort_pairs(ctx, keys.p, list, n, bit_width32((uint32_t)std::max<uint64_t>(nq - 1, 1)));
+}
+
 template <int D, int KCAP, bool BOX>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
                         const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
@@ -1620,8 +1641,10 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 8, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   if (hovf[1]) {
+    sort_positions(ctx, redo.p, hovf[1], nq);
     launch(ctx, "knn_deep", k_knn_deep<D, KCAP, BOX>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
-           qs, qk, (const uint32_t *)redo.p, hovf[1], k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
+           qs, qk, (const uint32_t *)order.p, (const uint32_t *)redo.p, hovf[1], k, F0, tol, ids.p, cnt.p, ovf.p,
+           band.p, Fq.p);
     OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   }
@@ -1796,8 +1819,10 @@ static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   if (hovf[1]) {
+    sort_positions(ctx, redo.p, hovf[1], nq);
     launch(ctx, "ray_deep", k_ray_deep<D, BOX, RCAP>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
-           qs, (const uint32_t *)redo.p, hovf[1], tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p, ovf.p + 2);
+           qs, (const uint32_t *)order.p, (const uint32_t *)redo.p, hovf[1], tol, inc, max_dist, ids.p, cnt.p, ovf.p,
+           band.p, Mq.p, ovf.p + 2);
     OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
     OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (hovf[2]) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
