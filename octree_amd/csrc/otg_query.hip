@@ -273,12 +273,9 @@ __device__ __forceinline__ void knn_scan_box(const TreeView &tv, const double (&
   }
 }
 
-// Traversal stacks.  Entries are (node, squared wall distance at push time).
-// ArrayStack: private memory with the worst-case depth (root + maxDepth x
-// (2^D - 1) pending siblings).  LdsStack: a short stack in LDS, lane-interleaved
-// so a wavefront's accesses to one depth fall in distinct banks; the wall distance
-// is kept as a float rounded down, so a pop prunes only nodes the exact value
-// would prune too (an unpruned extra node only costs its distance tests).
+// Traversal stack for the generic walk: entries (node, squared wall distance at
+// push time) in private memory with the worst-case depth (root + maxDepth x
+// (2^D - 1) pending siblings).
 template <int D>
 struct ArrayStack {
   static constexpr int cap = StackCap<D>::value;
@@ -294,17 +291,6 @@ struct ArrayStack {
 
 constexpr int kLdsStack = 24;  // entries per lane: 24 x 8 B x 128 lanes = 24 KiB per block
 
-struct LdsStack {
-  static constexpr int cap = kLdsStack;
-  uint32_t *n;
-  float *w;
-  __device__ __forceinline__ uint32_t node(int i) const { return n[i * kQB]; }
-  __device__ __forceinline__ double wall(int i) const { return (double)w[i * kQB]; }
-  __device__ __forceinline__ void set(int i, uint32_t nd, double wl) {
-    n[i * kQB] = nd;
-    w[i * kQB] = __double2float_rd(wl);
-  }
-};
 
 // One lane per query (queries sorted by (set, Morton code) so neighbouring lanes
 // walk nearby nodes); depth-first with the nearest child popped first.  Stack
@@ -383,7 +369,7 @@ template <int D, int KCAP>
 __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D], uint32_t j, uint32_t je,
                                          double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &L, uint32_t k,
                                          double tol) {
-  constexpr int U = 4;
+  constexpr int U = D <= 3 ? 4 : (D == 4 ? 2 : 1);  // entities loaded ahead (register budget)
   for (; j + U <= je; j += U) {
     double d2[U];
 #pragma unroll
@@ -555,8 +541,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
 }
 
 // views/qview: the trees of this launch and each query's tree (qview null: one tree);
-// qk: per-query k (null: k for all).  LDS short stack; a query whose stack would
-// overflow is appended to redo[] for k_knn_deep.
+// qk: per-query k (null: k for all).
 template <int D, int KCAP, bool BOX>
 __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                               const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
@@ -564,45 +549,25 @@ __global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views,
                                               uint64_t nq, uint32_t k, double F0, double tol,
                                               uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
                                               uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
-                                              double *__restrict__ out_F, uint32_t *__restrict__ redo,
-                                              uint32_t *__restrict__ redo_n) {
-  __shared__ uint32_t s_n[kLdsStack * kQB];
-  __shared__ uint32_t s_w[kLdsStack * kQB];
+                                              double *__restrict__ out_F) {
+  // 2-D/3-D: level entries in LDS, one per tree level.  4-D..6-D: up to 2^D - 1
+  // pending siblings per level outgrow any short stack, so the walk keeps the
+  // worst-case private stack.  Neither can overflow.
+  static_assert(kLdsStack >= kMaxDepth + 1, "one level entry per depth");
+  __shared__ uint32_t s_n[(D <= 3 ? kLdsStack : 1) * kQB];
+  __shared__ uint32_t s_w[(D <= 3 ? kLdsStack : 1) * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
   const uint32_t kq = qk ? qk[qi] : k;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    bool ok;
     if constexpr (D <= 3) {
       LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
-      ok = knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band,
-                                          out_F);
+      knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
     } else {
-      LdsStack S{&s_n[threadIdx.x], reinterpret_cast<float *>(&s_w[threadIdx.x])};
-      ok = knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
+      ArrayStack<D> S;
+      knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
     }
-    if (!ok) redo[atomicAdd(redo_n, 1u)] = (uint32_t)ti;  // position in the query order
-  });
-}
-
-// The queries whose short stack overflowed, with the worst-case private stack.
-template <int D, int KCAP, bool BOX>
-__global__ void __launch_bounds__(kQB) k_knn_deep(const TreeView *__restrict__ views,
-                                                   const uint32_t *__restrict__ qview, const double *__restrict__ qpts,
-                                                   const uint32_t *__restrict__ qset, const uint32_t *__restrict__ qk,
-                                                   const uint32_t *__restrict__ order,
-                                                   const uint32_t *__restrict__ list, uint32_t nlist, uint32_t k,
-                                                   double F0, double tol, uint32_t *__restrict__ out_ids,
-                                                   uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
-                                                   uint32_t *__restrict__ band, double *__restrict__ out_F) {
-  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (ti >= nlist) return;
-  const uint64_t qi = order ? order[list[ti]] : list[ti];
-  const uint32_t kq = qk ? qk[qi] : k;
-  ArrayStack<D> S;
-  with_tree(views, qview, qi, [&](const TreeView &tv) {
-    knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
   });
 }
 
@@ -1632,22 +1597,14 @@ template <int D, int KCAP, bool BOX>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
                         const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
-  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 2), redo(ctx, nq), band(ctx, nq);
+  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1), band(ctx, nq);
   Buf<double> Fq(ctx, nq);
-  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 8, ctx->stream));  // [0] band count, [1] redo count
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));  // band count
   launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
-         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p, redo.p, ovf.p + 1);
-  uint32_t hovf[2] = {0, 0};
-  OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+         (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
+  uint32_t hovf[1] = {0};
+  OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  if (hovf[1]) {
-    sort_positions(ctx, redo.p, hovf[1], nq);
-    launch(ctx, "knn_deep", k_knn_deep<D, KCAP, BOX>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
-           qs, qk, (const uint32_t *)order.p, (const uint32_t *)redo.p, hovf[1], k, F0, tol, ids.p, cnt.p, ovf.p,
-           band.p, Fq.p);
-    OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  }
   if (footprint) {
     Buf<unsigned long long> acc(ctx, 2);
     OTG_HIP_CHECK(hipMemsetAsync(acc.p, 0, 16, ctx->stream));

@@ -263,11 +263,11 @@ def test_forest_queries_match_per_set_trees(gpu_ctx):
             assert sorted(rids[roff[qi]:roff[qi + 1]].tolist()) == sorted(oids[ooff[j]:ooff[j + 1]].tolist())
 
 
-# ---- short stacks: level entries bound the 2-D/3-D walks; 6-D kNN falls back ------
+# ---- stack-heavy walks ------------------------------------------------------------------
 def test_short_stack_paths(gpu_ctx):
-    """3-D kNN and all range walks keep one stack entry per tree level (never
-    overflow); a 6-D kNN walk whose LDS short stack overflows is redone with the
-    worst-case private stack.  The ids are the oracle's either way."""
+    """Walks that keep many siblings pending: far-away 3-D kNN with a large k and
+    ranges cutting nodes at every level (one LDS entry per level), and 6-D kNN
+    with up to 63 pending siblings per level (private worst-case stack)."""
     import octree_amd
 
     ctx = octree_amd.Context(0)
@@ -294,13 +294,8 @@ def test_short_stack_paths(gpu_ctx):
     g6 = _tree(ctx, "point", 6, p6, None, 4, 2)
     o6 = O.OracleTree(6, "point", p6, None, 4, 2)
     q6 = np.concatenate([uniform_points(100, 6, seed=95) + 300.0, uniform_points(200, 6, seed=96)])
-    ctx.set_profiling(True)
-    ctx.reset_kernel_times()
     got6 = g6.knn_batch(q6, 32).csr()
-    kt = ctx.kernel_times()
-    ctx.set_profiling(False)
     exp6 = o6.knn(q6, 32, threads=8)
     assert np.array_equal(got6[0], exp6[0]) and np.array_equal(got6[1], exp6[1])
-    assert "knn_deep" in kt, sorted(kt)
     g6.free()
     ctx.close()
