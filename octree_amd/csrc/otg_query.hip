@@ -543,7 +543,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
 // views/qview: the trees of this launch and each query's tree (qview null: one tree);
 // qk: per-query k (null: k for all).
 template <int D, int KCAP, bool BOX>
-__global__ void __launch_bounds__(kQB) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+__global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                               const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                                               const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
                                               uint64_t nq, uint32_t k, double F0, double tol,
