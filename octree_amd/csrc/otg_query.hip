@@ -274,18 +274,19 @@ __device__ __forceinline__ void knn_scan_box(const TreeView &tv, const double (&
 }
 
 // Traversal stack for the generic walk: entries (node, squared wall distance at
-// push time) in private memory with the worst-case depth (root + maxDepth x
-// (2^D - 1) pending siblings).
+// push time, as a float rounded down) in private memory with the worst-case depth
+// (root + maxDepth x (2^D - 1) pending siblings).  An entry the rounding leaves
+// unpruned only costs its distance tests.
 template <int D>
 struct ArrayStack {
   static constexpr int cap = StackCap<D>::value;
   uint32_t n[cap];
-  double w[cap];
+  float w[cap];  // rounded down: a pop prunes only what the exact value prunes
   __device__ __forceinline__ uint32_t node(int i) const { return n[i]; }
-  __device__ __forceinline__ double wall(int i) const { return w[i]; }
+  __device__ __forceinline__ double wall(int i) const { return (double)w[i]; }
   __device__ __forceinline__ void set(int i, uint32_t nd, double wl) {
     n[i] = nd;
-    w[i] = wl;
+    w[i] = __double2float_rd(wl);
   }
 };
 
