@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <vector>
 
 #include "otg/orthotree_gpu.hpp"
@@ -116,6 +117,19 @@ int main() {
     auto one = op.GetNearestNeighbors(q[i], 8, p3);
     EXPECT(std::vector<index_t>(br[i].begin(), br[i].end()) == one);
   }
+  // box-tree kNN with MinMaxDistance (general.tests.cpp:2648, :2772, :2788)
+  STEP("box knn");
+  std::vector<BoundingBox2D> mm = {{{+0.20, 0.00}, {+2.31, 2.31}}, {{+1.00, 0.00}, {+1.01, 1.41}},
+                                   {{+0.40, 0.00}, {+0.89, 0.89}}, {{+2.10, 0.00}, {+3.66, 3.66}},
+                                   {{-2.21, 0.00}, {-0.20, 2.21}}, {{-2.50, 0.00}, {-0.90, 2.47}}};
+  StaticQuadtreeBox qm(PAR_EXEC, mm, 3, std::nullopt, 2);
+  EXPECT(same_set(qm.GetNearestNeighbors({0.0, 0.0}, 1, std::numeric_limits<double>::max(), mm,
+                                         std::numeric_limits<double>::epsilon()),
+                  {0, 2, 4}));
+  std::vector<BoundingBox2D> b3 = {{{0.0, 0.0}, {1.0, 1.0}}, {{1.0, 1.0}, {3.0, 3.0}}, {{4.0, 4.0}, {5.0, 5.0}}};
+  StaticQuadtreeBox q3(PAR_EXEC, b3, 3, std::nullopt, 2);
+  EXPECT(same_set(q3.GetNearestNeighbors({2.0, 2.0}, 1, b3), {1}));
+  EXPECT(same_set(q3.GetNearestNeighbors({-1.0, 1.0}, 1, b3), {0}));
   if (failures == 0) std::printf("DROPIN OK\n");
   return failures ? 1 : 0;
 }
