@@ -775,11 +775,34 @@ struct LdsLevelIter {
   __device__ __forceinline__ uint8_t &left(int i) { return b[i * kQB]; }
 };
 
+// A contiguous run of ids a range result takes whole (a contained subtree, or
+// 0..n-1 for the root shortcut when src is null).  Runs longer than kInlineCopy are
+// copied afterwards by k_range_copy, one wavefront per run, instead of one lane.
+struct RangeCopy {
+  uint64_t dst;
+  const uint32_t *src;
+  uint32_t len;
+  uint32_t pad;
+};
+constexpr uint32_t kInlineCopy = 16;
+
+template <bool FILL>
+__device__ __forceinline__ void range_take(uint32_t *__restrict__ out, uint64_t dst, const uint32_t *src, uint32_t len,
+                                           uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
+  if (len > kInlineCopy) {
+    const uint32_t i = atomicAdd(ncopy, 1u);
+    if (FILL) copies[i] = RangeCopy{dst, src, len, 0u};
+  } else if (FILL) {
+    for (uint32_t t = 0; t < len; ++t) out[dst + t] = src ? src[t] : t;
+  }
+}
+
 template <int D, bool BOX, bool FILL>
 __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
                                             const uint32_t *__restrict__ qset, int overlap_mode, double tol,
                                             LdsLevelIter &stack, uint32_t *__restrict__ counts,
-                                            const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
+                                            const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
+                                            uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double rmin[D], rmax[D];
@@ -804,8 +827,7 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
     for (int d = 0; d < D; ++d) vol *= rmax[d] - rmin[d];
     if (root_in) {
       cnt = (uint32_t)sp.ent_cnt;  // iota over the entity span (:488-512)
-      if (FILL)
-        for (uint32_t i = 0; i < cnt; ++i) out[o + i] = i;
+      range_take<FILL>(out, o, nullptr, cnt, ncopy, copies);
     } else if (vol > 0.0) {
       double rsz[D];
 #pragma unroll
@@ -830,8 +852,7 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
         if (overlap && contained) {
           // whole subtree: its entities are the contiguous storage range
           const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
-          if (FILL)
-            for (uint32_t j = e0; j < e1; ++j) out[o + cnt + (j - e0)] = tv.ids[j];
+          range_take<FILL>(out, o + cnt, tv.ids + e0, e1 - e0, ncopy, copies);
           cnt += e1 - e0;
         } else if (overlap) {
           for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
@@ -893,7 +914,8 @@ __global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ view
                                                 const double *__restrict__ ranges, const uint32_t *__restrict__ qset,
                                                 const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode,
                                                 double tol, uint32_t *__restrict__ counts,
-                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out) {
+                                                const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
+                                                uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
   __shared__ uint32_t s_a[kRangeStack * kQB];
   __shared__ uint8_t s_b[kRangeStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
@@ -901,8 +923,17 @@ __global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ view
   const uint64_t qi = order ? order[ti] : ti;
   LdsLevelIter S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out);
+    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out, ncopy, copies);
   });
+}
+
+// One wavefront per deferred run of a range result.
+__global__ void __launch_bounds__(256) k_range_copy(const RangeCopy *__restrict__ copies, uint32_t n,
+                                                     uint32_t *__restrict__ out) {
+  const uint32_t w = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x & 63;
+  if (w >= n) return;
+  const RangeCopy c = copies[w];
+  for (uint32_t t = lane; t < c.len; t += 64) out[c.dst + t] = c.src ? c.src[t] : t;
 }
 
 // ---------------------------------------------------------------------------
@@ -1715,24 +1746,32 @@ template <int D, bool BOX>
 static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, int mode,
                           double tol) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
-  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
+  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1)), ncopy(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
   launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr);
+         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
+         (RangeCopy *)nullptr);
   Result *r = new Result();
   r->ctx = ctx;
   r->nq = nq;
   Buf<uint32_t> scan(ctx, nq + 1), total(ctx, 1);
   exclusive_scan_u32(ctx, cnt.p, scan.p, nq, total.p);
-  uint32_t ht = 0;
+  uint32_t ht = 0, hcopy = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
+  Buf<RangeCopy> copies(ctx, std::max<uint32_t>(hcopy, 1));
+  OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
   launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
+         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
+         ncopy.p, copies.p);
+  launch(ctx, "range_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
+         r->d_ids);
   return r;
 }
 
