@@ -270,24 +270,60 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restric
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
+  // per digit: waves' offsets inside the tile's run, and the tile's run start
+  // (exclusive scan of the tile's digit counts over the 256 digits)
+  __shared__ uint32_t tstart[256];
+  uint32_t run = 0;
   {
-    uint32_t t = threadIdx.x;
-    uint32_t run = 0;
+    const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) {
-      uint32_t c = wcnt[w][t];
+      const uint32_t c = wcnt[w][t];
       wcnt[w][t] = run;
       run += c;
     }
   }
+  {
+    __shared__ uint32_t wsum[kBlock / 64];
+    // block-wide exclusive scan of `run` (one digit per thread) in digit order
+    uint32_t x = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < wave; ++w) add += wsum[w];
+    tstart[threadIdx.x] = add + x - run;
+  }
   __syncthreads();
+  // stage the tile in digit order in LDS, then write each digit's run with
+  // consecutive threads so the global stores coalesce
+  __shared__ uint64_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
-    uint64_t i = tile + (uint64_t)r * 64 + lane;
+    const uint64_t i = tile + (uint64_t)r * 64 + lane;
     if (i < n) {
-      uint32_t pos = base_off[dig[r]] + wcnt[wave][dig[r]] + off[r];
-      kout[pos] = key[r];
-      vout[pos] = val[r];
+      const uint32_t lp = tstart[dig[r]] + wcnt[wave][dig[r]] + off[r];
+      skey[lp] = key[r];
+      sval[lp] = val[r];
+    }
+  }
+  __syncthreads();
+  const uint64_t tile0 = (uint64_t)blockIdx.x * kTile;
+  const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)kTile ? (n - tile0) : (uint64_t)kTile);
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t j = (uint32_t)r * kBlock + threadIdx.x;
+    if (j < cnt) {
+      const uint64_t k = skey[j];
+      const uint32_t d = (uint32_t)(k >> shift) & 255u;
+      const uint32_t pos = base_off[d] + (j - tstart[d]);
+      kout[pos] = k;
+      vout[pos] = sval[j];
     }
   }
 }
