@@ -290,6 +290,46 @@ __device__ __forceinline__ uint64_t upper_bound_u64(const uint64_t *a, uint64_t 
   return lo;
 }
 
+// Bytes of x below n, as their high bits (every byte and n below 128): the per-byte
+// subtraction (x | 0x80) - n never borrows across bytes, so each byte is exact.
+__device__ __forceinline__ uint64_t bytes_below(uint64_t x, uint32_t n) {
+  const uint64_t H = 0x8080808080808080ull;
+  return ~((x | H) - 0x0101010101010101ull * (uint64_t)n) & H;
+}
+
+// Number of consecutive levels v[p], v[p+1], ... that are >= vi (stops at the first
+// smaller one), counted 8 bytes per load; may overshoot `limit` by up to 7.  The
+// walk ends at a set boundary (level 0) at the latest, and v is padded to 8 bytes.
+__device__ __forceinline__ uint32_t run_right(const uint8_t *__restrict__ v, uint64_t p, uint32_t vi,
+                                              uint32_t limit) {
+  uint32_t c = 0;
+  while (c < limit) {
+    const uint64_t base = p & ~7ull;
+    const uint32_t o = (uint32_t)(p - base);
+    const uint64_t m = bytes_below(*reinterpret_cast<const uint64_t *>(v + base), vi) & (~0ull << (8 * o));
+    if (m) return c + ((uint32_t)__builtin_ctzll(m) >> 3) - o;
+    c += 8 - o;
+    p = base + 8;
+  }
+  return c;
+}
+
+// Same, walking down from p.
+__device__ __forceinline__ uint32_t run_left(const uint8_t *__restrict__ v, uint64_t p, uint32_t vi, uint32_t limit) {
+  uint32_t c = 0;
+  while (c < limit) {
+    const uint64_t base = p & ~7ull;
+    const uint32_t o = (uint32_t)(p - base);
+    const uint64_t keep = (o == 7) ? ~0ull : ((1ull << (8 * (o + 1))) - 1ull);
+    const uint64_t m = bytes_below(*reinterpret_cast<const uint64_t *>(v + base), vi) & keep;
+    if (m) return c + o - ((63u - (uint32_t)__builtin_clzll(m)) >> 3);
+    c += o + 1;
+    if (base == 0) return c;
+    p = base - 1;
+  }
+  return c;
+}
+
 template <bool BOX>
 __global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ v,
                                                 uint64_t n, KeyView kv, const SetParams *__restrict__ sets,
@@ -304,18 +344,11 @@ __global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys
     const uint32_t L = vi - 1;
     if (L < sp.max_depth) {
       const uint64_t cap = sp.cap;
-      uint64_t cnt = 2;
-      // walk left over boundaries i-1, i-2, ... and right over i+1, ...
-      uint64_t j = i - 1;
-      while (cnt <= cap && v[j] >= vi) {
-        ++cnt;
-        --j;
-      }
-      j = i + 1;
-      while (cnt <= cap && v[j] >= vi) {
-        ++cnt;
-        ++j;
-      }
+      // entities of the run around boundary i: the two beside it plus one per
+      // neighbouring boundary with a level >= vi, on either side
+      const uint32_t lim = (uint32_t)(cap < 0xFFFFFFF0ull ? cap : 0xFFFFFFF0ull);
+      uint64_t cnt = 2 + (uint64_t)run_left(v, i - 1, vi, lim);
+      if (cnt <= cap) cnt += run_right(v, i + 1, vi, lim);
       if (cnt > cap) {
         a = 1;
         if (BOX && keys[i - 1] == keys[i]) {
@@ -817,7 +850,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   radix_sort_pairs(ctx, t->d_keys, vals.p, n, L.total);
 
   // 3. levels (+ per-set minimum level), root flags
-  Buf<uint8_t> v(ctx, n + 1);
+  Buf<uint8_t> v(ctx, n + 9);  // levels v[0..n]; 8 bytes of padding for word-wise scans
   launch(ctx, "levels", k_levels<D>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys, n,
          (const uint64_t *)d_off.p, nsets, t->d_sets, kv, v.p);
   launch(ctx, "root_flags", k_root_flags<BOX>, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)t->d_keys,
