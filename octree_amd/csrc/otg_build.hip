@@ -693,10 +693,9 @@ __global__ void k_mark_roots(const SetParams *__restrict__ sets, uint32_t nsets,
 }
 
 // leaves holding more than `cap` entities (they sit at max depth or are all non-refinable)
-// Largest leaf of each set (maxleaf[s]) and whether any leaf is over capacity.
 __global__ void __launch_bounds__(kB) k_oversized(const NodeInfo *__restrict__ info, uint64_t nnodes,
                                                    const SetParams *__restrict__ sets, uint32_t nsets,
-                                                   uint32_t *__restrict__ flag, uint32_t *__restrict__ maxleaf) {
+                                                   uint32_t *__restrict__ flag) {
   uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (x >= nnodes) return;
   NodeInfo ni = info[x];
@@ -707,10 +706,7 @@ __global__ void __launch_bounds__(kB) k_oversized(const NodeInfo *__restrict__ i
     if (sets[mid].node_off <= x) lo = mid;
     else hi = mid;
   }
-  if (ni.ecount > sets[lo].cap) {
-    atomicOr(flag, 1u);
-    atomicMax(&maxleaf[lo], ni.ecount);
-  }
+  if (ni.ecount > sets[lo].cap) atomicOr(flag, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -955,14 +951,14 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
     bool deeper = false;
     for (const SetParams &sp : t->sets) deeper |= sp.max_depth < fmax;
     if (deeper) {
-      Buf<uint32_t> flag(ctx, 1 + nsets);  // [0] any leaf over capacity, [1 + s] largest such leaf of set s
-      OTG_HIP_CHECK(hipMemsetAsync(flag.p, 0, 4 * (1 + nsets), ctx->stream));
+      Buf<uint32_t> flag(ctx, 1);
+      OTG_HIP_CHECK(hipMemsetAsync(flag.p, 0, 4, ctx->stream));
       launch(ctx, "oversized", k_oversized, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
-             (const SetParams *)t->d_sets, nsets, flag.p, flag.p + 1);
-      std::vector<uint32_t> hflag(1 + nsets);
-      OTG_HIP_CHECK(hipMemcpyAsync(hflag.data(), flag.p, 4 * (1 + nsets), hipMemcpyDeviceToHost, ctx->stream));
+             (const SetParams *)t->d_sets, nsets, flag.p);
+      uint32_t hflag = 0;
+      OTG_HIP_CHECK(hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
       OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      if (hflag[0]) {
+      if (hflag) {
         std::vector<double> sboxes((size_t)nsets * 2 * D);
         std::vector<uint32_t> fdepth(nsets);
         for (uint32_t s = 0; s < nsets; ++s) {
@@ -970,12 +966,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
             sboxes[(size_t)s * 2 * D + d] = t->sets[s].space_min[d];
             sboxes[(size_t)s * 2 * D + D + d] = t->sets[s].space_max[d];
           }
-          // deep enough to split the largest leaf into leaves of about cap entities (one
-          // level of margin for uneven density); sets without over-full leaves keep M
-          const uint32_t M = t->sets[s].max_depth, big = hflag[1 + s], cap = std::max(t->sets[s].cap, 1u);
-          uint32_t extra = 0;
-          if (big > cap) extra = (uint32_t)std::ceil(std::log2((double)big / (double)cap) / (double)D) + 1u;
-          fdepth[s] = std::max(M, std::min(fmax, M + extra));
+          fdepth[s] = std::max(t->sets[s].max_depth, fmax);
         }
         Tree *aux = new Tree();
         aux->ctx = ctx;
