@@ -1460,7 +1460,7 @@ __global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q
                                                      const uint32_t *__restrict__ qset, uint64_t nq,
                                                      const TreeView *__restrict__ views,
                                                      const uint32_t *__restrict__ qview, uint32_t cbits,
-                                                     uint32_t vshift, uint64_t *__restrict__ keys,
+                                                     uint32_t vshift, uint32_t drop, uint64_t *__restrict__ keys,
                                                      uint32_t *__restrict__ idx) {
   uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= nq) return;
@@ -1475,7 +1475,9 @@ __global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q
     const double x = (q[i * stride + d] - sp.space_min[d]) * sp.raster[d];
     g[d] = (x > 0.0) ? (uint32_t)(x < top ? x : top) : 0u;
   }
-  keys[i] = ((uint64_t)v << vshift) | ((uint64_t)s << cbits) | morton_encode<D>(g);
+  // the code aligned to the widest tree of the launch, then its leading bits kept
+  const uint32_t up = cbits + drop - D * M;
+  keys[i] = ((uint64_t)v << vshift) | ((uint64_t)s << cbits) | ((morton_encode<D>(g) << up) >> drop);
   idx[i] = (uint32_t)i;
 }
 
@@ -1498,12 +1500,16 @@ static Buf<uint32_t> query_order(Ctx *ctx, const ViewSet &vs, const double *q, u
     if (nq) OTG_HIP_CHECK(hipMemsetAsync(idx.p, 0, 4, ctx->stream));
     return idx;
   }
+  // the leading 30 bits of the Morton code (10 levels in 3-D) give the wavefronts
+  // their locality; the order decides speed only, never results
+  const uint32_t drop = vs.cbits > 30 ? vs.cbits - 30 : 0;
+  const uint32_t cbits = vs.cbits - drop;
   const uint32_t vbits = bit_width32(vs.nviews - 1);
-  const uint32_t vshift = vs.cbits + vs.sbits;
+  const uint32_t vshift = cbits + vs.sbits;
   if (vshift + vbits > 64) throw Error{OTG_ERR_UNSUPPORTED, "query ordering key exceeds 64 bits"};
   Buf<uint64_t> keys(ctx, nq);
   launch(ctx, "query_keys", k_query_keys<D>, dim3(div_up(nq, 256)), dim3(256), 0, q, stride, qs, nq, vs.views,
-         vs.qview, vs.cbits, vshift, keys.p, idx.p);
+         vs.qview, cbits, vshift, drop, keys.p, idx.p);
   radix_sort_pairs(ctx, keys.p, idx.p, nq, vshift + vbits);
   return idx;
 }
