@@ -46,6 +46,12 @@ def lib() -> C.CDLL:
     L.oc_ray_first.argtypes = [P, P, C.c_uint64, P, C.c_double, C.c_double, C.c_double, C.c_int]
     L.oc_collision.restype = P
     L.oc_collision.argtypes = [P, P, C.c_double]
+    L.oc_collision_trees.restype = P
+    L.oc_collision_trees.argtypes = [P, P, P, P, C.c_double]
+    L.oc_pick.restype = P
+    L.oc_pick.argtypes = [P, P, C.c_uint64, P, C.c_double, C.c_int]
+    L.oc_ray_all.restype = P
+    L.oc_ray_all.argtypes = [P, P, C.c_uint64, P, C.c_double, C.c_double, C.c_double, C.c_int]
     L.oc_result_nids.restype = C.c_uint64
     L.oc_result_nids.argtypes = [P]
     L.oc_result_nq.restype = C.c_uint64
@@ -157,6 +163,20 @@ class OracleTree:
     def collide(self, tol=BASE_TOLERANCE):
         off, ids = _result(lib().oc_collision(self.h, _p(self.geom), float(tol)))
         return ids.reshape(-1, 2)
+
+    def collide_with(self, other: "OracleTree", tol=BASE_TOLERANCE):
+        """Two-tree CollisionDetection: (self id, other id) pairs, sorted."""
+        off, ids = _result(lib().oc_collision_trees(self.h, _p(self.geom), other.h, _p(other.geom), float(tol)))
+        return ids.reshape(-1, 2)
+
+    def pick(self, points, tol=BASE_TOLERANCE, threads=1):
+        q = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, self.dim))
+        return _result(lib().oc_pick(self.h, _p(self.geom), q.shape[0], _p(q), float(tol), int(threads)))
+
+    def ray_all(self, rays, tol=BASE_TOLERANCE, inc=0.0, max_dist=DBL_MAX, threads=1):
+        q = np.ascontiguousarray(np.asarray(rays, np.float64).reshape(-1, 2 * self.dim))
+        return _result(lib().oc_ray_all(self.h, _p(self.geom), q.shape[0], _p(q), float(tol), float(inc),
+                                        float(max_dist), int(threads)))
 
 
 def lists(off, ids):

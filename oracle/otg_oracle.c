@@ -956,6 +956,80 @@ static void oc_range_one(const oc_tree *t, const double *geom, const double *ran
   }
 }
 
+/* internal_geometry_module.h:372-396 DoesBoxContainPointAD(box, point, tol) */
+static int oc_box_contains_point(const double *mn, const double *mx, const double *p, int D, double tol) {
+  if (tol != 0.0) {
+    for (int d = 0; d < D; ++d)
+      if (!(mn[d] - tol < p[d] && p[d] < mx[d] + tol)) return 0;
+  } else {
+    for (int d = 0; d < D; ++d)
+      if (!(mn[d] <= p[d] && p[d] <= mx[d])) return 0;
+  }
+  return 1;
+}
+
+/* internal_geometry_module.h:398-422 DoesBoxContainPointAD(minPoint, size, point, tol):
+ * centre form |p - min - size/2| against size/2 (+ tol). */
+static int oc_node_contains_point(const double *mn, const double *sz, const double *p, int D, double tol) {
+  for (int d = 0; d < D; ++d) {
+    double half = sz[d] * 0.5;
+    double dist = fabs(p[d] - mn[d] - half);
+    if (tol != 0.0) {
+      if (dist >= half + tol) return 0;
+    } else if (dist > half) {
+      return 0;
+    }
+  }
+  return 1;
+}
+
+/* ot_query.h:426-466 PickSearch: root box test (GetNodeBox, min/max form), then
+ * DFS (last child popped first) skipping nodes whose box does not contain the
+ * pick point; entities: points ArePointsEqual (general.h:218-221, Distance2 <=
+ * tol^2), boxes DoesBoxContainPoint (general.h:225-243).  Output in DFS order. */
+static void oc_pick_one(const oc_tree *t, const double *geom, const double *p, double tol, oc_vec *out, uint32_t **stack,
+                        size_t *scap) {
+  int D = t->dim;
+  if (t->node_count == 0) return;
+  double rmx[OC_MAXD];
+  for (int d = 0; d < D; ++d) rmx[d] = oc_nmin(t, 0)[d] + oc_nsize(t, 0)[d];
+  if (!oc_box_contains_point(oc_nmin(t, 0), rmx, p, D, tol)) return;
+  size_t sp = 0;
+  if (*scap < 64) {
+    *scap = 64;
+    *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+  }
+  (*stack)[sp++] = 0;
+  while (sp) {
+    uint32_t node = (*stack)[--sp];
+    if (!oc_node_contains_point(oc_nmin(t, node), oc_nsize(t, node), p, D, tol)) continue;
+    for (uint32_t j = t->node_ebegin[node]; j < t->node_ebegin[node] + t->node_ecount[node]; ++j) {
+      uint32_t id = t->entity_ids[j];
+      int ok;
+      if (t->is_box) {
+        const double *e = &geom[(size_t)id * 2 * D];
+        ok = oc_box_contains_point(e, e + D, p, D, tol);
+      } else {
+        const double *e = &geom[(size_t)id * D];
+        double d2 = 0.0;
+        for (int d = 0; d < D; ++d) {
+          double x = e[d] - p[d];
+          d2 += x * x;
+        }
+        ok = d2 <= tol * tol;
+      }
+      if (ok) vec_push(out, id);
+    }
+    for (uint32_t c = t->node_cbegin[node]; c < t->node_cbegin[node] + t->node_ccount[node]; ++c) {
+      if (sp == *scap) {
+        *scap *= 2;
+        *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+      }
+      (*stack)[sp++] = c;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Ray first hit                                                              */
 /* ------------------------------------------------------------------------- */
@@ -1105,6 +1179,59 @@ static size_t oc_ray_one(const oc_tree *t, const double *geom, const double *o, 
   return ncand;
 }
 
+static int cand_cmp_asc(const void *a, const void *b) {
+  const oc_cand *x = (const oc_cand *)a, *y = (const oc_cand *)b;
+  if (x->enter != y->enter) return x->enter < y->enter ? -1 : 1;
+  return x->id < y->id ? -1 : (x->id > y->id);
+}
+
+/* ot_query.h:1929-2012 RayIntersectedAll<SHOULD_SORT = true>: DFS skipping nodes
+ * the ray misses (Hit(minPoint, size)); every entity hit whose enter distance is
+ * within maxExaminationDistance (or any, when it is 0) is kept with its enter
+ * distance; the list is sorted ascending by enter distance (ties by id here: the
+ * reference's std::ranges::sort leaves them unspecified). */
+static size_t oc_ray_all_one(const oc_tree *t, const double *geom, const double *o, const double *dir, double tol, double inc,
+                             double max_dist, oc_vec *out, uint32_t **stack, size_t *scap) {
+  int D = t->dim;
+  oc_ray r;
+  if (t->node_count == 0 || !oc_ray_make(&r, o, dir, D, tol, inc)) return 0;
+  size_t ncand = 0, ccap = 16;
+  oc_cand *cand = (oc_cand *)malloc(ccap * sizeof(oc_cand));
+  size_t sp = 0;
+  if (*scap < 64) {
+    *scap = 64;
+    *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+  }
+  (*stack)[sp++] = 0;
+  double en, ex;
+  while (sp) {
+    uint32_t node = (*stack)[--sp];
+    if (!oc_hit_node(&r, oc_nmin(t, node), oc_nsize(t, node), D, &en, &ex)) continue;
+    for (uint32_t j = t->node_ebegin[node]; j < t->node_ebegin[node] + t->node_ecount[node]; ++j) {
+      uint32_t id = t->entity_ids[j];
+      int hit = t->is_box ? oc_hit_box(&r, &geom[(size_t)id * 2 * D], D, &en, &ex) : oc_hit_point(&r, &geom[(size_t)id * D], D, &en, &ex);
+      if (!hit) continue;
+      if (!(max_dist == 0 || en <= max_dist)) continue;
+      if (ncand == ccap) {
+        ccap *= 2;
+        cand = (oc_cand *)realloc(cand, ccap * sizeof(oc_cand));
+      }
+      cand[ncand++] = (oc_cand){id, en};
+    }
+    for (uint32_t c = t->node_cbegin[node]; c < t->node_cbegin[node] + t->node_ccount[node]; ++c) {
+      if (sp == *scap) {
+        *scap *= 2;
+        *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+      }
+      (*stack)[sp++] = c;
+    }
+  }
+  qsort(cand, ncand, sizeof(oc_cand), cand_cmp_asc);
+  for (size_t i = 0; i < ncand; ++i) vec_push(out, cand[i].id);
+  free(cand);
+  return ncand;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Collision detection (boxes, self)                                          */
 /* ------------------------------------------------------------------------- */
@@ -1139,10 +1266,14 @@ static uint32_t *sorted_node_entities(const oc_tree *t, const double *geom, uint
   return v;
 }
 
-/* ot_query.h:1307-1458 two-tree CollisionDetection (same tree, two nodes) */
-static void oc_collide_two_nodes(const oc_tree *t, const double *geom, uint32_t left, uint32_t right, double tol, oc_pairs *out,
-                                 uint32_t **cache_ids, uint32_t *cache_n) {
-  int D = t->dim;
+/* ot_query.h:1307-1458 two-tree CollisionDetection: breadth-first over node pairs
+ * (left subtree x right subtree), sweep and prune on minX within each pair, a
+ * traversed node kept as its own "child" so its entities meet the other side's
+ * children.  Used for sibling subtrees of one tree (tl == tr) and for two trees. */
+static void oc_collide_dual(const oc_tree *tl, const double *gl, uint32_t left, const oc_tree *tr, const double *gr, uint32_t right,
+                            double tol, oc_pairs *out, uint32_t **cache_l, uint32_t *cache_ln, uint32_t **cache_r,
+                            uint32_t *cache_rn) {
+  int D = tl->dim;
   typedef struct {
     uint32_t l, r;
     uint8_t lt, rt;
@@ -1153,21 +1284,21 @@ static void oc_collide_two_nodes(const oc_tree *t, const double *geom, uint32_t 
   while (qh < qt) {
     pr par = q[qh++];
     uint32_t *L, *R, nl, nr;
-    if (!cache_ids[par.l]) cache_ids[par.l] = sorted_node_entities(t, geom, par.l, &cache_n[par.l]);
-    if (!cache_ids[par.r]) cache_ids[par.r] = sorted_node_entities(t, geom, par.r, &cache_n[par.r]);
-    L = cache_ids[par.l];
-    nl = cache_n[par.l];
-    R = cache_ids[par.r];
-    nr = cache_n[par.r];
+    if (!cache_l[par.l]) cache_l[par.l] = sorted_node_entities(tl, gl, par.l, &cache_ln[par.l]);
+    if (!cache_r[par.r]) cache_r[par.r] = sorted_node_entities(tr, gr, par.r, &cache_rn[par.r]);
+    L = cache_l[par.l];
+    nl = cache_ln[par.l];
+    R = cache_r[par.r];
+    nr = cache_rn[par.r];
     size_t rb = 0;
     for (uint32_t a = 0; a < nl; ++a) {
-      const double *lg = &geom[(size_t)L[a] * 2 * D];
+      const double *lg = &gl[(size_t)L[a] * 2 * D];
       for (; rb < nr; ++rb) {
-        const double *rg = &geom[(size_t)R[rb] * 2 * D];
+        const double *rg = &gr[(size_t)R[rb] * 2 * D];
         if (rg[D + 0] >= lg[0]) break;
       }
       for (size_t b = rb; b < nr; ++b) {
-        const double *rg = &geom[(size_t)R[b] * 2 * D];
+        const double *rg = &gr[(size_t)R[b] * 2 * D];
         if (lg[D + 0] < rg[0]) break;
         if (oc_boxes_overlap_strict(lg, rg, D, tol)) pairs_push(out, L[a], R[b]);
       }
@@ -1177,28 +1308,28 @@ static void oc_collide_two_nodes(const oc_tree *t, const double *geom, uint32_t 
     uint8_t ltr[1 << 10], rtr[1 << 10];
     uint32_t nlc = 0, nrc = 0;
     if (!par.lt)
-      for (uint32_t c = t->node_cbegin[par.l]; c < t->node_cbegin[par.l] + t->node_ccount[par.l]; ++c) {
+      for (uint32_t c = tl->node_cbegin[par.l]; c < tl->node_cbegin[par.l] + tl->node_ccount[par.l]; ++c) {
         lc[nlc] = c;
         ltr[nlc++] = 0;
       }
     if (!par.rt)
-      for (uint32_t c = t->node_cbegin[par.r]; c < t->node_cbegin[par.r] + t->node_ccount[par.r]; ++c) {
+      for (uint32_t c = tr->node_cbegin[par.r]; c < tr->node_cbegin[par.r] + tr->node_ccount[par.r]; ++c) {
         rc[nrc] = c;
         rtr[nrc++] = 0;
       }
     if (nlc == 0 && nrc == 0) continue;
-    if (t->node_ecount[par.l] != 0) {
+    if (tl->node_ecount[par.l] != 0) {
       lc[nlc] = par.l;
       ltr[nlc++] = 1;
     }
-    if (t->node_ecount[par.r] != 0) {
+    if (tr->node_ecount[par.r] != 0) {
       rc[nrc] = par.r;
       rtr[nrc++] = 1;
     }
     for (uint32_t a = 0; a < nlc; ++a)
       for (uint32_t b = 0; b < nrc; ++b) {
         if (lc[a] == par.l && rc[b] == par.r) continue;
-        if (!oc_overlap_minpoint(oc_nmin(t, lc[a]), oc_nsize(t, lc[a]), oc_nmin(t, rc[b]), oc_nsize(t, rc[b]), D, tol)) continue;
+        if (!oc_overlap_minpoint(oc_nmin(tl, lc[a]), oc_nsize(tl, lc[a]), oc_nmin(tr, rc[b]), oc_nsize(tr, rc[b]), D, tol)) continue;
         if (qt == qcap) {
           qcap *= 2;
           q = (pr *)realloc(q, qcap * sizeof(pr));
@@ -1266,7 +1397,7 @@ static void oc_collide_subtree(const oc_tree *t, const double *geom, uint32_t no
   /* loose tree: sibling subtree pairs */
   for (uint32_t c1 = t->node_cbegin[node]; c1 < t->node_cbegin[node] + t->node_ccount[node]; ++c1)
     for (uint32_t c2 = c1 + 1; c2 < t->node_cbegin[node] + t->node_ccount[node]; ++c2)
-      oc_collide_two_nodes(t, geom, c1, c2, tol, out, cache_ids, cache_n);
+      oc_collide_dual(t, geom, c1, t, geom, c2, tol, out, cache_ids, cache_n, cache_ids, cache_n);
   free(ctx->ids);
   ctx->ids = NULL;
 }
@@ -1309,10 +1440,39 @@ OC_API oc_result *oc_collision(const oc_tree *t, const double *geom, double tol)
   return r;
 }
 
+/* ot_query.h:1462-1484 static CollisionDetection(leftTree, leftEntities,
+ * rightTree, rightEntities, tol): the dual walk from the two roots.  Pairs are
+ * (left id, right id), sorted (the reference's order is its walk order). */
+OC_API oc_result *oc_collision_trees(const oc_tree *tl, const double *gl, const oc_tree *tr, const double *gr, double tol) {
+  oc_result *r = (oc_result *)calloc(1, sizeof(oc_result));
+  oc_pairs p = {{0}};
+  if (tl->node_count && tr->node_count && tl->is_box && tr->is_box && tl->dim == tr->dim) {
+    uint32_t **cl = (uint32_t **)calloc(tl->node_count, sizeof(uint32_t *));
+    uint32_t *cln = (uint32_t *)calloc(tl->node_count, 4);
+    uint32_t **cr = (uint32_t **)calloc(tr->node_count, sizeof(uint32_t *));
+    uint32_t *crn = (uint32_t *)calloc(tr->node_count, 4);
+    oc_collide_dual(tl, gl, 0, tr, gr, 0, tol, &p, cl, cln, cr, crn);
+    for (uint32_t i = 0; i < tl->node_count; ++i) free(cl[i]);
+    for (uint32_t i = 0; i < tr->node_count; ++i) free(cr[i]);
+    free(cl);
+    free(cln);
+    free(cr);
+    free(crn);
+  }
+  size_t np = p.pairs.n / 2;
+  if (np) qsort(p.pairs.ids, np, 8, pair_cmp);
+  r->nq = np;
+  r->nids = 2 * np;
+  r->ids = p.pairs.ids;
+  r->offsets = (uint64_t *)calloc(np + 1, 8);
+  for (size_t i = 0; i <= np; ++i) r->offsets[i] = 2 * i;
+  return r;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Batched query drivers (pthreads over contiguous query chunks)              */
 /* ------------------------------------------------------------------------- */
-enum { OC_Q_KNN = 0, OC_Q_RANGE = 1, OC_Q_RAY = 2 };
+enum { OC_Q_KNN = 0, OC_Q_RANGE = 1, OC_Q_RAY = 2, OC_Q_PICK = 3, OC_Q_RAY_ALL = 4 };
 
 typedef struct {
   const oc_tree *t;
@@ -1357,6 +1517,11 @@ static void *oc_job_run(void *arg) {
       }
     } else if (j->kind == OC_Q_RANGE) {
       oc_range_one(j->t, j->geom, &j->queries[q * 2 * D], j->overlap_mode, j->tol, &j->out, &stack, &scap);
+    } else if (j->kind == OC_Q_PICK) {
+      oc_pick_one(j->t, j->geom, &j->queries[q * D], j->tol, &j->out, &stack, &scap);
+    } else if (j->kind == OC_Q_RAY_ALL) {
+      const double *qr = &j->queries[q * 2 * D];
+      oc_ray_all_one(j->t, j->geom, qr, qr + D, j->tol, j->inc, j->max_dist, &j->out, &stack, &scap);
     } else {
       const double *qr = &j->queries[q * 2 * D];
       oc_ray_one(j->t, j->geom, qr, qr + D, j->tol, j->inc, j->max_dist, &j->out, &pq, &pqcap);
@@ -1440,6 +1605,31 @@ OC_API oc_result *oc_ray_first(const oc_tree *t, const double *geom, uint64_t nq
   p.geom = geom;
   p.queries = rays;
   p.kind = OC_Q_RAY;
+  p.tol = tol;
+  p.inc = inc;
+  p.max_dist = max_dist;
+  return oc_run_jobs(&p, nq, nthreads);
+}
+
+/* picks: nq x D points */
+OC_API oc_result *oc_pick(const oc_tree *t, const double *geom, uint64_t nq, const double *picks, double tol, int nthreads) {
+  oc_job p = {0};
+  p.t = t;
+  p.geom = geom;
+  p.queries = picks;
+  p.kind = OC_Q_PICK;
+  p.tol = tol;
+  return oc_run_jobs(&p, nq, nthreads);
+}
+
+/* rays: nq x (origin[D], dir[D]); max_dist 0 = no limit (as the reference) */
+OC_API oc_result *oc_ray_all(const oc_tree *t, const double *geom, uint64_t nq, const double *rays, double tol, double inc,
+                             double max_dist, int nthreads) {
+  oc_job p = {0};
+  p.t = t;
+  p.geom = geom;
+  p.queries = rays;
+  p.kind = OC_Q_RAY_ALL;
   p.tol = tol;
   p.inc = inc;
   p.max_dist = max_dist;

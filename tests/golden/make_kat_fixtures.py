@@ -17,6 +17,42 @@ point_set_no1 = [  # getPointSetNo1<2>() general.tests.cpp:72-130
 ]
 
 boxes4 = [[0.0, 0.0, 1.0, 1.0], [1.0, 1.0, 2.0, 2.0], [2.0, 2.0, 3.0, 3.0], [3.0, 3.0, 4.0, 4.0]]
+boxes5 = boxes4 + [[1.2, 1.2, 2.8, 2.8]]
+boxes_r5 = [[2.0, 0.0, 3.0, 1.0], [3.0, 1.0, 4.0, 2.0], [3.0, 2.0, 4.0, 3.0], [2.0, 3.0, 4.0, 4.0], [2.5, 2.5, 3.5, 3.5]]
+EPS = 2.220446049250313e-16  # std::numeric_limits<double>::epsilon()
+INV_SQRT3 = float("0.577350269189625764509148780502")  # std::numbers::inv_sqrt3 (correctly rounded)
+INV_SQRT5 = 1.0 / math.sqrt(5.0)  # 1.0 / std::sqrt(5.0)
+
+
+def lift_1d(boxes):
+    """1-D boxes [lo, hi] of a reference test as 2-D boxes with the second axis [0, 1] for every box.
+
+    Every box spans the added axis, so a point with 0.5 there lies inside each box on that axis and
+    any two boxes overlap strictly on it: the 1-D test's expected id set is the 2-D answer."""
+    return [[lo, 0.0, hi, 1.0] for lo, hi in boxes]
+
+
+# PickSearch_Issue8 general.tests.cpp:3776-4365: 144 cells {x0, 0, z0} - {x0 + 63, 0, z0 + 63}
+_ISSUE8_CELLS = [  # (x0, z0) of mGridCells[0..143] in order
+    (-1024, 1024), (-1024, 1088), (-1024, 1152), (-1024, 1216), (-1024, 1280), (-1024, 1344), (-1024, 1408), (-1024, 1472),
+    (-960, 1024), (-960, 1088), (-960, 1152), (-960, 1216), (-960, 1280), (-960, 1344), (-960, 1408), (-960, 1472),
+    (-896, 1024), (-896, 1088), (-896, 1152), (-896, 1216), (-896, 1280), (-896, 1344), (-896, 1408), (-896, 1472),
+    (-832, 1024), (-832, 1088), (-832, 1152), (-832, 1216), (-832, 1280), (-832, 1344), (-832, 1408), (-832, 1472),
+    (-768, 1024), (-768, 1088), (-768, 1152), (-768, 1216), (-768, 1280), (-768, 1344), (-768, 1408), (-768, 1472),
+    (-704, 1024), (-704, 1088), (-704, 1152), (-704, 1216), (-704, 1280), (-704, 1344), (-704, 1408), (-704, 1472),
+    (-640, 1024), (-640, 1088), (-640, 1152), (-640, 1216), (-640, 1280), (-640, 1344), (-640, 1408), (-640, 1472),
+    (-576, 1024), (-576, 1088), (-576, 1152), (-576, 1216), (-576, 1280), (-576, 1344), (-576, 1408), (-576, 1472),
+    (-1152, 896), (-1152, 960), (-1152, 1024), (-1152, 1088), (-1152, 1152), (-1152, 1216), (-1152, 1280), (-1152, 1344),
+    (-1152, 1408), (-1152, 1472), (-1152, 1536), (-1152, 1600), (-1088, 896), (-1088, 960), (-1088, 1024), (-1088, 1088),
+    (-1088, 1152), (-1088, 1216), (-1088, 1280), (-1088, 1344), (-1088, 1408), (-1088, 1472), (-1088, 1536), (-1088, 1600),
+    (-512, 896), (-512, 960), (-512, 1024), (-512, 1088), (-512, 1152), (-512, 1216), (-512, 1280), (-512, 1344),
+    (-512, 1408), (-512, 1472), (-512, 1536), (-512, 1600), (-448, 896), (-448, 960), (-448, 1024), (-448, 1088),
+    (-448, 1152), (-448, 1216), (-448, 1280), (-448, 1344), (-448, 1408), (-448, 1472), (-448, 1536), (-448, 1600),
+    (-1024, 896), (-1024, 960), (-960, 896), (-960, 960), (-896, 896), (-896, 960), (-832, 896), (-832, 960),
+    (-768, 896), (-768, 960), (-704, 896), (-704, 960), (-640, 896), (-640, 960), (-576, 896), (-576, 960),
+    (-1024, 1536), (-1024, 1600), (-960, 1536), (-960, 1600), (-896, 1536), (-896, 1600), (-832, 1536), (-832, 1600),
+    (-768, 1536), (-768, 1600), (-704, 1536), (-704, 1600), (-640, 1536), (-640, 1600), (-576, 1536), (-576, 1600),
+]
 
 fixtures = {
     "morton_encode": [  # general.tests.cpp:136-217 (MortonTest)
@@ -108,6 +144,70 @@ fixtures = {
          "q": [2.0, 2.0], "k": 1, "expect": [1], "src": "general.tests.cpp:2772"},
         {"boxes": [[0.0, 0.0, 1.0, 1.0], [1.0, 1.0, 3.0, 3.0], [4.0, 4.0, 5.0, 5.0]], "depth": 3, "cap": 2,
          "q": [-1.0, 1.0], "k": 1, "expect": [0], "src": "general.tests.cpp:2788"},
+    ],
+    "pick": [  # PickSearch KATs; expected sets (the tests compare permutations)
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "p": [2.5, 2.5], "expect": [2, 4], "src": "general.tests.cpp:2928"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "p": [2.0, 2.0], "expect": [1, 2, 4], "src": "general.tests.cpp:2947"},
+        {"dim": 2, "boxes": lift_1d([[0, 1], [3, 4], [4, 5], [5, 6], [7, 8]]), "depth": 3, "cap": 2, "p": [4.0, 0.5],
+         "expect": [1, 2], "src": "general.tests.cpp:3672 (1-D, lifted)"},
+        {"dim": 2, "boxes": lift_1d([[0, 16], [0, 1], [3, 4], [4, 5], [5, 6], [2, 4], [4, 6], [2, 6]]), "depth": 4, "cap": 2,
+         "p": [4.0, 0.5], "expect": [0, 2, 3, 5, 6, 7], "src": "general.tests.cpp:3694 (1-D, lifted)"},
+        {"dim": 2, "boxes": lift_1d([[-8, 8], [0, 1], [-4, -3], [4, 5], [-6, -5], [-4, 2], [4, 6], [-6, 2]]), "depth": 4,
+         "cap": 2, "p": [-4.0, 0.5], "expect": [0, 2, 5, 7], "src": "general.tests.cpp:3721 (1-D, lifted)"},
+        {"dim": 2, "boxes": lift_1d([[-17, -1], [-2, -1], [-4, -3], [-5, -4], [-6, -5], [-4, -2], [-6, -4], [-6, -2]]),
+         "depth": 4, "cap": 2, "p": [-4.0, 0.5], "expect": [0, 2, 3, 5, 6, 7], "src": "general.tests.cpp:3748 (1-D, lifted)"},
+        {"dim": 3, "boxes": [[x, 0.0, z, x + 63.0, 0.0, z + 63.0] for x, z in _ISSUE8_CELLS], "depth": 3, "cap": 21,
+         "p": [-864.0, 0.0, 1471.0], "tol": 0.0, "expect": [22], "src": "general.tests.cpp:3776"},
+        {"dim": 3, "points": [[1.0, 1.0, 1.0], [2.0, 2.0, 2.0], [3.0, 3.0, 3.0]], "depth": 3, "cap": 2,
+         "p": [2.0, 2.0, 2.0], "tol": 0.1, "expect": [1], "src": "general.tests.cpp:5293"},
+    ],
+    "ray_all": [  # RayIntersectedAll KATs (tolerance given as the tests give it); "ordered": the test asserts the order
+        {"dim": 2, "boxes": boxes_r5, "depth": 3, "cap": 2, "o": [2.7, 2.6], "d": [SQ2H, SQ2H], "tol": 0.0,
+         "expect": [4, 2, 3], "ordered": True, "src": "general.tests.cpp:1545"},
+        {"dim": 2, "boxes": boxes_r5, "depth": 3, "cap": 2, "o": [2.6, 2.8], "d": [SQ2H, SQ2H], "tol": 0.0,
+         "expect": [3, 4], "src": "general.tests.cpp:1563"},
+        {"dim": 3, "boxes": [[2.0, 0.0, 1.0, 3.0, 1.0, 2.0], [3.0, 1.0, 1.0, 4.0, 2.0, 2.0], [3.0, 2.0, 1.0, 4.0, 3.0, 2.0],
+                             [2.0, 3.0, 1.5, 4.0, 4.0, 2.0], [2.5, 2.5, 1.5, 3.5, 3.5, 2.0]],
+         "depth": 3, "cap": 2, "o": [2.6, 2.8, 1.0], "d": [INV_SQRT3] * 3, "tol": 0.0, "expect": [3, 4],
+         "src": "general.tests.cpp:1581"},
+        {"dim": 5, "boxes": [[2.0, 0.0, 1.0, 0.0, 0.0, 3.0, 1.0, 2.0, 2.0, 2.0], [3.0, 1.0, 1.0, 0.0, 0.0, 4.0, 2.0, 2.0, 2.0, 2.0],
+                             [3.0, 2.0, 1.0, 0.0, 0.0, 4.0, 3.0, 2.0, 2.0, 2.0], [2.0, 3.0, 1.5, 0.0, 0.0, 4.0, 4.0, 2.0, 2.0, 2.0],
+                             [2.5, 2.5, 1.5, 0.0, 0.0, 3.5, 3.5, 2.0, 2.0, 2.0]],
+         "depth": 3, "cap": 2, "o": [2.6, 2.8, 1.0, 0.0, 0.0], "d": [INV_SQRT5] * 5, "tol": 0.0, "expect": [3, 4],
+         "src": "general.tests.cpp:1599"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "o": [1.1, 2.0], "d": [1.0, 0.0], "tol": 0.0,
+         "expect": [1, 4, 2], "src": "general.tests.cpp:1639"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "o": [0.0, 2.001], "d": [1.0, 0.0], "tol": 0.001 - EPS,
+         "expect": [4, 2], "src": "general.tests.cpp:1645"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "o": [2.0005, 2.0], "d": [1.0, 0.0], "tol": 0.001,
+         "expect": [2, 4], "src": "general.tests.cpp:1651"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "o": [1.0, 1.9], "d": [1.0, 0.0], "tol": 0.899,
+         "expect": [1, 4, 2], "src": "general.tests.cpp:1657"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "o": [1.0, 1.9], "d": [1.0, 0.0], "tol": 0.9 - EPS,
+         "expect": [1, 4, 2], "src": "general.tests.cpp:1663"},
+        {"dim": 2, "boxes": boxes5, "depth": 3, "cap": 2, "o": [1.0, 1.9], "d": [1.0, 0.0], "tol": 0.901,
+         "expect": [0, 1, 4, 2], "src": "general.tests.cpp:1669"},
+        {"dim": 2, "boxes": [[1.0, 0.01, 2.0, 0.5]], "depth": 3, "cap": 2, "o": [0.0, 0.0], "d": [1.0, 0.0], "tol": 0.0,
+         "inc": 0.0, "expect": [], "src": "general.tests.cpp:1697"},
+        {"dim": 2, "boxes": [[1.0, 0.01, 2.0, 0.5]], "depth": 3, "cap": 2, "o": [0.0, 0.0], "d": [1.0, 0.0], "tol": 0.0,
+         "inc": 0.01, "expect": [0], "src": "general.tests.cpp:1708"},
+        {"dim": 2, "boxes": [[2.0, -1.0, 3.0, 1.0], [10.0, -1.0, 11.0, 1.0]], "depth": 3, "cap": 2, "o": [0.0, 0.0],
+         "d": [1.0, 0.0], "tol": 0.0, "inc": 0.0, "max_dist": 5.0, "expect": [0], "src": "general.tests.cpp:1733"},
+        {"dim": 2, "boxes": [[2.0, -1.0, 3.0, 1.0], [10.0, -1.0, 11.0, 1.0]], "depth": 3, "cap": 2, "o": [0.0, 0.0],
+         "d": [1.0, 0.0], "tol": 0.0, "inc": 0.0, "max_dist": 100.0, "expect": [0, 1], "ordered": True,
+         "src": "general.tests.cpp:1746"},
+    ],
+    "collision_two_trees": [  # static CollisionDetection(leftTree, left, rightTree, right); expected (left, right) sets
+        {"left": boxes4, "right": boxes_r5, "depth": 3, "cap": 2, "expect": [[3, 3], [2, 4], [3, 4]],
+         "src": "general.tests.cpp:3081"},
+        {"left": lift_1d([[0.0, 1.0], [1.0, 2.0], [2.0, 3.0], [3.0, 4.0]]), "right": lift_1d([[3.5, 4.5], [4.5, 5.5], [5.5, 6.5]]),
+         "depth": 3, "cap": 1, "expect": [[3, 0]], "src": "general.tests.cpp:2202 (1-D, lifted)"},
+        {"left": lift_1d([[0.0, 4.0], [0.0, 2.0], [2.0, 4.0], [0.0, 1.0], [1.0, 2.0], [2.0, 3.0], [3.0, 4.0]]),
+         "right": lift_1d([[2.0, 6.0], [2.0, 4.0], [4.0, 6.0], [2.0, 3.0], [3.0, 4.0], [4.0, 5.0], [5.0, 6.0]]),
+         "depth": 3, "cap": 1,
+         "expect": [[0, 0], [0, 1], [2, 0], [2, 1], [0, 3], [2, 3], [5, 0], [5, 1], [5, 3], [0, 4], [2, 4], [6, 0], [6, 1],
+                    [6, 4]],
+         "src": "general.tests.cpp:2235 (1-D, lifted)"},
     ],
     "issue38_collision": {  # general.tests.cpp:4728-4772: 10 rounds x 1000 loose 3D boxes, mt19937(0); vs brute force
         "rounds": 10, "box_no": 1000, "space_max": 8.0, "size_max": 2.0, "depth": 10, "cap": 5,

@@ -201,6 +201,43 @@ def range_brute_points(r, pts, tol=BASE_TOL):
     return np.nonzero(ok)[0].tolist()
 
 
+def pick_brute(p, geom, is_box, tol=BASE_TOL):
+    """PickSearch's entity test over all entities: boxes DoesBoxContainPoint
+    (general.h:225-243), points ArePointsEqual (general.h:218-221)."""
+    D = len(p)
+    if is_box:
+        ok = np.ones(geom.shape[0], bool)
+        for d in range(D):
+            if tol != 0.0:
+                ok &= (geom[:, d] - tol < p[d]) & (p[d] < geom[:, D + d] + tol)
+            else:
+                ok &= (geom[:, d] <= p[d]) & (p[d] <= geom[:, D + d])
+        return np.nonzero(ok)[0].tolist()
+    d2 = np.zeros(geom.shape[0])
+    for d in range(D):
+        x = geom[:, d] - p[d]
+        d2 = d2 + x * x
+    return np.nonzero(d2 <= tol * tol)[0].tolist()
+
+
+def ray_hit_points(ray, pts, tol, inc):
+    """RayHitTester::Hit(point) (internal_geometry_module.h:599-626) over all points:
+    (hit, enter, exit)."""
+    D = len(ray) // 2
+    o, dv = np.asarray(ray[:D]), np.asarray(ray[D:])
+    v = pts - o
+    t = np.zeros(pts.shape[0])
+    for d in range(D):
+        t = t + v[:, d] * dv[d]
+    len2 = np.zeros(pts.shape[0])
+    for d in range(D):
+        len2 = len2 + v[:, d] * v[:, d]
+    dist2 = np.maximum(0.0, len2 - t * t)
+    ct = np.maximum(0.0, t) * inc + tol
+    hit = (dist2 <= ct * ct) & (t >= -ct)
+    return hit, np.maximum(0.0, t - ct), t + ct
+
+
 def boxes_overlap_strict(a, B, tol):
     """general.h:262-289 GetBoxRelation(a, B[i]) == Overlapped, vectorised over B."""
     D = a.shape[0] // 2
@@ -217,6 +254,15 @@ def collision_brute(boxes, tol=BASE_TOL):
         m = boxes_overlap_strict(boxes[i], boxes[i + 1:], tol)
         for j in np.nonzero(m)[0]:
             pairs.append((i, i + 1 + int(j)))
+    return np.array(pairs, dtype=np.int64).reshape(-1, 2)
+
+
+def collision_two_brute(left, right, tol=BASE_TOL):
+    """All (left id, right id) with GetBoxRelation == Overlapped, sorted."""
+    pairs = []
+    for i in range(left.shape[0]):
+        for j in np.nonzero(boxes_overlap_strict(left[i], right, tol))[0]:
+            pairs.append((i, int(j)))
     return np.array(pairs, dtype=np.int64).reshape(-1, 2)
 
 

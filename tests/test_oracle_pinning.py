@@ -5,8 +5,9 @@ import math
 import numpy as np
 import pytest
 
-from helpers import (BASE_TOL, DBL_MAX, MT19937, box_minmax, check_node_table_rules, collision_brute, issue38_boxes,
-                     kat, knn_box_brute, knn_brute, lognormal_boxes, minmaxdist_textbook, range_brute_points,
+from helpers import (BASE_TOL, DBL_MAX, MT19937, box_minmax, check_node_table_rules, collision_brute,
+                     collision_two_brute, gaussian_mixture, issue38_boxes, kat, knn_box_brute, knn_brute,
+                     lognormal_boxes, minmaxdist_textbook, pick_brute, range_brute_points, ray_hit_points,
                      uniform_points, unit_dirs)
 from oracle import oracle as O
 
@@ -323,3 +324,103 @@ def test_ray_oracle_vs_brute():
         sel = idx[en[idx] <= M]
         exp = sel[np.lexsort((sel, -en[sel]))].tolist()
         assert got[qi] == exp, qi
+
+
+# ---- PickSearch (ot_query.h:426-466) -------------------------------------------------
+def test_pick_kats():
+    for c in kat()["pick"]:
+        if "boxes" in c:
+            t = O.OracleTree(c["dim"], "box", c["boxes"], None, c["depth"], c["cap"])
+        else:
+            t = O.OracleTree(c["dim"], "point", c["points"], None, c["depth"], c["cap"])
+        off, ids = t.pick([c["p"]], c.get("tol", BASE_TOL))
+        assert sorted(ids.tolist()) == sorted(c["expect"]), c["src"]
+
+
+@pytest.mark.parametrize("tol", [BASE_TOL, 0.0, 0.5])
+def test_pick_boxes_vs_brute(tol):
+    boxes = lognormal_boxes(3000, 3, 0.0, 100.0, np.log(3.0), 0.6, seed=21)
+    t = O.OracleTree(3, "box", boxes, None, 6, 8)
+    picks = uniform_points(300, 3, seed=22)
+    picks[:100] = boxes[:100, :3]          # on a box corner (border case)
+    picks[100:150] = boxes[100:150, 3:]    # on a max corner
+    off, ids = t.pick(picks, tol)
+    got = O.lists(off, ids)
+    for qi in range(len(picks)):
+        assert sorted(got[qi]) == pick_brute(picks[qi], boxes, True, tol), qi
+
+
+def test_pick_points_vs_brute():
+    pts = np.round(uniform_points(4000, 3, seed=23), 1)  # many exact duplicates
+    t = O.OracleTree(3, "point", pts, None, 6, 8)
+    picks = np.concatenate([pts[:200], uniform_points(100, 3, seed=24)])
+    for tol in (BASE_TOL, 0.06, 0.0):
+        off, ids = t.pick(picks, tol)
+        got = O.lists(off, ids)
+        for qi in range(len(picks)):
+            assert sorted(got[qi]) == pick_brute(picks[qi], pts, False, tol), (tol, qi)
+
+
+# ---- RayIntersectedAll (ot_query.h:1929-2012) --------------------------------------------
+def test_ray_all_kats():
+    for c in kat()["ray_all"]:
+        t = O.OracleTree(c["dim"], "box", c["boxes"], None, c["depth"], c["cap"])
+        off, ids = t.ray_all([c["o"] + c["d"]], c["tol"], c.get("inc", 0.0), c.get("max_dist", DBL_MAX))
+        if c.get("ordered"):
+            assert ids.tolist() == c["expect"], c["src"]
+        else:
+            assert sorted(ids.tolist()) == sorted(c["expect"]), c["src"]
+
+
+@pytest.mark.parametrize("max_dist", [DBL_MAX, 0.0, 60.0])
+def test_ray_all_boxes_vs_brute(max_dist):
+    boxes = lognormal_boxes(3000, 3, 0.0, 100.0, np.log(1.0), 0.6, seed=31)
+    t = O.OracleTree(3, "box", boxes, None, 6, 8)
+    rays = np.concatenate([uniform_points(300, 3, seed=32), unit_dirs(300, 3, seed=33)], axis=1)
+    rays[0, 3:] = [1.0, 0.0, 0.0]  # axis-parallel (NaN slabs)
+    off, ids = t.ray_all(rays, BASE_TOL, 0.0, max_dist)
+    got = O.lists(off, ids)
+    for qi in range(len(rays)):
+        hit, en, ex = _hit_all(rays[qi], boxes, BASE_TOL, 0.0)
+        if max_dist != 0.0:
+            hit &= en <= max_dist
+        sel = np.nonzero(hit)[0]
+        exp = sel[np.lexsort((sel, en[sel]))].tolist()
+        assert got[qi] == exp, qi
+
+
+def test_ray_all_points_vs_brute():
+    pts = gaussian_mixture(4000, 3, 6, 0.0, 100.0, 2.0, 8.0, seed=34)
+    t = O.OracleTree(3, "point", pts, None, 6, 8)
+    rays = np.concatenate([uniform_points(200, 3, seed=35), unit_dirs(200, 3, seed=36)], axis=1)
+    for tol in (0.05, 0.5):
+        off, ids = t.ray_all(rays, tol, 0.0, DBL_MAX)
+        got = O.lists(off, ids)
+        for qi in range(len(rays)):
+            hit, en, ex = ray_hit_points(rays[qi], pts, tol, 0.0)
+            sel = np.nonzero(hit)[0]
+            exp = sel[np.lexsort((sel, en[sel]))].tolist()
+            assert got[qi] == exp, (tol, qi)
+
+
+# ---- two-tree CollisionDetection (ot_query.h:1462-1484) --------------------------------------
+def test_collision_two_trees_kats():
+    for c in kat()["collision_two_trees"]:
+        tl = O.OracleTree(2, "box", c["left"], None, c["depth"], c["cap"])
+        tr = O.OracleTree(2, "box", c["right"], None, c["depth"], c["cap"])
+        got = sorted(map(tuple, tl.collide_with(tr).tolist()))
+        assert got == sorted(map(tuple, c["expect"])), c["src"]
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_collision_two_trees_vs_brute(D):
+    left = lognormal_boxes(1500, D, 0.0, 60.0, seed=41 + D)
+    right = lognormal_boxes(1000, D, 10.0, 80.0, seed=51 + D)
+    tl = O.OracleTree(D, "box", left, None, 5, 8)
+    tr = O.OracleTree(D, "box", right, None, 6, 4)
+    got = tl.collide_with(tr).astype(np.int64)
+    assert len(got) > 0
+    assert np.array_equal(got, collision_two_brute(left, right))
+    # and the other way round: the same pairs with the sides swapped
+    back = tr.collide_with(tl).astype(np.int64)
+    assert np.array_equal(back[np.lexsort((back[:, 0], back[:, 1]))][:, ::-1], got)
