@@ -9,6 +9,8 @@
 //                                                         (core/ot_static_linear_core.h:103-129, :523-613)
 //   GetNearestNeighbors, RangeSearch, RayIntersectedFirst, CollisionDetection
 //                                                         (core/ot_query.h:1208-1270, :476-583, :2035-2146, :1901-1906)
+//   PickSearch, RayIntersectedAll, two-tree CollisionDetection
+//                                                         (core/ot_query.h:426-466, :1929-2022, :1462-1496)
 //   GetNodeCount / GetNodeChildren / GetNodeEntities / GetNodeMinPoint / GetNodeSize / GetNodeBox
 //                                                         (core/ot_static_linear_core.h:178-268)
 // plus batch overloads over spans of queries returning CSR results.
@@ -234,19 +236,78 @@ class StaticOrthoTreeGPU {
     return take(res);
   }
 
+  // RayIntersectedAll (ot_query.h:1929-2022): every hit, ascending enter distance.
+  std::vector<EntityID> RayIntersectedAll(TVector const &rayBasePoint, TVector const &rayHeading, EntityContainerView,
+                                          double tolerance = BASE_TOLERANCE, double toleranceIncrement = 0.0,
+                                          double maxExaminationDistance = std::numeric_limits<double>::max()) const noexcept {
+    TRay ray{rayBasePoint, rayHeading};
+    auto r = RayIntersectedAll(std::span<const TRay>(&ray, 1), tolerance, toleranceIncrement, maxExaminationDistance);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
+  }
+  std::vector<EntityID> RayIntersectedAll(TRay const &ray, EntityContainerView entities, double tolerance = BASE_TOLERANCE,
+                                          double toleranceIncrement = 0.0,
+                                          double maxExaminationDistance = std::numeric_limits<double>::max()) const noexcept {
+    return RayIntersectedAll(ray.Origin, ray.Direction, entities, tolerance, toleranceIncrement, maxExaminationDistance);
+  }
+  BatchResult RayIntersectedAll(std::span<const TRay> rays, double tolerance = BASE_TOLERANCE,
+                                double toleranceIncrement = 0.0,
+                                double maxExaminationDistance = std::numeric_limits<double>::max()) const noexcept {
+    otg_result *res = nullptr;
+    if (!tree_) return empty(rays.size());
+    detail::check(otg_ray_all(tree_, rays.size(), reinterpret_cast<const double *>(rays.data()), nullptr, OTG_MEM_HOST,
+                              tolerance, toleranceIncrement, maxExaminationDistance, &res),
+                  "otg_ray_all", ctx().get());
+    return take(res);
+  }
+
+  // PickSearch (ot_query.h:426-466)
+  std::vector<EntityID> PickSearch(TVector const &pickPoint, EntityContainerView,
+                                   double tolerance = BASE_TOLERANCE) const noexcept {
+    auto r = PickSearch(std::span<const TVector>(&pickPoint, 1), tolerance);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
+  }
+  BatchResult PickSearch(std::span<const TVector> pickPoints, double tolerance = BASE_TOLERANCE) const noexcept {
+    otg_result *res = nullptr;
+    if (!tree_) return empty(pickPoints.size());
+    detail::check(otg_pick(tree_, pickPoints.size(), reinterpret_cast<const double *>(pickPoints.data()), nullptr,
+                           OTG_MEM_HOST, tolerance, &res),
+                  "otg_pick", ctx().get());
+    return take(res);
+  }
+
   template <typename TExecMode = ParExec>
   std::vector<std::pair<EntityID, EntityID>> CollisionDetection(EntityContainerView, double tolerance = BASE_TOLERANCE,
                                                                 TExecMode = {}) const noexcept
     requires(IS_BOX)
   {
-    std::vector<std::pair<EntityID, EntityID>> out;
-    if (!tree_) return out;
+    if (!tree_) return {};
     otg_result *res = nullptr;
     detail::check(otg_collide(tree_, 0, tolerance, &res), "otg_collide", ctx().get());
-    BatchResult r = take(res);
-    out.resize(r.ids.size() / 2);
-    for (size_t i = 0; i < out.size(); ++i) out[i] = {r.ids[2 * i], r.ids[2 * i + 1]};
-    return out;
+    return pairs(take(res));
+  }
+
+  // Two-tree collision detection (ot_query.h:1462-1484): pairs (left id, right id).
+  static std::vector<std::pair<EntityID, EntityID>> CollisionDetection(StaticOrthoTreeGPU const &leftTree,
+                                                                       EntityContainerView,
+                                                                       StaticOrthoTreeGPU const &rightTree,
+                                                                       EntityContainerView,
+                                                                       double tolerance = BASE_TOLERANCE) noexcept
+    requires(IS_BOX)
+  {
+    if (!leftTree.tree_ || !rightTree.tree_) return {};
+    otg_result *res = nullptr;
+    detail::check(otg_collide_trees(leftTree.tree_, 0, rightTree.tree_, 0, tolerance, &res), "otg_collide_trees",
+                  leftTree.ctx().get());
+    return pairs(leftTree.take(res));
+  }
+  // Member form (ot_query.h:1488-1496): this tree on the left.
+  std::vector<std::pair<EntityID, EntityID>> CollisionDetection(EntityContainerView boxes,
+                                                                StaticOrthoTreeGPU const &otherTree,
+                                                                EntityContainerView otherBoxes,
+                                                                double tolerance = BASE_TOLERANCE) const noexcept
+    requires(IS_BOX)
+  {
+    return CollisionDetection(*this, boxes, otherTree, otherBoxes, tolerance);
   }
 
   // ---- node API (host copy of the device node table, made on first use) ----------------
@@ -316,6 +377,12 @@ class StaticOrthoTreeGPU {
     TVector v;
     for (int d = 0; d < DIM; ++d) v[d] = a[(size_t)node * DIM + d];
     return v;
+  }
+
+  static std::vector<std::pair<EntityID, EntityID>> pairs(BatchResult const &r) {
+    std::vector<std::pair<EntityID, EntityID>> out(r.ids.size() / 2);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = {r.ids[2 * i], r.ids[2 * i + 1]};
+    return out;
   }
 
   static BatchResult empty(size_t nq) {

@@ -161,6 +161,25 @@ OTG_API otg_status otg_ray_first(const otg_tree *tree, uint64_t nq, const double
  * (i < j), sorted by (i, j). */
 OTG_API otg_status otg_collide(const otg_tree *tree, uint32_t set, double tolerance, otg_result **out);
 
+/* Static CollisionDetection(leftTree, leftEntities, rightTree, rightEntities,
+ * tolerance) (core/ot_query.h:1462-1484) and its member form (:1488-1496), for
+ * one set of each box tree (same dimension, same device).  Result: nq = pair
+ * count, ids = 2*pairs (left id, right id), sorted by (left, right). */
+OTG_API otg_status otg_collide_trees(const otg_tree *left, uint32_t left_set, const otg_tree *right,
+                                     uint32_t right_set, double tolerance, otg_result **out);
+
+/* PickSearch(pickPoint, entities, tolerance) (core/ot_query.h:426-466): boxes that
+ * contain the point, or points within tolerance of it.  tolerance >= 0. */
+OTG_API otg_status otg_pick(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
+                            double tolerance, otg_result **out);
+
+/* RayIntersectedAll(origin, direction, entities, tolerance, toleranceIncrement,
+ * maxExaminationDistance) (core/ot_query.h:1929-2022): every entity the ray hits
+ * entering within max_distance (0 = no limit, as the reference), ascending enter
+ * distance (ties by id). */
+OTG_API otg_status otg_ray_all(const otg_tree *tree, uint64_t nq, const double *rays, const uint32_t *set_ids, int mem,
+                               double tolerance, double tolerance_increment, double max_distance, otg_result **out);
+
 OTG_API otg_status otg_result_get_info(const otg_result *res, uint64_t *nq, uint64_t *nids);
 OTG_API otg_status otg_result_copy(const otg_result *res, uint64_t *offsets, uint32_t *ids);
 /* Device-resident CSR (valid until otg_result_free). */

@@ -344,6 +344,33 @@ class StaticTree:
         _check(st, "otg_ray_first", self.ctx)
         return Result(h, self.ctx)
 
+    def pick_batch(self, points, tolerance: float = BASE_TOLERANCE, set_ids=None) -> Result:
+        qp, nq, sp, mem, _keep = self._query_args(points, self.dim, set_ids)
+        h = C.c_void_p()
+        st = _abi.lib().otg_pick(self.handle, nq, qp, sp, mem, float(tolerance), C.byref(h))
+        _check(st, "otg_pick", self.ctx)
+        return Result(h, self.ctx)
+
+    def ray_all_batch(self, rays, tolerance: float = BASE_TOLERANCE, tolerance_increment: float = 0.0,
+                      max_distance: float = DBL_MAX, set_ids=None) -> Result:
+        qp, nq, sp, mem, _keep = self._query_args(rays, 2 * self.dim, set_ids)
+        h = C.c_void_p()
+        st = _abi.lib().otg_ray_all(self.handle, nq, qp, sp, mem, float(tolerance), float(tolerance_increment),
+                                    float(max_distance), C.byref(h))
+        _check(st, "otg_ray_all", self.ctx)
+        return Result(h, self.ctx)
+
+    def collide_with(self, other: "StaticTree", tolerance: float = BASE_TOLERANCE, s: int = 0,
+                     other_set: int = 0) -> np.ndarray:
+        """Two-tree collision: (id in this tree, id in other) pairs, sorted."""
+        h = C.c_void_p()
+        _check(_abi.lib().otg_collide_trees(self.handle, s, other.handle, other_set, float(tolerance), C.byref(h)),
+               "otg_collide_trees", self.ctx)
+        r = Result(h, self.ctx)
+        _, ids = r.csr()
+        r.free()
+        return ids.reshape(-1, 2)
+
     def collide(self, tolerance: float = BASE_TOLERANCE, s: int = 0) -> np.ndarray:
         h = C.c_void_p()
         _check(_abi.lib().otg_collide(self.handle, s, float(tolerance), C.byref(h)), "otg_collide", self.ctx)
@@ -373,8 +400,34 @@ class StaticTree:
         ray = np.concatenate([np.asarray(origin, np.float64).reshape(-1), np.asarray(direction, np.float64).reshape(-1)])
         return self.ray_first_batch([ray], tolerance, tolerance_increment, max_distance).lists()[0]
 
-    def CollisionDetection(self, entities=None, tolerance: float = BASE_TOLERANCE, exec_tag=PAR_EXEC):
+    def PickSearch(self, pick_point, entities=None, tolerance: float = BASE_TOLERANCE):
+        return self.pick_batch([np.asarray(pick_point, dtype=np.float64).reshape(-1)], tolerance).lists()[0]
+
+    def RayIntersectedAll(self, origin, direction, entities=None, tolerance: float = BASE_TOLERANCE,
+                          tolerance_increment: float = 0.0, max_distance: float = DBL_MAX):
+        ray = np.concatenate([np.asarray(origin, np.float64).reshape(-1), np.asarray(direction, np.float64).reshape(-1)])
+        return self.ray_all_batch([ray], tolerance, tolerance_increment, max_distance).lists()[0]
+
+    def CollisionDetection(self, *args, tolerance: float = BASE_TOLERANCE, exec_tag=PAR_EXEC):
+        """CollisionDetection(entities, tol) on this tree (ot_query.h:1901-1906), or
+        CollisionDetection(entities, otherTree, otherEntities, tol) against another tree
+        (:1488-1496): pairs (this id, other id)."""
+        rest = list(args)
+        if len(rest) >= 2 and isinstance(rest[1], StaticTree):
+            other = rest[1]
+            if len(rest) >= 4:
+                tolerance = float(rest[3])
+            return [tuple(p) for p in self.collide_with(other, tolerance).tolist()]
+        if len(rest) >= 2:
+            tolerance = float(rest[1])
         return [tuple(p) for p in self.collide(tolerance).tolist()]
+
+    @staticmethod
+    def CollisionDetectionTrees(left: "StaticTree", left_entities, right: "StaticTree", right_entities,
+                                tolerance: float = BASE_TOLERANCE):
+        """Static CollisionDetection(leftTree, leftEntities, rightTree, rightEntities, tol)
+        (ot_query.h:1462-1484)."""
+        return [tuple(p) for p in left.collide_with(right, tolerance).tolist()]
 
 
 class Server:

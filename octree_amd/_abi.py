@@ -35,6 +35,7 @@ EXPORTED = [
     "otg_ctx_mem_stats", "otg_ctx_release_cache", "otg_ctx_set_profiling", "otg_ctx_kernel_times",
     "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_free", "otg_tree_get_info",
     "otg_tree_get_set_info", "otg_tree_export", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
+    "otg_collide_trees", "otg_pick", "otg_ray_all",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
     "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_stats",
@@ -101,6 +102,9 @@ def lib() -> C.CDLL:
         "otg_range": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_int, C.c_double, PP]),
         "otg_ray_first": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, C.c_double, C.c_double, PP]),
         "otg_collide": (C.c_int, [P, C.c_uint32, C.c_double, PP]),
+        "otg_collide_trees": (C.c_int, [P, C.c_uint32, P, C.c_uint32, C.c_double, PP]),
+        "otg_pick": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, PP]),
+        "otg_ray_all": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, C.c_double, C.c_double, PP]),
         "otg_result_get_info": (C.c_int, [P, U64P, U64P]),
         "otg_result_copy": (C.c_int, [P, P, P]),
         "otg_result_device": (C.c_int, [P, PP, PP]),

@@ -429,6 +429,40 @@ OTG_API otg_status otg_collide(const otg_tree *t, uint32_t set, double tolerance
   });
 }
 
+OTG_API otg_status otg_collide_trees(const otg_tree *left, uint32_t left_set, const otg_tree *right,
+                                     uint32_t right_set, double tolerance, otg_result **out) {
+  if (!left || !right || !out) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = left->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(otg::query_collide_trees(left, left_set, right, right_set, tolerance));
+  });
+}
+
+OTG_API otg_status otg_pick(const otg_tree *t, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
+                            double tolerance, otg_result **out) {
+  if (!t || !out || (nq && !points) || !(tolerance >= 0.0)) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(otg::query_pick(t, nq, points, set_ids, mem, tolerance));
+  });
+}
+
+OTG_API otg_status otg_ray_all(const otg_tree *t, uint64_t nq, const double *rays, const uint32_t *set_ids, int mem,
+                               double tolerance, double tolerance_increment, double max_distance, otg_result **out) {
+  if (!t || !out || (nq && !rays)) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(
+        otg::query_ray_all(t, nq, rays, set_ids, mem, tolerance, tolerance_increment, max_distance));
+  });
+}
+
 OTG_API otg_status otg_result_get_info(const otg_result *r, uint64_t *nq, uint64_t *nids) {
   if (!r) return OTG_ERR_INVALID_ARG;
   if (nq) *nq = r->nq;

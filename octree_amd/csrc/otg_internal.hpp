@@ -233,6 +233,10 @@ Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t
 Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees, uint64_t nq, const double *q,
                     const uint32_t *qview, const uint32_t *qk, double p0, double p1, double p2, int mode);
 Result *query_collide(const Tree *t, uint32_t set, double tol);
+Result *query_collide_trees(const Tree *lt, uint32_t lset, const Tree *rt, uint32_t rset, double tol);
+Result *query_pick(const Tree *t, uint64_t nq, const double *picks, const uint32_t *sets, int mem, double tol);
+Result *query_ray_all(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
+                      double inc, double max_dist);
 
 // host-side mirrors of reference helpers
 uint32_t max_theoretical_depth(int dim);

@@ -639,14 +639,14 @@ __device__ __forceinline__ bool pair_less(double da, uint32_t ia, double db, uin
   return (DESC ? da > db : da < db) || (da == db && ia < ib);
 }
 
-template <bool DESC>
-__global__ void __launch_bounds__(kQB) k_sort_band(const uint32_t *__restrict__ offsets, uint32_t nband,
+template <bool DESC, typename Off = uint32_t>
+__global__ void __launch_bounds__(kQB) k_sort_band(const Off *__restrict__ offsets, uint64_t nband,
                                                     double *__restrict__ d, uint32_t *__restrict__ id) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nband) return;
   double *a = d + offsets[ti];
   uint32_t *b = id + offsets[ti];
-  const uint32_t n = offsets[ti + 1] - offsets[ti];
+  const uint32_t n = (uint32_t)(offsets[ti + 1] - offsets[ti]);
   auto sift = [&](uint32_t i, uint32_t m) {
     for (;;) {
       uint32_t l = 2 * i + 1, r = l + 1, x = i;
@@ -934,6 +934,121 @@ __global__ void __launch_bounds__(256) k_range_copy(const RangeCopy *__restrict_
   if (w >= n) return;
   const RangeCopy c = copies[w];
   for (uint32_t t = lane; t < c.len; t += 64) out[c.dst + t] = c.src ? c.src[t] : t;
+}
+
+// ---------------------------------------------------------------------------
+// pick search
+// ---------------------------------------------------------------------------
+// internal_geometry_module.h:372-396 DoesBoxContainPointAD(box, point, tol) and
+// adapters/general.h:225-243 DoesBoxContainPoint (the same comparisons)
+template <int D>
+__device__ __forceinline__ bool box_contains_point(const double *mn, const double *mx, const double (&p)[D], double tol) {
+  bool ok = true;
+  if (tol != 0.0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) ok &= (mn[d] - tol < p[d] && p[d] < mx[d] + tol);
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) ok &= (mn[d] <= p[d] && p[d] <= mx[d]);
+  }
+  return ok;
+}
+
+// internal_geometry_module.h:398-422 DoesBoxContainPointAD(minPoint, size, point, tol):
+// |p - min - size/2| against size/2 (+ tol)
+template <int D>
+__device__ __forceinline__ bool node_contains_point(const double *mn, const double *sz, const double (&p)[D], double tol) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double half = sz[d] * 0.5;
+    const double dist = fabs(p[d] - mn[d] - half);
+    ok &= (tol != 0.0) ? !(dist >= half + tol) : !(dist > half);
+  }
+  return ok;
+}
+
+// PickSearch (ot_query.h:426-466): root box test, then depth-first (last child
+// first, the reference's output order) skipping nodes whose box misses the point.
+// Entities: boxes containing the point, or points within tol (Distance2 <= tol^2,
+// general.h:218-221).  Count pass (FILL = false) and fill pass.
+template <int D, bool BOX, bool FILL>
+__device__ __forceinline__ void pick_query(const TreeView &tv, uint64_t qi, const double *__restrict__ picks,
+                                           const uint32_t *__restrict__ qset, double tol, LdsLevelIter &stack,
+                                           uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                           uint32_t *__restrict__ out) {
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  double p[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) p[d] = picks[qi * D + d];
+  const uint64_t o = FILL ? offsets[qi] : 0;
+  uint32_t cnt = 0;
+  if (sp.node_cnt != 0) {
+    const uint32_t root = (uint32_t)sp.node_off;
+    double rmx[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) rmx[d] = tv.nmin[(uint64_t)root * D + d] + tv.nsize[(uint64_t)root * D + d];
+    if (box_contains_point<D>(&tv.nmin[(uint64_t)root * D], rmx, p, tol)) {
+      int top = 0;
+      uint32_t node = root;
+      for (;;) {
+        if (node_contains_point<D>(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], p, tol)) {
+          const NodeInfo ni = tv.info[node];
+          for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+            bool ok;
+            if (BOX) {
+              double emn[D], emx[D];
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                emn[d] = tv.geom[(uint64_t)d * tv.n + j];
+                emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
+              }
+              ok = box_contains_point<D>(emn, emx, p, tol);
+            } else {
+              double d2 = 0.0;
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const double x = tv.geom[(uint64_t)d * tv.n + j] - p[d];
+                d2 += x * x;
+              }
+              ok = d2 <= tol * tol;
+            }
+            if (ok) {
+              if (FILL) out[o + cnt] = tv.ids[j];
+              ++cnt;
+            }
+          }
+          if (ni.ccount) {
+            stack.nx(top) = ni.cbegin + ni.ccount - 1;
+            stack.left(top) = (uint8_t)ni.ccount;
+            ++top;
+          }
+        }
+        if (top == 0) break;
+        node = stack.nx(top - 1)--;
+        if (--stack.left(top - 1) == 0) --top;
+      }
+    }
+  }
+  if (!FILL) counts[qi] = cnt;
+}
+
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_pick(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                               const double *__restrict__ picks, const uint32_t *__restrict__ qset,
+                                               const uint32_t *__restrict__ order, uint64_t nq, double tol,
+                                               uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                               uint32_t *__restrict__ out) {
+  __shared__ uint32_t s_a[kRangeStack * kQB];
+  __shared__ uint8_t s_b[kRangeStack * kQB];
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  LdsLevelIter S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    pick_query<D, BOX, FILL>(tv, qi, picks, qset, tol, S, counts, offsets, out);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,33 +1511,119 @@ __global__ void k_replay_copy(const uint32_t *__restrict__ band, uint32_t nband,
 }
 
 // ---------------------------------------------------------------------------
-// self collision (boxes): every entity queries the tree with its own box and
-// keeps partners with a larger id.
+// ray, all hits
 // ---------------------------------------------------------------------------
-template <int D, bool FILL>
-__global__ void __launch_bounds__(kQB) k_collide(TreeView tv, uint32_t set, double tol, uint32_t *__restrict__ counts,
-                                                  const uint64_t *__restrict__ offsets, uint64_t *__restrict__ pairs) {
-  const SetParams &sp = tv.sets[set];
+// RayIntersectedAll (ot_query.h:1929-2012): depth-first (last child first)
+// skipping nodes the ray misses; every entity hit whose enter distance is within
+// max_dist (any, when max_dist is 0) is kept.  Count pass, then a fill pass that
+// writes (enter distance, id) at the query's CSR offset; k_sort_band orders each
+// list ascending by enter distance (ties by id).
+template <int D, bool BOX, bool FILL>
+__device__ __forceinline__ void ray_all_query(const TreeView &tv, uint64_t qi, const double *__restrict__ rays,
+                                              const uint32_t *__restrict__ qset, double tol, double inc,
+                                              double max_dist, LdsLevelIter &stack, uint32_t *__restrict__ counts,
+                                              const uint64_t *__restrict__ offsets, double *__restrict__ out_d,
+                                              uint32_t *__restrict__ out_id) {
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  const uint64_t o = FILL ? offsets[qi] : 0;
+  uint32_t cnt = 0;
+  RayTester<D> rt;
+  if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
+    int top = 0;
+    uint32_t node = (uint32_t)sp.node_off;
+    double en, ex;
+    for (;;) {
+      if (rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex)) {
+        const NodeInfo ni = tv.info[node];
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          bool hit;
+          if (BOX) {
+            double lo[D], hi[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) lo[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
+#pragma unroll
+            for (int d = 0; d < D; ++d) hi[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
+            hit = rt.box_hit(lo, hi, &en, &ex);
+          } else {
+            double pnt[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) pnt[d] = tv.geom[(uint64_t)d * tv.n + j];
+            hit = rt.hit_point(pnt, &en, &ex);
+          }
+          if (!hit || !(max_dist == 0 || en <= max_dist)) continue;
+          if (FILL) {
+            out_d[o + cnt] = en;
+            out_id[o + cnt] = tv.ids[j];
+          }
+          ++cnt;
+        }
+        if (ni.ccount) {
+          stack.nx(top) = ni.cbegin + ni.ccount - 1;
+          stack.left(top) = (uint8_t)ni.ccount;
+          ++top;
+        }
+      }
+      if (top == 0) break;
+      node = stack.nx(top - 1)--;
+      if (--stack.left(top - 1) == 0) --top;
+    }
+  }
+  if (!FILL) counts[qi] = cnt;
+}
+
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(kQB) k_ray_all(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                                  const double *__restrict__ rays, const uint32_t *__restrict__ qset,
+                                                  const uint32_t *__restrict__ order, uint64_t nq, double tol,
+                                                  double inc, double max_dist, uint32_t *__restrict__ counts,
+                                                  const uint64_t *__restrict__ offsets, double *__restrict__ out_d,
+                                                  uint32_t *__restrict__ out_id) {
+  __shared__ uint32_t s_a[kRangeStack * kQB];
+  __shared__ uint8_t s_b[kRangeStack * kQB];
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  LdsLevelIter S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    ray_all_query<D, BOX, FILL>(tv, qi, rays, qset, tol, inc, max_dist, S, counts, offsets, out_d, out_id);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// collision (boxes): every entity of the left set queries the right set's tree
+// with its own box.  Self collision (SELF: the same set on both sides) keeps
+// partners with a larger id, so each unordered pair is found once; two trees
+// keep every (left, right) pair.  Pair predicate: GetBoxRelation == Overlapped
+// (adapters/general.h:262-289); node pruning: AreBoxesOverlappingByMinPoint on
+// the MBR (internal_geometry_module.h:148-161).
+// ---------------------------------------------------------------------------
+template <int D, bool FILL, bool SELF>
+__global__ void __launch_bounds__(kQB) k_collide(TreeView lv, uint32_t lset, TreeView rv, uint32_t rset, double tol,
+                                                  uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                                  uint64_t *__restrict__ pairs) {
+  const SetParams &lp = lv.sets[lset];
+  const SetParams &rp = rv.sets[rset];
   const uint64_t li = (uint64_t)blockIdx.x * kQB + threadIdx.x;
-  if (li >= sp.ent_cnt) return;
-  const uint64_t j0 = sp.ent_off + li;  // storage position
-  const uint32_t id0 = tv.ids[j0];
+  if (li >= lp.ent_cnt) return;
+  const uint64_t j0 = lp.ent_off + li;  // storage position
+  const uint32_t id0 = lv.ids[j0];
   double emn[D], emx[D], esz[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    emn[d] = tv.geom[(uint64_t)d * tv.n + j0];
-    emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j0];
+    emn[d] = lv.geom[(uint64_t)d * lv.n + j0];
+    emx[d] = lv.geom[(uint64_t)(D + d) * lv.n + j0];
     esz[d] = emx[d] - emn[d];
   }
   uint64_t o = FILL ? offsets[li] : 0;
   uint32_t cnt = 0;
   uint32_t stack[StackCap<D>::value];
   int top = 0;
-  stack[top++] = (uint32_t)sp.node_off;
+  if (rp.node_cnt != 0) stack[top++] = (uint32_t)rp.node_off;
   while (top > 0) {
     const uint32_t node = stack[--top];
-    const double *nm = &tv.nmin[(uint64_t)node * D];
-    const double *ns = &tv.nsize[(uint64_t)node * D];
+    const double *nm = &rv.nmin[(uint64_t)node * D];
+    const double *ns = &rv.nsize[(uint64_t)node * D];
     bool overlap = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -1431,15 +1632,15 @@ __global__ void __launch_bounds__(kQB) k_collide(TreeView tv, uint32_t set, doub
       if (maxL <= emn[d] - tol || maxR <= nm[d] - tol) overlap = false;
     }
     if (!overlap) continue;
-    const NodeInfo ni = tv.info[node];
+    const NodeInfo ni = rv.info[node];
     for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-      const uint32_t id1 = tv.ids[j];
-      if (id1 <= id0) continue;
+      const uint32_t id1 = rv.ids[j];
+      if (SELF && id1 <= id0) continue;
       double fmn[D], fmx[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        fmn[d] = tv.geom[(uint64_t)d * tv.n + j];
-        fmx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
+        fmn[d] = rv.geom[(uint64_t)d * rv.n + j];
+        fmx[d] = rv.geom[(uint64_t)(D + d) * rv.n + j];
       }
       if (boxes_overlap_strict<D>(emn, emx, fmn, fmx, tol)) {
         if (FILL) pairs[o + cnt] = ((uint64_t)id0 << 32) | id1;
@@ -1671,7 +1872,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   launch(ctx, "knn_band_fill", k_knn_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
          qs, (const uint32_t *)band.p, nb, (const double *)Fq.p, tol, (uint32_t *)nullptr, (const uint32_t *)boff.p,
          bd.p, bi.p);
-  launch(ctx, "knn_band_sort", k_sort_band<false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, nb,
+  launch(ctx, "knn_band_sort", k_sort_band<false, uint32_t>, dim3(div_up(nb, kQB)), dim3(kQB), 0, (const uint32_t *)boff.p, (uint64_t)nb,
          bd.p, bi.p);
   launch(ctx, "knn_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
          (const uint32_t *)bcnt.p, cnt.p);
@@ -1807,6 +2008,112 @@ Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint
   return range_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, mode, tol);
 }
 
+// Count pass, CSR offsets, fill pass into the result: the shape shared by pick
+// and ray-all.  count(counts) and fill(offsets, ids, dist) launch the walks.
+template <typename Count, typename Fill>
+static Result *count_fill_csr(Ctx *ctx, uint64_t nq, Count count, Fill fill) {
+  Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1));
+  count(cnt.p);
+  Result *r = new Result();
+  r->ctx = ctx;
+  r->nq = nq;
+  Buf<uint32_t> scan(ctx, nq + 1), total(ctx, 1);
+  exclusive_scan_u32(ctx, cnt.p, scan.p, nq, total.p);
+  uint32_t ht = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  r->nids = ht;
+  r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
+  r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
+  launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
+         r->d_offsets);
+  fill(r, ht);
+  return r;
+}
+
+template <int D, bool BOX>
+static Result *pick_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol) {
+  Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
+  const uint32_t *ord = order.p;
+  return count_fill_csr(
+      ctx, nq,
+      [&](uint32_t *cnt) {
+        launch(ctx, "pick_count", k_pick<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+               qs, ord, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr);
+      },
+      [&](Result *r, uint32_t) {
+        launch(ctx, "pick_fill", k_pick<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
+               ord, nq, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids);
+      });
+}
+
+Result *query_pick(const Tree *t, uint64_t nq, const double *picks, const uint32_t *sets, int mem, double tol) {
+  // the reference tree: its node boxes decide the walk (ot_query.h:433-437)
+  Ctx *ctx = run_ctx(t);
+  DevInput in;
+  stage_inputs(ctx, in, picks, sets, nq * t->dim, nq, mem);
+  const bool box = t->kind == OTG_KIND_BOX;
+  const ViewSet vs = single_view(t);
+#define OTG_PICK(DD) \
+  case DD: return box ? pick_impl<DD, true>(ctx, vs, nq, in.q, in.sets, tol) : pick_impl<DD, false>(ctx, vs, nq, in.q, in.sets, tol);
+  switch (t->dim) {
+    OTG_PICK(2)
+    OTG_PICK(3)
+    OTG_PICK(4)
+    OTG_PICK(5)
+    OTG_PICK(6)
+  }
+#undef OTG_PICK
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
+template <int D, bool BOX>
+static Result *ray_all_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol,
+                            double inc, double max_dist) {
+  Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
+  const uint32_t *ord = order.p;
+  Buf<double> dist;
+  Result *r = count_fill_csr(
+      ctx, nq,
+      [&](uint32_t *cnt) {
+        launch(ctx, "ray_all_count", k_ray_all<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views,
+               vs.qview, q, qs, ord, nq, tol, inc, max_dist, cnt, (const uint64_t *)nullptr, (double *)nullptr,
+               (uint32_t *)nullptr);
+      },
+      [&](Result *res, uint32_t total) {
+        dist = Buf<double>(ctx, std::max<uint32_t>(total, 1));
+        launch(ctx, "ray_all_fill", k_ray_all<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview,
+               q, qs, ord, nq, tol, inc, max_dist, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets, dist.p,
+               res->d_ids);
+      });
+  launch(ctx, "ray_all_sort", k_sort_band<false, uint64_t>, dim3(div_up(nq, kQB)), dim3(kQB), 0,
+         (const uint64_t *)r->d_offsets, nq, dist.p, r->d_ids);
+  return r;
+}
+
+Result *query_ray_all(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
+                      double inc, double max_dist) {
+  // the reference tree, as for the first hit (node hit tests decide the walk)
+  Ctx *ctx = run_ctx(t);
+  DevInput in;
+  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
+  const bool box = t->kind == OTG_KIND_BOX;
+  const ViewSet vs = single_view(t);
+#define OTG_RAY_ALL(DD)                                                                   \
+  case DD:                                                                                \
+    return box ? ray_all_impl<DD, true>(ctx, vs, nq, in.q, in.sets, tol, inc, max_dist)   \
+               : ray_all_impl<DD, false>(ctx, vs, nq, in.q, in.sets, tol, inc, max_dist);
+  switch (t->dim) {
+    OTG_RAY_ALL(2)
+    OTG_RAY_ALL(3)
+    OTG_RAY_ALL(4)
+    OTG_RAY_ALL(5)
+    OTG_RAY_ALL(6)
+  }
+#undef OTG_RAY_ALL
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
 template <int D, bool BOX>
 static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol,
                         double inc, double max_dist) {
@@ -1938,17 +2245,20 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
   return r;
 }
 
-template <int D>
-static Result *collide_impl(const Tree *t, uint32_t set, double tol) {
-  Ctx *ctx = run_ctx(t);
-  const SetParams &sp = t->sets[set];
-  const uint64_t ne = sp.ent_cnt;
-  TreeView tv = view_of(t);
+// Left set's entities query the right set's tree (SELF: the same set, partners
+// with a larger id).  Pairs (left id, right id) packed as left << jbits | right
+// and radix-sorted.
+template <int D, bool SELF>
+static Result *collide_impl(const Tree *lt, uint32_t lset, const Tree *rt, uint32_t rset, double tol) {
+  Ctx *ctx = run_ctx(lt);
+  const uint64_t ne = lt->sets[lset].ent_cnt;
+  const uint64_t nr = rt->sets[rset].ent_cnt;
+  TreeView lv = view_of(lt), rv = view_of(rt);
   Result *r = new Result();
   r->ctx = ctx;
   Buf<uint32_t> cnt(ctx, std::max<uint64_t>(ne, 1));
-  launch(ctx, "collide_count", k_collide<D, false>, dim3(div_up(ne, kQB)), dim3(kQB), 0, tv, set, tol, cnt.p,
-         (const uint64_t *)nullptr, (uint64_t *)nullptr);
+  launch(ctx, "collide_count", k_collide<D, false, SELF>, dim3(div_up(ne, kQB)), dim3(kQB), 0, lv, lset, rv, rset, tol,
+         cnt.p, (const uint64_t *)nullptr, (uint64_t *)nullptr);
   Buf<uint32_t> scan(ctx, ne + 1), total(ctx, 1);
   exclusive_scan_u32(ctx, cnt.p, scan.p, ne, total.p);
   uint32_t np = 0;
@@ -1958,9 +2268,10 @@ static Result *collide_impl(const Tree *t, uint32_t set, double tol) {
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(ne + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, ne, np,
          off.p);
   Buf<uint64_t> pairs(ctx, std::max<uint64_t>(np, 1));
-  launch(ctx, "collide_fill", k_collide<D, true>, dim3(div_up(ne, kQB)), dim3(kQB), 0, tv, set, tol, (uint32_t *)nullptr,
-         (const uint64_t *)off.p, pairs.p);
-  const uint32_t jbits = ne > 1 ? (64 - __builtin_clzll(ne - 1)) : 1;
+  launch(ctx, "collide_fill", k_collide<D, true, SELF>, dim3(div_up(ne, kQB)), dim3(kQB), 0, lv, lset, rv, rset, tol,
+         (uint32_t *)nullptr, (const uint64_t *)off.p, pairs.p);
+  const uint64_t idmax = std::max(ne, nr);
+  const uint32_t jbits = idmax > 1 ? (64 - __builtin_clzll(idmax - 1)) : 1;
   Buf<uint32_t> dummy(ctx, std::max<uint64_t>(np, 1));
   launch(ctx, "pack_pairs", k_pack_pairs, dim3(div_up(np, 256)), dim3(256), 0, pairs.p, (uint64_t)np, jbits);
   radix_sort_pairs(ctx, pairs.p, dummy.p, np, 2 * jbits);
@@ -1973,17 +2284,32 @@ static Result *collide_impl(const Tree *t, uint32_t set, double tol) {
   return r;
 }
 
+template <bool SELF>
+static Result *collide_dispatch(const Tree *lt, uint32_t lset, const Tree *rt, uint32_t rset, double tol) {
+  switch (lt->dim) {
+    case 2: return collide_impl<2, SELF>(lt, lset, rt, rset, tol);
+    case 3: return collide_impl<3, SELF>(lt, lset, rt, rset, tol);
+    case 4: return collide_impl<4, SELF>(lt, lset, rt, rset, tol);
+    case 5: return collide_impl<5, SELF>(lt, lset, rt, rset, tol);
+    case 6: return collide_impl<6, SELF>(lt, lset, rt, rset, tol);
+  }
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
 Result *query_collide(const Tree *t, uint32_t set, double tol) {
   if (t->kind != OTG_KIND_BOX) throw Error{OTG_ERR_UNSUPPORTED, "collision detection of point trees"};
   if (set >= t->sets.size()) throw Error{OTG_ERR_INVALID_ARG, "set index"};
-  switch (t->dim) {
-    case 2: return collide_impl<2>(t, set, tol);
-    case 3: return collide_impl<3>(t, set, tol);
-    case 4: return collide_impl<4>(t, set, tol);
-    case 5: return collide_impl<5>(t, set, tol);
-    case 6: return collide_impl<6>(t, set, tol);
-  }
-  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+  return collide_dispatch<true>(t, set, t, set, tol);
+}
+
+Result *query_collide_trees(const Tree *lt, uint32_t lset, const Tree *rt, uint32_t rset, double tol) {
+  if (lt->kind != OTG_KIND_BOX || rt->kind != OTG_KIND_BOX)
+    throw Error{OTG_ERR_UNSUPPORTED, "collision detection of point trees"};
+  if (lt->dim != rt->dim) throw Error{OTG_ERR_INVALID_ARG, "trees of different dimension"};
+  if (lset >= lt->sets.size() || rset >= rt->sets.size()) throw Error{OTG_ERR_INVALID_ARG, "set index"};
+  if (lt->ctx != rt->ctx && lt->ctx->device != rt->ctx->device)
+    throw Error{OTG_ERR_INVALID_ARG, "trees on different devices"};
+  return collide_dispatch<false>(lt, lset, rt, rset, tol);
 }
 
 }  // namespace otg

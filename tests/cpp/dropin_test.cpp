@@ -130,6 +130,28 @@ int main() {
   StaticQuadtreeBox q3(PAR_EXEC, b3, 3, std::nullopt, 2);
   EXPECT(same_set(q3.GetNearestNeighbors({2.0, 2.0}, 1, b3), {1}));
   EXPECT(same_set(q3.GetNearestNeighbors({-1.0, 1.0}, 1, b3), {0}));
+  // PickSearch KATs (general.tests.cpp:2928, :2947): QuadtreeBox(boxes, 3, nullopt, 2)
+  STEP("pick search");
+  std::vector<BoundingBox2D> b5 = {{{0, 0}, {1, 1}}, {{1, 1}, {2, 2}}, {{2, 2}, {3, 3}}, {{3, 3}, {4, 4}}, {{1.2, 1.2}, {2.8, 2.8}}};
+  StaticQuadtreeBox q5(PAR_EXEC, b5, 3, std::nullopt, 2);
+  EXPECT(same_set(q5.PickSearch({2.5, 2.5}, b5), {2, 4}));
+  EXPECT(same_set(q5.PickSearch({2.0, 2.0}, b5), {1, 2, 4}));
+  // RayIntersectedAll KATs (general.tests.cpp:1545, :1639-1669)
+  STEP("ray all");
+  std::vector<BoundingBox2D> r5 = {{{2.0, 0.0}, {3.0, 1.0}}, {{3.0, 1.0}, {4.0, 2.0}}, {{3.0, 2.0}, {4.0, 3.0}},
+                                   {{2.0, 3.0}, {4.0, 4.0}}, {{2.5, 2.5}, {3.5, 3.5}}};
+  StaticQuadtreeBox qr(PAR_EXEC, r5, 3, std::nullopt, 2);
+  const double h = 1.4142135623730951 * 0.5;
+  EXPECT((qr.RayIntersectedAll({2.7, 2.6}, {h, h}, r5, 0.0) == std::vector<index_t>{4, 2, 3}));
+  EXPECT(same_set(q5.RayIntersectedAll({1.1, 2.0}, {1.0, 0.0}, b5, 0.0), {1, 4, 2}));
+  EXPECT(same_set(q5.RayIntersectedAll({1.0, 1.9}, {1.0, 0.0}, b5, 0.901), {0, 1, 4, 2}));
+  // two-tree collision (general.tests.cpp:3081): { {3, 3}, {2, 4}, {3, 4} }
+  STEP("two-tree collision");
+  StaticQuadtreeBox ql(PAR_EXEC, boxes, 3, std::nullopt, 2);
+  auto dual = StaticQuadtreeBox::CollisionDetection(ql, boxes, qr, r5);
+  std::sort(dual.begin(), dual.end());
+  EXPECT((dual == std::vector<std::pair<index_t, index_t>>{{2, 4}, {3, 3}, {3, 4}}));
+  EXPECT((ql.CollisionDetection(boxes, qr, r5) == dual));
   if (failures == 0) std::printf("DROPIN OK\n");
   return failures ? 1 : 0;
 }
