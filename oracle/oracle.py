@@ -50,6 +50,8 @@ def lib() -> C.CDLL:
     L.oc_collision_trees.argtypes = [P, P, P, P, C.c_double]
     L.oc_pick.restype = P
     L.oc_pick.argtypes = [P, P, C.c_uint64, P, C.c_double, C.c_int]
+    L.oc_planes.restype = P
+    L.oc_planes.argtypes = [P, P, C.c_uint64, C.c_uint32, P, C.c_int, C.c_double, C.c_int]
     L.oc_ray_all.restype = P
     L.oc_ray_all.argtypes = [P, P, C.c_uint64, P, C.c_double, C.c_double, C.c_double, C.c_int]
     L.oc_result_nids.restype = C.c_uint64
@@ -172,6 +174,15 @@ class OracleTree:
     def pick(self, points, tol=BASE_TOLERANCE, threads=1):
         q = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, self.dim))
         return _result(lib().oc_pick(self.h, _p(self.geom), q.shape[0], _p(q), float(tol), int(threads)))
+
+    def planes(self, planes, mode, tol=BASE_TOLERANCE, threads=1):
+        """planes: (nq, P, 1 + D) = origo distance, normal; mode 0 PlaneSearch,
+        1 PlanePositiveSegmentation (P = 1), 2 FrustumCulling."""
+        q = np.ascontiguousarray(np.asarray(planes, np.float64))
+        if q.ndim == 2:
+            q = q[:, None, :]
+        return _result(lib().oc_planes(self.h, _p(self.geom), q.shape[0], q.shape[1], _p(q), int(mode), float(tol),
+                                       int(threads)))
 
     def ray_all(self, rays, tol=BASE_TOLERANCE, inc=0.0, max_dist=DBL_MAX, threads=1):
         q = np.ascontiguousarray(np.asarray(rays, np.float64).reshape(-1, 2 * self.dim))

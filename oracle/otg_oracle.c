@@ -1031,6 +1031,127 @@ static void oc_pick_one(const oc_tree *t, const double *geom, const double *p, d
 }
 
 /* ------------------------------------------------------------------------- */
+/* Planes: PlaneSearch, PlanePositiveSegmentation, FrustumCulling             */
+/* ------------------------------------------------------------------------- */
+enum { OC_REL_NEGATIVE = -1, OC_REL_HIT = 0, OC_REL_POSITIVE = 1 };
+
+/* internal_geometry_module.h:254-280 GetBoxPlaneRelationAD(minPoint, size, d, n, tol) */
+static int oc_box_plane_relation(const double *mn, const double *sz, double dist, const double *n, int D, double tol) {
+  double center[OC_MAXD], half[OC_MAXD];
+  for (int d = 0; d < D; ++d) {
+    half[d] = sz[d] * 0.5;
+    center[d] = mn[d] + half[d];
+  }
+  double radius = tol;
+  for (int d = 0; d < D; ++d) radius += half[d] * fabs(n[d]);
+  double dot = 0.0; /* DotAD(planeNormal, center) :180-188 */
+  for (int d = 0; d < D; ++d) dot += n[d] * center[d];
+  double cp = dot - dist;
+  if (cp + radius < 0.0) return OC_REL_NEGATIVE;
+  if (cp - radius > 0.0) return OC_REL_POSITIVE;
+  return OC_REL_HIT;
+}
+
+/* ot_query.h:585-602 GetEntityPlaneRelation: points GetPointPlaneRelation
+ * (general.h:322-335), boxes GetBoxPlaneRelationAD(min, max - min). */
+static int oc_entity_plane_relation(const oc_tree *t, const double *geom, uint32_t id, double dist, const double *n, double tol) {
+  int D = t->dim;
+  if (t->is_box) {
+    const double *e = &geom[(size_t)id * 2 * D];
+    double sz[OC_MAXD];
+    for (int d = 0; d < D; ++d) sz[d] = e[D + d] - e[d];
+    return oc_box_plane_relation(e, sz, dist, n, D, tol);
+  }
+  const double *p = &geom[(size_t)id * D];
+  double proj = 0.0;
+  for (int d = 0; d < D; ++d) proj += n[d] * p[d];
+  if (proj < dist - tol) return OC_REL_NEGATIVE;
+  if (proj > dist + tol) return OC_REL_POSITIVE;
+  return OC_REL_HIT;
+}
+
+enum { OC_PLANE_SEARCH = 0, OC_PLANE_POSITIVE = 1, OC_FRUSTUM = 2 };
+
+/* One query over planes[nplanes] (each: origo distance, then D normal components).
+ * OC_PLANE_SEARCH ot_query.h:612-636: DFS into nodes the plane hits, entities that
+ * the plane hits.  OC_PLANE_POSITIVE :691-741: nodes on the negative side are
+ * skipped, a node wholly on the positive side contributes its whole subtree (no
+ * entity test), otherwise entities not on the negative side.  OC_FRUSTUM :756-822:
+ * a node is visited unless a plane, met before any plane that hits it, has it on
+ * the negative side; an entity is kept when no plane has it on the negative side. */
+static void oc_planes_one(const oc_tree *t, const double *geom, const double *planes, uint32_t nplanes, int mode, double tol,
+                          oc_vec *out, uint32_t **stack, size_t *scap) {
+  int D = t->dim;
+  if (t->node_count == 0 || nplanes == 0) return;
+  size_t sp = 0;
+  if (*scap < 64) {
+    *scap = 64;
+    *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+  }
+  (*stack)[sp++] = 0;
+  while (sp) {
+    uint32_t node = (*stack)[--sp];
+    const double *mn = oc_nmin(t, node), *sz = oc_nsize(t, node);
+    int visit = 1, whole = 0;
+    if (mode == OC_FRUSTUM) {
+      for (uint32_t k = 0; k < nplanes; ++k) {
+        const double *pl = &planes[(size_t)k * (D + 1)];
+        int rel = oc_box_plane_relation(mn, sz, pl[0], pl + 1, D, tol);
+        if (rel == OC_REL_HIT) break;
+        if (rel == OC_REL_NEGATIVE) {
+          visit = 0;
+          break;
+        }
+      }
+    } else {
+      int rel = oc_box_plane_relation(mn, sz, planes[0], planes + 1, D, tol);
+      if (mode == OC_PLANE_SEARCH) visit = rel == OC_REL_HIT;
+      else if (rel == OC_REL_NEGATIVE) visit = 0;
+      else if (rel == OC_REL_POSITIVE) whole = 1;
+    }
+    if (!visit) continue;
+    if (whole) {
+      /* nested DFS over the subtree, every node's entities */
+      size_t base = sp;
+      (*stack)[sp++] = node;
+      while (sp > base) {
+        uint32_t s = (*stack)[--sp];
+        for (uint32_t j = t->node_ebegin[s]; j < t->node_ebegin[s] + t->node_ecount[s]; ++j) vec_push(out, t->entity_ids[j]);
+        for (uint32_t c = t->node_cbegin[s]; c < t->node_cbegin[s] + t->node_ccount[s]; ++c) {
+          if (sp == *scap) {
+            *scap *= 2;
+            *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+          }
+          (*stack)[sp++] = c;
+        }
+      }
+      continue;
+    }
+    for (uint32_t j = t->node_ebegin[node]; j < t->node_ebegin[node] + t->node_ecount[node]; ++j) {
+      uint32_t id = t->entity_ids[j];
+      int ok;
+      if (mode == OC_PLANE_SEARCH) {
+        ok = oc_entity_plane_relation(t, geom, id, planes[0], planes + 1, tol) == OC_REL_HIT;
+      } else {
+        ok = 1;
+        for (uint32_t k = 0; k < nplanes && ok; ++k) {
+          const double *pl = &planes[(size_t)k * (D + 1)];
+          ok = oc_entity_plane_relation(t, geom, id, pl[0], pl + 1, tol) != OC_REL_NEGATIVE;
+        }
+      }
+      if (ok) vec_push(out, id);
+    }
+    for (uint32_t c = t->node_cbegin[node]; c < t->node_cbegin[node] + t->node_ccount[node]; ++c) {
+      if (sp == *scap) {
+        *scap *= 2;
+        *stack = (uint32_t *)realloc(*stack, *scap * 4u);
+      }
+      (*stack)[sp++] = c;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Ray first hit                                                              */
 /* ------------------------------------------------------------------------- */
 typedef struct {
@@ -1472,7 +1593,7 @@ OC_API oc_result *oc_collision_trees(const oc_tree *tl, const double *gl, const 
 /* ------------------------------------------------------------------------- */
 /* Batched query drivers (pthreads over contiguous query chunks)              */
 /* ------------------------------------------------------------------------- */
-enum { OC_Q_KNN = 0, OC_Q_RANGE = 1, OC_Q_RAY = 2, OC_Q_PICK = 3, OC_Q_RAY_ALL = 4 };
+enum { OC_Q_KNN = 0, OC_Q_RANGE = 1, OC_Q_RAY = 2, OC_Q_PICK = 3, OC_Q_RAY_ALL = 4, OC_Q_PLANES = 5 };
 
 typedef struct {
   const oc_tree *t;
@@ -1482,6 +1603,7 @@ typedef struct {
   uint32_t k;
   double max_within, tol, inc, max_dist;
   int overlap_mode;
+  uint32_t nplanes; /* OC_Q_PLANES: planes per query; overlap_mode holds the plane mode */
   uint64_t q0, q1;
   oc_vec out;
   uint64_t *counts;
@@ -1519,6 +1641,9 @@ static void *oc_job_run(void *arg) {
       oc_range_one(j->t, j->geom, &j->queries[q * 2 * D], j->overlap_mode, j->tol, &j->out, &stack, &scap);
     } else if (j->kind == OC_Q_PICK) {
       oc_pick_one(j->t, j->geom, &j->queries[q * D], j->tol, &j->out, &stack, &scap);
+    } else if (j->kind == OC_Q_PLANES) {
+      oc_planes_one(j->t, j->geom, &j->queries[q * (size_t)j->nplanes * (D + 1)], j->nplanes, j->overlap_mode, j->tol, &j->out,
+                    &stack, &scap);
     } else if (j->kind == OC_Q_RAY_ALL) {
       const double *qr = &j->queries[q * 2 * D];
       oc_ray_all_one(j->t, j->geom, qr, qr + D, j->tol, j->inc, j->max_dist, &j->out, &stack, &scap);
@@ -1618,6 +1743,21 @@ OC_API oc_result *oc_pick(const oc_tree *t, const double *geom, uint64_t nq, con
   p.geom = geom;
   p.queries = picks;
   p.kind = OC_Q_PICK;
+  p.tol = tol;
+  return oc_run_jobs(&p, nq, nthreads);
+}
+
+/* planes: nq x nplanes x (origo distance, normal[D]); mode 0 PlaneSearch, 1
+ * PlanePositiveSegmentation (nplanes 1), 2 FrustumCulling */
+OC_API oc_result *oc_planes(const oc_tree *t, const double *geom, uint64_t nq, uint32_t nplanes, const double *planes, int mode,
+                            double tol, int nthreads) {
+  oc_job p = {0};
+  p.t = t;
+  p.geom = geom;
+  p.queries = planes;
+  p.kind = OC_Q_PLANES;
+  p.nplanes = nplanes;
+  p.overlap_mode = mode;
   p.tol = tol;
   return oc_run_jobs(&p, nq, nthreads);
 }

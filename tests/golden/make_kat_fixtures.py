@@ -24,6 +24,23 @@ INV_SQRT3 = float("0.577350269189625764509148780502")  # std::numbers::inv_sqrt3
 INV_SQRT5 = 1.0 / math.sqrt(5.0)  # 1.0 / std::sqrt(5.0)
 
 
+SQRT2 = math.sqrt(2.0)  # std::sqrt(2.0)
+F32_001 = 0.009999999776482582  # 0.01f widened to double
+
+# PlaneSearch_3D_* general.tests.cpp:2407-2470
+_PLANE_PTS_A = [[2.0, -1.0, 0.0], [3.0, -2.0, 1.0], [4.0, -3.0, 0.02], [4.0, -2.0, 0.0], [5.0, 0.0, 0.00999]]
+_PLANE_PTS_B = _PLANE_PTS_A[:4] + [[5.0, 0.0, 1.00999]]
+# PointGeneral3D adapter.tests.cpp:194-198
+_PTS10 = [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [2.0, 2.0, 2.0], [3.0, 3.0, 3.0], [4.0, 4.0, 4.0], [0.0, 0.0, 4.0],
+          [0.0, 4.0, 0.0], [4.0, 0.0, 0.0], [1.5, 1.5, 1.0], [1.0, 1.5, 1.5]]
+
+
+def _plane_boxes(z3lo, z3hi, z4lo, z4hi):
+    """The five boxes of the PlaneIntersection_3D_* tests (general.tests.cpp:2966-3075); boxes 3 and 4 vary in z."""
+    return [[0.0, 0.0, 0.0, 1.0, 1.0, 0.0], [1.0, 1.0, -1.0, 2.0, 2.0, 0.0], [2.0, 2.0, -1.0, 3.0, 3.0, -0.2],
+            [3.0, 3.0, z3lo, 4.0, 4.0, z3hi], [1.2, 1.2, z4lo, 2.8, 2.8, z4hi]]
+
+
 def lift_1d(boxes):
     """1-D boxes [lo, hi] of a reference test as 2-D boxes with the second axis [0, 1] for every box.
 
@@ -208,6 +225,38 @@ fixtures = {
          "expect": [[0, 0], [0, 1], [2, 0], [2, 1], [0, 3], [2, 3], [5, 0], [5, 1], [5, 3], [0, 4], [2, 4], [6, 0], [6, 1],
                     [6, 4]],
          "src": "general.tests.cpp:2235 (1-D, lifted)"},
+    ],
+    "planes": [  # PlaneSearch / PlaneIntersection / FrustumCulling KATs; expected sets (tests compare permutations)
+        # planes: [origo distance, normal...] each; mode 0 PlaneSearch, 2 FrustumCulling
+        {"kind": "point", "dim": 3, "geom": _PLANE_PTS_A, "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[0.0, 0.0, 0.0, 1.0]], "tol": 0.01, "expect": [0, 3, 4], "src": "general.tests.cpp:2419"},
+        {"kind": "point", "dim": 3, "geom": _PLANE_PTS_B, "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[1.0, 0.0, 0.0, 1.0]], "tol": 0.01, "expect": [1, 4], "src": "general.tests.cpp:2436"},
+        {"kind": "point", "dim": 3, "geom": _PLANE_PTS_B, "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[4.0, 1.0, 0.0, 0.0]], "tol": 0.01, "expect": [2, 3], "src": "general.tests.cpp:2453"},
+        {"kind": "box", "dim": 3, "geom": _plane_boxes(1.0, 1.2, -1.0, 1.0), "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[0.0, 0.0, 0.0, 1.0]], "tol": 0.01, "expect": [0, 1, 4], "src": "general.tests.cpp:2977"},
+        {"kind": "box", "dim": 3, "geom": _plane_boxes(0.8, 0.9, -1.0, 1.1), "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[1.0, 0.0, 0.0, 1.0]], "tol": 0.01, "expect": [4], "src": "general.tests.cpp:2994"},
+        {"kind": "box", "dim": 3, "geom": _plane_boxes(1.0, 1.2, -1.1, 1.1), "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[-1.0, 0.0, 0.0, 1.0]], "tol": 0.01, "expect": [1, 2, 4], "src": "general.tests.cpp:3010"},
+        {"kind": "box", "dim": 3, "geom": _plane_boxes(1.0, 1.2, -1.1, 1.1), "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[1.0, 1.0, 0.0, 0.0]], "tol": 0.01, "expect": [0, 1], "src": "general.tests.cpp:3027"},
+        {"kind": "box", "dim": 3, "geom": _plane_boxes(1.0, 1.2, -1.1, 1.1), "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[-1.0, 1.0, 0.0, 0.0]], "tol": 0.01, "expect": [], "src": "general.tests.cpp:3043"},
+        {"kind": "box", "dim": 3, "geom": _plane_boxes(1.0, 1.2, -1.1, 1.1), "depth": 3, "cap": 2, "mode": 0,
+         "planes": [[SQRT2, 0.0, SQRT2 * 0.5, SQRT2 * 0.5]], "tol": 0.01, "expect": [1, 2, 4],
+         "src": "general.tests.cpp:3075"},
+        {"kind": "point", "dim": 3, "geom": _PTS10, "depth": 3, "cap": 1, "mode": 0,
+         "planes": [[2.6, 1.0 / math.sqrt(3.0), 1.0 / math.sqrt(3.0), 1.0 / math.sqrt(3.0)]], "tol": 0.3,
+         "expect": [5, 6, 7, 8, 9], "src": "adapter.tests.cpp:208"},
+        {"kind": "point", "dim": 3, "geom": _PTS10, "depth": 3, "cap": 1, "mode": 2,
+         "planes": [[2 * SQRT2, 1.0 / SQRT2, 1.0 / SQRT2, 0.0], [2 * SQRT2, 0.0, 1.0 / SQRT2, 1.0 / SQRT2],
+                    [0.0, 1.0 / SQRT2, -1.0 / SQRT2, 0.0]], "tol": 0.01, "expect": [2, 3, 4],
+         "src": "adapter.tests.cpp:217 (Eigen planes: origo distance = -offset)"},
+        {"kind": "box", "dim": 2, "geom": boxes5, "depth": 3, "cap": 1, "mode": 2,
+         "planes": [[2 * SQRT2, 1.0 / SQRT2, 1.0 / SQRT2], [-2.0, 0.0, -1.0]], "tol": F32_001, "expect": [1, 2, 4],
+         "src": "adapter.tests.cpp:283 (tolerance 0.01f)"},
     ],
     "issue38_collision": {  # general.tests.cpp:4728-4772: 10 rounds x 1000 loose 3D boxes, mt19937(0); vs brute force
         "rounds": 10, "box_no": 1000, "space_max": 8.0, "size_max": 2.0, "depth": 10, "cap": 5,

@@ -238,6 +238,27 @@ def ray_hit_points(ray, pts, tol, inc):
     return hit, np.maximum(0.0, t - ct), t + ct
 
 
+def plane_relation(geom, is_box, dist, normal, tol):
+    """Entity-plane relation over all entities (-1 negative, 0 hit, 1 positive):
+    boxes GetBoxPlaneRelationAD(min, max - min) (internal_geometry_module.h:254-280),
+    points GetPointPlaneRelation (general.h:322-335), same operation order."""
+    D = len(normal)
+    if is_box:
+        radius = np.full(geom.shape[0], tol)
+        dot = np.zeros(geom.shape[0])
+        half = [(geom[:, D + d] - geom[:, d]) * 0.5 for d in range(D)]
+        for d in range(D):
+            radius = radius + half[d] * abs(normal[d])
+        for d in range(D):
+            dot = dot + normal[d] * (geom[:, d] + half[d])
+        cp = dot - dist
+        return np.where(cp + radius < 0.0, -1, np.where(cp - radius > 0.0, 1, 0))
+    proj = np.zeros(geom.shape[0])
+    for d in range(D):
+        proj = proj + normal[d] * geom[:, d]
+    return np.where(proj < dist - tol, -1, np.where(proj > dist + tol, 1, 0))
+
+
 def boxes_overlap_strict(a, B, tol):
     """general.h:262-289 GetBoxRelation(a, B[i]) == Overlapped, vectorised over B."""
     D = a.shape[0] // 2

@@ -7,7 +7,7 @@ import pytest
 
 from helpers import (BASE_TOL, DBL_MAX, MT19937, box_minmax, check_node_table_rules, collision_brute,
                      collision_two_brute, gaussian_mixture, issue38_boxes, kat, knn_box_brute, knn_brute,
-                     lognormal_boxes, minmaxdist_textbook, pick_brute, range_brute_points, ray_hit_points,
+                     lognormal_boxes, minmaxdist_textbook, pick_brute, plane_relation, range_brute_points, ray_hit_points,
                      uniform_points, unit_dirs)
 from oracle import oracle as O
 
@@ -424,3 +424,46 @@ def test_collision_two_trees_vs_brute(D):
     # and the other way round: the same pairs with the sides swapped
     back = tr.collide_with(tl).astype(np.int64)
     assert np.array_equal(back[np.lexsort((back[:, 0], back[:, 1]))][:, ::-1], got)
+
+
+# ---- PlaneSearch / PlanePositiveSegmentation / FrustumCulling (ot_query.h:612-822) ------------------
+def test_plane_kats():
+    for c in kat()["planes"]:
+        t = O.OracleTree(c["dim"], c["kind"], c["geom"], None, c["depth"], c["cap"])
+        off, ids = t.planes([c["planes"]], c["mode"], c["tol"])
+        assert sorted(ids.tolist()) == sorted(c["expect"]), c["src"]
+
+
+def _random_planes(n, D, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    nrm = unit_dirs(n, D, seed=seed + 1)
+    pts = rng.uniform(lo, hi, (n, D))
+    dist = np.einsum("ij,ij->i", nrm, pts)
+    return np.concatenate([dist[:, None], nrm], axis=1)
+
+
+@pytest.mark.parametrize("kind,D,tol", [("point", 3, 0.5), ("box", 3, BASE_TOL), ("box", 2, 0.0), ("point", 4, 1.0)])
+def test_planes_vs_brute(kind, D, tol):
+    if kind == "point":
+        geom = uniform_points(3000, D, seed=101)
+    else:
+        geom = lognormal_boxes(3000, D, 0.0, 100.0, np.log(2.0), 0.6, seed=102)
+    t = O.OracleTree(D, kind, geom, None, 6, 8)
+    pl = _random_planes(120, D, 0.0, 100.0, seed=103)
+    pl[0, 0] = -1e6  # every entity on the positive side: the positive-node shortcut at the root
+    for mode in (0, 1):
+        off, ids = t.planes(pl, mode, tol)
+        got = O.lists(off, ids)
+        for qi in range(len(pl)):
+            rel = plane_relation(geom, kind == "box", pl[qi, 0], pl[qi, 1:], tol)
+            exp = np.nonzero(rel == 0)[0] if mode == 0 else np.nonzero(rel != -1)[0]
+            assert sorted(got[qi]) == exp.tolist(), (mode, qi)
+    # frustums of 4 planes each
+    fr = _random_planes(4 * 60, D, 20.0, 80.0, seed=104).reshape(60, 4, D + 1)
+    off, ids = t.planes(fr, 2, tol)
+    got = O.lists(off, ids)
+    for qi in range(len(fr)):
+        ok = np.ones(geom.shape[0], bool)
+        for k in range(4):
+            ok &= plane_relation(geom, kind == "box", fr[qi, k, 0], fr[qi, k, 1:], tol) != -1
+        assert sorted(got[qi]) == np.nonzero(ok)[0].tolist(), qi
