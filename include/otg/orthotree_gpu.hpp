@@ -11,6 +11,8 @@
 //                                                         (core/ot_query.h:1208-1270, :476-583, :2035-2146, :1901-1906)
 //   PickSearch, RayIntersectedAll, two-tree CollisionDetection
 //                                                         (core/ot_query.h:426-466, :1929-2022, :1462-1496)
+//   PlaneSearch / PlaneIntersection, PlanePositiveSegmentation, FrustumCulling
+//                                                         (core/ot_query.h:612-822)
 //   GetNodeCount / GetNodeChildren / GetNodeEntities / GetNodeMinPoint / GetNodeSize / GetNodeBox
 //                                                         (core/ot_static_linear_core.h:178-268)
 // plus batch overloads over spans of queries returning CSR results.
@@ -60,6 +62,13 @@ template <int D>
 struct RayND {
   PointND<D> Origin, Direction;
 };
+// Plane equation dot(Normal, x) = OrigoDistance, Normal of unit length (adapters/general.h:364-368)
+template <int D>
+struct PlaneND {
+  double OrigoDistance;
+  PointND<D> Normal;
+};
+static_assert(sizeof(PlaneND<3>) == 4 * sizeof(double), "plane layout: origo distance, then the normal");
 using Point2D = PointND<2>;
 using Point3D = PointND<3>;
 using BoundingBox2D = BoundingBoxND<2>;
@@ -113,6 +122,7 @@ class StaticOrthoTreeGPU {
   using TVector = PointND<DIM>;
   using TBox = BoundingBoxND<DIM>;
   using TRay = RayND<DIM>;
+  using TPlane = PlaneND<DIM>;
   using Entity = std::conditional_t<IS_BOX, TBox, TVector>;
   using EntityContainerView = std::span<const Entity>;
   using NodeID = uint32_t;
@@ -260,6 +270,43 @@ class StaticOrthoTreeGPU {
     return take(res);
   }
 
+  // PlaneSearch / PlaneIntersection (ot_query.h:612-688), PlanePositiveSegmentation
+  // (:691-753), FrustumCulling (:756-822)
+  std::vector<EntityID> PlaneSearch(double distanceOfOrigo, TVector const &planeNormal, EntityContainerView,
+                                    double tolerance = BASE_TOLERANCE) const noexcept {
+    return planes_one(distanceOfOrigo, planeNormal, OTG_PLANE_SEARCH, tolerance);
+  }
+  std::vector<EntityID> PlaneSearch(TPlane const &plane, EntityContainerView entities,
+                                    double tolerance = BASE_TOLERANCE) const noexcept {
+    return PlaneSearch(plane.OrigoDistance, plane.Normal, entities, tolerance);
+  }
+  std::vector<EntityID> PlaneIntersection(double distanceOfOrigo, TVector const &planeNormal, EntityContainerView entities,
+                                          double tolerance = BASE_TOLERANCE) const noexcept {
+    return PlaneSearch(distanceOfOrigo, planeNormal, entities, tolerance);
+  }
+  std::vector<EntityID> PlaneIntersection(TPlane const &plane, EntityContainerView entities,
+                                          double tolerance = BASE_TOLERANCE) const noexcept {
+    return PlaneSearch(plane, entities, tolerance);
+  }
+  std::vector<EntityID> PlanePositiveSegmentation(double distanceOfOrigo, TVector const &planeNormal, EntityContainerView,
+                                                  double tolerance = BASE_TOLERANCE) const noexcept {
+    return planes_one(distanceOfOrigo, planeNormal, OTG_PLANE_POSITIVE, tolerance);
+  }
+  std::vector<EntityID> PlanePositiveSegmentation(TPlane const &plane, EntityContainerView entities,
+                                                  double tolerance = BASE_TOLERANCE) const noexcept {
+    return PlanePositiveSegmentation(plane.OrigoDistance, plane.Normal, entities, tolerance);
+  }
+  std::vector<EntityID> FrustumCulling(std::span<TPlane const> const &boundaryPlanes, EntityContainerView,
+                                       double tolerance = BASE_TOLERANCE) const noexcept {
+    if (!tree_ || boundaryPlanes.empty()) return {};
+    otg_result *res = nullptr;
+    detail::check(otg_planes(tree_, 1, (uint32_t)boundaryPlanes.size(), reinterpret_cast<const double *>(boundaryPlanes.data()),
+                             nullptr, OTG_MEM_HOST, OTG_PLANE_FRUSTUM, tolerance, &res),
+                  "otg_planes", ctx().get());
+    BatchResult r = take(res);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
+  }
+
   // PickSearch (ot_query.h:426-466)
   std::vector<EntityID> PickSearch(TVector const &pickPoint, EntityContainerView,
                                    double tolerance = BASE_TOLERANCE) const noexcept {
@@ -377,6 +424,17 @@ class StaticOrthoTreeGPU {
     TVector v;
     for (int d = 0; d < DIM; ++d) v[d] = a[(size_t)node * DIM + d];
     return v;
+  }
+
+  std::vector<EntityID> planes_one(double distanceOfOrigo, TVector const &planeNormal, int mode,
+                                   double tolerance) const noexcept {
+    if (!tree_) return {};
+    TPlane pl{distanceOfOrigo, planeNormal};
+    otg_result *res = nullptr;
+    detail::check(otg_planes(tree_, 1, 1, reinterpret_cast<const double *>(&pl), nullptr, OTG_MEM_HOST, mode, tolerance, &res),
+                  "otg_planes", ctx().get());
+    BatchResult r = take(res);
+    return std::vector<EntityID>(r.ids.begin(), r.ids.end());
   }
 
   static std::vector<std::pair<EntityID, EntityID>> pairs(BatchResult const &r) {

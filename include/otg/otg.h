@@ -54,6 +54,10 @@ typedef int otg_status;
 #define OTG_RANGE_INSIDE 0  /* RangeSearchMode::Inside  (core/types.h) */
 #define OTG_RANGE_OVERLAP 1 /* RangeSearchMode::Overlap */
 
+#define OTG_PLANE_SEARCH 0   /* PlaneSearch / PlaneIntersection: entities the plane hits */
+#define OTG_PLANE_POSITIVE 1 /* PlanePositiveSegmentation: entities not on the negative side */
+#define OTG_PLANE_FRUSTUM 2  /* FrustumCulling: entities on no boundary plane's negative side */
+
 typedef struct otg_ctx otg_ctx;
 typedef struct otg_tree otg_tree;
 typedef struct otg_result otg_result;
@@ -172,6 +176,14 @@ OTG_API otg_status otg_collide_trees(const otg_tree *left, uint32_t left_set, co
  * contain the point, or points within tolerance of it.  tolerance >= 0. */
 OTG_API otg_status otg_pick(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
                             double tolerance, otg_result **out);
+
+/* PlaneSearch(distanceOfOrigo, planeNormal, entities, tolerance) (core/ot_query.h:612-636),
+ * PlanePositiveSegmentation (:691-741) and FrustumCulling(boundaryPlanes, entities,
+ * tolerance) (:756-822).  planes: nq x nplanes x (1 + dim) doubles, each plane its origo
+ * distance then its unit normal (PlaneND, adapters/general.h:364-368); nplanes must be 1 for
+ * OTG_PLANE_SEARCH and OTG_PLANE_POSITIVE; a frustum with no planes gives nothing. */
+OTG_API otg_status otg_planes(const otg_tree *tree, uint64_t nq, uint32_t nplanes, const double *planes,
+                              const uint32_t *set_ids, int mem, int mode, double tolerance, otg_result **out);
 
 /* RayIntersectedAll(origin, direction, entities, tolerance, toleranceIncrement,
  * maxExaminationDistance) (core/ot_query.h:1929-2022): every entity the ray hits

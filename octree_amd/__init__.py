@@ -351,6 +351,29 @@ class StaticTree:
         _check(st, "otg_pick", self.ctx)
         return Result(h, self.ctx)
 
+    def planes_batch(self, planes, mode: int = _abi.OTG_PLANE_SEARCH, tolerance: float = BASE_TOLERANCE,
+                     set_ids=None) -> Result:
+        """planes: (nq, P, 1 + D) -- origo distance, unit normal -- or (nq, 1 + D) for one
+        plane per query.  mode: OTG_PLANE_SEARCH, OTG_PLANE_POSITIVE or OTG_PLANE_FRUSTUM."""
+        if _is_device(planes):
+            nplanes = 1 if planes.dim() == 2 else planes.shape[1]
+            qp, nq = _dev(planes.reshape(planes.shape[0], -1), nplanes * (self.dim + 1))
+            sp = None if set_ids is None else C.c_void_p(set_ids.data_ptr())
+            mem, keep = _abi.OTG_MEM_DEVICE, (planes, set_ids)
+        else:
+            a = np.asarray(planes, dtype=np.float64)
+            if a.ndim == 2:
+                a = a[:, None, :]
+            nplanes = a.shape[1]
+            qa = np.ascontiguousarray(a.reshape(a.shape[0], -1))
+            nq = qa.shape[0]
+            sids = self._sets_arg(set_ids, nq)
+            qp, sp, mem, keep = _ptr(qa), _ptr(sids), _abi.OTG_MEM_HOST, (qa, sids)
+        h = C.c_void_p()
+        st = _abi.lib().otg_planes(self.handle, nq, int(nplanes), qp, sp, mem, int(mode), float(tolerance), C.byref(h))
+        _check(st, "otg_planes", self.ctx)
+        return Result(h, self.ctx)
+
     def ray_all_batch(self, rays, tolerance: float = BASE_TOLERANCE, tolerance_increment: float = 0.0,
                       max_distance: float = DBL_MAX, set_ids=None) -> Result:
         qp, nq, sp, mem, _keep = self._query_args(rays, 2 * self.dim, set_ids)
@@ -402,6 +425,24 @@ class StaticTree:
 
     def PickSearch(self, pick_point, entities=None, tolerance: float = BASE_TOLERANCE):
         return self.pick_batch([np.asarray(pick_point, dtype=np.float64).reshape(-1)], tolerance).lists()[0]
+
+    def PlaneSearch(self, distance_of_origo: float, plane_normal, entities=None, tolerance: float = BASE_TOLERANCE):
+        pl = np.concatenate([[float(distance_of_origo)], np.asarray(plane_normal, np.float64).reshape(-1)])
+        return self.planes_batch([pl], _abi.OTG_PLANE_SEARCH, tolerance).lists()[0]
+
+    PlaneIntersection = PlaneSearch
+
+    def PlanePositiveSegmentation(self, distance_of_origo: float, plane_normal, entities=None,
+                                  tolerance: float = BASE_TOLERANCE):
+        pl = np.concatenate([[float(distance_of_origo)], np.asarray(plane_normal, np.float64).reshape(-1)])
+        return self.planes_batch([pl], _abi.OTG_PLANE_POSITIVE, tolerance).lists()[0]
+
+    def FrustumCulling(self, boundary_planes, entities=None, tolerance: float = BASE_TOLERANCE):
+        """boundary_planes: (P, 1 + D) rows of (origo distance, unit normal)."""
+        pl = np.asarray(boundary_planes, np.float64).reshape(1, -1, self.dim + 1)
+        if pl.shape[1] == 0:
+            return []
+        return self.planes_batch(pl, _abi.OTG_PLANE_FRUSTUM, tolerance).lists()[0]
 
     def RayIntersectedAll(self, origin, direction, entities=None, tolerance: float = BASE_TOLERANCE,
                           tolerance_increment: float = 0.0, max_distance: float = DBL_MAX):

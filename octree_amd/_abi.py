@@ -17,6 +17,9 @@ OTG_KIND_POINT = 0
 OTG_KIND_BOX = 1
 OTG_RANGE_INSIDE = 0
 OTG_RANGE_OVERLAP = 1
+OTG_PLANE_SEARCH = 0
+OTG_PLANE_POSITIVE = 1
+OTG_PLANE_FRUSTUM = 2
 
 STATUS_NAMES = {
     0: "OTG_OK",
@@ -35,7 +38,7 @@ EXPORTED = [
     "otg_ctx_mem_stats", "otg_ctx_release_cache", "otg_ctx_set_profiling", "otg_ctx_kernel_times",
     "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_free", "otg_tree_get_info",
     "otg_tree_get_set_info", "otg_tree_export", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
-    "otg_collide_trees", "otg_pick", "otg_ray_all",
+    "otg_collide_trees", "otg_pick", "otg_ray_all", "otg_planes",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
     "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_stats",
@@ -104,6 +107,7 @@ def lib() -> C.CDLL:
         "otg_collide": (C.c_int, [P, C.c_uint32, C.c_double, PP]),
         "otg_collide_trees": (C.c_int, [P, C.c_uint32, P, C.c_uint32, C.c_double, PP]),
         "otg_pick": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, PP]),
+        "otg_planes": (C.c_int, [P, C.c_uint64, C.c_uint32, P, P, C.c_int, C.c_int, C.c_double, PP]),
         "otg_ray_all": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, C.c_double, C.c_double, PP]),
         "otg_result_get_info": (C.c_int, [P, U64P, U64P]),
         "otg_result_copy": (C.c_int, [P, P, P]),

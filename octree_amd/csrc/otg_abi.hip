@@ -451,6 +451,19 @@ OTG_API otg_status otg_pick(const otg_tree *t, uint64_t nq, const double *points
   });
 }
 
+OTG_API otg_status otg_planes(const otg_tree *t, uint64_t nq, uint32_t nplanes, const double *planes,
+                              const uint32_t *set_ids, int mem, int mode, double tolerance, otg_result **out) {
+  if (!t || !out || (nq && nplanes && !planes)) return OTG_ERR_INVALID_ARG;
+  if (mode != OTG_PLANE_SEARCH && mode != OTG_PLANE_POSITIVE && mode != OTG_PLANE_FRUSTUM) return OTG_ERR_INVALID_ARG;
+  if (mode != OTG_PLANE_FRUSTUM && nplanes != 1) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = static_cast<otg_result *>(otg::query_planes(t, nq, nplanes, planes, set_ids, mem, mode, tolerance));
+  });
+}
+
 OTG_API otg_status otg_ray_all(const otg_tree *t, uint64_t nq, const double *rays, const uint32_t *set_ids, int mem,
                                double tolerance, double tolerance_increment, double max_distance, otg_result **out) {
   if (!t || !out || (nq && !rays)) return OTG_ERR_INVALID_ARG;

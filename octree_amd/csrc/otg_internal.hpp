@@ -235,6 +235,8 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
 Result *query_collide(const Tree *t, uint32_t set, double tol);
 Result *query_collide_trees(const Tree *lt, uint32_t lset, const Tree *rt, uint32_t rset, double tol);
 Result *query_pick(const Tree *t, uint64_t nq, const double *picks, const uint32_t *sets, int mem, double tol);
+Result *query_planes(const Tree *t, uint64_t nq, uint32_t nplanes, const double *planes, const uint32_t *sets, int mem,
+                     int mode, double tol);
 Result *query_ray_all(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
                       double inc, double max_dist);
 

@@ -937,6 +937,150 @@ __global__ void __launch_bounds__(256) k_range_copy(const RangeCopy *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// planes: PlaneSearch, PlanePositiveSegmentation, FrustumCulling
+// ---------------------------------------------------------------------------
+// internal_geometry_module.h:254-280 GetBoxPlaneRelationAD(minPoint, size, d, n, tol):
+// -1 negative side, 0 hit, 1 positive side.  pl = (origo distance, normal[D]).
+template <int D>
+__device__ __forceinline__ int box_plane_relation(const double (&mn)[D], const double (&sz)[D], const double *pl,
+                                                  double tol) {
+  double half[D], center[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    half[d] = sz[d] * 0.5;
+    center[d] = mn[d] + half[d];
+  }
+  double radius = tol;
+#pragma unroll
+  for (int d = 0; d < D; ++d) radius += half[d] * fabs(pl[1 + d]);
+  double dot = 0.0;  // DotAD(planeNormal, center) :180-188
+#pragma unroll
+  for (int d = 0; d < D; ++d) dot += pl[1 + d] * center[d];
+  const double cp = dot - pl[0];
+  if (cp + radius < 0.0) return -1;
+  if (cp - radius > 0.0) return 1;
+  return 0;
+}
+
+// ot_query.h:585-602 GetEntityPlaneRelation: points GetPointPlaneRelation
+// (general.h:322-335), boxes GetBoxPlaneRelationAD(min, max - min)
+template <int D, bool BOX>
+__device__ __forceinline__ int entity_plane_relation(const TreeView &tv, uint32_t j, const double *pl, double tol) {
+  if (BOX) {
+    double mn[D], sz[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      mn[d] = tv.geom[(uint64_t)d * tv.n + j];
+      sz[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - mn[d];
+    }
+    return box_plane_relation<D>(mn, sz, pl, tol);
+  }
+  double proj = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) proj += pl[1 + d] * tv.geom[(uint64_t)d * tv.n + j];
+  if (proj < pl[0] - tol) return -1;
+  if (proj > pl[0] + tol) return 1;
+  return 0;
+}
+
+// MODE 0 PlaneSearch (ot_query.h:612-636): nodes and entities the plane hits.
+// MODE 1 PlanePositiveSegmentation (:691-741): a node wholly on the positive side
+// gives its whole subtree (a contiguous storage range), a hit node its entities not
+// on the negative side.  MODE 2 FrustumCulling (:756-822): a node is visited unless a
+// plane met before any hitting plane has it on the negative side; entities on no
+// plane's negative side.  Depth-first, last child first (the reference's order).
+template <int D, bool BOX, int MODE, bool FILL>
+__device__ __forceinline__ void planes_query(const TreeView &tv, uint64_t qi, const double *__restrict__ planes,
+                                             uint32_t nplanes, const uint32_t *__restrict__ qset, double tol,
+                                             LdsLevelIter &stack, uint32_t *__restrict__ counts,
+                                             const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
+                                             uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  const double *pq = planes + qi * (uint64_t)nplanes * (D + 1);
+  const uint64_t o = FILL ? offsets[qi] : 0;
+  uint32_t cnt = 0;
+  if (sp.node_cnt != 0 && nplanes != 0) {
+    int top = 0;
+    uint32_t node = (uint32_t)sp.node_off;
+    for (;;) {
+      double mn[D], sz[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        mn[d] = tv.nmin[(uint64_t)node * D + d];
+        sz[d] = tv.nsize[(uint64_t)node * D + d];
+      }
+      bool visit = true, whole = false;
+      if (MODE == 2) {
+        for (uint32_t k = 0; k < nplanes; ++k) {
+          const int rel = box_plane_relation<D>(mn, sz, pq + k * (D + 1), tol);
+          if (rel == 0) break;
+          if (rel < 0) {
+            visit = false;
+            break;
+          }
+        }
+      } else {
+        const int rel = box_plane_relation<D>(mn, sz, pq, tol);
+        if (MODE == 0) visit = rel == 0;
+        else {
+          visit = rel >= 0;
+          whole = rel > 0;
+        }
+      }
+      if (visit) {
+        const NodeInfo ni = tv.info[node];
+        if (whole) {
+          const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
+          range_take<FILL>(out, o + cnt, tv.ids + e0, e1 - e0, ncopy, copies);
+          cnt += e1 - e0;
+        } else {
+          for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+            bool ok;
+            if (MODE == 0) {
+              ok = entity_plane_relation<D, BOX>(tv, j, pq, tol) == 0;
+            } else {
+              ok = true;
+              for (uint32_t k = 0; k < nplanes && ok; ++k) ok = entity_plane_relation<D, BOX>(tv, j, pq + k * (D + 1), tol) >= 0;
+            }
+            if (ok) {
+              if (FILL) out[o + cnt] = tv.ids[j];
+              ++cnt;
+            }
+          }
+          if (ni.ccount) {
+            stack.nx(top) = ni.cbegin + ni.ccount - 1;
+            stack.left(top) = (uint8_t)ni.ccount;
+            ++top;
+          }
+        }
+      }
+      if (top == 0) break;
+      node = stack.nx(top - 1)--;
+      if (--stack.left(top - 1) == 0) --top;
+    }
+  }
+  if (!FILL) counts[qi] = cnt;
+}
+
+template <int D, bool BOX, int MODE, bool FILL>
+__global__ void __launch_bounds__(kQB) k_planes(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                                 const double *__restrict__ planes, uint32_t nplanes,
+                                                 const uint32_t *__restrict__ qset, uint64_t nq, double tol,
+                                                 uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                                 uint32_t *__restrict__ out, uint32_t *__restrict__ ncopy,
+                                                 RangeCopy *__restrict__ copies) {
+  __shared__ uint32_t s_a[kRangeStack * kQB];
+  __shared__ uint8_t s_b[kRangeStack * kQB];
+  const uint64_t qi = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (qi >= nq) return;
+  LdsLevelIter S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    planes_query<D, BOX, MODE, FILL>(tv, qi, planes, nplanes, qset, tol, S, counts, offsets, out, ncopy, copies);
+  });
+}
+
+// ---------------------------------------------------------------------------
 // pick search
 // ---------------------------------------------------------------------------
 // internal_geometry_module.h:372-396 DoesBoxContainPointAD(box, point, tol) and
@@ -2029,6 +2173,68 @@ static Result *count_fill_csr(Ctx *ctx, uint64_t nq, Count count, Fill fill) {
          r->d_offsets);
   fill(r, ht);
   return r;
+}
+
+template <int D, bool BOX, int MODE>
+static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, uint32_t nplanes,
+                           const uint32_t *qs, double tol) {
+  Buf<uint32_t> ncopy(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
+  Buf<RangeCopy> copies;
+  uint32_t hcopy = 0;
+  Result *r = count_fill_csr(
+      ctx, nq,
+      [&](uint32_t *cnt) {
+        launch(ctx, "planes_count", k_planes<D, BOX, MODE, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views,
+               vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
+               (RangeCopy *)nullptr);
+        OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      },
+      [&](Result *res, uint32_t) {
+        copies = Buf<RangeCopy>(ctx, std::max<uint32_t>(hcopy, 1));
+        OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
+        launch(ctx, "planes_fill", k_planes<D, BOX, MODE, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views,
+               vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets, res->d_ids,
+               ncopy.p, copies.p);
+      });
+  launch(ctx, "planes_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
+         r->d_ids);
+  return r;
+}
+
+Result *query_planes(const Tree *t, uint64_t nq, uint32_t nplanes, const double *planes, const uint32_t *sets, int mem,
+                     int mode, double tol) {
+  if (mode < 0 || mode > 2) throw Error{OTG_ERR_INVALID_ARG, "plane query mode"};
+  if (mode != 2 && nplanes != 1) throw Error{OTG_ERR_INVALID_ARG, "plane search takes one plane per query"};
+  // the reference tree: a node on the positive side contributes its whole subtree
+  Ctx *ctx = run_ctx(t);
+  DevInput in;
+  stage_inputs(ctx, in, planes, sets, nq * nplanes * (t->dim + 1), nq, mem);
+  const bool box = t->kind == OTG_KIND_BOX;
+  const ViewSet vs = single_view(t);
+#define OTG_PLANES_M(DD, BB)                                                                        \
+  switch (mode) {                                                                                   \
+    case 0: return planes_impl<DD, BB, 0>(ctx, vs, nq, in.q, nplanes, in.sets, tol);                \
+    case 1: return planes_impl<DD, BB, 1>(ctx, vs, nq, in.q, nplanes, in.sets, tol);                \
+    default: return planes_impl<DD, BB, 2>(ctx, vs, nq, in.q, nplanes, in.sets, tol);               \
+  }
+#define OTG_PLANES(DD)       \
+  case DD:                   \
+    if (box) {               \
+      OTG_PLANES_M(DD, true) \
+    } else {                 \
+      OTG_PLANES_M(DD, false)\
+    }
+  switch (t->dim) {
+    OTG_PLANES(2)
+    OTG_PLANES(3)
+    OTG_PLANES(4)
+    OTG_PLANES(5)
+    OTG_PLANES(6)
+  }
+#undef OTG_PLANES
+#undef OTG_PLANES_M
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
 }
 
 template <int D, bool BOX>

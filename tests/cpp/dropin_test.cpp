@@ -152,6 +152,17 @@ int main() {
   std::sort(dual.begin(), dual.end());
   EXPECT((dual == std::vector<std::pair<index_t, index_t>>{{2, 4}, {3, 3}, {3, 4}}));
   EXPECT((ql.CollisionDetection(boxes, qr, r5) == dual));
+  // plane KATs: PlaneIntersection_3D_XY_d0 (general.tests.cpp:2977), FrustumCulling (adapter.tests.cpp:283)
+  STEP("planes");
+  std::vector<BoundingBox3D> pb = {{{0.0, 0.0, 0.0}, {1.0, 1.0, 0.0}}, {{1.0, 1.0, -1.0}, {2.0, 2.0, 0.0}},
+                                   {{2.0, 2.0, -1.0}, {3.0, 3.0, -0.2}}, {{3.0, 3.0, 1.0}, {4.0, 4.0, 1.2}},
+                                   {{1.2, 1.2, -1.0}, {2.8, 2.8, 1.0}}};
+  StaticOctreeBox opb(PAR_EXEC, pb, 3, std::nullopt, 2);
+  EXPECT(same_set(opb.PlaneIntersection(0.0, Point3D{0.0, 0.0, 1.0}, pb, 0.01), {0, 1, 4}));
+  StaticQuadtreeBox qf(PAR_EXEC, b5, 3, std::nullopt, 1);
+  const double s2 = std::sqrt(2.0);
+  std::vector<PlaneND<2>> fr = {{2 * s2, {1.0 / s2, 1.0 / s2}}, {-2.0, {0.0, -1.0}}};
+  EXPECT(same_set(qf.FrustumCulling(fr, b5, double(0.01f)), {1, 2, 4}));
   if (failures == 0) std::printf("DROPIN OK\n");
   return failures ? 1 : 0;
 }

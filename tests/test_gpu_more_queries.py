@@ -151,3 +151,45 @@ def test_forest_sets_pick_ray_all_and_dual_collision(gpu_ctx):
             assert aids[aoff[qi]:aoff[qi + 1]].tolist() == oids[ooff[j]:ooff[j + 1]].tolist()
     got = f.collide_with(f, BASE_TOL, 0, 3).astype(np.int64)
     assert np.array_equal(got, trees[0].collide_with(trees[3]).astype(np.int64))
+
+
+# ---- PlaneSearch / PlanePositiveSegmentation / FrustumCulling ---------------------------------------
+def test_plane_kats(gpu_ctx):
+    for c in kat()["planes"]:
+        g = _tree(gpu_ctx, c["kind"], c["dim"], np.array(c["geom"], float), None, c["depth"], c["cap"])
+        pl = c["planes"]
+        if c["mode"] == 0:
+            got = g.PlaneSearch(pl[0][0], pl[0][1:], None, c["tol"])
+        else:
+            got = g.FrustumCulling(pl, None, c["tol"])
+        assert sorted(got) == sorted(c["expect"]), c["src"]
+
+
+def _random_planes(n, D, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    nrm = unit_dirs(n, D, seed=seed + 1)
+    pts = rng.uniform(lo, hi, (n, D))
+    return np.concatenate([np.einsum("ij,ij->i", nrm, pts)[:, None], nrm], axis=1)
+
+
+@pytest.mark.parametrize("kind,D,tol", [("point", 3, 0.5), ("box", 3, BASE_TOL), ("box", 2, 0.0), ("point", 6, 0.01),
+                                        ("box", 4, 0.1)])
+def test_planes_vs_oracle(gpu_ctx, kind, D, tol):
+    n = 20000
+    if kind == "point":
+        geom = uniform_points(n, D, 0.0, 100.0, seed=110 + D)
+    else:
+        geom = lognormal_boxes(n, D, 0.0, 100.0, np.log(1.0), 0.6, seed=111 + D)
+    g = _tree(gpu_ctx, kind, D, geom, None, 6, 8)
+    o = O.OracleTree(D, kind, geom, None, 6, 8)
+    pl = _random_planes(400, D, 0.0, 100.0, seed=112)
+    pl[0, 0] = -1e6  # everything on the positive side
+    for mode in (0, 1):
+        goff, gids = g.planes_batch(pl, mode, tol).csr()
+        ooff, oids = o.planes(pl, mode, tol, threads=8)
+        _same_sets(goff, gids, ooff, oids)
+    fr = _random_planes(4 * 300, D, 20.0, 80.0, seed=113).reshape(300, 4, D + 1)
+    goff, gids = g.planes_batch(fr, 2, tol).csr()
+    ooff, oids = o.planes(fr, 2, tol, threads=8)
+    _same_sets(goff, gids, ooff, oids)
+    assert goff[-1] > 0
