@@ -13,6 +13,8 @@
 //                                                         (core/ot_query.h:426-466, :1929-2022, :1462-1496)
 //   PlaneSearch / PlaneIntersection, PlanePositiveSegmentation, FrustumCulling
 //                                                         (core/ot_query.h:612-822)
+//   OrthoTreeManaged (entity-owning) + StaticOctreePointM / StaticOctreeBoxM / ...
+//                                                         (core/ot_managed.h:47-893, core/ot_aliases.h:417-447)
 //   GetNodeCount / GetNodeChildren / GetNodeEntities / GetNodeMinPoint / GetNodeSize / GetNodeBox
 //                                                         (core/ot_static_linear_core.h:178-268)
 // plus batch overloads over spans of queries returning CSR results.
@@ -32,6 +34,7 @@
 #include <limits>
 #include <optional>
 #include <span>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -484,5 +487,146 @@ using StaticOctreePoint = StaticOrthoTreePointND<3>;
 using StaticOctreeBox = StaticOrthoTreeBoxND<3, true>;
 using StaticHexatreePoint = StaticOrthoTreePointND<4>;
 using StaticHexatreeBox = StaticOrthoTreeBoxND<4, true>;
+
+
+// ---- managed (entity-owning) static trees: OrthoTreeManaged (core/ot_managed.h:47-893) ----
+// The tree keeps its own copy of the entities (GetData) and the query methods drop the
+// entity-span argument, as the reference's managed wrapper does.  Only the static-tree
+// members exist (Add / Update / Erase belong to the dynamic core).
+template <typename TOrthoTreeCore>
+class OrthoTreeManaged {
+ public:
+  using TVector = typename TOrthoTreeCore::TVector;
+  using TBox = typename TOrthoTreeCore::TBox;
+  using TRay = typename TOrthoTreeCore::TRay;
+  using TPlane = typename TOrthoTreeCore::TPlane;
+  using Entity = typename TOrthoTreeCore::Entity;
+  using EntityID = typename TOrthoTreeCore::EntityID;
+  using EntityContainer = std::vector<Entity>;
+  static constexpr std::size_t DEFAULT_TARGET_ELEMENT_NUM_IN_NODES = 16;  // core/configuration.h:83
+
+  OrthoTreeManaged() = default;
+  // ot_managed.h:213-316: runtime or compile-time execution tag, with box space
+  template <typename TExecMode = ParExec>
+  explicit OrthoTreeManaged(std::span<Entity const> const &entities, std::optional<depth_t> maxDepthID = std::nullopt,
+                            std::optional<TBox> boxSpace = std::nullopt,
+                            std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES,
+                            TExecMode execMode = {}) noexcept
+      : m_entities(entities.begin(), entities.end()) {
+    m_tree.Create(m_entities, maxDepthID, boxSpace, maxElementNoInNode, execMode);
+  }
+  template <typename TExecMode = ParExec>
+  explicit OrthoTreeManaged(EntityContainer &&entities, std::optional<depth_t> maxDepthID = std::nullopt,
+                            std::optional<TBox> boxSpace = std::nullopt,
+                            std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES,
+                            TExecMode execMode = {}) noexcept
+      : m_entities(std::move(entities)) {
+    m_tree.Create(m_entities, maxDepthID, boxSpace, maxElementNoInNode, execMode);
+  }
+  template <typename TExecMode>
+    requires(std::is_same_v<TExecMode, SeqExec> || std::is_same_v<TExecMode, ParExec>)
+  explicit OrthoTreeManaged(TExecMode execMode, std::span<Entity const> const &entities,
+                            std::optional<depth_t> maxDepthID = std::nullopt, std::optional<TBox> boxSpace = std::nullopt,
+                            std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES) noexcept
+      : OrthoTreeManaged(entities, maxDepthID, boxSpace, maxElementNoInNode, execMode) {}
+
+  template <typename TExecMode = ParExec>
+  static OrthoTreeManaged Create(std::span<Entity const> const &entities, std::optional<depth_t> maxDepthID = std::nullopt,
+                                 std::optional<TBox> boxSpace = std::nullopt,
+                                 std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES,
+                                 TExecMode execMode = {}) noexcept {
+    return OrthoTreeManaged(entities, maxDepthID, boxSpace, maxElementNoInNode, execMode);
+  }
+
+  TOrthoTreeCore const &GetCore() const noexcept { return m_tree; }
+  EntityContainer const &GetData() const noexcept { return m_entities; }
+  Entity const &Get(EntityID entityID) const noexcept { return m_entities[entityID]; }
+
+  std::vector<EntityID> PickSearch(TVector const &pickPoint, double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PickSearch(pickPoint, m_entities, tolerance);
+  }
+  std::vector<EntityID> RangeSearch(TBox const &range, RangeSearchMode mode = RangeSearchMode::Inside,
+                                    double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.RangeSearch(range, m_entities, mode, tolerance);
+  }
+  std::vector<EntityID> PlaneSearch(double distanceOfOrigo, TVector const &planeNormal,
+                                    double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PlaneSearch(distanceOfOrigo, planeNormal, m_entities, tolerance);
+  }
+  std::vector<EntityID> PlaneSearch(TPlane const &plane, double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PlaneSearch(plane, m_entities, tolerance);
+  }
+  std::vector<EntityID> PlaneIntersection(double distanceOfOrigo, TVector const &planeNormal,
+                                          double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PlaneIntersection(distanceOfOrigo, planeNormal, m_entities, tolerance);
+  }
+  std::vector<EntityID> PlaneIntersection(TPlane const &plane, double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PlaneIntersection(plane, m_entities, tolerance);
+  }
+  std::vector<EntityID> PlanePositiveSegmentation(double distanceOfOrigo, TVector const &planeNormal,
+                                                  double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PlanePositiveSegmentation(distanceOfOrigo, planeNormal, m_entities, tolerance);
+  }
+  std::vector<EntityID> PlanePositiveSegmentation(TPlane const &plane, double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.PlanePositiveSegmentation(plane, m_entities, tolerance);
+  }
+  std::vector<EntityID> FrustumCulling(std::span<TPlane const> const &boundaryPlanes,
+                                       double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.FrustumCulling(boundaryPlanes, m_entities, tolerance);
+  }
+  std::vector<EntityID> GetNearestNeighbors(TVector const &pt, std::size_t k,
+                                            double maxDistanceWithin = std::numeric_limits<double>::max(),
+                                            double tolerance = BASE_TOLERANCE) const noexcept {
+    return m_tree.GetNearestNeighbors(pt, k, maxDistanceWithin, m_entities, tolerance);
+  }
+  template <typename TExecMode = ParExec>
+  std::vector<std::pair<EntityID, EntityID>> CollisionDetection(double tolerance = BASE_TOLERANCE,
+                                                                TExecMode execMode = {}) const noexcept {
+    return m_tree.CollisionDetection(m_entities, tolerance, execMode);
+  }
+  std::vector<std::pair<EntityID, EntityID>> CollisionDetection(OrthoTreeManaged const &otherTree,
+                                                                double tolerance = BASE_TOLERANCE) const noexcept {
+    return TOrthoTreeCore::CollisionDetection(m_tree, m_entities, otherTree.m_tree, otherTree.m_entities, tolerance);
+  }
+  static std::vector<std::pair<EntityID, EntityID>> CollisionDetection(OrthoTreeManaged const &leftTree,
+                                                                       OrthoTreeManaged const &rightTree,
+                                                                       double tolerance = BASE_TOLERANCE) noexcept {
+    return leftTree.CollisionDetection(rightTree, tolerance);
+  }
+  std::vector<EntityID> RayIntersectedAll(TVector const &rayBasePoint, TVector const &rayHeading,
+                                          double tolerance = BASE_TOLERANCE, double toleranceIncrement = 0.0,
+                                          double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    return m_tree.RayIntersectedAll(rayBasePoint, rayHeading, m_entities, tolerance, toleranceIncrement, maxDistance);
+  }
+  std::vector<EntityID> RayIntersectedAll(TRay const &ray, double tolerance = BASE_TOLERANCE,
+                                          double toleranceIncrement = 0.0,
+                                          double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    return RayIntersectedAll(ray.Origin, ray.Direction, tolerance, toleranceIncrement, maxDistance);
+  }
+  std::vector<EntityID> RayIntersectedFirst(TVector const &rayBasePoint, TVector const &rayHeading,
+                                            double tolerance = BASE_TOLERANCE, double toleranceIncrement = 0.0,
+                                            double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    return m_tree.RayIntersectedFirst(rayBasePoint, rayHeading, m_entities, tolerance, toleranceIncrement, maxDistance);
+  }
+  std::vector<EntityID> RayIntersectedFirst(TRay const &ray, double tolerance = BASE_TOLERANCE,
+                                            double toleranceIncrement = 0.0,
+                                            double maxDistance = std::numeric_limits<double>::max()) const noexcept {
+    return RayIntersectedFirst(ray.Origin, ray.Direction, tolerance, toleranceIncrement, maxDistance);
+  }
+
+ private:
+  TOrthoTreeCore m_tree;
+  EntityContainer m_entities;
+};
+
+// core/ot_aliases.h:417-447
+template <int DIM>
+using StaticTreePointManagedND = OrthoTreeManaged<StaticOrthoTreePointND<DIM>>;
+template <int DIM, bool IS_LOOSE_TREE = true>
+using StaticTreeBoxManagedND = OrthoTreeManaged<StaticOrthoTreeBoxND<DIM, IS_LOOSE_TREE>>;
+using StaticQuadtreePointM = StaticTreePointManagedND<2>;
+using StaticQuadtreeBoxM = StaticTreeBoxManagedND<2, true>;
+using StaticOctreePointM = StaticTreePointManagedND<3>;
+using StaticOctreeBoxM = StaticTreeBoxManagedND<3, true>;
 
 }  // namespace otg

@@ -163,6 +163,25 @@ int main() {
   const double s2 = std::sqrt(2.0);
   std::vector<PlaneND<2>> fr = {{2 * s2, {1.0 / s2, 1.0 / s2}}, {-2.0, {0.0, -1.0}}};
   EXPECT(same_set(qf.FrustumCulling(fr, b5, double(0.01f)), {1, 2, 4}));
+  // managed (entity-owning) trees: PlaneSearchC_3D_XZ_dP4 (general.tests.cpp:2458, OctreePointM) and
+  // PlaneIntersectionC_3D_YZ_dN1 (:3048, OctreeBoxM)
+  STEP("managed trees");
+  std::vector<Point3D> vpt = {{2.0, -1.0, 0.0}, {3.0, -2.0, 1.0}, {4.0, -3.0, 0.02}, {4.0, -2.0, 0.00}, {5.0, 0.0, 1.00999}};
+  StaticOctreePointM mpt(vpt, 3, std::nullopt, 2);
+  EXPECT(same_set(mpt.PlaneSearch(4.0, Point3D{1.0, 0.0, 0.0}, 0.01), {2, 3}));
+  EXPECT(mpt.GetData().size() == vpt.size() && mpt.Get(4)[2] == 1.00999);
+  EXPECT(same_set(mpt.GetNearestNeighbors({4.0, -2.5, 0.0}, 2), {2, 3}));
+  std::vector<BoundingBox3D> mbx = {{{0.0, 0.0, 0.0}, {1.0, 1.0, 0.0}}, {{1.0, 1.0, -1.0}, {2.0, 2.0, 0.0}},
+                                    {{2.0, 2.0, -1.0}, {3.0, 3.0, -0.2}}, {{3.0, 3.0, 1.0}, {4.0, 4.0, 1.2}},
+                                    {{1.2, 1.2, -1.1}, {2.8, 2.8, 1.1}}};
+  auto mbox = StaticOctreeBoxM::Create(mbx, 3, std::nullopt, 2);
+  EXPECT(mbox.PlaneIntersection(-1.0, Point3D{1.0, 0.0, 0.0}, 0.01).empty());
+  EXPECT(same_set(mbox.PlaneIntersection(-1.0, Point3D{0.0, 0.0, 1.0}, 0.01), {1, 2, 4}));
+  EXPECT((mbox.CollisionDetection() == std::vector<std::pair<index_t, index_t>>{{1, 4}, {2, 4}}));
+  StaticQuadtreeBoxM ml(PAR_EXEC, boxes, 3, std::nullopt, 2), mr(PAR_EXEC, r5, 3, std::nullopt, 2);
+  auto mdual = StaticQuadtreeBoxM::CollisionDetection(ml, mr);
+  std::sort(mdual.begin(), mdual.end());
+  EXPECT((mdual == std::vector<std::pair<index_t, index_t>>{{2, 4}, {3, 3}, {3, 4}}));
   if (failures == 0) std::printf("DROPIN OK\n");
   return failures ? 1 : 0;
 }
