@@ -418,6 +418,45 @@ def run_single(args, octree_amd, torch):
                                            "depth 8, cap 32; self CollisionDetection + 100,000 rays"},
                     "build_ms": tb * 1e3, "build_mboxes_s": n / tb / 1e6, "ray_ms": tr * 1e3,
                     "collision_ms": tcol * 1e3, "collision_pairs": npairs, "collision_boxes_s": n / tcol})
+        # the other read-only queries on the same tree: PickSearch, RayIntersectedAll, PlaneSearch,
+        # FrustumCulling (cubes of edge 30 as 6 planes), and two-tree collision of the two halves
+        picks = rng.uniform(0.0, 1000.0, (100_000, 3))
+        nrm = rng.normal(0.0, 1.0, (1_000, 3))
+        nrm /= np.sqrt((nrm * nrm).sum(axis=1))[:, None]
+        planes = np.concatenate([(nrm * rng.uniform(0.0, 1000.0, (1_000, 3))).sum(axis=1)[:, None], nrm], axis=1)
+        fc = rng.uniform(15.0, 985.0, (1_000, 3))
+        fr = np.zeros((1_000, 6, 4))
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = 1.0
+            fr[:, 2 * a, 0], fr[:, 2 * a, 1:] = fc[:, a] - 15.0, e      # x_a >= c - 15
+            fr[:, 2 * a + 1, 0], fr[:, 2 * a + 1, 1:] = -(fc[:, a] + 15.0), -e  # x_a <= c + 15
+        d_pk, d_pl, d_fr = (torch.from_numpy(a).to(dev) for a in (picks, planes, fr))
+        half = n // 2
+        left, right = octree_amd.StaticOctreeBox(ctx=ctx), octree_amd.StaticOctreeBox(ctx=ctx)
+        left.Create(d_b[:half].contiguous(), 8, None, 32)
+        right.Create(d_b[half:].contiguous(), 8, None, 32)
+        torch.cuda.synchronize()
+        tpk, _ = _timed(ctx, lambda: tree.pick_batch(d_pk), reps)
+        tra, _ = _timed(ctx, lambda: tree.ray_all_batch(d_r), reps)
+        tpl, _ = _timed(ctx, lambda: tree.planes_batch(d_pl, 0), reps)
+        tfr, _ = _timed(ctx, lambda: tree.planes_batch(d_fr, 2), reps)
+        tdual, _ = _timed(ctx, lambda: left.collide_with(right), reps)
+        counts = {}
+        for nm, fn in (("pick", lambda: tree.pick_batch(d_pk)), ("ray_all", lambda: tree.ray_all_batch(d_r)),
+                       ("plane", lambda: tree.planes_batch(d_pl, 0)), ("frustum", lambda: tree.planes_batch(d_fr, 2))):
+            r = fn()
+            counts[nm] = int(r.csr()[0][-1])
+            r.free()
+        ndual = len(left.collide_with(right))
+        more = {"pick_qps": len(picks) / tpk, "ray_all_rays_s": len(rays) / tra, "plane_qps": len(planes) / tpl,
+                "frustum_qps": len(fr) / tfr, "two_tree_collision_ms": tdual * 1e3, "two_tree_pairs": ndual,
+                "result_ids": counts,
+                "sizes": "100,000 picks; 100,000 rays (all hits); 1,000 planes; 1,000 frusta of 6 planes "
+                         "(cubes of edge 30); two trees of 2,000,000 boxes each"}
+        out["more_queries"] = more
+        left.free()
+        right.free()
         if not args.no_cpu:
             t0 = time.perf_counter()
             o = O.OracleTree(3, "box", boxes[:m], None, 8, 32)
@@ -431,6 +470,28 @@ def run_single(args, octree_amd, torch):
             out["cpu_baseline"] = {"value": len(rays) / cr, "unit": "rays/s", "cores": threads, "kind": "port",
                                    "sample": f"first {m} boxes; 100,000 rays", "build_mboxes_s": m / cb / 1e6,
                                    "collision_boxes_s_single_thread": m / cc}
+            # the other queries on the same sample tree (rates per second, `threads` workers)
+            t0 = time.perf_counter()
+            o.pick(picks, threads=threads)
+            cpk = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.ray_all(rays, threads=threads)
+            cra = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.planes(planes, 0, threads=threads)
+            cpl = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o.planes(fr, 2, threads=threads)
+            cfr = time.perf_counter() - t0
+            ol = O.OracleTree(3, "box", boxes[:m // 2], None, 8, 32)
+            orr = O.OracleTree(3, "box", boxes[half:half + m // 2], None, 8, 32)
+            t0 = time.perf_counter()
+            ol.collide_with(orr)
+            cdu = time.perf_counter() - t0
+            out["cpu_baseline"]["more_queries"] = {
+                "pick_qps": len(picks) / cpk, "ray_all_rays_s": len(rays) / cra, "plane_qps": len(planes) / cpl,
+                "frustum_qps": len(fr) / cfr, "two_tree_collision_boxes_s_single_thread": m / cdu,
+                "sample": f"same tree of the first {m} boxes; two-tree collision of {m // 2} + {m // 2} boxes"}
     else:  # point6d_8m
         rng = np.random.default_rng(13)
         k = 32

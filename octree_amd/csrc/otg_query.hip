@@ -989,6 +989,13 @@ __device__ __forceinline__ int entity_plane_relation(const TreeView &tv, uint32_
 // on the negative side.  MODE 2 FrustumCulling (:756-822): a node is visited unless a
 // plane met before any hitting plane has it on the negative side; entities on no
 // plane's negative side.  Depth-first, last child first (the reference's order).
+template <int D, bool BOX, int MODE>
+__device__ __forceinline__ bool planes_node(const TreeView &tv, uint32_t node, const double *pq, uint32_t nplanes,
+                                            double tol, NodeInfo &ni, bool &whole);
+template <int D, bool BOX, int MODE>
+__device__ __forceinline__ bool planes_entity(const TreeView &tv, uint32_t j, const double *pq, uint32_t nplanes,
+                                              double tol);
+
 template <int D, bool BOX, int MODE, bool FILL>
 __device__ __forceinline__ void planes_query(const TreeView &tv, uint64_t qi, const double *__restrict__ planes,
                                              uint32_t nplanes, const uint32_t *__restrict__ qset, double tol,
@@ -1004,50 +1011,19 @@ __device__ __forceinline__ void planes_query(const TreeView &tv, uint64_t qi, co
     int top = 0;
     uint32_t node = (uint32_t)sp.node_off;
     for (;;) {
-      double mn[D], sz[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        mn[d] = tv.nmin[(uint64_t)node * D + d];
-        sz[d] = tv.nsize[(uint64_t)node * D + d];
-      }
-      bool visit = true, whole = false;
-      if (MODE == 2) {
-        for (uint32_t k = 0; k < nplanes; ++k) {
-          const int rel = box_plane_relation<D>(mn, sz, pq + k * (D + 1), tol);
-          if (rel == 0) break;
-          if (rel < 0) {
-            visit = false;
-            break;
-          }
-        }
-      } else {
-        const int rel = box_plane_relation<D>(mn, sz, pq, tol);
-        if (MODE == 0) visit = rel == 0;
-        else {
-          visit = rel >= 0;
-          whole = rel > 0;
-        }
-      }
-      if (visit) {
-        const NodeInfo ni = tv.info[node];
+      NodeInfo ni;
+      bool whole;
+      if (planes_node<D, BOX, MODE>(tv, node, pq, nplanes, tol, ni, whole)) {
         if (whole) {
           const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
           range_take<FILL>(out, o + cnt, tv.ids + e0, e1 - e0, ncopy, copies);
           cnt += e1 - e0;
         } else {
-          for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-            bool ok;
-            if (MODE == 0) {
-              ok = entity_plane_relation<D, BOX>(tv, j, pq, tol) == 0;
-            } else {
-              ok = true;
-              for (uint32_t k = 0; k < nplanes && ok; ++k) ok = entity_plane_relation<D, BOX>(tv, j, pq + k * (D + 1), tol) >= 0;
-            }
-            if (ok) {
+          for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j)
+            if (planes_entity<D, BOX, MODE>(tv, j, pq, nplanes, tol)) {
               if (FILL) out[o + cnt] = tv.ids[j];
               ++cnt;
             }
-          }
           if (ni.ccount) {
             stack.nx(top) = ni.cbegin + ni.ccount - 1;
             stack.left(top) = (uint8_t)ni.ccount;
@@ -1078,6 +1054,158 @@ __global__ void __launch_bounds__(kQB) k_planes(const TreeView *__restrict__ vie
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     planes_query<D, BOX, MODE, FILL>(tv, qi, planes, nplanes, qset, tol, S, counts, offsets, out, ncopy, copies);
   });
+}
+
+// Node test and entity pass of one visited node of a plane query, shared by the
+// one-lane walk and the wavefront walk.  Returns whether the walk descends into the
+// node's children; ids go through emit(count or id source) -- appended ids or whole
+// subtree runs (MODE 1, a node wholly on the positive side).
+template <int D, bool BOX, int MODE>
+__device__ __forceinline__ bool planes_node(const TreeView &tv, uint32_t node, const double *pq, uint32_t nplanes,
+                                            double tol, NodeInfo &ni, bool &whole) {
+  double mn[D], sz[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    mn[d] = tv.nmin[(uint64_t)node * D + d];
+    sz[d] = tv.nsize[(uint64_t)node * D + d];
+  }
+  bool visit = true;
+  whole = false;
+  if (MODE == 2) {
+    for (uint32_t k = 0; k < nplanes; ++k) {
+      const int rel = box_plane_relation<D>(mn, sz, pq + k * (D + 1), tol);
+      if (rel == 0) break;
+      if (rel < 0) {
+        visit = false;
+        break;
+      }
+    }
+  } else {
+    const int rel = box_plane_relation<D>(mn, sz, pq, tol);
+    if (MODE == 0) visit = rel == 0;
+    else {
+      visit = rel >= 0;
+      whole = rel > 0;
+    }
+  }
+  if (visit) ni = tv.info[node];
+  return visit;
+}
+
+template <int D, bool BOX, int MODE>
+__device__ __forceinline__ bool planes_entity(const TreeView &tv, uint32_t j, const double *pq, uint32_t nplanes,
+                                              double tol) {
+  if (MODE == 0) return entity_plane_relation<D, BOX>(tv, j, pq, tol) == 0;
+  bool ok = true;
+  for (uint32_t k = 0; k < nplanes && ok; ++k) ok = entity_plane_relation<D, BOX>(tv, j, pq + k * (D + 1), tol) >= 0;
+  return ok;
+}
+
+// One wavefront per plane query: a plane or frustum meets a whole slab of the
+// tree, so a query is thousands of node visits and a launch holds few queries.
+// Phase 1: the wavefront expands the top of the tree breadth-first (one lane per
+// frontier node, the node test and entity pass as in planes_query) until the
+// frontier holds at least 64 nodes or nothing is left to expand; the frontier
+// never exceeds 64 * 2^D + 64 entries.  Phase 2: each lane takes frontier nodes
+// from an LDS counter and walks their subtrees with its own level iterators (the
+// walk of planes_query, bounded by the tree depth).  Every node a walk reaches has
+// all its ancestors visited, so the set of ids is planes_query's; ids are appended
+// at the query's CSR offset through an LDS counter, in no particular order.
+constexpr int kPlaneFrontier = 64 * 8 + 64;
+
+template <int D, bool BOX, int MODE, bool FILL>
+__global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                                     const double *__restrict__ planes, uint32_t nplanes,
+                                                     const uint32_t *__restrict__ qset, uint64_t nq, double tol,
+                                                     uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
+                                                     uint32_t *__restrict__ out, uint32_t *__restrict__ ncopy,
+                                                     RangeCopy *__restrict__ copies) {
+  static_assert(D <= 3, "frontier sized for up to 8 children per node");
+  __shared__ uint32_t s_front[2][kPlaneFrontier];
+  __shared__ uint32_t s_nf[2], s_cnt, s_next;
+  __shared__ uint32_t s_a[kRangeStack * kQB];  // LdsLevelIter strides by kQB
+  __shared__ uint8_t s_b[kRangeStack * kQB];
+  const uint64_t qi = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const TreeView &tv = views[qview ? qview[qi] : 0u];
+  const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+  const double *pq = planes + qi * (uint64_t)nplanes * (D + 1);
+  const uint64_t o = FILL ? offsets[qi] : 0;
+  auto emit_run = [&](const uint32_t *src, uint32_t len) {
+    const uint32_t at = atomicAdd(&s_cnt, len);
+    range_take<FILL>(out, o + at, src, len, ncopy, copies);
+  };
+  auto entities = [&](const NodeInfo &ni) {
+    for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j)
+      if (planes_entity<D, BOX, MODE>(tv, j, pq, nplanes, tol)) {
+        const uint32_t at = atomicAdd(&s_cnt, 1u);
+        if (FILL) out[o + at] = tv.ids[j];
+      }
+  };
+  if (lane == 0) {
+    s_cnt = 0;
+    s_next = 0;
+    s_nf[0] = 0;
+    s_nf[1] = 0;
+    if (sp.node_cnt != 0 && nplanes != 0) {
+      s_front[0][0] = (uint32_t)sp.node_off;
+      s_nf[0] = 1;
+    }
+  }
+  __syncthreads();
+  // phase 1: breadth-first expansion while the frontier is narrower than the wavefront
+  int cur = 0;
+  while (s_nf[cur] != 0 && s_nf[cur] < 64) {
+    const uint32_t nf = s_nf[cur];
+    if (lane == 0) s_nf[cur ^ 1] = 0;
+    __syncthreads();
+    if (lane < nf) {
+      const uint32_t node = s_front[cur][lane];
+      NodeInfo ni;
+      bool whole;
+      if (planes_node<D, BOX, MODE>(tv, node, pq, nplanes, tol, ni, whole)) {
+        if (whole) emit_run(tv.ids + ni.ebegin, tv.rend[node] - ni.ebegin);
+        else {
+          entities(ni);
+          if (ni.ccount) {
+            const uint32_t at = atomicAdd(&s_nf[cur ^ 1], ni.ccount);
+            for (uint32_t c = 0; c < ni.ccount; ++c) s_front[cur ^ 1][at + c] = ni.cbegin + c;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  // phase 2: the lanes walk the frontier's subtrees
+  const uint32_t nf = s_nf[cur];
+  LdsLevelIter stack{&s_a[lane], &s_b[lane]};
+  for (;;) {
+    const uint32_t item = atomicAdd(&s_next, 1u);
+    if (item >= nf) break;
+    int top = 0;
+    uint32_t node = s_front[cur][item];
+    for (;;) {
+      NodeInfo ni;
+      bool whole;
+      if (planes_node<D, BOX, MODE>(tv, node, pq, nplanes, tol, ni, whole)) {
+        if (whole) emit_run(tv.ids + ni.ebegin, tv.rend[node] - ni.ebegin);
+        else {
+          entities(ni);
+          if (ni.ccount) {
+            stack.nx(top) = ni.cbegin + ni.ccount - 1;
+            stack.left(top) = (uint8_t)ni.ccount;
+            ++top;
+          }
+        }
+      }
+      if (top == 0) break;
+      node = stack.nx(top - 1)--;
+      if (--stack.left(top - 1) == 0) --top;
+    }
+  }
+  __syncthreads();
+  if (!FILL && lane == 0) counts[qi] = s_cnt;
 }
 
 // ---------------------------------------------------------------------------
@@ -2175,9 +2303,13 @@ static Result *count_fill_csr(Ctx *ctx, uint64_t nq, Count count, Fill fill) {
   return r;
 }
 
+constexpr uint64_t kPlaneWaveQueries = 1u << 16;
+
 template <int D, bool BOX, int MODE>
 static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, uint32_t nplanes,
                            const uint32_t *qs, double tol) {
+  // few queries, each a slab of the tree: one wavefront per query (2-D / 3-D)
+  const bool wave = D <= 3 && nq < kPlaneWaveQueries;
   Buf<uint32_t> ncopy(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
   Buf<RangeCopy> copies;
@@ -2185,17 +2317,34 @@ static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doubl
   Result *r = count_fill_csr(
       ctx, nq,
       [&](uint32_t *cnt) {
+        if constexpr (D <= 3) {
+          if (wave) {
+            launch(ctx, "planes_count", k_planes_wave<D, BOX, MODE, false>, dim3((uint32_t)nq), dim3(64), 0,
+                   vs.views, vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr,
+                   ncopy.p, (RangeCopy *)nullptr);
+            OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+            return;
+          }
+        }
         launch(ctx, "planes_count", k_planes<D, BOX, MODE, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views,
-               vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
-               (RangeCopy *)nullptr);
+                 vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
+                 (RangeCopy *)nullptr);
         OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
       },
       [&](Result *res, uint32_t) {
         copies = Buf<RangeCopy>(ctx, std::max<uint32_t>(hcopy, 1));
         OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
+        if constexpr (D <= 3) {
+          if (wave) {
+            launch(ctx, "planes_fill", k_planes_wave<D, BOX, MODE, true>, dim3((uint32_t)nq), dim3(64), 0, vs.views,
+                   vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets,
+                   res->d_ids, ncopy.p, copies.p);
+            return;
+          }
+        }
         launch(ctx, "planes_fill", k_planes<D, BOX, MODE, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views,
-               vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets, res->d_ids,
-               ncopy.p, copies.p);
+                 vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets, res->d_ids,
+                 ncopy.p, copies.p);
       });
   launch(ctx, "planes_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
          r->d_ids);

@@ -193,3 +193,25 @@ def test_planes_vs_oracle(gpu_ctx, kind, D, tol):
     ooff, oids = o.planes(fr, 2, tol, threads=8)
     _same_sets(goff, gids, ooff, oids)
     assert goff[-1] > 0
+
+
+def _sorted_segments(off, ids):
+    seg = np.repeat(np.arange(len(off) - 1), np.diff(off.astype(np.int64)))
+    return ids[np.lexsort((ids, seg))]
+
+
+def test_planes_many_queries_one_lane_walk(gpu_ctx):
+    """Launches of 65,536 or more plane queries walk one query per lane (fewer take a
+    wavefront per query): both walks give the oracle's sets."""
+    D, n = 3, 8000
+    geom = uniform_points(n, D, 0.0, 100.0, seed=120)
+    g = _tree(gpu_ctx, "point", D, geom, None, 5, 8)
+    o = O.OracleTree(D, "point", geom, None, 5, 8)
+    pl = _random_planes(70_000, D, 0.0, 100.0, seed=121)
+    fr = _random_planes(4 * 70_000, D, 40.0, 60.0, seed=122).reshape(70_000, 4, D + 1)
+    for qs, mode in ((pl, 0), (fr, 2)):
+        goff, gids = g.planes_batch(qs, mode, 0.25).csr()
+        ooff, oids = o.planes(qs, mode, 0.25, threads=8)
+        assert np.array_equal(goff, ooff)
+        assert np.array_equal(_sorted_segments(goff, gids), _sorted_segments(ooff, oids))
+        assert goff[-1] > 0
