@@ -363,48 +363,66 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
   return true;
 }
 
-// Distance tests of entities [j, je) against the candidate list.  Coordinates of
-// four entities are loaded before any is tested, so a lane keeps several memory
-// requests in flight; the acceptance order is the entity order, as in a plain loop.
+// Distance tests of entities [j, je) against the candidate list, in chunks of 64.
+// Pass 1 computes every distance of the chunk (coordinates of four entities are
+// loaded before any is tested, so a lane keeps several memory requests in flight)
+// and marks the entities below the lane's bound at the start of the chunk.  Pass 2
+// offers the marked entities in entity order, each re-checked against the bound at
+// that moment (the bound only shrinks), so the list evolves exactly as in a plain
+// sequential scan.  The costly list insertion then runs once per marked entity of
+// the busiest lane instead of once per entity position that any lane accepts.
 template <int D, int KCAP>
 __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D], uint32_t j, uint32_t je,
                                          double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &L, uint32_t k,
                                          double tol) {
   constexpr int U = D <= 3 ? 4 : (D == 4 ? 2 : 1);  // entities loaded ahead (register budget)
-  for (; j + U <= je; j += U) {
-    double d2[U];
+  for (uint32_t base = j; base < je; base += 64) {
+    const uint32_t cend = min(je, base + 64u);
+    const double T0 = L.T;
+    uint64_t mark = 0;
+    uint32_t e = base;
+    for (; e + U <= cend; e += U) {
+      double d2[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) d2[u] = 0.0;
+      for (int u = 0; u < U; ++u) d2[u] = 0.0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      double c[U];
+      for (int d = 0; d < D; ++d) {
+        double c[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) c[u] = tv.geom[(uint64_t)d * tv.n + j + u];
+        for (int u = 0; u < U; ++u) c[u] = tv.geom[(uint64_t)d * tv.n + e + u];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const double v = q[d] - c[u];
-        d2[u] += v * v;
+        for (int u = 0; u < U; ++u) {
+          const double v = q[d] - c[u];
+          d2[u] += v * v;
+        }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (d2[u] >= L.T) continue;
-      const double dist = sqrt(d2[u]);
+      for (int u = 0; u < U; ++u)
+        if (d2[u] < T0) mark |= 1ull << (e + u - base);
+    }
+    for (; e < cend; ++e) {
+      double d2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double v = q[d] - tv.geom[(uint64_t)d * tv.n + e];
+        d2 += v * v;
+      }
+      if (d2 < T0) mark |= 1ull << (e - base);
+    }
+    while (mark) {
+      const uint32_t x = base + (uint32_t)__builtin_ctzll(mark);
+      mark &= mark - 1;
+      double d2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double v = q[d] - tv.geom[(uint64_t)d * tv.n + x];
+        d2 += v * v;
+      }
+      if (d2 >= L.T) continue;
+      const double dist = sqrt(d2);
       if (dist >= L.F) continue;
-      knn_offer(bd, bid, L, dist, tv.ids[j + u], k, tol);
+      knn_offer(bd, bid, L, dist, tv.ids[x], k, tol);
     }
-  }
-  for (; j < je; ++j) {
-    double d2 = 0.0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
-      d2 += v * v;
-    }
-    if (d2 >= L.T) continue;
-    const double dist = sqrt(d2);
-    if (dist >= L.F) continue;
-    knn_offer(bd, bid, L, dist, tv.ids[j], k, tol);
   }
 }
 
