@@ -139,27 +139,29 @@ __device__ __forceinline__ bool knn_node_out(const KnnState<KCAP> &S, double w2,
 template <int KCAP>
 __device__ __forceinline__ void knn_insert(double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &S, double dist,
                                            uint32_t id) {
-  uint32_t pos = 0;
+  // before[t]: entry t stays ahead of the new one.  The list is sorted, so these form
+  // a prefix; the new entry lands at the first t that is not before, and the entries
+  // from there on move up one place: one compare and one select per slot.
+  bool before[KCAP];
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t < S.cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id))) ++pos;
+    before[t] = (uint32_t)t < S.cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id));
   if (S.cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
-    const double out_d = (pos == (uint32_t)KCAP) ? dist : bd[KCAP - 1];
+    const double out_d = before[KCAP - 1] ? dist : bd[KCAP - 1];
     S.dropped = S.any_dropped ? smin(S.dropped, out_d) : out_d;
     S.any_dropped = true;
   }
 #pragma unroll
-  for (int t = KCAP - 1; t >= 1; --t)
-    if ((uint32_t)t > pos && (uint32_t)t <= S.cnt) {
-      bd[t] = bd[t - 1];
-      bid[t] = bid[t - 1];
+  for (int t = KCAP - 1; t >= 1; --t) {
+    if (!before[t]) {
+      bd[t] = before[t - 1] ? dist : bd[t - 1];
+      bid[t] = before[t - 1] ? id : bid[t - 1];
     }
-#pragma unroll
-  for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t == pos) {
-      bd[t] = dist;
-      bid[t] = id;
-    }
+  }
+  if (!before[0]) {
+    bd[0] = dist;
+    bid[0] = id;
+  }
   if (S.cnt < KCAP) ++S.cnt;
 }
 
@@ -2177,11 +2179,13 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
 static Result *knn_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
                             const uint32_t *qs, const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol,
                             uint64_t *footprint = nullptr) {
+// list capacity k + 1 for the largest k of a class: one spare slot holds an entry of the
+// tolerance band above the k-th distance; a longer band is redone by the band pass
 #define OTG_KNN_K(DD, BB)                                                                              \
-  if (kmax <= 8) return knn_impl<DD, 16, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);              \
-  if (kmax <= 16) return knn_impl<DD, 20, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
-  if (kmax <= 32) return knn_impl<DD, 36, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
-  return knn_impl<DD, 72, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);
+  if (kmax <= 8) return knn_impl<DD, 9, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);               \
+  if (kmax <= 16) return knn_impl<DD, 17, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
+  if (kmax <= 32) return knn_impl<DD, 33, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
+  return knn_impl<DD, 65, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);
 #define OTG_KNN(DD)          \
   case DD:                   \
     if (box) {               \
