@@ -186,6 +186,23 @@ __device__ __forceinline__ void knn_limit(const double (&bd)[KCAP], KnnState<KCA
   }
 }
 
+// k = KCAP - 1 (the largest k of the list's class): the k-th entry and the one slot
+// that can leave sit at fixed places.  Entries 0..k-1 are <= dk <= F = tol_upper(dk),
+// so only the last slot is tested against a shrunk F.
+template <int KCAP>
+__device__ __forceinline__ void knn_limit_top(const double (&bd)[KCAP], KnnState<KCAP> &S, double tol) {
+  const double lim = tol_upper(bd[KCAP - 2], tol);
+  if (!S.heapified) {
+    S.heapified = true;
+    S.F = lim;
+    S.T = sq_reject_bound(S.F);
+  } else if (lim < S.F) {
+    S.F = lim;
+    S.T = sq_reject_bound(S.F);
+    S.cnt = (S.cnt == (uint32_t)KCAP && !(bd[KCAP - 1] > S.F)) ? KCAP : KCAP - 1;
+  }
+}
+
 // Points: the optimistic and pessimistic distances coincide, so the pessimistic
 // k-th is the optimistic list's k-th entry.
 template <int KCAP>
@@ -193,6 +210,10 @@ __device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KC
                                           uint32_t id, uint32_t k, double tol) {
   knn_insert(bd, bid, S, dist, id);
   if (S.cnt >= k) {
+    if (k == (uint32_t)KCAP - 1) {
+      knn_limit_top(bd, S, tol);
+      return;
+    }
     double dk = 0.0;
 #pragma unroll
     for (int t = 0; t < KCAP; ++t)
