@@ -819,6 +819,10 @@ struct LdsLevelIter {
 // A contiguous run of ids a range result takes whole (a contained subtree, or
 // 0..n-1 for the root shortcut when src is null).  Runs longer than kInlineCopy are
 // copied afterwards by k_range_copy, one wavefront per run, instead of one lane.
+// ids a range query's count walk also writes into its inline row; queries with no
+// more results skip the fill walk
+constexpr uint32_t kRangeInline = 8;
+
 struct RangeCopy {
   uint64_t dst;
   const uint32_t *src;
@@ -838,12 +842,80 @@ __device__ __forceinline__ void range_take(uint32_t *__restrict__ out, uint64_t 
   }
 }
 
+// Entity test of RangeSearch (ot_query.h:543-580): points DoesBoxContainPoint
+// (general.h:225-243); boxes Inside = AreBoxesOverlapped(range, e, true) (:296-308),
+// Overlap = GetBoxRelation == Overlapped.
+template <int D, bool BOX>
+__device__ __forceinline__ bool range_entity(const TreeView &tv, uint32_t j, const double (&rmin)[D],
+                                             const double (&rmax)[D], int overlap_mode, double tol) {
+  bool ok = true;
+  if (!BOX) {
+    if (tol != 0.0) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double p = tv.geom[(uint64_t)d * tv.n + j];
+        ok &= (rmin[d] - tol < p && p < rmax[d] + tol);
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double p = tv.geom[(uint64_t)d * tv.n + j];
+        ok &= (rmin[d] <= p && p <= rmax[d]);
+      }
+    }
+  } else {
+    double emn[D], emx[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      emn[d] = tv.geom[(uint64_t)d * tv.n + j];
+      emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
+    }
+    if (overlap_mode) {
+      ok = boxes_overlap_strict<D>(rmin, rmax, emn, emx, tol);
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (rmin[d] > emn[d] || emn[d] > rmax[d]) ok = false;
+        if (rmin[d] > emx[d] || emx[d] > rmax[d]) ok = false;
+      }
+    }
+  }
+  return ok;
+}
+
+// Node test of the range walk: AreBoxesOverlappingByMinPoint(range, node)
+// (internal_geometry_module.h:148-161) and DoesRangeContainBoxAD (:201-252).
+template <int D>
+__device__ __forceinline__ void range_node_test(const TreeView &tv, uint32_t node, const double (&rmin)[D],
+                                                const double (&rmax)[D], const double (&rsz)[D], double tol,
+                                                bool &overlap, bool &contained) {
+  const double *nm = &tv.nmin[(uint64_t)node * D];
+  const double *ns = &tv.nsize[(uint64_t)node * D];
+  overlap = true;
+  contained = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double maxL = rmin[d] + rsz[d];
+    const double maxR = nm[d] + ns[d];
+    if (maxL <= nm[d] - tol || maxR <= rmin[d] - tol) overlap = false;
+    contained &= range_contains(rmin[d], rmax[d], nm[d], nm[d] + ns[d], tol);
+  }
+}
+
+// count of a query whose one-lane count walk stopped at the visit cap
+constexpr uint32_t kRangeCapped = 0xFFFFFFFFu;
+
+// Frontier of the wavefront walks (range and plane queries): nodes are expanded
+// breadth-first only while fewer than 64 are pending, so at most 63 * 2^D new ones.
+constexpr int kPlaneFrontier = 64 * 8 + 64;
+
 template <int D, bool BOX, bool FILL>
 __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
                                             const uint32_t *__restrict__ qset, int overlap_mode, double tol,
                                             LdsLevelIter &stack, uint32_t *__restrict__ counts,
                                             const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
-                                            uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
+                                            uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies,
+                                            uint32_t *__restrict__ inl, uint32_t vcap) {
   const uint32_t s = qset ? qset[qi] : 0u;
   const SetParams &sp = tv.sets[s];
   double rmin[D], rmax[D];
@@ -853,7 +925,8 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
     rmax[d] = ranges[qi * 2 * D + D + d];
   }
   uint64_t o = FILL ? offsets[qi] : 0;
-  uint32_t cnt = 0;
+  uint32_t cnt = 0, visits = 0;
+  bool capped = false;
   if (sp.node_cnt != 0) {
     const uint32_t root = (uint32_t)sp.node_off;
     bool root_in = true;
@@ -869,6 +942,8 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
     if (root_in) {
       cnt = (uint32_t)sp.ent_cnt;  // iota over the entity span (:488-512)
       range_take<FILL>(out, o, nullptr, cnt, ncopy, copies);
+      if (!FILL && inl && cnt <= kRangeInline)
+        for (uint32_t t = 0; t < cnt; ++t) inl[t] = t;
     } else if (vol > 0.0) {
       double rsz[D];
 #pragma unroll
@@ -878,60 +953,25 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
       int top = 0;
       uint32_t node = root;
       for (;;) {
-        const double *nm = &tv.nmin[(uint64_t)node * D];
-        const double *ns = &tv.nsize[(uint64_t)node * D];
-        const NodeInfo ni = tv.info[node];  // loaded with the box that decides the visit
-        bool overlap = true, contained = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          // AreBoxesOverlappingByMinPoint(range min/size, node min/size) igm:148-161
-          const double maxL = rmin[d] + rsz[d];
-          const double maxR = nm[d] + ns[d];
-          if (maxL <= nm[d] - tol || maxR <= rmin[d] - tol) overlap = false;
-          contained &= range_contains(rmin[d], rmax[d], nm[d], nm[d] + ns[d], tol);
+        if (!FILL && vcap && ++visits > vcap) {  // a long walk: left to a wavefront
+          capped = true;
+          break;
         }
+        const NodeInfo ni = tv.info[node];  // loaded with the box that decides the visit
+        bool overlap, contained;
+        range_node_test<D>(tv, node, rmin, rmax, rsz, tol, overlap, contained);
         if (overlap && contained) {
           // whole subtree: its entities are the contiguous storage range
           const uint32_t e0 = ni.ebegin, e1 = tv.rend[node];
           range_take<FILL>(out, o + cnt, tv.ids + e0, e1 - e0, ncopy, copies);
+          if (!FILL && inl && cnt + (e1 - e0) <= kRangeInline)
+            for (uint32_t t = 0; t < e1 - e0; ++t) inl[cnt + t] = tv.ids[e0 + t];
           cnt += e1 - e0;
         } else if (overlap) {
           for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
-            bool ok = true;
-            if (!BOX) {
-              if (tol != 0.0) {
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                  const double p = tv.geom[(uint64_t)d * tv.n + j];
-                  ok &= (rmin[d] - tol < p && p < rmax[d] + tol);
-                }
-              } else {
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                  const double p = tv.geom[(uint64_t)d * tv.n + j];
-                  ok &= (rmin[d] <= p && p <= rmax[d]);
-                }
-              }
-            } else {
-              double emn[D], emx[D];
-#pragma unroll
-              for (int d = 0; d < D; ++d) {
-                emn[d] = tv.geom[(uint64_t)d * tv.n + j];
-                emx[d] = tv.geom[(uint64_t)(D + d) * tv.n + j];
-              }
-              if (overlap_mode) {
-                ok = boxes_overlap_strict<D>(rmin, rmax, emn, emx, tol);
-              } else {
-                // AreBoxesOverlapped(range, e, true, ..) general.h:296-308
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                  if (rmin[d] > emn[d] || emn[d] > rmax[d]) ok = false;
-                  if (rmin[d] > emx[d] || emx[d] > rmax[d]) ok = false;
-                }
-              }
-            }
-            if (ok) {
+            if (range_entity<D, BOX>(tv, j, rmin, rmax, overlap_mode, tol)) {
               if (FILL) out[o + cnt] = tv.ids[j];
+              else if (inl && cnt < kRangeInline) inl[cnt] = tv.ids[j];
               ++cnt;
             }
           }
@@ -947,7 +987,7 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
       }
     }
   }
-  if (!FILL) counts[qi] = cnt;
+  if (!FILL) counts[qi] = capped ? kRangeCapped : cnt;
 }
 
 template <int D, bool BOX, bool FILL>
@@ -956,16 +996,213 @@ __global__ void __launch_bounds__(kQB) k_range(const TreeView *__restrict__ view
                                                 const uint32_t *__restrict__ order, uint64_t nq, int overlap_mode,
                                                 double tol, uint32_t *__restrict__ counts,
                                                 const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
-                                                uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
+                                                uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies,
+                                                uint32_t *__restrict__ inl, uint32_t vcap) {
   __shared__ uint32_t s_a[kRangeStack * kQB];
   __shared__ uint8_t s_b[kRangeStack * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
   LdsLevelIter S{&s_a[threadIdx.x], &s_b[threadIdx.x]};
+  uint32_t *row = inl ? inl + ti * kRangeInline : nullptr;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out, ncopy, copies);
+    range_query<D, BOX, FILL>(tv, qi, ranges, qset, overlap_mode, tol, S, counts, offsets, out, ncopy, copies, row,
+                              vcap);
   });
+}
+
+// One wavefront per range query of a list (the queries the one-lane walk leaves:
+// more than kRangeInline ids, or more node visits than its cap): the walk of
+// range_query with the frontier expansion and per-lane subtree walks of
+// k_planes_wave.  Every node reached has all its ancestors visited, so the id set is
+// range_query's; ids are appended at the query's CSR offset in no particular order.
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ views,
+                                                    const uint32_t *__restrict__ qview,
+                                                    const double *__restrict__ ranges,
+                                                    const uint32_t *__restrict__ qset,
+                                                    const uint32_t *__restrict__ list, int overlap_mode, double tol,
+                                                    uint32_t *__restrict__ counts,
+                                                    const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
+                                                    uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
+  static_assert(D <= 3, "frontier sized for up to 8 children per node");
+  __shared__ uint32_t s_front[2][kPlaneFrontier];
+  __shared__ uint32_t s_nf[2], s_cnt, s_next;
+  __shared__ uint32_t s_a[kRangeStack * kQB];  // LdsLevelIter strides by kQB
+  __shared__ uint8_t s_b[kRangeStack * kQB];
+  const uint64_t qi = list[blockIdx.x];
+  const uint32_t lane = threadIdx.x;
+  const TreeView &tv = views[qview ? qview[qi] : 0u];
+  const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+  double rmin[D], rmax[D], rsz[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    rmin[d] = ranges[qi * 2 * D + d];
+    rmax[d] = ranges[qi * 2 * D + D + d];
+    rsz[d] = rmax[d] - rmin[d];
+  }
+  const uint64_t o = FILL ? offsets[qi] : 0;
+  // 0 nothing, 1 the whole set (root contained: iota, :488-512), 2 walk
+  int start = 0;
+  if (sp.node_cnt != 0) {
+    const uint32_t root = (uint32_t)sp.node_off;
+    bool root_in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double mn = tv.nmin[(uint64_t)root * D + d];
+      root_in &= range_contains(rmin[d], rmax[d], mn, mn + tv.nsize[(uint64_t)root * D + d], tol);
+    }
+    double vol = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vol *= rmax[d] - rmin[d];
+    start = root_in ? 1 : (vol > 0.0 ? 2 : 0);
+  }
+  auto emit_run = [&](const uint32_t *src, uint32_t len) {
+    const uint32_t at = atomicAdd(&s_cnt, len);
+    range_take<FILL>(out, o + at, src, len, ncopy, copies);
+  };
+  // node visit: whole subtree, or entities + children (returned through ni)
+  auto visit = [&](uint32_t node, NodeInfo &ni) -> bool {
+    bool overlap, contained;
+    range_node_test<D>(tv, node, rmin, rmax, rsz, tol, overlap, contained);
+    if (!overlap) return false;
+    ni = tv.info[node];
+    if (contained) {
+      emit_run(tv.ids + ni.ebegin, tv.rend[node] - ni.ebegin);
+      return false;
+    }
+    for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j)
+      if (range_entity<D, BOX>(tv, j, rmin, rmax, overlap_mode, tol)) {
+        const uint32_t at = atomicAdd(&s_cnt, 1u);
+        if (FILL) out[o + at] = tv.ids[j];
+      }
+    return ni.ccount != 0;
+  };
+  if (lane == 0) {
+    s_cnt = 0;
+    s_next = 0;
+    s_nf[0] = 0;
+    s_nf[1] = 0;
+    if (start == 1) emit_run(nullptr, (uint32_t)sp.ent_cnt);
+    if (start == 2) {
+      s_front[0][0] = (uint32_t)sp.node_off;
+      s_nf[0] = 1;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  while (s_nf[cur] != 0 && s_nf[cur] < 64) {
+    const uint32_t nf = s_nf[cur];
+    if (lane == 0) s_nf[cur ^ 1] = 0;
+    __syncthreads();
+    if (lane < nf) {
+      NodeInfo ni;
+      if (visit(s_front[cur][lane], ni)) {
+        const uint32_t at = atomicAdd(&s_nf[cur ^ 1], ni.ccount);
+        for (uint32_t c = 0; c < ni.ccount; ++c) s_front[cur ^ 1][at + c] = ni.cbegin + c;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const uint32_t nf = s_nf[cur];
+  LdsLevelIter stack{&s_a[lane], &s_b[lane]};
+  for (;;) {
+    const uint32_t item = atomicAdd(&s_next, 1u);
+    if (item >= nf) break;
+    int top = 0;
+    uint32_t node = s_front[cur][item];
+    for (;;) {
+      NodeInfo ni;
+      if (visit(node, ni)) {
+        stack.nx(top) = ni.cbegin + ni.ccount - 1;
+        stack.left(top) = (uint8_t)ni.ccount;
+        ++top;
+      }
+      if (top == 0) break;
+      node = stack.nx(top - 1)--;
+      if (--stack.left(top - 1) == 0) --top;
+    }
+  }
+  __syncthreads();
+  if (!FILL && lane == 0) counts[qi] = s_cnt;
+}
+
+// Queries whose count walk found at most kRangeInline ids have them all in their
+// row of the inline buffer (walk order): copied into the CSR here, one thread per
+// (query, slot).  The others are flagged for the fill walk.
+// (ids ascending: every range result is returned sorted by id)
+__global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, const uint32_t *__restrict__ cnt,
+                               const uint32_t *__restrict__ capped, const uint64_t *__restrict__ off,
+                               const uint32_t *__restrict__ inl, uint32_t *__restrict__ out,
+                               uint32_t *__restrict__ heavy) {
+  const uint64_t ti = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (ti >= nq) return;
+  const uint64_t qi = order ? order[ti] : ti;
+  const uint32_t c = cnt[qi];
+  const bool h = capped[ti] || c > kRangeInline;  // a capped walk left its row incomplete
+  heavy[ti] = h;
+  if (h || c == 0) return;
+  uint32_t v[kRangeInline];
+#pragma unroll
+  for (uint32_t t = 0; t < kRangeInline; ++t) v[t] = t < c ? inl[ti * kRangeInline + t] : 0xFFFFFFFFu;
+#pragma unroll
+  for (uint32_t a = 1; a < kRangeInline; ++a)
+#pragma unroll
+    for (uint32_t b = a; b > 0; --b)
+      if (v[b] < v[b - 1]) {
+        const uint32_t tmp = v[b];
+        v[b] = v[b - 1];
+        v[b - 1] = tmp;
+      }
+  const uint64_t o = off[qi];
+#pragma unroll
+  for (uint32_t t = 0; t < kRangeInline; ++t)
+    if (t < c) out[o + t] = v[t];
+}
+
+// Sorting the larger results: key (rank of the query among them, id) per id.
+__global__ void k_range_sort_keys(const uint32_t *__restrict__ hlist, uint32_t nheavy, const uint64_t *__restrict__ off,
+                                  const uint32_t *__restrict__ hpos, const uint32_t *__restrict__ ids,
+                                  uint64_t *__restrict__ keys) {
+  const uint32_t h = blockIdx.x;
+  if (h >= nheavy) return;
+  const uint32_t qi = hlist[h];
+  const uint64_t o = off[qi], c = off[qi + 1] - o;
+  for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) keys[hpos[h] + t] = ((uint64_t)h << 32) | ids[o + t];
+}
+
+__global__ void k_range_sort_back(const uint32_t *__restrict__ hlist, uint32_t nheavy, const uint64_t *__restrict__ off,
+                                  const uint32_t *__restrict__ hpos, const uint64_t *__restrict__ keys,
+                                  uint32_t *__restrict__ ids) {
+  const uint32_t h = blockIdx.x;
+  if (h >= nheavy) return;
+  const uint32_t qi = hlist[h];
+  const uint64_t o = off[qi], c = off[qi + 1] - o;
+  for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) ids[o + t] = (uint32_t)keys[hpos[h] + t];
+}
+
+__global__ void k_iota_u32(uint32_t *__restrict__ a, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+__global__ void k_range_heavy_counts(const uint32_t *__restrict__ hlist, uint32_t nheavy,
+                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ hc) {
+  const uint64_t h = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (h < nheavy) hc[h] = (uint32_t)(off[hlist[h] + 1] - off[hlist[h]]);
+}
+
+__global__ void k_range_capped(const uint32_t *__restrict__ order, uint64_t nq, const uint32_t *__restrict__ cnt,
+                               uint32_t *__restrict__ capped) {
+  const uint64_t ti = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (ti < nq) capped[ti] = cnt[order ? order[ti] : ti] == kRangeCapped;
+}
+
+__global__ void k_range_heavy_list(const uint32_t *__restrict__ heavy, const uint32_t *__restrict__ pos, uint64_t nq,
+                                   const uint32_t *__restrict__ order, uint32_t *__restrict__ horder) {
+  const uint64_t ti = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (ti < nq && heavy[ti]) horder[pos[ti]] = order ? order[ti] : (uint32_t)ti;
 }
 
 // One wavefront per deferred run of a range result.
@@ -1152,8 +1389,6 @@ __device__ __forceinline__ bool planes_entity(const TreeView &tv, uint32_t j, co
 // walk of planes_query, bounded by the tree depth).  Every node a walk reaches has
 // all its ancestors visited, so the set of ids is planes_query's; ids are appended
 // at the query's CSR offset through an LDS counter, in no particular order.
-constexpr int kPlaneFrontier = 64 * 8 + 64;
-
 template <int D, bool BOX, int MODE, bool FILL>
 __global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                                      const double *__restrict__ planes, uint32_t nplanes,
@@ -2264,15 +2499,77 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
                       knn_F0(max_within, tol), tol, footprint);
 }
 
+// One-lane walks stop counting after this many node visits; such queries, and those
+// with more than kRangeInline ids, are walked by a wavefront each (2-D / 3-D).
+constexpr uint32_t kRangeVisitCap = 256;
+
+// Positions (in walk order) of the flagged queries, as query indices.
+static uint32_t flagged_list(Ctx *ctx, const uint32_t *flags, const uint32_t *order, uint64_t nq, Buf<uint32_t> &list) {
+  Buf<uint32_t> pos(ctx, nq + 1), total(ctx, 1);
+  exclusive_scan_u32(ctx, flags, pos.p, nq, total.p);
+  uint32_t n = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&n, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  list = Buf<uint32_t>(ctx, std::max<uint32_t>(n, 1));
+  launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, flags,
+         (const uint32_t *)pos.p, nq, order, list.p);
+  return n;
+}
+
+// Sorts the ids of the listed queries of a CSR result (qlist null: queries 0..n-1)
+// ascending: one radix sort of (rank in the list, id) keys over all their ids.
+static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n) {
+  if (n == 0) return;
+  Buf<uint32_t> all;
+  if (!qlist) {
+    all = Buf<uint32_t>(ctx, n);
+    launch(ctx, "iota_u32", k_iota_u32, dim3(div_up(n, 256)), dim3(256), 0, all.p, n);
+    qlist = all.p;
+  }
+  Buf<uint32_t> hc(ctx, n), hpos(ctx, n + 1), htot(ctx, 1);
+  launch(ctx, "range_heavy_counts", k_range_heavy_counts, dim3(div_up(n, 256)), dim3(256), 0, qlist, n,
+         (const uint64_t *)r->d_offsets, hc.p);
+  exclusive_scan_u32(ctx, hc.p, hpos.p, n, htot.p);
+  uint32_t nh = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&nh, htot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (nh < 2) return;
+  Buf<uint64_t> keys(ctx, nh);
+  Buf<uint32_t> vals(ctx, nh);
+  launch(ctx, "range_sort_keys", k_range_sort_keys, dim3(n), dim3(256), 0, qlist, n, (const uint64_t *)r->d_offsets,
+         (const uint32_t *)hpos.p, (const uint32_t *)r->d_ids, keys.p);
+  radix_sort_pairs(ctx, keys.p, vals.p, nh, 32 + std::max(1u, bit_width32(n - 1)));
+  launch(ctx, "range_sort_back", k_range_sort_back, dim3(n), dim3(256), 0, qlist, n, (const uint64_t *)r->d_offsets,
+         (const uint32_t *)hpos.p, (const uint64_t *)keys.p, r->d_ids);
+}
+
 template <int D, bool BOX>
 static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, int mode,
                           double tol) {
+  constexpr bool kWave = D <= 3;
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
+  const uint32_t *ord = order.p;
   Buf<uint32_t> cnt(ctx, std::max<uint64_t>(nq, 1)), ncopy(ctx, 1);
+  Buf<uint32_t> inl(ctx, std::max<uint64_t>(nq, 1) * kRangeInline);
   OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
+  // 1. one-lane count walks (ids of small results kept inline), capped for the wavefront path
   launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
-         (RangeCopy *)nullptr);
+         ord, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p, (RangeCopy *)nullptr,
+         inl.p, kWave ? kRangeVisitCap : 0u);
+  Buf<uint32_t> capped(ctx, std::max<uint64_t>(nq, 1));
+  if constexpr (kWave) {
+    // 2. the capped ones: counted by a wavefront each
+    launch(ctx, "range_capped", k_range_capped, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq, (const uint32_t *)cnt.p,
+           capped.p);
+    Buf<uint32_t> clist;
+    const uint32_t ncap = flagged_list(ctx, capped.p, ord, nq, clist);
+    launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(ncap), dim3(64), 0, vs.views, vs.qview, q, qs,
+           (const uint32_t *)clist.p, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
+           (RangeCopy *)nullptr);
+  } else {
+    OTG_HIP_CHECK(hipMemsetAsync(capped.p, 0, std::max<uint64_t>(nq, 1) * 4, ctx->stream));
+  }
+  // 3. offsets
   Result *r = new Result();
   r->ctx = ctx;
   r->nq = nq;
@@ -2285,15 +2582,33 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
+  // hcopy counts the deferred runs of every count walk (a capped query's twice): a bound
   Buf<RangeCopy> copies(ctx, std::max<uint32_t>(hcopy, 1));
   OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
-  launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-         (const uint32_t *)order.p, nq, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
-         ncopy.p, copies.p);
+  // 4. small complete results from the inline rows; the rest are filled by a second walk
+  Buf<uint32_t> heavy(ctx, std::max<uint64_t>(nq, 1));
+  launch(ctx, "range_inline", k_range_inline, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq,
+         (const uint32_t *)cnt.p, (const uint32_t *)capped.p, (const uint64_t *)r->d_offsets, (const uint32_t *)inl.p,
+         r->d_ids, heavy.p);
+  Buf<uint32_t> hlist;
+  const uint32_t nheavy = flagged_list(ctx, heavy.p, ord, nq, hlist);
+  if constexpr (kWave)
+    launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(nheavy), dim3(64), 0, vs.views, vs.qview, q, qs,
+           (const uint32_t *)hlist.p, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
+           ncopy.p, copies.p);
+  else
+    launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nheavy, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+           qs, (const uint32_t *)hlist.p, (uint64_t)nheavy, mode, tol, (uint32_t *)nullptr,
+           (const uint64_t *)r->d_offsets, r->d_ids, ncopy.p, copies.p, (uint32_t *)nullptr, 0u);
+  // 5. long contained runs, one wavefront each
+  OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   launch(ctx, "range_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
          r->d_ids);
+  // 6. the larger results sorted by id
+  sort_segments(ctx, r, hlist.p, nheavy);
   return r;
 }
 
@@ -2391,6 +2706,8 @@ static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doubl
       });
   launch(ctx, "planes_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
          r->d_ids);
+  // the wavefront walks append in no fixed order: every list is returned sorted by id
+  sort_segments(ctx, r, nullptr, (uint32_t)nq);
   return r;
 }
 

@@ -146,7 +146,8 @@ def test_range_vs_oracle(gpu_ctx, kind, D, mode):
     ooff, oids = o.range(ranges, overlap=bool(mode), threads=8)
     assert np.array_equal(goff, ooff)
     for qi in range(len(ranges)):
-        assert sorted(gids[goff[qi]:goff[qi + 1]].tolist()) == sorted(oids[ooff[qi]:ooff[qi + 1]].tolist()), qi
+        # each list sorted by id; the oracle's is the reference's walk order
+        assert gids[goff[qi]:goff[qi + 1]].tolist() == sorted(oids[ooff[qi]:ooff[qi + 1]].tolist()), qi
 
 
 # ---- ray first -----------------------------------------------------------------------

@@ -128,8 +128,13 @@ def test_served_vs_oracle():
     b = srv.submit_range(7, rg)
     ra, _, _ = srv.wait(a)
     rb, _, _ = srv.wait(b)
-    for got, exp in ((ra.csr(), o.knn(qs, 8, DBL_MAX, BASE_TOL, threads=8)), (rb.csr(), o.range(rg, 0, threads=8))):
-        assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1])
+    got, exp = ra.csr(), o.knn(qs, 8, DBL_MAX, BASE_TOL, threads=8)
+    assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1])
+    # range lists come back sorted by id (the reference returns its walk order; compared as sets)
+    got, exp = rb.csr(), o.range(rg, 0, threads=8)
+    assert np.array_equal(got[0], exp[0])
+    for qi in range(len(rg)):
+        assert got[1][got[0][qi]:got[0][qi + 1]].tolist() == sorted(exp[1][exp[0][qi]:exp[0][qi + 1]].tolist())
     srv.close()
 
 
