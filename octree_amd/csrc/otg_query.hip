@@ -313,7 +313,10 @@ struct ArrayStack {
   }
 };
 
-constexpr int kLdsStack = 24;  // entries per lane: 24 x 8 B x 128 lanes = 24 KiB per block
+// Level entries per lane: only internal nodes push one, and they sit at depths 0..maxDepth-1 on
+// the current path, so kMaxDepth entries always suffice (19 x 8 B x 128 lanes = 19 KiB per
+// block).
+constexpr int kLdsStack = kMaxDepth;
 
 
 // One lane per query (queries sorted by (set, Morton code) so neighbouring lanes
@@ -432,19 +435,36 @@ __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D
       }
       if (d2 < T0) mark |= 1ull << (e - base);
     }
-    while (mark) {
-      const uint32_t x = base + (uint32_t)__builtin_ctzll(mark);
-      mark &= mark - 1;
+    // the next marked entity's coordinates and id are loaded before the current one is
+    // offered, so the reload overlaps the list update
+    if (!mark) continue;
+    uint32_t x = base + (uint32_t)__builtin_ctzll(mark);
+    mark &= mark - 1;
+    double c[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) c[d] = tv.geom[(uint64_t)d * tv.n + x];
+    uint32_t id = tv.ids[x];
+    for (;;) {
       double d2 = 0.0;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const double v = q[d] - tv.geom[(uint64_t)d * tv.n + x];
+        const double v = q[d] - c[d];
         d2 += v * v;
       }
-      if (d2 >= L.T) continue;
-      const double dist = sqrt(d2);
-      if (dist >= L.F) continue;
-      knn_offer(bd, bid, L, dist, tv.ids[x], k, tol);
+      const uint32_t cur_id = id;
+      const bool more = mark != 0;
+      if (more) {
+        x = base + (uint32_t)__builtin_ctzll(mark);
+        mark &= mark - 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[d] = tv.geom[(uint64_t)d * tv.n + x];
+        id = tv.ids[x];
+      }
+      if (d2 < L.T) {
+        const double dist = sqrt(d2);
+        if (dist < L.F) knn_offer(bd, bid, L, dist, cur_id, k, tol);
+      }
+      if (!more) break;
     }
   }
 }
@@ -595,7 +615,7 @@ __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ vie
   // 2-D/3-D: level entries in LDS, one per tree level.  4-D..6-D: up to 2^D - 1
   // pending siblings per level outgrow any short stack, so the walk keeps the
   // worst-case private stack.  Neither can overflow.
-  static_assert(kLdsStack >= kMaxDepth + 1, "one level entry per depth");
+  static_assert(kLdsStack >= kMaxDepth, "one level entry per internal depth");
   __shared__ uint32_t s_n[(D <= 3 ? kLdsStack : 1) * kQB];
   __shared__ uint32_t s_w[(D <= 3 ? kLdsStack : 1) * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
