@@ -2521,7 +2521,7 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
 
 // One-lane walks stop counting after this many node visits; such queries, and those
 // with more than kRangeInline ids, are walked by a wavefront each (2-D / 3-D).
-constexpr uint32_t kRangeVisitCap = 256;
+constexpr uint32_t kRangeVisitCap = 64;
 
 // Positions (in walk order) of the flagged queries, as query indices.
 static uint32_t flagged_list(Ctx *ctx, const uint32_t *flags, const uint32_t *order, uint64_t nq, Buf<uint32_t> &list) {
