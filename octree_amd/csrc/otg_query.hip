@@ -832,8 +832,9 @@ constexpr int kRangeStack = kMaxDepth + 1;  // 20 x 5 B x 128 lanes = 12.5 KiB p
 struct LdsLevelIter {
   uint32_t *a;  // next pending child (children are taken last to first)
   uint8_t *b;   // children left at this level (<= 2^D)
-  __device__ __forceinline__ uint32_t &nx(int i) { return a[i * kQB]; }
-  __device__ __forceinline__ uint8_t &left(int i) { return b[i * kQB]; }
+  int stride = kQB;  // lanes of the block sharing the arrays
+  __device__ __forceinline__ uint32_t &nx(int i) { return a[i * stride]; }
+  __device__ __forceinline__ uint8_t &left(int i) { return b[i * stride]; }
 };
 
 // A contiguous run of ids a range result takes whole (a contained subtree, or
@@ -927,7 +928,7 @@ constexpr uint32_t kRangeCapped = 0xFFFFFFFFu;
 
 // Frontier of the wavefront walks (range and plane queries): nodes are expanded
 // breadth-first only while fewer than 64 are pending, so at most 63 * 2^D new ones.
-constexpr int kPlaneFrontier = 64 * 8 + 64;
+constexpr int kWaveFrontier = 64 * 8 + 64;
 
 template <int D, bool BOX, bool FILL>
 __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
@@ -1046,10 +1047,10 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
                                                     const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
                                                     uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
-  __shared__ uint32_t s_front[2][kPlaneFrontier];
+  __shared__ uint32_t s_front[2][kWaveFrontier];
   __shared__ uint32_t s_nf[2], s_cnt, s_next;
-  __shared__ uint32_t s_a[kRangeStack * kQB];  // LdsLevelIter strides by kQB
-  __shared__ uint8_t s_b[kRangeStack * kQB];
+  __shared__ uint32_t s_a[kRangeStack * 64];
+  __shared__ uint8_t s_b[kRangeStack * 64];
   const uint64_t qi = list[blockIdx.x];
   const uint32_t lane = threadIdx.x;
   const TreeView &tv = views[qview ? qview[qi] : 0u];
@@ -1126,7 +1127,7 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     cur ^= 1;
   }
   const uint32_t nf = s_nf[cur];
-  LdsLevelIter stack{&s_a[lane], &s_b[lane]};
+  LdsLevelIter stack{&s_a[lane], &s_b[lane], 64};
   for (;;) {
     const uint32_t item = atomicAdd(&s_next, 1u);
     if (item >= nf) break;
@@ -1417,10 +1418,10 @@ __global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__
                                                      uint32_t *__restrict__ out, uint32_t *__restrict__ ncopy,
                                                      RangeCopy *__restrict__ copies) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
-  __shared__ uint32_t s_front[2][kPlaneFrontier];
+  __shared__ uint32_t s_front[2][kWaveFrontier];
   __shared__ uint32_t s_nf[2], s_cnt, s_next;
-  __shared__ uint32_t s_a[kRangeStack * kQB];  // LdsLevelIter strides by kQB
-  __shared__ uint8_t s_b[kRangeStack * kQB];
+  __shared__ uint32_t s_a[kRangeStack * 64];
+  __shared__ uint8_t s_b[kRangeStack * 64];
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   const TreeView &tv = views[qview ? qview[qi] : 0u];
@@ -1475,7 +1476,7 @@ __global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__
   }
   // phase 2: the lanes walk the frontier's subtrees
   const uint32_t nf = s_nf[cur];
-  LdsLevelIter stack{&s_a[lane], &s_b[lane]};
+  LdsLevelIter stack{&s_a[lane], &s_b[lane], 64};
   for (;;) {
     const uint32_t item = atomicAdd(&s_next, 1u);
     if (item >= nf) break;
