@@ -17,6 +17,8 @@
 //                                                         (core/ot_managed.h:47-893, core/ot_aliases.h:417-447)
 //   GetNodeCount / GetNodeChildren / GetNodeEntities / GetNodeMinPoint / GetNodeSize / GetNodeBox
 //                                                         (core/ot_static_linear_core.h:178-268)
+//   Serialize(): the BinaryOutputArchive bytes of the core / managed tree
+//                                                         (serialization.h:373-386, :433-439)
 // plus batch overloads over spans of queries returning CSR results.
 //
 // The tree copies the entities into HBM at Create and keeps them resident; later
@@ -388,6 +390,20 @@ class StaticOrthoTreeGPU {
 
   otg_tree *native_handle() const noexcept { return tree_; }
 
+  // Archive of the core in the reference's BinaryOutputArchive layout
+  // (serialize(StaticLinearOrthoTreeCore) serialization.h:373-386); largeDataset
+  // writes the ORTHOTREE__LARGE_DATASET widths (core/types.h:51-55).
+  std::vector<uint8_t> Serialize(bool largeDataset = false) const noexcept {
+    std::vector<uint8_t> out;
+    if (!tree_) return out;
+    const uint32_t flags = largeDataset ? OTG_ARCHIVE_LARGE_DATASET : 0u;
+    uint64_t n = 0;
+    detail::check(otg_tree_serialize(tree_, 0, flags, nullptr, 0, &n), "otg_tree_serialize", ctx().get());
+    out.resize(n);
+    detail::check(otg_tree_serialize(tree_, 0, flags, out.data(), n, &n), "otg_tree_serialize", ctx().get());
+    return out;
+  }
+
  private:
   struct HostNodes {
     bool loaded = false;
@@ -539,6 +555,35 @@ class OrthoTreeManaged {
   }
 
   TOrthoTreeCore const &GetCore() const noexcept { return m_tree; }
+
+  // serialize(OrthoTreeManaged) serialization.h:433-439: the core, then the
+  // entity vector (points as std::array, boxes as BoundingBoxND {Min, Max}).
+  std::vector<uint8_t> Serialize(bool largeDataset = false) const noexcept {
+    std::vector<uint8_t> out = m_tree.Serialize(largeDataset);
+    auto put = [&out](const void *p, std::size_t bytes) {
+      const auto *b = static_cast<const uint8_t *>(p);
+      out.insert(out.end(), b, b + bytes);
+    };
+    auto count = [&](uint64_t c) {
+      if (largeDataset) {
+        put(&c, 8);
+      } else {
+        const uint32_t c32 = static_cast<uint32_t>(c);
+        put(&c32, 4);
+      }
+    };
+    constexpr int D = TOrthoTreeCore::DIMENSION_NO;
+    constexpr int ARRAYS = std::is_same_v<Entity, TBox> ? 2 : 1;
+    count(m_entities.size());
+    for (const Entity &e : m_entities) {
+      const double *v = reinterpret_cast<const double *>(&e);
+      for (int a = 0; a < ARRAYS; ++a) {
+        count(D);
+        put(v + a * D, sizeof(double) * D);
+      }
+    }
+    return out;
+  }
   EntityContainer const &GetData() const noexcept { return m_entities; }
   Entity const &Get(EntityID entityID) const noexcept { return m_entities[entityID]; }
 

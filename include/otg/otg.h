@@ -134,6 +134,23 @@ OTG_API otg_status otg_tree_export(const otg_tree *tree, uint32_t set, uint8_t *
                                    uint64_t *node_key, double *node_min, double *node_size, uint32_t *entity_ids,
                                    uint64_t *sorted_keys);
 
+/* Static-core archive of one set: the byte stream the reference's
+ * BinaryOutputArchive writes for a StaticLinearOrthoTreeCore (serialize of the
+ * core serialization.h:373-386 and of its base :359-371; std::variant
+ * serialization/stl/variant.h, std::vector stl/vector.h, std::array
+ * stl/array.h; BinaryArchive::operator& serialization/binary_archive.h:85-175,
+ * little-endian).  The node storage alternative is the one Create picks
+ * (ot_static_linear_core.h:330-344, :367-372, :573-578).  flags:
+ * OTG_ARCHIVE_LARGE_DATASET writes sizes and NodeStorageGeneral segments as
+ * uint64 (ORTHOTREE__LARGE_DATASET, core/types.h:51-55 and :172-176).
+ * *size receives the archive length; the bytes are written only when buf is
+ * not NULL, and buf_size must then be at least that length.  The archive of an
+ * OrthoTreeManaged (serialization.h:433-439) is this followed by the entity
+ * vector, which the owner of the entities appends. */
+#define OTG_ARCHIVE_LARGE_DATASET 1u
+OTG_API otg_status otg_tree_serialize(const otg_tree *tree, uint32_t set, uint32_t flags, uint8_t *buf,
+                                      uint64_t buf_size, uint64_t *size);
+
 /* ---- queries ------------------------------------------------------------ */
 /* Results are CSR: offsets[nq+1] (uint64) and ids (uint32, local entity ids).
  * set_ids: per-query set of a forest (NULL = set 0); same memory space as the

@@ -290,6 +290,25 @@ class StaticTree:
         out["space_max"] = np.array(si.space_max[:D])
         return out
 
+    def serialize(self, s: int = 0, large_dataset: bool = False, entities=None) -> bytes:
+        """Binary archive of set s in the reference's BinaryOutputArchive layout
+        (serialization.h:373-386, serialization/binary_archive.h:85-175).  With
+        `entities` (the build input of the set) the archive is the one of the
+        owning OrthoTreeManaged tree (serialization.h:433-439): the core
+        followed by the entity vector."""
+        flags = _abi.OTG_ARCHIVE_LARGE_DATASET if large_dataset else 0
+        n = C.c_uint64(0)
+        L = _abi.lib()
+        _check(L.otg_tree_serialize(self.handle, s, flags, None, 0, C.byref(n)), "otg_tree_serialize", self.ctx)
+        buf = np.empty(int(n.value), np.uint8)
+        _check(L.otg_tree_serialize(self.handle, s, flags, _ptr(buf), buf.size, C.byref(n)), "otg_tree_serialize",
+               self.ctx)
+        out = buf.tobytes()
+        if entities is not None:
+            from .archive import entity_vector_bytes
+            out += entity_vector_bytes(_f64(entities, self._cols()), self.dim, self.kind == "box", large_dataset)
+        return out
+
     # -- batched queries (CSR) -------------------------------------------------
     def _sets_arg(self, set_ids, nq):
         if set_ids is None:

@@ -20,6 +20,7 @@ OTG_RANGE_OVERLAP = 1
 OTG_PLANE_SEARCH = 0
 OTG_PLANE_POSITIVE = 1
 OTG_PLANE_FRUSTUM = 2
+OTG_ARCHIVE_LARGE_DATASET = 1
 
 STATUS_NAMES = {
     0: "OTG_OK",
@@ -37,7 +38,7 @@ EXPORTED = [
     "otg_ctx_create", "otg_ctx_destroy", "otg_ctx_synchronize", "otg_status_string", "otg_ctx_last_error",
     "otg_ctx_mem_stats", "otg_ctx_release_cache", "otg_ctx_set_profiling", "otg_ctx_kernel_times",
     "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_free", "otg_tree_get_info",
-    "otg_tree_get_set_info", "otg_tree_export", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
+    "otg_tree_get_set_info", "otg_tree_export", "otg_tree_serialize", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
     "otg_collide_trees", "otg_pick", "otg_ray_all", "otg_planes",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
@@ -100,6 +101,7 @@ def lib() -> C.CDLL:
         "otg_tree_get_info": (C.c_int, [P, C.POINTER(TreeInfo)]),
         "otg_tree_get_set_info": (C.c_int, [P, C.c_uint32, C.POINTER(SetInfo)]),
         "otg_tree_export": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P, P]),
+        "otg_tree_serialize": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64, U64P]),
         "otg_knn": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, PP]),
         "otg_knn_footprint": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, U64P]),
         "otg_range": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_int, C.c_double, PP]),

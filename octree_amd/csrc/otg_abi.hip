@@ -1,5 +1,6 @@
 // otg_abi.hip -- the extern "C" boundary of libotg: contexts, trees, results.
 // No C++ exception crosses this boundary; every failure becomes a status code.
+#include <algorithm>
 #include <cfloat>
 #include <cstdio>
 #include <memory>
@@ -123,6 +124,128 @@ Tree::~Tree() {
   ctx->free(d_nkey);
   ctx->free(d_nmin);
   ctx->free(d_nsize);
+}
+
+// ---- static-core archive ----------------------------------------------------
+// Byte sink of the reference's BinaryOutputArchive (serialization/binary_archive.h:
+// 85-175): arithmetic values in their own width, little-endian; counts as
+// serialized_size_t (uint32, or uint64 with ORTHOTREE__LARGE_DATASET).  With a
+// NULL destination it only counts.
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "the archive is written in host byte order");
+
+struct ArchiveOut {
+  uint8_t *dst;
+  bool large;
+  uint64_t n = 0;
+  template <typename T>
+  void put(T v) {
+    if (dst) std::memcpy(dst + n, &v, sizeof(T));
+    n += sizeof(T);
+  }
+  void count(uint64_t c) {  // make_size_tag(size) of a std::vector / std::array / variant index
+    if (large) {
+      put<uint64_t>(c);
+    } else {
+      if (c > 0xFFFFFFFFull) throw Error{OTG_ERR_UNSUPPORTED, "archive count exceeds serialized_size_t (uint32)"};
+      put<uint32_t>((uint32_t)c);
+    }
+  }
+  void vec(const double *v, int D) {  // std::array<double, D> (stl/array.h)
+    count((uint64_t)D);
+    for (int d = 0; d < D; ++d) put<double>(v[d]);
+  }
+};
+
+struct ArchiveTable {
+  int D;
+  bool box;
+  uint32_t max_depth, cap;
+  uint64_t ne, nn;
+  const double *space_min, *space_size, *depth_size;  // depth_size: (max_depth + 2) x D
+  const uint8_t *depth;
+  const uint32_t *ebegin, *ecount, *cbegin, *ccount, *ids;
+  const double *nmin, *nsize;
+};
+
+// GetMaxPossibleNodeCount + IsFit (ot_static_linear_core.h:330-344, :367-372):
+// 0 = NodeStorage256, 1 = NodeStorage65536, 2 = NodeStorageGeneral.
+static int node_storage_index(uint64_t n, uint32_t M, int D) {
+  auto full_tree = [D](uint64_t depth) -> uint64_t {  // (2^(D*depth) - 1) / (2^D - 1)
+    if ((uint64_t)D * depth >= 64) return ~0ull;
+    return ((1ull << (D * depth)) - 1) / ((1ull << D) - 1);
+  };
+  const uint64_t by_depth = full_tree(M);
+  const uint64_t bw = n > 1 ? 64 - (uint64_t)__builtin_clzll(n - 1) : 0;
+  const uint64_t full = std::min<uint64_t>(M, bw / (uint64_t)D);
+  const uint64_t by_entities = full_tree(full) + ((uint64_t)M - full) * n;
+  const uint64_t max_nodes = std::min(by_entities, by_depth);
+  if (max_nodes < 0xFFull && n < 0xFFull) return 0;
+  if (max_nodes < 0xFFFFull && n < 0xFFFFull) return 1;
+  return 2;
+}
+
+template <typename Index, typename Seg>
+static void write_node_storage(ArchiveOut &ar, const ArchiveTable &t) {
+  // nodeChildSegmentBegins starts as {1}; a node with children extends it to
+  // id + 2 entries and adds its child count to the last (CreateNode, :383-412)
+  std::vector<uint64_t> begins{1};
+  for (uint64_t i = 0; i < t.nn; ++i) {
+    if (!t.ccount[i]) continue;
+    if (begins.back() != t.cbegin[i]) throw Error{OTG_ERR_HIP, "node table children are not in BFS order"};
+    begins.resize(i + 2, begins.back());
+    begins.back() += t.ccount[i];
+  }
+  ar.count(begins.size());
+  for (uint64_t b : begins) ar.put<Index>((Index)b);
+  ar.count(t.nn);
+  for (uint64_t i = 0; i < t.nn; ++i) {  // Segment: begin, length (serialization.h:84-89)
+    ar.put<Seg>((Seg)t.ebegin[i]);
+    ar.put<Seg>((Seg)t.ecount[i]);
+  }
+}
+
+// serialize(StaticLinearOrthoTreeCore) serialization.h:373-386 and of the base
+// OrthoTreeCoreBase :359-371 with the members of ot_base.h:76-89 and
+// ot_static_linear_core.h:85-89.  Point trees store MinPoint geometry (no tree
+// box, m_nodeSize per depth); loose box trees store MBRs (the nominal tree box,
+// no m_nodeSize).  An empty tree keeps the default variant (NodeStorage256,
+// empty vectors): Create returns before choosing one (:538-539).
+static void write_static_archive(ArchiveOut &ar, const ArchiveTable &t) {
+  const int D = t.D;
+  ar.put<uint32_t>(0);  // StaticLinearOrthoTreeCore::SERIALIZED_VERSION_ID
+  ar.put<uint32_t>(0);  // OrthoTreeCoreBase::SERIALIZED_VERSION_ID
+  ar.put<uint64_t>(t.cap);        // m_maxElementNum (std::size_t)
+  ar.put<uint32_t>(t.max_depth);  // m_maxDepthID (depth_t)
+  if (t.box) {                    // m_nominalTreeBox {minPoint, size}
+    ar.vec(t.space_min, D);
+    ar.vec(t.space_size, D);
+  } else {  // m_nodeSize: max_depth + 2 entries (InitBase, ot_base.h:119-130)
+    ar.count(t.max_depth + 2);
+    for (uint32_t k = 0; k < t.max_depth + 2; ++k) ar.vec(t.depth_size + (size_t)k * D, D);
+  }
+  const int kind = t.ne ? node_storage_index(t.ne, t.max_depth, D) : 0;
+  ar.count((uint64_t)kind);  // std::variant index (stl/variant.h)
+  if (t.ne == 0) {
+    ar.count(0);
+    ar.count(0);
+  } else if (kind == 0) {
+    write_node_storage<uint8_t, uint8_t>(ar, t);
+  } else if (kind == 1) {
+    write_node_storage<uint16_t, uint16_t>(ar, t);
+  } else if (ar.large) {
+    write_node_storage<uint32_t, uint64_t>(ar, t);
+  } else {
+    write_node_storage<uint32_t, uint32_t>(ar, t);
+  }
+  ar.count(t.nn);  // m_nodeDepthIDs
+  for (uint64_t i = 0; i < t.nn; ++i) ar.put<uint8_t>(t.depth[i]);
+  ar.count(t.ne);  // m_entityStorage
+  for (uint64_t i = 0; i < t.ne; ++i) ar.put<uint32_t>(t.ids[i]);
+  ar.count(t.nn);  // m_nodeGeometry
+  for (uint64_t i = 0; i < t.nn; ++i) {
+    ar.vec(t.nmin + i * D, D);
+    if (t.box) ar.vec(t.nsize + i * D, D);
+  }
 }
 
 }  // namespace otg
@@ -367,6 +490,32 @@ OTG_API otg_status otg_tree_export(const otg_tree *t, uint32_t set, uint8_t *nod
       const uint64_t mask = (L.cbits + L.dbits >= 64) ? ~0ull : ((1ull << (L.cbits + L.dbits)) - 1);
       for (uint64_t i = 0; i < ne; ++i) sorted_keys[i] = keys[i] & mask;
     }
+  });
+}
+
+OTG_API otg_status otg_tree_serialize(const otg_tree *t, uint32_t set, uint32_t flags, uint8_t *buf,
+                                      uint64_t buf_size, uint64_t *size) {
+  if (!t || !size || set >= t->sets.size() || (flags & ~OTG_ARCHIVE_LARGE_DATASET)) return OTG_ERR_INVALID_ARG;
+  const otg::SetParams &sp = t->sets[set];
+  const uint64_t nn = sp.node_cnt, ne = sp.ent_cnt;
+  std::vector<uint8_t> depth(nn);
+  std::vector<uint32_t> eb(nn), ec(nn), cb(nn), cc(nn), ids(ne);
+  std::vector<double> nmin(nn * t->dim), nsize(nn * t->dim);
+  otg_status st = otg_tree_export(t, set, depth.data(), eb.data(), ec.data(), cb.data(), cc.data(), nullptr,
+                                  nmin.data(), nsize.data(), ids.data(), nullptr);
+  if (st != OTG_OK) return st;
+  return guarded(t->ctx, [&] {
+    otg::ArchiveTable tab{t->dim, t->kind == OTG_KIND_BOX, sp.max_depth, sp.cap, ne, nn, sp.space_min, sp.size,
+                          sp.depth_size, depth.data(), eb.data(), ec.data(), cb.data(), cc.data(), ids.data(),
+                          nmin.data(), nsize.data()};
+    const bool large = (flags & OTG_ARCHIVE_LARGE_DATASET) != 0;
+    otg::ArchiveOut probe{nullptr, large};
+    otg::write_static_archive(probe, tab);
+    *size = probe.n;
+    if (!buf) return;
+    if (buf_size < probe.n) throw Error{OTG_ERR_INVALID_ARG, "archive buffer is smaller than the archive"};
+    otg::ArchiveOut out{buf, large};
+    otg::write_static_archive(out, tab);
   });
 }
 

@@ -182,6 +182,43 @@ int main() {
   auto mdual = StaticQuadtreeBoxM::CollisionDetection(ml, mr);
   std::sort(mdual.begin(), mdual.end());
   EXPECT((mdual == std::vector<std::pair<index_t, index_t>>{{2, 4}, {3, 3}, {3, 4}}));
+  // archives (serialization.h:373-386, :433-439) of a tree small enough to
+  // derive by hand: root + three one-point leaves at depth 2, NodeStorage256
+  STEP("archives");
+  std::vector<Point2D> hp = {{0.5, 0.5}, {3.5, 0.5}, {0.5, 3.5}};
+  const BoundingBox2D hbox{{0.0, 0.0}, {4.0, 4.0}};
+  StaticQuadtreePoint ht(PAR_EXEC, hp, 2, hbox, 1);
+  std::vector<uint8_t> hand;
+  auto put = [&hand](const auto &v) {
+    const auto *b = reinterpret_cast<const uint8_t *>(&v);
+    hand.insert(hand.end(), b, b + sizeof(v));
+  };
+  auto arr = [&](double x, double y) {
+    put(uint32_t(2));
+    put(x);
+    put(y);
+  };
+  put(uint32_t(0));  // core and base SERIALIZED_VERSION_ID
+  put(uint32_t(0));
+  put(uint64_t(1));  // m_maxElementNum
+  put(uint32_t(2));  // m_maxDepthID
+  put(uint32_t(4));  // m_nodeSize
+  arr(4, 4), arr(2, 2), arr(1, 1), arr(0.5, 0.5);
+  put(uint32_t(0));  // variant index: NodeStorage256
+  put(uint32_t(2)), put(uint8_t(1)), put(uint8_t(4));  // nodeChildSegmentBegins
+  put(uint32_t(4));                                    // nodeEntitySegment
+  for (uint8_t b : {0, 0, 0, 1, 1, 1, 2, 1}) put(b);
+  put(uint32_t(4));  // m_nodeDepthIDs
+  for (uint8_t b : {0, 2, 2, 2}) put(b);
+  put(uint32_t(3));  // m_entityStorage
+  for (uint32_t i : {0u, 1u, 2u}) put(i);
+  put(uint32_t(4));  // m_nodeGeometry (min points)
+  arr(0, 0), arr(0, 0), arr(3, 0), arr(0, 3);
+  EXPECT(ht.Serialize() == hand);
+  StaticQuadtreePointM hm(hp, 2, hbox, 1);
+  put(uint32_t(3));  // the managed tree's entity vector
+  for (const Point2D &p : hp) arr(p[0], p[1]);
+  EXPECT(hm.Serialize() == hand);
   if (failures == 0) std::printf("DROPIN OK\n");
   return failures ? 1 : 0;
 }

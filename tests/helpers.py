@@ -9,6 +9,7 @@ core/ot_static_linear_core.h:417-519 without running a BFS.
 from __future__ import annotations
 
 import json
+import struct
 from pathlib import Path
 
 import numpy as np
@@ -405,3 +406,102 @@ def check_node_table_rules(t: dict, n: int, cap: int, D: int, is_box: bool):
                 assert int(t["depth"][c]) > dep
                 pos = cb2
             assert pos == b
+
+
+# ---- hand-derived static-core archives -----------------------------------------
+# Written field by field from serialization.h:359-386, the std::vector /
+# std::array / std::variant serializers (serialization/stl/*.h) and
+# BinaryArchive::operator& (serialization/binary_archive.h:85-175), for two
+# trees small enough to build by hand (node tables checked against the oracle).
+
+def _arr(*xs, cnt="<I"):
+    return struct.pack(cnt, len(xs)) + struct.pack(f"<{len(xs)}d", *xs)
+
+
+def hand_archive_point_2d() -> bytes:
+    """Quadtree of (0.5,0.5), (3.5,0.5), (0.5,3.5) in [0,4]^2, depth 2, cap 1:
+    root (depth 0, entity segment {0,0}, 3 children) and three one-point leaves
+    at depth 2 (path-compressed) with minimum points (0,0), (3,0), (0,3).
+    At most 4 nodes for 3 entities -> NodeStorage256 (u8 begins and segments)."""
+    b = struct.pack("<I", 0) + struct.pack("<I", 0)           # core, base version
+    b += struct.pack("<Q", 1) + struct.pack("<I", 2)          # m_maxElementNum, m_maxDepthID
+    b += struct.pack("<I", 4) + _arr(4.0, 4.0) + _arr(2.0, 2.0) + _arr(1.0, 1.0) + _arr(0.5, 0.5)  # m_nodeSize
+    b += struct.pack("<I", 0)                                 # variant index: NodeStorage256
+    b += struct.pack("<I", 2) + bytes([1, 4])                 # nodeChildSegmentBegins
+    b += struct.pack("<I", 4) + bytes([0, 0, 0, 1, 1, 1, 2, 1])  # nodeEntitySegment {begin, length}
+    b += struct.pack("<I", 4) + bytes([0, 2, 2, 2])           # m_nodeDepthIDs
+    b += struct.pack("<I", 3) + struct.pack("<3I", 0, 1, 2)   # m_entityStorage
+    b += struct.pack("<I", 4) + _arr(0.0, 0.0) + _arr(0.0, 0.0) + _arr(3.0, 0.0) + _arr(0.0, 3.0)  # min points
+    return b
+
+
+def hand_archive_box_2d() -> bytes:
+    """Loose quadtree of boxes [0,1]^2 and [3,4]^2, no box space (bounding box
+    [0,4]^2), depth 2, cap 1: root with MBR (0,0)+(4,4) and two leaves at depth 2
+    whose MBRs are the boxes.  NodeStorage256."""
+    b = struct.pack("<I", 0) + struct.pack("<I", 0)
+    b += struct.pack("<Q", 1) + struct.pack("<I", 2)
+    b += _arr(0.0, 0.0) + _arr(4.0, 4.0)                      # m_nominalTreeBox {minPoint, size}
+    b += struct.pack("<I", 0)
+    b += struct.pack("<I", 2) + bytes([1, 3])
+    b += struct.pack("<I", 3) + bytes([0, 0, 0, 1, 1, 1])
+    b += struct.pack("<I", 3) + bytes([0, 2, 2])
+    b += struct.pack("<I", 2) + struct.pack("<2I", 0, 1)
+    b += struct.pack("<I", 3) + _arr(0.0, 0.0) + _arr(4.0, 4.0) + _arr(0.0, 0.0) + _arr(1.0, 1.0) \
+        + _arr(3.0, 3.0) + _arr(1.0, 1.0)                     # MBRs {minPoint, size}
+    return b
+
+
+HAND_POINTS_2D = [[0.5, 0.5], [3.5, 0.5], [0.5, 3.5]]
+HAND_BOXES_2D = [[0.0, 0.0, 1.0, 1.0], [3.0, 3.0, 4.0, 4.0]]
+
+
+def archive_children(begins, nn):
+    """GetNodeChildren on an archived nodeChildSegmentBegins (ot_static_linear_core.h:191-202)."""
+    out = []
+    for i in range(nn):
+        if i >= len(begins) - 1:
+            out.append((0, 0))
+        else:
+            out.append((int(begins[i]), int(begins[i + 1])))
+    return out
+
+
+def check_archive_vs_table(a: dict, t: dict, is_box: bool, cap: int, space_min, space_max):
+    """Archive fields against an oracle node table: header, children through the
+    reference's GetNodeChildren rule, segments, depths, geometry bit for bit,
+    and the entity storage as equal multisets per run of equal location keys."""
+    D = t["min"].shape[1]
+    nn = len(t["depth"])
+    assert a["max_elements"] == cap and a["max_depth"] == t["max_depth"]
+    size = np.asarray(space_max, float) - np.asarray(space_min, float)
+    if is_box:
+        assert np.array_equal(a["tree_box"][0], np.asarray(space_min, float))
+        assert np.array_equal(a["tree_box"][1], size)
+        assert np.array_equal(a["node_size_mbr"].view(np.uint64), t["size"].view(np.uint64))
+    else:
+        M = t["max_depth"]
+        expect = np.stack([size * 0.5 ** k for k in range(M + 2)])
+        assert np.array_equal(a["node_size"], expect)
+    assert np.array_equal(a["depth"], t["depth"])
+    assert np.array_equal(a["entity_begin"], t["ebegin"].astype(np.uint64))
+    assert np.array_equal(a["entity_count"], t["ecount"].astype(np.uint64))
+    assert np.array_equal(a["node_min"].view(np.uint64), t["min"].view(np.uint64))
+    kids = archive_children(a["child_begins"], nn)
+    for i in range(nn):
+        lo, hi = kids[i]
+        assert hi - lo == t["ccount"][i], ("children", i)
+        if t["ccount"][i]:
+            assert lo == t["cbegin"][i], ("first child", i)
+    gi, oi = a["entity_ids"], t["ids"]
+    assert len(gi) == len(oi)
+    if is_box:
+        db = max(1, int(t["max_depth"]).bit_length())
+        keys = (t["sorted_code"].astype(np.uint64) << np.uint64(db)) | t["sorted_depth"].astype(np.uint64)
+    else:
+        keys = t["sorted_code"]
+    n = len(keys)
+    if n:
+        starts = np.concatenate([[0], np.nonzero(keys[1:] != keys[:-1])[0] + 1, [n]])
+        for s, e in zip(starts[:-1], starts[1:]):
+            assert sorted(gi[s:e]) == sorted(oi[s:e]), ("entity run", s, e)
