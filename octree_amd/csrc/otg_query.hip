@@ -2262,14 +2262,15 @@ struct ViewSet {
 
 static uint32_t bit_width32(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
 
+constexpr uint64_t kOrderMinQueries = 16384;
+
 template <int D>
 static Buf<uint32_t> query_order(Ctx *ctx, const ViewSet &vs, const double *q, uint32_t stride, const uint32_t *qs,
                                  uint64_t nq) {
+  // below this many queries the ordering (key pass + radix sort, about 20 launches)
+  // costs more than the locality it buys: the queries run in input order
+  if (nq < kOrderMinQueries) return Buf<uint32_t>();
   Buf<uint32_t> idx(ctx, nq);
-  if (nq < 2) {
-    if (nq) OTG_HIP_CHECK(hipMemsetAsync(idx.p, 0, 4, ctx->stream));
-    return idx;
-  }
   // the leading 30 bits of the Morton code (10 levels in 3-D) give the wavefronts
   // their locality; the order decides speed only, never results
   const uint32_t drop = vs.cbits > 30 ? vs.cbits - 30 : 0;
