@@ -603,6 +603,110 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
 }
 
 // views/qview: the trees of this launch and each query's tree (qview null: one tree);
+// Levels of a 4-D..6-D walk: one per internal node on the path, and internal nodes
+// sit at depths below the tree depth, which is at most 63 / D.
+template <int D>
+struct MaskLevels {
+  static constexpr int value = (D >= 4 && D <= 6) ? 63 / D : 1;
+};
+
+// kNN walk for 4-D..6-D with one LDS entry per level: the first child and a bit
+// mask of the children within the radius when their parent was visited (2^D <= 64).
+// The nearest of them is visited at once; the others are taken in index order, and
+// each is tested against the radius of that moment, as the reference's queue pop
+// does.  Same visits rule and acceptance rule as knn_query (only the order in
+// which siblings are met differs); no per-lane stack in scratch memory.
+template <int D, int KCAP, bool BOX>
+__device__ __forceinline__ bool knn_query_masks(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
+                                                const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
+                                                uint32_t *s_cb, uint32_t *s_lo, uint32_t *s_hi,
+                                                uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                                uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
+                                                double *__restrict__ out_F) {
+  static_assert(D >= 4 && D <= 6, "child masks of 2^D bits need D <= 6");
+  constexpr int cap = MaskLevels<D>::value;
+  const uint32_t s = qset ? qset[qi] : 0u;
+  const SetParams &sp = tv.sets[s];
+  double q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+  double bd[KCAP];
+  uint32_t bid[KCAP];
+  KnnState<KCAP> L;
+  knn_init(L, F0);
+  double pes[KCAP];
+  uint32_t npes = 0;
+  if (sp.node_cnt != 0 && k != 0) {
+    int top = 0;
+    uint32_t node = (uint32_t)sp.node_off;
+    bool have;
+    {
+      bool in;
+      const double w = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+      have = !knn_node_out(L, w, false);
+    }
+    for (;;) {
+      if (have) {
+        const NodeInfo ni = tv.info[node];
+        if constexpr (BOX)
+          knn_scan_box<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, pes, npes, L, k, tol);
+        else
+          knn_scan<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, L, k, tol);
+        if (ni.ccount) {
+          uint64_t mask = 0;
+          double best = __builtin_inf();
+          uint32_t bi = 0;
+          for (uint32_t c = 0; c < ni.ccount; ++c) {
+            const uint32_t ch = ni.cbegin + c;
+            bool cin;
+            const double wc = wall_distance2<D>(q, &tv.nmin[(uint64_t)ch * D], &tv.nsize[(uint64_t)ch * D], &cin);
+            if (knn_node_out(L, wc, false)) continue;
+            mask |= 1ull << c;
+            if (wc < best) {
+              best = wc;
+              bi = c;
+            }
+          }
+          if (mask) {
+            mask &= ~(1ull << bi);
+            if (mask) {
+              if (top == cap) return false;
+              s_cb[top * kQB] = ni.cbegin;
+              s_lo[top * kQB] = (uint32_t)mask;
+              s_hi[top * kQB] = (uint32_t)(mask >> 32);
+              ++top;
+            }
+            node = ni.cbegin + bi;  // within the radius an instant ago: visited next
+            continue;
+          }
+        }
+      }
+      if (top == 0) break;
+      // next pending child of the innermost level, re-tested against the radius now
+      uint64_t m = ((uint64_t)s_hi[(top - 1) * kQB] << 32) | s_lo[(top - 1) * kQB];
+      node = s_cb[(top - 1) * kQB] + (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      if (m) {
+        s_lo[(top - 1) * kQB] = (uint32_t)m;
+        s_hi[(top - 1) * kQB] = (uint32_t)(m >> 32);
+      } else {
+        --top;
+      }
+      bool in;
+      const double w = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+      have = !knn_node_out(L, w, false);
+    }
+    // the band below F holds more ids than the list: the query is redone by k_knn_band
+    if (L.any_dropped && L.dropped <= L.F) band[atomicAdd(overflow, 1u)] = (uint32_t)qi;
+  }
+  out_cnt[qi] = L.cnt;
+  if (out_F) out_F[qi] = L.F;
+#pragma unroll
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
+  return true;
+}
+
 // qk: per-query k (null: k for all).
 template <int D, int KCAP, bool BOX>
 __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
@@ -612,12 +716,14 @@ __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ vie
                                               uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
                                               uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                                               double *__restrict__ out_F) {
-  // 2-D/3-D: level entries in LDS, one per tree level.  4-D..6-D: up to 2^D - 1
-  // pending siblings per level outgrow any short stack, so the walk keeps the
-  // worst-case private stack.  Neither can overflow.
+  // 2-D/3-D: level entries in LDS, one per tree level.  4-D..6-D: one child mask
+  // per level in LDS.  7-D and up: the worst-case private stack.  None can overflow.
   static_assert(kLdsStack >= kMaxDepth, "one level entry per internal depth");
-  __shared__ uint32_t s_n[(D <= 3 ? kLdsStack : 1) * kQB];
-  __shared__ uint32_t s_w[(D <= 3 ? kLdsStack : 1) * kQB];
+  // 2-D/3-D: level stack (first child, packed order); 4-D..6-D: first child and the
+  // two halves of the child mask, MaskLevels entries each
+  constexpr int ML = MaskLevels<D>::value;
+  __shared__ uint32_t s_n[(D <= 3 ? kLdsStack : ML) * kQB];
+  __shared__ uint32_t s_w[(D <= 3 ? kLdsStack : 2 * ML) * kQB];
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nq) return;
   const uint64_t qi = order ? order[ti] : ti;
@@ -626,6 +732,9 @@ __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ vie
     if constexpr (D <= 3) {
       LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
       knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
+    } else if constexpr (D <= 6) {
+      knn_query_masks<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, &s_n[threadIdx.x], &s_w[threadIdx.x],
+                                    &s_w[ML * kQB + threadIdx.x], out_ids, out_cnt, overflow, band, out_F);
     } else {
       ArrayStack<D> S;
       knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);

@@ -37,6 +37,21 @@ def test_knn_vs_oracle(gpu_ctx, D, n, k, cap, M):
     assert np.array_equal(gids, oids)
 
 
+@pytest.mark.parametrize("D,k", [(4, 8), (5, 16), (6, 16), (6, 1)])
+def test_knn_wide_dims_clustered(gpu_ctx, D, k):
+    """4-D..6-D walks (one child mask per level): clustered points under a shallow
+    depth, so leaves overflow and the finer kNN index is walked, equal the oracle."""
+    pts = gaussian_mixture(20000, D, 8, 0.0, 1.0, 0.01, 0.05, seed=80 + D)
+    rng = np.random.default_rng(90 + D)
+    qs = pts[rng.integers(0, len(pts), 2000)] + rng.normal(0.0, 0.01, (2000, D))
+    g = _tree(gpu_ctx, "point", D, pts, None, 3, 8)
+    o = O.OracleTree(D, "point", pts, None, 3, 8)
+    goff, gids = g.knn_batch(qs, k).csr()
+    ooff, oids = o.knn(qs, k, threads=8)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(gids, oids)
+
+
 def test_knn_max_within_and_tolerance(gpu_ctx):
     pts = uniform_points(5000, 3, seed=1)
     qs = uniform_points(1000, 3, seed=2)
