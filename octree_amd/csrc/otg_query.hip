@@ -1519,18 +1519,19 @@ __device__ __forceinline__ bool planes_entity(const TreeView &tv, uint32_t j, co
 // walk of planes_query, bounded by the tree depth).  Every node a walk reaches has
 // all its ancestors visited, so the set of ids is planes_query's; ids are appended
 // at the query's CSR offset through an LDS counter, in no particular order.
-template <int D, bool BOX, int MODE, bool FILL>
-__global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+template <int D, bool BOX, int MODE, bool FILL, int W = 64>
+__global__ void __launch_bounds__(W) k_planes_wave(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                                      const double *__restrict__ planes, uint32_t nplanes,
                                                      const uint32_t *__restrict__ qset, uint64_t nq, double tol,
                                                      uint32_t *__restrict__ counts, const uint64_t *__restrict__ offsets,
                                                      uint32_t *__restrict__ out, uint32_t *__restrict__ ncopy,
                                                      RangeCopy *__restrict__ copies) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
-  __shared__ uint32_t s_front[2][kWaveFrontier];
+  // the frontier grows while narrower than the block: fewer than W nodes of <= 8 children
+  __shared__ uint32_t s_front[2][W * 8 + 64];
   __shared__ uint32_t s_nf[2], s_cnt, s_next;
-  __shared__ uint32_t s_a[kRangeStack * 64];
-  __shared__ uint8_t s_b[kRangeStack * 64];
+  __shared__ uint32_t s_a[kRangeStack * W];
+  __shared__ uint8_t s_b[kRangeStack * W];
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   const TreeView &tv = views[qview ? qview[qi] : 0u];
@@ -1559,9 +1560,9 @@ __global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__
     }
   }
   __syncthreads();
-  // phase 1: breadth-first expansion while the frontier is narrower than the wavefront
+  // phase 1: breadth-first expansion while the frontier is narrower than the block
   int cur = 0;
-  while (s_nf[cur] != 0 && s_nf[cur] < 64) {
+  while (s_nf[cur] != 0 && s_nf[cur] < W) {
     const uint32_t nf = s_nf[cur];
     if (lane == 0) s_nf[cur ^ 1] = 0;
     __syncthreads();
@@ -1585,7 +1586,7 @@ __global__ void __launch_bounds__(64) k_planes_wave(const TreeView *__restrict__
   }
   // phase 2: the lanes walk the frontier's subtrees
   const uint32_t nf = s_nf[cur];
-  LdsLevelIter stack{&s_a[lane], &s_b[lane], 64};
+  LdsLevelIter stack{&s_a[lane], &s_b[lane], W};
   for (;;) {
     const uint32_t item = atomicAdd(&s_next, 1u);
     if (item >= nf) break;
@@ -2793,12 +2794,15 @@ static Result *count_fill_csr(Ctx *ctx, uint64_t nq, Count count, Fill fill) {
 }
 
 constexpr uint64_t kPlaneWaveQueries = 1u << 16;
+constexpr uint64_t kPlaneWideQueries = 4096;
 
 template <int D, bool BOX, int MODE>
 static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, uint32_t nplanes,
                            const uint32_t *qs, double tol) {
   // few queries, each a slab of the tree: one wavefront per query (2-D / 3-D)
   const bool wave = D <= 3 && nq < kPlaneWaveQueries;
+  // up to a few thousand queries: four wavefronts per query so the chip fills
+  const bool wide = nq <= kPlaneWideQueries;
   Buf<uint32_t> ncopy(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
   Buf<RangeCopy> copies;
@@ -2808,9 +2812,14 @@ static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doubl
       [&](uint32_t *cnt) {
         if constexpr (D <= 3) {
           if (wave) {
-            launch(ctx, "planes_count", k_planes_wave<D, BOX, MODE, false>, dim3((uint32_t)nq), dim3(64), 0,
-                   vs.views, vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr,
-                   ncopy.p, (RangeCopy *)nullptr);
+            if (wide)
+              launch(ctx, "planes_count", k_planes_wave<D, BOX, MODE, false, 256>, dim3((uint32_t)nq), dim3(256), 0,
+                     vs.views, vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr,
+                     ncopy.p, (RangeCopy *)nullptr);
+            else
+              launch(ctx, "planes_count", k_planes_wave<D, BOX, MODE, false>, dim3((uint32_t)nq), dim3(64), 0,
+                     vs.views, vs.qview, q, nplanes, qs, nq, tol, cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr,
+                     ncopy.p, (RangeCopy *)nullptr);
             OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
             return;
           }
@@ -2825,9 +2834,14 @@ static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doubl
         OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
         if constexpr (D <= 3) {
           if (wave) {
-            launch(ctx, "planes_fill", k_planes_wave<D, BOX, MODE, true>, dim3((uint32_t)nq), dim3(64), 0, vs.views,
-                   vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets,
-                   res->d_ids, ncopy.p, copies.p);
+            if (wide)
+              launch(ctx, "planes_fill", k_planes_wave<D, BOX, MODE, true, 256>, dim3((uint32_t)nq), dim3(256), 0,
+                     vs.views, vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets,
+                     res->d_ids, ncopy.p, copies.p);
+            else
+              launch(ctx, "planes_fill", k_planes_wave<D, BOX, MODE, true>, dim3((uint32_t)nq), dim3(64), 0, vs.views,
+                     vs.qview, q, nplanes, qs, nq, tol, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets,
+                     res->d_ids, ncopy.p, copies.p);
             return;
           }
         }

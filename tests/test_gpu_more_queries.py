@@ -215,3 +215,22 @@ def test_planes_many_queries_one_lane_walk(gpu_ctx):
         assert np.array_equal(goff, ooff)
         assert np.array_equal(_sorted_segments(goff, gids), _sorted_segments(ooff, oids))
         assert goff[-1] > 0
+
+
+@pytest.mark.parametrize("kind", ["point", "box"])
+def test_planes_mid_batch_wavefront_walk(gpu_ctx, kind):
+    """Between 4,097 and 65,535 plane queries one wavefront walks each query (up to
+    4,096 take four wavefronts each, 65,536 and more one lane each): the oracle's sets."""
+    D, n = 3, 6000
+    geom = uniform_points(n, D, 0.0, 100.0, seed=130) if kind == "point" else \
+        lognormal_boxes(n, D, 0.0, 100.0, np.log(1.0), 0.6, seed=130)
+    g = _tree(gpu_ctx, kind, D, geom, None, 5, 8)
+    o = O.OracleTree(D, kind, geom, None, 5, 8)
+    pl = _random_planes(6000, D, 0.0, 100.0, seed=131)
+    fr = _random_planes(4 * 5000, D, 30.0, 70.0, seed=132).reshape(5000, 4, D + 1)
+    for qs, mode in ((pl, 0), (pl, 1), (fr, 2)):
+        goff, gids = g.planes_batch(qs, mode, 0.25).csr()
+        ooff, oids = o.planes(qs, mode, 0.25, threads=8)
+        assert np.array_equal(goff, ooff)
+        assert np.array_equal(_sorted_segments(goff, gids), _sorted_segments(ooff, oids))
+        assert goff[-1] > 0
