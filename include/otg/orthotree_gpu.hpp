@@ -568,6 +568,8 @@ class OrthoTreeManaged {
       if (largeDataset) {
         put(&c, 8);
       } else {
+        // serialized_size_t is uint32 without ORTHOTREE__LARGE_DATASET
+        if (c > 0xFFFFFFFFull) detail::fail(OTG_ERR_UNSUPPORTED, "OrthoTreeManaged::Serialize", nullptr);
         const uint32_t c32 = static_cast<uint32_t>(c);
         put(&c32, 4);
       }
