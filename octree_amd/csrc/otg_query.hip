@@ -20,7 +20,7 @@
 
 namespace otg {
 
-constexpr int kQB = 128;  // lanes per block for traversals (2 wavefronts)
+constexpr int kQB = 64;  // lanes per block for traversals (1 wavefront)
 
 
 // Context whose stream and allocator run a query: the calling thread's
@@ -936,7 +936,7 @@ __device__ __forceinline__ bool boxes_overlap_strict(const double *mn1, const do
 // Level stack in LDS for the range walk: per level with pending children, the
 // next child and the end of the node's children.  A path holds at most one entry
 // per tree depth, so the short stack never overflows.
-constexpr int kRangeStack = kMaxDepth + 1;  // 20 x 5 B x 128 lanes = 12.5 KiB per block
+constexpr int kRangeStack = kMaxDepth + 1;  // 20 x 5 B x 64 lanes = 6.25 KiB per block
 
 struct LdsLevelIter {
   uint32_t *a;  // next pending child (children are taken last to first)
