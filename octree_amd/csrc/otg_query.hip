@@ -1868,7 +1868,7 @@ __device__ __forceinline__ void ray_output(uint64_t qi, uint32_t cnt, double (&c
 // sifts up while smaller than the parent, pop moves the last entry to the root and
 // sifts it down towards the smaller child, so equal enter distances leave in a
 // fixed order.  LdsHeap: lane-interleaved in LDS; ArrayHeap: private.
-constexpr int kRayHeap = 16;  // 16 x 12 B x 128 lanes = 24 KiB per block
+constexpr int kRayHeap = 16;  // 16 x 12 B x 64 lanes = 12 KiB per block
 
 struct LdsHeap {
   static constexpr int cap = kRayHeap;
