@@ -1035,10 +1035,6 @@ __device__ __forceinline__ void range_node_test(const TreeView &tv, uint32_t nod
 // count of a query whose one-lane count walk stopped at the visit cap
 constexpr uint32_t kRangeCapped = 0xFFFFFFFFu;
 
-// Frontier of the wavefront walks (range and plane queries): nodes are expanded
-// breadth-first only while fewer than 64 are pending, so at most 63 * 2^D new ones.
-constexpr int kWaveFrontier = 64 * 8 + 64;
-
 template <int D, bool BOX, bool FILL>
 __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, const double *__restrict__ ranges,
                                             const uint32_t *__restrict__ qset, int overlap_mode, double tol,
@@ -1156,7 +1152,8 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
                                                     const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
                                                     uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
-  __shared__ uint32_t s_front[2][kWaveFrontier];
+  // one frontier, rewritten in place each round: fewer than 64 nodes of <= 8 children
+  __shared__ uint32_t s_front[64 * 8];
   __shared__ uint32_t s_nf[2], s_cnt, s_next;
   __shared__ uint32_t s_a[kRangeStack * 64];
   __shared__ uint8_t s_b[kRangeStack * 64];
@@ -1215,7 +1212,7 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     s_nf[1] = 0;
     if (start == 1) emit_run(nullptr, (uint32_t)sp.ent_cnt);
     if (start == 2) {
-      s_front[0][0] = (uint32_t)sp.node_off;
+      s_front[0] = (uint32_t)sp.node_off;
       s_nf[0] = 1;
     }
   }
@@ -1223,13 +1220,14 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
   int cur = 0;
   while (s_nf[cur] != 0 && s_nf[cur] < 64) {
     const uint32_t nf = s_nf[cur];
+    const uint32_t mine = lane < nf ? s_front[lane] : 0u;  // read before the buffer is rewritten
     if (lane == 0) s_nf[cur ^ 1] = 0;
     __syncthreads();
     if (lane < nf) {
       NodeInfo ni;
-      if (visit(s_front[cur][lane], ni)) {
+      if (visit(mine, ni)) {
         const uint32_t at = atomicAdd(&s_nf[cur ^ 1], ni.ccount);
-        for (uint32_t c = 0; c < ni.ccount; ++c) s_front[cur ^ 1][at + c] = ni.cbegin + c;
+        for (uint32_t c = 0; c < ni.ccount; ++c) s_front[at + c] = ni.cbegin + c;
       }
     }
     __syncthreads();
@@ -1241,7 +1239,7 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     const uint32_t item = atomicAdd(&s_next, 1u);
     if (item >= nf) break;
     int top = 0;
-    uint32_t node = s_front[cur][item];
+    uint32_t node = s_front[item];
     for (;;) {
       NodeInfo ni;
       if (visit(node, ni)) {
@@ -1527,8 +1525,9 @@ __global__ void __launch_bounds__(W) k_planes_wave(const TreeView *__restrict__ 
                                                      uint32_t *__restrict__ out, uint32_t *__restrict__ ncopy,
                                                      RangeCopy *__restrict__ copies) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
-  // the frontier grows while narrower than the block: fewer than W nodes of <= 8 children
-  __shared__ uint32_t s_front[2][W * 8 + 64];
+  // one frontier, rewritten in place each round; it grows while narrower than the
+  // block: fewer than W nodes of <= 8 children
+  __shared__ uint32_t s_front[W * 8];
   __shared__ uint32_t s_nf[2], s_cnt, s_next;
   __shared__ uint32_t s_a[kRangeStack * W];
   __shared__ uint8_t s_b[kRangeStack * W];
@@ -1555,7 +1554,7 @@ __global__ void __launch_bounds__(W) k_planes_wave(const TreeView *__restrict__ 
     s_nf[0] = 0;
     s_nf[1] = 0;
     if (sp.node_cnt != 0 && nplanes != 0) {
-      s_front[0][0] = (uint32_t)sp.node_off;
+      s_front[0] = (uint32_t)sp.node_off;
       s_nf[0] = 1;
     }
   }
@@ -1564,10 +1563,11 @@ __global__ void __launch_bounds__(W) k_planes_wave(const TreeView *__restrict__ 
   int cur = 0;
   while (s_nf[cur] != 0 && s_nf[cur] < W) {
     const uint32_t nf = s_nf[cur];
+    const uint32_t mine = lane < nf ? s_front[lane] : 0u;  // read before the buffer is rewritten
     if (lane == 0) s_nf[cur ^ 1] = 0;
     __syncthreads();
     if (lane < nf) {
-      const uint32_t node = s_front[cur][lane];
+      const uint32_t node = mine;
       NodeInfo ni;
       bool whole;
       if (planes_node<D, BOX, MODE>(tv, node, pq, nplanes, tol, ni, whole)) {
@@ -1576,7 +1576,7 @@ __global__ void __launch_bounds__(W) k_planes_wave(const TreeView *__restrict__ 
           entities(ni);
           if (ni.ccount) {
             const uint32_t at = atomicAdd(&s_nf[cur ^ 1], ni.ccount);
-            for (uint32_t c = 0; c < ni.ccount; ++c) s_front[cur ^ 1][at + c] = ni.cbegin + c;
+            for (uint32_t c = 0; c < ni.ccount; ++c) s_front[at + c] = ni.cbegin + c;
           }
         }
       }
@@ -1591,7 +1591,7 @@ __global__ void __launch_bounds__(W) k_planes_wave(const TreeView *__restrict__ 
     const uint32_t item = atomicAdd(&s_next, 1u);
     if (item >= nf) break;
     int top = 0;
-    uint32_t node = s_front[cur][item];
+    uint32_t node = s_front[item];
     for (;;) {
       NodeInfo ni;
       bool whole;
