@@ -89,6 +89,53 @@ def make_workload(nsets=256, nq=50_000, seed=1234, lo=3.0, hi=6.0, size_scale=1.
     }
 
 
+def make_point_1m():
+    """OctreePoint: 1,000,000 points U[0,100)^3 and 10,000 kNN query points in the same box."""
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0.0, 100.0, (1_000_000, 3))
+    qs = rng.uniform(0.0, 100.0, (10_000, 3))
+    return pts, qs, [0, 0, 0, 100, 100, 100]
+
+
+def make_box_4m():
+    """OctreeBox: 4,000,000 boxes, centres U[0,1000)^3, half-extents lognormal(ln 0.5, 0.6) per
+    axis; 100,000 rays with origins U[0,1000)^3 and unit directions.  The generator is returned
+    too: the other box-tree queries draw from it next."""
+    rng = np.random.default_rng(12)
+    n = 4_000_000
+    c = rng.uniform(0.0, 1000.0, (n, 3))
+    h = np.exp(rng.normal(np.log(0.5), 0.6, (n, 3)))
+    boxes = np.concatenate([c - h, c + h], axis=1)
+    o_ = rng.uniform(0.0, 1000.0, (100_000, 3))
+    v = rng.normal(0.0, 1.0, (100_000, 3))
+    rays = np.concatenate([o_, v / np.sqrt((v * v).sum(axis=1))[:, None]], axis=1)
+    return boxes, rays, rng
+
+
+def make_point6d_8m(n=8_000_000, nq=1_000_000):
+    """TreePointND<6>: n points of a 32-cluster Gaussian mixture in [0,1]^6 (sigma 0.01-0.05,
+    redrawn until inside) and nq kNN query points from the same mixture."""
+    rng = np.random.default_rng(13)
+    k = 32
+    centers = rng.uniform(0.0, 1.0, (k, 6))
+    sig = rng.uniform(0.01, 0.05, k)
+
+    def draw(m):
+        lab = rng.integers(0, k, m)
+        p = centers[lab] + rng.normal(0.0, 1.0, (m, 6)) * sig[lab, None]
+        bad = ((p < 0.0) | (p > 1.0)).any(axis=1)
+        while bad.any():
+            kk = int(bad.sum())
+            lab2 = rng.integers(0, k, kk)
+            p[bad] = centers[lab2] + rng.normal(0.0, 1.0, (kk, 6)) * sig[lab2, None]
+            bad = ((p < 0.0) | (p > 1.0)).any(axis=1)
+        return p
+
+    pts = draw(n)
+    qs = draw(nq)
+    return pts, qs, [0.0] * 6 + [1.0] * 6
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,10 +420,7 @@ def run_single(args, octree_amd, torch):
     out = {"metric": METRIC, "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)"}
     if args.workload == "octree_point_1m":
-        rng = np.random.default_rng(11)
-        pts = rng.uniform(0.0, 100.0, (1_000_000, 3))
-        qs = rng.uniform(0.0, 100.0, (10_000, 3))
-        box = [0, 0, 0, 100, 100, 100]
+        pts, qs, box = make_point_1m()
         d_pts, d_q = torch.from_numpy(pts).to(dev), torch.from_numpy(qs).to(dev)
         torch.cuda.synchronize()
         tree = octree_amd.StaticOctreePoint(ctx=ctx)
@@ -396,14 +440,8 @@ def run_single(args, octree_amd, torch):
             out["cpu_baseline"] = {"value": len(qs) / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
                                    "sample": "whole workload", "build_mpts_s": 1.0 / cb, "build_cores": 1}
     elif args.workload == "octree_box_4m":
-        rng = np.random.default_rng(12)
-        n = 4_000_000
-        c = rng.uniform(0.0, 1000.0, (n, 3))
-        h = np.exp(rng.normal(np.log(0.5), 0.6, (n, 3)))
-        boxes = np.concatenate([c - h, c + h], axis=1)
-        o_ = rng.uniform(0.0, 1000.0, (100_000, 3))
-        v = rng.normal(0.0, 1.0, (100_000, 3))
-        rays = np.concatenate([o_, v / np.sqrt((v * v).sum(axis=1))[:, None]], axis=1)
+        boxes, rays, rng = make_box_4m()
+        n = boxes.shape[0]
         d_b, d_r = torch.from_numpy(boxes).to(dev), torch.from_numpy(rays).to(dev)
         torch.cuda.synchronize()
         tree = octree_amd.StaticOctreeBox(ctx=ctx)
@@ -493,25 +531,7 @@ def run_single(args, octree_amd, torch):
                 "frustum_qps": len(fr) / cfr, "two_tree_collision_boxes_s_single_thread": m / cdu,
                 "sample": f"same tree of the first {m} boxes; two-tree collision of {m // 2} + {m // 2} boxes"}
     else:  # point6d_8m
-        rng = np.random.default_rng(13)
-        k = 32
-        centers = rng.uniform(0.0, 1.0, (k, 6))
-        sig = rng.uniform(0.01, 0.05, k)
-
-        def draw(m):
-            lab = rng.integers(0, k, m)
-            p = centers[lab] + rng.normal(0.0, 1.0, (m, 6)) * sig[lab, None]
-            bad = ((p < 0.0) | (p > 1.0)).any(axis=1)
-            while bad.any():
-                kk = int(bad.sum())
-                lab2 = rng.integers(0, k, kk)
-                p[bad] = centers[lab2] + rng.normal(0.0, 1.0, (kk, 6)) * sig[lab2, None]
-                bad = ((p < 0.0) | (p > 1.0)).any(axis=1)
-            return p
-
-        pts = draw(8_000_000)
-        qs = draw(1_000_000)
-        box = [0.0] * 6 + [1.0] * 6
+        pts, qs, box = make_point6d_8m()
         d_p, d_q = torch.from_numpy(pts).to(dev), torch.from_numpy(qs).to(dev)
         torch.cuda.synchronize()
         T = octree_amd.StaticOrthoTreePointND(6)

@@ -154,7 +154,9 @@ OTG_API otg_status otg_tree_serialize(const otg_tree *tree, uint32_t set, uint32
 /* ---- queries ------------------------------------------------------------ */
 /* Results are CSR: offsets[nq+1] (uint64) and ids (uint32, local entity ids).
  * set_ids: per-query set of a forest (NULL = set 0); same memory space as the
- * queries. */
+ * queries.  Every set id must be below the tree's set count (OTG_ERR_INVALID_ARG
+ * otherwise, checked before any query runs).  A batch whose result would hold
+ * 2^32 - 1 or more ids is rejected with OTG_ERR_UNSUPPORTED (split the batch). */
 
 /* OrthoTreeQueryBase::GetNearestNeighbors(point, k, maxDistanceWithin, entities,
  * tolerance) (core/ot_query.h:1208-1270).  max_within = DBL_MAX for the

@@ -143,7 +143,10 @@ inline void launch(Ctx *ctx, const char *name, Kernel k, dim3 grid, dim3 block, 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // ---- primitives (otg_primitives.hip) ----
-// Exclusive prefix sum of n uint32 values into out (may alias in); returns total on host if total != nullptr.
+// Exclusive prefix sum of n uint32 values into out (may alias in), accumulated in 64 bits.
+// *d_total (device, if not null) receives the total, or kScanOverflow when it does not
+// fit 32 bits (out is then not meaningful).
+constexpr uint32_t kScanOverflow = 0xFFFFFFFFu;
 void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total);
 // Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
 // result is in keys/vals on return.

@@ -15,11 +15,13 @@ constexpr int kTile = kBlock * kItems;
 // ---------------------------------------------------------------------------
 // Generic block scan
 // ---------------------------------------------------------------------------
-struct OpAdd {
-  using T = uint32_t;
-  __device__ static T identity() { return 0u; }
+// Count scans accumulate in 64 bits, so a total beyond 2^32 - 1 is seen (and
+// reported saturated) instead of wrapping.
+struct OpAdd64 {
+  using T = uint64_t;
+  __device__ static T identity() { return 0ull; }
   __device__ static T combine(T a, T b) { return a + b; }
-  __device__ static T shfl_up(T x, int off) { return __shfl_up(x, off, 64); }
+  __device__ static T shfl_up(T x, int off) { return __shfl_up((unsigned long long)x, off, 64); }
 };
 
 // State of the "running minima" monoid over boundary levels.  For a segment,
@@ -83,7 +85,7 @@ __device__ typename Op::T block_scan(typename Op::T x, typename Op::T *excl, typ
 // ---- loaders / storers ----
 struct U32Source {
   const uint32_t *p;
-  __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
+  __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
 };
 struct LevelSource {
   const uint8_t *v;
@@ -137,28 +139,30 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(typename Op::T *partia
   }
 }
 
-// Final pass: exclusive u32 output.
+// Final pass: exclusive u32 output (exact while the total fits 32 bits); the total
+// is written saturated at 0xFFFFFFFF, which callers treat as overflow.
 __global__ void __launch_bounds__(kBlock) k_scan_final_u32(const uint32_t *in, uint32_t *out, uint64_t n,
-                                                            const uint32_t *partials, uint32_t *d_total,
+                                                            const uint64_t *partials, uint32_t *d_total,
                                                             uint32_t nblocks) {
-  __shared__ uint32_t lds[kBlock / 64];
+  __shared__ uint64_t lds[kBlock / 64];
   uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   uint32_t items[kItems];
-  uint32_t acc = 0;
+  uint64_t acc = 0;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     items[i] = (base + i < n) ? in[base + i] : 0u;
     acc += items[i];
   }
-  uint32_t ex, tot;
-  block_scan<OpAdd>(acc, &ex, &tot, lds);
-  uint32_t run = partials[blockIdx.x] + ex;
+  uint64_t ex, tot;
+  block_scan<OpAdd64>(acc, &ex, &tot, lds);
+  uint64_t run = partials[blockIdx.x] + ex;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    if (base + i < n) out[base + i] = run;
+    if (base + i < n) out[base + i] = (uint32_t)run;
     run += items[i];
   }
-  if (d_total && blockIdx.x == nblocks - 1 && threadIdx.x == kBlock - 1) *d_total = run;
+  if (d_total && blockIdx.x == nblocks - 1 && threadIdx.x == kBlock - 1)
+    *d_total = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_final_level(LevelSource src, uint64_t n,
@@ -189,11 +193,11 @@ void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,
     return;
   }
   uint32_t nb = div_up(n, kTile);
-  Buf<uint32_t> partials(ctx, nb);
-  launch(ctx, "scan_reduce_u32", k_scan_reduce<OpAdd, U32Source>, dim3(nb), dim3(kBlock), 0, U32Source{in}, n,
+  Buf<uint64_t> partials(ctx, nb);
+  launch(ctx, "scan_reduce_u32", k_scan_reduce<OpAdd64, U32Source>, dim3(nb), dim3(kBlock), 0, U32Source{in}, n,
          partials.p);
-  launch(ctx, "scan_partials_u32", k_scan_partials<OpAdd>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
-  launch(ctx, "scan_final_u32", k_scan_final_u32, dim3(nb), dim3(kBlock), 0, in, out, n, (const uint32_t *)partials.p,
+  launch(ctx, "scan_partials_u32", k_scan_partials<OpAdd64>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
+  launch(ctx, "scan_final_u32", k_scan_final_u32, dim3(nb), dim3(kBlock), 0, in, out, n, (const uint64_t *)partials.p,
          d_total, nb);
 }
 

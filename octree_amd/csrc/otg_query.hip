@@ -1511,8 +1511,9 @@ __device__ __forceinline__ bool planes_entity(const TreeView &tv, uint32_t j, co
 // tree, so a query is thousands of node visits and a launch holds few queries.
 // Phase 1: the wavefront expands the top of the tree breadth-first (one lane per
 // frontier node, the node test and entity pass as in planes_query) until the
-// frontier holds at least 64 nodes or nothing is left to expand; the frontier
-// never exceeds 64 * 2^D + 64 entries.  Phase 2: each lane takes frontier nodes
+// frontier holds at least W nodes or nothing is left to expand; a round starts
+// from fewer than W nodes of at most 2^D children, so the frontier holds at most
+// (W - 1) * 2^D entries and fits s_front[W * 8].  Phase 2: each lane takes frontier nodes
 // from an LDS counter and walks their subtrees with its own level iterators (the
 // walk of planes_query, bounded by the tree depth).  Every node a walk reaches has
 // all its ancestors visited, so the set of ids is planes_query's; ids are appended
@@ -2452,8 +2453,37 @@ struct DevInput {
   const uint32_t *sets = nullptr;
 };
 
+// Largest per-query set id of a device array (validation before any walk reads it).
+__global__ void k_max_u32(const uint32_t *__restrict__ a, uint64_t n, uint32_t *__restrict__ out) {
+  uint32_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    m = a[i] > m ? a[i] : m;
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// Queries (and per-query set ids) in device memory: host arrays are copied.  Every
+// set id must name a set of the tree (nsets): host ids are checked here, device ids
+// by a max-reduction read back before any walk indexes the set table with them.
 static void stage_inputs(Ctx *ctx, DevInput &in, const double *q, const uint32_t *sets, uint64_t count_doubles,
-                         uint64_t nq, int mem) {
+                         uint64_t nq, int mem, uint32_t nsets) {
+  if (sets && nq) {
+    uint32_t mx = 0;
+    if (mem == OTG_MEM_DEVICE) {
+      Buf<uint32_t> dm(ctx, 1);
+      OTG_HIP_CHECK(hipMemsetAsync(dm.p, 0, 4, ctx->stream));
+      launch(ctx, "check_sets", k_max_u32, dim3(std::min<uint64_t>(div_up(nq, 256), 1024)), dim3(256), 0, sets, nq,
+             dm.p);
+      OTG_HIP_CHECK(hipMemcpyAsync(&mx, dm.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } else {
+      mx = *std::max_element(sets, sets + nq);
+    }
+    if (mx >= nsets) throw Error{OTG_ERR_INVALID_ARG, "a query's set id is not a set of the tree"};
+  }
   if (mem == OTG_MEM_DEVICE) {
     in.q = q;
     in.sets = sets;
@@ -2470,6 +2500,15 @@ static void stage_inputs(Ctx *ctx, DevInput &in, const double *q, const uint32_t
   }
 }
 
+// Result lists are indexed with 32-bit positions: a batch whose ids total 2^32 - 1
+// or more (the scan reports it saturated) is rejected before anything is sized from it.
+static void check_id_total(uint32_t total, Result *owned = nullptr) {
+  if (total == kScanOverflow) {
+    delete owned;
+    throw Error{OTG_ERR_UNSUPPORTED, "a batch result of 2^32 - 1 or more ids (split the batch)"};
+  }
+}
+
 // Fixed-stride per-query results -> CSR
 static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, uint32_t stride, uint64_t nq) {
   Result *r = new Result();
@@ -2481,6 +2520,7 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
   uint32_t ht = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  check_id_total(ht, r);
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
@@ -2546,6 +2586,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   uint32_t hb = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  check_id_total(hb);
   OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
   Buf<double> bd(ctx, std::max<uint32_t>(hb, 1));
   Buf<uint32_t> bi(ctx, std::max<uint32_t>(hb, 1));
@@ -2625,7 +2666,7 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
   Ctx *ctx = run_ctx(t);
   t = knn_index(t);
   DevInput in;
-  stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem);
+  stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem, (uint32_t)t->sets.size());
   if (nq == 0 || t->n == 0 || k == 0) return empty_result(ctx, nq);
   return knn_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, nullptr, k, k,
                       knn_F0(max_within, tol), tol, footprint);
@@ -2665,6 +2706,7 @@ static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n
   uint32_t nh = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&nh, htot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  check_id_total(nh);
   if (nh < 2) return;
   Buf<uint64_t> keys(ctx, nh);
   Buf<uint32_t> vals(ctx, nh);
@@ -2711,6 +2753,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  check_id_total(ht, r);
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
@@ -2766,7 +2809,7 @@ Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint
   Ctx *ctx = run_ctx(t);
   t = range_index(t);
   DevInput in;
-  stage_inputs(ctx, in, ranges, sets, nq * 2 * t->dim, nq, mem);
+  stage_inputs(ctx, in, ranges, sets, nq * 2 * t->dim, nq, mem, (uint32_t)t->sets.size());
   return range_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, mode, tol);
 }
 
@@ -2784,6 +2827,7 @@ static Result *count_fill_csr(Ctx *ctx, uint64_t nq, Count count, Fill fill) {
   uint32_t ht = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  check_id_total(ht, r);
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
   r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(ht, 1) * 4);
@@ -2863,7 +2907,7 @@ Result *query_planes(const Tree *t, uint64_t nq, uint32_t nplanes, const double 
   // the reference tree: a node on the positive side contributes its whole subtree
   Ctx *ctx = run_ctx(t);
   DevInput in;
-  stage_inputs(ctx, in, planes, sets, nq * nplanes * (t->dim + 1), nq, mem);
+  stage_inputs(ctx, in, planes, sets, nq * nplanes * (t->dim + 1), nq, mem, (uint32_t)t->sets.size());
   const bool box = t->kind == OTG_KIND_BOX;
   const ViewSet vs = single_view(t);
 #define OTG_PLANES_M(DD, BB)                                                                        \
@@ -2911,7 +2955,7 @@ Result *query_pick(const Tree *t, uint64_t nq, const double *picks, const uint32
   // the reference tree: its node boxes decide the walk (ot_query.h:433-437)
   Ctx *ctx = run_ctx(t);
   DevInput in;
-  stage_inputs(ctx, in, picks, sets, nq * t->dim, nq, mem);
+  stage_inputs(ctx, in, picks, sets, nq * t->dim, nq, mem, (uint32_t)t->sets.size());
   const bool box = t->kind == OTG_KIND_BOX;
   const ViewSet vs = single_view(t);
 #define OTG_PICK(DD) \
@@ -2956,7 +3000,7 @@ Result *query_ray_all(const Tree *t, uint64_t nq, const double *rays, const uint
   // the reference tree, as for the first hit (node hit tests decide the walk)
   Ctx *ctx = run_ctx(t);
   DevInput in;
-  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
+  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem, (uint32_t)t->sets.size());
   const bool box = t->kind == OTG_KIND_BOX;
   const ViewSet vs = single_view(t);
 #define OTG_RAY_ALL(DD)                                                                   \
@@ -3054,7 +3098,7 @@ Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t
   // result depends on the order hits are met -- the reference tree's order.
   Ctx *ctx = run_ctx(t);
   DevInput in;
-  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem);
+  stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem, (uint32_t)t->sets.size());
   return ray_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, tol, inc, max_dist);
 }
 
@@ -3124,6 +3168,11 @@ static Result *collide_impl(const Tree *lt, uint32_t lset, const Tree *rt, uint3
   uint32_t np = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&np, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  // the pair list holds 2 ids per pair, indexed with 32-bit positions
+  if (np == kScanOverflow || np > 0x7FFFFFFFu) {
+    delete r;
+    throw Error{OTG_ERR_UNSUPPORTED, "2^31 or more collision pairs"};
+  }
   Buf<uint64_t> off(ctx, ne + 1);
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(ne + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, ne, np,
          off.p);

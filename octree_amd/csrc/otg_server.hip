@@ -501,16 +501,24 @@ class Server {
         split_ms += now_ms() - t2;
       }
     } catch (const Error &e) {
+      drain(ctx);
       for (auto *r : group) {
         r->status = e.status;
         r->error = e.msg;
       }
     } catch (const std::bad_alloc &) {
+      drain(ctx);
       for (auto *r : group) r->status = OTG_ERR_OOM;
     } catch (...) {
+      drain(ctx);
       for (auto *r : group) r->status = OTG_ERR_HIP;
     }
   }
+
+  // A failed launch may have left kernels queued that still read the pinned trees:
+  // the worker's stream drains before the pins are released (its error, if any, is
+  // the one already reported).
+  static void drain(Ctx *ctx) { (void)hipStreamSynchronize(ctx->stream); }
 };
 
 }  // namespace otg

@@ -1365,8 +1365,9 @@ static void pairs_push(oc_pairs *p, uint32_t a, uint32_t b) {
   vec_push(&p->pairs, b);
 }
 
-static const double *g_sort_geom;
-static int g_sort_dim;
+/* per thread: collisions of different trees may run on concurrent threads */
+static _Thread_local const double *g_sort_geom;
+static _Thread_local int g_sort_dim;
 /* ot_query.h:1699-1716 comparator: (minX, id) */
 static int sort_minx_id(const void *a, const void *b) {
   uint32_t i = *(const uint32_t *)a, j = *(const uint32_t *)b;

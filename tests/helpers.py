@@ -323,6 +323,59 @@ def compare_tables(gpu: dict, ora: dict, is_box: bool):
                 assert sorted(ga[s:e]) == sorted(oa[s:e]), ("entity run", s, e)
 
 
+def compare_tables_fast(gpu: dict, ora: dict, is_box: bool):
+    """compare_tables without per-run Python loops (for tables of millions of
+    entities): entity ids are compared after sorting each equal-key run by id."""
+    for name in ("depth", "ebegin", "ecount", "ccount", "key", "min", "size"):
+        a, b = gpu[name], ora[name]
+        assert a.shape == b.shape, (name, a.shape, b.shape)
+        if a.dtype.kind == "f":
+            bad = np.nonzero((a.view(np.uint64) != b.view(np.uint64)).reshape(a.shape[0], -1).any(axis=1))[0]
+        else:
+            bad = np.nonzero(a != b)[0]
+        assert len(bad) == 0, f"{name} differs at nodes {bad[:10].tolist()}"
+    m = ora["ccount"] > 0
+    assert np.array_equal(gpu["cbegin"][m], ora["cbegin"][m]), "cbegin differs"
+    if is_box:
+        db = max(1, int(ora["max_depth"]).bit_length())
+        okeys = (ora["sorted_code"].astype(np.uint64) << np.uint64(db)) | ora["sorted_depth"].astype(np.uint64)
+    else:
+        okeys = ora["sorted_code"]
+    assert np.array_equal(gpu["sorted_keys"], okeys), "sorted location keys differ"
+    ga = gpu["ids"][np.lexsort((gpu["ids"], okeys))]
+    oa = ora["ids"][np.lexsort((ora["ids"], okeys))]
+    assert np.array_equal(ga, oa), "entity storage differs beyond the order of equal keys"
+
+
+def csr_take(off, ids, rows):
+    """Sub-CSR of the listed rows (in the given order)."""
+    off = np.asarray(off, np.int64)
+    rows = np.asarray(rows, np.int64)
+    lens = off[rows + 1] - off[rows]
+    out_off = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum(lens, out=out_off[1:])
+    starts = np.repeat(off[rows] - out_off[:-1], lens)
+    return out_off, ids[np.arange(out_off[-1], dtype=np.int64) + starts]
+
+
+def csr_sort_rows(off, ids):
+    """Every row's ids in ascending order (vectorised)."""
+    off = np.asarray(off, np.int64)
+    row = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    return ids[np.lexsort((ids, row))]
+
+
+def oracle_trees(specs, threads=16):
+    """Oracle trees built concurrently (the C builds run without the GIL).
+    specs: (D, kind, geom, box, max_depth, cap) tuples."""
+    import concurrent.futures as cf
+
+    from oracle import oracle as O
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(lambda a: O.OracleTree(*a), specs))
+
+
 def check_node_table_rules(t: dict, n: int, cap: int, D: int, is_box: bool):
     """Structural rules of the static linear tree (ot_static_linear_core.h:417-519),
     checked directly on a node table + sorted codes/depths:
