@@ -11,6 +11,13 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libotg.so"
 
 OTG_OK = 0
+OTG_ERR_INVALID_ARG = 1
+OTG_ERR_HIP = 2
+OTG_ERR_OOM = 3
+OTG_ERR_UNSUPPORTED = 4
+OTG_ERR_OUT_OF_SPACE = 5
+OTG_ERR_KEY_OVERFLOW = 6
+OTG_ERR_NO_DEVICE = 7
 OTG_MEM_HOST = 0
 OTG_MEM_DEVICE = 1
 OTG_KIND_POINT = 0

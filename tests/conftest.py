@@ -15,6 +15,14 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu_ctx():
+    # torch ships its own HIP runtime; it must initialise the device before libotg's
+    # runtime does, or torch reports no GPU (device tensors are handed to libotg)
+    try:
+        import torch
+
+        torch.cuda.init()
+    except ImportError:
+        pass
     import octree_amd
 
     ctx = octree_amd.Context(0)
