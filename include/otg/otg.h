@@ -44,6 +44,7 @@ typedef int otg_status;
 #define OTG_ERR_OUT_OF_SPACE 5  /* a point outside an explicit box: the reference Build never ends */
 #define OTG_ERR_KEY_OVERFLOW 6  /* set-id + location bits of a forest exceed 64 */
 #define OTG_ERR_NO_DEVICE 7     /* no HIP device visible */
+#define OTG_ERR_CAPACITY 8      /* a caller's output buffer is too small (the size needed is reported) */
 
 #define OTG_MEM_HOST 0   /* pointer is host memory */
 #define OTG_MEM_DEVICE 1 /* pointer is device memory of the context's device */
@@ -210,6 +211,22 @@ OTG_API otg_status otg_planes(const otg_tree *tree, uint64_t nq, uint32_t nplane
  * distance (ties by id). */
 OTG_API otg_status otg_ray_all(const otg_tree *tree, uint64_t nq, const double *rays, const uint32_t *set_ids, int mem,
                                double tolerance, double tolerance_increment, double max_distance, otg_result **out);
+
+/* Allocation-free kNN.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same ids in the same order as otg_knn, for
+ * device-resident points (and set_ids, or NULL), written into the caller's device
+ * CSR: offsets[nq + 1] and at most ids_capacity ids; *nids receives the id count,
+ * also with OTG_ERR_CAPACITY (then it is the size needed and ids is untouched).
+ * Every temporary lives in a fixed 96 MiB scratch array per device, held until the
+ * process ends, so a call takes at most 262,144 queries (OTG_ERR_INVALID_ARG
+ * beyond) with k <= 64; a batch whose tolerance bands need more than the array
+ * holds fails with OTG_ERR_UNSUPPORTED.  Calls on one device share its array and
+ * are serialised by a process-wide lock (the array keeps the last call's contents).
+ * Queries are walked in input order; the call synchronises the tree's stream. */
+OTG_API otg_status otg_knn_noalloc(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids,
+                                   uint32_t k, double max_within, double tolerance, uint64_t *offsets, uint32_t *ids,
+                                   uint64_t ids_capacity, uint64_t *nids);
 
 OTG_API otg_status otg_result_get_info(const otg_result *res, uint64_t *nq, uint64_t *nids);
 OTG_API otg_status otg_result_copy(const otg_result *res, uint64_t *offsets, uint32_t *ids);

@@ -18,6 +18,7 @@ OTG_ERR_UNSUPPORTED = 4
 OTG_ERR_OUT_OF_SPACE = 5
 OTG_ERR_KEY_OVERFLOW = 6
 OTG_ERR_NO_DEVICE = 7
+OTG_ERR_CAPACITY = 8
 OTG_MEM_HOST = 0
 OTG_MEM_DEVICE = 1
 OTG_KIND_POINT = 0
@@ -38,6 +39,7 @@ STATUS_NAMES = {
     5: "OTG_ERR_OUT_OF_SPACE",
     6: "OTG_ERR_KEY_OVERFLOW",
     7: "OTG_ERR_NO_DEVICE",
+    8: "OTG_ERR_CAPACITY",
 }
 
 # every symbol declared in include/otg/otg.h
@@ -46,7 +48,7 @@ EXPORTED = [
     "otg_ctx_mem_stats", "otg_ctx_release_cache", "otg_ctx_set_profiling", "otg_ctx_kernel_times",
     "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_free", "otg_tree_get_info",
     "otg_tree_get_set_info", "otg_tree_export", "otg_tree_serialize", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
-    "otg_collide_trees", "otg_pick", "otg_ray_all", "otg_planes",
+    "otg_collide_trees", "otg_pick", "otg_ray_all", "otg_planes", "otg_knn_noalloc",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
     "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_stats",
@@ -110,6 +112,8 @@ def lib() -> C.CDLL:
         "otg_tree_export": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P, P]),
         "otg_tree_serialize": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64, U64P]),
         "otg_knn": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, PP]),
+        "otg_knn_noalloc": (C.c_int, [P, C.c_uint64, P, P, C.c_uint32, C.c_double, C.c_double, P, P, C.c_uint64,
+                                      C.POINTER(C.c_uint64)]),
         "otg_knn_footprint": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, U64P]),
         "otg_range": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_int, C.c_double, PP]),
         "otg_ray_first": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, C.c_double, C.c_double, PP]),

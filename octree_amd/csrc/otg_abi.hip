@@ -288,6 +288,7 @@ OTG_API const char *otg_status_string(otg_status s) {
     case OTG_ERR_OUT_OF_SPACE: return "point outside the given tree box";
     case OTG_ERR_KEY_OVERFLOW: return "key bits exceed 64";
     case OTG_ERR_NO_DEVICE: return "no HIP device";
+    case OTG_ERR_CAPACITY: return "output buffer too small";
   }
   return "unknown status";
 }
@@ -309,6 +310,8 @@ OTG_API otg_status otg_ctx_create(int device, otg_ctx **out) {
     delete c;
     return OTG_ERR_HIP;
   }
+  // the allocation-free entry point's scratch address on this device (once per device)
+  otg::noalloc_init_device(device);
   *out = c;
   return OTG_OK;
 }

@@ -89,7 +89,28 @@ class Ctx {
   void collect_times();  // synchronises the stream
 };
 
-// Scoped device buffer from the context allocator.
+// Error carrying a status code across the host orchestration (never crosses the ABI).
+struct Error {
+  otg_status status;
+  std::string msg;
+};
+
+// Bump allocator over a fixed block of device memory the caller owns (the
+// allocation-free entry point's scratch array).  Buffers taken from it are
+// released all at once by the caller, never one by one.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void *take(size_t bytes) {
+    const size_t b = (bytes + 255) & ~size_t(255);
+    if (b > cap - used) throw Error{OTG_ERR_UNSUPPORTED, "the batch needs more scratch memory than the limit"};
+    void *p = base + used;
+    used += b;
+    return p;
+  }
+};
+
+// Scoped device buffer from the context allocator, or (owning nothing) from an arena.
 template <typename T>
 struct Buf {
   Ctx *ctx = nullptr;
@@ -97,6 +118,9 @@ struct Buf {
   size_t n = 0;
   Buf() = default;
   Buf(Ctx *c, size_t count) : ctx(c), n(count) { p = count ? (T *)c->alloc(count * sizeof(T)) : nullptr; }
+  Buf(Arena *a, size_t count) : ctx(nullptr), n(count) { p = count ? (T *)a->take(count * sizeof(T)) : nullptr; }
+  // from the arena when there is one, else from the context
+  Buf(Ctx *c, Arena *a, size_t count) : Buf(a ? Buf(a, count) : Buf(c, count)) {}
   Buf(const Buf &) = delete;
   Buf &operator=(const Buf &) = delete;
   Buf(Buf &&o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
@@ -113,12 +137,6 @@ struct Buf {
     n = 0;
   }
   T *release() { T *q = p; p = nullptr; n = 0; return q; }
-};
-
-// Error carrying a status code across the host orchestration (never crosses the ABI).
-struct Error {
-  otg_status status;
-  std::string msg;
 };
 
 #define OTG_HIP_CHECK(expr)                                                              \
@@ -147,7 +165,8 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // *d_total (device, if not null) receives the total, or kScanOverflow when it does not
 // fit 32 bits (out is then not meaningful).
 constexpr uint32_t kScanOverflow = 0xFFFFFFFFu;
-void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total);
+void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total,
+                        Arena *arena = nullptr);
 // Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
 // result is in keys/vals on return.
 void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits);
@@ -226,6 +245,19 @@ void set_thread_run_ctx(Ctx *c);
 Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
                   double max_within, double tol,
                   uint64_t *footprint = nullptr);
+// Caller-owned CSR of the allocation-free kNN path: offsets[nq + 1] and up to
+// `capacity` ids, device memory; nids is set to the result's id count (also when the
+// capacity is short, so the caller learns the size it needs).
+struct CsrOut {
+  uint64_t *offsets = nullptr;
+  uint32_t *ids = nullptr;
+  uint64_t capacity = 0;
+  uint64_t nids = 0;
+};
+// kNN of device-resident queries with every temporary taken from `arena`: no device
+// allocation, queries walked in input order; synchronises before returning.
+void query_knn_noalloc(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, uint32_t k,
+                       double max_within, double tol, Arena &arena, CsrOut &out);
 Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
                     double tol);
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
@@ -242,6 +274,10 @@ Result *query_planes(const Tree *t, uint64_t nq, uint32_t nplanes, const double 
                      int mode, double tol);
 Result *query_ray_all(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
                       double inc, double max_dist);
+
+// Looks up the allocation-free entry point's scratch array on `device` (current),
+// once per device; a failure is kept and reported by that entry point.
+void noalloc_init_device(int device);
 
 // host-side mirrors of reference helpers
 uint32_t max_theoretical_depth(int dim);

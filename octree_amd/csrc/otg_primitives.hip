@@ -187,13 +187,13 @@ __global__ void __launch_bounds__(kBlock) k_scan_final_level(LevelSource src, ui
   }
 }
 
-void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total) {
+void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total, Arena *arena) {
   if (n == 0) {
     if (d_total) OTG_HIP_CHECK(hipMemsetAsync(d_total, 0, 4, ctx->stream));
     return;
   }
   uint32_t nb = div_up(n, kTile);
-  Buf<uint64_t> partials(ctx, nb);
+  Buf<uint64_t> partials(ctx, arena, nb);
   launch(ctx, "scan_reduce_u32", k_scan_reduce<OpAdd64, U32Source>, dim3(nb), dim3(kBlock), 0, U32Source{in}, n,
          partials.p);
   launch(ctx, "scan_partials_u32", k_scan_partials<OpAdd64>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);

@@ -2552,12 +2552,33 @@ static void sort_positions(Ctx *ctx, uint32_t *list, uint32_t n, uint64_t nq) {
   radix_sort_pairs(ctx, keys.p, list, n, bit_width32((uint32_t)std::max<uint64_t>(nq - 1, 1)));
 }
 
+// Counts -> CSR offsets and compacted ids in the caller's buffers (the
+// allocation-free path); nothing is written past the ids capacity.
+static void csr_into(Ctx *ctx, Arena *ar, const uint32_t *ids, const uint32_t *cnt, uint32_t stride, uint64_t nq,
+                     CsrOut &co) {
+  Buf<uint32_t> scan(ctx, ar, nq + 1), total(ctx, ar, 1);
+  exclusive_scan_u32(ctx, cnt, scan.p, nq, total.p, ar);
+  uint32_t ht = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  check_id_total(ht);
+  co.nids = ht;
+  if (ht > co.capacity) throw Error{OTG_ERR_CAPACITY, "the result holds more ids than the output capacity"};
+  launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
+         co.offsets);
+  launch(ctx, "compact_fixed", k_compact_fixed, dim3(div_up(nq * stride, 256)), dim3(256), 0, ids, stride,
+         (const uint64_t *)co.offsets, nq, co.ids);
+}
+
+// ar / co: the allocation-free path -- every temporary from the arena, the result
+// into the caller's CSR (returns null), queries in input order.
 template <int D, int KCAP, bool BOX>
 static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs,
-                        const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr) {
-  Buf<uint32_t> order = query_order<D>(ctx, vs, q, D, qs, nq);
-  Buf<uint32_t> ids(ctx, nq * KCAP), cnt(ctx, nq), ovf(ctx, 1), band(ctx, nq);
-  Buf<double> Fq(ctx, nq);
+                        const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr,
+                        Arena *ar = nullptr, CsrOut *co = nullptr) {
+  Buf<uint32_t> order = ar ? Buf<uint32_t>() : query_order<D>(ctx, vs, q, D, qs, nq);
+  Buf<uint32_t> ids(ctx, ar, nq * KCAP), cnt(ctx, ar, nq), ovf(ctx, ar, 1), band(ctx, ar, nq);
+  Buf<double> Fq(ctx, ar, nq);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));  // band count
   launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
          (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
@@ -2576,20 +2597,26 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
     footprint[1] += h[1];
   }
   const uint32_t nb = hovf[0];
-  if (nb == 0) return fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
+  if (nb == 0) {
+    if (co) {
+      csr_into(ctx, ar, ids.p, cnt.p, KCAP, nq, *co);
+      return nullptr;
+    }
+    return fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
+  }
   // band queries: count, fill, sort, then splice into the CSR
-  Buf<uint32_t> bcnt(ctx, nb), boff(ctx, nb + 1), btot(ctx, 1);
+  Buf<uint32_t> bcnt(ctx, ar, nb), boff(ctx, ar, nb + 1), btot(ctx, ar, 1);
   launch(ctx, "knn_band_count", k_knn_band<D, BOX, false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
          qs, (const uint32_t *)band.p, nb, (const double *)Fq.p, tol, bcnt.p, (const uint32_t *)nullptr,
          (double *)nullptr, (uint32_t *)nullptr);
-  exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p);
+  exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p, ar);
   uint32_t hb = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   check_id_total(hb);
   OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
-  Buf<double> bd(ctx, std::max<uint32_t>(hb, 1));
-  Buf<uint32_t> bi(ctx, std::max<uint32_t>(hb, 1));
+  Buf<double> bd(ctx, ar, std::max<uint32_t>(hb, 1));
+  Buf<uint32_t> bi(ctx, ar, std::max<uint32_t>(hb, 1));
   launch(ctx, "knn_band_fill", k_knn_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
          qs, (const uint32_t *)band.p, nb, (const double *)Fq.p, tol, (uint32_t *)nullptr, (const uint32_t *)boff.p,
          bd.p, bi.p);
@@ -2597,9 +2624,20 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
          bd.p, bi.p);
   launch(ctx, "knn_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
          (const uint32_t *)bcnt.p, cnt.p);
-  Result *r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
+  Result *r = nullptr;
+  uint64_t *offsets;
+  uint32_t *out_ids;
+  if (co) {
+    csr_into(ctx, ar, ids.p, cnt.p, KCAP, nq, *co);
+    offsets = co->offsets;
+    out_ids = co->ids;
+  } else {
+    r = fixed_to_csr(ctx, ids.p, cnt.p, KCAP, nq);
+    offsets = r->d_offsets;
+    out_ids = r->d_ids;
+  }
   launch(ctx, "knn_band_copy", k_band_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
-         (const uint32_t *)boff.p, (const uint32_t *)bi.p, (const uint64_t *)r->d_offsets, r->d_ids);
+         (const uint32_t *)boff.p, (const uint32_t *)bi.p, (const uint64_t *)offsets, out_ids);
   OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the staged count above lives on this stack frame
   return r;
 }
@@ -2607,14 +2645,14 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
 // kmax = the largest k of the launch (per-query k in qk, or k for all)
 static Result *knn_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
                             const uint32_t *qs, const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol,
-                            uint64_t *footprint = nullptr) {
+                            uint64_t *footprint = nullptr, Arena *ar = nullptr, CsrOut *co = nullptr) {
 // list capacity k + 1 for the largest k of a class: one spare slot holds an entry of the
 // tolerance band above the k-th distance; a longer band is redone by the band pass
-#define OTG_KNN_K(DD, BB)                                                                              \
-  if (kmax <= 8) return knn_impl<DD, 9, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);               \
-  if (kmax <= 16) return knn_impl<DD, 17, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
-  if (kmax <= 32) return knn_impl<DD, 33, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);             \
-  return knn_impl<DD, 65, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint);
+#define OTG_KNN_K(DD, BB)                                                                                  \
+  if (kmax <= 8) return knn_impl<DD, 9, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint, ar, co);           \
+  if (kmax <= 16) return knn_impl<DD, 17, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint, ar, co);         \
+  if (kmax <= 32) return knn_impl<DD, 33, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint, ar, co);         \
+  return knn_impl<DD, 65, BB>(ctx, vs, nq, q, qs, qk, k, F0, tol, footprint, ar, co);
 #define OTG_KNN(DD)          \
   case DD:                   \
     if (box) {               \
@@ -2670,6 +2708,31 @@ Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t 
   if (nq == 0 || t->n == 0 || k == 0) return empty_result(ctx, nq);
   return knn_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, nullptr, k, k,
                       knn_F0(max_within, tol), tol, footprint);
+}
+
+void query_knn_noalloc(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, uint32_t k,
+                       double max_within, double tol, Arena &ar, CsrOut &co) {
+  if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
+  Ctx *ctx = run_ctx(t);
+  const Tree *x = knn_index(t);
+  if (sets && nq) {
+    Buf<uint32_t> dm(&ar, 1);
+    OTG_HIP_CHECK(hipMemsetAsync(dm.p, 0, 4, ctx->stream));
+    launch(ctx, "check_sets", k_max_u32, dim3(std::min<uint64_t>(div_up(nq, 256), 1024)), dim3(256), 0, sets, nq,
+           dm.p);
+    uint32_t mx = 0;
+    OTG_HIP_CHECK(hipMemcpyAsync(&mx, dm.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (mx >= x->sets.size()) throw Error{OTG_ERR_INVALID_ARG, "a query's set id is not a set of the tree"};
+  }
+  co.nids = 0;
+  if (nq == 0 || x->n == 0 || k == 0) {
+    OTG_HIP_CHECK(hipMemsetAsync(co.offsets, 0, (nq + 1) * 8, ctx->stream));
+  } else {
+    knn_dispatch(ctx, x->dim, x->kind == OTG_KIND_BOX, single_view(x), nq, pts, sets, nullptr, k, k,
+                 knn_F0(max_within, tol), tol, nullptr, &ar, &co);
+  }
+  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 // One-lane walks stop counting after this many node visits; such queries, and those
