@@ -65,6 +65,17 @@ void Ctx::free(void *p) {
   bytes_cached += sz;
 }
 
+void Ctx::sync() {
+  if (d_err) OTG_HIP_CHECK(hipMemcpyAsync(h_err, d_err, 4, hipMemcpyDeviceToHost, stream));
+  OTG_HIP_CHECK(hipStreamSynchronize(stream));
+  if (h_err && *h_err) {
+    *h_err = 0;
+    OTG_HIP_CHECK(hipMemsetAsync(d_err, 0, 4, stream));
+    OTG_HIP_CHECK(hipStreamSynchronize(stream));
+    throw Error{OTG_ERR_HIP, "a device-wide wait ran past its bound (results discarded)"};
+  }
+}
+
 void Ctx::release_cache() {
   std::lock_guard<std::mutex> g(mu);
   (void)hipStreamSynchronize(stream);
@@ -115,9 +126,14 @@ Tree::~Tree() {
   if (!ctx) return;
   ctx->free(d_view);
   ctx->free(d_sets);
-  ctx->free(d_ids);
+  if (!shares_entities) {
+    ctx->free(d_ids);
+    ctx->free(d_geom);
+  }
+  ctx->free(d_ids_ref);
+  ctx->free(d_geom_ref);
+  ctx->free(d_view_ref);
   ctx->free(d_keys);
-  ctx->free(d_geom);
   ctx->free(d_info);
   ctx->free(d_rend);
   ctx->free(d_depth);
@@ -310,6 +326,15 @@ OTG_API otg_status otg_ctx_create(int device, otg_ctx **out) {
     delete c;
     return OTG_ERR_HIP;
   }
+  if (hipMalloc(&c->d_err, 256) != hipSuccess || hipMemset(c->d_err, 0, 256) != hipSuccess ||
+      hipHostMalloc(&c->h_err, 4, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    if (c->d_err) (void)hipFree(c->d_err);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OTG_ERR_HIP;
+  }
+  *c->h_err = 0;
   // the allocation-free entry point's scratch address on this device (once per device)
   otg::noalloc_init_device(device);
   *out = c;
@@ -327,6 +352,8 @@ OTG_API void otg_ctx_destroy(otg_ctx *ctx) {
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
   for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
+  if (ctx->d_err) (void)hipFree(ctx->d_err);
+  if (ctx->h_err) (void)hipHostFree(ctx->h_err);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -475,7 +502,10 @@ OTG_API otg_status otg_tree_export(const otg_tree *t, uint32_t set, uint8_t *nod
     if (node_key && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_key, t->d_nkey + no, nn * 8, hipMemcpyDeviceToHost, st));
     if (node_min && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_min, t->d_nmin + no * D, nn * D * 8, hipMemcpyDeviceToHost, st));
     if (node_size && nn) OTG_HIP_CHECK(hipMemcpyAsync(node_size, t->d_nsize + no * D, nn * D * 8, hipMemcpyDeviceToHost, st));
-    if (entity_ids && ne) OTG_HIP_CHECK(hipMemcpyAsync(entity_ids, t->d_ids + eo, ne * 4, hipMemcpyDeviceToHost, st));
+    if (entity_ids && ne) {
+      const uint32_t *ids = otg::reference_view_of(t).ids;
+      OTG_HIP_CHECK(hipMemcpyAsync(entity_ids, ids + eo, ne * 4, hipMemcpyDeviceToHost, st));
+    }
     std::vector<uint64_t> keys;
     if (sorted_keys && ne) {
       keys.resize(ne);

@@ -746,29 +746,29 @@ static double ord_to_double(uint64_t u) {
   return x;
 }
 
+// Key layout of a tree's sets: set id above D * (largest depth) code bits (and depth
+// bits for boxes).
 template <int D, bool BOX>
-static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *offsets, uint32_t nsets,
-                       const double *boxes, const uint32_t *depths, const uint32_t *caps, bool allow_aux = true) {
+static void set_layout(Tree *t) {
+  uint32_t maxM = 1;
+  for (const SetParams &sp : t->sets) maxM = std::max(maxM, sp.max_depth);
+  const uint32_t nsets = (uint32_t)t->sets.size();
+  KeyLayout &L = t->layout;
+  L.dbits = BOX ? (32 - __builtin_clz(maxM)) : 0;
+  L.cbits = D * maxM;
+  L.sbits = nsets > 1 ? (32 - __builtin_clz(nsets - 1)) : 0;
+  L.total = L.sbits + L.cbits + L.dbits;
+  if (L.total > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "set id and location bits exceed 64"};
+}
+
+// Per-set parameters: depth (given or EstimateMaxDepth), capacity, tree box (given,
+// or the entities' bounding box: one synchronisation when any set needs it).
+template <int D, bool BOX>
+static void prepare_sets(Tree *t, const double *dgeom, const uint64_t *offsets, uint32_t nsets, const double *boxes,
+                         const uint32_t *depths, const uint32_t *caps) {
   Ctx *ctx = t->ctx;
-  constexpr int S = BOX ? 2 * D : D;
-  const uint64_t n = offsets[nsets];
-  t->n = n;
-  if (n >= (1ull << 30)) throw Error{OTG_ERR_UNSUPPORTED, "more than 2^30 entities"};
-
-  // entity geometry on the device (input order)
-  Buf<double> dgeom_own;
-  const double *dgeom = geom_in;
-  if (mem == OTG_MEM_HOST) {
-    dgeom_own = Buf<double>(ctx, n * S);
-    if (n) OTG_HIP_CHECK(hipMemcpyAsync(dgeom_own.p, geom_in, n * S * 8, hipMemcpyHostToDevice, ctx->stream));
-    dgeom = dgeom_own.p;
-  }
-
-  // per-set parameters
   t->sets.assign(nsets, SetParams{});
   std::vector<uint32_t> need_bbox;
-  uint32_t maxM = 1;
-  uint64_t max_n = 1;
   for (uint32_t s = 0; s < nsets; ++s) {
     SetParams &sp = t->sets[s];
     sp.ent_off = offsets[s];
@@ -780,8 +780,6 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
       throw Error{OTG_ERR_INVALID_ARG, "max depth out of range"};
     sp.max_depth = M;
     sp.min_lcp = 255;
-    maxM = std::max(maxM, M);
-    max_n = std::max<uint64_t>(max_n, sp.ent_cnt);
     if (!boxes) need_bbox.push_back(s);
   }
   if (!need_bbox.empty()) {
@@ -801,7 +799,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
              bb.p + q * 2 * D);
     }
     OTG_HIP_CHECK(hipMemcpyAsync(init.data(), bb.p, init.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->sync();
     for (size_t q = 0; q < need_bbox.size(); ++q) {
       double bmin[kMaxDim], bmax[kMaxDim];
       for (int d = 0; d < D; ++d) {
@@ -817,42 +815,60 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   }
   if (boxes)
     for (uint32_t s = 0; s < nsets; ++s) init_set_space(t->sets[s], D, BOX, &boxes[(size_t)s * 2 * D], &boxes[(size_t)s * 2 * D + D]);
+  set_layout<D, BOX>(t);
+}
 
-  KeyLayout &L = t->layout;
-  L.dbits = BOX ? (32 - __builtin_clz(maxM)) : 0;
-  L.cbits = D * maxM;
-  L.sbits = nsets > 1 ? (32 - __builtin_clz(nsets - 1)) : 0;
-  L.total = L.sbits + L.cbits + L.dbits;
-  if (L.total > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "set id and location bits exceed 64"};
-  KeyView kv{L.dbits, L.cbits};
-
-  Buf<uint64_t> d_off(ctx, nsets + 1);
-  OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, offsets, (nsets + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
-  t->d_sets = (SetParams *)ctx->alloc(sizeof(SetParams) * nsets);
+static void upload_sets(Tree *t) {
+  Ctx *ctx = t->ctx;
+  const size_t nsets = t->sets.size();
+  if (!t->d_sets) t->d_sets = (SetParams *)ctx->alloc(sizeof(SetParams) * nsets);
   OTG_HIP_CHECK(hipMemcpyAsync(t->d_sets, t->sets.data(), sizeof(SetParams) * nsets, hipMemcpyHostToDevice, ctx->stream));
+}
 
-  if (n == 0) {
-    t->node_count = 0;
-    return;
-  }
-
-  // 1. encode
-  Buf<uint32_t> err(ctx, 1);
-  OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, ctx->stream));
+// 1-2. location keys of every entity, sorted (t->d_keys); vals = input positions in
+// that order; encode error bits land in *err (device).
+template <int D, bool BOX>
+static void encode_sort(Tree *t, const double *dgeom, const uint64_t *d_off, uint32_t *vals, uint32_t *err) {
+  Ctx *ctx = t->ctx;
+  const uint64_t n = t->n;
+  const KeyView kv{t->layout.dbits, t->layout.cbits};
   t->d_keys = (uint64_t *)ctx->alloc(n * 8);
-  Buf<uint32_t> vals(ctx, n);
-  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, (const uint64_t *)d_off.p, nsets,
-         (const SetParams *)t->d_sets, kv, t->d_keys, vals.p, err.p);
-  uint32_t herr = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, d_off,
+         (uint32_t)t->sets.size(), (const SetParams *)t->d_sets, kv, t->d_keys, vals, err);
+  radix_sort_pairs(ctx, t->d_keys, vals, n, t->layout.total);
+}
 
-  // 2. sort
-  radix_sort_pairs(ctx, t->d_keys, vals.p, n, L.total);
+// Keys of the reference tree from the sorted keys of the finer tree over the same
+// points: the grids differ by the exact power of two 2^(F - M) per axis, so the code
+// at depth M is the code at depth F shifted right by D * (F - M); the order is kept.
+template <int D>
+__global__ void __launch_bounds__(kB) k_coarse_keys(const uint64_t *__restrict__ fine, uint64_t n, KeyView fkv,
+                                                     const SetParams *__restrict__ fsets, KeyView ckv,
+                                                     const SetParams *__restrict__ csets, uint64_t *__restrict__ coarse) {
+  const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = fine[i];
+  const uint32_t s = fkv.set(k);
+  const uint32_t sh = D * (fsets[s].max_depth - csets[s].max_depth);
+  coarse[i] = ckv.set_base(s) | ((fkv.code(k) >> sh) << ckv.dbits);
+}
+
+// 3-7. node table from the sorted keys t->d_keys.  herr: host copy of the encode
+// error bits, whose copy was queued before; checked after the first synchronisation.
+template <int D, bool BOX>
+static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
+  Ctx *ctx = t->ctx;
+  const uint64_t n = t->n;
+  const uint32_t nsets = (uint32_t)t->sets.size();
+  const KeyLayout &L = t->layout;
+  const KeyView kv{L.dbits, L.cbits};
+  uint64_t max_n = 1;
+  for (const SetParams &sp : t->sets) max_n = std::max<uint64_t>(max_n, sp.ent_cnt);
 
   // 3. levels (+ per-set minimum level), root flags
   Buf<uint8_t> v(ctx, n + 9);  // levels v[0..n]; 8 bytes of padding for word-wise scans
-  launch(ctx, "levels", k_levels<D>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys, n,
-         (const uint64_t *)d_off.p, nsets, t->d_sets, kv, v.p);
+  launch(ctx, "levels", k_levels<D>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys, n, d_off, nsets,
+         t->d_sets, kv, v.p);
   launch(ctx, "root_flags", k_root_flags<BOX>, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)t->d_keys,
          t->d_sets, nsets, kv);
 
@@ -875,21 +891,20 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
   exclusive_scan_u32(ctx, int_flag.p, int_idx.p, n, totals.p + 1);
   uint32_t ht[2];
   OTG_HIP_CHECK(hipMemcpyAsync(ht, totals.p, 8, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  if (herr & kErrOutOfSpace) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
-  if (herr & kErrUnsupported) throw Error{OTG_ERR_UNSUPPORTED, "box level or grid id beyond the tree depth"};
+  ctx->sync();
+  if (herr && (*herr & kErrOutOfSpace)) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
+  if (herr && (*herr & kErrUnsupported)) throw Error{OTG_ERR_UNSUPPORTED, "box level or grid id beyond the tree depth"};
   const uint32_t nseg = ht[0], nint = ht[1];
 
   Buf<uint32_t> seg_list(ctx, (size_t)nseg + 1), seg_node(ctx, nseg), seg_node_idx(ctx, nseg);
   launch(ctx, "seg_compact", k_seg_compact, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)seg_flag.p,
          (const uint32_t *)seg_idx.p, n, seg_list.p, nseg);
   launch(ctx, "seg_nodes", k_seg_nodes<BOX>, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
-         (const uint8_t *)v.p, (const uint64_t *)t->d_keys, (const uint64_t *)d_off.p, nsets,
-         (const SetParams *)t->d_sets, kv, seg_node.p);
+         (const uint8_t *)v.p, (const uint64_t *)t->d_keys, d_off, nsets, (const SetParams *)t->d_sets, kv, seg_node.p);
   exclusive_scan_u32(ctx, seg_node.p, seg_node_idx.p, nseg, totals.p + 2);
   uint32_t nleaf = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&nleaf, totals.p + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   const uint64_t nnodes = (uint64_t)nint + nleaf;
   t->node_count = nnodes;
 
@@ -922,67 +937,272 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
          (const uint32_t *)ndesc.p, nnodes, (const uint64_t *)t->d_keys, (const uint8_t *)v.p,
          (const uint32_t *)seg_list.p, (const SetParams *)t->d_sets, kv, nk, t->d_info, t->d_rend, t->d_depth,
          t->d_nkey, t->d_nmin, t->d_nsize);
+}
 
-  // sorted geometry + local ids
+// Sorted SoA geometry and local ids of the entity storage.
+template <int D, bool BOX>
+static void gather_entities(Tree *t, const double *dgeom, const uint32_t *vals, const uint64_t *d_off) {
+  Ctx *ctx = t->ctx;
+  constexpr int S = BOX ? 2 * D : D;
+  const uint64_t n = t->n;
+  const KeyView kv{t->layout.dbits, t->layout.cbits};
   t->d_geom = (double *)ctx->alloc(n * S * 8);
   t->d_ids = (uint32_t *)ctx->alloc(n * 4);
-  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, (const uint32_t *)vals.p, n,
-         (const uint64_t *)t->d_keys, kv, (const uint64_t *)d_off.p, t->d_geom, t->d_ids);
+  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, vals, n,
+         (const uint64_t *)t->d_keys, kv, d_off, t->d_geom, t->d_ids);
+}
 
-  if (BOX) {
-    Buf<uint32_t> parent(ctx, nnodes), pending(ctx, nnodes);
-    Buf<uint8_t> is_root(ctx, nnodes);
-    OTG_HIP_CHECK(hipMemsetAsync(is_root.p, 0, nnodes, ctx->stream));
-    launch(ctx, "parents", k_parents, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
-           parent.p, pending.p);
-    launch(ctx, "mark_roots", k_mark_roots, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)t->d_sets, nsets,
-           is_root.p);
-    launch(ctx, "mbr", k_mbr<D>, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
-           (const uint32_t *)parent.p, pending.p, (const uint8_t *)is_root.p, (const double *)t->d_geom, n, t->d_nmin,
-           t->d_nsize);
+template <int D>
+static void build_mbr(Tree *t) {
+  Ctx *ctx = t->ctx;
+  const uint64_t nnodes = t->node_count;
+  const uint32_t nsets = (uint32_t)t->sets.size();
+  Buf<uint32_t> parent(ctx, nnodes), pending(ctx, nnodes);
+  Buf<uint8_t> is_root(ctx, nnodes);
+  OTG_HIP_CHECK(hipMemsetAsync(is_root.p, 0, nnodes, ctx->stream));
+  launch(ctx, "parents", k_parents, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
+         parent.p, pending.p);
+  launch(ctx, "mark_roots", k_mark_roots, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)t->d_sets, nsets,
+         is_root.p);
+  launch(ctx, "mbr", k_mbr<D>, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
+         (const uint32_t *)parent.p, pending.p, (const uint8_t *)is_root.p, (const double *)t->d_geom, t->n,
+         t->d_nmin, t->d_nsize);
+}
+
+// Point sets whose max-depth cells are expected to overflow: n / cap above an eighth
+// of the 2^(D M) cells.  EstimateMaxDepth picks M with n / cap >= 2^(D M), so every
+// set of estimated depth qualifies; a set that does not is checked after its build.
+static bool expects_overfull_leaves(const SetParams &sp, int D) {
+  const uint32_t bits = D * sp.max_depth;
+  if (bits >= 60) return false;
+  return (double)sp.ent_cnt / (double)sp.cap > (double)(1ull << bits) / 8.0;
+}
+
+// Depth of the finer index over a tree of these sets: as deep as the key allows.
+static uint32_t finer_depth(int D, uint32_t sbits) {
+  return std::min<uint32_t>({(uint32_t)kMaxDepth, max_theoretical_depth(D), (64 - sbits) / (uint32_t)D});
+}
+
+static Tree *new_finer_tree(const Tree *t, uint32_t fmax) {
+  Tree *aux = new Tree();
+  aux->ctx = t->ctx;
+  aux->kind = t->kind;
+  aux->dim = t->dim;
+  aux->n = t->n;
+  aux->sets = t->sets;
+  for (SetParams &sp : aux->sets) {
+    sp.max_depth = std::max(sp.max_depth, fmax);
+    sp.min_lcp = 255;
+    sp.root_splits = 0;
+    double bmin[kMaxDim], bmax[kMaxDim];
+    for (int d = 0; d < t->dim; ++d) {
+      bmin[d] = sp.space_min[d];
+      bmax[d] = sp.space_max[d];
+    }
+    init_set_space(sp, t->dim, false, bmin, bmax);
   }
-  OTG_HIP_CHECK(hipMemcpyAsync(t->sets.data(), t->d_sets, sizeof(SetParams) * nsets, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  t->device_bytes = n * (S * 8 + 4 + 8) + nnodes * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+  return aux;
+}
 
-  if (!BOX && allow_aux) {
-    // kNN acceleration: a deeper tree over the same points when max-depth leaves overflow
-    const uint32_t fmax = std::min<uint32_t>({(uint32_t)kMaxDepth, max_theoretical_depth(D), (64 - L.sbits) / D});
-    bool deeper = false;
-    for (const SetParams &sp : t->sets) deeper |= sp.max_depth < fmax;
-    if (deeper) {
-      Buf<uint32_t> flag(ctx, 1);
-      OTG_HIP_CHECK(hipMemsetAsync(flag.p, 0, 4, ctx->stream));
-      launch(ctx, "oversized", k_oversized, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nnodes,
-             (const SetParams *)t->d_sets, nsets, flag.p);
-      uint32_t hflag = 0;
-      OTG_HIP_CHECK(hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-      OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      if (hflag) {
-        std::vector<double> sboxes((size_t)nsets * 2 * D);
-        std::vector<uint32_t> fdepth(nsets);
-        for (uint32_t s = 0; s < nsets; ++s) {
-          for (int d = 0; d < D; ++d) {
-            sboxes[(size_t)s * 2 * D + d] = t->sets[s].space_min[d];
-            sboxes[(size_t)s * 2 * D + D + d] = t->sets[s].space_max[d];
-          }
-          fdepth[s] = std::max(t->sets[s].max_depth, fmax);
+template <int D, bool BOX>
+static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *offsets, uint32_t nsets,
+                       const double *boxes, const uint32_t *depths, const uint32_t *caps, bool allow_aux = true) {
+  Ctx *ctx = t->ctx;
+  constexpr int S = BOX ? 2 * D : D;
+  const uint64_t n = offsets[nsets];
+  t->n = n;
+  if (n >= (1ull << 30)) throw Error{OTG_ERR_UNSUPPORTED, "more than 2^30 entities"};
+
+  // entity geometry on the device (input order)
+  Buf<double> dgeom_own;
+  const double *dgeom = geom_in;
+  if (mem == OTG_MEM_HOST) {
+    dgeom_own = Buf<double>(ctx, n * S);
+    if (n) OTG_HIP_CHECK(hipMemcpyAsync(dgeom_own.p, geom_in, n * S * 8, hipMemcpyHostToDevice, ctx->stream));
+    dgeom = dgeom_own.p;
+  }
+  prepare_sets<D, BOX>(t, dgeom, offsets, nsets, boxes, depths, caps);
+  Buf<uint64_t> d_off(ctx, nsets + 1);
+  OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, offsets, (nsets + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  upload_sets(t);
+  if (n == 0) {
+    t->node_count = 0;
+    return;
+  }
+
+  // points: a finer index for kNN / range when max-depth leaves overflow.  When that
+  // is expected, one sort by the finer keys serves both trees (shared storage).
+  const uint32_t fmax = finer_depth(D, t->layout.sbits);
+  bool finer_possible = false, shared = false;
+  if (!BOX && allow_aux)
+    for (const SetParams &sp : t->sets) {
+      finer_possible |= sp.max_depth < fmax;
+      shared |= sp.max_depth < fmax && expects_overfull_leaves(sp, D);
+    }
+
+  Buf<uint32_t> err(ctx, 1), vals(ctx, n);
+  OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, ctx->stream));
+  uint32_t herr = 0;
+  Tree *aux = nullptr;
+  try {
+    if (shared) {
+      aux = new_finer_tree(t, fmax);
+      set_layout<D, false>(aux);
+      upload_sets(aux);
+      encode_sort<D, false>(aux, dgeom, d_off.p, vals.p, err.p);
+      OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      t->d_keys = (uint64_t *)ctx->alloc(n * 8);
+      launch(ctx, "coarse_keys", k_coarse_keys<D>, dim3(div_up(n, kB)), dim3(kB), 0, (const uint64_t *)aux->d_keys, n,
+             KeyView{aux->layout.dbits, aux->layout.cbits}, (const SetParams *)aux->d_sets,
+             KeyView{t->layout.dbits, t->layout.cbits}, (const SetParams *)t->d_sets, t->d_keys);
+      node_table<D, false>(t, d_off.p, &herr);
+      node_table<D, false>(aux, d_off.p, nullptr);
+      gather_entities<D, false>(t, dgeom, vals.p, d_off.p);
+      aux->d_geom = t->d_geom;
+      aux->d_ids = t->d_ids;
+      aux->shares_entities = true;
+      t->fine_order = true;
+    } else {
+      encode_sort<D, BOX>(t, dgeom, d_off.p, vals.p, err.p);
+      OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      node_table<D, BOX>(t, d_off.p, &herr);
+      gather_entities<D, BOX>(t, dgeom, vals.p, d_off.p);
+      if (BOX) build_mbr<D>(t);
+    }
+    for (Tree *x = t; x; x = (x == t ? aux : nullptr))
+      OTG_HIP_CHECK(hipMemcpyAsync(x->sets.data(), x->d_sets, sizeof(SetParams) * nsets, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    ctx->sync();
+  } catch (...) {
+    delete aux;
+    throw;
+  }
+  t->device_bytes = n * (S * 8 + 4 + 8) + t->node_count * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+  if (aux) {
+    aux->device_bytes = n * 8 + aux->node_count * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+    t->knn_tree = aux;
+    t->device_bytes += aux->device_bytes;
+    return;
+  }
+
+  if (finer_possible) {
+    // not expected, but max-depth leaves may still overflow (clustered points)
+    Buf<uint32_t> flag(ctx, 1);
+    OTG_HIP_CHECK(hipMemsetAsync(flag.p, 0, 4, ctx->stream));
+    launch(ctx, "oversized", k_oversized, dim3(div_up(t->node_count, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info,
+           t->node_count, (const SetParams *)t->d_sets, nsets, flag.p);
+    uint32_t hflag = 0;
+    OTG_HIP_CHECK(hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (hflag) {
+      std::vector<double> sboxes((size_t)nsets * 2 * D);
+      std::vector<uint32_t> fdepth(nsets);
+      for (uint32_t s = 0; s < nsets; ++s) {
+        for (int d = 0; d < D; ++d) {
+          sboxes[(size_t)s * 2 * D + d] = t->sets[s].space_min[d];
+          sboxes[(size_t)s * 2 * D + D + d] = t->sets[s].space_max[d];
         }
-        Tree *aux = new Tree();
-        aux->ctx = ctx;
-        aux->kind = t->kind;
-        aux->dim = t->dim;
-        try {
-          build_impl<D, false>(aux, dgeom, OTG_MEM_DEVICE, offsets, nsets, sboxes.data(), fdepth.data(), caps, false);
-        } catch (...) {
-          delete aux;
-          throw;
-        }
-        t->knn_tree = aux;
-        t->device_bytes += aux->device_bytes;
+        fdepth[s] = std::max(t->sets[s].max_depth, fmax);
       }
+      Tree *x = new Tree();
+      x->ctx = ctx;
+      x->kind = t->kind;
+      x->dim = t->dim;
+      try {
+        build_impl<D, false>(x, dgeom, OTG_MEM_DEVICE, offsets, nsets, sboxes.data(), fdepth.data(), caps, false);
+      } catch (...) {
+        delete x;
+        throw;
+      }
+      t->knn_tree = x;
+      t->device_bytes += x->device_bytes;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// reference storage order of a shared-sort point tree (equal keys by id)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kB) k_ref_ids(const uint32_t *__restrict__ ids, uint64_t n, uint64_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ pos) {
+  const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = ids[i];
+  pos[i] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(kB) k_ref_keys(const uint64_t *__restrict__ coarse, const uint32_t *__restrict__ pos,
+                                                  uint64_t n, uint64_t *__restrict__ keys) {
+  const uint64_t j = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (j < n) keys[j] = coarse[pos[j]];
+}
+
+template <int D>
+__global__ void __launch_bounds__(kB) k_ref_gather(const double *__restrict__ geom, const uint32_t *__restrict__ ids,
+                                                    const uint32_t *__restrict__ pos, uint64_t n,
+                                                    double *__restrict__ geom_out, uint32_t *__restrict__ ids_out) {
+  const uint64_t j = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (j >= n) return;
+  const uint64_t p = pos[j];
+#pragma unroll
+  for (int c = 0; c < D; ++c) geom_out[(uint64_t)c * n + j] = geom[(uint64_t)c * n + p];
+  ids_out[j] = ids[p];
+}
+
+// Stable sort of the storage by local id, then stable by location key: equal keys by
+// id, the order the reference's Create yields from entities taken in id order.
+static void make_reference_order(const Tree *t) {
+  Ctx *ctx = t->ctx;
+  const uint64_t n = t->n;
+  uint64_t max_n = 1;
+  for (const SetParams &sp : t->sets) max_n = std::max<uint64_t>(max_n, sp.ent_cnt);
+  Buf<uint64_t> keys(ctx, n);
+  Buf<uint32_t> pos(ctx, n);
+  launch(ctx, "ref_ids", k_ref_ids, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)t->d_ids, n, keys.p, pos.p);
+  radix_sort_pairs(ctx, keys.p, pos.p, n, 64 - __builtin_clzll(max_n));
+  launch(ctx, "ref_keys", k_ref_keys, dim3(div_up(n, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys,
+         (const uint32_t *)pos.p, n, keys.p);
+  radix_sort_pairs(ctx, keys.p, pos.p, n, t->layout.total);
+  double *geom = (double *)ctx->alloc(n * t->dim * 8);
+  uint32_t *ids = (uint32_t *)ctx->alloc(n * 4);
+  switch (t->dim) {
+#define OTG_REF(DD)                                                                                                  \
+  case DD:                                                                                                           \
+    launch(ctx, "ref_gather", k_ref_gather<DD>, dim3(div_up(n, kB)), dim3(kB), 0, (const double *)t->d_geom,          \
+           (const uint32_t *)t->d_ids, (const uint32_t *)pos.p, n, geom, ids);                                      \
+    break;
+    OTG_REF(2)
+    OTG_REF(3)
+    OTG_REF(4)
+    OTG_REF(5)
+    OTG_REF(6)
+#undef OTG_REF
+  }
+  TreeView hv = view_of(t);
+  hv.geom = geom;
+  hv.ids = ids;
+  TreeView *dv = (TreeView *)ctx->alloc(sizeof(TreeView));
+  OTG_HIP_CHECK(hipMemcpyAsync(dv, &hv, sizeof(TreeView), hipMemcpyHostToDevice, ctx->stream));
+  ctx->sync();
+  t->d_geom_ref = geom;
+  t->d_ids_ref = ids;
+  t->d_view_ref = dv;
+}
+
+TreeView reference_view_of(const Tree *t) {
+  TreeView hv = view_of(t);
+  if (!t->fine_order || t->n == 0) return hv;
+  std::lock_guard<std::mutex> g(t->ref_mu);
+  if (!t->d_view_ref) make_reference_order(t);
+  hv.geom = t->d_geom_ref;
+  hv.ids = t->d_ids_ref;
+  return hv;
+}
+
+const TreeView *reference_device_view(const Tree *t) {
+  if (!t->fine_order || t->n == 0) return t->d_view;
+  reference_view_of(t);
+  return t->d_view_ref;
 }
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets, const double *boxes,

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -82,6 +83,12 @@ class Ctx {
   void *alloc(size_t bytes);
   void free(void *p);
   void release_cache();
+
+  // Error word kernels set when a bounded device-wide wait runs out, and its pinned
+  // host copy.  sync() waits for the stream and raises such an error.
+  uint32_t *d_err = nullptr;
+  uint32_t *h_err = nullptr;
+  void sync();
 
   hipEvent_t get_event();
   void begin_kernel(const char *name, hipEvent_t *start);
@@ -214,6 +221,18 @@ struct Tree {
   // leaves over capacity at max depth (kNN ids depend on distances only).
   Tree *knn_tree = nullptr;
   TreeView *d_view = nullptr;  // device copy of this tree's TreeView
+  // A finer index sorted together with its reference tree shares that tree's entity
+  // storage (d_ids, d_geom), which the reference tree owns.
+  bool shares_entities = false;
+  // The reference tree of a shared sort keeps equal location keys in the finer
+  // index's order instead of by id.  Ray-first (whose bound depends on the order
+  // hits are met), the export and the archive use the reference's order (equal keys
+  // by id), made on first use and kept: ids, geometry and a device view of them.
+  bool fine_order = false;
+  mutable std::mutex ref_mu;
+  mutable uint32_t *d_ids_ref = nullptr;
+  mutable double *d_geom_ref = nullptr;
+  mutable TreeView *d_view_ref = nullptr;
 
   ~Tree();
 };
@@ -221,6 +240,10 @@ struct Tree {
 // Host TreeView of a built tree, and its device copy in t->d_view (and the finer index's).
 TreeView view_of(const Tree *t);
 void publish_views(Tree *t);
+// The tree's view with its entity storage in the reference's order (equal keys by
+// id): the tree's own view unless fine_order, else made on first use.
+TreeView reference_view_of(const Tree *t);
+const TreeView *reference_device_view(const Tree *t);
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets,
                   const double *boxes, const uint32_t *depths, const uint32_t *caps);

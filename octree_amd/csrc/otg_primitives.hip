@@ -216,36 +216,83 @@ void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, ui
 }
 
 // ---------------------------------------------------------------------------
-// Radix sort
+// Radix sort: LSD, 8-bit digits, one launch per digit ("onesweep").
+//
+// One histogram launch counts every digit position at once; a one-block launch
+// turns the counts into each digit's global start.  Each scatter launch then
+// ranks a 2048-key tile in LDS (wavefront ballots), publishes the tile's 256
+// digit counts and finds the counts of all earlier tiles by decoupled look-back:
+// a tile's status word per digit holds 30 bits of count and a flag, AGG (this
+// tile's own count) or INC (count of this tile and every earlier one).  Tile ids
+// come from an atomic counter, so every tile a block waits on belongs to a block
+// that started earlier and publishes its AGG without waiting on anything.  The
+// wait is bounded all the same: past kOsSpinLimit polls the block records an
+// error (the host raises it) and carries on.  Status words live in two buffers
+// used by alternate passes; a pass clears the other buffer's words of its tile
+// for the pass after it (the first pass's buffer is cleared by the histogram
+// launch's predecessor).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_rs_hist(const uint64_t *__restrict__ keys, uint64_t n, uint32_t shift,
-                                                     uint32_t *__restrict__ hist, uint32_t nblocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * kTile;
-#pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+constexpr uint32_t kOsAgg = 1u << 30, kOsInc = 2u << 30, kOsCount = (1u << 30) - 1u;
+constexpr uint32_t kOsMaxPasses = 8;
+constexpr uint32_t kOsSpinLimit = 1u << 22;
+
+struct OsState {
+  uint32_t base[kOsMaxPasses][256];  // digit counts, then each digit's global start
+  uint32_t tile_counter[kOsMaxPasses];
+};
+
+__global__ void __launch_bounds__(kBlock) k_os_init(OsState *__restrict__ st, uint32_t *__restrict__ status0,
+                                                     uint64_t nstatus) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint32_t *w = reinterpret_cast<uint32_t *>(st);
+  for (uint64_t i = t; i < sizeof(OsState) / 4; i += stride) w[i] = 0u;
+  for (uint64_t i = t; i < nstatus; i += stride) status0[i] = 0u;
 }
 
-__global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+__global__ void __launch_bounds__(kBlock) k_os_hist(const uint64_t *__restrict__ keys, uint64_t n, uint32_t passes,
+                                                     OsState *__restrict__ st) {
+  __shared__ uint32_t h[kOsMaxPasses][256];
+  for (uint32_t p = 0; p < passes; ++p) h[p][threadIdx.x] = 0u;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t k = keys[i];
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&h[p][(uint32_t)(k >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = 0; p < passes; ++p)
+    if (h[p][threadIdx.x]) atomicAdd(&st->base[p][threadIdx.x], h[p][threadIdx.x]);
+}
+
+// digit counts -> exclusive starts, one pass per wavefront-sized step of a block scan
+__global__ void __launch_bounds__(kBlock) k_os_prefix(OsState *__restrict__ st, uint32_t passes) {
+  __shared__ uint64_t lds[kBlock / 64];
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint64_t c = st->base[p][threadIdx.x];
+    uint64_t ex, tot;
+    block_scan<OpAdd64>(c, &ex, &tot, lds);
+    st->base[p][threadIdx.x] = (uint32_t)ex;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_os_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                         uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                        uint64_t n, uint32_t shift,
-                                                        const uint32_t *__restrict__ offs, uint32_t nblocks) {
+                                                        uint64_t n, uint32_t pass, OsState *__restrict__ st,
+                                                        uint32_t *__restrict__ status, uint32_t *__restrict__ status_next,
+                                                        uint32_t *__restrict__ err) {
   __shared__ uint32_t wcnt[kBlock / 64][256];
   __shared__ uint32_t base_off[256];
+  __shared__ uint32_t s_tile;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const uint32_t shift = 8 * pass;
+  if (threadIdx.x == 0) s_tile = atomicAdd(&st->tile_counter[pass], 1u);
   for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
-  base_off[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
   __syncthreads();
-
-  const uint64_t tile = (uint64_t)blockIdx.x * kTile + (uint64_t)wave * (64 * kItems);
+  const uint32_t tile_id = s_tile;
+  const uint64_t tile0 = (uint64_t)tile_id * kTile;
+  const uint64_t tile = tile0 + (uint64_t)wave * (64 * kItems);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint64_t key[kItems];
   uint32_t val[kItems];
@@ -274,9 +321,7 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restric
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  // per digit: waves' offsets inside the tile's run, and the tile's run start
-  // (exclusive scan of the tile's digit counts over the 256 digits)
-  __shared__ uint32_t tstart[256];
+  // per digit: waves' offsets inside the tile's run, and the tile's count
   uint32_t run = 0;
   {
     const uint32_t t = threadIdx.x;
@@ -287,9 +332,40 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restric
       run += c;
     }
   }
+  // look-back: this tile's start of digit d among all tiles
+  {
+    const uint32_t d = threadIdx.x;
+    uint32_t *mine = &status[(uint64_t)tile_id * 256 + d];
+    uint32_t excl = 0;
+    if (tile_id == 0) {
+      __hip_atomic_store(mine, kOsInc | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(mine, kOsAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint64_t j = tile_id - 1;
+      uint32_t spins = 0;
+      for (;;) {
+        const uint32_t sv = __hip_atomic_load(&status[j * 256 + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((sv & ~kOsCount) == 0u) {
+          if (++spins > kOsSpinLimit) {
+            atomicOr(err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += sv & kOsCount;
+        if (sv & kOsInc) break;
+        --j;  // AGG: the tile before j has published too, or will
+      }
+      __hip_atomic_store(mine, kOsInc | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    base_off[d] = st->base[pass][d] + excl;
+    status_next[(uint64_t)tile_id * 256 + d] = 0u;
+  }
+  // tile-local start of each digit (exclusive scan of the tile counts in digit order)
+  __shared__ uint32_t tstart[256];
   {
     __shared__ uint32_t wsum[kBlock / 64];
-    // block-wide exclusive scan of `run` (one digit per thread) in digit order
     uint32_t x = run;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -317,7 +393,6 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restric
     }
   }
   __syncthreads();
-  const uint64_t tile0 = (uint64_t)blockIdx.x * kTile;
   const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)kTile ? (n - tile0) : (uint64_t)kTile);
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
@@ -334,19 +409,24 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint64_t *__restric
 
 void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits) {
   if (n <= 1 || bits == 0) return;
-  uint32_t passes = (bits + 7) / 8;
-  uint32_t nb = div_up(n, kTile);
+  if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
+  const uint32_t passes = std::min<uint32_t>((bits + 7) / 8, kOsMaxPasses);
+  const uint32_t ntiles = div_up(n, kTile);
   Buf<uint64_t> k2(ctx, n);
   Buf<uint32_t> v2(ctx, n);
-  Buf<uint32_t> hist(ctx, (size_t)256 * nb);
+  Buf<OsState> st(ctx, 1);
+  Buf<uint32_t> status(ctx, (size_t)2 * 256 * ntiles);
+  uint32_t *stat[2] = {status.p, status.p + (size_t)256 * ntiles};
+  launch(ctx, "radix_init", k_os_init, dim3(std::min<uint32_t>(div_up((uint64_t)256 * ntiles, kBlock), 1024)),
+         dim3(kBlock), 0, st.p, stat[0], (uint64_t)256 * ntiles);
+  launch(ctx, "radix_hist", k_os_hist, dim3(std::min<uint32_t>(div_up(n, kBlock), 2048)), dim3(kBlock), 0,
+         (const uint64_t *)keys, n, passes, st.p);
+  launch(ctx, "radix_prefix", k_os_prefix, dim3(1), dim3(kBlock), 0, st.p, passes);
   uint64_t *ka = keys, *kb = k2.p;
   uint32_t *va = vals, *vb = v2.p;
   for (uint32_t p = 0; p < passes; ++p) {
-    uint32_t shift = p * 8;
-    launch(ctx, "radix_hist", k_rs_hist, dim3(nb), dim3(kBlock), 0, (const uint64_t *)ka, n, shift, hist.p, nb);
-    exclusive_scan_u32(ctx, hist.p, hist.p, (uint64_t)256 * nb, nullptr);
-    launch(ctx, "radix_scatter", k_rs_scatter, dim3(nb), dim3(kBlock), 0, (const uint64_t *)ka, (const uint32_t *)va,
-           kb, vb, n, shift, (const uint32_t *)hist.p, nb);
+    launch(ctx, "radix_scatter", k_os_scatter, dim3(ntiles), dim3(kBlock), 0, (const uint64_t *)ka,
+           (const uint32_t *)va, kb, vb, n, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err);
     std::swap(ka, kb);
     std::swap(va, vb);
   }

@@ -2478,7 +2478,7 @@ static void stage_inputs(Ctx *ctx, DevInput &in, const double *q, const uint32_t
       launch(ctx, "check_sets", k_max_u32, dim3(std::min<uint64_t>(div_up(nq, 256), 1024)), dim3(256), 0, sets, nq,
              dm.p);
       OTG_HIP_CHECK(hipMemcpyAsync(&mx, dm.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-      OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      ctx->sync();
     } else {
       mx = *std::max_element(sets, sets + nq);
     }
@@ -2519,7 +2519,7 @@ static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, 
   exclusive_scan_u32(ctx, cnt, scan.p, nq, total.p);
   uint32_t ht = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   check_id_total(ht, r);
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
@@ -2560,7 +2560,7 @@ static void csr_into(Ctx *ctx, Arena *ar, const uint32_t *ids, const uint32_t *c
   exclusive_scan_u32(ctx, cnt, scan.p, nq, total.p, ar);
   uint32_t ht = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   check_id_total(ht);
   co.nids = ht;
   if (ht > co.capacity) throw Error{OTG_ERR_CAPACITY, "the result holds more ids than the output capacity"};
@@ -2584,7 +2584,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
          (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
   uint32_t hovf[1] = {0};
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   if (footprint) {
     Buf<unsigned long long> acc(ctx, 2);
     OTG_HIP_CHECK(hipMemsetAsync(acc.p, 0, 16, ctx->stream));
@@ -2592,7 +2592,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
            (const uint32_t *)order.p, nq, (const double *)Fq.p, acc.p);
     unsigned long long h[2] = {0, 0};
     OTG_HIP_CHECK(hipMemcpyAsync(h, acc.p, 16, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->sync();
     footprint[0] += h[0];
     footprint[1] += h[1];
   }
@@ -2612,7 +2612,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p, ar);
   uint32_t hb = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   check_id_total(hb);
   OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
   Buf<double> bd(ctx, ar, std::max<uint32_t>(hb, 1));
@@ -2638,7 +2638,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   }
   launch(ctx, "knn_band_copy", k_band_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
          (const uint32_t *)boff.p, (const uint32_t *)bi.p, (const uint64_t *)offsets, out_ids);
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the staged count above lives on this stack frame
+  ctx->sync();  // the staged count above lives on this stack frame
   return r;
 }
 
@@ -2722,7 +2722,7 @@ void query_knn_noalloc(const Tree *t, uint64_t nq, const double *pts, const uint
            dm.p);
     uint32_t mx = 0;
     OTG_HIP_CHECK(hipMemcpyAsync(&mx, dm.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->sync();
     if (mx >= x->sets.size()) throw Error{OTG_ERR_INVALID_ARG, "a query's set id is not a set of the tree"};
   }
   co.nids = 0;
@@ -2732,7 +2732,7 @@ void query_knn_noalloc(const Tree *t, uint64_t nq, const double *pts, const uint
     knn_dispatch(ctx, x->dim, x->kind == OTG_KIND_BOX, single_view(x), nq, pts, sets, nullptr, k, k,
                  knn_F0(max_within, tol), tol, nullptr, &ar, &co);
   }
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
 }
 
 // One-lane walks stop counting after this many node visits; such queries, and those
@@ -2745,7 +2745,7 @@ static uint32_t flagged_list(Ctx *ctx, const uint32_t *flags, const uint32_t *or
   exclusive_scan_u32(ctx, flags, pos.p, nq, total.p);
   uint32_t n = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&n, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   list = Buf<uint32_t>(ctx, std::max<uint32_t>(n, 1));
   launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, flags,
          (const uint32_t *)pos.p, nq, order, list.p);
@@ -2768,7 +2768,7 @@ static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n
   exclusive_scan_u32(ctx, hc.p, hpos.p, n, htot.p);
   uint32_t nh = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&nh, htot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   check_id_total(nh);
   if (nh < 2) return;
   Buf<uint64_t> keys(ctx, nh);
@@ -2815,7 +2815,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   uint32_t ht = 0, hcopy = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   check_id_total(ht, r);
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
@@ -2842,7 +2842,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
            (const uint64_t *)r->d_offsets, r->d_ids, ncopy.p, copies.p, (uint32_t *)nullptr, 0u);
   // 5. long contained runs, one wavefront each
   OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   launch(ctx, "range_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
          r->d_ids);
   // 6. the larger results sorted by id
@@ -2889,7 +2889,7 @@ static Result *count_fill_csr(Ctx *ctx, uint64_t nq, Count count, Fill fill) {
   exclusive_scan_u32(ctx, cnt.p, scan.p, nq, total.p);
   uint32_t ht = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&ht, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   check_id_total(ht, r);
   r->nids = ht;
   r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
@@ -3094,14 +3094,14 @@ static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
          (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p, redo.p, ovf.p + 1);
   uint32_t hovf[3] = {0, 0, 0};
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   if (hovf[1]) {
     sort_positions(ctx, redo.p, hovf[1], nq);
     launch(ctx, "ray_deep", k_ray_deep<D, BOX, RCAP>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
            qs, (const uint32_t *)order.p, (const uint32_t *)redo.p, hovf[1], tol, inc, max_dist, ids.p, cnt.p, ovf.p,
            band.p, Mq.p, ovf.p + 2);
     OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->sync();
     if (hovf[2]) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
   }
   const uint32_t nb = hovf[0];
@@ -3121,7 +3121,7 @@ static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
     uint32_t hfail = 0;
     OTG_HIP_CHECK(hipMemcpyAsync(h.data(), bcnt.p, (uint64_t)nb * 4, hipMemcpyDeviceToHost, ctx->stream));
     OTG_HIP_CHECK(hipMemcpyAsync(&hfail, fail.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->sync();
     if (hfail) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
     bool all = true;
     for (uint32_t v : h) all &= v != 0xFFFFFFFFu;
@@ -3162,7 +3162,9 @@ Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t
   Ctx *ctx = run_ctx(t);
   DevInput in;
   stage_inputs(ctx, in, rays, sets, nq * 2 * t->dim, nq, mem, (uint32_t)t->sets.size());
-  return ray_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, tol, inc, max_dist);
+  ViewSet vs = single_view(t);
+  vs.views = reference_device_view(t);  // hit order within a leaf: equal keys by id
+  return ray_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, vs, nq, in.q, in.sets, tol, inc, max_dist);
 }
 
 // One launch over queries on several trees of the same kind and dimension
@@ -3178,7 +3180,7 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
   for (const Tree *t : trees) {
     if (t->kind != kind || t->dim != dim) throw Error{OTG_ERR_INVALID_ARG, "trees of one launch differ in kind or dim"};
     const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : t);
-    hv.push_back(view_of(x));
+    hv.push_back(query == 2 ? reference_view_of(x) : view_of(x));
     vs.cbits = std::max(vs.cbits, x->layout.cbits);
     vs.sbits = std::max(vs.sbits, x->layout.sbits);
     total_n += x->n;
@@ -3208,7 +3210,7 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
   else if (query == 1) r = range_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, mode, p0);
   else r = ray_dispatch(ctx, dim, kind == OTG_KIND_BOX, vs, nq, dq.p, nullptr, p0, p1, p2);
   // the staged host arrays must outlive the copies: wait before returning
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   return r;
 }
 
@@ -3230,7 +3232,7 @@ static Result *collide_impl(const Tree *lt, uint32_t lset, const Tree *rt, uint3
   exclusive_scan_u32(ctx, cnt.p, scan.p, ne, total.p);
   uint32_t np = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&np, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  OTG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  ctx->sync();
   // the pair list holds 2 ids per pair, indexed with 32-bit positions
   if (np == kScanOverflow || np > 0x7FFFFFFFu) {
     delete r;
