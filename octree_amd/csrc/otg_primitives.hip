@@ -235,6 +235,7 @@ void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, ui
 constexpr uint32_t kOsAgg = 1u << 30, kOsInc = 2u << 30, kOsCount = (1u << 30) - 1u;
 constexpr uint32_t kOsMaxPasses = 8;
 constexpr uint32_t kOsSpinLimit = 1u << 22;
+constexpr uint32_t kOsLook = 8;  // predecessor status words read per look-back step
 
 struct OsState {
   uint32_t base[kOsMaxPasses][256];  // digit counts, then each digit's global start
@@ -276,22 +277,34 @@ __global__ void __launch_bounds__(kBlock) k_os_prefix(OsState *__restrict__ st, 
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_os_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                                        uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                        uint64_t n, uint32_t pass, OsState *__restrict__ st,
-                                                        uint32_t *__restrict__ status, uint32_t *__restrict__ status_next,
-                                                        uint32_t *__restrict__ err) {
-  __shared__ uint32_t wcnt[kBlock / 64][256];
+// Scatter tile: OB threads (OB / 64 wavefronts) x 8 keys = 4096 keys, so a digit's
+// run in the output averages 16 keys (a whole 128-byte line of keys).
+constexpr int kOsBlock = 512;
+constexpr int kOsWaves = kOsBlock / 64;
+constexpr int kOsTile = kOsBlock * kItems;
+
+__global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                          uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                          uint64_t n, uint32_t pass, OsState *__restrict__ st,
+                                                          uint32_t *__restrict__ status, uint32_t *__restrict__ status_next,
+                                                          uint32_t *__restrict__ err) {
+  __shared__ uint32_t wcnt[kOsWaves][256];
   __shared__ uint32_t base_off[256];
+  __shared__ uint32_t tstart[256];
+  __shared__ uint32_t wsum[4];
   __shared__ uint32_t s_tile;
+  __shared__ uint64_t skey[kOsTile];
+  __shared__ uint32_t sval[kOsTile];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const uint32_t shift = 8 * pass;
+  const bool digit_thread = threadIdx.x < 256;  // threads 0..255 own one digit each
   if (threadIdx.x == 0) s_tile = atomicAdd(&st->tile_counter[pass], 1u);
-  for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
+  if (digit_thread)
+    for (int w = 0; w < kOsWaves; ++w) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t tile_id = s_tile;
-  const uint64_t tile0 = (uint64_t)tile_id * kTile;
+  const uint64_t tile0 = (uint64_t)tile_id * kOsTile;
   const uint64_t tile = tile0 + (uint64_t)wave * (64 * kItems);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint64_t key[kItems];
@@ -323,17 +336,15 @@ __global__ void __launch_bounds__(kBlock) k_os_scatter(const uint64_t *__restric
   __syncthreads();
   // per digit: waves' offsets inside the tile's run, and the tile's count
   uint32_t run = 0;
-  {
+  if (digit_thread) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < kOsWaves; ++w) {
       const uint32_t c = wcnt[w][t];
       wcnt[w][t] = run;
       run += c;
     }
-  }
-  // look-back: this tile's start of digit d among all tiles
-  {
+    // look-back: this tile's start of digit d among all tiles
     const uint32_t d = threadIdx.x;
     uint32_t *mine = &status[(uint64_t)tile_id * 256 + d];
     uint32_t excl = 0;
@@ -341,31 +352,41 @@ __global__ void __launch_bounds__(kBlock) k_os_scatter(const uint64_t *__restric
       __hip_atomic_store(mine, kOsInc | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       __hip_atomic_store(mine, kOsAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint64_t j = tile_id - 1;
+      // each step reads up to kOsLook predecessors at once (independent loads in
+      // flight together) and takes the published prefix nearest to this tile
+      uint64_t j = tile_id;  // tiles j-1, j-2, ... are still to be taken
       uint32_t spins = 0;
       for (;;) {
-        const uint32_t sv = __hip_atomic_load(&status[j * 256 + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((sv & ~kOsCount) == 0u) {
+        uint32_t sv[kOsLook];
+        const uint32_t m = j < kOsLook ? (uint32_t)j : kOsLook;
+#pragma unroll
+        for (uint32_t q = 0; q < kOsLook; ++q)
+          sv[q] = q < m ? __hip_atomic_load(&status[(j - 1 - q) * 256 + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : 0u;
+        uint32_t used = 0;
+        bool done = false;
+#pragma unroll
+        for (uint32_t q = 0; q < kOsLook; ++q) {
+          if (done || q != used || q >= m || (sv[q] & ~kOsCount) == 0u) continue;
+          excl += sv[q] & kOsCount;
+          used = q + 1;
+          done = (sv[q] & kOsInc) != 0u;
+        }
+        if (done) break;
+        j -= used;
+        if (used == 0) {  // the nearest remaining tile has not published yet
           if (++spins > kOsSpinLimit) {
             atomicOr(err, 1u);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          continue;
         }
-        excl += sv & kOsCount;
-        if (sv & kOsInc) break;
-        --j;  // AGG: the tile before j has published too, or will
       }
       __hip_atomic_store(mine, kOsInc | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     base_off[d] = st->base[pass][d] + excl;
     status_next[(uint64_t)tile_id * 256 + d] = 0u;
-  }
-  // tile-local start of each digit (exclusive scan of the tile counts in digit order)
-  __shared__ uint32_t tstart[256];
-  {
-    __shared__ uint32_t wsum[kBlock / 64];
+    // tile-local start of each digit (exclusive scan of the tile counts in digit order)
     uint32_t x = run;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -373,16 +394,17 @@ __global__ void __launch_bounds__(kBlock) k_os_scatter(const uint64_t *__restric
       if (lane >= o) x += y;
     }
     if (lane == 63) wsum[wave] = x;
-    __syncthreads();
+    tstart[threadIdx.x] = x - run;  // within the wavefront; the waves' sums are added below
+  }
+  __syncthreads();
+  if (digit_thread) {
     uint32_t add = 0;
     for (int w = 0; w < wave; ++w) add += wsum[w];
-    tstart[threadIdx.x] = add + x - run;
+    tstart[threadIdx.x] += add;
   }
   __syncthreads();
   // stage the tile in digit order in LDS, then write each digit's run with
   // consecutive threads so the global stores coalesce
-  __shared__ uint64_t skey[kTile];
-  __shared__ uint32_t sval[kTile];
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint64_t i = tile + (uint64_t)r * 64 + lane;
@@ -393,10 +415,10 @@ __global__ void __launch_bounds__(kBlock) k_os_scatter(const uint64_t *__restric
     }
   }
   __syncthreads();
-  const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)kTile ? (n - tile0) : (uint64_t)kTile);
+  const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)kOsTile ? (n - tile0) : (uint64_t)kOsTile);
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
-    const uint32_t j = (uint32_t)r * kBlock + threadIdx.x;
+    const uint32_t j = (uint32_t)r * kOsBlock + threadIdx.x;
     if (j < cnt) {
       const uint64_t k = skey[j];
       const uint32_t d = (uint32_t)(k >> shift) & 255u;
@@ -411,7 +433,7 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint
   if (n <= 1 || bits == 0) return;
   if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
   const uint32_t passes = std::min<uint32_t>((bits + 7) / 8, kOsMaxPasses);
-  const uint32_t ntiles = div_up(n, kTile);
+  const uint32_t ntiles = div_up(n, kOsTile);
   Buf<uint64_t> k2(ctx, n);
   Buf<uint32_t> v2(ctx, n);
   Buf<OsState> st(ctx, 1);
@@ -425,7 +447,7 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint
   uint64_t *ka = keys, *kb = k2.p;
   uint32_t *va = vals, *vb = v2.p;
   for (uint32_t p = 0; p < passes; ++p) {
-    launch(ctx, "radix_scatter", k_os_scatter, dim3(ntiles), dim3(kBlock), 0, (const uint64_t *)ka,
+    launch(ctx, "radix_scatter", k_os_scatter, dim3(ntiles), dim3(kOsBlock), 0, (const uint64_t *)ka,
            (const uint32_t *)va, kb, vb, n, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err);
     std::swap(ka, kb);
     std::swap(va, vb);
