@@ -375,13 +375,12 @@ __device__ __forceinline__ uint32_t internal_gen(const uint32_t *lm, const uint3
   return (uint32_t)__popc(anc) + 1u;
 }
 
-// flags per position p in [0, n): seg_flag (segment start), int_count
+// events per position p in [0, n): bit 0 segment start, bits 1-2 internal nodes
 // (extra root + internal representative)
 __global__ void __launch_bounds__(kB) k_event_flags(const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
                                                      const uint32_t *__restrict__ lm, uint64_t n,
                                                      const uint64_t *__restrict__ keys, KeyView kv,
-                                                     const SetParams *__restrict__ sets, uint32_t *__restrict__ seg_flag,
-                                                     uint32_t *__restrict__ int_flag) {
+                                                     const SetParams *__restrict__ sets, uint8_t *__restrict__ ev) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (p >= n) return;
   const uint32_t s = kv.set(keys[p]);
@@ -393,14 +392,13 @@ __global__ void __launch_bounds__(kB) k_event_flags(const uint8_t *__restrict__ 
     const uint32_t vp = v[p];
     if (!((lm[p - 1] >> vp) & 1u)) ++ic;  // leftmost boundary of its node
   }
-  seg_flag[p] = (set_start || active[p]) ? 1u : 0u;
-  int_flag[p] = ic;
+  ev[p] = (uint8_t)(((set_start || active[p]) ? 1u : 0u) | (ic << 1));
 }
 
-__global__ void __launch_bounds__(kB) k_seg_compact(const uint32_t *__restrict__ seg_flag, const uint32_t *__restrict__ seg_idx,
+__global__ void __launch_bounds__(kB) k_seg_compact(const uint8_t *__restrict__ ev, const uint64_t *__restrict__ idx,
                                                      uint64_t n, uint32_t *__restrict__ seg_list, uint32_t nseg) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (p < n && seg_flag[p]) seg_list[seg_idx[p]] = (uint32_t)p;
+  if (p < n && (ev[p] & 1u)) seg_list[idx[p] >> 32] = (uint32_t)p;
   if (p == 0) seg_list[nseg] = (uint32_t)n;
 }
 
@@ -438,17 +436,17 @@ struct NodeKeyLayout {
 
 constexpr uint32_t kTypeRoot = 0u, kTypeInternal = 1u, kTypeLeaf = 2u;
 
-__global__ void __launch_bounds__(kB) k_emit_internal(const uint32_t *__restrict__ int_flag, const uint32_t *__restrict__ int_idx,
+__global__ void __launch_bounds__(kB) k_emit_internal(const uint8_t *__restrict__ ev, const uint64_t *__restrict__ idx,
                                                        const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
                                                        const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
                                                        uint64_t n, const uint64_t *__restrict__ keys, KeyView kv,
                                                        const SetParams *__restrict__ sets, NodeKeyLayout nk,
                                                        uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (p >= n || !int_flag[p]) return;
+  if (p >= n || !(ev[p] >> 1)) return;
   const uint32_t s = kv.set(keys[p]);
   const SetParams &sp = sets[s];
-  uint32_t out = int_idx[p];
+  uint32_t out = (uint32_t)idx[p];
   const bool set_start = (sp.ent_off == p);
   if (set_start && sp.root_splits && sp.min_lcp > 1u) {
     nkeys[out] = nk.make(s, 0, 0);
@@ -882,13 +880,13 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
   level_scans(ctx, v.p, n + 1, lm.p, rm.p);
 
   // 6. node events
-  Buf<uint32_t> seg_flag(ctx, n), int_flag(ctx, n), seg_idx(ctx, n), int_idx(ctx, n);
+  Buf<uint8_t> ev(ctx, n);
+  Buf<uint64_t> eidx(ctx, n);
   launch(ctx, "event_flags", k_event_flags, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)v.p,
          (const uint8_t *)active.p, (const uint32_t *)lm.p, n, (const uint64_t *)t->d_keys, kv,
-         (const SetParams *)t->d_sets, seg_flag.p, int_flag.p);
+         (const SetParams *)t->d_sets, ev.p);
   Buf<uint32_t> totals(ctx, 4);
-  exclusive_scan_u32(ctx, seg_flag.p, seg_idx.p, n, totals.p + 0);
-  exclusive_scan_u32(ctx, int_flag.p, int_idx.p, n, totals.p + 1);
+  event_scan(ctx, ev.p, eidx.p, n, totals.p);
   uint32_t ht[2];
   OTG_HIP_CHECK(hipMemcpyAsync(ht, totals.p, 8, hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
@@ -897,8 +895,8 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
   const uint32_t nseg = ht[0], nint = ht[1];
 
   Buf<uint32_t> seg_list(ctx, (size_t)nseg + 1), seg_node(ctx, nseg), seg_node_idx(ctx, nseg);
-  launch(ctx, "seg_compact", k_seg_compact, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)seg_flag.p,
-         (const uint32_t *)seg_idx.p, n, seg_list.p, nseg);
+  launch(ctx, "seg_compact", k_seg_compact, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)ev.p,
+         (const uint64_t *)eidx.p, n, seg_list.p, nseg);
   launch(ctx, "seg_nodes", k_seg_nodes<BOX>, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
          (const uint8_t *)v.p, (const uint64_t *)t->d_keys, d_off, nsets, (const SetParams *)t->d_sets, kv, seg_node.p);
   exclusive_scan_u32(ctx, seg_node.p, seg_node_idx.p, nseg, totals.p + 2);
@@ -914,8 +912,8 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
   if (nk.cobits + nk.gbits + L.sbits > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "node key overflow"};
   Buf<uint64_t> nkeys(ctx, nnodes);
   Buf<uint32_t> ndesc(ctx, nnodes);
-  launch(ctx, "emit_internal", k_emit_internal, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)int_flag.p,
-         (const uint32_t *)int_idx.p, (const uint8_t *)v.p, (const uint8_t *)active.p, (const uint32_t *)lm.p,
+  launch(ctx, "emit_internal", k_emit_internal, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)ev.p,
+         (const uint64_t *)eidx.p, (const uint8_t *)v.p, (const uint8_t *)active.p, (const uint32_t *)lm.p,
          (const uint32_t *)rm.p, n, (const uint64_t *)t->d_keys, kv, (const SetParams *)t->d_sets, nk, nkeys.p,
          ndesc.p);
   launch(ctx, "emit_leaves", k_emit_leaves, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
