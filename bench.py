@@ -4,7 +4,8 @@ Workload (default, one MI355X): 256 independent 3-D point sets, sizes log-unifor
 in [1e3, 1e6] (~30M points), each a mixture of 8 isotropic Gaussians
 (sigma U[2,20]) clipped to [0,1000]^3, FP64; per set 50,000 kNN queries (k=16,
 drawn from the set's mixture) and 50,000 RangeSearch boxes (edge U[1,30]).
-Leaf capacity is picked by a short autotune over {8,16,32,64}; max depth is the
+Leaf capacity and the query settings come from the library's autotuner (otg_autotune,
+leaf capacity over {8,16,32,64}); max depth is the
 reference's EstimateMaxDepth per set.
 
 One step = build all 256 trees (one forest build) + all kNN + all range queries,
@@ -89,6 +90,25 @@ def make_workload(nsets=256, nq=50_000, seed=1234, lo=3.0, hi=6.0, size_scale=1.
     }
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def make_point_1m():
     """OctreePoint: 1,000,000 points U[0,100)^3 and 10,000 kNN query points in the same box."""
     rng = np.random.default_rng(11)
@@ -144,10 +164,12 @@ def main():
     ap.add_argument("--nsets", type=int, default=256)
     ap.add_argument("--nq", type=int, default=50_000)
     ap.add_argument("--k", type=int, default=16)
-    ap.add_argument("--cap", type=int, default=0, help="0 = autotune over {8,16,32,64}")
+    ap.add_argument("--cap", type=int, default=0, help="0 = the library autotuner picks it (otg_autotune)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-queries-per-set", type=int, default=2000,
+                    help="batched workload: kNN queries per set in the CPU baseline's stratified sample")
     ap.add_argument("--requests", type=int, default=1500, help="served workload: request count")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (the known-size kernel that scales PMC byte counters)")
@@ -246,23 +268,16 @@ def main():
         if timings is not None:
             timings.append((tb, tk, tr, time.perf_counter() - T0))
 
-    # ---- autotune leaf capacity (batched step time) ----
-    caps = [args.cap] if args.cap else [8, 16, 32, 64]
+    # ---- autotune (the library's otg_autotune: leaf capacity, finer-index capacity,
+    # kNN occupancy, query order) on the step's own workload ----
     tune = {}
-    if len(caps) > 1:
-        for c in caps:
-            tt = []
-            f, rk, rr = batched_step(c, tt)
-            rk.free(); rr.free(); f.free()
-            tt = []
-            f, rk, rr = batched_step(c, tt)
-            rk.free(); rr.free(); f.free()
-            tune[c] = tt[0][3]
-            log(f"autotune cap={c}: step {tt[0][3] * 1e3:.1f} ms (build {tt[0][0] * 1e3:.1f}, knn {tt[0][1] * 1e3:.1f}, "
-                f"range {tt[0][2] * 1e3:.1f})")
-        cap = min(tune, key=tune.get)
+    if args.cap:
+        cap = args.cap
     else:
-        cap = caps[0]
+        t0 = time.perf_counter()
+        tune = ctx.autotune("point", 3, offs, d_pts, box, None, d_knn, d_sid, args.k, d_rng, d_sid)
+        cap = int(tune["leaf_capacity"])
+        log(f"autotune ({time.perf_counter() - t0:.1f}s): {tune}")
     log(f"leaf capacity: {cap}")
 
     # ---- batched timed steps ----
@@ -292,15 +307,20 @@ def main():
     kt = ctx.kernel_times()
     rk.free(); rr.free(); f.free()
     ctx.set_profiling(False)
-    build_kernels = {k: v for k, v in kt.items()}
     log("kernel times (ms, launches): " + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in sorted(kt.items(), key=lambda x: -x[1][0])))
 
-    # build roofline: algorithmic bytes per point = input 24 + key/id write 12 + 24 per radix pass
-    # + boundary scan 8 + id write 4 (+ node records), over the build kernels
-    bits = 5 * 3 + 8  # per-set max depth 5 (sets <= 1e6 points, cap >= 8) + 8 set bits
+    # build roofline: B = N [24 (read the points) + 12 (write key, id) + 24 P (P radix passes of
+    # 12-byte pairs, read and write) + 8 (boundary scan of the keys) + 4 (write the id array)]
+    # + M x 37 (node records: entity segment 8, child begin 4, depth 1, MinPoint 24), with P the
+    # 8-bit passes over the varying key bits: 3 x (largest depth EstimateMaxDepth gives at this
+    # capacity) + 8 set bits.  Time: the build's median wall-clock time in the timed steps.
+    fb = octree_amd.StaticOctreePoint.forest(None, cap, None, box, ctx=ctx, device_geom=(d_pts, offs))
+    nnodes = int(fb.info().node_count)
+    maxM = max(int(fb.GetMaxDepthID(s)) for s in range(nsets))
+    fb.free()
+    bits = 3 * maxM + int(math.ceil(math.log2(nsets)))
     passes = math.ceil(bits / 8)
-    build_names = [k for k in kt if k not in ("knn", "range_count", "range_fill", "compact_fixed", "offsets_u64")]
-    build_ms = sum(kt[k][0] for k in build_names)
+    build_bytes = N * (24 + 12 + 24 * passes + 8 + 4) + nnodes * 37
 
     out = {
         "metric": METRIC,
@@ -318,12 +338,18 @@ def main():
         "config": {"workload": "batched uneven build+query: 256 sets, sizes log-uniform [1e3,1e6], 8-Gaussian mixtures "
                                "in [0,1000]^3, 50k kNN (k=16) + 50k RangeSearch (edge 1-30) per set",
                    "total_points": N, "knn_queries": Q, "range_queries": Q, "k": args.k, "leaf_capacity": cap,
-                   "autotune_step_s": tune, "parallelism": "single GPU"},
+                   "tuning": tune, "parallelism": "single GPU"},
         "batched": {"build_mpts_s": N / b_build / 1e6, "knn_qps": Q / b_knn, "range_qps": Q / b_range,
                     "step_ms": b_step * 1e3, "build_ms": b_build * 1e3, "knn_ms": b_knn * 1e3, "range_ms": b_range * 1e3,
                     "p50_step_ms": float(np.percentile([x[3] for x in bt], 50)) * 1e3,
                     "p99_step_ms": float(np.percentile([x[3] for x in bt], 99)) * 1e3},
         "kernel_ms": {k: v[0] for k, v in kt.items()},
+        "build_roofline": {"bound": "hbm", "algorithmic_bytes": build_bytes, "radix_passes": passes,
+                           "key_bits": bits, "nodes": nnodes, "build_ms": b_build * 1e3,
+                           "achieved": build_bytes / b_build / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": build_bytes / b_build / 1e9 / HBM_PEAK_GBS,
+                           "note": "time covers the reference tree and the finer kNN/range index built from "
+                                   "the same sort; bytes count the reference tree's build"},
     }
 
     if not args.no_plain and not args.profile_only:
@@ -362,34 +388,32 @@ def main():
                        "avg_launch_ms": avg_ms,
                        "traffic_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9}
 
-    # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
+    # ---- CPU baseline: the oracle (a C port of the reference's algorithm) on the host
+    # cores: every set built with the PAR path's parallel encode + sort (serial BFS, as
+    # the reference), then a stratified kNN sample (the first queries of every set) on
+    # the same threads ----
     if not args.no_cpu and not args.profile_only:
         from oracle import oracle as O
 
-        threads = min(16, os.cpu_count() or 1)
-        done_q, t_q, done_p, t_b = 0, 0.0, 0, 0.0
-        s = 0
-        t_start = time.perf_counter()
-        chunk = 4096
-        while s < nsets and time.perf_counter() - t_start < args.cpu_seconds:
+        threads = cpu_threads()
+        per_set = max(1, min(nq, args.cpu_queries_per_set))
+        t0 = time.perf_counter()
+        trees = []
+        for s in range(nsets):
             a, b = int(offs[s]), int(offs[s + 1])
-            t0 = time.perf_counter()
-            tree = O.OracleTree(3, "point", W["points"][a:b], W["box"], 0, cap)
-            t1 = time.perf_counter()
-            t_b += t1 - t0
-            done_p += b - a
-            for c0 in range(0, nq, chunk):
-                t2 = time.perf_counter()
-                tree.knn(W["knn"][s * nq + c0:s * nq + min(nq, c0 + chunk)], args.k, threads=threads)
-                t_q += time.perf_counter() - t2
-                done_q += min(nq, c0 + chunk) - c0
-                if time.perf_counter() - t_start > args.cpu_seconds:
-                    break
-            s += 1
-        out["cpu_baseline"] = {"value": done_q / t_q, "unit": "kNN queries/s", "cores": threads, "kind": "port",
-                               "sample": f"first {s} of {nsets} sets ({done_p} points built, {done_q} kNN k={args.k}); "
-                                         f"build single-threaded {done_p / t_b / 1e6:.2f} Mpts/s",
-                               "build_mpts_s": done_p / t_b / 1e6}
+            trees.append(O.OracleTree(3, "point", W["points"][a:b], W["box"], 0, cap, threads=threads))
+        t_b = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for s in range(nsets):
+            trees[s].knn(W["knn"][s * nq:s * nq + per_set], args.k, threads=threads)
+        t_q = time.perf_counter() - t0
+        del trees
+        out["cpu_baseline"] = {"value": nsets * per_set / t_q, "unit": "kNN queries/s", "cores": threads,
+                               "kind": "port", "cpu": cpu_model(),
+                               "sample": f"all {nsets} sets built ({N} points, parallel encode+sort on {threads} "
+                                         f"threads, {N / t_b / 1e6:.1f} Mpts/s); kNN k={args.k} on the first "
+                                         f"{per_set} queries of every set ({nsets * per_set} queries)",
+                               "build_mpts_s": N / t_b / 1e6, "build_s": t_b, "knn_s": t_q}
     print(json.dumps(out), flush=True)
 
 
@@ -415,7 +439,7 @@ def run_single(args, octree_amd, torch):
 
     dev = torch.device("cuda:0")
     ctx = octree_amd.Context(0)
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     reps = max(1, args.steps)
     out = {"metric": METRIC, "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)"}
@@ -432,13 +456,13 @@ def run_single(args, octree_amd, torch):
                     "node_count": tree.GetNodeCount()})
         if not args.no_cpu:
             t0 = time.perf_counter()
-            o = O.OracleTree(3, "point", pts, box, 10, 16)
+            o = O.OracleTree(3, "point", pts, box, 10, 16, threads=threads)
             cb = time.perf_counter() - t0
             t0 = time.perf_counter()
             o.knn(qs, 8, threads=threads)
             ck = time.perf_counter() - t0
             out["cpu_baseline"] = {"value": len(qs) / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
-                                   "sample": "whole workload", "build_mpts_s": 1.0 / cb, "build_cores": 1}
+                                   "sample": "whole workload", "build_mpts_s": 1.0 / cb, "cpu": cpu_model()}
     elif args.workload == "octree_box_4m":
         boxes, rays, rng = make_box_4m()
         n = boxes.shape[0]
@@ -497,7 +521,7 @@ def run_single(args, octree_amd, torch):
         right.free()
         if not args.no_cpu:
             t0 = time.perf_counter()
-            o = O.OracleTree(3, "box", boxes[:m], None, 8, 32)
+            o = O.OracleTree(3, "box", boxes[:m], None, 8, 32, threads=threads)
             cb = time.perf_counter() - t0
             t0 = time.perf_counter()
             o.ray_first(rays, threads=threads)
@@ -506,6 +530,7 @@ def run_single(args, octree_amd, torch):
             o.collide()
             cc = time.perf_counter() - t0
             out["cpu_baseline"] = {"value": len(rays) / cr, "unit": "rays/s", "cores": threads, "kind": "port",
+                                   "cpu": cpu_model(),
                                    "sample": f"first {m} boxes; 100,000 rays", "build_mboxes_s": m / cb / 1e6,
                                    "collision_boxes_s_single_thread": m / cc}
             # the other queries on the same sample tree (rates per second, `threads` workers)
@@ -546,12 +571,13 @@ def run_single(args, octree_amd, torch):
                     "build_ms": tb * 1e3, "build_mpts_s": 8.0 / tb, "knn_ms": tk * 1e3})
         if not args.no_cpu:
             t0 = time.perf_counter()
-            o = O.OracleTree(6, "point", pts[:m], box, 8, 32)
+            o = O.OracleTree(6, "point", pts[:m], box, 8, 32, threads=threads)
             cb = time.perf_counter() - t0
             t0 = time.perf_counter()
             o.knn(qs[:20_000], 16, threads=threads)
             ck = time.perf_counter() - t0
             out["cpu_baseline"] = {"value": 20_000 / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
+                                   "cpu": cpu_model(),
                                    "sample": f"tree of the first {m} points; 20,000 queries", "build_mpts_s": 1.0 / cb}
     kt = None
     return out

@@ -152,6 +152,38 @@ OTG_API otg_status otg_tree_export(const otg_tree *tree, uint32_t set, uint8_t *
 OTG_API otg_status otg_tree_serialize(const otg_tree *tree, uint32_t set, uint32_t flags, uint8_t *buf,
                                       uint64_t buf_size, uint64_t *size);
 
+/* ---- autotuning ---------------------------------------------------------- */
+/* Settings of the batched path.  leaf_capacity is the build's maxElementNumInNode
+ * (ot_static_linear_core.h:523-529), which the caller passes to otg_forest_build;
+ * the others live in the context and apply to every later build and query on it:
+ * index_capacity = leaf capacity of the finer kNN / range index (0: the tree's),
+ * waves_per_simd = occupancy cap of the kNN walks (0: as registers allow),
+ * order_queries = 1 to walk batches of >= 16,384 queries in Morton order.  Query
+ * results never depend on them. */
+typedef struct {
+  uint32_t leaf_capacity;
+  uint32_t index_capacity;
+  uint32_t waves_per_simd;
+  uint32_t order_queries;
+  double step_ms; /* measured build + queries with these settings (otg_autotune) */
+} otg_tuning;
+
+/* No counterpart in the reference (its leaf capacity is a caller's constant):
+ * measures the forest build of the given sets (as otg_forest_build, leaf capacity
+ * from {8, 16, 32, 64}) followed by the kNN batch (k, default tolerance) and/or the
+ * RangeSearch batch (Inside), over the candidate settings, and returns the fastest.
+ * The pick is cached per workload shape for the process and applied to ctx.  Either
+ * query batch may be empty (nq 0); qmem is the queries' memory space. */
+OTG_API otg_status otg_autotune(otg_ctx *ctx, int kind, int dim, uint32_t nsets, const uint64_t *set_offsets,
+                                const double *geom, int mem, const double *boxes_or_null,
+                                const uint32_t *max_depths_or_null, uint64_t nq_knn, const double *knn_points,
+                                const uint32_t *knn_sets, uint32_t k, uint64_t nq_range, const double *ranges,
+                                const uint32_t *range_sets, int qmem, otg_tuning *out);
+/* Applies index_capacity, waves_per_simd and order_queries to the context (the
+ * untuned defaults are 0, 0, 1), or reads them back. */
+OTG_API otg_status otg_ctx_set_tuning(otg_ctx *ctx, const otg_tuning *tuning);
+OTG_API otg_status otg_ctx_get_tuning(const otg_ctx *ctx, otg_tuning *tuning);
+
 /* ---- queries ------------------------------------------------------------ */
 /* Results are CSR: offsets[nq+1] (uint64) and ids (uint32, local entity ids).
  * set_ids: per-query set of a forest (NULL = set 0); same memory space as the

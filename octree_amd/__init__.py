@@ -75,6 +75,53 @@ class Context:
     def reset_kernel_times(self) -> None:
         _check(_abi.lib().otg_ctx_reset_kernel_times(self.handle), "reset_kernel_times", self)
 
+    def autotune(self, kind: str, dim: int, offsets, geom, boxes=None, max_depths=None, knn=None, knn_sets=None,
+                 k: int = 0, ranges=None, range_sets=None) -> dict:
+        """otg_autotune: measure the forest build of `geom` (device tensor or host array
+        of all sets, split by `offsets`) and the kNN / range batches under candidate
+        settings; the pick is applied to this context and returned (leaf_capacity is
+        for the caller's builds)."""
+        offs = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+        ns = len(offs) - 1
+        cols = dim if kind == "point" else 2 * dim
+        keep = []
+
+        def arg(a, c, dt=np.float64):
+            if a is None:
+                return None, 0
+            if _is_device(a):
+                keep.append(a)
+                return C.c_void_p(a.data_ptr()), a.shape[0]
+            x = np.ascontiguousarray(np.asarray(a, dtype=dt).reshape((-1, c) if c else -1))
+            keep.append(x)
+            return C.c_void_p(x.ctypes.data), x.shape[0]
+
+        gp, _ = arg(geom, cols)
+        gmem = _abi.OTG_MEM_DEVICE if _is_device(geom) else _abi.OTG_MEM_HOST
+        qp, nk = arg(knn, dim)
+        sp_, _ = arg(knn_sets, 0, np.uint32)
+        rp, nr = arg(ranges, 2 * dim)
+        rsp, _ = arg(range_sets, 0, np.uint32)
+        qdev = any(_is_device(a) for a in (knn, ranges) if a is not None)
+        bx, _ = arg(boxes, 2 * dim)
+        dp, _ = arg(max_depths, 0, np.uint32)
+        out = _abi.Tuning()
+        kindc = _abi.OTG_KIND_POINT if kind == "point" else _abi.OTG_KIND_BOX
+        st = _abi.lib().otg_autotune(self.handle, kindc, dim, ns, offs.ctypes.data_as(_abi.U64P), gp, gmem, bx, dp,
+                                     nk, qp, sp_, int(k), nr, rp, rsp,
+                                     _abi.OTG_MEM_DEVICE if qdev else _abi.OTG_MEM_HOST, C.byref(out))
+        _check(st, "otg_autotune", self)
+        return {f: getattr(out, f) for f, _ in _abi.Tuning._fields_}
+
+    def set_tuning(self, index_capacity: int = 0, waves_per_simd: int = 0, order_queries: int = 1) -> None:
+        t = _abi.Tuning(0, index_capacity, waves_per_simd, order_queries, 0.0)
+        _check(_abi.lib().otg_ctx_set_tuning(self.handle, C.byref(t)), "otg_ctx_set_tuning", self)
+
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        _check(_abi.lib().otg_ctx_get_tuning(self.handle, C.byref(t)), "otg_ctx_get_tuning", self)
+        return {f: getattr(t, f) for f, _ in _abi.Tuning._fields_}
+
     def mem_stats(self):
         a, b = C.c_uint64(), C.c_uint64()
         _check(_abi.lib().otg_ctx_mem_stats(self.handle, C.byref(a), C.byref(b)), "mem_stats", self)

@@ -49,6 +49,7 @@ EXPORTED = [
     "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_free", "otg_tree_get_info",
     "otg_tree_get_set_info", "otg_tree_export", "otg_tree_serialize", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
     "otg_collide_trees", "otg_pick", "otg_ray_all", "otg_planes", "otg_knn_noalloc",
+    "otg_autotune", "otg_ctx_set_tuning", "otg_ctx_get_tuning",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
     "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_stats",
@@ -68,6 +69,11 @@ class SetInfo(C.Structure):
         ("node_count", C.c_uint64), ("max_depth", C.c_uint32), ("max_elements", C.c_uint32),
         ("space_min", C.c_double * 16), ("space_max", C.c_double * 16),
     ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("leaf_capacity", C.c_uint32), ("index_capacity", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("order_queries", C.c_uint32), ("step_ms", C.c_double)]
 
 
 class OtgError(RuntimeError):
@@ -112,6 +118,10 @@ def lib() -> C.CDLL:
         "otg_tree_export": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P, P]),
         "otg_tree_serialize": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_uint64, U64P]),
         "otg_knn": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, PP]),
+        "otg_autotune": (C.c_int, [P, C.c_int, C.c_int, C.c_uint32, P, P, C.c_int, P, P, C.c_uint64, P, P, C.c_uint32,
+                                   C.c_uint64, P, P, C.c_int, C.POINTER(Tuning)]),
+        "otg_ctx_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "otg_ctx_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "otg_knn_noalloc": (C.c_int, [P, C.c_uint64, P, P, C.c_uint32, C.c_double, C.c_double, P, P, C.c_uint64,
                                       C.POINTER(C.c_uint64)]),
         "otg_knn_footprint": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, U64P]),

@@ -21,7 +21,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libotg.so"
-SOURCES = ["otg_primitives.hip", "otg_build.hip", "otg_query.hip", "otg_abi.hip", "otg_server.hip", "otg_noalloc.hip"]
+SOURCES = ["otg_primitives.hip", "otg_build.hip", "otg_query.hip", "otg_abi.hip", "otg_server.hip", "otg_noalloc.hip", "otg_tune.hip"]
 
 
 def hipcc() -> str:

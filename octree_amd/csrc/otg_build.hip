@@ -1000,6 +1000,7 @@ static Tree *new_finer_tree(const Tree *t, uint32_t fmax) {
   aux->sets = t->sets;
   for (SetParams &sp : aux->sets) {
     sp.max_depth = finer_set_depth(sp, t->dim, fmax);
+    if (t->ctx->tuning.index_capacity) sp.cap = t->ctx->tuning.index_capacity;
     sp.min_lcp = 255;
     sp.root_splits = 0;
     double bmin[kMaxDim], bmax[kMaxDim];
@@ -1116,8 +1117,11 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
       x->ctx = ctx;
       x->kind = t->kind;
       x->dim = t->dim;
+      std::vector<uint32_t> icaps(caps, caps + nsets);
+      if (ctx->tuning.index_capacity) std::fill(icaps.begin(), icaps.end(), ctx->tuning.index_capacity);
       try {
-        build_impl<D, false>(x, dgeom, OTG_MEM_DEVICE, offsets, nsets, sboxes.data(), fdepth.data(), caps, false);
+        build_impl<D, false>(x, dgeom, OTG_MEM_DEVICE, offsets, nsets, sboxes.data(), fdepth.data(), icaps.data(),
+                             false);
       } catch (...) {
         delete x;
         throw;

@@ -60,12 +60,20 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// Settings the autotuner chooses (otg_autotune); defaults are the untuned path.
+struct Tuning {
+  uint32_t index_capacity = 0;  // leaf capacity of the finer kNN / range index (0: the tree's)
+  uint32_t waves_per_simd = 0;  // occupancy cap of the kNN walks (0: as registers allow)
+  uint32_t order_queries = 1;   // Morton-order batches of >= 16,384 queries before walking
+};
+
 class Ctx {
  public:
   int device = 0;
   hipStream_t stream = nullptr;
   std::string last_error;
   bool profiling = false;
+  Tuning tuning;
   struct KTime {
     double ms = 0;
     uint64_t launches = 0;

@@ -14,6 +14,8 @@
 // ray bound (counted as measure-zero divergences by the tests).
 #include <algorithm>
 #include <cfloat>
+#include <map>
+#include <mutex>
 
 #include "otg_device.hpp"
 #include "otg_internal.hpp"
@@ -2362,6 +2364,31 @@ __global__ void __launch_bounds__(256) k_query_keys(const double *__restrict__ q
   idx[i] = (uint32_t)i;
 }
 
+// Dynamic LDS that caps a one-wavefront-block kernel at `waves` wavefronts per SIMD
+// (4 SIMDs per CU share the CU's LDS): 0 = no cap.
+static size_t occupancy_pad(Ctx *ctx, const void *kernel, uint32_t waves) {
+  if (!waves) return 0;
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, std::pair<size_t, size_t>> cache;  // (lds per CU, static)
+  std::pair<size_t, size_t> v;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find({ctx->device, kernel});
+    if (it == cache.end()) {
+      hipDeviceProp_t prop;
+      OTG_HIP_CHECK(hipGetDeviceProperties(&prop, ctx->device));
+      hipFuncAttributes fa;
+      OTG_HIP_CHECK(hipFuncGetAttributes(&fa, kernel));
+      it = cache.emplace(std::make_pair(ctx->device, kernel),
+                         std::make_pair((size_t)prop.maxSharedMemoryPerMultiProcessor, (size_t)fa.sharedSizeBytes))
+               .first;
+    }
+    v = it->second;
+  }
+  const size_t per_block = v.first / (4 * (size_t)waves);
+  return per_block > v.second ? per_block - v.second : 0;
+}
+
 // The trees one launch serves: device TreeViews, each query's view index
 // (null: a single tree) and the widest key layout among them.
 struct ViewSet {
@@ -2380,7 +2407,7 @@ static Buf<uint32_t> query_order(Ctx *ctx, const ViewSet &vs, const double *q, u
                                  uint64_t nq) {
   // below this many queries the ordering (key pass + radix sort, about 20 launches)
   // costs more than the locality it buys: the queries run in input order
-  if (nq < kOrderMinQueries) return Buf<uint32_t>();
+  if (nq < kOrderMinQueries || !ctx->tuning.order_queries) return Buf<uint32_t>();
   Buf<uint32_t> idx(ctx, nq);
   // the leading 30 bits of the Morton code (10 levels in 3-D) give the wavefronts
   // their locality; the order decides speed only, never results
@@ -2580,7 +2607,8 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   Buf<uint32_t> ids(ctx, ar, nq * KCAP), cnt(ctx, ar, nq), ovf(ctx, ar, 1), band(ctx, ar, nq);
   Buf<double> Fq(ctx, ar, nq);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));  // band count
-  launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs, qk,
+  launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB),
+         occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd), vs.views, vs.qview, q, qs, qk,
          (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
   uint32_t hovf[1] = {0};
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));

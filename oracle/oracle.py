@@ -29,6 +29,8 @@ def lib() -> C.CDLL:
     P = C.c_void_p
     L.oc_build.restype = P
     L.oc_build.argtypes = [C.c_int, C.c_int, C.c_uint32, P, P, C.c_uint32, C.c_uint32]
+    L.oc_build_mt.restype = P
+    L.oc_build_mt.argtypes = [C.c_int, C.c_int, C.c_uint32, P, P, C.c_uint32, C.c_uint32, C.c_int]
     L.oc_tree_free.argtypes = [P]
     L.oc_tree_status.restype = C.c_int
     L.oc_tree_status.argtypes = [P]
@@ -110,14 +112,14 @@ def estimate_max_depth(n, cap, D):
 class OracleTree:
     """CPU restatement of StaticLinearOrthoTreeCore (points: MinPoint, boxes: loose 2.0 + MBR)."""
 
-    def __init__(self, dim: int, kind: str, geom, box=None, max_depth: int = 0, cap: int = 16):
+    def __init__(self, dim: int, kind: str, geom, box=None, max_depth: int = 0, cap: int = 16, threads: int = 1):
         self.dim = dim
         self.kind = kind
         cols = dim if kind == "point" else 2 * dim
         self.geom = np.ascontiguousarray(np.asarray(geom, np.float64).reshape(-1, cols))
         bx = None if box is None else np.ascontiguousarray(np.asarray(box, np.float64).reshape(-1))
-        self.h = lib().oc_build(dim, 0 if kind == "point" else 1, self.geom.shape[0], _p(self.geom), _p(bx),
-                                int(max_depth or 0), int(cap))
+        self.h = lib().oc_build_mt(dim, 0 if kind == "point" else 1, self.geom.shape[0], _p(self.geom), _p(bx),
+                                   int(max_depth or 0), int(cap), int(threads))
         if not self.h:
             raise ValueError("oracle rejected the build parameters")
         self.status = lib().oc_tree_status(self.h)

@@ -413,11 +413,115 @@ OC_API void oc_bounding_box_export(const double *geom, uint32_t n, int D, int is
   oc_bounding_box(geom, n, D, is_box, bmin, bmax);
 }
 
+/* Parallel encode + sort of the PAR path (ot_static_linear_core.h:549-572: the
+ * transform and the sort run par_unseq; Build<true> stays serial): each thread
+ * encodes and sorts one chunk, then sorted chunks are merged pairwise, a round of
+ * merges at a time.  The comparator is a total order (ids break ties), so the
+ * result equals the serial sort's. */
+typedef struct {
+  const oc_tree *t;
+  const double *geom;
+  oc_loc *locs;
+  uint32_t lo, hi;
+  int status;
+} oc_enc_job;
+
+static void *oc_enc_run(void *arg) {
+  oc_enc_job *j = (oc_enc_job *)arg;
+  oc_tree local = *j->t; /* private header: the location functions record status in it */
+  local.status = 0;
+  const int D = local.dim;
+  for (uint32_t i = j->lo; i < j->hi; ++i) {
+    j->locs[i].id = i;
+    if (local.is_box) oc_box_location(&local, &j->geom[(size_t)i * 2 * D], &j->locs[i].code, &j->locs[i].depth);
+    else oc_point_location(&local, &j->geom[(size_t)i * D], &j->locs[i].code, &j->locs[i].depth);
+    if (!local.is_box && j->locs[i].depth == 0) local.status = 2;
+  }
+  qsort(j->locs + j->lo, j->hi - j->lo, sizeof(oc_loc), oc_loc_cmp);
+  j->status = local.status;
+  return NULL;
+}
+
+typedef struct {
+  const oc_loc *src;
+  oc_loc *dst;
+  uint32_t lo, mid, hi;
+} oc_merge_job;
+
+static void *oc_merge_run(void *arg) {
+  oc_merge_job *m = (oc_merge_job *)arg;
+  uint32_t a = m->lo, b = m->mid, o = m->lo;
+  while (a < m->mid && b < m->hi) m->dst[o++] = (oc_loc_cmp(&m->src[b], &m->src[a]) < 0) ? m->src[b++] : m->src[a++];
+  while (a < m->mid) m->dst[o++] = m->src[a++];
+  while (b < m->hi) m->dst[o++] = m->src[b++];
+  return NULL;
+}
+
+/* locs[0..n) encoded and sorted with `threads` threads; returns the location status. */
+static int oc_encode_sort_mt(const oc_tree *t, const double *geom, oc_loc *locs, uint32_t n, int threads) {
+  if (threads < 1) threads = 1;
+  if ((uint32_t)threads > n) threads = n ? (int)n : 1;
+  oc_enc_job *jobs = (oc_enc_job *)calloc((size_t)threads, sizeof(oc_enc_job));
+  pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+  uint32_t *bounds = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)threads + 1));
+  for (int i = 0; i <= threads; ++i) bounds[i] = (uint32_t)(((uint64_t)n * (uint64_t)i) / (uint64_t)threads);
+  for (int i = 0; i < threads; ++i) {
+    jobs[i] = (oc_enc_job){t, geom, locs, bounds[i], bounds[i + 1], 0};
+    if (threads > 1) pthread_create(&th[i], NULL, oc_enc_run, &jobs[i]);
+    else oc_enc_run(&jobs[i]);
+  }
+  int status = 0;
+  for (int i = 0; i < threads; ++i) {
+    if (threads > 1) pthread_join(th[i], NULL);
+    if (jobs[i].status) status = jobs[i].status;
+  }
+  /* pairwise merges of sorted runs */
+  oc_loc *tmp = threads > 1 ? (oc_loc *)malloc(sizeof(oc_loc) * (size_t)n) : NULL;
+  oc_loc *src = locs, *dst = tmp;
+  int runs = threads;
+  uint32_t *rb = bounds;
+  while (runs > 1) {
+    const int pairs = runs / 2;
+    oc_merge_job *mj = (oc_merge_job *)calloc((size_t)pairs + 1, sizeof(oc_merge_job));
+    for (int p2 = 0; p2 < pairs; ++p2) {
+      mj[p2] = (oc_merge_job){src, dst, rb[2 * p2], rb[2 * p2 + 1], rb[2 * p2 + 2]};
+      pthread_create(&th[p2], NULL, oc_merge_run, &mj[p2]);
+    }
+    if (runs & 1) { /* the odd run is copied over */
+      mj[pairs] = (oc_merge_job){src, dst, rb[runs - 1], rb[runs], rb[runs]};
+      oc_merge_run(&mj[pairs]);
+    }
+    for (int p2 = 0; p2 < pairs; ++p2) pthread_join(th[p2], NULL);
+    free(mj);
+    int nr = 0;
+    for (int r = 0; r < runs; r += 2) rb[nr++] = rb[r];
+    rb[nr] = n;
+    runs = nr;
+    oc_loc *x = src;
+    src = dst;
+    dst = x;
+  }
+  if (src != locs) memcpy(locs, src, sizeof(oc_loc) * (size_t)n);
+  free(tmp);
+  free(bounds);
+  free(th);
+  free(jobs);
+  return status;
+}
+
+OC_API oc_tree *oc_build_mt(int D, int is_box, uint32_t n, const double *geom, const double *box_or_null,
+                            uint32_t max_depth_in, uint32_t cap, int threads);
+
 /* ot_static_linear_core.h:523-599 Create<ParExec>: box (given or bounding),
  * depth (given or EstimateMaxDepth), encode every entity, sort by location,
  * BFS build, MBR pass.  Returns NULL on invalid parameters. */
 OC_API oc_tree *oc_build(int D, int is_box, uint32_t n, const double *geom, const double *box_or_null,
                          uint32_t max_depth_in, uint32_t cap) {
+  return oc_build_mt(D, is_box, n, geom, box_or_null, max_depth_in, cap, 1);
+}
+
+OC_API oc_tree *oc_build_mt(int D, int is_box, uint32_t n, const double *geom, const double *box_or_null,
+                            uint32_t max_depth_in, uint32_t cap, int threads) {
   if (D < 1 || D > OC_MAXD || cap == 0) return NULL;
   oc_tree *t = (oc_tree *)calloc(1, sizeof(oc_tree));
   t->dim = D;
@@ -443,17 +547,13 @@ OC_API oc_tree *oc_build(int D, int is_box, uint32_t n, const double *geom, cons
   if (n == 0) return t;
 
   oc_loc *locs = (oc_loc *)malloc(sizeof(oc_loc) * n);
-  for (uint32_t i = 0; i < n; ++i) {
-    locs[i].id = i;
-    if (is_box) oc_box_location(t, &geom[(size_t)i * 2 * D], &locs[i].code, &locs[i].depth);
-    else oc_point_location(t, &geom[(size_t)i * D], &locs[i].code, &locs[i].depth);
-    if (!is_box && locs[i].depth == 0) t->status = 2; /* out-of-space point: reference Build never ends */
-  }
+  /* an out-of-space point sets status 2: the reference Build never ends for it */
+  const int st = oc_encode_sort_mt(t, geom, locs, n, threads);
+  if (st) t->status = st;
   if (t->status == 2) {
     free(locs);
     return t;
   }
-  qsort(locs, n, sizeof(oc_loc), oc_loc_cmp);
   t->entity_ids = (uint32_t *)malloc(4u * n);
   t->sorted_code = (uint64_t *)malloc(8u * n);
   t->sorted_depth = (uint32_t *)malloc(4u * n);

@@ -467,3 +467,20 @@ def test_planes_vs_brute(kind, D, tol):
         for k in range(4):
             ok &= plane_relation(geom, kind == "box", fr[qi, k, 0], fr[qi, k, 1:], tol) != -1
         assert sorted(got[qi]) == np.nonzero(ok)[0].tolist(), qi
+
+
+# ---- threaded build (the PAR path's parallel encode and sort) ------------------------
+@pytest.mark.parametrize("kind,threads", [("point", 3), ("point", 8), ("box", 5)])
+def test_threaded_build_equals_serial(kind, threads):
+    """Encode + sort split over threads and merged gives the serial build's table
+    exactly (ids break key ties, so the merged order is the serial order)."""
+    D = 3
+    geom = uniform_points(20000, D, seed=4) if kind == "point" else lognormal_boxes(20000, D, 0.0, 100.0,
+                                                                                    np.log(1.0), 0.6, seed=4)
+    if kind == "point":
+        geom[:500] = geom[500:1000]
+    a = O.OracleTree(D, kind, geom, None, 6, 8).export()
+    b = O.OracleTree(D, kind, geom, None, 6, 8, threads=threads).export()
+    for key in a:
+        if isinstance(a[key], np.ndarray):
+            assert np.array_equal(a[key], b[key]), key
