@@ -193,7 +193,8 @@ OTG_API otg_status otg_ctx_get_tuning(const otg_ctx *ctx, otg_tuning *tuning);
 
 /* OrthoTreeQueryBase::GetNearestNeighbors(point, k, maxDistanceWithin, entities,
  * tolerance) (core/ot_query.h:1208-1270).  max_within = DBL_MAX for the
- * overload without a distance limit.  Ids sorted by ascending distance. */
+ * overload without a distance limit.  Ids sorted by ascending distance.  Any k:
+ * above 64 the final radius comes from a k-entry heap per query in global memory. */
 OTG_API otg_status otg_knn(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids, int mem,
                            uint32_t k, double max_within, double tolerance, otg_result **out);
 

@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "otg_device.hpp"
 #include "otg_internal.hpp"
@@ -801,6 +802,97 @@ __global__ void __launch_bounds__(kQB) k_knn_band(const TreeView *__restrict__ v
       }
     }
     if (!FILL) counts[ti] = cnt;
+  });
+}
+
+// kNN with k above the register lists (k > 64): the final radius of each query,
+// F = tol_upper(k-th smallest pessimistic distance) (points: the distance) among
+// the entities whose optimistic distance is below the running F, or F0 when fewer
+// than k are -- the bound the reference's AddEntityDistance ends with
+// (ot_query.h:1058-1164: F assigned at the k-th entry, lowered afterwards).  A
+// depth-first walk keeps the k smallest pessimistic distances in a max-heap of k
+// doubles in global memory (heap + ti * k); nodes and entities are skipped exactly
+// where the reference skips them (wall or optimistic distance >= F).  The ids are
+// then collected by the band pass with this F, which returns {optimistic < F}
+// sorted by (distance, id): the reference's list.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kQB) k_knn_radius(const TreeView *__restrict__ views,
+                                                     const uint32_t *__restrict__ qview, const double *__restrict__ qpts,
+                                                     const uint32_t *__restrict__ qset, uint64_t q0, uint64_t nlist,
+                                                     uint32_t k, double F0, double tol, double *__restrict__ heap,
+                                                     double *__restrict__ Fq) {
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nlist) return;
+  const uint64_t qi = q0 + ti;
+  double *h = heap + ti * k;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+    double F = F0, T = sq_reject_bound(F0);
+    double q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = qpts[qi * D + d];
+    uint32_t hn = 0;
+    if (sp.node_cnt != 0) {
+      uint32_t stack[StackCap<D>::value];
+      int top = 0;
+      stack[top++] = (uint32_t)sp.node_off;
+      while (top > 0) {
+        const uint32_t node = stack[--top];
+        bool in;
+        const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
+        if (w2 >= T || sqrt(w2) >= F) continue;
+        const NodeInfo ni = tv.info[node];
+        for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+          double x2, mm2 = 0.0;
+          if (BOX) {
+            box_minmax2<D>(q, tv, j, tol, &x2, &mm2);
+          } else {
+            x2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double v = q[d] - tv.geom[(uint64_t)d * tv.n + j];
+              x2 += v * v;
+            }
+          }
+          if (x2 >= T || !(sqrt(x2) < F)) continue;
+          const double pes = BOX ? sqrt(mm2) : sqrt(x2);
+          if (hn < k) {
+            uint32_t i = hn++;  // sift up
+            while (i > 0) {
+              const uint32_t par = (i - 1) / 2;
+              if (!(h[par] < pes)) break;
+              h[i] = h[par];
+              i = par;
+            }
+            h[i] = pes;
+            if (hn == k) {
+              F = tol_upper(h[0], tol);
+              T = sq_reject_bound(F);
+            }
+          } else if (pes < h[0]) {
+            uint32_t i = 0;  // replace the largest, sift down
+            for (;;) {
+              const uint32_t l = 2 * i + 1, r = l + 1;
+              uint32_t m = i;
+              double vm = pes;
+              if (l < k && vm < h[l]) m = l, vm = h[l];
+              if (r < k && vm < h[r]) m = r;
+              if (m == i) break;
+              h[i] = h[m];
+              i = m;
+            }
+            h[i] = pes;
+            const double lim = tol_upper(h[0], tol);
+            if (lim < F) {
+              F = lim;
+              T = sq_reject_bound(F);
+            }
+          }
+        }
+        for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
+      }
+    }
+    Fq[qi] = F;
   });
 }
 
@@ -1877,6 +1969,7 @@ struct LdsHeap {
   static constexpr int cap = kRayHeap;
   uint32_t *n;
   double *p;
+  __device__ __forceinline__ int capacity() const { return cap; }
   __device__ __forceinline__ uint32_t &node(int i) { return n[i * kQB]; }
   __device__ __forceinline__ double &prio(int i) { return p[i * kQB]; }
 };
@@ -1886,6 +1979,19 @@ struct ArrayHeap {
   static constexpr int cap = 2 * StackCap<D>::value;
   uint32_t nd[cap];
   double pr[cap];
+  __device__ __forceinline__ int capacity() const { return cap; }
+  __device__ __forceinline__ uint32_t &node(int i) { return nd[i]; }
+  __device__ __forceinline__ double &prio(int i) { return pr[i]; }
+};
+
+// Per-ray heap in global memory holding up to `c` nodes: with c = the tree's node
+// count it never fills (a node is queued at most once).  For the rare rays whose
+// frontier outgrows the private heap.
+struct GlobalHeap {
+  uint32_t *nd;
+  double *pr;
+  int c;
+  __device__ __forceinline__ int capacity() const { return c; }
   __device__ __forceinline__ uint32_t &node(int i) { return nd[i]; }
   __device__ __forceinline__ double &prio(int i) { return pr[i]; }
 };
@@ -2013,7 +2119,7 @@ __device__ __forceinline__ bool ray_query(const TreeView &tv, uint64_t qi, const
       for (uint32_t c = 0; c < ni.ccount; ++c) {
         const uint32_t x = ni.cbegin + c;
         if (!rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &en, &ex)) continue;
-        if (n == Heap::cap) return false;
+        if (n == H.capacity()) return false;
         heap_push(H, n, x, en);
       }
     }
@@ -2046,8 +2152,10 @@ __global__ void __launch_bounds__(kQB) k_ray(const TreeView *__restrict__ views,
   });
 }
 
-// Rays whose LDS heap would overflow, with the large private heap.
-template <int D, bool BOX, int RCAP>
+// Rays whose LDS heap would overflow, with the large private heap (GLOBAL: with a
+// heap of hcap nodes in global memory, heap_n / heap_p + ti * hcap).  A ray that
+// outgrows the private heap is listed in fail_list (count in *fail).
+template <int D, bool BOX, int RCAP, bool GLOBAL>
 __global__ void __launch_bounds__(kQB) k_ray_deep(const TreeView *__restrict__ views,
                                                    const uint32_t *__restrict__ qview, const double *__restrict__ rays,
                                                    const uint32_t *__restrict__ qset, const uint32_t *__restrict__ order,
@@ -2055,39 +2163,57 @@ __global__ void __launch_bounds__(kQB) k_ray_deep(const TreeView *__restrict__ v
                                                    double inc, double max_dist, uint32_t *__restrict__ out_ids,
                                                    uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
                                                    uint32_t *__restrict__ band, double *__restrict__ out_M,
-                                                   uint32_t *__restrict__ fail) {
+                                                   uint32_t *__restrict__ fail, uint32_t *__restrict__ fail_list,
+                                                   uint32_t *__restrict__ heap_n, double *__restrict__ heap_p,
+                                                   uint32_t hcap) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nlist) return;
   const uint64_t qi = order ? order[list[ti]] : list[ti];
-  ArrayHeap<D> H;
   with_tree(views, qview, qi, [&](const TreeView &tv) {
-    if (!ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M))
-      atomicOr(fail, 1u);
+    bool ok;
+    if constexpr (GLOBAL) {
+      GlobalHeap H{heap_n + ti * hcap, heap_p + ti * hcap, (int)hcap};
+      ok = ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M);
+    } else {
+      ArrayHeap<D> H;
+      ok = ray_query<D, BOX, RCAP>(tv, qi, rays, qset, tol, inc, max_dist, H, out_ids, out_cnt, overflow, band, out_M);
+    }
+    if (!ok) fail_list[atomicAdd(fail, 1u)] = list[ti];
   });
 }
+
+constexpr uint32_t kReplayListFull = 0xFFFFFFFFu;  // candidate list outgrown: replay with a longer one
+constexpr uint32_t kReplayHeapFull = 0xFFFFFFFEu;  // node heap outgrown: replay with a global heap
 
 // Rays with more candidates than the register list are replayed exactly (same
 // best-first walk, same bound updates) with a candidate list of `cap` entries in
 // global memory; a replay that outgrows it sets its count to UINT32_MAX and is run
 // again with a larger list.  The candidates end sorted by descending enter, ties by id.
-template <int D, bool BOX>
+// GLOBAL: the node heap in global memory (heap_n / heap_p + ti * hcap), for rays
+// whose frontier outgrows the private heap; a ray that finds its heap full is
+// listed as failed (counts[ti] = UINT32_MAX - 1, *fail set) and replayed with it.
+template <int D, bool BOX, bool GLOBAL>
 __global__ void __launch_bounds__(kQB) k_ray_replay(const TreeView *__restrict__ views,
                                                      const uint32_t *__restrict__ qview,
                                                      const double *__restrict__ rays, const uint32_t *__restrict__ qset,
                                                      const uint32_t *__restrict__ list, uint32_t nlist, double tol,
                                                      double inc, double max_dist, uint32_t cap,
                                                      double *__restrict__ cand_e, uint32_t *__restrict__ cand_id,
-                                                     uint32_t *__restrict__ counts, uint32_t *__restrict__ fail) {
+                                                     uint32_t *__restrict__ counts, uint32_t *__restrict__ fail,
+                                                     uint32_t *__restrict__ heap_n, double *__restrict__ heap_p,
+                                                     uint32_t hcap) {
   const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
   if (ti >= nlist) return;
   const uint64_t qi = list[ti];
   double *ce = cand_e + ti * cap;
   uint32_t *cid = cand_id + ti * cap;
-  ArrayHeap<D> H;
+  using Heap = std::conditional_t<GLOBAL, GlobalHeap, ArrayHeap<D>>;
+  Heap H;
+  if constexpr (GLOBAL) H = GlobalHeap{heap_n + ti * hcap, heap_p + ti * hcap, (int)hcap};
   with_tree(views, qview, qi, [&](const TreeView &tv) {
     const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
     uint32_t cnt = 0;
-    bool too_many = false;
+    bool too_many = false, heap_full = false;
     RayTester<D> rt;
     if (sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc)) {
       const double applied = (tol == 0) ? DBL_EPSILON : tol;
@@ -2122,25 +2248,48 @@ __global__ void __launch_bounds__(kQB) k_ray_replay(const TreeView *__restrict__
             too_many = true;
             break;
           }
-          ce[cnt] = en;
-          cid[cnt] = tv.ids[j];
-          ++cnt;
+          // candidates form a max-heap on enter distance (the reference's priority
+          // queue): a lower bound pops the entries above it from the top
+          {
+            uint32_t i = cnt++;
+            while (i > 0) {
+              const uint32_t par = (i - 1) / 2;
+              if (!(ce[par] < en)) break;
+              ce[i] = ce[par];
+              cid[i] = cid[par];
+              i = par;
+            }
+            ce[i] = en;
+            cid[i] = tv.ids[j];
+          }
           if (ex >= M) continue;
           M = (1.0 + inc) * ex + applied;
-          uint32_t w = 0;
-          for (uint32_t t = 0; t < cnt; ++t)
-            if (!(ce[t] > M)) {
-              ce[w] = ce[t];
-              cid[w] = cid[t];
-              ++w;
+          while (cnt > 0 && ce[0] > M) {
+            --cnt;
+            const double le = ce[cnt];
+            const uint32_t li = cid[cnt];
+            uint32_t i = 0;
+            for (;;) {
+              const uint32_t l = 2 * i + 1, r = l + 1;
+              uint32_t m = i;
+              double vm = le;
+              if (l < cnt && vm < ce[l]) m = l, vm = ce[l];
+              if (r < cnt && vm < ce[r]) m = r;
+              if (m == i) break;
+              ce[i] = ce[m];
+              cid[i] = cid[m];
+              i = m;
             }
-          cnt = w;
+            ce[i] = le;
+            cid[i] = li;
+          }
         }
         for (uint32_t c = 0; c < ni.ccount && !too_many; ++c) {
           const uint32_t x = ni.cbegin + c;
           if (!rt.hit_node(&tv.nmin[(uint64_t)x * D], &tv.nsize[(uint64_t)x * D], &en, &ex)) continue;
-          if (n == ArrayHeap<D>::cap) {
+          if (n == H.capacity()) {
             atomicOr(fail, 1u);
+            heap_full = true;
             too_many = true;
             break;
           }
@@ -2149,7 +2298,7 @@ __global__ void __launch_bounds__(kQB) k_ray_replay(const TreeView *__restrict__
       }
     }
     if (too_many) {
-      counts[ti] = 0xFFFFFFFFu;
+      counts[ti] = heap_full ? kReplayHeapFull : kReplayListFull;
       return;
     }
     counts[ti] = cnt;
@@ -2182,14 +2331,15 @@ __global__ void __launch_bounds__(kQB) k_ray_replay(const TreeView *__restrict__
   });
 }
 
-// Copies replayed candidate lists (stride cap) into their CSR slots.
-__global__ void k_replay_copy(const uint32_t *__restrict__ band, uint32_t nband, uint32_t cap,
-                              const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ cid,
-                              const uint64_t *__restrict__ off, uint32_t *__restrict__ dst) {
+// Copies replayed candidate lists (packed, list i at src + src_off[i]) into the CSR
+// slots of their rays.
+__global__ void k_replay_splice(const uint32_t *__restrict__ rays, uint32_t nrays, const uint64_t *__restrict__ src_off,
+                                const uint32_t *__restrict__ src, const uint64_t *__restrict__ off,
+                                uint32_t *__restrict__ dst) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= nband) return;
-  const uint64_t o = off[band[i]];
-  for (uint32_t t = 0; t < bcnt[i]; ++t) dst[o + t] = cid[i * cap + t];
+  if (i >= nrays) return;
+  const uint64_t o = off[rays[i]];
+  for (uint64_t t = src_off[i]; t < src_off[i + 1]; ++t) dst[o + (t - src_off[i])] = src[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -2396,6 +2546,7 @@ struct ViewSet {
   const uint32_t *qview = nullptr;
   uint32_t nviews = 1;
   uint32_t cbits = 0, sbits = 0;
+  uint64_t max_nodes = 0, max_entities = 0;  // largest tree of the launch (bounds of ray walks)
 };
 
 static uint32_t bit_width32(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
@@ -2670,6 +2821,69 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   return r;
 }
 
+// k > 64: the radius pass in chunks (k doubles of heap per query, at most 256 MiB at
+// a time), then the band pass over every query with that radius gives the lists.
+template <int D, bool BOX>
+static Result *knn_large_k(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, uint32_t k,
+                           double F0, double tol) {
+  Buf<double> Fq(ctx, nq);
+  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, (256ull << 20) / (8ull * k)));
+  {
+    Buf<double> heap(ctx, chunk * k);
+    for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+      const uint64_t m = std::min(chunk, nq - q0);
+      launch(ctx, "knn_radius", k_knn_radius<D, BOX>, dim3(div_up(m, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
+             q0, m, k, F0, tol, heap.p, Fq.p);
+    }
+  }
+  if (nq > 0xFFFFFFFFull) throw Error{OTG_ERR_UNSUPPORTED, "more than 2^32 - 1 queries with k > 64"};
+  const uint32_t nb = (uint32_t)nq;
+  Buf<uint32_t> all(ctx, nb), bcnt(ctx, nb), boff(ctx, nb + 1), btot(ctx, 1);
+  launch(ctx, "iota_u32", k_iota_u32, dim3(div_up(nb, 256)), dim3(256), 0, all.p, nb);
+  launch(ctx, "knn_band_count", k_knn_band<D, BOX, false>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+         qs, (const uint32_t *)all.p, nb, (const double *)Fq.p, tol, bcnt.p, (const uint32_t *)nullptr,
+         (double *)nullptr, (uint32_t *)nullptr);
+  exclusive_scan_u32(ctx, bcnt.p, boff.p, nb, btot.p);
+  uint32_t hb = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&hb, btot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  check_id_total(hb);
+  OTG_HIP_CHECK(hipMemcpyAsync(boff.p + nb, &hb, 4, hipMemcpyHostToDevice, ctx->stream));
+  Buf<double> bd(ctx, std::max<uint32_t>(hb, 1));
+  Result *r = new Result();
+  r->ctx = ctx;
+  r->nq = nq;
+  r->nids = hb;
+  r->d_ids = (uint32_t *)ctx->alloc(std::max<uint64_t>(hb, 1) * 4);
+  r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
+  launch(ctx, "knn_band_fill", k_knn_band<D, BOX, true>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
+         qs, (const uint32_t *)all.p, nb, (const double *)Fq.p, tol, (uint32_t *)nullptr, (const uint32_t *)boff.p,
+         bd.p, r->d_ids);
+  launch(ctx, "knn_band_sort", k_sort_band<false, uint32_t>, dim3(div_up(nb, kQB)), dim3(kQB), 0,
+         (const uint32_t *)boff.p, (uint64_t)nb, bd.p, r->d_ids);
+  launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)boff.p, nq, hb,
+         r->d_offsets);
+  ctx->sync();  // the staged total above lives on this stack frame
+  return r;
+}
+
+static Result *knn_large_k_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
+                                    const uint32_t *qs, uint32_t k, double F0, double tol) {
+#define OTG_KNNL(DD)                                                                        \
+  case DD:                                                                                  \
+    return box ? knn_large_k<DD, true>(ctx, vs, nq, q, qs, k, F0, tol)                      \
+               : knn_large_k<DD, false>(ctx, vs, nq, q, qs, k, F0, tol);
+  switch (dim) {
+    OTG_KNNL(2)
+    OTG_KNNL(3)
+    OTG_KNNL(4)
+    OTG_KNNL(5)
+    OTG_KNNL(6)
+  }
+#undef OTG_KNNL
+  throw Error{OTG_ERR_INVALID_ARG, "dimension"};
+}
+
 // kmax = the largest k of the launch (per-query k in qk, or k for all)
 static Result *knn_dispatch(Ctx *ctx, int dim, bool box, const ViewSet &vs, uint64_t nq, const double *q,
                             const uint32_t *qs, const uint32_t *qk, uint32_t k, uint32_t kmax, double F0, double tol,
@@ -2711,6 +2925,8 @@ static ViewSet single_view(const Tree *t) {
   vs.views = t->d_view;
   vs.cbits = t->layout.cbits;
   vs.sbits = t->layout.sbits;
+  vs.max_nodes = t->node_count;
+  vs.max_entities = t->n;
   return vs;
 }
 
@@ -2728,12 +2944,15 @@ static double knn_F0(double max_within, double tol) {
 
 Result *query_knn(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, int mem, uint32_t k,
                   double max_within, double tol, uint64_t *footprint) {
-  if (k > 64) throw Error{OTG_ERR_UNSUPPORTED, "k > 64"};
+  if (k > 64 && footprint) throw Error{OTG_ERR_UNSUPPORTED, "footprint of k > 64"};
   Ctx *ctx = run_ctx(t);
   t = knn_index(t);
   DevInput in;
   stage_inputs(ctx, in, pts, sets, nq * t->dim, nq, mem, (uint32_t)t->sets.size());
   if (nq == 0 || t->n == 0 || k == 0) return empty_result(ctx, nq);
+  if (k > 64)
+    return knn_large_k_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, k,
+                                knn_F0(max_within, tol), tol);
   return knn_dispatch(ctx, t->dim, t->kind == OTG_KIND_BOX, single_view(t), nq, in.q, in.sets, nullptr, k, k,
                       knn_F0(max_within, tol), tol, footprint);
 }
@@ -3109,59 +3328,148 @@ Result *query_ray_all(const Tree *t, uint64_t nq, const double *rays, const uint
   throw Error{OTG_ERR_INVALID_ARG, "dimension"};
 }
 
+// Node heap capacity that no ray walk of these trees can outgrow: the node count.
+static uint32_t ray_heap_nodes(const ViewSet &vs) {
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(vs.max_nodes, 1), 0x7FFFFFFFull);
+}
+
 template <int D, bool BOX>
 static Result *ray_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *q, const uint32_t *qs, double tol,
                         double inc, double max_dist) {
   constexpr int RCAP = 16;
+  constexpr uint64_t kScratch = 1ull << 30;  // bytes of global node heaps / candidate lists at a time
   const uint64_t nq1 = std::max<uint64_t>(nq, 1);
   Buf<uint32_t> ids(ctx, nq * RCAP), cnt(ctx, nq1), ovf(ctx, 3), band(ctx, nq1), redo(ctx, nq1);
   Buf<double> Mq(ctx, nq1);
-  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 12, ctx->stream));  // [0] band count, [1] redo count, [2] deep failure
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 12, ctx->stream));  // [0] band count, [1] redo count, [2] deep failures
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
   launch(ctx, "ray_first", k_ray<D, BOX, RCAP>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
          (const uint32_t *)order.p, nq, tol, inc, max_dist, ids.p, cnt.p, ovf.p, band.p, Mq.p, redo.p, ovf.p + 1);
   uint32_t hovf[3] = {0, 0, 0};
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
+  const uint32_t hcap = ray_heap_nodes(vs);
   if (hovf[1]) {
     sort_positions(ctx, redo.p, hovf[1], nq);
-    launch(ctx, "ray_deep", k_ray_deep<D, BOX, RCAP>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
-           qs, (const uint32_t *)order.p, (const uint32_t *)redo.p, hovf[1], tol, inc, max_dist, ids.p, cnt.p, ovf.p,
-           band.p, Mq.p, ovf.p + 2);
+    Buf<uint32_t> deep_fail(ctx, hovf[1]);
+    launch(ctx, "ray_deep", k_ray_deep<D, BOX, RCAP, false>, dim3(div_up(hovf[1], kQB)), dim3(kQB), 0, vs.views,
+           vs.qview, q, qs, (const uint32_t *)order.p, (const uint32_t *)redo.p, hovf[1], tol, inc, max_dist, ids.p,
+           cnt.p, ovf.p, band.p, Mq.p, ovf.p + 2, deep_fail.p, (uint32_t *)nullptr, (double *)nullptr, 0u);
     OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
-    if (hovf[2]) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
+    if (hovf[2]) {
+      // frontiers beyond the private heap: a node heap of the tree's size in global memory
+      const uint32_t nf = hovf[2];
+      const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nf, kScratch / (12ull * hcap)));
+      Buf<uint32_t> hn(ctx, chunk * hcap);
+      Buf<double> hp(ctx, chunk * hcap);
+      Buf<uint32_t> gfail(ctx, chunk + 1);  // [0] count, then the list (a heap of the tree's size never fills)
+      OTG_HIP_CHECK(hipMemsetAsync(gfail.p, 0, 4, ctx->stream));
+      for (uint64_t c0 = 0; c0 < nf; c0 += chunk) {
+        const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, nf - c0);
+        launch(ctx, "ray_deep_global", k_ray_deep<D, BOX, RCAP, true>, dim3(div_up(m, kQB)), dim3(kQB), 0, vs.views,
+               vs.qview, q, qs, (const uint32_t *)order.p, (const uint32_t *)deep_fail.p + c0, m, tol, inc, max_dist,
+               ids.p, cnt.p, ovf.p, band.p, Mq.p, gfail.p, gfail.p + 1, hn.p, hp.p, hcap);
+        uint32_t hf = 0;
+        OTG_HIP_CHECK(hipMemcpyAsync(&hf, gfail.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        if (hf) throw Error{OTG_ERR_HIP, "a ray walk outgrew a node heap of the tree's size"};
+      }
+      OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 12, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+    }
   }
   const uint32_t nb = hovf[0];
   if (nb == 0) return fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
-  // replay the overflowing rays with a global candidate list, growing it until all fit
-  Buf<uint32_t> bcnt(ctx, nb), fail(ctx, 1);
+  // Replay the overflowing rays with candidate lists in global memory: every ray
+  // whose list fills is replayed again with one eight times longer (at most the set's
+  // entity count, which no ray can exceed), one whose node heap fills with a heap of
+  // the tree's node count; lists and heaps are taken a bounded number of rays at a time.
+  std::vector<uint32_t> hband(nb);
+  OTG_HIP_CHECK(hipMemcpyAsync(hband.data(), band.p, (uint64_t)nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  uint64_t max_n = 1;
+  max_n = std::max<uint64_t>(max_n, vs.max_entities);
+  struct Done {
+    uint32_t q, cnt;
+    std::vector<uint32_t> ids;
+  };
+  std::vector<Done> done;
+  done.reserve(nb);
+  std::vector<uint32_t> todo = hband;
   uint32_t cap = 256;
-  Buf<double> ce;
-  Buf<uint32_t> ci;
-  for (;;) {
-    ce = Buf<double>(ctx, (uint64_t)nb * cap);
-    ci = Buf<uint32_t>(ctx, (uint64_t)nb * cap);
-    OTG_HIP_CHECK(hipMemsetAsync(fail.p, 0, 4, ctx->stream));
-    launch(ctx, "ray_replay", k_ray_replay<D, BOX>, dim3(div_up(nb, kQB)), dim3(kQB), 0, vs.views, vs.qview, q, qs,
-           (const uint32_t *)band.p, nb, tol, inc, max_dist, cap, ce.p, ci.p, bcnt.p, fail.p);
-    std::vector<uint32_t> h(nb);
-    uint32_t hfail = 0;
-    OTG_HIP_CHECK(hipMemcpyAsync(h.data(), bcnt.p, (uint64_t)nb * 4, hipMemcpyDeviceToHost, ctx->stream));
-    OTG_HIP_CHECK(hipMemcpyAsync(&hfail, fail.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    ctx->sync();
-    if (hfail) throw Error{OTG_ERR_UNSUPPORTED, "ray walk frontier exceeds the node heap"};
-    bool all = true;
-    for (uint32_t v : h) all &= v != 0xFFFFFFFFu;
-    if (all) break;
-    if ((uint64_t)cap * 8 > (1ull << 26)) throw Error{OTG_ERR_UNSUPPORTED, "more first-hit candidates than supported"};
-    cap *= 8;
+  bool global_heap = false;
+  Buf<uint32_t> fail(ctx, 1);
+  while (!todo.empty()) {
+    const uint64_t per_ray = 12ull * cap + (global_heap ? 12ull * hcap : 0ull);
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(todo.size(), kScratch / per_ray));
+    std::vector<uint32_t> again_list, again_heap;
+    for (uint64_t c0 = 0; c0 < todo.size(); c0 += chunk) {
+      const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, todo.size() - c0);
+      Buf<uint32_t> lst(ctx, m), bcnt(ctx, m);
+      Buf<double> ce(ctx, (uint64_t)m * cap);
+      Buf<uint32_t> ci(ctx, (uint64_t)m * cap);
+      Buf<uint32_t> hn(ctx, global_heap ? (uint64_t)m * hcap : 0);
+      Buf<double> hp(ctx, global_heap ? (uint64_t)m * hcap : 0);
+      OTG_HIP_CHECK(hipMemcpyAsync(lst.p, todo.data() + c0, (uint64_t)m * 4, hipMemcpyHostToDevice, ctx->stream));
+      OTG_HIP_CHECK(hipMemsetAsync(fail.p, 0, 4, ctx->stream));
+      if (global_heap)
+        launch(ctx, "ray_replay_global", k_ray_replay<D, BOX, true>, dim3(div_up(m, kQB)), dim3(kQB), 0, vs.views,
+               vs.qview, q, qs, (const uint32_t *)lst.p, m, tol, inc, max_dist, cap, ce.p, ci.p, bcnt.p, fail.p, hn.p,
+               hp.p, hcap);
+      else
+        launch(ctx, "ray_replay", k_ray_replay<D, BOX, false>, dim3(div_up(m, kQB)), dim3(kQB), 0, vs.views,
+               vs.qview, q, qs, (const uint32_t *)lst.p, m, tol, inc, max_dist, cap, ce.p, ci.p, bcnt.p, fail.p,
+               (uint32_t *)nullptr, (double *)nullptr, 0u);
+      std::vector<uint32_t> hc(m), hi((uint64_t)m * cap);
+      OTG_HIP_CHECK(hipMemcpyAsync(hc.data(), bcnt.p, (uint64_t)m * 4, hipMemcpyDeviceToHost, ctx->stream));
+      OTG_HIP_CHECK(hipMemcpyAsync(hi.data(), ci.p, (uint64_t)m * cap * 4, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t qi = todo[c0 + i];
+        if (hc[i] == kReplayListFull) again_list.push_back(qi);
+        else if (hc[i] == kReplayHeapFull) again_heap.push_back(qi);
+        else done.push_back(Done{qi, hc[i], std::vector<uint32_t>(hi.begin() + (uint64_t)i * cap,
+                                                                  hi.begin() + (uint64_t)i * cap + hc[i])});
+      }
+    }
+    if (!again_heap.empty() && global_heap && hcap >= vs.max_nodes)
+      throw Error{OTG_ERR_HIP, "a ray walk outgrew a node heap of the tree's size"};
+    // heap-full rays first move to the global heap; list-full ones get a longer list
+    if (!again_heap.empty()) {
+      global_heap = true;
+      todo = again_heap;
+      todo.insert(todo.end(), again_list.begin(), again_list.end());
+      if (!again_list.empty()) cap = (uint32_t)std::min<uint64_t>((uint64_t)cap * 8, max_n);
+    } else {
+      todo = again_list;
+      if (!todo.empty()) {
+        if (cap >= max_n) throw Error{OTG_ERR_HIP, "a ray outgrew a candidate list of the set's size"};
+        cap = (uint32_t)std::min<uint64_t>((uint64_t)cap * 8, max_n);
+      }
+    }
   }
-  launch(ctx, "ray_band_counts", k_band_counts, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb,
-         (const uint32_t *)bcnt.p, cnt.p);
+  // splice: each replayed ray's count into cnt, then its ids after the CSR is made
+  std::vector<uint32_t> dq(done.size()), dc(done.size());
+  for (size_t i = 0; i < done.size(); ++i) dq[i] = done[i].q, dc[i] = done[i].cnt;
+  const uint32_t nd = (uint32_t)done.size();
+  Buf<uint32_t> dqd(ctx, nd), dcd(ctx, nd);
+  OTG_HIP_CHECK(hipMemcpyAsync(dqd.p, dq.data(), (uint64_t)nd * 4, hipMemcpyHostToDevice, ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(dcd.p, dc.data(), (uint64_t)nd * 4, hipMemcpyHostToDevice, ctx->stream));
+  launch(ctx, "ray_band_counts", k_band_counts, dim3(div_up(nd, 256)), dim3(256), 0, (const uint32_t *)dqd.p, nd,
+         (const uint32_t *)dcd.p, cnt.p);
   Result *r = fixed_to_csr(ctx, ids.p, cnt.p, RCAP, nq);
-  launch(ctx, "ray_replay_copy", k_replay_copy, dim3(div_up(nb, 256)), dim3(256), 0, (const uint32_t *)band.p, nb, cap,
-         (const uint32_t *)bcnt.p, (const uint32_t *)ci.p, (const uint64_t *)r->d_offsets, r->d_ids);
+  std::vector<uint64_t> poff(nd + 1, 0);
+  for (uint32_t i = 0; i < nd; ++i) poff[i + 1] = poff[i] + done[i].cnt;
+  std::vector<uint32_t> packed(std::max<uint64_t>(poff[nd], 1));
+  for (uint32_t i = 0; i < nd; ++i) std::copy(done[i].ids.begin(), done[i].ids.end(), packed.begin() + poff[i]);
+  Buf<uint64_t> poffd(ctx, nd + 1);
+  Buf<uint32_t> packedd(ctx, packed.size());
+  OTG_HIP_CHECK(hipMemcpyAsync(poffd.p, poff.data(), (uint64_t)(nd + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(packedd.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  launch(ctx, "ray_replay_splice", k_replay_splice, dim3(div_up(nd, 256)), dim3(256), 0, (const uint32_t *)dqd.p, nd,
+         (const uint64_t *)poffd.p, (const uint32_t *)packedd.p, (const uint64_t *)r->d_offsets, r->d_ids);
+  ctx->sync();  // the host arrays above must outlive the copies
   return r;
 }
 
@@ -3209,6 +3517,8 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
     if (t->kind != kind || t->dim != dim) throw Error{OTG_ERR_INVALID_ARG, "trees of one launch differ in kind or dim"};
     const Tree *x = query == 0 ? knn_index(t) : (query == 1 ? range_index(t) : t);
     hv.push_back(query == 2 ? reference_view_of(x) : view_of(x));
+    vs.max_nodes = std::max<uint64_t>(vs.max_nodes, x->node_count);
+    vs.max_entities = std::max<uint64_t>(vs.max_entities, x->n);
     vs.cbits = std::max(vs.cbits, x->layout.cbits);
     vs.sbits = std::max(vs.sbits, x->layout.sbits);
     total_n += x->n;

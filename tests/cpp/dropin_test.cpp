@@ -49,6 +49,12 @@ int main() {
   EXPECT(same_set(qt.GetNearestNeighbors({3.5, 5.5}, 4, 0.0, pts), {17, 18}));           // :2510
   EXPECT(same_set(qt.GetNearestNeighbors({-1.0, -1.0}, 5, pts), {0, 1, 5, 6, 11}));      // :2593
   EXPECT(same_set(qt.GetNearestNeighbors({2.0, 2.0}, 2, pts), {1, 11}));                 // :2558
+  {  // :2532 -- k = 100 over 32 points: every point, nearest first
+    auto all = qt.GetNearestNeighbors({3.5, 5.5}, 100, pts);
+    std::vector<index_t> every(pts.size());
+    for (size_t i = 0; i < every.size(); ++i) every[i] = (index_t)i;
+    EXPECT(all.size() == pts.size() && same_set(all, every));
+  }
   STEP("node api");
   // node API: children ranges cover every node once, entities cover every id once
   {

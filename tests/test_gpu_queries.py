@@ -67,8 +67,6 @@ def test_knn_kats(gpu_ctx):
     k = kat()["knn_point_set_no1"]
     g = _tree(gpu_ctx, "point", 2, np.array(k["points"]), None, k["max_depth"], k["cap"])
     for c in k["cases"]:
-        if c["k"] > 64:
-            continue
         got = g.GetNearestNeighbors(c["q"], c["k"], c.get("max_within", DBL_MAX), None, c.get("tol", BASE_TOL))
         assert sorted(got) == sorted(c["expect"]), c["src"]
     for c in kat()["knn_issue9"]:
@@ -231,6 +229,59 @@ def test_ray_band_longer_than_list(gpu_ctx):
     ooff, oids = o.ray_first(rays, 0.5, threads=8)
     assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
     assert goff[1] - goff[0] > 16 and "ray_replay" in kt, (goff, sorted(kt))
+
+
+def test_ray_frontier_beyond_private_heaps(gpu_ctx):
+    """Rays from far outside a dense tree with tolerance cones that widen with
+    distance (increment 0.2) reach every node, best-first in enter order: the queued
+    frontier becomes a slab of the tree, wider than the LDS heap and the private
+    heap, and every entity is a candidate.  The walk then runs with a node heap of
+    the tree's size in global memory and the replay grows its candidate list to the
+    set's size; the ids equal the oracle's.  A tolerance that hits everything at
+    enter distance 0 (a depth-first frontier) is checked too."""
+    D, n = 3, 30000
+    pts = uniform_points(n, D, seed=83)
+    g = _tree(gpu_ctx, "point", D, pts, None, 8, 2)
+    o = O.OracleTree(D, "point", pts, None, 8, 2)
+    org = np.array([[-1000.0, 50.0, 50.0], [50.0, -1000.0, 40.0], [-600.0, -700.0, 30.0]])
+    dirs = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]] + [list(-org[2] / np.linalg.norm(org[2]))])
+    rays = np.concatenate([org, dirs], axis=1)
+    for tol, inc, wide in ((1.0, 0.2, True), (1e6, 0.0, False)):
+        gpu_ctx.set_profiling(True)
+        gpu_ctx.reset_kernel_times()
+        goff, gids = g.ray_first_batch(rays, tol, inc).csr()
+        kt = gpu_ctx.kernel_times()
+        gpu_ctx.set_profiling(False)
+        print(f"tolerance {tol}: {int(goff[-1])} ids, kernels {sorted(kt)}", flush=True)
+        ooff, oids = o.ray_first(rays, tol, inc, threads=8)
+        assert np.array_equal(goff, ooff), (tol, goff, ooff)
+        for i in range(len(rays)):  # equal enter distances: compared as sets within each run
+            a, b = gids[goff[i]:goff[i + 1]], oids[ooff[i]:ooff[i + 1]]
+            assert sorted(a.tolist()) == sorted(b.tolist()), (tol, i)
+        assert goff[1] - goff[0] > 500
+        if wide:
+            assert "ray_deep_global" in kt and "ray_replay_global" in kt, sorted(kt)
+
+
+# ---- kNN with k above the register lists ---------------------------------------------------
+@pytest.mark.parametrize("kind,D,k", [("point", 3, 100), ("point", 2, 257), ("point", 6, 80), ("box", 3, 90)])
+def test_knn_k_above_64(gpu_ctx, kind, D, k):
+    n = 20000
+    if kind == "point":
+        geom = gaussian_mixture(n, D, 6, 0.0, 100.0, 1.0, 6.0, seed=D + k)
+        geom[:300] = geom[300:600]  # exact duplicates: equal distances
+    else:
+        geom = lognormal_boxes(n, D, 0.0, 100.0, np.log(0.5), 0.8, seed=D)
+    qs = uniform_points(3000, D, 0.0, 100.0, seed=k)
+    qs[:30] = (geom[:30, :D] if kind == "point" else geom[:30, :D] + 0.1)
+    g = _tree(gpu_ctx, kind, D, geom, None, 6, 16)
+    o = O.OracleTree(D, kind, geom, None, 6, 16)
+    for mw in (DBL_MAX, 3.0):
+        goff, gids = g.knn_batch(qs, k, mw).csr()
+        ooff, oids = o.knn(qs, k, mw, threads=8)
+        assert np.array_equal(goff, ooff) and np.array_equal(gids, oids), (kind, D, k, mw)
+        if mw == DBL_MAX:
+            assert (np.diff(goff) >= k).all()
 
 
 # ---- collision -----------------------------------------------------------------------
