@@ -62,6 +62,7 @@ typedef int otg_status;
 typedef struct otg_ctx otg_ctx;
 typedef struct otg_tree otg_tree;
 typedef struct otg_result otg_result;
+typedef struct otg_knn_plan otg_knn_plan;
 
 typedef struct {
   int32_t kind;        /* OTG_KIND_* */
@@ -158,13 +159,16 @@ OTG_API otg_status otg_tree_serialize(const otg_tree *tree, uint32_t set, uint32
  * the others live in the context and apply to every later build and query on it:
  * index_capacity = leaf capacity of the finer kNN / range index (0: the tree's),
  * waves_per_simd = occupancy cap of the kNN walks (0: as registers allow),
- * order_queries = 1 to walk batches of >= 16,384 queries in Morton order.  Query
- * results never depend on them. */
+ * order_queries = 1 to walk batches of >= 16,384 queries in Morton order,
+ * persistent = 1 to run the kNN walks as a persistent launch.  Query results never
+ * depend on them. */
 typedef struct {
   uint32_t leaf_capacity;
   uint32_t index_capacity;
   uint32_t waves_per_simd;
   uint32_t order_queries;
+  uint32_t persistent; /* 1: kNN walks as a persistent launch pulling 64-query tiles across sets */
+  uint32_t reserved;
   double step_ms; /* measured build + queries with these settings (otg_autotune) */
 } otg_tuning;
 
@@ -260,6 +264,27 @@ OTG_API otg_status otg_ray_all(const otg_tree *tree, uint64_t nq, const double *
 OTG_API otg_status otg_knn_noalloc(const otg_tree *tree, uint64_t nq, const double *points, const uint32_t *set_ids,
                                    uint32_t k, double max_within, double tolerance, uint64_t *offsets, uint32_t *ids,
                                    uint64_t ids_capacity, uint64_t *nids);
+
+/* kNN plans.  No counterpart in the reference (its queries are single calls): for
+ * callers that send many batches of the same size, e.g. a solver that queries the
+ * same particles every step.  otg_knn_plan_create records one otg_knn batch over
+ * device-resident points[nq * dim] (and set_ids, or NULL) as a HIP graph, with
+ * every buffer allocated up front; otg_knn_plan_run replays it on the current
+ * contents of those buffers (the caller may rewrite them between runs, not move
+ * them) and synchronises.  Results are the ids otg_knn returns for the same
+ * contents, in the same order.  The tuning in force at creation is kept.
+ * otg_knn_plan_result gives the last run's device CSR, valid until the next run or
+ * otg_knn_plan_free.  1 <= k <= 64 (OTG_ERR_UNSUPPORTED beyond); a set id outside
+ * the tree fails the run with OTG_ERR_INVALID_ARG.  Free plans before their tree. */
+OTG_API otg_status otg_knn_plan_create(const otg_tree *tree, uint64_t nq, const double *points,
+                                       const uint32_t *set_ids, uint32_t k, double max_within, double tolerance,
+                                       otg_knn_plan **out);
+OTG_API otg_status otg_knn_plan_run(otg_knn_plan *plan);
+OTG_API otg_status otg_knn_plan_result(const otg_knn_plan *plan, const uint64_t **offsets, const uint32_t **ids,
+                                       uint64_t *nids);
+/* Host copy of the last run's CSR: offsets[nq + 1], ids[nids]. */
+OTG_API otg_status otg_knn_plan_copy(const otg_knn_plan *plan, uint64_t *offsets, uint32_t *ids);
+OTG_API void otg_knn_plan_free(otg_knn_plan *plan);
 
 OTG_API otg_status otg_result_get_info(const otg_result *res, uint64_t *nq, uint64_t *nids);
 OTG_API otg_status otg_result_copy(const otg_result *res, uint64_t *offsets, uint32_t *ids);

@@ -113,8 +113,9 @@ class Context:
         _check(st, "otg_autotune", self)
         return {f: getattr(out, f) for f, _ in _abi.Tuning._fields_}
 
-    def set_tuning(self, index_capacity: int = 0, waves_per_simd: int = 0, order_queries: int = 1) -> None:
-        t = _abi.Tuning(0, index_capacity, waves_per_simd, order_queries, 0.0)
+    def set_tuning(self, index_capacity: int = 0, waves_per_simd: int = 0, order_queries: int = 1,
+                   persistent: int = 0) -> None:
+        t = _abi.Tuning(0, index_capacity, waves_per_simd, order_queries, persistent, 0, 0.0)
         _check(_abi.lib().otg_ctx_set_tuning(self.handle, C.byref(t)), "otg_ctx_set_tuning", self)
 
     def tuning(self) -> dict:
@@ -176,6 +177,47 @@ class Result:
     def free(self):
         if self.handle:
             _abi.lib().otg_result_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class KnnPlan:
+    """A captured kNN batch (otg_knn_plan): run() replays it on the current contents of
+    the query tensors it was made from; csr() / device_csr() read the last run."""
+
+    def __init__(self, handle, tree, nq: int, keep):
+        self.handle = handle
+        self.tree = tree
+        self.nq = nq
+        self._keep = keep  # the query tensors the graph reads
+
+    def run(self) -> "KnnPlan":
+        _check(_abi.lib().otg_knn_plan_run(self.handle), "otg_knn_plan_run", self.tree.ctx)
+        return self
+
+    def device_csr(self):
+        """(offsets pointer, ids pointer, id count) of the last run, in device memory."""
+        off, ids, n = C.c_void_p(), C.c_void_p(), C.c_uint64()
+        _check(_abi.lib().otg_knn_plan_result(self.handle, C.byref(off), C.byref(ids), C.byref(n)),
+               "otg_knn_plan_result", self.tree.ctx)
+        return off.value, ids.value, n.value
+
+    def csr(self):
+        n = self.device_csr()[2]
+        offsets = np.empty(self.nq + 1, dtype=np.uint64)
+        ids = np.empty(max(n, 1), dtype=np.uint32)
+        _check(_abi.lib().otg_knn_plan_copy(self.handle, _ptr(offsets), _ptr(ids)), "otg_knn_plan_copy",
+               self.tree.ctx)
+        return offsets, ids[:n]
+
+    def free(self):
+        if getattr(self, "handle", None):
+            _abi.lib().otg_knn_plan_free(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -293,6 +335,8 @@ class StaticTree:
 
     def free(self):
         if getattr(self, "handle", None):
+            for plan in list(getattr(self, "_plans", ())):
+                plan.free()
             _abi.lib().otg_tree_free(self.handle)
             self.handle = None
 
@@ -381,6 +425,26 @@ class StaticTree:
         st = _abi.lib().otg_knn(self.handle, nq, qp, sp, mem, int(k), float(max_within), float(tolerance), C.byref(h))
         _check(st, "otg_knn", self.ctx)
         return Result(h, self.ctx)
+
+    def knn_plan(self, points, k: int, max_within: float = DBL_MAX, tolerance: float = BASE_TOLERANCE,
+                 set_ids=None) -> "KnnPlan":
+        """otg_knn_plan_create: the kNN batch over these device tensors, captured once
+        and replayed by KnnPlan.run() on whatever they hold at that time."""
+        if not _is_device(points):
+            raise ValueError("a kNN plan takes device-resident query points")
+        qp, nq = _dev(points, self.dim)
+        if set_ids is not None and (not _is_device(set_ids) or not set_ids.is_contiguous()):
+            raise ValueError("set ids of a plan must be a contiguous device tensor")
+        sp = None if set_ids is None else C.c_void_p(set_ids.data_ptr())
+        h = C.c_void_p()
+        st = _abi.lib().otg_knn_plan_create(self.handle, nq, qp, sp, int(k), float(max_within), float(tolerance),
+                                            C.byref(h))
+        _check(st, "otg_knn_plan_create", self.ctx)
+        plan = KnnPlan(h, self, nq, (points, set_ids))
+        if not hasattr(self, "_plans"):
+            self._plans = weakref.WeakSet()
+        self._plans.add(plan)
+        return plan
 
     def knn_footprint(self, points, k: int, max_within: float = DBL_MAX, tolerance: float = BASE_TOLERANCE,
                       set_ids=None) -> dict:
@@ -657,7 +721,7 @@ def StaticOrthoTreeBoxND(dim: int, loose: bool = True):
 
 
 __all__ = [
-    "Context", "Result", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
+    "Context", "Result", "KnnPlan", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
     "StaticQuadtreePoint", "StaticQuadtreeBox", "StaticOctreePoint", "StaticOctreeBox", "StaticHexatreePoint",
     "StaticHexatreeBox", "StaticOrthoTreePointND", "StaticOrthoTreeBoxND", "default_context",
 ]

@@ -563,6 +563,59 @@ OTG_API otg_status otg_knn(const otg_tree *t, uint64_t nq, const double *points,
   });
 }
 
+OTG_API otg_status otg_knn_plan_create(const otg_tree *t, uint64_t nq, const double *points, const uint32_t *set_ids,
+                                       uint32_t k, double max_within, double tolerance, otg_knn_plan **out) {
+  if (!t || !out || !points || nq == 0) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = reinterpret_cast<otg_knn_plan *>(otg::knn_plan_create(t, nq, points, set_ids, k, max_within, tolerance));
+  });
+}
+
+OTG_API otg_status otg_knn_plan_run(otg_knn_plan *plan) {
+  if (!plan) return OTG_ERR_INVALID_ARG;
+  auto *p = reinterpret_cast<otg::KnnPlan *>(plan);
+  Ctx *ctx = otg::knn_plan_ctx(p);
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::knn_plan_run(p);
+  });
+}
+
+OTG_API otg_status otg_knn_plan_result(const otg_knn_plan *plan, const uint64_t **offsets, const uint32_t **ids,
+                                       uint64_t *nids) {
+  if (!plan || !offsets || !ids || !nids) return OTG_ERR_INVALID_ARG;
+  otg::knn_plan_result(reinterpret_cast<const otg::KnnPlan *>(plan), offsets, ids, nids);
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_knn_plan_copy(const otg_knn_plan *plan, uint64_t *offsets, uint32_t *ids) {
+  if (!plan || !offsets) return OTG_ERR_INVALID_ARG;
+  auto *p = reinterpret_cast<const otg::KnnPlan *>(plan);
+  Ctx *ctx = otg::knn_plan_ctx(const_cast<otg::KnnPlan *>(p));
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    const uint64_t *d_off;
+    const uint32_t *d_ids;
+    uint64_t nids = 0;
+    otg::knn_plan_result(p, &d_off, &d_ids, &nids);
+    if (nids && !ids) throw Error{OTG_ERR_INVALID_ARG, "ids buffer"};
+    OTG_HIP_CHECK(hipMemcpyAsync(offsets, d_off, (otg::knn_plan_queries(p) + 1) * 8, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    if (nids) OTG_HIP_CHECK(hipMemcpyAsync(ids, d_ids, nids * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+OTG_API void otg_knn_plan_free(otg_knn_plan *plan) {
+  if (!plan) return;
+  auto *p = reinterpret_cast<otg::KnnPlan *>(plan);
+  (void)hipSetDevice(otg::knn_plan_ctx(p)->device);
+  otg::knn_plan_free(p);
+}
+
 OTG_API otg_status otg_knn_footprint(const otg_tree *t, uint64_t nq, const double *points, const uint32_t *set_ids,
                                      int mem, uint32_t k, double max_within, double tolerance, uint64_t *out) {
   if (!t || !out || (nq && !points)) return OTG_ERR_INVALID_ARG;

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -65,6 +66,7 @@ struct Tuning {
   uint32_t index_capacity = 0;  // leaf capacity of the finer kNN / range index (0: the tree's)
   uint32_t waves_per_simd = 0;  // occupancy cap of the kNN walks (0: as registers allow)
   uint32_t order_queries = 1;   // Morton-order batches of >= 16,384 queries before walking
+  uint32_t persistent = 0;      // kNN walks as a persistent launch pulling 64-query tiles
 };
 
 class Ctx {
@@ -188,7 +190,7 @@ void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,
 void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals);
 // Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
 // result is in keys/vals on return.
-void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits);
+void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits, Arena *arena = nullptr);
 // Boundary-level scans: for v[0..n) (uint8 levels, 0 = hard boundary) computes
 // left[i] = set of running minima of v[i], v[i-1], ... (inclusive, stops at level 0) and
 // right[i] = same going right, as bit masks.
@@ -293,6 +295,18 @@ struct CsrOut {
 // allocation, queries walked in input order; synchronises before returning.
 void query_knn_noalloc(const Tree *t, uint64_t nq, const double *pts, const uint32_t *sets, uint32_t k,
                        double max_within, double tol, Arena &arena, CsrOut &out);
+// kNN plan: one batch of device-resident queries captured once as a HIP graph (set
+// check, query ordering, walk, CSR) and replayed on new contents of the same
+// buffers.  run() synchronises; a tolerance band longer than a list is completed by
+// the band pass outside the graph.  The result stays valid until the next run.
+struct KnnPlan;
+KnnPlan *knn_plan_create(const Tree *t, uint64_t nq, const double *q, const uint32_t *sets, uint32_t k,
+                         double max_within, double tol);
+void knn_plan_run(KnnPlan *p);
+void knn_plan_result(const KnnPlan *p, const uint64_t **offsets, const uint32_t **ids, uint64_t *nids);
+void knn_plan_free(KnnPlan *p);
+Ctx *knn_plan_ctx(KnnPlan *p);
+uint64_t knn_plan_queries(const KnnPlan *p);
 Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
                     double tol);
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,

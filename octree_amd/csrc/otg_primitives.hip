@@ -476,15 +476,15 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   }
 }
 
-void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits) {
+void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits, Arena *arena) {
   if (n <= 1 || bits == 0) return;
   if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
   const uint32_t passes = std::min<uint32_t>((bits + 7) / 8, kOsMaxPasses);
   const uint32_t ntiles = div_up(n, kOsTile);
-  Buf<uint64_t> k2(ctx, n);
-  Buf<uint32_t> v2(ctx, n);
-  Buf<OsState> st(ctx, 1);
-  Buf<uint32_t> status(ctx, (size_t)2 * 256 * ntiles);
+  Buf<uint64_t> k2(ctx, arena, n);
+  Buf<uint32_t> v2(ctx, arena, n);
+  Buf<OsState> st(ctx, arena, 1);
+  Buf<uint32_t> status(ctx, arena, (size_t)2 * 256 * ntiles);
   uint32_t *stat[2] = {status.p, status.p + (size_t)256 * ntiles};
   launch(ctx, "radix_init", k_os_init, dim3(std::min<uint32_t>(div_up((uint64_t)256 * ntiles, kBlock), 1024)),
          dim3(kBlock), 0, st.p, stat[0], (uint64_t)256 * ntiles);

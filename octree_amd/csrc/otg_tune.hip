@@ -6,7 +6,8 @@
 //                                     (ot_static_linear_core.h:523-529);
 //   index capacity  {leaf, 16, 32, 64} leaf capacity of the finer kNN / range index;
 //   waves per SIMD  {registers, 2, 1}  occupancy cap of the kNN walks;
-//   query order     {Morton, input}    walk order of large batches.
+//   query order     {Morton, input}    walk order of large batches;
+//   kNN launch      {one tile per block, persistent tile pulling}.
 // Each later stage keeps the earlier stages' choices.  The pick is cached per
 // workload shape (kind, dimension, sets, entities, queries, k) for the process, and
 // left in the context's settings, where every later build and query on it uses it.
@@ -39,10 +40,11 @@ extern "C" {
 
 OTG_API otg_status otg_ctx_set_tuning(otg_ctx *ctx, const otg_tuning *t) {
   if (!ctx || !t) return OTG_ERR_INVALID_ARG;
-  if (t->waves_per_simd > 8 || t->order_queries > 1) return OTG_ERR_INVALID_ARG;
+  if (t->waves_per_simd > 8 || t->order_queries > 1 || t->persistent > 1) return OTG_ERR_INVALID_ARG;
   ctx->tuning.index_capacity = t->index_capacity;
   ctx->tuning.waves_per_simd = t->waves_per_simd;
   ctx->tuning.order_queries = t->order_queries;
+  ctx->tuning.persistent = t->persistent;
   return OTG_OK;
 }
 
@@ -52,6 +54,8 @@ OTG_API otg_status otg_ctx_get_tuning(const otg_ctx *ctx, otg_tuning *t) {
   t->index_capacity = ctx->tuning.index_capacity;
   t->waves_per_simd = ctx->tuning.waves_per_simd;
   t->order_queries = ctx->tuning.order_queries;
+  t->persistent = ctx->tuning.persistent;
+  t->reserved = 0;
   t->step_ms = 0.0;
   return OTG_OK;
 }
@@ -129,14 +133,16 @@ OTG_API otg_status otg_autotune(otg_ctx *ctx, int kind, int dim, uint32_t nsets,
     double qbest = 1e300;
     Tuning qpick = pick;
     for (uint32_t waves : {0u, 2u, 1u})
-      for (uint32_t order : {1u, 0u}) {
-        ctx->tuning = pick;
-        ctx->tuning.waves_per_simd = waves;
-        ctx->tuning.order_queries = order;
-        std::unique_ptr<otg_tree> none;
-        const double ms = best_of([&] { return step(best_cap, false, tree.get(), none); });
-        if (ms < qbest) qbest = ms, qpick = ctx->tuning;
-      }
+      for (uint32_t order : {1u, 0u})
+        for (uint32_t persist : {0u, 1u}) {
+          ctx->tuning = pick;
+          ctx->tuning.waves_per_simd = waves;
+          ctx->tuning.order_queries = order;
+          ctx->tuning.persistent = persist;
+          std::unique_ptr<otg_tree> none;
+          const double ms = best_of([&] { return step(best_cap, false, tree.get(), none); });
+          if (ms < qbest) qbest = ms, qpick = ctx->tuning;
+        }
     const double qdefault = [&] {
       ctx->tuning = pick;
       std::unique_ptr<otg_tree> none;
@@ -148,6 +154,8 @@ OTG_API otg_status otg_autotune(otg_ctx *ctx, int kind, int dim, uint32_t nsets,
     res.index_capacity = qpick.index_capacity;
     res.waves_per_simd = qpick.waves_per_simd;
     res.order_queries = qpick.order_queries;
+    res.persistent = qpick.persistent;
+    res.reserved = 0;
     res.step_ms = best - qdefault + qbest;
     ctx->tuning = qpick;
     {

@@ -48,6 +48,7 @@ def test_tuned_results_equal_untuned():
         assert tune["leaf_capacity"] in (8, 16, 32, 64)
         assert tune["index_capacity"] in (0, 16, 32, 64)
         assert tune["waves_per_simd"] in (0, 1, 2) and tune["order_queries"] in (0, 1)
+        assert tune["persistent"] in (0, 1)
         assert tune["step_ms"] > 0.0
         assert ctx.tuning()["waves_per_simd"] == tune["waves_per_simd"]
         got = _run(ctx, sets, box, tune["leaf_capacity"], knn, sid, rq)
@@ -56,8 +57,8 @@ def test_tuned_results_equal_untuned():
         # the pick is cached for the shape: a second call returns it unchanged
         assert ctx.autotune("point", 3, offs, pts, box, None, knn, sid, 16, rq, sid) == tune
         # every knob forced away from its default leaves the ids alone
-        for setting in (dict(index_capacity=8), dict(waves_per_simd=1), dict(order_queries=0),
-                        dict(index_capacity=64, waves_per_simd=2, order_queries=0)):
+        for setting in (dict(index_capacity=8), dict(waves_per_simd=1), dict(order_queries=0), dict(persistent=1),
+                        dict(index_capacity=64, waves_per_simd=2, order_queries=0, persistent=1)):
             ctx.set_tuning(**setting)
             for cap in (8, 64):
                 got = _run(ctx, sets, box, cap, knn, sid, rq)
