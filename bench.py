@@ -300,6 +300,16 @@ def main():
     b_range = float(np.median([x[2] for x in bt]))
     b_step = float(np.median([x[3] for x in bt]))
 
+    # ---- the same kNN batch as a captured plan (otg_knn_plan: one HIP graph replay
+    # per batch, every buffer allocated once) beside the eager call on one forest ----
+    fg = octree_amd.StaticOctreePoint.forest(None, cap, None, box, ctx=ctx, device_geom=(d_pts, offs))
+    plan = fg.knn_plan(d_knn, args.k, set_ids=d_sid)
+    g_ms, _ = _timed(ctx, lambda: plan.run() and None, max(3, args.steps))
+    e_ms, _ = _timed(ctx, lambda: fg.knn_batch(d_knn, args.k, set_ids=d_sid), max(3, args.steps))
+    plan.free()
+    fg.free()
+    log(f"kNN batch: eager {e_ms * 1e3:.2f} ms, graph replay {g_ms * 1e3:.2f} ms")
+
     # ---- per-kernel times (HIP events on the library stream) for the roofline ----
     ctx.set_profiling(True)
     ctx.reset_kernel_times()
@@ -343,6 +353,7 @@ def main():
                     "step_ms": b_step * 1e3, "build_ms": b_build * 1e3, "knn_ms": b_knn * 1e3, "range_ms": b_range * 1e3,
                     "p50_step_ms": float(np.percentile([x[3] for x in bt], 50)) * 1e3,
                     "p99_step_ms": float(np.percentile([x[3] for x in bt], 99)) * 1e3},
+        "knn_graph": {"eager_ms": e_ms * 1e3, "graph_ms": g_ms * 1e3, "graph_qps": Q / g_ms},
         "kernel_ms": {k: v[0] for k, v in kt.items()},
         "build_roofline": {"bound": "hbm", "algorithmic_bytes": build_bytes, "radix_passes": passes,
                            "key_bits": bits, "nodes": nnodes, "build_ms": b_build * 1e3,
@@ -450,6 +461,10 @@ def run_single(args, octree_amd, torch):
         tree = octree_amd.StaticOctreePoint(ctx=ctx)
         tb, _ = _timed(ctx, lambda: tree.Create(d_pts, 10, box, 16), reps)
         tk, _ = _timed(ctx, lambda: tree.knn_batch(d_q, 8), reps)
+        plan = tree.knn_plan(d_q, 8)
+        tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
+        plan.free()
+        out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
                     "config": {"workload": "OctreePoint 1,000,000 pts U[0,100)^3, depth 10, cap 16; 10,000 kNN k=8"},
                     "build_ms": tb * 1e3, "build_mpts_s": 1.0 / tb, "knn_ms": tk * 1e3,
@@ -563,6 +578,10 @@ def run_single(args, octree_amd, torch):
         tree = T(ctx=ctx)
         tb, _ = _timed(ctx, lambda: tree.Create(d_p, 8, box, 32), reps)
         tk, _ = _timed(ctx, lambda: tree.knn_batch(d_q, 16), reps)
+        plan = tree.knn_plan(d_q, 16)
+        tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
+        plan.free()
+        out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
         # CPU: oracle build of a 1M-point sample and 20k queries on it
         m = 1_000_000
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,

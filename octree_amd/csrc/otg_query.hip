@@ -718,9 +718,42 @@ __device__ __forceinline__ uint64_t next_tile(uint32_t *tiles) {
   return (uint64_t)__shfl(t, 0, 64);
 }
 
-// qk: per-query k (null: k for all).  tiles: null = one 64-query tile per block;
-// else a persistent launch -- each block takes tiles from the counter (zeroed
-// before the launch) until none is left, so uneven sets finish as one pool.
+// One query of a kNN launch (one lane): its tree, then the walk of its dimension.
+template <int D, int KCAP, bool BOX>
+__device__ __forceinline__ void knn_lane(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                                         const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
+                                         const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
+                                         uint64_t ti, uint32_t k, double F0, double tol,
+                                         uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                         uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
+                                         double *__restrict__ out_F, uint32_t *s_n, uint32_t *s_w) {
+  constexpr int ML = MaskLevels<D>::value;
+  const uint64_t qi = order ? order[ti] : ti;
+  const uint32_t kq = qk ? qk[qi] : k;
+  with_tree(views, qview, qi, [&](const TreeView &tv) {
+    if constexpr (D <= 3) {
+      LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
+      knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
+    } else if constexpr (D <= 6) {
+      knn_query_masks<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, &s_n[threadIdx.x], &s_w[threadIdx.x],
+                                    &s_w[ML * kQB + threadIdx.x], out_ids, out_cnt, overflow, band, out_F);
+    } else {
+      ArrayStack<D> S;
+      knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
+    }
+  });
+}
+
+// 2-D/3-D: level entries in LDS, one per tree level.  4-D..6-D: one child mask per
+// level in LDS.  7-D and up: the worst-case private stack.  None can overflow.
+static_assert(kLdsStack >= kMaxDepth, "one level entry per internal depth");
+// 2-D/3-D: level stack (first child, packed order); 4-D..6-D: first child and the two
+// halves of the child mask, MaskLevels entries each
+#define OTG_KNN_LDS(D)                                                        \
+  __shared__ uint32_t s_n[(D <= 3 ? kLdsStack : MaskLevels<D>::value) * kQB]; \
+  __shared__ uint32_t s_w[(D <= 3 ? kLdsStack : 2 * MaskLevels<D>::value) * kQB];
+
+// One 64-query tile per block.  qk: per-query k (null: k for all).
 template <int D, int KCAP, bool BOX>
 __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                                               const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
@@ -728,35 +761,36 @@ __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ vie
                                               uint64_t nq, uint32_t k, double F0, double tol,
                                               uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
                                               uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
-                                              double *__restrict__ out_F, uint32_t *__restrict__ tiles) {
-  // 2-D/3-D: level entries in LDS, one per tree level.  4-D..6-D: one child mask
-  // per level in LDS.  7-D and up: the worst-case private stack.  None can overflow.
-  static_assert(kLdsStack >= kMaxDepth, "one level entry per internal depth");
-  // 2-D/3-D: level stack (first child, packed order); 4-D..6-D: first child and the
-  // two halves of the child mask, MaskLevels entries each
-  constexpr int ML = MaskLevels<D>::value;
-  __shared__ uint32_t s_n[(D <= 3 ? kLdsStack : ML) * kQB];
-  __shared__ uint32_t s_w[(D <= 3 ? kLdsStack : 2 * ML) * kQB];
+                                              double *__restrict__ out_F) {
+  OTG_KNN_LDS(D)
+  const uint64_t ti = (uint64_t)blockIdx.x * kQB + threadIdx.x;
+  if (ti >= nq) return;
+  knn_lane<D, KCAP, BOX>(views, qview, qpts, qset, qk, order, ti, k, F0, tol, out_ids, out_cnt, overflow, band, out_F,
+                         s_n, s_w);
+}
+
+// Persistent launch of the same walk: each block takes 64-query tiles from the counter
+// (zeroed before the launch) until none is left, so uneven sets finish as one pool.
+// A separate kernel: the tile loop keeps every argument live, which costs the
+// one-tile kernel registers.
+template <int D, int KCAP, bool BOX>
+__global__ void __launch_bounds__(kQB, 3)
+    k_knn_persistent(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
+                     const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
+                     const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order, uint64_t nq, uint32_t k,
+                     double F0, double tol, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                     uint32_t *__restrict__ overflow, uint32_t *__restrict__ band, double *__restrict__ out_F,
+                     uint32_t *__restrict__ tiles) {
+  OTG_KNN_LDS(D)
   const uint64_t ntile = (nq + kQB - 1) / kQB;
-  for (uint64_t tile = tiles ? next_tile(tiles) : blockIdx.x; tile < ntile; tile = tiles ? next_tile(tiles) : ntile) {
+  for (uint64_t tile = next_tile(tiles); tile < ntile; tile = next_tile(tiles)) {
     const uint64_t ti = tile * kQB + threadIdx.x;
-    if (ti >= nq) continue;
-    const uint64_t qi = order ? order[ti] : ti;
-    const uint32_t kq = qk ? qk[qi] : k;
-    with_tree(views, qview, qi, [&](const TreeView &tv) {
-      if constexpr (D <= 3) {
-        LdsLevelStack S{&s_n[threadIdx.x], &s_w[threadIdx.x]};
-        knn_query_levels<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
-      } else if constexpr (D <= 6) {
-        knn_query_masks<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, &s_n[threadIdx.x], &s_w[threadIdx.x],
-                                      &s_w[ML * kQB + threadIdx.x], out_ids, out_cnt, overflow, band, out_F);
-      } else {
-        ArrayStack<D> S;
-        knn_query<D, KCAP, BOX>(tv, qi, qpts, qset, kq, F0, tol, S, out_ids, out_cnt, overflow, band, out_F);
-      }
-    });
+    if (ti < nq)
+      knn_lane<D, KCAP, BOX>(views, qview, qpts, qset, qk, order, ti, k, F0, tol, out_ids, out_cnt, overflow, band,
+                             out_F, s_n, s_w);
   }
 }
+#undef OTG_KNN_LDS
 
 // Queries whose band below the final F held more ids than the register list: a
 // fixed-radius walk collects every entity with (optimistic) distance below F --
@@ -2799,16 +2833,19 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   Buf<double> Fq(ctx, ar, nq);
   OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));  // band count
   {
-    const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
-    Buf<uint32_t> tiles;
-    uint32_t grid = div_up(nq, kQB);
     if (ctx->tuning.persistent) {
-      tiles = Buf<uint32_t>(ctx, ar, 1);
+      const void *kp = (const void *)k_knn_persistent<D, KCAP, BOX>;
+      const size_t pad = occupancy_pad(ctx, kp, ctx->tuning.waves_per_simd);
+      Buf<uint32_t> tiles(ctx, ar, 1);
       launch(ctx, "fill_u32", k_fill_u32, dim3(1), dim3(64), 0, tiles.p, (uint64_t)1, 0u);
-      grid = persistent_grid(ctx, (const void *)k_knn<D, KCAP, BOX>, kQB, pad, grid);
+      launch(ctx, "knn", k_knn_persistent<D, KCAP, BOX>, dim3(persistent_grid(ctx, kp, kQB, pad, div_up(nq, kQB))),
+             dim3(kQB), pad, vs.views, vs.qview, q, qs, qk, (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p,
+             ovf.p, band.p, Fq.p, tiles.p);
+    } else {
+      const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
+      launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), pad, vs.views, vs.qview, q, qs, qk,
+             (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
     }
-    launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(grid), dim3(kQB), pad, vs.views, vs.qview, q, qs, qk,
-           (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p, tiles.p);
   }
   uint32_t hovf[1] = {0};
   OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -3724,16 +3761,18 @@ static void knn_plan_body(KnnPlan &p, Arena &ar) {
     qs = p.sets_clean;
   }
   Buf<uint32_t> order = query_order<D>(ctx, vs, p.q, D, qs, nq, &ar);
-  const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
-  uint32_t grid = div_up(nq, kQB);
-  uint32_t *tiles = nullptr;
   if (ctx->tuning.persistent) {
-    tiles = p.flags + 3;
-    grid = persistent_grid(ctx, (const void *)k_knn<D, KCAP, BOX>, kQB, pad, grid);
+    const void *kp = (const void *)k_knn_persistent<D, KCAP, BOX>;
+    const size_t pad = occupancy_pad(ctx, kp, ctx->tuning.waves_per_simd);
+    launch(ctx, "knn", k_knn_persistent<D, KCAP, BOX>, dim3(persistent_grid(ctx, kp, kQB, pad, div_up(nq, kQB))),
+           dim3(kQB), pad, vs.views, vs.qview, p.q, qs, (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k,
+           p.F0, p.tol, p.ids_fixed, p.cnt, p.flags, p.band, p.Fq, p.flags + 3);
+  } else {
+    const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
+    launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), pad, vs.views, vs.qview, p.q, qs,
+           (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, p.ids_fixed, p.cnt, p.flags,
+           p.band, p.Fq);
   }
-  launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(grid), dim3(kQB), pad, vs.views, vs.qview, p.q, qs,
-         (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, p.ids_fixed, p.cnt, p.flags,
-         p.band, p.Fq, tiles);
   exclusive_scan_u32(ctx, p.cnt, p.scan, nq, p.flags + 2, &ar);
   launch(ctx, "offsets_dev", k_offsets_dev, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)p.scan, nq,
          (const uint32_t *)(p.flags + 2), p.offsets);
@@ -3797,11 +3836,12 @@ static void knn_plan_capture(KnnPlan &p) {
   OTG_HIP_CHECK(hipHostMalloc(&p.h_flags, 16, hipHostMallocDefault));
   std::memset(p.h_flags, 0, 16);
   // launch-shape queries before the capture (cached)
-  (void)occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
-  if (ctx->tuning.persistent)
-    (void)persistent_grid(ctx, (const void *)k_knn<D, KCAP, BOX>, kQB,
-                          occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd),
-                          div_up(nq, kQB));
+  if (ctx->tuning.persistent) {
+    const void *kp = (const void *)k_knn_persistent<D, KCAP, BOX>;
+    (void)persistent_grid(ctx, kp, kQB, occupancy_pad(ctx, kp, ctx->tuning.waves_per_simd), div_up(nq, kQB));
+  } else {
+    (void)occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
+  }
   ctx->sync();
   const bool prof = ctx->profiling;
   ctx->profiling = false;
