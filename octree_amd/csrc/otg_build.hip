@@ -365,6 +365,88 @@ __global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys
   active[i] = a;
 }
 
+// The same test with the levels of a block's 2048 boundaries and a halo on either
+// side staged in LDS: the runs are walked there instead of through the cache.  Every
+// set's cap + 8 must fit the halo (the host checks).
+constexpr uint32_t kActTile = 2048;  // boundaries per block, 8 per thread
+constexpr uint32_t kActHalo = 1024;  // bytes of levels staged on either side
+
+__device__ __forceinline__ uint32_t lds_run_right(const uint64_t *sv, uint32_t lp, uint32_t vi, uint32_t limit) {
+  uint32_t c = 0;
+  while (c < limit) {
+    const uint32_t o = lp & 7u;
+    const uint64_t m = bytes_below(sv[lp >> 3], vi) & (~0ull << (8 * o));
+    if (m) return c + ((uint32_t)__builtin_ctzll(m) >> 3) - o;
+    c += 8 - o;
+    lp = (lp & ~7u) + 8;
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t lds_run_left(const uint64_t *sv, uint32_t lp, uint32_t vi, uint32_t limit) {
+  uint32_t c = 0;
+  while (c < limit) {
+    const uint32_t o = lp & 7u;
+    const uint64_t keep = (o == 7) ? ~0ull : ((1ull << (8 * (o + 1))) - 1ull);
+    const uint64_t m = bytes_below(sv[lp >> 3], vi) & keep;
+    if (m) return c + o - ((63u - (uint32_t)__builtin_clzll(m)) >> 3);
+    c += o + 1;
+    lp = (lp & ~7u) - 1;  // stays inside the window: the halo exceeds limit + 7
+  }
+  return c;
+}
+
+template <bool BOX>
+__global__ void __launch_bounds__(kB) k_active_lds(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ v,
+                                                    uint64_t n, KeyView kv, const SetParams *__restrict__ sets,
+                                                    uint8_t *__restrict__ active) {
+  constexpr uint32_t kWords = (kActTile + 2 * kActHalo) / 8;
+  __shared__ uint64_t sv[kWords];
+  const uint64_t i0 = (uint64_t)blockIdx.x * kActTile;
+  const int64_t w0 = (int64_t)i0 - (int64_t)kActHalo;  // byte position of sv[0]
+  // levels v[0..n] (bytes past n read as 0: every walk stops at v[n] = 0 anyway)
+  for (uint32_t w = threadIdx.x; w < kWords; w += kB) {
+    const int64_t b = w0 + 8 * (int64_t)w;
+    uint64_t x = 0;
+    if (b >= 0 && (uint64_t)b <= n) {
+      if ((uint64_t)b + 8 <= n + 1) {
+        x = *reinterpret_cast<const uint64_t *>(v + b);
+      } else {
+        for (uint64_t q = (uint64_t)b; q <= n; ++q) x |= (uint64_t)v[q] << (8 * (q - (uint64_t)b));
+      }
+    }
+    sv[w] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kActTile / kB; ++j) {
+    const uint64_t i = i0 + (uint64_t)j * kB + threadIdx.x;
+    if (i > n) break;
+    uint8_t a = 0;
+    const uint32_t lp = (uint32_t)((int64_t)i - w0);
+    const uint32_t vi = (i > 0 && i < n) ? (uint32_t)((sv[lp >> 3] >> (8 * (lp & 7u))) & 0xFFu) : 0u;
+    if (vi) {
+      const uint32_t s = kv.set(keys[i]);
+      const SetParams &sp = sets[s];
+      if (vi - 1 < sp.max_depth) {
+        const uint32_t lim = sp.cap;
+        uint32_t cnt = 2 + lds_run_left(sv, lp - 1, vi, lim);
+        if (cnt <= lim) cnt += lds_run_right(sv, lp + 1, vi, lim);
+        if (cnt > lim) {
+          a = 1;
+          if (BOX && keys[i - 1] == keys[i]) {
+            const uint64_t sb = sp.ent_off, se = sp.ent_off + sp.ent_cnt;
+            uint64_t lo = lower_bound_u64(keys, sb, i, keys[i]);
+            uint64_t hi = upper_bound_u64(keys, i, se, keys[i]);
+            if (v[lo] < vi && v[hi] < vi) a = 0;
+          }
+        }
+      }
+    }
+    active[i] = a;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 5/6. node events
 // ---------------------------------------------------------------------------
@@ -872,8 +954,14 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
 
   // 4. active boundaries
   Buf<uint8_t> active(ctx, n + 1);
-  launch(ctx, "active", k_active<BOX>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys,
-         (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
+  uint64_t max_cap = 0;
+  for (const SetParams &sp : t->sets) max_cap = std::max<uint64_t>(max_cap, sp.cap);
+  if (max_cap + 8 < kActHalo)
+    launch(ctx, "active", k_active_lds<BOX>, dim3(div_up(n + 1, kActTile)), dim3(kB), 0, (const uint64_t *)t->d_keys,
+           (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
+  else
+    launch(ctx, "active", k_active<BOX>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys,
+           (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
 
   // 5. level scans
   Buf<uint32_t> lm(ctx, n + 1), rm(ctx, n + 1);

@@ -71,6 +71,26 @@ struct StackCap {
   static constexpr int value = 1 + kDepth * ((1 << D) - 1);
 };
 
+// Tree arrays live in device global memory: reading them through a global-space
+// pointer gives global (or, at wave-uniform addresses, scalar) loads instead of flat
+// ones, which would also hold up every LDS access behind them.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gp(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+
+// A record read through a global-space pointer, 32 bits at a time.
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+  static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "records of 32-bit words");
+  uint32_t w[sizeof(T) / 4];
+#pragma unroll
+  for (size_t i = 0; i < sizeof(T) / 4; ++i) w[i] = gp(reinterpret_cast<const uint32_t *>(p))[i];
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 
@@ -95,8 +115,8 @@ __device__ __forceinline__ double wall_distance2(const double (&q)[D], const dou
   bool in = true;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const double hs = sz[d] * 0.5;
-    const double cd = fabs(mn[d] + hs - q[d]);
+    const double hs = gp(sz)[d] * 0.5;
+    const double cd = fabs(gp(mn)[d] + hs - q[d]);
     in &= cd <= hs;
     const double v = smax(0.0, cd - hs);
     s2 += v * v;
@@ -265,8 +285,8 @@ __device__ __forceinline__ void box_minmax2(const double (&q)[D], const TreeView
   bool inside = true;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const double minDist = q[d] - tv.geom[(uint64_t)d * tv.n + j];
-    const double maxDist = tv.geom[(uint64_t)(D + d) * tv.n + j] - q[d];
+    const double minDist = q[d] - gp(tv.geom)[(uint64_t)d * tv.n + j];
+    const double maxDist = gp(tv.geom)[(uint64_t)(D + d) * tv.n + j] - q[d];
     const bool in_c = (-tol <= minDist && -tol <= maxDist);
     inside &= in_c;
     double a = minDist * minDist, b = maxDist * maxDist;
@@ -296,7 +316,7 @@ __device__ __forceinline__ void knn_scan_box(const TreeView &tv, const double (&
     if (min2 >= L.T) continue;
     const double dmin = sqrt(min2);
     if (dmin >= L.F) continue;
-    knn_offer_box(bd, bid, pes, npes, L, dmin, sqrt(mm2), tv.ids[j], k, tol);
+    knn_offer_box(bd, bid, pes, npes, L, dmin, sqrt(mm2), gp(tv.ids)[j], k, tol);
   }
 }
 
@@ -358,7 +378,7 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
       const uint32_t node = S.node(top);
       // wall2 is 0 inside the node box, and (0 >= F) is the reference's test there too
       if (knn_node_out(L, S.wall(top), false)) continue;
-      const NodeInfo ni = tv.info[node];
+      const NodeInfo ni = gload(tv.info + node);
       if constexpr (BOX)
         knn_scan_box<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, pes, npes, L, k, tol);
       else
@@ -419,7 +439,7 @@ __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D
       for (int d = 0; d < D; ++d) {
         double c[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) c[u] = tv.geom[(uint64_t)d * tv.n + e + u];
+        for (int u = 0; u < U; ++u) c[u] = gp(tv.geom)[(uint64_t)d * tv.n + e + u];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const double v = q[d] - c[u];
@@ -434,7 +454,7 @@ __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D
       double d2 = 0.0;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const double v = q[d] - tv.geom[(uint64_t)d * tv.n + e];
+        const double v = q[d] - gp(tv.geom)[(uint64_t)d * tv.n + e];
         d2 += v * v;
       }
       if (d2 < T0) mark |= 1ull << (e - base);
@@ -446,8 +466,8 @@ __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D
     mark &= mark - 1;
     double c[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) c[d] = tv.geom[(uint64_t)d * tv.n + x];
-    uint32_t id = tv.ids[x];
+    for (int d = 0; d < D; ++d) c[d] = gp(tv.geom)[(uint64_t)d * tv.n + x];
+    uint32_t id = gp(tv.ids)[x];
     for (;;) {
       double d2 = 0.0;
 #pragma unroll
@@ -461,8 +481,8 @@ __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D
         x = base + (uint32_t)__builtin_ctzll(mark);
         mark &= mark - 1;
 #pragma unroll
-        for (int d = 0; d < D; ++d) c[d] = tv.geom[(uint64_t)d * tv.n + x];
-        id = tv.ids[x];
+        for (int d = 0; d < D; ++d) c[d] = gp(tv.geom)[(uint64_t)d * tv.n + x];
+        id = gp(tv.ids)[x];
       }
       if (d2 < L.T) {
         const double dist = sqrt(d2);
@@ -522,7 +542,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
     int top = 0;
     uint32_t node = (uint32_t)sp.node_off;
     // the node record is loaded together with the box that decides the visit
-    NodeInfo ni = tv.info[node];
+    NodeInfo ni = gload(tv.info + node);
     bool have = true;  // `node` is to be visited
     {
       bool in;
@@ -587,7 +607,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
       const uint32_t packed = S.pk[(top - 1) * kQB];
       const uint32_t cnt = packed >> 28;
       node = S.cb[(top - 1) * kQB] + (packed & (NC - 1));
-      ni = tv.info[node];
+      ni = gload(tv.info + node);
       if (cnt == 1) --top;
       else S.pk[(top - 1) * kQB] = ((cnt - 1) << 28) | ((packed & 0x0FFFFFFFu) >> D);
       bool in;
@@ -651,7 +671,7 @@ __device__ __forceinline__ bool knn_query_masks(const TreeView &tv, uint64_t qi,
     }
     for (;;) {
       if (have) {
-        const NodeInfo ni = tv.info[node];
+        const NodeInfo ni = gload(tv.info + node);
         if constexpr (BOX)
           knn_scan_box<D, KCAP>(tv, q, ni.ebegin, ni.ebegin + ni.ecount, bd, bid, pes, npes, L, k, tol);
         else
@@ -823,7 +843,7 @@ __global__ void __launch_bounds__(kQB) k_knn_band(const TreeView *__restrict__ v
         bool in;
         const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
         if (w2 >= T || sqrt(w2) >= F) continue;
-        const NodeInfo ni = tv.info[node];
+        const NodeInfo ni = gload(tv.info + node);
         for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
           double x2, mm2 = 0.0;
           if (BOX) {
@@ -888,7 +908,7 @@ __global__ void __launch_bounds__(kQB) k_knn_radius(const TreeView *__restrict__
         bool in;
         const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
         if (w2 >= T || sqrt(w2) >= F) continue;
-        const NodeInfo ni = tv.info[node];
+        const NodeInfo ni = gload(tv.info + node);
         for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
           double x2, mm2 = 0.0;
           if (BOX) {
@@ -1029,7 +1049,7 @@ __global__ void __launch_bounds__(kQB) k_knn_footprint(const TreeView *__restric
         bool in;
         const double w2 = wall_distance2<D>(q, &tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &in);
         if (sqrt(w2) >= F) continue;
-        const NodeInfo ni = tv.info[node];
+        const NodeInfo ni = gload(tv.info + node);
         ++nodes;
         ents += ni.ecount;
         for (uint32_t c = 0; c < ni.ccount; ++c) stack[top++] = ni.cbegin + c;
@@ -1224,7 +1244,7 @@ __device__ __forceinline__ void range_query(const TreeView &tv, uint64_t qi, con
           capped = true;
           break;
         }
-        const NodeInfo ni = tv.info[node];  // loaded with the box that decides the visit
+        const NodeInfo ni = gload(tv.info + node);  // loaded with the box that decides the visit
         bool overlap, contained;
         range_node_test<D>(tv, node, rmin, rmax, rsz, tol, overlap, contained);
         if (overlap && contained) {
@@ -1815,7 +1835,7 @@ __device__ __forceinline__ void pick_query(const TreeView &tv, uint64_t qi, cons
       uint32_t node = root;
       for (;;) {
         if (node_contains_point<D>(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], p, tol)) {
-          const NodeInfo ni = tv.info[node];
+          const NodeInfo ni = gload(tv.info + node);
           for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
             bool ok;
             if (BOX) {
@@ -2117,7 +2137,7 @@ __device__ __forceinline__ bool ray_query(const TreeView &tv, uint64_t qi, const
       double prio;
       heap_pop(H, n, &node, &prio);
       if (prio > M) break;
-      const NodeInfo ni = tv.info[node];
+      const NodeInfo ni = gload(tv.info + node);
       for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
         bool hit;
         if (BOX) {
@@ -2274,7 +2294,7 @@ __global__ void __launch_bounds__(kQB) k_ray_replay(const TreeView *__restrict__
         double prio;
         heap_pop(H, n, &node, &prio);
         if (prio > M) break;
-        const NodeInfo ni = tv.info[node];
+        const NodeInfo ni = gload(tv.info + node);
         for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
           bool hit;
           if (BOX) {
@@ -2414,7 +2434,7 @@ __device__ __forceinline__ void ray_all_query(const TreeView &tv, uint64_t qi, c
     double en, ex;
     for (;;) {
       if (rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex)) {
-        const NodeInfo ni = tv.info[node];
+        const NodeInfo ni = gload(tv.info + node);
         for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
           bool hit;
           if (BOX) {
