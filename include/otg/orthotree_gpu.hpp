@@ -17,8 +17,8 @@
 //                                                         (core/ot_managed.h:47-893, core/ot_aliases.h:417-447)
 //   GetNodeCount / GetNodeChildren / GetNodeEntities / GetNodeMinPoint / GetNodeSize / GetNodeBox
 //                                                         (core/ot_static_linear_core.h:178-268)
-//   Serialize(): the BinaryOutputArchive bytes of the core / managed tree
-//                                                         (serialization.h:373-386, :433-439)
+//   Serialize() / Deserialize(): the BinaryOutputArchive bytes of the core / managed
+//                                   tree, and loading them back (serialization.h:373-386, :433-439)
 // plus batch overloads over spans of queries returning CSR results.
 //
 // The tree copies the entities into HBM at Create and keeps them resident; later
@@ -32,6 +32,7 @@
 #include <array>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <exception>
 #include <limits>
 #include <optional>
@@ -404,7 +405,37 @@ class StaticOrthoTreeGPU {
     return out;
   }
 
+  // serialize(BinaryInputArchive&, StaticLinearOrthoTreeCore&) (serialization.h:373-386):
+  // loads an archive written for these entities; the node table and entity order
+  // come from the archive.  false for an archive that is malformed or was not
+  // written for these entities.
+  bool Deserialize(std::span<const uint8_t> archive, EntityContainerView entities, bool largeDataset = false) noexcept {
+    return load(archive, reinterpret_cast<const double *>(entities.data()), entities.size(), largeDataset);
+  }
+
+  // The archive of an OrthoTreeManaged (serialization.h:433-439) carries the entities
+  // after the core: OrthoTreeManaged::Deserialize reads them back.
+  bool DeserializeWithEntities(std::span<const uint8_t> archive, bool largeDataset = false) noexcept {
+    return load(archive, nullptr, 0, largeDataset);
+  }
+
  private:
+  bool load(std::span<const uint8_t> archive, const double *geom, std::size_t n, bool largeDataset) noexcept {
+    otg_tree_free(tree_);
+    tree_ = nullptr;
+    nodes_ = {};
+    entity_count_ = 0;
+    otg_status st = otg_tree_deserialize(ctx().get(), IS_BOX ? OTG_KIND_BOX : OTG_KIND_POINT, DIM, archive.data(),
+                                         archive.size(), largeDataset ? OTG_ARCHIVE_LARGE_DATASET : 0u, n, geom,
+                                         OTG_MEM_HOST, &tree_);
+    if (st == OTG_ERR_INVALID_ARG) return false;
+    detail::check(st, "otg_tree_deserialize", ctx().get());
+    otg_set_info si{};
+    detail::check(otg_tree_get_set_info(tree_, 0, &si), "otg_tree_get_set_info", ctx().get());
+    entity_count_ = si.entity_count;
+    return true;
+  }
+
   struct HostNodes {
     bool loaded = false;
     depth_t max_depth = 0;
@@ -585,6 +616,39 @@ class OrthoTreeManaged {
       }
     }
     return out;
+  }
+  // serialize(BinaryInputArchive&, OrthoTreeManaged&) (serialization.h:433-439): the
+  // core, then the entity vector it carries.  false for a malformed archive.
+  bool Deserialize(std::span<const uint8_t> archive, bool largeDataset = false) noexcept {
+    if (!m_tree.DeserializeWithEntities(archive, largeDataset)) return false;
+    // the entity vector follows the core: its offset is the core's own length
+    std::size_t pos = m_tree.Serialize(largeDataset).size();
+    auto take = [&](void *dst, std::size_t bytes) {
+      if (archive.size() - pos < bytes) detail::fail(OTG_ERR_INVALID_ARG, "OrthoTreeManaged::Deserialize", nullptr);
+      std::memcpy(dst, archive.data() + pos, bytes);
+      pos += bytes;
+    };
+    auto count = [&]() -> uint64_t {
+      if (largeDataset) {
+        uint64_t c;
+        take(&c, 8);
+        return c;
+      }
+      uint32_t c;
+      take(&c, 4);
+      return c;
+    };
+    constexpr int D = TOrthoTreeCore::DIMENSION_NO;
+    constexpr int ARRAYS = std::is_same_v<Entity, TBox> ? 2 : 1;
+    m_entities.assign(count(), Entity{});
+    for (Entity &e : m_entities) {
+      double *v = reinterpret_cast<double *>(&e);
+      for (int a = 0; a < ARRAYS; ++a) {
+        (void)count();
+        take(v + a * D, sizeof(double) * D);
+      }
+    }
+    return true;
   }
   EntityContainer const &GetData() const noexcept { return m_entities; }
   Entity const &Get(EntityID entityID) const noexcept { return m_entities[entityID]; }

@@ -153,6 +153,21 @@ OTG_API otg_status otg_tree_export(const otg_tree *tree, uint32_t set, uint8_t *
 OTG_API otg_status otg_tree_serialize(const otg_tree *tree, uint32_t set, uint32_t flags, uint8_t *buf,
                                       uint64_t buf_size, uint64_t *size);
 
+/* Loads a static-core archive into a tree on ctx: serialize(BinaryInputArchive&,
+ * StaticLinearOrthoTreeCore&) (serialization.h:359-386) of the bytes
+ * otg_tree_serialize writes (flags as there).  geom = the tree's n entities in id
+ * order (memory space mem), or NULL for the archive of an OrthoTreeManaged, whose
+ * entity vector follows the core (serialization.h:433-439).  The node table, node
+ * geometry and entity order are taken from the archive, not rebuilt; it is checked
+ * to be a static linear tree (children after their parent and deeper, each node's
+ * entities followed by its children's, ids a permutation) whose entities lie in
+ * their nodes, else OTG_ERR_INVALID_ARG.  Queries then answer as on the archived
+ * tree; point trees with overfull max-depth leaves get the finer kNN / range index
+ * a build makes.  A loaded point tree has no location keys (the archive holds
+ * none): otg_tree_export of sorted_keys returns OTG_ERR_UNSUPPORTED for it. */
+OTG_API otg_status otg_tree_deserialize(otg_ctx *ctx, int kind, int dim, const uint8_t *buf, uint64_t size,
+                                        uint32_t flags, uint64_t n, const double *geom, int mem, otg_tree **out);
+
 /* ---- autotuning ---------------------------------------------------------- */
 /* Settings of the batched path.  leaf_capacity is the build's maxElementNumInNode
  * (ot_static_linear_core.h:523-529), which the caller passes to otg_forest_build;

@@ -363,7 +363,9 @@ class StaticTree:
     def GetMaxDepthID(self, s: int = 0) -> int:
         return int(self.set_info(s).max_depth)
 
-    def export(self, s: int = 0) -> dict:
+    def export(self, s: int = 0, sorted_keys: bool = True) -> dict:
+        """Node table of set s (otg_tree_export); sorted_keys=False leaves out the
+        entities' location keys (a point tree loaded from an archive has none)."""
         si = self.set_info(s)
         nn, ne, D = int(si.node_count), int(si.entity_count), self.dim
         out = {
@@ -372,7 +374,9 @@ class StaticTree:
             "min": np.empty((nn, D), np.float64), "size": np.empty((nn, D), np.float64),
             "ids": np.empty(ne, np.uint32), "sorted_keys": np.empty(ne, np.uint64),
         }
-        st = _abi.lib().otg_tree_export(self.handle, s, *[_ptr(out[k]) for k in
+        if not sorted_keys:
+            del out["sorted_keys"]
+        st = _abi.lib().otg_tree_export(self.handle, s, *[_ptr(out.get(k)) for k in
                                                           ("depth", "ebegin", "ecount", "cbegin", "ccount", "key",
                                                            "min", "size", "ids", "sorted_keys")])
         _check(st, "otg_tree_export", self.ctx)
@@ -399,6 +403,38 @@ class StaticTree:
             from .archive import entity_vector_bytes
             out += entity_vector_bytes(_f64(entities, self._cols()), self.dim, self.kind == "box", large_dataset)
         return out
+
+    @classmethod
+    def deserialize(cls, archive: bytes, entities=None, large_dataset: bool = False, *, ctx=None, dim=None,
+                    kind=None):
+        """Tree loaded from a static-core archive (otg_tree_deserialize, the reading side
+        of serialization.h:359-386): `entities` in id order (host array or device
+        tensor), or None for an OrthoTreeManaged archive that carries them."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        self.kind = kind or cls.KIND
+        self.dim = dim or cls.DIM
+        self.handle = None
+        buf = np.frombuffer(bytes(archive), dtype=np.uint8)
+        keep = None
+        if entities is None:
+            gp, n, mem = None, 0, _abi.OTG_MEM_HOST
+        elif _is_device(entities):
+            gp, n = _dev(entities, self._cols())
+            mem, keep = _abi.OTG_MEM_DEVICE, entities
+        else:
+            keep = _f64(entities, self._cols())
+            gp, n, mem = _ptr(keep), keep.shape[0], _abi.OTG_MEM_HOST
+        h = C.c_void_p()
+        flags = _abi.OTG_ARCHIVE_LARGE_DATASET if large_dataset else 0
+        st = _abi.lib().otg_tree_deserialize(self.ctx.handle, _abi.OTG_KIND_POINT if self.kind == "point" else
+                                             _abi.OTG_KIND_BOX, self.dim, _ptr(buf), buf.size, flags, n, gp, mem,
+                                             C.byref(h))
+        _check(st, "otg_tree_deserialize", self.ctx)
+        self.handle = h
+        self.ctx._children.add(self)
+        del keep
+        return self
 
     # -- batched queries (CSR) -------------------------------------------------
     def _sets_arg(self, set_ids, nq):

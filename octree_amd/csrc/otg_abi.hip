@@ -443,6 +443,31 @@ OTG_API otg_status otg_forest_build(otg_ctx *ctx, int kind, int dim, uint32_t ns
   return OTG_OK;
 }
 
+OTG_API otg_status otg_tree_deserialize(otg_ctx *ctx, int kind, int dim, const uint8_t *buf, uint64_t size,
+                                        uint32_t flags, uint64_t n, const double *geom, int mem, otg_tree **out) {
+  if (!ctx || !out || !buf || (flags & ~OTG_ARCHIVE_LARGE_DATASET)) return OTG_ERR_INVALID_ARG;
+  if (kind != OTG_KIND_POINT && kind != OTG_KIND_BOX) return OTG_ERR_INVALID_ARG;
+  if (dim < 2 || dim > otg::kMaxDim) return OTG_ERR_INVALID_ARG;
+  if (geom && mem != OTG_MEM_HOST && mem != OTG_MEM_DEVICE) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  otg_tree *t = new (std::nothrow) otg_tree();
+  if (!t) return OTG_ERR_OOM;
+  t->ctx = ctx;
+  t->kind = kind;
+  t->dim = dim;
+  otg_status st = guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::load_static_archive(t, buf, size, (flags & OTG_ARCHIVE_LARGE_DATASET) != 0, n, geom, mem);
+  });
+  if (st != OTG_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    delete t;
+    return st;
+  }
+  *out = t;
+  return OTG_OK;
+}
+
 OTG_API otg_status otg_tree_build(otg_ctx *ctx, int kind, int dim, uint64_t n, const double *geom, int mem,
                                   const double *box_or_null, uint32_t max_depth, uint32_t max_elements, otg_tree **out) {
   uint64_t offs[2] = {0, n};
@@ -507,6 +532,8 @@ OTG_API otg_status otg_tree_export(const otg_tree *t, uint32_t set, uint8_t *nod
       OTG_HIP_CHECK(hipMemcpyAsync(entity_ids, ids + eo, ne * 4, hipMemcpyDeviceToHost, st));
     }
     std::vector<uint64_t> keys;
+    if (sorted_keys && ne && !t->d_keys)
+      throw Error{OTG_ERR_UNSUPPORTED, "a point tree loaded from an archive has no location keys (the archive holds none)"};
     if (sorted_keys && ne) {
       keys.resize(ne);
       OTG_HIP_CHECK(hipMemcpyAsync(keys.data(), t->d_keys + eo, ne * 8, hipMemcpyDeviceToHost, st));

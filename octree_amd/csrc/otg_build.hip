@@ -27,6 +27,8 @@
 //                node keys), MinPoint geometry (:346-368) / MBR (:280-327).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "otg_device.hpp"
 #include "otg_internal.hpp"
@@ -1303,6 +1305,377 @@ const TreeView *reference_device_view(const Tree *t) {
   if (!t->fine_order || t->n == 0) return t->d_view;
   reference_view_of(t);
   return t->d_view_ref;
+}
+
+// ---------------------------------------------------------------------------
+// loading a static-core archive: serialize(BinaryInputArchive&,
+// StaticLinearOrthoTreeCore&) (serialization.h:359-386, binary_archive.h) reads the
+// members otg_tree_serialize writes, in the same order
+// ---------------------------------------------------------------------------
+struct ArchiveIn {
+  const uint8_t *p;
+  uint64_t n, pos;
+  bool large;
+  template <typename T>
+  T get() {
+    if (n - pos < sizeof(T)) throw Error{OTG_ERR_INVALID_ARG, "the archive is truncated"};
+    T v;
+    std::memcpy(&v, p + pos, sizeof(T));
+    pos += sizeof(T);
+    return v;
+  }
+  uint64_t count() { return large ? get<uint64_t>() : (uint64_t)get<uint32_t>(); }
+  void vec(double *out, int D) {  // std::array<double, D>: its size, then the values
+    if (count() != (uint64_t)D) throw Error{OTG_ERR_INVALID_ARG, "archive array length is not the dimension"};
+    for (int d = 0; d < D; ++d) out[d] = get<double>();
+  }
+  // count of a vector whose elements take at least `min_bytes` each
+  uint64_t count_of(uint64_t min_bytes) {
+    const uint64_t c = count();
+    if (min_bytes && c > (n - pos) / min_bytes) throw Error{OTG_ERR_INVALID_ARG, "archive count exceeds its bytes"};
+    return c;
+  }
+};
+
+struct LoadedTable {
+  uint64_t cap = 0;
+  uint32_t M = 0;
+  double box_min[kMaxDim] = {}, box_size[kMaxDim] = {};
+  std::vector<double> depth_size;  // points: m_nodeSize, (M + 2) x D
+  std::vector<uint64_t> begins, eb, ec;
+  std::vector<uint8_t> depth;
+  std::vector<uint32_t> ids;
+  std::vector<double> nmin, nsize;
+  std::vector<double> entities;  // OrthoTreeManaged archives: the entity vector after the core
+};
+
+template <typename Index, typename Seg>
+static void read_node_storage(ArchiveIn &ar, LoadedTable &L) {
+  const uint64_t nb = ar.count_of(sizeof(Index));
+  L.begins.resize(nb);
+  for (uint64_t i = 0; i < nb; ++i) L.begins[i] = ar.get<Index>();
+  const uint64_t nn = ar.count_of(2 * sizeof(Seg));
+  L.eb.resize(nn);
+  L.ec.resize(nn);
+  for (uint64_t i = 0; i < nn; ++i) {
+    L.eb[i] = ar.get<Seg>();
+    L.ec[i] = ar.get<Seg>();
+  }
+}
+
+static LoadedTable read_static_archive(ArchiveIn &ar, int D, bool box, bool want_entities) {
+  LoadedTable L;
+  if (ar.get<uint32_t>() != 0 || ar.get<uint32_t>() != 0)
+    throw Error{OTG_ERR_INVALID_ARG, "archive version is not StaticLinearOrthoTreeCore 0 / OrthoTreeCoreBase 0"};
+  L.cap = ar.get<uint64_t>();
+  L.M = ar.get<uint32_t>();
+  if (L.cap == 0 || L.M < 1 || L.M > (uint32_t)kMaxDepth || L.M > max_theoretical_depth(D))
+    throw Error{OTG_ERR_INVALID_ARG, "archive capacity or depth out of range"};
+  if (box) {
+    ar.vec(L.box_min, D);
+    ar.vec(L.box_size, D);
+  } else {
+    if (ar.count() != (uint64_t)L.M + 2) throw Error{OTG_ERR_INVALID_ARG, "archive node sizes are not depth + 2"};
+    L.depth_size.resize((size_t)(L.M + 2) * D);
+    for (uint32_t k = 0; k < L.M + 2; ++k) ar.vec(&L.depth_size[(size_t)k * D], D);
+  }
+  const uint64_t kind = ar.count();
+  if (kind == 0) read_node_storage<uint8_t, uint8_t>(ar, L);
+  else if (kind == 1) read_node_storage<uint16_t, uint16_t>(ar, L);
+  else if (kind == 2 && ar.large) read_node_storage<uint32_t, uint64_t>(ar, L);
+  else if (kind == 2) read_node_storage<uint32_t, uint32_t>(ar, L);
+  else throw Error{OTG_ERR_INVALID_ARG, "archive node storage index is not 0, 1 or 2"};
+  const uint64_t nn = L.eb.size();
+  if (ar.count_of(1) != nn) throw Error{OTG_ERR_INVALID_ARG, "archive depth count differs from the node count"};
+  L.depth.resize(nn);
+  for (uint64_t i = 0; i < nn; ++i) L.depth[i] = ar.get<uint8_t>();
+  const uint64_t ne = ar.count_of(4);
+  L.ids.resize(ne);
+  for (uint64_t i = 0; i < ne; ++i) L.ids[i] = ar.get<uint32_t>();
+  if (ar.count_of(8) != nn) throw Error{OTG_ERR_INVALID_ARG, "archive geometry count differs from the node count"};
+  L.nmin.resize(nn * D);
+  if (box) L.nsize.resize(nn * D);
+  for (uint64_t i = 0; i < nn; ++i) {
+    ar.vec(&L.nmin[i * D], D);
+    if (box) ar.vec(&L.nsize[i * D], D);
+  }
+  if (want_entities) {
+    const int S = box ? 2 * D : D;
+    if (ar.count_of(8) != ne) throw Error{OTG_ERR_INVALID_ARG, "archive entity vector differs from the entity count"};
+    L.entities.resize(ne * S);
+    for (uint64_t i = 0; i < ne; ++i) {
+      ar.vec(&L.entities[i * S], D);
+      if (box) ar.vec(&L.entities[i * S + D], D);
+    }
+  }
+  if (ar.pos != ar.n) throw Error{OTG_ERR_INVALID_ARG, "archive has bytes after the tree"};
+  return L;
+}
+
+// Node keys of a loaded box tree: the code of the node's first subtree entity at
+// the node's depth (the node's cell), with the depth's leading bit (si_morton.h:216-226);
+// also the sorted keys of the storage order.
+__global__ void __launch_bounds__(kB) k_loaded_keys(const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ ids,
+                                                     uint64_t n, uint64_t *__restrict__ keys_sorted) {
+  const uint64_t j = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (j < n) keys_sorted[j] = keys_in[ids[j]];
+}
+
+template <int D>
+__global__ void __launch_bounds__(kB) k_loaded_node_keys(const NodeInfo *__restrict__ info, const uint8_t *__restrict__ depth,
+                                                          uint64_t nn, const uint64_t *__restrict__ keys_sorted,
+                                                          uint64_t ne, KeyView kv, uint32_t M,
+                                                          uint64_t *__restrict__ nodekey) {
+  const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nn) return;
+  const uint32_t d = depth[x];
+  const uint64_t code = info[x].ebegin < ne ? kv.code(keys_sorted[info[x].ebegin]) : 0ull;
+  const uint32_t sh = (M - d) * D;
+  nodekey[x] = (1ull << (d * D)) | ((sh >= 64) ? 0ull : (code >> sh));
+}
+
+// Entities of each node's own segment lie in the node's box (MinPoint box for
+// points, MBR for boxes), to a relative 1e-9: the archive was written for these entities.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_loaded_check(const NodeInfo *__restrict__ info, uint64_t nn,
+                                                      const double *__restrict__ geom, uint64_t n,
+                                                      const double *__restrict__ nmin, const double *__restrict__ nsize,
+                                                      uint32_t *__restrict__ bad) {
+  const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nn) return;
+  const NodeInfo ni = info[x];
+  bool ok = true;
+  for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount && ok; ++j)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double lo = nmin[x * D + d], hi = lo + nsize[x * D + d];
+      const double eps = 1e-9 * (fabs(lo) + fabs(hi) + 1.0);
+      const double a = geom[(uint64_t)d * n + j], b = BOX ? geom[(uint64_t)(D + d) * n + j] : a;
+      ok &= (a >= lo - eps) && (b <= hi + eps);
+    }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+template <int D, bool BOX>
+static void load_impl(Tree *t, const uint8_t *buf, uint64_t size, bool large, uint64_t n, const double *geom_in,
+                      int mem) {
+  Ctx *ctx = t->ctx;
+  constexpr int S = BOX ? 2 * D : D;
+  ArchiveIn ar{buf, size, 0, large};
+  LoadedTable L = read_static_archive(ar, D, BOX, geom_in == nullptr);
+  const uint64_t nn = L.eb.size(), ne = L.ids.size();
+  if (geom_in && n != ne) throw Error{OTG_ERR_INVALID_ARG, "the entity count differs from the archive's"};
+  if (ne >= (1ull << 30)) throw Error{OTG_ERR_UNSUPPORTED, "more than 2^30 entities"};
+  if ((nn == 0) != (ne == 0)) throw Error{OTG_ERR_INVALID_ARG, "archive has entities but no nodes, or nodes but no entities"};
+
+  // structure: ids a permutation; children after their parent, deeper, within the
+  // table; the static linear layout (a node's own entities, then each child's
+  // subtree, contiguous), which also bounds every walk the queries make
+  std::vector<uint8_t> seen(ne, 0);
+  for (uint32_t id : L.ids) {
+    if (id >= ne || seen[id]) throw Error{OTG_ERR_INVALID_ARG, "archive entity ids are not a permutation"};
+    seen[id] = 1;
+  }
+  std::vector<NodeInfo> info(nn);
+  std::vector<uint32_t> rend(nn);
+  if (nn) {
+    if (L.begins.empty() || L.begins[0] != 1 || L.begins.size() > nn + 1)
+      throw Error{OTG_ERR_INVALID_ARG, "archive child segments are malformed"};
+    for (uint64_t i = 0; i < nn; ++i) {
+      NodeInfo &ni = info[i];
+      if (L.depth[i] > L.M || L.eb[i] > ne || L.ec[i] > ne - L.eb[i])
+        throw Error{OTG_ERR_INVALID_ARG, "archive node depth or entity segment out of range"};
+      ni.ebegin = (uint32_t)L.eb[i];
+      ni.ecount = (uint32_t)L.ec[i];
+      ni.cbegin = 0;
+      ni.ccount = 0;
+      if (i + 1 < L.begins.size()) {
+        if (L.begins[i + 1] < L.begins[i] || L.begins[i + 1] > nn)
+          throw Error{OTG_ERR_INVALID_ARG, "archive child segments are malformed"};
+        if (L.begins[i + 1] > L.begins[i]) {
+          if (L.begins[i] <= i) throw Error{OTG_ERR_INVALID_ARG, "archive children precede their parent"};
+          ni.cbegin = (uint32_t)L.begins[i];
+          ni.ccount = (uint32_t)(L.begins[i + 1] - L.begins[i]);
+        }
+      }
+    }
+    if (L.depth[0] != 0) throw Error{OTG_ERR_INVALID_ARG, "archive root is not at depth 0"};
+    std::vector<uint8_t> has_parent(nn, 0);
+    for (uint64_t i = nn; i-- > 0;) {
+      const NodeInfo &ni = info[i];
+      uint64_t pos = (uint64_t)ni.ebegin + ni.ecount;
+      for (uint32_t c = ni.cbegin; c < ni.cbegin + ni.ccount; ++c) {
+        if (L.depth[c] <= L.depth[i] || has_parent[c] || info[c].ebegin != pos)
+          throw Error{OTG_ERR_INVALID_ARG, "archive tree is not a static linear layout"};
+        has_parent[c] = 1;
+        pos = rend[c];
+      }
+      rend[i] = (uint32_t)pos;
+    }
+    for (uint64_t i = 1; i < nn; ++i)
+      if (!has_parent[i]) throw Error{OTG_ERR_INVALID_ARG, "archive node without a parent"};
+    if (info[0].ebegin != 0 || rend[0] != ne) throw Error{OTG_ERR_INVALID_ARG, "archive root does not hold every entity"};
+  }
+
+  // set parameters: the tree box of a box tree is stored; a point tree stores its
+  // node sizes, and its root MinPoint stands in for the box minimum (used only to
+  // order query batches)
+  t->n = ne;
+  t->node_count = nn;
+  t->sets.assign(1, SetParams{});
+  SetParams &sp = t->sets[0];
+  sp.ent_off = 0;
+  sp.ent_cnt = ne;
+  sp.node_off = 0;
+  sp.node_cnt = nn;
+  sp.max_depth = L.M;
+  sp.cap = (uint32_t)std::min<uint64_t>(L.cap, 0xFFFFFFFFull);
+  sp.min_lcp = 255;
+  sp.root_splits = (nn && info[0].ccount) ? 1u : 0u;
+  {
+    volatile double res = (double)(1ull << L.M);
+    for (int d = 0; d < D; ++d) {
+      const double mn = BOX ? L.box_min[d] : (nn ? L.nmin[d] : 0.0);
+      volatile double sz = BOX ? L.box_size[d] : L.depth_size[d];
+      sp.space_min[d] = mn;
+      sp.space_max[d] = mn + sz;
+      sp.size[d] = sz;
+      sp.deraster[d] = sz / res;
+      sp.raster[d] = (sz == 0) ? 1.0 : (res / sz);
+    }
+    if (BOX) {
+      double loose = 2.0;
+      for (uint32_t k = 0; k < L.M + 2; ++k, loose *= 0.5)
+        for (int d = 0; d < D; ++d) sp.depth_size[k * D + d] = sp.size[d] * loose;
+    } else {
+      std::copy(L.depth_size.begin(), L.depth_size.end(), sp.depth_size);
+    }
+  }
+  set_layout<D, BOX>(t);
+  upload_sets(t);
+  if (ne == 0) return;
+
+  // node arrays
+  std::vector<uint64_t> nkey(nn, 0);
+  std::vector<double> nsize(nn * D);
+  for (uint64_t i = 0; i < nn; ++i)
+    for (int d = 0; d < D; ++d) nsize[i * D + d] = BOX ? L.nsize[i * D + d] : L.depth_size[(size_t)L.depth[i] * D + d];
+  if (!BOX)  // the node's cell at its depth, from its MinPoint relative to the root's
+    for (uint64_t i = 0; i < nn; ++i) {
+      const uint32_t dep = L.depth[i];
+      uint32_t g[D];
+      for (int d = 0; d < D; ++d) {
+        const double cell = L.depth_size[(size_t)dep * D + d];
+        const double x = cell > 0 ? std::nearbyint((L.nmin[i * D + d] - L.nmin[d]) / cell) : 0.0;
+        g[d] = (uint32_t)std::min<double>(std::max(x, 0.0), (double)((1ull << dep) - 1));
+      }
+      uint64_t code = 0;
+      for (uint32_t b = 0; b < dep; ++b)
+        for (int d = 0; d < D; ++d) code |= (uint64_t)((g[d] >> b) & 1u) << (b * D + d);
+      nkey[i] = (1ull << (dep * D)) | code;
+    }
+  t->d_info = (NodeInfo *)ctx->alloc(nn * sizeof(NodeInfo));
+  t->d_rend = (uint32_t *)ctx->alloc(nn * 4);
+  t->d_depth = (uint8_t *)ctx->alloc(nn);
+  t->d_nkey = (uint64_t *)ctx->alloc(nn * 8);
+  t->d_nmin = (double *)ctx->alloc(nn * D * 8);
+  t->d_nsize = (double *)ctx->alloc(nn * D * 8);
+  t->d_ids = (uint32_t *)ctx->alloc(ne * 4);
+  t->d_geom = (double *)ctx->alloc(ne * S * 8);
+  hipStream_t st = ctx->stream;
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_info, info.data(), nn * sizeof(NodeInfo), hipMemcpyHostToDevice, st));
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_rend, rend.data(), nn * 4, hipMemcpyHostToDevice, st));
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_depth, L.depth.data(), nn, hipMemcpyHostToDevice, st));
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_nkey, nkey.data(), nn * 8, hipMemcpyHostToDevice, st));
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_nmin, L.nmin.data(), nn * D * 8, hipMemcpyHostToDevice, st));
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_nsize, nsize.data(), nn * D * 8, hipMemcpyHostToDevice, st));
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_ids, L.ids.data(), ne * 4, hipMemcpyHostToDevice, st));
+
+  // entities (input order) on the device, then in storage order
+  Buf<double> dgeom_own;
+  const double *dgeom = geom_in;
+  if (!geom_in || mem == OTG_MEM_HOST) {
+    dgeom_own = Buf<double>(ctx, ne * S);
+    OTG_HIP_CHECK(hipMemcpyAsync(dgeom_own.p, geom_in ? geom_in : L.entities.data(), ne * S * 8,
+                                 hipMemcpyHostToDevice, st));
+    dgeom = dgeom_own.p;
+  }
+  const KeyView kv{t->layout.dbits, t->layout.cbits};
+  Buf<uint64_t> zero_key(ctx, ne);  // k_gather reads set ids from keys: one set, zero keys will do
+  OTG_HIP_CHECK(hipMemsetAsync(zero_key.p, 0, ne * 8, st));
+  Buf<uint64_t> d_off(ctx, 2);
+  const uint64_t offs[2] = {0, ne};
+  OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, offs, 16, hipMemcpyHostToDevice, st));
+  Buf<uint32_t> ids_copy(ctx, ne);
+  OTG_HIP_CHECK(hipMemcpyAsync(ids_copy.p, t->d_ids, ne * 4, hipMemcpyDeviceToDevice, st));
+  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(ne, kB)), dim3(kB), 0, dgeom, (const uint32_t *)ids_copy.p, ne,
+         (const uint64_t *)zero_key.p, kv, (const uint64_t *)d_off.p, t->d_geom, t->d_ids);
+  Buf<uint32_t> err(ctx, 2);
+  OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 8, st));
+  launch(ctx, "loaded_check", k_loaded_check<D, BOX>, dim3(div_up(nn, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info,
+         nn, (const double *)t->d_geom, ne, (const double *)t->d_nmin, (const double *)t->d_nsize, err.p);
+  if (BOX) {  // location keys against the stored tree box, then node keys
+    Buf<uint64_t> keys(ctx, ne);
+    Buf<uint32_t> vals(ctx, ne);
+    launch(ctx, "encode", k_encode<D, true>, dim3(div_up(ne, kB)), dim3(kB), 0, dgeom, ne, (const uint64_t *)d_off.p,
+           1u, (const SetParams *)t->d_sets, kv, keys.p, vals.p, err.p + 1);
+    t->d_keys = (uint64_t *)ctx->alloc(ne * 8);
+    launch(ctx, "loaded_keys", k_loaded_keys, dim3(div_up(ne, kB)), dim3(kB), 0, (const uint64_t *)keys.p,
+           (const uint32_t *)t->d_ids, ne, t->d_keys);
+    launch(ctx, "loaded_node_keys", k_loaded_node_keys<D>, dim3(div_up(nn, kB)), dim3(kB), 0,
+           (const NodeInfo *)t->d_info, (const uint8_t *)t->d_depth, nn, (const uint64_t *)t->d_keys, ne, kv, L.M,
+           t->d_nkey);
+  }
+  uint32_t herr[2] = {0, 0};
+  OTG_HIP_CHECK(hipMemcpyAsync(herr, err.p, 8, hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  if (herr[0]) throw Error{OTG_ERR_INVALID_ARG, "an entity lies outside its node in the archive: not these entities"};
+  if (herr[1]) throw Error{OTG_ERR_INVALID_ARG, "an entity lies outside the archive's tree box"};
+  t->device_bytes = ne * (S * 8 + 4 + (BOX ? 8 : 0)) + nn * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+
+  // points: the finer kNN / range index when max-depth leaves overflow, as a build makes it
+  if (!BOX) {
+    bool overfull = false;
+    for (uint64_t i = 0; i < nn && !overfull; ++i) overfull = !info[i].ccount && info[i].ecount > sp.cap;
+    const uint32_t fmax = finer_depth(D, 0);
+    if (overfull && sp.max_depth < fmax) {
+      const uint32_t fdepth = finer_set_depth(sp, D, fmax);
+      uint32_t icap = ctx->tuning.index_capacity ? ctx->tuning.index_capacity : sp.cap;
+      Tree *x = new Tree();
+      x->ctx = ctx;
+      x->kind = t->kind;
+      x->dim = t->dim;
+      try {
+        build_impl<D, false>(x, dgeom, OTG_MEM_DEVICE, offs, 1, nullptr, &fdepth, &icap, false);
+      } catch (...) {
+        delete x;
+        throw;
+      }
+      t->knn_tree = x;
+      t->device_bytes += x->device_bytes;
+    }
+  }
+}
+
+void load_static_archive(Tree *t, const uint8_t *buf, uint64_t size, bool large, uint64_t n, const double *geom,
+                         int mem) {
+  const bool box = t->kind == OTG_KIND_BOX;
+  switch (t->dim) {
+#define OTG_CASE(DD)                                               \
+  case DD:                                                         \
+    if (box) load_impl<DD, true>(t, buf, size, large, n, geom, mem);  \
+    else load_impl<DD, false>(t, buf, size, large, n, geom, mem);     \
+    break;
+    OTG_CASE(2)
+    OTG_CASE(3)
+    OTG_CASE(4)
+    OTG_CASE(5)
+    OTG_CASE(6)
+#undef OTG_CASE
+    default:
+      throw Error{OTG_ERR_INVALID_ARG, "dimension must be 2..6"};
+  }
+  publish_views(t);
 }
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets, const double *boxes,

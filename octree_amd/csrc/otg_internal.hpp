@@ -259,6 +259,11 @@ void publish_views(Tree *t);
 TreeView reference_view_of(const Tree *t);
 const TreeView *reference_device_view(const Tree *t);
 
+// Loads a StaticLinearOrthoTreeCore archive (as otg_tree_serialize writes it) into t
+// (kind and dim set): geom = the n entities in id order (host or device per mem), or
+// null for an archive that carries them after the core (OrthoTreeManaged).
+void load_static_archive(Tree *t, const uint8_t *buf, uint64_t size, bool large, uint64_t n, const double *geom,
+                         int mem);
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets,
                   const double *boxes, const uint32_t *depths, const uint32_t *caps);
 

@@ -225,6 +225,28 @@ int main() {
   put(uint32_t(3));  // the managed tree's entity vector
   for (const Point2D &p : hp) arr(p[0], p[1]);
   EXPECT(hm.Serialize() == hand);
+
+  // loading archives back (serialization.h:373-386, :433-439): the hand archive into a
+  // core and a managed tree, the reference's TestStaticOctreeBinary round trip
+  // (tests/unittests/serialize.tests.cpp:133-146), and a rejected mismatch
+  StaticQuadtreePointM lm;
+  EXPECT(lm.Deserialize(hand));
+  EXPECT(lm.GetData().size() == 3 && lm.Get(2)[1] == hp[2][1]);
+  EXPECT(lm.Serialize() == hand);
+  EXPECT(lm.GetNearestNeighbors(Point2D{3.4, 0.6}, 1).size() == 1 && lm.GetNearestNeighbors(Point2D{3.4, 0.6}, 1)[0] == 1);
+  {
+    const std::vector<Point3D> two = {{0.1, 0.1, 0.1}, {0.9, 0.9, 0.9}};
+    StaticOctreePointM saved(two, 3, BoundingBox3D{{0, 0, 0}, {1, 1, 1}});
+    StaticOctreePointM loaded;
+    EXPECT(loaded.Deserialize(saved.Serialize()));
+    auto a = saved.RangeSearch(BoundingBox3D{{0, 0, 0}, {1, 1, 1}});
+    auto b = loaded.RangeSearch(BoundingBox3D{{0, 0, 0}, {1, 1, 1}});
+    EXPECT(a.size() == 2 && a == b);
+    StaticOctreePoint core;
+    EXPECT(core.Deserialize(saved.GetCore().Serialize(), two));
+    const std::vector<Point3D> moved = {{0.1, 0.1, 0.1}, {1.5, 0.9, 0.9}};  // outside the root
+    EXPECT(!core.Deserialize(saved.GetCore().Serialize(), moved));
+  }
   if (failures == 0) std::printf("DROPIN OK\n");
   return failures ? 1 : 0;
 }
