@@ -465,6 +465,11 @@ def run_single(args, octree_amd, torch):
         tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
         plan.free()
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
+        # Create<SeqExec>: the same tree numbered as the reference's default (unsorted) build
+        seq = octree_amd.StaticOctreePoint(ctx=ctx)
+        ts, _ = _timed(ctx, lambda: seq.Create(d_pts, 10, box, 16, exec_tag=octree_amd.SEQ_EXEC), reps)
+        seq.free()
+        out["seq_build_ms"] = ts * 1e3
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
                     "config": {"workload": "OctreePoint 1,000,000 pts U[0,100)^3, depth 10, cap 16; 10,000 kNN k=8"},
                     "build_ms": tb * 1e3, "build_mpts_s": 1.0 / tb, "knn_ms": tk * 1e3,
@@ -476,8 +481,12 @@ def run_single(args, octree_amd, torch):
             t0 = time.perf_counter()
             o.knn(qs, 8, threads=threads)
             ck = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            O.OracleTree(3, "point", pts, box, 10, 16, seq=True)
+            cs = time.perf_counter() - t0
             out["cpu_baseline"] = {"value": len(qs) / ck, "unit": "kNN queries/s", "cores": threads, "kind": "port",
-                                   "sample": "whole workload", "build_mpts_s": 1.0 / cb, "cpu": cpu_model()}
+                                   "sample": "whole workload", "build_mpts_s": 1.0 / cb, "cpu": cpu_model(),
+                                   "seq_build_ms_single_thread": cs * 1e3}
     elif args.workload == "octree_box_4m":
         boxes, rays, rng = make_box_4m()
         n = boxes.shape[0]

@@ -138,13 +138,14 @@ class StaticOrthoTreeGPU {
   explicit StaticOrthoTreeGPU(Context &ctx) noexcept : ctx_(&ctx) {}
 
   template <typename TExecMode>
-  StaticOrthoTreeGPU(TExecMode, EntityContainerView entities, std::optional<depth_t> maxDepthID = std::nullopt,
+  StaticOrthoTreeGPU(TExecMode execMode, EntityContainerView entities, std::optional<depth_t> maxDepthID = std::nullopt,
                      std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16) noexcept {
-    [[maybe_unused]] bool ok = Create(entities, maxDepthID, boxSpace, maxElementNoInNode);
+    [[maybe_unused]] bool ok = Create(entities, maxDepthID, boxSpace, maxElementNoInNode, execMode);
   }
+  // without an execution tag the reference builds with SeqExec (ot_static_linear_core.h:105-115)
   explicit StaticOrthoTreeGPU(EntityContainerView entities, std::optional<depth_t> maxDepthID = std::nullopt,
                               std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16) noexcept {
-    [[maybe_unused]] bool ok = Create(entities, maxDepthID, boxSpace, maxElementNoInNode);
+    [[maybe_unused]] bool ok = Create(entities, maxDepthID, boxSpace, maxElementNoInNode, SEQ_EXEC);
   }
   StaticOrthoTreeGPU(const StaticOrthoTreeGPU &) = delete;
   StaticOrthoTreeGPU &operator=(const StaticOrthoTreeGPU &) = delete;
@@ -159,7 +160,9 @@ class StaticOrthoTreeGPU {
   ~StaticOrthoTreeGPU() { otg_tree_free(tree_); }
 
   // ---- build ------------------------------------------------------------------
-  template <typename TExecMode = ParExec>
+  // Create<ParExec> numbers the tree by sorted locations, Create<SeqExec> (the
+  // reference's default) as the unsorted partition build does (:417-519).
+  template <typename TExecMode = SeqExec>
   bool Create(EntityContainerView entities, std::optional<depth_t> maxDepthIn = std::nullopt,
               std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16,
               TExecMode = {}) noexcept {
@@ -172,17 +175,18 @@ class StaticOrthoTreeGPU {
         box[d] = boxSpace->Min[d];
         box[DIM + d] = boxSpace->Max[d];
       }
-    otg_status st = otg_tree_build(ctx().get(), IS_BOX ? OTG_KIND_BOX : OTG_KIND_POINT, DIM, entities.size(),
-                                   reinterpret_cast<const double *>(entities.data()), OTG_MEM_HOST,
-                                   boxSpace ? box.data() : nullptr, maxDepthIn.value_or(0),
-                                   static_cast<uint32_t>(maxElementNoInNode), &tree_);
+    auto build = std::is_same_v<TExecMode, SeqExec> ? otg_tree_build_seq : otg_tree_build;
+    otg_status st = build(ctx().get(), IS_BOX ? OTG_KIND_BOX : OTG_KIND_POINT, DIM, entities.size(),
+                          reinterpret_cast<const double *>(entities.data()), OTG_MEM_HOST,
+                          boxSpace ? box.data() : nullptr, maxDepthIn.value_or(0),
+                          static_cast<uint32_t>(maxElementNoInNode), &tree_);
     if (st == OTG_ERR_OUT_OF_SPACE) return false;
     detail::check(st, "otg_tree_build", ctx().get());
     entity_count_ = entities.size();
     return true;
   }
 
-  template <typename TExecMode = ParExec>
+  template <typename TExecMode = SeqExec>
   static bool Create(StaticOrthoTreeGPU &tree, EntityContainerView entities, std::optional<depth_t> maxDepthIn = std::nullopt,
                      std::optional<TBox> boxSpace = std::nullopt, std::size_t maxElementNoInNode = 16,
                      TExecMode execMode = {}) noexcept {
@@ -553,8 +557,9 @@ class OrthoTreeManaged {
   static constexpr std::size_t DEFAULT_TARGET_ELEMENT_NUM_IN_NODES = 16;  // core/configuration.h:83
 
   OrthoTreeManaged() = default;
-  // ot_managed.h:213-316: runtime or compile-time execution tag, with box space
-  template <typename TExecMode = ParExec>
+  // ot_managed.h:213-316: runtime or compile-time execution tag (SeqExec by default, as
+  // ot_managed.h:84-213), with box space
+  template <typename TExecMode = SeqExec>
   explicit OrthoTreeManaged(std::span<Entity const> const &entities, std::optional<depth_t> maxDepthID = std::nullopt,
                             std::optional<TBox> boxSpace = std::nullopt,
                             std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES,
@@ -562,7 +567,7 @@ class OrthoTreeManaged {
       : m_entities(entities.begin(), entities.end()) {
     m_tree.Create(m_entities, maxDepthID, boxSpace, maxElementNoInNode, execMode);
   }
-  template <typename TExecMode = ParExec>
+  template <typename TExecMode = SeqExec>
   explicit OrthoTreeManaged(EntityContainer &&entities, std::optional<depth_t> maxDepthID = std::nullopt,
                             std::optional<TBox> boxSpace = std::nullopt,
                             std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES,
@@ -577,7 +582,7 @@ class OrthoTreeManaged {
                             std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES) noexcept
       : OrthoTreeManaged(entities, maxDepthID, boxSpace, maxElementNoInNode, execMode) {}
 
-  template <typename TExecMode = ParExec>
+  template <typename TExecMode = SeqExec>
   static OrthoTreeManaged Create(std::span<Entity const> const &entities, std::optional<depth_t> maxDepthID = std::nullopt,
                                  std::optional<TBox> boxSpace = std::nullopt,
                                  std::size_t maxElementNoInNode = DEFAULT_TARGET_ELEMENT_NUM_IN_NODES,

@@ -111,6 +111,17 @@ OTG_API otg_status otg_ctx_reset_kernel_times(otg_ctx *ctx);
 OTG_API otg_status otg_tree_build(otg_ctx *ctx, int kind, int dim, uint64_t n, const double *geom, int mem,
                                   const double *box_or_null, uint32_t max_depth, uint32_t max_elements, otg_tree **out);
 
+/* Create<SeqExec> (ot_static_linear_core.h:523-599 with ARE_LOCATIONS_SORTED = false):
+ * the same tree as otg_tree_build, numbered as the reference's unsorted build
+ * numbers it (Build<false> :417-519): each node's entities are split by repeated
+ * two-pointer partitions (detail/partitioning.h:215-241) in the order they arrive,
+ * so children follow their pivots' order and entities the partitions' order.  The
+ * reference's default execution mode.  otg_tree_export's sorted_keys are then the
+ * location keys in that storage order. */
+OTG_API otg_status otg_tree_build_seq(otg_ctx *ctx, int kind, int dim, uint64_t n, const double *geom, int mem,
+                                      const double *box_or_null, uint32_t max_depth, uint32_t max_elements,
+                                      otg_tree **out);
+
 /* Batched build: nsets independent trees of uneven sizes built together.
  * set_offsets: nsets+1 host values (entity ranges in geom); boxes_or_null:
  * nsets x 2*dim host doubles; max_depths_or_null: nsets (0 = estimate);

@@ -276,7 +276,9 @@ class StaticTree:
 
     def Create(self, entities, max_depth: Optional[int] = None, box_space=None,
                max_elements: int = 16, exec_tag=PAR_EXEC) -> bool:
-        """StaticLinearOrthoTreeCore::Create<ParExec> (ot_static_linear_core.h:523-599)."""
+        """StaticLinearOrthoTreeCore::Create (ot_static_linear_core.h:523-599): PAR_EXEC
+        numbers the tree as Create<ParExec> (sorted locations), SEQ_EXEC as
+        Create<SeqExec> (the unsorted partition build, the reference's default)."""
         self.free()
         if _is_device(entities):
             gptr, n = _dev(entities, self._cols())
@@ -288,9 +290,10 @@ class StaticTree:
         h = C.c_void_p()
         L = _abi.lib()
         kind = _abi.OTG_KIND_POINT if self.kind == "point" else _abi.OTG_KIND_BOX
-        st = L.otg_tree_build(self.ctx.handle, kind, self.dim, n, gptr, mem, _ptr(box), int(max_depth or 0),
-                              int(max_elements), C.byref(h))
-        _check(st, "otg_tree_build", self.ctx)
+        fn = L.otg_tree_build_seq if exec_tag == SEQ_EXEC else L.otg_tree_build
+        st = fn(self.ctx.handle, kind, self.dim, n, gptr, mem, _ptr(box), int(max_depth or 0), int(max_elements),
+                C.byref(h))
+        _check(st, "otg_tree_build_seq" if exec_tag == SEQ_EXEC else "otg_tree_build", self.ctx)
         self.handle = h
         self.ctx._children.add(self)
         self.n = n

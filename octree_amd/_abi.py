@@ -46,7 +46,8 @@ STATUS_NAMES = {
 EXPORTED = [
     "otg_ctx_create", "otg_ctx_destroy", "otg_ctx_synchronize", "otg_status_string", "otg_ctx_last_error",
     "otg_ctx_mem_stats", "otg_ctx_release_cache", "otg_ctx_set_profiling", "otg_ctx_kernel_times",
-    "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_forest_build", "otg_tree_deserialize", "otg_tree_free",
+    "otg_ctx_reset_kernel_times", "otg_tree_build", "otg_tree_build_seq", "otg_forest_build",
+    "otg_tree_deserialize", "otg_tree_free",
     "otg_tree_get_info",
     "otg_tree_get_set_info", "otg_tree_export", "otg_tree_serialize", "otg_knn", "otg_range", "otg_ray_first", "otg_collide", "otg_knn_footprint",
     "otg_collide_trees", "otg_pick", "otg_ray_all", "otg_planes", "otg_knn_noalloc",
@@ -127,6 +128,7 @@ def lib() -> C.CDLL:
         "otg_ctx_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "otg_knn_noalloc": (C.c_int, [P, C.c_uint64, P, P, C.c_uint32, C.c_double, C.c_double, P, P, C.c_uint64,
                                       C.POINTER(C.c_uint64)]),
+        "otg_tree_build_seq": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, P, C.c_int, P, C.c_uint32, C.c_uint32, PP]),
         "otg_tree_deserialize": (C.c_int, [P, C.c_int, C.c_int, P, C.c_uint64, C.c_uint32, C.c_uint64, P, C.c_int,
                                            PP]),
         "otg_knn_plan_create": (C.c_int, [P, C.c_uint64, P, P, C.c_uint32, C.c_double, C.c_double, PP]),

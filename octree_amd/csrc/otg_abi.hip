@@ -443,6 +443,33 @@ OTG_API otg_status otg_forest_build(otg_ctx *ctx, int kind, int dim, uint32_t ns
   return OTG_OK;
 }
 
+OTG_API otg_status otg_tree_build_seq(otg_ctx *ctx, int kind, int dim, uint64_t n, const double *geom, int mem,
+                                      const double *box_or_null, uint32_t max_depth, uint32_t max_elements,
+                                      otg_tree **out) {
+  if (!ctx || !out || max_elements == 0) return OTG_ERR_INVALID_ARG;
+  if (kind != OTG_KIND_POINT && kind != OTG_KIND_BOX) return OTG_ERR_INVALID_ARG;
+  if (dim < 2 || dim > otg::kMaxDim) return OTG_ERR_INVALID_ARG;
+  if (mem != OTG_MEM_HOST && mem != OTG_MEM_DEVICE) return OTG_ERR_INVALID_ARG;
+  if (n && !geom) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  otg_tree *t = new (std::nothrow) otg_tree();
+  if (!t) return OTG_ERR_OOM;
+  t->ctx = ctx;
+  t->kind = kind;
+  t->dim = dim;
+  otg_status st = guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::build_seq(t, geom, mem, n, box_or_null, max_depth, max_elements);
+  });
+  if (st != OTG_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    delete t;
+    return st;
+  }
+  *out = t;
+  return OTG_OK;
+}
+
 OTG_API otg_status otg_tree_deserialize(otg_ctx *ctx, int kind, int dim, const uint8_t *buf, uint64_t size,
                                         uint32_t flags, uint64_t n, const double *geom, int mem, otg_tree **out) {
   if (!ctx || !out || !buf || (flags & ~OTG_ARCHIVE_LARGE_DATASET)) return OTG_ERR_INVALID_ARG;

@@ -1307,6 +1307,429 @@ const TreeView *reference_device_view(const Tree *t) {
   return t->d_view_ref;
 }
 
+__global__ void __launch_bounds__(kB) k_nonzero_u32(const uint32_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n) out[i] = in[i] ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kB) k_add_u32(uint32_t *__restrict__ a, uint64_t n, uint32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n) a[i] += v;
+}
+
+// ---------------------------------------------------------------------------
+// SEQ-path numbering: Build<ARE_LOCATIONS_SORTED = false> (ot_static_linear_core.h:417-519)
+// ---------------------------------------------------------------------------
+// The unsorted build splits each node's entity range, in the order the entities
+// have reached, by repeated partitions: for boxes first the entities at the node's
+// depth (std::partition, :449-459), then one child at a time, the child of the
+// range's first entity (the pivot) moved to the front (Partitioning::partition,
+// detail/partitioning.h:215-241).  Both are the two-pointer scheme, whose result is
+// this: with T the entities that belong in front, the i-th entity from the left of
+// [0, T) that does not belong there swaps with the i-th entity from the right of
+// [T, n) that does.  A block of 256 threads does one node: it ranks those entities
+// with ballots and swaps the pairs.  Each BFS level is one launch; a child is
+// recorded at its first position, so the next level, in position order, is the
+// reference's queue order (parents in order, each parent's children in pivot order).
+constexpr int kSB = 256;
+
+__device__ __forceinline__ uint32_t seq_rank(bool flag, uint32_t *s_w, uint32_t &total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t b = __ballot(flag);
+  const uint32_t in_wave = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+  if (lane == 0) s_w[wave] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kSB / 64; ++w) {
+    const uint32_t c = s_w[w];
+    if ((uint32_t)w < wave) off += c;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return off + in_wave;
+}
+
+struct SeqPred {
+  bool by_depth;     // boxes' first partition: entity depth == node depth
+  uint32_t depth;
+  uint32_t shift;    // child partitions: digit at the examined level equals `flag`
+  uint64_t flag;
+  uint64_t mask;
+  __device__ bool operator()(uint64_t key, KeyView kv, uint32_t M) const {
+    return by_depth ? (kv.depth(key, M) == depth) : (((kv.code(key) >> shift) & mask) == flag);
+  }
+};
+
+struct SeqAcc {
+  uint32_t count;
+  uint64_t lo, hi;  // smallest and largest code of the front part
+  uint32_t mind;    // smallest entity depth of the front part
+};
+
+// Partition [lo, hi) of (keys, ids) by pred; returns the front part's count and
+// the reduction of its codes.  ff / bt: position scratch over [lo, hi).
+__device__ SeqAcc seq_partition(uint64_t *keys, uint32_t *ids, uint32_t lo, uint32_t hi, SeqPred pred, KeyView kv,
+                                uint32_t M, uint32_t *ff, uint32_t *bt, uint32_t *s_w, uint64_t *s_r) {
+  // front part size and its code range
+  uint32_t cnt = 0, mind = 255;
+  uint64_t cmin = ~0ull, cmax = 0;
+  for (uint32_t p = lo + threadIdx.x; p < hi; p += kSB) {
+    const uint64_t k = keys[p];
+    if (pred(k, kv, M)) {
+      ++cnt;
+      const uint64_t c = kv.code(k);
+      cmin = c < cmin ? c : cmin;
+      cmax = c > cmax ? c : cmax;
+      const uint32_t dk = kv.depth(k, M);
+      mind = dk < mind ? dk : mind;
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    cnt += __shfl_xor(cnt, o, 64);
+    const uint64_t a = __shfl_xor(cmin, o, 64), b = __shfl_xor(cmax, o, 64);
+    const uint32_t m = __shfl_xor(mind, o, 64);
+    cmin = a < cmin ? a : cmin;
+    cmax = b > cmax ? b : cmax;
+    mind = m < mind ? m : mind;
+  }
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0) {
+    s_r[4 * wave + 0] = cnt;
+    s_r[4 * wave + 1] = cmin;
+    s_r[4 * wave + 2] = cmax;
+    s_r[4 * wave + 3] = mind;
+  }
+  __syncthreads();
+  SeqAcc acc{0, ~0ull, 0, 255};
+#pragma unroll
+  for (int w = 0; w < kSB / 64; ++w) {
+    acc.count += (uint32_t)s_r[4 * w + 0];
+    acc.lo = s_r[4 * w + 1] < acc.lo ? s_r[4 * w + 1] : acc.lo;
+    acc.hi = s_r[4 * w + 2] > acc.hi ? s_r[4 * w + 2] : acc.hi;
+    acc.mind = (uint32_t)s_r[4 * w + 3] < acc.mind ? (uint32_t)s_r[4 * w + 3] : acc.mind;
+  }
+  __syncthreads();
+  const uint32_t T = acc.count, mid = lo + T;
+  // entities of [lo, mid) that leave, left to right
+  uint32_t m = 0;
+  for (uint32_t p0 = lo; p0 < mid; p0 += kSB) {
+    const uint32_t p = p0 + threadIdx.x;
+    const bool f = p < mid && !pred(keys[p], kv, M);
+    uint32_t tot;
+    const uint32_t r = seq_rank(f, s_w, tot);
+    if (f) ff[lo + m + r] = p;
+    m += tot;
+  }
+  // entities of [mid, hi) that enter, right to left
+  uint32_t m2 = 0;
+  for (int64_t q0 = (int64_t)hi - 1; q0 >= (int64_t)mid; q0 -= kSB) {
+    const int64_t p = q0 - (int64_t)threadIdx.x;
+    const bool f = p >= (int64_t)mid && pred(keys[p], kv, M);
+    uint32_t tot;
+    const uint32_t r = seq_rank(f, s_w, tot);
+    if (f) bt[lo + m2 + r] = (uint32_t)p;
+    m2 += tot;
+  }
+  __syncthreads();  // every pair position written before any is read
+  // m == m2: as many leave the front as enter it
+  for (uint32_t r = threadIdx.x; r < m; r += kSB) {
+    const uint32_t i = ff[lo + r], j = bt[lo + r];
+    const uint64_t ki = keys[i];
+    keys[i] = keys[j];
+    keys[j] = ki;
+    const uint32_t ii = ids[i];
+    ids[i] = ids[j];
+    ids[j] = ii;
+  }
+  __syncthreads();
+  return acc;
+}
+
+struct SeqItem {
+  uint64_t key;      // location key of a member (the child's pivot)
+  uint32_t begin, count;
+  uint32_t depth;    // LCA depth (the node's depth)
+  uint32_t own;      // the node's own segment length (all of it for a leaf)
+  uint32_t nchild;
+  uint32_t cfirst;   // BFS id of the first child
+};
+
+// One BFS level: every item a block.  Children go to (kid_key, kid_cnt, kid_depth)
+// at their first position.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kSB) k_seq_level(SeqItem *__restrict__ items, uint64_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ ids, uint32_t *__restrict__ ff,
+                                                    uint32_t *__restrict__ bt, KeyView kv, uint32_t M, uint32_t cap,
+                                                    uint64_t *__restrict__ kid_key, uint32_t *__restrict__ kid_cnt,
+                                                    uint8_t *__restrict__ kid_depth) {
+  __shared__ uint32_t s_w[kSB / 64];
+  __shared__ uint64_t s_r[4 * (kSB / 64)];
+  const SeqItem it = items[blockIdx.x];
+  const uint32_t b = it.begin, e = it.begin + it.count;
+  uint32_t own = it.count, nch = 0;
+  if (it.count > cap && it.depth < M) {
+    own = 0;
+    if (BOX) {
+      const SeqAcc a = seq_partition(keys, ids, b, e, SeqPred{true, it.depth, 0, 0, 0}, kv, M, ff, bt, s_w, s_r);
+      own = a.count;
+    }
+    if (own < it.count) {
+      const uint32_t shift = (uint32_t)D * (M - (it.depth + 1));
+      uint32_t cb = b + own;
+      while (cb < e) {
+        const uint64_t pivot = keys[cb];
+        const uint64_t flag = (kv.code(pivot) >> shift) & ((1ull << D) - 1ull);
+        const SeqAcc a = seq_partition(keys, ids, cb, e, SeqPred{false, 0, shift, flag, (1ull << D) - 1ull}, kv, M, ff,
+                                       bt, s_w, s_r);
+        const uint64_t diff = a.lo ^ a.hi;
+        const uint32_t level_diff = (bw64(diff) + D - 1) / D;
+        uint32_t lca = M - level_diff;
+        if (BOX && a.mind < lca) lca = a.mind;
+        if (threadIdx.x == 0) {
+          kid_key[cb] = pivot;
+          kid_cnt[cb] = a.count;
+          kid_depth[cb] = (uint8_t)lca;
+        }
+        ++nch;
+        cb += a.count;
+      }
+    } else {
+      own = it.count;  // only entities of the node's depth: a leaf (:460-465)
+    }
+  }
+  if (threadIdx.x == 0) {
+    items[blockIdx.x].own = own;
+    items[blockIdx.x].nchild = nch;
+  }
+}
+
+// next level: the recorded children in position order (clears the records)
+__global__ void __launch_bounds__(kB) k_seq_gather_level(const uint32_t *__restrict__ scan, uint64_t n,
+                                                          uint64_t *__restrict__ kid_key, uint32_t *__restrict__ kid_cnt,
+                                                          const uint8_t *__restrict__ kid_depth,
+                                                          SeqItem *__restrict__ next) {
+  const uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (p >= n || !kid_cnt[p]) return;
+  SeqItem x;
+  x.key = kid_key[p];
+  x.begin = (uint32_t)p;
+  x.count = kid_cnt[p];
+  x.depth = kid_depth[p];
+  x.own = 0;
+  x.nchild = 0;
+  x.cfirst = 0;
+  next[scan[p]] = x;
+  kid_cnt[p] = 0;
+}
+
+__global__ void k_seq_child_flags(const SeqItem *__restrict__ items, uint32_t n, uint32_t *__restrict__ nch) {
+  const uint32_t i = blockIdx.x * kB + threadIdx.x;
+  if (i < n) nch[i] = items[i].nchild;
+}
+
+// node records from the BFS items (node x = item x); cfirst: BFS id of the first child
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_seq_finalize(const SeqItem *__restrict__ items, const uint32_t *__restrict__ cfirst,
+                                                      uint64_t nn, const SetParams *__restrict__ sets, KeyView kv,
+                                                      NodeInfo *__restrict__ info, uint32_t *__restrict__ rend,
+                                                      uint8_t *__restrict__ ndepth, uint64_t *__restrict__ nodekey,
+                                                      double *__restrict__ nmin, double *__restrict__ nsize) {
+  const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nn) return;
+  const SeqItem it = items[x];
+  const SetParams &sp = sets[0];
+  const uint32_t M = sp.max_depth, depth = it.depth;
+  NodeInfo ni;
+  ni.ebegin = it.begin;
+  ni.ecount = it.own;
+  ni.cbegin = it.nchild ? cfirst[x] : 0u;
+  ni.ccount = it.nchild;
+  info[x] = ni;
+  rend[x] = it.begin + it.count;
+  ndepth[x] = (uint8_t)depth;
+  const uint32_t sh = (M - depth) * D;
+  const uint64_t nkey = (1ull << (depth * D)) | ((sh >= 64) ? 0ull : (kv.code(it.key) >> sh));
+  nodekey[x] = nkey;
+#pragma unroll
+  for (int d = 0; d < D; ++d) nsize[x * D + d] = sp.depth_size[(BOX ? 0 : depth) * D + d];
+  if (!BOX) {  // InitNodeGeometry (MinPoint), as k_finalize
+    uint32_t g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = 0;
+    uint32_t shift = 0;
+    for (uint32_t lvl = M - depth; lvl < M; ++lvl)
+#pragma unroll
+      for (int d = 0; d < D; ++d, ++shift) g[d] |= (uint32_t)((nkey >> shift) & 1ull) << lvl;
+    const double half_grid = (double)(1ull << (M - depth)) * 0.5;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double center = ((double)g[d] + half_grid) * sp.deraster[d] + sp.space_min[d];
+      nmin[x * D + d] = center - sp.depth_size[(depth + 1) * D + d];
+      nsize[x * D + d] = sp.depth_size[depth * D + d];
+    }
+  }
+}
+
+template <int D, bool BOX>
+static void build_seq_impl(Tree *t, const double *geom_in, int mem, uint64_t n, const double *box, uint32_t depth,
+                           uint32_t cap) {
+  Ctx *ctx = t->ctx;
+  constexpr int S = BOX ? 2 * D : D;
+  t->n = n;
+  if (n >= (1ull << 30)) throw Error{OTG_ERR_UNSUPPORTED, "more than 2^30 entities"};
+  Buf<double> dgeom_own;
+  const double *dgeom = geom_in;
+  if (mem == OTG_MEM_HOST) {
+    dgeom_own = Buf<double>(ctx, n * S);
+    if (n) OTG_HIP_CHECK(hipMemcpyAsync(dgeom_own.p, geom_in, n * S * 8, hipMemcpyHostToDevice, ctx->stream));
+    dgeom = dgeom_own.p;
+  }
+  const uint64_t offs[2] = {0, n};
+  prepare_sets<D, BOX>(t, dgeom, offs, 1, box, depth ? &depth : nullptr, &cap);
+  Buf<uint64_t> d_off(ctx, 2);
+  OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, offs, 16, hipMemcpyHostToDevice, ctx->stream));
+  upload_sets(t);
+  if (n == 0) {
+    t->node_count = 0;
+    return;
+  }
+  const SetParams &sp = t->sets[0];
+  const uint32_t M = sp.max_depth;
+  const KeyView kv{t->layout.dbits, t->layout.cbits};
+  // locations in input order; the entity order the partitions evolve
+  t->d_keys = (uint64_t *)ctx->alloc(n * 8);
+  Buf<uint32_t> order(ctx, n), err(ctx, 1);
+  OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, ctx->stream));
+  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, (const uint64_t *)d_off.p, 1u,
+         (const SetParams *)t->d_sets, kv, t->d_keys, order.p, err.p);
+  Buf<uint32_t> ff(ctx, n), bt(ctx, n), kid_cnt(ctx, n), scan(ctx, n + 1), total(ctx, 1);
+  Buf<uint64_t> kid_key(ctx, n);
+  Buf<uint8_t> kid_depth(ctx, n);
+  OTG_HIP_CHECK(hipMemsetAsync(kid_cnt.p, 0, n * 4, ctx->stream));
+  // BFS levels, concatenated: node x = item x
+  std::vector<SeqItem> root(1);
+  root[0] = SeqItem{0, 0, (uint32_t)n, 0, 0, 0, 0};
+  std::vector<uint64_t> level_off{0, 1};
+  uint64_t cap_items = std::max<uint64_t>(1024, n / 4);
+  Buf<SeqItem> items(ctx, cap_items);
+  OTG_HIP_CHECK(hipMemcpyAsync(items.p, root.data(), sizeof(SeqItem), hipMemcpyHostToDevice, ctx->stream));
+  uint32_t herr = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  if (herr & kErrOutOfSpace) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
+  if (herr & kErrUnsupported) throw Error{OTG_ERR_UNSUPPORTED, "box level or grid id beyond the tree depth"};
+  for (;;) {
+    const uint64_t l0 = level_off[level_off.size() - 2], l1 = level_off.back();
+    launch(ctx, "seq_level", k_seq_level<D, BOX>, dim3((uint32_t)(l1 - l0)), dim3(kSB), 0, items.p + l0, t->d_keys,
+           order.p, ff.p, bt.p, kv, M, sp.cap, kid_key.p, kid_cnt.p, kid_depth.p);
+    // children recorded at their first position, in position order
+    Buf<uint32_t> flags(ctx, n);
+    launch(ctx, "seq_flags", k_nonzero_u32, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)kid_cnt.p, n, flags.p);
+    exclusive_scan_u32(ctx, flags.p, scan.p, n, total.p);
+    uint32_t next = 0;
+    OTG_HIP_CHECK(hipMemcpyAsync(&next, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (next == 0) break;
+    if (l1 + next > cap_items) {
+      Buf<SeqItem> bigger(ctx, std::max<uint64_t>(2 * cap_items, l1 + next));
+      OTG_HIP_CHECK(hipMemcpyAsync(bigger.p, items.p, l1 * sizeof(SeqItem), hipMemcpyDeviceToDevice, ctx->stream));
+      std::swap(items, bigger);
+      cap_items = std::max<uint64_t>(2 * cap_items, l1 + next);
+    }
+    launch(ctx, "seq_gather_level", k_seq_gather_level, dim3(div_up(n, kB)), dim3(kB), 0, (const uint32_t *)scan.p, n,
+           kid_key.p, kid_cnt.p, (const uint8_t *)kid_depth.p, items.p + l1);
+    level_off.push_back(l1 + next);
+  }
+  const uint64_t nn = level_off.back();
+  // first child of each node: the children of level L are level L + 1 in parent order
+  Buf<uint32_t> nch(ctx, nn), cfirst(ctx, nn);
+  launch(ctx, "seq_child_counts", k_seq_child_flags, dim3(div_up(nn, kB)), dim3(kB), 0, (const SeqItem *)items.p,
+         (uint32_t)nn, nch.p);
+  exclusive_scan_u32(ctx, nch.p, cfirst.p, nn, total.p);
+  launch(ctx, "seq_child_base", k_add_u32, dim3(div_up(nn, kB)), dim3(kB), 0, cfirst.p, nn, 1u);  // + the root
+  t->node_count = nn;
+  t->sets[0].node_off = 0;
+  t->sets[0].node_cnt = nn;
+  t->d_info = (NodeInfo *)ctx->alloc(nn * sizeof(NodeInfo));
+  t->d_rend = (uint32_t *)ctx->alloc(nn * 4);
+  t->d_depth = (uint8_t *)ctx->alloc(nn);
+  t->d_nkey = (uint64_t *)ctx->alloc(nn * 8);
+  t->d_nmin = (double *)ctx->alloc(nn * D * 8);
+  t->d_nsize = (double *)ctx->alloc(nn * D * 8);
+  upload_sets(t);
+  launch(ctx, "seq_finalize", k_seq_finalize<D, BOX>, dim3(div_up(nn, kB)), dim3(kB), 0, (const SeqItem *)items.p,
+         (const uint32_t *)cfirst.p, nn, (const SetParams *)t->d_sets, kv, t->d_info, t->d_rend, t->d_depth, t->d_nkey,
+         t->d_nmin, t->d_nsize);
+  // entities in the order the partitions left them
+  t->d_geom = (double *)ctx->alloc(n * S * 8);
+  t->d_ids = (uint32_t *)ctx->alloc(n * 4);
+  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, (const uint32_t *)order.p, n,
+         (const uint64_t *)t->d_keys, kv, (const uint64_t *)d_off.p, t->d_geom, t->d_ids);
+  if (BOX) build_mbr<D>(t);
+  SeqItem r0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&r0, items.p, sizeof(SeqItem), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  t->sets[0].root_splits = r0.nchild ? 1u : 0u;
+  upload_sets(t);
+  t->device_bytes = n * (S * 8 + 4 + 8) + nn * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+  // points: the finer kNN / range index when max-depth leaves overflow, as a build makes it
+  if (!BOX) {
+    const uint32_t fmax = finer_depth(D, 0);
+    if (M < fmax) {
+      Buf<uint32_t> flag(ctx, 1);
+      OTG_HIP_CHECK(hipMemsetAsync(flag.p, 0, 4, ctx->stream));
+      launch(ctx, "oversized", k_oversized, dim3(div_up(nn, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info, nn,
+             (const SetParams *)t->d_sets, 1u, flag.p);
+      uint32_t hflag = 0;
+      OTG_HIP_CHECK(hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      if (hflag) {
+        const uint32_t fdepth = finer_set_depth(t->sets[0], D, fmax);
+        uint32_t icap = ctx->tuning.index_capacity ? ctx->tuning.index_capacity : cap;
+        double sbox[2 * kMaxDim];
+        for (int d = 0; d < D; ++d) {
+          sbox[d] = t->sets[0].space_min[d];
+          sbox[D + d] = t->sets[0].space_max[d];
+        }
+        Tree *x = new Tree();
+        x->ctx = ctx;
+        x->kind = t->kind;
+        x->dim = t->dim;
+        try {
+          build_impl<D, false>(x, dgeom, OTG_MEM_DEVICE, offs, 1, sbox, &fdepth, &icap, false);
+        } catch (...) {
+          delete x;
+          throw;
+        }
+        t->knn_tree = x;
+        t->device_bytes += x->device_bytes;
+      }
+    }
+  }
+}
+
+void build_seq(Tree *t, const double *geom, int mem, uint64_t n, const double *box, uint32_t depth, uint32_t cap) {
+  const bool box_kind = t->kind == OTG_KIND_BOX;
+  switch (t->dim) {
+#define OTG_CASE(DD)                                                          \
+  case DD:                                                                    \
+    if (box_kind) build_seq_impl<DD, true>(t, geom, mem, n, box, depth, cap); \
+    else build_seq_impl<DD, false>(t, geom, mem, n, box, depth, cap);         \
+    break;
+    OTG_CASE(2)
+    OTG_CASE(3)
+    OTG_CASE(4)
+    OTG_CASE(5)
+    OTG_CASE(6)
+#undef OTG_CASE
+    default:
+      throw Error{OTG_ERR_INVALID_ARG, "dimension must be 2..6"};
+  }
+  publish_views(t);
+}
+
 // ---------------------------------------------------------------------------
 // loading a static-core archive: serialize(BinaryInputArchive&,
 // StaticLinearOrthoTreeCore&) (serialization.h:359-386, binary_archive.h) reads the

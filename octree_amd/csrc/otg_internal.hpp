@@ -259,6 +259,9 @@ void publish_views(Tree *t);
 TreeView reference_view_of(const Tree *t);
 const TreeView *reference_device_view(const Tree *t);
 
+// Create<SeqExec> of one tree (kind and dim set): the node numbering and entity order of
+// the reference's unsorted build (Build<false>, ot_static_linear_core.h:417-519).
+void build_seq(Tree *t, const double *geom, int mem, uint64_t n, const double *box, uint32_t depth, uint32_t cap);
 // Loads a StaticLinearOrthoTreeCore archive (as otg_tree_serialize writes it) into t
 // (kind and dim set): geom = the n entities in id order (host or device per mem), or
 // null for an archive that carries them after the core (OrthoTreeManaged).

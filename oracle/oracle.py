@@ -29,6 +29,10 @@ def lib() -> C.CDLL:
     P = C.c_void_p
     L.oc_build.restype = P
     L.oc_build.argtypes = [C.c_int, C.c_int, C.c_uint32, P, P, C.c_uint32, C.c_uint32]
+    L.oc_build_seq.restype = P
+    L.oc_build_seq.argtypes = [C.c_int, C.c_int, C.c_uint32, P, P, C.c_uint32, C.c_uint32]
+    L.oc_partition_codes.restype = C.c_uint32
+    L.oc_partition_codes.argtypes = [P, P, C.c_uint32, C.c_uint64, C.c_uint64]
     L.oc_build_mt.restype = P
     L.oc_build_mt.argtypes = [C.c_int, C.c_int, C.c_uint32, P, P, C.c_uint32, C.c_uint32, C.c_int]
     L.oc_tree_free.argtypes = [P]
@@ -112,14 +116,19 @@ def estimate_max_depth(n, cap, D):
 class OracleTree:
     """CPU restatement of StaticLinearOrthoTreeCore (points: MinPoint, boxes: loose 2.0 + MBR)."""
 
-    def __init__(self, dim: int, kind: str, geom, box=None, max_depth: int = 0, cap: int = 16, threads: int = 1):
+    def __init__(self, dim: int, kind: str, geom, box=None, max_depth: int = 0, cap: int = 16, threads: int = 1,
+                 seq: bool = False):
         self.dim = dim
         self.kind = kind
         cols = dim if kind == "point" else 2 * dim
         self.geom = np.ascontiguousarray(np.asarray(geom, np.float64).reshape(-1, cols))
         bx = None if box is None else np.ascontiguousarray(np.asarray(box, np.float64).reshape(-1))
-        self.h = lib().oc_build_mt(dim, 0 if kind == "point" else 1, self.geom.shape[0], _p(self.geom), _p(bx),
-                                   int(max_depth or 0), int(cap), int(threads))
+        if seq:  # Create<SeqExec>: the unsorted partition build
+            self.h = lib().oc_build_seq(dim, 0 if kind == "point" else 1, self.geom.shape[0], _p(self.geom), _p(bx),
+                                        int(max_depth or 0), int(cap))
+        else:
+            self.h = lib().oc_build_mt(dim, 0 if kind == "point" else 1, self.geom.shape[0], _p(self.geom), _p(bx),
+                                       int(max_depth or 0), int(cap), int(threads))
         if not self.h:
             raise ValueError("oracle rejected the build parameters")
         self.status = lib().oc_tree_status(self.h)
@@ -194,3 +203,12 @@ class OracleTree:
 
 def lists(off, ids):
     return [ids[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
+
+
+def partition_codes(codes, ids, mask, flag):
+    """Partitioning::partition (detail/partitioning.h:215-241) of (code, id) pairs by
+    (code & mask) == flag; returns (codes, ids, front count)."""
+    c = np.ascontiguousarray(np.asarray(codes, np.uint64)).copy()
+    i = np.ascontiguousarray(np.asarray(ids, np.uint32)).copy()
+    m = lib().oc_partition_codes(_p(c), _p(i), len(c), int(mask), int(flag))
+    return c, i, int(m)

@@ -568,6 +568,152 @@ OC_API oc_tree *oc_build_mt(int D, int is_box, uint32_t n, const double *geom, c
   return t;
 }
 
+/* ------------------------------------------------------------------------- */
+/* SEQ path: Create<SeqExec> with ARE_LOCATIONS_SORTED = false                */
+/* ------------------------------------------------------------------------- */
+/* detail/partitioning.h:215-241 Partitioning::partition, the two-pointer scheme
+ * (std::partition of the non-refinable boxes, ot_static_linear_core.h:455-456, is
+ * the same scheme in libstdc++'s bidirectional __partition).  Predicate: entity
+ * depth == depth (by_depth), else (code & mask) == flag.  Returns the end of the
+ * front part. */
+static uint32_t oc_partition(oc_loc *a, uint32_t first, uint32_t last, int by_depth, uint32_t depth, uint64_t mask,
+                             uint64_t flag) {
+#define OC_P(x) (by_depth ? ((x).depth == depth) : (((x).code & mask) == flag))
+  for (;;) {
+    while (first != last && OC_P(a[first])) ++first;
+    if (first == last) return first;
+    for (;;) {
+      if (first == --last) return first;
+      if (OC_P(a[last])) break;
+    }
+    oc_loc tmp = a[first];
+    a[first] = a[last];
+    a[last] = tmp;
+    ++first;
+  }
+#undef OC_P
+}
+
+/* The partition on (code, id) arrays, for checking the scheme against the C++
+ * library's std::partition. */
+OC_API uint32_t oc_partition_codes(uint64_t *codes, uint32_t *ids, uint32_t n, uint64_t mask, uint64_t flag) {
+  oc_loc *a = (oc_loc *)malloc(sizeof(oc_loc) * (n ? n : 1));
+  for (uint32_t i = 0; i < n; ++i) a[i] = (oc_loc){codes[i], 0, ids[i]};
+  uint32_t m = oc_partition(a, 0, n, 0, 0, mask, flag);
+  for (uint32_t i = 0; i < n; ++i) {
+    codes[i] = a[i].code;
+    ids[i] = a[i].id;
+  }
+  free(a);
+  return m;
+}
+
+/* ot_static_linear_core.h:417-519 Build<false>: the BFS queue of Build<true>, but
+ * a node's range is split in the order its entities have reached: the boxes at
+ * the node's depth are partitioned to the front (:449-459), then repeatedly the
+ * first entity of the rest is the pivot and its child's entities are partitioned
+ * to the front (:484-500), each child queued with the LCA of its entities. */
+static void oc_build_bfs_seq(oc_tree *t, oc_loc *locs, uint32_t n) {
+  int D = t->dim;
+  uint32_t M = t->max_depth;
+  uint64_t child_mask = ((uint64_t)1 << D) - 1;
+  oc_qitem *queue = (oc_qitem *)malloc(sizeof(oc_qitem) * (size_t)(2 * n + 16));
+  size_t qh = 0, qt = 0;
+  queue[qt++] = (oc_qitem){0, 0, 0, n};
+  while (qh < qt) {
+    oc_qitem it = queue[qh++];
+    uint32_t depth = it.depth;
+    if (it.length <= t->cap || depth >= M) {
+      oc_create_node(t, it.code, depth, it.begin, it.length, it.begin, it.begin + it.length);
+      continue;
+    }
+    uint32_t b = it.begin, e = it.begin + it.length;
+    uint32_t nr = 0;
+    if (t->is_box) {
+      nr = oc_partition(locs, b, e, 1, depth, 0, 0) - b;
+      if (nr == it.length) {
+        oc_create_node(t, it.code, depth, it.begin, it.length, it.begin, e);
+        continue;
+      }
+    }
+    uint32_t examined = M - (depth + 1);
+    uint64_t mask = child_mask << (examined * (unsigned)D);
+    uint32_t cb = b + nr;
+    uint32_t first_child_slot = (uint32_t)qt, nchild = 0;
+    while (cb < e) {
+      const oc_loc pivot = locs[cb];
+      uint32_t ce = oc_partition(locs, cb, e, 0, 0, mask, pivot.code & mask);
+      uint32_t min_depth = pivot.depth;
+      uint64_t diff = 0;
+      for (uint32_t i = cb; i < ce; ++i) { /* LowestCommonAncestorCalculator (si_morton.h:346-387) */
+        if (t->is_box && locs[i].depth < min_depth) min_depth = locs[i].depth;
+        diff |= pivot.code ^ locs[i].code;
+      }
+      uint32_t lca = M - (bit_width64(diff) + (unsigned)D - 1) / (unsigned)D;
+      if (min_depth < lca) lca = min_depth;
+      queue[qt++] = (oc_qitem){pivot.code, lca, cb, ce - cb};
+      ++nchild;
+      cb = ce;
+    }
+    uint32_t id = oc_create_node(t, it.code, depth, it.begin, nr, it.begin, e);
+    t->node_cbegin[id] = first_child_slot; /* BFS: node id == queue slot */
+    t->node_ccount[id] = nchild;
+  }
+  free(queue);
+}
+
+/* ot_static_linear_core.h:523-599 Create<SeqExec>: locations in entity order, no
+ * sort, Build<false>, MBR pass. */
+OC_API oc_tree *oc_build_seq(int D, int is_box, uint32_t n, const double *geom, const double *box_or_null,
+                             uint32_t max_depth_in, uint32_t cap) {
+  if (D < 1 || D > OC_MAXD || cap == 0) return NULL;
+  oc_tree *t = (oc_tree *)calloc(1, sizeof(oc_tree));
+  t->dim = D;
+  t->is_box = is_box;
+  t->cap = cap;
+  t->n = n;
+  t->geom = geom;
+  double bmin[OC_MAXD], bmax[OC_MAXD];
+  if (box_or_null) {
+    for (int d = 0; d < D; ++d) {
+      bmin[d] = box_or_null[d];
+      bmax[d] = box_or_null[D + d];
+    }
+  } else {
+    oc_bounding_box(geom, n, D, is_box, bmin, bmax);
+  }
+  t->max_depth = max_depth_in ? max_depth_in : oc_estimate_max_depth(n, cap, D);
+  if (t->max_depth < 1 || t->max_depth > oc_max_theoretical_depth(D) || t->max_depth > 19) {
+    free(t);
+    return NULL;
+  }
+  oc_init_space(t, bmin, bmax);
+  if (n == 0) return t;
+  oc_loc *locs = (oc_loc *)malloc(sizeof(oc_loc) * n);
+  for (uint32_t i = 0; i < n; ++i) {
+    locs[i].id = i;
+    if (is_box) oc_box_location(t, &geom[(size_t)i * 2 * D], &locs[i].code, &locs[i].depth);
+    else oc_point_location(t, &geom[(size_t)i * D], &locs[i].code, &locs[i].depth);
+    if (!is_box && locs[i].depth == 0) t->status = 2;
+  }
+  if (t->status == 2) {
+    free(locs);
+    return t;
+  }
+  oc_build_bfs_seq(t, locs, n);
+  t->entity_ids = (uint32_t *)malloc(4u * n);
+  t->sorted_code = (uint64_t *)malloc(8u * n);
+  t->sorted_depth = (uint32_t *)malloc(4u * n);
+  for (uint32_t i = 0; i < n; ++i) {
+    t->entity_ids[i] = locs[i].id;
+    t->sorted_code[i] = locs[i].code;
+    t->sorted_depth[i] = locs[i].depth;
+  }
+  free(locs);
+  if (is_box) oc_build_mbr(t);
+  return t;
+}
+
 OC_API void oc_tree_free(oc_tree *t) {
   if (!t) return;
   free(t->depth_size);
