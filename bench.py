@@ -183,8 +183,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the forest shards by whole trees; see DESIGN.md); "
-              "--gpus must be 1", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only (a forest would shard by whole trees, one process per "
+              "GPU); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -445,6 +445,31 @@ def _timed(ctx, fn, reps):
     return float(np.median(ts)), ts
 
 
+def knn_roofline(ctx, tree, d_q, k, D, set_ids=None):
+    """Roofline of one kNN launch: minimal touched bytes (otg_knn_footprint: the query
+    point 8D B, 4 B per returned id, a node record of 13 + 8D B -- entity segment 8,
+    child begin 4, depth 1, MinPoint 8D -- for every node whose wall distance is below
+    the final radius, 8D + 4 B per entity those nodes hold) over the kNN kernel's
+    average HIP-event time on the library stream."""
+    r = tree.knn_batch(d_q, k, set_ids=set_ids)  # warm
+    r.free()
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    r = tree.knn_batch(d_q, k, set_ids=set_ids)
+    ctx.synchronize()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    r.free()
+    ms, launches = kt["knn"]
+    fp = tree.knn_footprint(d_q, k, set_ids=set_ids)
+    nq = d_q.shape[0]
+    algo = 8 * D * nq + 4 * fp["ids"] + (13 + 8 * D) * fp["nodes"] + (8 * D + 4) * fp["entities"]
+    avg = ms / launches
+    return {"bound": "hbm", "kernel": "knn", "achieved": algo / (avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": algo / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes": algo, "avg_launch_ms": avg, "footprint": fp, "per_query_bytes": algo / nq}
+
+
 def run_single(args, octree_amd, torch):
     from oracle import oracle as O
 
@@ -465,6 +490,7 @@ def run_single(args, octree_amd, torch):
         tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
         plan.free()
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
+        out["roofline"] = knn_roofline(ctx, tree, d_q, 8, 3)
         # Create<SeqExec>: the same tree numbered as the reference's default (unsorted) build
         seq = octree_amd.StaticOctreePoint(ctx=ctx)
         ts, _ = _timed(ctx, lambda: seq.Create(d_pts, 10, box, 16, exec_tag=octree_amd.SEQ_EXEC), reps)
@@ -591,6 +617,7 @@ def run_single(args, octree_amd, torch):
         tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
         plan.free()
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
+        out["roofline"] = knn_roofline(ctx, tree, d_q, 16, 6)
         # CPU: oracle build of a 1M-point sample and 20k queries on it
         m = 1_000_000
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
@@ -717,18 +744,30 @@ def run_served(args, octree_amd):
 
             th = threading.Thread(target=producer)
             th.start()
+        det = [None] * len(reqs)
         for j in range(len(reqs)):
             while ids[j] is None:
                 time.sleep(0.0002)
-            res, l_ms, _ = srv.wait(ids[j])
-            lat[j] = l_ms
+            res, det[j] = srv.wait_detail(ids[j])
+            lat[j] = det[j]["latency_ms"]
             res.free()
         T1 = time.perf_counter()
         if load_qps is not None:
             th.join()
         st = srv.stats()
         srv.close()
-        return {"seconds": T1 - T0, "queries_per_s": float(nq.sum() / (T1 - T0)),
+        # where the slowest 1% of requests spent their time, beside the median request
+        slow = lat >= np.percentile(lat, 99)
+        mid = np.abs(lat - np.percentile(lat, 50)) <= np.percentile(np.abs(lat - np.percentile(lat, 50)), 10)
+
+        def mean_of(mask):
+            return {k: float(np.mean([det[j][k] for j in np.nonzero(mask)[0]]))
+                    for k in ("latency_ms", "queue_ms", "service_ms", "resident_ms", "batch_requests", "cold")}
+
+        breakdown = {"p99_and_above": dict(mean_of(slow), requests=int(slow.sum())),
+                     "around_p50": dict(mean_of(mid), requests=int(mid.sum())),
+                     "cold_requests": int(sum(d["cold"] for d in det))}
+        return {"seconds": T1 - T0, "queries_per_s": float(nq.sum() / (T1 - T0)), "latency_breakdown": breakdown,
                 "requests_per_s": len(reqs) / (T1 - T0), "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "launches": st["launches"] - st0["launches"],
                 "batches": st["batches"] - st0["batches"], "evictions": st["evictions"] - st0["evictions"],

@@ -350,6 +350,12 @@ OTG_API otg_status otg_server_submit(otg_server *server, uint32_t tree_id, int q
  * latency_ms = submit to done, queue_ms = submit to batch start (either may be NULL). */
 OTG_API otg_status otg_server_wait(otg_server *server, uint64_t request_id, otg_result **out, double *latency_ms,
                                    double *queue_ms);
+/* otg_server_wait with where the time went: detail[0] latency, [1] queueing (submit to
+ * batch start, including the wait that gathers requests for an evicted tree), [2] the
+ * batch's service time, [3] of which making trees resident (rebuilds, waits for
+ * memory), [4] requests in the batch, [5] 1 if the request's tree was not resident
+ * when the batch formed (all in ms). */
+OTG_API otg_status otg_server_wait_detail(otg_server *server, uint64_t request_id, otg_result **out, double *detail);
 /* stats[15] = batches, launches, requests served, queries served, evictions, resident bytes,
  * resident trees, pending requests, rebuilds of evicted trees, registered trees, and
  * microseconds spent building trees, in query launches overall, and within those staging

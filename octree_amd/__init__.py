@@ -714,6 +714,17 @@ class Server:
         _check(st, "otg_server_wait")
         return Result(h, None), lat.value, qms.value
 
+    def wait_detail(self, request_id: int):
+        """-> (Result, dict): otg_server_wait_detail -- latency, queueing, service, time
+        making trees resident in the batch, batch size, and whether the tree was cold."""
+        h, d = C.c_void_p(), (C.c_double * 6)()
+        st = _abi.lib().otg_server_wait_detail(self.handle, int(request_id), C.byref(h), d)
+        if st != _abi.OTG_OK and h.value:
+            _abi.lib().otg_result_free(h)
+        _check(st, "otg_server_wait_detail")
+        keys = ("latency_ms", "queue_ms", "service_ms", "resident_ms", "batch_requests", "cold")
+        return Result(h, None), {k: float(d[i]) for i, k in enumerate(keys)}
+
     def stats(self) -> dict:
         v = (C.c_uint64 * 15)()
         _check(_abi.lib().otg_server_stats(self.handle, v), "otg_server_stats")

@@ -55,7 +55,7 @@ EXPORTED = [
     "otg_autotune", "otg_ctx_set_tuning", "otg_ctx_get_tuning",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
-    "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_stats",
+    "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_wait_detail", "otg_server_stats",
 ]
 
 
@@ -157,6 +157,7 @@ def lib() -> C.CDLL:
         "otg_server_submit": (C.c_int, [P, C.c_uint32, C.c_int, C.c_uint64, P, C.c_uint32, C.c_double, C.c_double,
                                         C.c_double, C.c_int, U64P]),
         "otg_server_wait": (C.c_int, [P, C.c_uint64, PP, F64P, F64P]),
+        "otg_server_wait_detail": (C.c_int, [P, C.c_uint64, PP, C.POINTER(C.c_double)]),
         "otg_server_stats": (C.c_int, [P, U64P]),
     }
     for name, (res, args) in sig.items():

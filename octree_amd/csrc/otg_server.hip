@@ -40,6 +40,9 @@ struct ServerRequest {
   uint64_t nq = 0;
   std::vector<double> data;
   double t_submit = 0, t_start = 0, t_done = 0;
+  bool cold = false;            // its tree was not resident when its batch formed
+  double batch_resident_ms = 0;  // wall time its batch spent making trees resident
+  uint32_t batch_size = 0;      // requests in its batch
   bool done = false;
   otg_status status = OTG_OK;
   std::string error;
@@ -305,6 +308,7 @@ class Server {
             }
             q += r->nq;
             r->t_start = now;
+            r->cold = cold;
             batch.push_back(r);
             it = pending.erase(it);
           }
@@ -314,10 +318,12 @@ class Server {
         ++batches;
         for (auto &kv : trees) kv.second.heat *= 0.9;
       }
-      run_batch(ctx, *staging[w], batch);
+      const double resident_ms = run_batch(ctx, *staging[w], batch);
       {
         std::lock_guard<std::mutex> g(mu);
         for (ServerRequest *r : batch) {
+          r->batch_resident_ms = resident_ms;
+          r->batch_size = (uint32_t)batch.size();
           r->t_done = now_ms();
           r->done = true;
           served_requests++;
@@ -332,7 +338,9 @@ class Server {
   // the others in chunks that fit the budget -- so each evicted tree is rebuilt at
   // most once per batch; within a chunk, one launch per (query, tree kind, dim,
   // k class, parameters).
-  void run_batch(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &batch) {
+  // Returns the wall time spent making trees resident (rebuilds and waits for memory).
+  double run_batch(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &batch) {
+    double resident_ms = 0.0;
     std::unordered_map<uint32_t, std::vector<ServerRequest *>> by_tree;
     std::vector<uint32_t> tree_order;
     std::vector<std::vector<uint32_t>> chunks;
@@ -386,11 +394,12 @@ class Server {
           }
         }
       }
-      for (auto &kv : groups) run_group(ctx, stg, kv.second);
+      for (auto &kv : groups) run_group(ctx, stg, kv.second, resident_ms);
     }
+    return resident_ms;
   }
 
-  void run_group(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &group) {
+  void run_group(Ctx *ctx, Staging &stg, std::vector<ServerRequest *> &group, double &resident_ms) {
     size_t i = 0;
     while (i < group.size()) {
       // make the trees resident and pin them for the launch; when memory for the
@@ -405,7 +414,9 @@ class Server {
           ServerRequest *r = group[i];
           if (!view_of_tree.count(r->tree_id)) {
             auto it = trees.find(r->tree_id);
+            const double tr = now_ms();
             otg_status st = it == trees.end() ? OTG_ERR_INVALID_ARG : make_resident(lk, it->second, !pinned.empty());
+            resident_ms += now_ms() - tr;
             it = trees.find(r->tree_id);  // the map may have changed while unlocked
             if (st != OTG_OK || it == trees.end() || !it->second.tree) {
               if (st == OTG_ERR_OOM && !pinned.empty()) break;  // retried with no pins held
@@ -696,6 +707,30 @@ OTG_API otg_status otg_server_wait(otg_server *s, uint64_t request_id, otg_resul
   lk.unlock();
   if (latency_ms) *latency_ms = r->t_done - r->t_submit;
   if (queue_ms) *queue_ms = r->t_start - r->t_submit;
+  otg_status st = r->status;
+  *out = r->result;
+  r->result = nullptr;
+  delete r;
+  return st;
+}
+
+OTG_API otg_status otg_server_wait_detail(otg_server *s, uint64_t request_id, otg_result **out, double *detail) {
+  if (!s || !out || !detail) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->requests.find(request_id);
+  if (it == s->requests.end()) return OTG_ERR_INVALID_ARG;
+  otg::ServerRequest *r = it->second;
+  s->cv_done.wait(lk, [&] { return r->done || s->stop; });
+  if (!r->done) return OTG_ERR_INVALID_ARG;
+  s->requests.erase(r->id);
+  lk.unlock();
+  detail[0] = r->t_done - r->t_submit;
+  detail[1] = r->t_start - r->t_submit;
+  detail[2] = r->t_done - r->t_start;
+  detail[3] = r->batch_resident_ms;
+  detail[4] = (double)r->batch_size;
+  detail[5] = r->cold ? 1.0 : 0.0;
   otg_status st = r->status;
   *out = r->result;
   r->result = nullptr;

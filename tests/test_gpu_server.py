@@ -160,9 +160,14 @@ def test_lru_eviction_under_budget():
     assert st["evictions"] >= 1
     assert srv.has_tree(3) and srv.has_tree(1) and not srv.has_tree(2)
     assert srv.tree_state(2) == 2  # registered, evicted
-    # a request on the evicted tree rebuilds it (evicting the least recently used one)
-    r, _, _ = srv.wait(srv.submit_knn(2, pts[1][:5], 1))
+    # a request on the evicted tree rebuilds it (evicting the least recently used one);
+    # its detail shows the cold tree and the time spent making it resident
+    r, d = srv.wait_detail(srv.submit_knn(2, pts[1][:5], 1))
     assert np.array_equal(r.csr()[1], np.arange(5, dtype=np.uint32))
+    assert d["cold"] == 1.0 and d["batch_requests"] >= 1 and d["resident_ms"] > 0.0
+    assert abs(d["latency_ms"] - (d["queue_ms"] + d["service_ms"])) < 1e-6 and d["resident_ms"] <= d["service_ms"]
+    r2, d2 = srv.wait_detail(srv.submit_knn(2, pts[1][:5], 1))
+    assert d2["cold"] == 0.0 and np.array_equal(r2.csr()[1], np.arange(5, dtype=np.uint32))
     st = srv.stats()
     assert st["rebuilds"] == 1 and srv.has_tree(2) and st["resident_trees"] == 2
     with pytest.raises(octree_amd.OtgError):
