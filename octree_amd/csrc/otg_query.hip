@@ -142,6 +142,63 @@ struct KnnState {
   bool heapified, any_dropped;
 };
 
+// Compare-exchange of (distance, id) list entries: ascending distance, ties by id.
+__device__ __forceinline__ void cas_id(double &da, uint32_t &ia, double &db, uint32_t &ib) {
+  const bool sw = db < da || (db == da && ib < ia);
+  const double tw = da;
+  const uint32_t ti = ia;
+  da = sw ? db : da;
+  ia = sw ? ib : ia;
+  db = sw ? tw : db;
+  ib = sw ? ti : ib;
+}
+
+// Entries of equal distance sit in arrival order in the list; the output orders them
+// by id.  The list is sorted by distance, so compare-exchanges of neighbours by
+// (distance, id) only reorder equal runs, and KCAP rounds of odd-even transposition
+// sort them completely.  Ties are rare: the rounds run only when one exists.
+template <int KCAP>
+__device__ __forceinline__ void knn_tie_order(double (&bd)[KCAP], uint32_t (&bid)[KCAP], uint32_t cnt) {
+  bool tie = false;
+#pragma unroll
+  for (int t = 0; t + 1 < KCAP; ++t) tie |= (uint32_t)(t + 1) < cnt && bd[t] == bd[t + 1];
+  if (!tie) return;
+  for (int r = 0; r < KCAP; ++r) {
+#pragma unroll
+    for (int t = 0; t + 1 < KCAP; ++t) {
+      if ((t & 1) != (r & 1)) continue;
+      if ((uint32_t)(t + 1) < cnt && bd[t] == bd[t + 1] && bid[t + 1] < bid[t]) {
+        const uint32_t x = bid[t];
+        bid[t] = bid[t + 1];
+        bid[t + 1] = x;
+      }
+    }
+  }
+}
+
+// Batcher's odd-even merge sort on N = 2^m inputs (Knuth, TAOCP 3, 5.3.4, Algorithm M)
+// as a compile-time list of compare-exchange pairs: 19 for N = 8, 63 for 16, 191 for
+// 32, 543 for 64.  Fully unrolled, every index is a constant (the list stays in
+// registers).
+template <int N>
+struct MergeNet {
+  static constexpr int kCap = N * 10;
+  int n = 0;
+  int a[kCap] = {};
+  int b[kCap] = {};
+  constexpr MergeNet() {
+    for (int p = 1; p < N; p <<= 1)
+      for (int k = p; k >= 1; k >>= 1)
+        for (int j = k % p; j + k < N; j += 2 * k)
+          for (int i = 0; i < k && i + j + k < N; ++i)
+            if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+              a[n] = i + j;
+              b[n] = i + j + k;
+              ++n;
+            }
+  }
+};
+
 template <int KCAP>
 __device__ __forceinline__ void knn_init(KnnState<KCAP> &S, double F0) {
   S.cnt = 0;
@@ -158,8 +215,11 @@ __device__ __forceinline__ bool knn_node_out(const KnnState<KCAP> &S, double w2,
   return inside ? (0.0 >= S.F) : (w2 >= S.T || sqrt(w2) >= S.F);
 }
 
-// Inserts (dist, id) into the optimistic list (ascending; ties by id), keeping
-// the KCAP smallest and remembering the smallest dropped distance.
+// Inserts (dist, id) into the optimistic list (ascending distance; equal distances in
+// arrival order, put in id order by knn_tie_order at the output), keeping the KCAP
+// smallest and remembering the smallest dropped distance.  Which of several entries
+// of equal distance leaves a full list changes nothing: the dropped distance is the
+// same, and a dropped distance <= F sends the query to the band pass anyway.
 template <int KCAP>
 __device__ __forceinline__ void knn_insert(double (&bd)[KCAP], uint32_t (&bid)[KCAP], KnnState<KCAP> &S, double dist,
                                            uint32_t id) {
@@ -168,8 +228,7 @@ __device__ __forceinline__ void knn_insert(double (&bd)[KCAP], uint32_t (&bid)[K
   // from there on move up one place: one compare and one select per slot.
   bool before[KCAP];
 #pragma unroll
-  for (int t = 0; t < KCAP; ++t)
-    before[t] = (uint32_t)t < S.cnt && (bd[t] < dist || (bd[t] == dist && bid[t] < id));
+  for (int t = 0; t < KCAP; ++t) before[t] = (uint32_t)t < S.cnt && bd[t] <= dist;
   if (S.cnt == KCAP) {  // the largest of the KCAP + 1 entries leaves the list
     const double out_d = before[KCAP - 1] ? dist : bd[KCAP - 1];
     S.dropped = S.any_dropped ? smin(S.dropped, out_d) : out_d;
@@ -244,6 +303,38 @@ __device__ __forceinline__ void knn_offer(double (&bd)[KCAP], uint32_t (&bid)[KC
       if ((uint32_t)t == k - 1) dk = bd[t];
     knn_limit(bd, S, dk, tol);
   }
+}
+
+// The first k = KCAP - 1 entries of an empty list (k at the top of its class) from
+// the first k entities of a leaf chunk: their distances go straight into the slots,
+// one sorting network orders them, and F follows as after the k-th insertion.  This
+// equals k sequential offers: each of them enters (checked below), an empty list
+// drops nothing, and the network's (distance, id) order is the list's.  Returns
+// false, the list still empty (its count is 0), when one of them would not enter.
+template <int D, int KCAP>
+__device__ __forceinline__ bool knn_fill(const TreeView &tv, const double (&q)[D], uint32_t e0, double (&bd)[KCAP],
+                                         uint32_t (&bid)[KCAP], KnnState<KCAP> &L, double tol) {
+  bool ok = true;
+#pragma unroll
+  for (int t = 0; t < KCAP - 1; ++t) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double v = q[d] - gp(tv.geom)[(uint64_t)d * tv.n + e0 + t];
+      d2 += v * v;
+    }
+    const double dist = sqrt(d2);
+    ok &= d2 < L.T && dist < L.F;
+    bd[t] = dist;
+    bid[t] = gp(tv.ids)[e0 + t];
+  }
+  if (!ok) return false;
+  constexpr MergeNet<KCAP - 1> net{};
+#pragma unroll
+  for (int p = 0; p < net.n; ++p) cas_id(bd[net.a[p]], bid[net.a[p]], bd[net.b[p]], bid[net.b[p]]);
+  L.cnt = KCAP - 1;
+  knn_limit_top(bd, L, tol);
+  return true;
 }
 
 // Boxes: optimistic distance into the list, pessimistic distance into the k
@@ -407,6 +498,7 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
   }
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
+  knn_tie_order(bd, bid, L.cnt);
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
     if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
@@ -428,9 +520,12 @@ __device__ __forceinline__ void knn_scan(const TreeView &tv, const double (&q)[D
   constexpr int U = D <= 3 ? 4 : (D == 4 ? 2 : 1);  // entities loaded ahead (register budget)
   for (uint32_t base = j; base < je; base += 64) {
     const uint32_t cend = min(je, base + 64u);
+    uint32_t e = base;
+    // an empty list and k at the top of its class: the first k entities fill it at once
+    if (L.cnt == 0 && k == (uint32_t)KCAP - 1 && cend - base >= k && knn_fill<D, KCAP>(tv, q, base, bd, bid, L, tol))
+      e = base + k;
     const double T0 = L.T;
     uint64_t mark = 0;
-    uint32_t e = base;
     for (; e + U <= cend; e += U) {
       double d2[U];
 #pragma unroll
@@ -620,6 +715,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
   }
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
+  knn_tie_order(bd, bid, L.cnt);
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
     if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
@@ -725,6 +821,7 @@ __device__ __forceinline__ bool knn_query_masks(const TreeView &tv, uint64_t qi,
   }
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
+  knn_tie_order(bd, bid, L.cnt);
 #pragma unroll
   for (int t = 0; t < KCAP; ++t)
     if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
