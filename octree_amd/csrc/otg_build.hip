@@ -586,6 +586,28 @@ __global__ void k_set_node_offsets(const uint64_t *__restrict__ nkeys, uint64_t 
 // ---------------------------------------------------------------------------
 // 7. finalize
 // ---------------------------------------------------------------------------
+// InitNodeGeometry (MinPoint, :346-368) of a point-tree node: decode the node key
+// (sentinel bit above depth * D code bits), cell centre minus half the node size.
+template <int D>
+__device__ __forceinline__ void point_node_geometry(uint64_t nkey, uint32_t depth, const SetParams &sp, double *nmin,
+                                                    double *nsize) {
+  const uint32_t M = sp.max_depth;
+  uint32_t g[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = 0;
+  uint32_t shift = 0;
+  for (uint32_t lvl = M - depth; lvl < M; ++lvl)
+#pragma unroll
+    for (int d = 0; d < D; ++d, ++shift) g[d] |= (uint32_t)((nkey >> shift) & 1ull) << lvl;
+  const double half_grid = (double)(1ull << (M - depth)) * 0.5;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    double center = ((double)g[d] + half_grid) * sp.deraster[d] + sp.space_min[d];
+    nmin[d] = center - sp.depth_size[(depth + 1) * D + d];
+    nsize[d] = sp.depth_size[depth * D + d];
+  }
+}
+
 template <int D, bool BOX>
 __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nkeys, const uint32_t *__restrict__ ndesc,
                                                   uint64_t nnodes, const uint64_t *__restrict__ keys,
@@ -670,23 +692,79 @@ __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nk
   nodekey[x] = nkey;
 #pragma unroll
   for (int d = 0; d < D; ++d) nsize[x * D + d] = sp.depth_size[(BOX ? 0 : depth) * D + d];
-  if (!BOX) {
-    // InitNodeGeometry (MinPoint): decode the key, cell centre minus half size
-    uint32_t g[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) g[d] = 0;
-    uint32_t shift = 0;
-    for (uint32_t lvl = M - depth; lvl < M; ++lvl)
-#pragma unroll
-      for (int d = 0; d < D; ++d, ++shift) g[d] |= (uint32_t)((nkey >> shift) & 1ull) << lvl;
-    const double half_grid = (double)(1ull << (M - depth)) * 0.5;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      double center = ((double)g[d] + half_grid) * sp.deraster[d] + sp.space_min[d];
-      nmin[x * D + d] = center - sp.depth_size[(depth + 1) * D + d];
-      nsize[x * D + d] = sp.depth_size[depth * D + d];
-    }
-  }
+  if (!BOX) point_node_geometry<D>(nkey, depth, sp, &nmin[x * D], &nsize[x * D]);
+}
+
+// ---------------------------------------------------------------------------
+// 7b. reference point tree from its finer index (same sorted points, same capacity)
+// ---------------------------------------------------------------------------
+// The grids differ by an exact power of two per axis, so a node of depth d < M holds
+// the same entities in both trees and splits in one iff it splits in the other (count
+// > cap, d below both depths); its children group the same entities by the same
+// digit, and a child's reference depth is min(its finer depth, M): the common prefix
+// of the coarse codes is that of the fine codes cut at M.  The reference nodes are
+// therefore the finer roots and the children of finer nodes shallower than M; a kept
+// node of finer depth >= M is a reference leaf of depth M holding its subtree's
+// entities.  Kept nodes keep all their ancestors, and both tables are numbered by
+// (set, generation, position), so the reference numbering is the finer numbering
+// restricted to the kept nodes: an exclusive scan of the keep flags.
+__global__ void __launch_bounds__(kB) k_ref_mark(const NodeInfo *__restrict__ finfo, const uint8_t *__restrict__ fdepth,
+                                                  uint64_t nf, const uint64_t *__restrict__ fkeys, KeyView fkv,
+                                                  const SetParams *__restrict__ rsets, uint32_t *__restrict__ keep) {
+  const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nf) return;
+  const NodeInfo ni = finfo[x];
+  if (!ni.ccount) return;
+  if (fdepth[x] >= rsets[fkv.set(fkeys[ni.ebegin])].max_depth) return;
+  for (uint32_t c = 0; c < ni.ccount; ++c) keep[ni.cbegin + c] = 1u;
+}
+
+__global__ void k_ref_roots(const SetParams *__restrict__ fsets, uint32_t nsets, uint32_t *__restrict__ keep) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsets && fsets[s].node_cnt) keep[fsets[s].node_off] = 1u;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kB) k_ref_emit(const NodeInfo *__restrict__ finfo, const uint32_t *__restrict__ frend,
+                                                  const uint8_t *__restrict__ fdepth, const uint64_t *__restrict__ fnkey,
+                                                  uint64_t nf, const uint64_t *__restrict__ fkeys, KeyView fkv,
+                                                  const SetParams *__restrict__ rsets, const uint32_t *__restrict__ keep,
+                                                  const uint32_t *__restrict__ ridx, NodeInfo *__restrict__ info,
+                                                  uint32_t *__restrict__ rend, uint8_t *__restrict__ ndepth,
+                                                  uint64_t *__restrict__ nodekey, double *__restrict__ nmin,
+                                                  double *__restrict__ nsize) {
+  const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  if (x >= nf || !keep[x]) return;
+  const uint64_t r = ridx[x];
+  const NodeInfo fi = finfo[x];
+  const SetParams &sp = rsets[fkv.set(fkeys[fi.ebegin])];
+  const uint32_t df = fdepth[x];
+  const uint32_t dr = df < sp.max_depth ? df : sp.max_depth;
+  const bool internal = fi.ccount && df < sp.max_depth;
+  NodeInfo ni;
+  ni.ebegin = fi.ebegin;
+  ni.ecount = internal ? 0u : frend[x] - fi.ebegin;
+  ni.cbegin = internal ? ridx[fi.cbegin] : 0u;
+  ni.ccount = internal ? fi.ccount : 0u;
+  info[r] = ni;
+  rend[r] = frend[x];
+  ndepth[r] = (uint8_t)dr;
+  const uint64_t key = fnkey[x] >> (D * (df - dr));
+  nodekey[r] = key;
+  point_node_geometry<D>(key, dr, sp, &nmin[r * D], &nsize[r * D]);
+}
+
+__global__ void k_ref_sets(const SetParams *__restrict__ fsets, SetParams *__restrict__ rsets, uint32_t nsets,
+                           uint64_t nf, const uint32_t *__restrict__ ridx, const uint32_t *__restrict__ total,
+                           const NodeInfo *__restrict__ info) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsets) return;
+  const uint64_t b = fsets[s].node_off, e = b + fsets[s].node_cnt;
+  const uint64_t rb = (b == nf) ? *total : ridx[b];
+  const uint64_t re = (e == nf) ? *total : ridx[e];
+  rsets[s].node_off = rb;
+  rsets[s].node_cnt = re - rb;
+  rsets[s].root_splits = (re > rb && info[rb].ccount) ? 1u : 0u;
 }
 
 // sorted SoA geometry and local ids
@@ -1027,6 +1105,39 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
          t->d_nkey, t->d_nmin, t->d_nsize);
 }
 
+// Node table of a point tree from its finer index (k_ref_* above); every set of both
+// trees has the same capacity.
+template <int D>
+static void reference_from_finer(Tree *t, const Tree *aux) {
+  Ctx *ctx = t->ctx;
+  const uint64_t nf = aux->node_count;
+  const uint32_t nsets = (uint32_t)t->sets.size();
+  const KeyView fkv{aux->layout.dbits, aux->layout.cbits};
+  Buf<uint32_t> keep(ctx, nf), ridx(ctx, nf), total(ctx, 1);
+  fill_u32(ctx, keep.p, nf, 0u);
+  launch(ctx, "ref_mark", k_ref_mark, dim3(div_up(nf, kB)), dim3(kB), 0, (const NodeInfo *)aux->d_info,
+         (const uint8_t *)aux->d_depth, nf, (const uint64_t *)aux->d_keys, fkv, (const SetParams *)t->d_sets, keep.p);
+  launch(ctx, "ref_roots", k_ref_roots, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)aux->d_sets, nsets,
+         keep.p);
+  exclusive_scan_u32(ctx, keep.p, ridx.p, nf, total.p);
+  uint32_t nr = 0;
+  OTG_HIP_CHECK(hipMemcpyAsync(&nr, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  t->node_count = nr;
+  t->d_info = (NodeInfo *)ctx->alloc(std::max<uint64_t>(nr, 1) * sizeof(NodeInfo));
+  t->d_rend = (uint32_t *)ctx->alloc(std::max<uint64_t>(nr, 1) * 4);
+  t->d_depth = (uint8_t *)ctx->alloc(std::max<uint64_t>(nr, 1));
+  t->d_nkey = (uint64_t *)ctx->alloc(std::max<uint64_t>(nr, 1) * 8);
+  t->d_nmin = (double *)ctx->alloc(std::max<uint64_t>(nr, 1) * D * 8);
+  t->d_nsize = (double *)ctx->alloc(std::max<uint64_t>(nr, 1) * D * 8);
+  launch(ctx, "ref_emit", k_ref_emit<D>, dim3(div_up(nf, kB)), dim3(kB), 0, (const NodeInfo *)aux->d_info,
+         (const uint32_t *)aux->d_rend, (const uint8_t *)aux->d_depth, (const uint64_t *)aux->d_nkey, nf,
+         (const uint64_t *)aux->d_keys, fkv, (const SetParams *)t->d_sets, (const uint32_t *)keep.p,
+         (const uint32_t *)ridx.p, t->d_info, t->d_rend, t->d_depth, t->d_nkey, t->d_nmin, t->d_nsize);
+  launch(ctx, "ref_sets", k_ref_sets, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)aux->d_sets, t->d_sets,
+         nsets, nf, (const uint32_t *)ridx.p, (const uint32_t *)total.p, (const NodeInfo *)t->d_info);
+}
+
 // Sorted SoA geometry and local ids of the entity storage.
 template <int D, bool BOX>
 static void gather_entities(Tree *t, const double *dgeom, const uint32_t *vals, const uint64_t *d_off) {
@@ -1154,8 +1265,11 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
       launch(ctx, "coarse_keys", k_coarse_keys<D>, dim3(div_up(n, kB)), dim3(kB), 0, (const uint64_t *)aux->d_keys, n,
              KeyView{aux->layout.dbits, aux->layout.cbits}, (const SetParams *)aux->d_sets,
              KeyView{t->layout.dbits, t->layout.cbits}, (const SetParams *)t->d_sets, t->d_keys);
-      node_table<D, false>(t, d_off.p, &herr);
-      node_table<D, false>(aux, d_off.p, nullptr);
+      node_table<D, false>(aux, d_off.p, &herr);
+      bool same_caps = true;
+      for (uint32_t q = 0; q < nsets; ++q) same_caps &= aux->sets[q].cap == t->sets[q].cap;
+      if (same_caps) reference_from_finer<D>(t, aux);
+      else node_table<D, false>(t, d_off.p, nullptr);
       gather_entities<D, false>(t, dgeom, vals.p, d_off.p);
       aux->d_geom = t->d_geom;
       aux->d_ids = t->d_ids;

@@ -184,6 +184,8 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 constexpr uint32_t kScanOverflow = 0xFFFFFFFFu;
 void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total,
                         Arena *arena = nullptr);
+// a[0..n) = v by a kernel (no memset node when captured into a graph)
+void fill_u32(Ctx *ctx, uint32_t *a, uint64_t n, uint32_t v);
 // Build node events (n > 0 bytes: bit 0 segment start, bits 1-2 internal nodes):
 // idx[p] = exclusive counts before p, segments in the high 32 bits and internal nodes
 // in the low 32; d_totals[0..1] = the two totals.

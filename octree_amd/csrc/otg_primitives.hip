@@ -234,6 +234,15 @@ void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint
          (const uint64_t *)partials.p, d_totals, nb);
 }
 
+__global__ void k_fill_words(uint32_t *__restrict__ a, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+void fill_u32(Ctx *ctx, uint32_t *a, uint64_t n, uint32_t v) {
+  if (n) launch(ctx, "fill_u32", k_fill_words, dim3((uint32_t)std::min<uint64_t>(div_up(n, 256), 4096)), dim3(256), 0, a, n, v);
+}
+
 void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total, Arena *arena) {
   if (n == 0) {
     if (d_total) OTG_HIP_CHECK(hipMemsetAsync(d_total, 0, 4, ctx->stream));
