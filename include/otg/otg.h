@@ -312,6 +312,37 @@ OTG_API otg_status otg_knn_plan_result(const otg_knn_plan *plan, const uint64_t 
 OTG_API otg_status otg_knn_plan_copy(const otg_knn_plan *plan, uint64_t *offsets, uint32_t *ids);
 OTG_API void otg_knn_plan_free(otg_knn_plan *plan);
 
+/* ---- build plans: a forest build replayed as a HIP graph ----
+ * No counterpart in the reference (a header-only library that builds on the host):
+ * for callers that rebuild the same sets every step from new coordinates (the
+ * batched path's "build + query" loop).  otg_build_plan_create builds the point
+ * forest otg_forest_build would make from device-resident points (every set's box
+ * given), then records that build as a HIP graph with every buffer allocated up
+ * front and node capacities of the current contents plus an eighth and 4,096;
+ * otg_build_plan_run replays it on the current contents of the same points buffer
+ * (rewritten in place, not moved) and synchronises.  The plan's tree
+ * (otg_build_plan_tree; freed with the plan, never by otg_tree_free) then equals a
+ * fresh otg_forest_build of those contents and takes every query.  New contents that
+ * need more nodes than the plan holds fail the run with OTG_ERR_CAPACITY (the tree is
+ * then left without nodes; make a new plan); a point outside its set's box fails it
+ * with OTG_ERR_OUT_OF_SPACE.  Free kNN and step plans made on the tree first. */
+typedef struct otg_build_plan otg_build_plan;
+OTG_API otg_status otg_build_plan_create(otg_ctx *ctx, int dim, uint32_t nsets, const uint64_t *set_offsets,
+                                         const double *points, const double *boxes, const uint32_t *max_depths_or_null,
+                                         const uint32_t *max_elements, otg_build_plan **out);
+OTG_API otg_status otg_build_plan_run(otg_build_plan *plan);
+OTG_API otg_status otg_build_plan_tree(const otg_build_plan *plan, const otg_tree **tree);
+OTG_API void otg_build_plan_free(otg_build_plan *plan);
+
+/* A build plan and a kNN plan made on its tree, replayed as one graph: the build,
+ * then the kNN batch.  otg_step_plan_run synchronises; afterwards the tree is
+ * current and otg_knn_plan_result / otg_knn_plan_copy give the batch's ids, as after
+ * otg_build_plan_run followed by otg_knn_plan_run.  Free it before either plan. */
+typedef struct otg_step_plan otg_step_plan;
+OTG_API otg_status otg_step_plan_create(otg_build_plan *build, otg_knn_plan *knn, otg_step_plan **out);
+OTG_API otg_status otg_step_plan_run(otg_step_plan *step);
+OTG_API void otg_step_plan_free(otg_step_plan *step);
+
 OTG_API otg_status otg_result_get_info(const otg_result *res, uint64_t *nq, uint64_t *nids);
 OTG_API otg_status otg_result_copy(const otg_result *res, uint64_t *offsets, uint32_t *ids);
 /* Device-resident CSR (valid until otg_result_free). */

@@ -227,6 +227,83 @@ class KnnPlan:
             pass
 
 
+class BuildPlan:
+    """A point-forest build captured as a HIP graph (otg_build_plan): made from device
+    points and host set offsets (every set's box given), replayed by run() on whatever
+    the points tensor holds then.  `tree` is the plan's tree (queries and kNN plans
+    take it; it is freed with the plan)."""
+
+    def __init__(self, points, offsets, max_elements, boxes, max_depths=None, *, ctx=None, dim: int = 3):
+        self.ctx = ctx or default_context()
+        self.handle = None
+        if not _is_device(points) or not points.is_contiguous():
+            raise ValueError("a build plan takes a contiguous device tensor of points")
+        offs = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+        ns = len(offs) - 1
+        caps = np.ascontiguousarray(np.broadcast_to(np.asarray(max_elements, dtype=np.uint32), (ns,)))
+        deps = None if max_depths is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(max_depths, dtype=np.uint32), (ns,)))
+        bxs = _f64(boxes, 2 * dim)
+        h = C.c_void_p()
+        st = _abi.lib().otg_build_plan_create(self.ctx.handle, dim, ns, offs.ctypes.data_as(_abi.U64P),
+                                              C.c_void_p(points.data_ptr()), _ptr(bxs),
+                                              None if deps is None else deps.ctypes.data_as(_abi.U32P),
+                                              caps.ctypes.data_as(_abi.U32P), C.byref(h))
+        _check(st, "otg_build_plan_create", self.ctx)
+        self.handle = h
+        self._points = points  # the graph reads this tensor's memory
+        th = C.c_void_p()
+        _check(_abi.lib().otg_build_plan_tree(h, C.byref(th)), "otg_build_plan_tree", self.ctx)
+        t = StaticTree.__new__(StaticTree)
+        t.ctx, t.kind, t.dim, t.handle, t.n, t._borrowed = self.ctx, "point", dim, th, int(offs[-1]), True
+        self.tree = t
+        self._steps = []
+
+    def run(self) -> "BuildPlan":
+        _check(_abi.lib().otg_build_plan_run(self.handle), "otg_build_plan_run", self.ctx)
+        return self
+
+    def step(self, knn_plan: KnnPlan) -> "StepPlan":
+        """The build and a kNN plan made on self.tree, replayed as one graph."""
+        h = C.c_void_p()
+        _check(_abi.lib().otg_step_plan_create(self.handle, knn_plan.handle, C.byref(h)), "otg_step_plan_create",
+               self.ctx)
+        sp = StepPlan(h, self, knn_plan)
+        self._steps.append(sp)
+        return sp
+
+    def free(self):
+        if getattr(self, "handle", None):
+            for sp in self._steps:
+                sp.free()
+            self.tree.free()
+            _abi.lib().otg_build_plan_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class StepPlan:
+    """A build plan and a kNN plan on its tree as one graph (otg_step_plan); after run()
+    the tree is current and the kNN plan's csr() holds the batch's ids."""
+
+    def __init__(self, handle, build: BuildPlan, knn: KnnPlan):
+        self.handle, self.build, self.knn = handle, build, knn
+
+    def run(self) -> "StepPlan":
+        _check(_abi.lib().otg_step_plan_run(self.handle), "otg_step_plan_run", self.build.ctx)
+        return self
+
+    def free(self):
+        if getattr(self, "handle", None):
+            _abi.lib().otg_step_plan_free(self.handle)
+            self.handle = None
+
+
 def _is_device(a) -> bool:
     """Objects exposing data_ptr() (e.g. torch device tensors) are passed as device memory."""
     return hasattr(a, "data_ptr") and getattr(getattr(a, "device", None), "type", "cpu") != "cpu"
@@ -340,7 +417,8 @@ class StaticTree:
         if getattr(self, "handle", None):
             for plan in list(getattr(self, "_plans", ())):
                 plan.free()
-            _abi.lib().otg_tree_free(self.handle)
+            if not getattr(self, "_borrowed", False):  # a build plan's tree is freed with its plan
+                _abi.lib().otg_tree_free(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -771,7 +849,7 @@ def StaticOrthoTreeBoxND(dim: int, loose: bool = True):
 
 
 __all__ = [
-    "Context", "Result", "KnnPlan", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
+    "Context", "Result", "KnnPlan", "BuildPlan", "StepPlan", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
     "StaticQuadtreePoint", "StaticQuadtreeBox", "StaticOctreePoint", "StaticOctreeBox", "StaticHexatreePoint",
     "StaticHexatreeBox", "StaticOrthoTreePointND", "StaticOrthoTreeBoxND", "default_context",
 ]

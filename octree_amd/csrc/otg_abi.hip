@@ -125,14 +125,15 @@ Tree::~Tree() {
   delete knn_tree;
   if (!ctx) return;
   ctx->free(d_view);
+  ctx->free(d_ids_ref);
+  ctx->free(d_geom_ref);
+  ctx->free(d_view_ref);
+  if (plan_arrays) return;
   ctx->free(d_sets);
   if (!shares_entities) {
     ctx->free(d_ids);
     ctx->free(d_geom);
   }
-  ctx->free(d_ids_ref);
-  ctx->free(d_geom_ref);
-  ctx->free(d_view_ref);
   ctx->free(d_keys);
   ctx->free(d_info);
   ctx->free(d_rend);
@@ -636,6 +637,76 @@ OTG_API otg_status otg_knn_plan_run(otg_knn_plan *plan) {
     OTG_HIP_CHECK(hipSetDevice(ctx->device));
     otg::knn_plan_run(p);
   });
+}
+
+OTG_API otg_status otg_build_plan_create(otg_ctx *ctx, int dim, uint32_t nsets, const uint64_t *set_offsets,
+                                         const double *points, const double *boxes, const uint32_t *max_depths_or_null,
+                                         const uint32_t *max_elements, otg_build_plan **out) {
+  if (!ctx || !out || !set_offsets || !max_elements || !points || !boxes || nsets == 0) return OTG_ERR_INVALID_ARG;
+  if (dim < 2 || dim > otg::kMaxDim) return OTG_ERR_INVALID_ARG;
+  for (uint32_t s = 0; s < nsets; ++s)
+    if (set_offsets[s + 1] < set_offsets[s]) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = reinterpret_cast<otg_build_plan *>(
+        otg::build_plan_create(ctx, dim, points, set_offsets, nsets, boxes, max_depths_or_null, max_elements));
+  });
+}
+
+OTG_API otg_status otg_build_plan_run(otg_build_plan *plan) {
+  if (!plan) return OTG_ERR_INVALID_ARG;
+  auto *p = reinterpret_cast<otg::BuildPlan *>(plan);
+  Ctx *ctx = otg::build_plan_tree(p)->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::build_plan_run(p);
+  });
+}
+
+OTG_API otg_status otg_build_plan_tree(const otg_build_plan *plan, const otg_tree **tree) {
+  if (!plan || !tree) return OTG_ERR_INVALID_ARG;
+  *tree = static_cast<const otg_tree *>(otg::build_plan_tree(reinterpret_cast<const otg::BuildPlan *>(plan)));
+  return OTG_OK;
+}
+
+OTG_API void otg_build_plan_free(otg_build_plan *plan) {
+  if (!plan) return;
+  auto *p = reinterpret_cast<otg::BuildPlan *>(plan);
+  Ctx *ctx = otg::build_plan_tree(p)->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  otg::build_plan_free(p);
+}
+
+OTG_API otg_status otg_step_plan_create(otg_build_plan *build, otg_knn_plan *knn, otg_step_plan **out) {
+  if (!build || !knn || !out) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  auto *bp = reinterpret_cast<otg::BuildPlan *>(build);
+  Ctx *ctx = otg::build_plan_tree(bp)->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = reinterpret_cast<otg_step_plan *>(otg::step_plan_create(bp, reinterpret_cast<otg::KnnPlan *>(knn)));
+  });
+}
+
+OTG_API otg_status otg_step_plan_run(otg_step_plan *step) {
+  if (!step) return OTG_ERR_INVALID_ARG;
+  auto *st = reinterpret_cast<otg::StepPlan *>(step);
+  Ctx *ctx = otg::step_plan_ctx(st);
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::step_plan_run(st);
+  });
+}
+
+OTG_API void otg_step_plan_free(otg_step_plan *step) {
+  if (!step) return;
+  auto *st = reinterpret_cast<otg::StepPlan *>(step);
+  Ctx *ctx = otg::step_plan_ctx(st);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  otg::step_plan_free(st);
 }
 
 OTG_API otg_status otg_knn_plan_result(const otg_knn_plan *plan, const uint64_t **offsets, const uint32_t **ids,

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "otg_device.hpp"
@@ -479,22 +480,23 @@ __global__ void __launch_bounds__(kB) k_event_flags(const uint8_t *__restrict__ 
   ev[p] = (uint8_t)(((set_start || active[p]) ? 1u : 0u) | (ic << 1));
 }
 
+// nseg: the segment count (device); seg_list holds cap + 1 entries
 __global__ void __launch_bounds__(kB) k_seg_compact(const uint8_t *__restrict__ ev, const uint64_t *__restrict__ idx,
-                                                     uint64_t n, uint32_t *__restrict__ seg_list, uint32_t nseg) {
+                                                     uint64_t n, uint32_t *__restrict__ seg_list, DevN nseg) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (p < n && (ev[p] & 1u)) seg_list[idx[p] >> 32] = (uint32_t)p;
-  if (p == 0) seg_list[nseg] = (uint32_t)n;
+  if (p < n && (ev[p] & 1u) && (idx[p] >> 32) < nseg.cap) seg_list[idx[p] >> 32] = (uint32_t)p;
+  if (p == 0) seg_list[nseg.get()] = (uint32_t)n;
 }
 
 // Which segments are nodes (leaves): all except single non-refinable boxes.
 template <bool BOX>
-__global__ void __launch_bounds__(kB) k_seg_nodes(const uint32_t *__restrict__ seg_list, uint32_t nseg,
+__global__ void __launch_bounds__(kB) k_seg_nodes(const uint32_t *__restrict__ seg_list, DevN dseg,
                                                    const uint8_t *__restrict__ v, const uint64_t *__restrict__ keys,
                                                    const uint64_t *__restrict__ set_off, uint32_t nsets,
                                                    const SetParams *__restrict__ sets, KeyView kv,
                                                    uint32_t *__restrict__ seg_node) {
   uint32_t k = blockIdx.x * kB + threadIdx.x;
-  if (k >= nseg) return;
+  if (k >= dseg.get()) return;
   uint32_t node = 1;
   if (BOX) {
     uint64_t s = seg_list[k], e = seg_list[k + 1];
@@ -525,9 +527,11 @@ __global__ void __launch_bounds__(kB) k_emit_internal(const uint8_t *__restrict_
                                                        const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
                                                        uint64_t n, const uint64_t *__restrict__ keys, KeyView kv,
                                                        const SetParams *__restrict__ sets, NodeKeyLayout nk,
-                                                       uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
+                                                       uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc,
+                                                       uint64_t cap) {
   uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (p >= n || !(ev[p] >> 1)) return;
+  if ((uint64_t)(uint32_t)idx[p] + (ev[p] >> 1) > cap) return;  // beyond the capacity (flagged by the caller)
   const uint32_t s = kv.set(keys[p]);
   const SetParams &sp = sets[s];
   uint32_t out = (uint32_t)idx[p];
@@ -544,15 +548,17 @@ __global__ void __launch_bounds__(kB) k_emit_internal(const uint8_t *__restrict_
   }
 }
 
-__global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__ seg_list, uint32_t nseg,
+__global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__ seg_list, DevN dseg,
                                                      const uint32_t *__restrict__ seg_node, const uint32_t *__restrict__ seg_node_idx,
-                                                     uint32_t out_base, const uint8_t *__restrict__ v,
+                                                     const uint32_t *__restrict__ nint, uint64_t cap, const uint8_t *__restrict__ v,
                                                      const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
                                                      const uint64_t *__restrict__ keys, KeyView kv,
                                                      const SetParams *__restrict__ sets, NodeKeyLayout nk,
                                                      uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc) {
   uint32_t k = blockIdx.x * kB + threadIdx.x;
-  if (k >= nseg || !seg_node[k]) return;
+  if (k >= dseg.get() || !seg_node[k]) return;
+  const uint64_t out = (uint64_t)*nint + seg_node_idx[k];
+  if (out >= cap) return;  // beyond the capacity (flagged by the caller)
   uint64_t s0 = seg_list[k], e = seg_list[k + 1];
   const uint32_t s = kv.set(keys[s0]);
   const SetParams &sp = sets[s];
@@ -567,16 +573,32 @@ __global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__
     uint64_t pb = (vs >= ve) ? s0 : e;
     gen = internal_gen(lm, rm, v, pb) + 1u;
   }
-  uint32_t out = out_base + seg_node_idx[k];
   nkeys[out] = nk.make(s, gen, 2 * (s0 - sp.ent_off));
   ndesc[out] = (kTypeLeaf << 30) | k;
 }
 
+// counts[3] = nodes = internal nodes (counts[1]) + leaves (counts[2]).  A count beyond
+// its capacity sets *overflow and the node count to 0: the table is then empty (every
+// set without nodes, so no walk reads it) instead of partly written.
+__global__ void k_node_total(uint32_t *__restrict__ counts, uint64_t cap_seg, uint64_t cap_nodes,
+                             uint32_t *__restrict__ overflow) {
+  const uint64_t nodes = (uint64_t)counts[1] + counts[2];
+  const bool over = counts[0] > cap_seg || nodes > cap_nodes;
+  counts[3] = over ? 0u : (uint32_t)nodes;
+  if (over && overflow) *overflow = 1u;
+}
+
+// *overflow when *count > cap
+__global__ void k_check_cap(const uint32_t *__restrict__ count, uint64_t cap, uint32_t *__restrict__ overflow) {
+  if (*count > cap) *overflow = 1u;
+}
+
 // node offsets per set: first node whose key has set id >= s
-__global__ void k_set_node_offsets(const uint64_t *__restrict__ nkeys, uint64_t nnodes, SetParams *__restrict__ sets,
+__global__ void k_set_node_offsets(const uint64_t *__restrict__ nkeys, DevN dnodes, SetParams *__restrict__ sets,
                                    uint32_t nsets, NodeKeyLayout nk) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsets) return;
+  const uint64_t nnodes = dnodes.get();
   uint64_t lo = lower_bound_u64(nkeys, 0, nnodes, nk.make(s, 0, 0));
   uint64_t hi = (s + 1 == nsets) ? nnodes : lower_bound_u64(nkeys, 0, nnodes, nk.make(s + 1, 0, 0));
   sets[s].node_off = lo;
@@ -610,13 +632,14 @@ __device__ __forceinline__ void point_node_geometry(uint64_t nkey, uint32_t dept
 
 template <int D, bool BOX>
 __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nkeys, const uint32_t *__restrict__ ndesc,
-                                                  uint64_t nnodes, const uint64_t *__restrict__ keys,
+                                                  DevN dnodes, const uint64_t *__restrict__ keys,
                                                   const uint8_t *__restrict__ v, const uint32_t *__restrict__ seg_list,
                                                   const SetParams *__restrict__ sets, KeyView kv, NodeKeyLayout nk,
                                                   NodeInfo *__restrict__ info, uint32_t *__restrict__ rend,
                                                   uint8_t *__restrict__ ndepth, uint64_t *__restrict__ nodekey,
                                                   double *__restrict__ nmin, double *__restrict__ nsize) {
   uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
+  const uint64_t nnodes = dnodes.get();
   if (x >= nnodes) return;
   const uint64_t key = nkeys[x];
   const uint32_t s = (uint32_t)(key >> (nk.gbits + nk.cobits));
@@ -709,10 +732,10 @@ __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nk
 // (set, generation, position), so the reference numbering is the finer numbering
 // restricted to the kept nodes: an exclusive scan of the keep flags.
 __global__ void __launch_bounds__(kB) k_ref_mark(const NodeInfo *__restrict__ finfo, const uint8_t *__restrict__ fdepth,
-                                                  uint64_t nf, const uint64_t *__restrict__ fkeys, KeyView fkv,
+                                                  DevN nf, const uint64_t *__restrict__ fkeys, KeyView fkv,
                                                   const SetParams *__restrict__ rsets, uint32_t *__restrict__ keep) {
   const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (x >= nf) return;
+  if (x >= nf.get()) return;
   const NodeInfo ni = finfo[x];
   if (!ni.ccount) return;
   if (fdepth[x] >= rsets[fkv.set(fkeys[ni.ebegin])].max_depth) return;
@@ -727,15 +750,16 @@ __global__ void k_ref_roots(const SetParams *__restrict__ fsets, uint32_t nsets,
 template <int D>
 __global__ void __launch_bounds__(kB) k_ref_emit(const NodeInfo *__restrict__ finfo, const uint32_t *__restrict__ frend,
                                                   const uint8_t *__restrict__ fdepth, const uint64_t *__restrict__ fnkey,
-                                                  uint64_t nf, const uint64_t *__restrict__ fkeys, KeyView fkv,
+                                                  DevN nf, const uint64_t *__restrict__ fkeys, KeyView fkv,
                                                   const SetParams *__restrict__ rsets, const uint32_t *__restrict__ keep,
-                                                  const uint32_t *__restrict__ ridx, NodeInfo *__restrict__ info,
+                                                  const uint32_t *__restrict__ ridx, uint64_t cap, NodeInfo *__restrict__ info,
                                                   uint32_t *__restrict__ rend, uint8_t *__restrict__ ndepth,
                                                   uint64_t *__restrict__ nodekey, double *__restrict__ nmin,
                                                   double *__restrict__ nsize) {
   const uint64_t x = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (x >= nf || !keep[x]) return;
+  if (x >= nf.get() || !keep[x]) return;
   const uint64_t r = ridx[x];
+  if (r >= cap) return;  // beyond the capacity (flagged by the caller)
   const NodeInfo fi = finfo[x];
   const SetParams &sp = rsets[fkv.set(fkeys[fi.ebegin])];
   const uint32_t df = fdepth[x];
@@ -755,16 +779,18 @@ __global__ void __launch_bounds__(kB) k_ref_emit(const NodeInfo *__restrict__ fi
 }
 
 __global__ void k_ref_sets(const SetParams *__restrict__ fsets, SetParams *__restrict__ rsets, uint32_t nsets,
-                           uint64_t nf, const uint32_t *__restrict__ ridx, const uint32_t *__restrict__ total,
-                           const NodeInfo *__restrict__ info) {
+                           DevN dnf, const uint32_t *__restrict__ ridx, const uint32_t *__restrict__ total,
+                           uint64_t cap, const NodeInfo *__restrict__ info) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsets) return;
+  const uint64_t nf = dnf.get();
   const uint64_t b = fsets[s].node_off, e = b + fsets[s].node_cnt;
   const uint64_t rb = (b == nf) ? *total : ridx[b];
   const uint64_t re = (e == nf) ? *total : ridx[e];
-  rsets[s].node_off = rb;
-  rsets[s].node_cnt = re - rb;
-  rsets[s].root_splits = (re > rb && info[rb].ccount) ? 1u : 0u;
+  const bool over = re > cap;  // beyond the capacity (flagged by the caller): the set gets no nodes
+  rsets[s].node_off = over ? 0 : rb;
+  rsets[s].node_cnt = over ? 0 : re - rb;
+  rsets[s].root_splits = (!over && re > rb && info[rb].ccount) ? 1u : 0u;
 }
 
 // sorted SoA geometry and local ids
@@ -978,6 +1004,22 @@ static void prepare_sets(Tree *t, const double *dgeom, const uint64_t *offsets, 
   set_layout<D, BOX>(t);
 }
 
+// Memory of a build: the context allocator, or (a build captured into a graph) the
+// plan's arena, whose blocks stay with the plan: nothing is allocated or freed while
+// capturing, and the tree's arrays keep their addresses on every replay.
+static void *build_alloc(Ctx *ctx, Arena *ar, size_t bytes) {
+  return ar ? ar->take(std::max<size_t>(bytes, 1)) : ctx->alloc(bytes);
+}
+
+// Counts of one node table.  A plain build reads them back and sizes its buffers
+// exactly; a captured build sizes them by capacities fixed when its plan was made,
+// keeps the counts on the device and flags one beyond its capacity.
+struct TableSizes {
+  uint32_t *counts = nullptr;    // device: [0] segments, [1] internal nodes, [2] leaves, [3] nodes
+  uint64_t cap_seg = 0, cap_nodes = 0;
+  uint32_t *overflow = nullptr;  // device flag
+};
+
 static void upload_sets(Tree *t) {
   Ctx *ctx = t->ctx;
   const size_t nsets = t->sets.size();
@@ -988,14 +1030,15 @@ static void upload_sets(Tree *t) {
 // 1-2. location keys of every entity, sorted (t->d_keys); vals = input positions in
 // that order; encode error bits land in *err (device).
 template <int D, bool BOX>
-static void encode_sort(Tree *t, const double *dgeom, const uint64_t *d_off, uint32_t *vals, uint32_t *err) {
+static void encode_sort(Tree *t, const double *dgeom, const uint64_t *d_off, uint32_t *vals, uint32_t *err,
+                        Arena *ar = nullptr) {
   Ctx *ctx = t->ctx;
   const uint64_t n = t->n;
   const KeyView kv{t->layout.dbits, t->layout.cbits};
-  t->d_keys = (uint64_t *)ctx->alloc(n * 8);
+  t->d_keys = (uint64_t *)build_alloc(ctx, ar, n * 8);
   launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, d_off,
          (uint32_t)t->sets.size(), (const SetParams *)t->d_sets, kv, t->d_keys, vals, err);
-  radix_sort_pairs(ctx, t->d_keys, vals, n, t->layout.total);
+  radix_sort_pairs(ctx, t->d_keys, vals, n, t->layout.total, ar);
 }
 
 // Keys of the reference tree from the sorted keys of the finer tree over the same
@@ -1014,9 +1057,11 @@ __global__ void __launch_bounds__(kB) k_coarse_keys(const uint64_t *__restrict__
 }
 
 // 3-7. node table from the sorted keys t->d_keys.  herr: host copy of the encode
-// error bits, whose copy was queued before; checked after the first synchronisation.
+// error bits, whose copy was queued before; checked after the first synchronisation
+// (a plain build).  ts: the capacities and device counts of a captured build.
 template <int D, bool BOX>
-static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
+static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Arena *ar = nullptr,
+                       const TableSizes *ts = nullptr) {
   Ctx *ctx = t->ctx;
   const uint64_t n = t->n;
   const uint32_t nsets = (uint32_t)t->sets.size();
@@ -1026,14 +1071,14 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
   for (const SetParams &sp : t->sets) max_n = std::max<uint64_t>(max_n, sp.ent_cnt);
 
   // 3. levels (+ per-set minimum level), root flags
-  Buf<uint8_t> v(ctx, n + 9);  // levels v[0..n]; 8 bytes of padding for word-wise scans
+  Buf<uint8_t> v(ctx, ar, n + 9);  // levels v[0..n]; 8 bytes of padding for word-wise scans
   launch(ctx, "levels", k_levels<D>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys, n, d_off, nsets,
          t->d_sets, kv, v.p);
   launch(ctx, "root_flags", k_root_flags<BOX>, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)t->d_keys,
          t->d_sets, nsets, kv);
 
   // 4. active boundaries
-  Buf<uint8_t> active(ctx, n + 1);
+  Buf<uint8_t> active(ctx, ar, n + 1);
   uint64_t max_cap = 0;
   for (const SetParams &sp : t->sets) max_cap = std::max<uint64_t>(max_cap, sp.cap);
   if (max_cap + 8 < kActHalo)
@@ -1044,63 +1089,83 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
            (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
 
   // 5. level scans
-  Buf<uint32_t> lm(ctx, n + 1), rm(ctx, n + 1);
-  level_scans(ctx, v.p, n + 1, lm.p, rm.p);
+  Buf<uint32_t> lm(ctx, ar, n + 1), rm(ctx, ar, n + 1);
+  level_scans(ctx, v.p, n + 1, lm.p, rm.p, ar);
 
   // 6. node events
-  Buf<uint8_t> ev(ctx, n);
-  Buf<uint64_t> eidx(ctx, n);
+  Buf<uint8_t> ev(ctx, ar, n);
+  Buf<uint64_t> eidx(ctx, ar, n);
   launch(ctx, "event_flags", k_event_flags, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)v.p,
          (const uint8_t *)active.p, (const uint32_t *)lm.p, n, (const uint64_t *)t->d_keys, kv,
          (const SetParams *)t->d_sets, ev.p);
-  Buf<uint32_t> totals(ctx, 4);
-  event_scan(ctx, ev.p, eidx.p, n, totals.p);
-  uint32_t ht[2];
-  OTG_HIP_CHECK(hipMemcpyAsync(ht, totals.p, 8, hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
-  if (herr && (*herr & kErrOutOfSpace)) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
-  if (herr && (*herr & kErrUnsupported)) throw Error{OTG_ERR_UNSUPPORTED, "box level or grid id beyond the tree depth"};
-  const uint32_t nseg = ht[0], nint = ht[1];
+  Buf<uint32_t> own_counts;
+  uint32_t *counts = ts ? ts->counts : nullptr;
+  if (!counts) {
+    own_counts = Buf<uint32_t>(ctx, ar, 4);
+    counts = own_counts.p;
+  }
+  event_scan(ctx, ev.p, eidx.p, n, counts, ar);
+  uint64_t cap_seg, cap_nodes;
+  if (!ts) {
+    uint32_t ht[2];
+    OTG_HIP_CHECK(hipMemcpyAsync(ht, counts, 8, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (herr && (*herr & kErrOutOfSpace)) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
+    if (herr && (*herr & kErrUnsupported)) throw Error{OTG_ERR_UNSUPPORTED, "box level or grid id beyond the tree depth"};
+    cap_seg = ht[0];
+    cap_nodes = (uint64_t)ht[0] + ht[1];  // refined below
+    t->seg_count = cap_seg;
+  } else {
+    cap_seg = ts->cap_seg;
+    cap_nodes = ts->cap_nodes;
+  }
+  const DevN dseg{cap_seg, counts};
 
-  Buf<uint32_t> seg_list(ctx, (size_t)nseg + 1), seg_node(ctx, nseg), seg_node_idx(ctx, nseg);
+  Buf<uint32_t> seg_list(ctx, ar, cap_seg + 1), seg_node(ctx, ar, cap_seg), seg_node_idx(ctx, ar, cap_seg);
   launch(ctx, "seg_compact", k_seg_compact, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)ev.p,
-         (const uint64_t *)eidx.p, n, seg_list.p, nseg);
-  launch(ctx, "seg_nodes", k_seg_nodes<BOX>, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
-         (const uint8_t *)v.p, (const uint64_t *)t->d_keys, d_off, nsets, (const SetParams *)t->d_sets, kv, seg_node.p);
-  exclusive_scan_u32(ctx, seg_node.p, seg_node_idx.p, nseg, totals.p + 2);
-  uint32_t nleaf = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&nleaf, totals.p + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
-  const uint64_t nnodes = (uint64_t)nint + nleaf;
-  t->node_count = nnodes;
+         (const uint64_t *)eidx.p, n, seg_list.p, dseg);
+  launch(ctx, "seg_nodes", k_seg_nodes<BOX>, dim3(div_up(cap_seg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p,
+         dseg, (const uint8_t *)v.p, (const uint64_t *)t->d_keys, d_off, nsets, (const SetParams *)t->d_sets, kv,
+         seg_node.p);
+  exclusive_scan_u32(ctx, seg_node.p, seg_node_idx.p, dseg, counts + 2, ar);
+  if (!ts) {
+    uint32_t h[3];
+    OTG_HIP_CHECK(hipMemcpyAsync(h, counts, 12, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    cap_nodes = (uint64_t)h[1] + h[2];
+    t->node_count = cap_nodes;
+  }
+  launch(ctx, "node_total", k_node_total, dim3(1), dim3(1), 0, counts, cap_seg, cap_nodes,
+         ts ? ts->overflow : (uint32_t *)nullptr);
+  const DevN dnodes{cap_nodes, counts + 3};
 
   NodeKeyLayout nk;
   nk.cobits = 64 - __builtin_clzll(2 * max_n + 1);
   nk.gbits = 5;
   if (nk.cobits + nk.gbits + L.sbits > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "node key overflow"};
-  Buf<uint64_t> nkeys(ctx, nnodes);
-  Buf<uint32_t> ndesc(ctx, nnodes);
+  Buf<uint64_t> nkeys(ctx, ar, cap_nodes);
+  Buf<uint32_t> ndesc(ctx, ar, cap_nodes);
   launch(ctx, "emit_internal", k_emit_internal, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)ev.p,
          (const uint64_t *)eidx.p, (const uint8_t *)v.p, (const uint8_t *)active.p, (const uint32_t *)lm.p,
          (const uint32_t *)rm.p, n, (const uint64_t *)t->d_keys, kv, (const SetParams *)t->d_sets, nk, nkeys.p,
-         ndesc.p);
-  launch(ctx, "emit_leaves", k_emit_leaves, dim3(div_up(nseg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, nseg,
-         (const uint32_t *)seg_node.p, (const uint32_t *)seg_node_idx.p, nint, (const uint8_t *)v.p,
-         (const uint32_t *)lm.p, (const uint32_t *)rm.p, (const uint64_t *)t->d_keys, kv,
+         ndesc.p, cap_nodes);
+  launch(ctx, "emit_leaves", k_emit_leaves, dim3(div_up(cap_seg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, dseg,
+         (const uint32_t *)seg_node.p, (const uint32_t *)seg_node_idx.p, (const uint32_t *)(counts + 1), cap_nodes,
+         (const uint8_t *)v.p, (const uint32_t *)lm.p, (const uint32_t *)rm.p, (const uint64_t *)t->d_keys, kv,
          (const SetParams *)t->d_sets, nk, nkeys.p, ndesc.p);
-  radix_sort_pairs(ctx, nkeys.p, ndesc.p, nnodes, nk.cobits + nk.gbits + L.sbits);
+  radix_sort_pairs(ctx, nkeys.p, ndesc.p, dnodes, nk.cobits + nk.gbits + L.sbits, ar);
   launch(ctx, "set_node_offsets", k_set_node_offsets, dim3(div_up(nsets, 64)), dim3(64), 0, (const uint64_t *)nkeys.p,
-         nnodes, t->d_sets, nsets, nk);
+         dnodes, t->d_sets, nsets, nk);
 
   // 7. finalize
-  t->d_info = (NodeInfo *)ctx->alloc(nnodes * sizeof(NodeInfo));
-  t->d_rend = (uint32_t *)ctx->alloc(nnodes * 4);
-  t->d_depth = (uint8_t *)ctx->alloc(nnodes);
-  t->d_nkey = (uint64_t *)ctx->alloc(nnodes * 8);
-  t->d_nmin = (double *)ctx->alloc(nnodes * D * 8);
-  t->d_nsize = (double *)ctx->alloc(nnodes * D * 8);
-  launch(ctx, "finalize", k_finalize<D, BOX>, dim3(div_up(nnodes, kB)), dim3(kB), 0, (const uint64_t *)nkeys.p,
-         (const uint32_t *)ndesc.p, nnodes, (const uint64_t *)t->d_keys, (const uint8_t *)v.p,
+  t->d_info = (NodeInfo *)build_alloc(ctx, ar, cap_nodes * sizeof(NodeInfo));
+  t->d_rend = (uint32_t *)build_alloc(ctx, ar, cap_nodes * 4);
+  t->d_depth = (uint8_t *)build_alloc(ctx, ar, cap_nodes);
+  t->d_nkey = (uint64_t *)build_alloc(ctx, ar, cap_nodes * 8);
+  t->d_nmin = (double *)build_alloc(ctx, ar, cap_nodes * D * 8);
+  t->d_nsize = (double *)build_alloc(ctx, ar, cap_nodes * D * 8);
+  launch(ctx, "finalize", k_finalize<D, BOX>, dim3(div_up(cap_nodes, kB)), dim3(kB), 0, (const uint64_t *)nkeys.p,
+         (const uint32_t *)ndesc.p, dnodes, (const uint64_t *)t->d_keys, (const uint8_t *)v.p,
          (const uint32_t *)seg_list.p, (const SetParams *)t->d_sets, kv, nk, t->d_info, t->d_rend, t->d_depth,
          t->d_nkey, t->d_nmin, t->d_nsize);
 }
@@ -1108,45 +1173,59 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr) {
 // Node table of a point tree from its finer index (k_ref_* above); every set of both
 // trees has the same capacity.
 template <int D>
-static void reference_from_finer(Tree *t, const Tree *aux) {
+static void reference_from_finer(Tree *t, const Tree *aux, Arena *ar = nullptr, const TableSizes *fts = nullptr,
+                                 const TableSizes *rts = nullptr) {
   Ctx *ctx = t->ctx;
-  const uint64_t nf = aux->node_count;
+  const DevN nf = fts ? DevN{fts->cap_nodes, fts->counts + 3} : DevN{aux->node_count, nullptr};
   const uint32_t nsets = (uint32_t)t->sets.size();
   const KeyView fkv{aux->layout.dbits, aux->layout.cbits};
-  Buf<uint32_t> keep(ctx, nf), ridx(ctx, nf), total(ctx, 1);
-  fill_u32(ctx, keep.p, nf, 0u);
-  launch(ctx, "ref_mark", k_ref_mark, dim3(div_up(nf, kB)), dim3(kB), 0, (const NodeInfo *)aux->d_info,
+  Buf<uint32_t> keep(ctx, ar, nf.cap), ridx(ctx, ar, nf.cap), own_total;
+  uint32_t *total = rts ? rts->counts + 3 : nullptr;
+  if (!total) {
+    own_total = Buf<uint32_t>(ctx, ar, 1);
+    total = own_total.p;
+  }
+  fill_u32(ctx, keep.p, nf.cap, 0u);
+  launch(ctx, "ref_mark", k_ref_mark, dim3(div_up(nf.cap, kB)), dim3(kB), 0, (const NodeInfo *)aux->d_info,
          (const uint8_t *)aux->d_depth, nf, (const uint64_t *)aux->d_keys, fkv, (const SetParams *)t->d_sets, keep.p);
   launch(ctx, "ref_roots", k_ref_roots, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)aux->d_sets, nsets,
          keep.p);
-  exclusive_scan_u32(ctx, keep.p, ridx.p, nf, total.p);
-  uint32_t nr = 0;
-  OTG_HIP_CHECK(hipMemcpyAsync(&nr, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
-  t->node_count = nr;
-  t->d_info = (NodeInfo *)ctx->alloc(std::max<uint64_t>(nr, 1) * sizeof(NodeInfo));
-  t->d_rend = (uint32_t *)ctx->alloc(std::max<uint64_t>(nr, 1) * 4);
-  t->d_depth = (uint8_t *)ctx->alloc(std::max<uint64_t>(nr, 1));
-  t->d_nkey = (uint64_t *)ctx->alloc(std::max<uint64_t>(nr, 1) * 8);
-  t->d_nmin = (double *)ctx->alloc(std::max<uint64_t>(nr, 1) * D * 8);
-  t->d_nsize = (double *)ctx->alloc(std::max<uint64_t>(nr, 1) * D * 8);
-  launch(ctx, "ref_emit", k_ref_emit<D>, dim3(div_up(nf, kB)), dim3(kB), 0, (const NodeInfo *)aux->d_info,
+  exclusive_scan_u32(ctx, keep.p, ridx.p, nf, total, ar);
+  uint64_t cap;
+  if (!rts) {
+    uint32_t nr = 0;
+    OTG_HIP_CHECK(hipMemcpyAsync(&nr, total, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    cap = nr;
+    t->node_count = nr;
+  } else {
+    cap = rts->cap_nodes;
+    launch(ctx, "check_cap", k_check_cap, dim3(1), dim3(1), 0, (const uint32_t *)total, cap, rts->overflow);
+  }
+  t->d_info = (NodeInfo *)build_alloc(ctx, ar, std::max<uint64_t>(cap, 1) * sizeof(NodeInfo));
+  t->d_rend = (uint32_t *)build_alloc(ctx, ar, std::max<uint64_t>(cap, 1) * 4);
+  t->d_depth = (uint8_t *)build_alloc(ctx, ar, std::max<uint64_t>(cap, 1));
+  t->d_nkey = (uint64_t *)build_alloc(ctx, ar, std::max<uint64_t>(cap, 1) * 8);
+  t->d_nmin = (double *)build_alloc(ctx, ar, std::max<uint64_t>(cap, 1) * D * 8);
+  t->d_nsize = (double *)build_alloc(ctx, ar, std::max<uint64_t>(cap, 1) * D * 8);
+  launch(ctx, "ref_emit", k_ref_emit<D>, dim3(div_up(nf.cap, kB)), dim3(kB), 0, (const NodeInfo *)aux->d_info,
          (const uint32_t *)aux->d_rend, (const uint8_t *)aux->d_depth, (const uint64_t *)aux->d_nkey, nf,
          (const uint64_t *)aux->d_keys, fkv, (const SetParams *)t->d_sets, (const uint32_t *)keep.p,
-         (const uint32_t *)ridx.p, t->d_info, t->d_rend, t->d_depth, t->d_nkey, t->d_nmin, t->d_nsize);
+         (const uint32_t *)ridx.p, cap, t->d_info, t->d_rend, t->d_depth, t->d_nkey, t->d_nmin, t->d_nsize);
   launch(ctx, "ref_sets", k_ref_sets, dim3(div_up(nsets, 64)), dim3(64), 0, (const SetParams *)aux->d_sets, t->d_sets,
-         nsets, nf, (const uint32_t *)ridx.p, (const uint32_t *)total.p, (const NodeInfo *)t->d_info);
+         nsets, nf, (const uint32_t *)ridx.p, (const uint32_t *)total, cap, (const NodeInfo *)t->d_info);
 }
 
 // Sorted SoA geometry and local ids of the entity storage.
 template <int D, bool BOX>
-static void gather_entities(Tree *t, const double *dgeom, const uint32_t *vals, const uint64_t *d_off) {
+static void gather_entities(Tree *t, const double *dgeom, const uint32_t *vals, const uint64_t *d_off,
+                            Arena *ar = nullptr) {
   Ctx *ctx = t->ctx;
   constexpr int S = BOX ? 2 * D : D;
   const uint64_t n = t->n;
   const KeyView kv{t->layout.dbits, t->layout.cbits};
-  t->d_geom = (double *)ctx->alloc(n * S * 8);
-  t->d_ids = (uint32_t *)ctx->alloc(n * 4);
+  t->d_geom = (double *)build_alloc(ctx, ar, n * S * 8);
+  t->d_ids = (uint32_t *)build_alloc(ctx, ar, n * 4);
   launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, vals, n,
          (const uint64_t *)t->d_keys, kv, d_off, t->d_geom, t->d_ids);
 }
@@ -2214,6 +2293,248 @@ void load_static_archive(Tree *t, const uint8_t *buf, uint64_t size, bool large,
   }
   publish_views(t);
 }
+
+// ---------------------------------------------------------------------------
+// build plan: a point-forest build captured once as a HIP graph and replayed on new
+// contents of the same device geometry (same sets, boxes, depths and capacities)
+// ---------------------------------------------------------------------------
+// Words the captured build leaves on the device (and copies to pinned memory last):
+// the counts of each node table, a capacity overflow flag and the encode error bits.
+constexpr int kPlanRefCounts = 0, kPlanFineCounts = 4, kPlanOverflow = 8, kPlanErr = 9, kPlanWords = 16;
+
+struct BuildPlan {
+  Ctx *ctx = nullptr;
+  std::unique_ptr<::otg_tree> t;  // the plan's tree (an ABI handle); its arrays live in `mem`
+  const double *geom = nullptr;
+  uint64_t n = 0;
+  int dim = 0;
+  bool fine = false;        // a finer kNN/range index sorted together with the tree
+  char *mem = nullptr;      // every device buffer of the captured build
+  TableSizes ref, fine_sz;  // capacities and device counts of the two node tables
+  uint64_t cap_ref_nodes = 0;
+  uint32_t *d_words = nullptr;
+  uint32_t *h_words = nullptr;  // pinned
+  SetParams *h_init = nullptr;  // pinned: sets as each replay starts them (tree, then finer index)
+  SetParams *h_sets = nullptr;  // pinned: sets as the last replay left them
+  uint64_t *h_off = nullptr;    // pinned set offsets
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BuildPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    t.reset();
+    for (void *h : {(void *)h_words, (void *)h_init, (void *)h_sets, (void *)h_off})
+      if (h) (void)hipHostFree(h);
+    if (ctx && mem) ctx->free(mem);
+  }
+};
+
+// The captured work: every buffer from `ar`, every count kept on the device.
+template <int D>
+static void build_plan_body(BuildPlan &p, Arena &ar) {
+  Ctx *ctx = p.ctx;
+  Tree *t = p.t.get();
+  Tree *aux = t->knn_tree;
+  const uint32_t nsets = (uint32_t)t->sets.size();
+  const uint64_t n = p.n;
+  fill_u32(ctx, p.d_words, kPlanWords, 0u);
+  OTG_HIP_CHECK(hipMemcpyAsync(t->d_sets, p.h_init, sizeof(SetParams) * nsets, hipMemcpyHostToDevice, ctx->stream));
+  if (aux)
+    OTG_HIP_CHECK(hipMemcpyAsync(aux->d_sets, p.h_init + nsets, sizeof(SetParams) * nsets, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  Buf<uint64_t> d_off(&ar, nsets + 1);
+  OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, p.h_off, (nsets + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  Buf<uint32_t> vals(&ar, n);
+  uint32_t *err = p.d_words + kPlanErr;
+  if (aux) {
+    encode_sort<D, false>(aux, p.geom, d_off.p, vals.p, err, &ar);
+    t->d_keys = (uint64_t *)ar.take(n * 8);
+    launch(ctx, "coarse_keys", k_coarse_keys<D>, dim3(div_up(n, kB)), dim3(kB), 0, (const uint64_t *)aux->d_keys, n,
+           KeyView{aux->layout.dbits, aux->layout.cbits}, (const SetParams *)aux->d_sets,
+           KeyView{t->layout.dbits, t->layout.cbits}, (const SetParams *)t->d_sets, t->d_keys);
+    node_table<D, false>(aux, d_off.p, nullptr, &ar, &p.fine_sz);
+    reference_from_finer<D>(t, aux, &ar, &p.fine_sz, &p.ref);
+    gather_entities<D, false>(t, p.geom, vals.p, d_off.p, &ar);
+    aux->d_geom = t->d_geom;
+    aux->d_ids = t->d_ids;
+  } else {
+    encode_sort<D, false>(t, p.geom, d_off.p, vals.p, err, &ar);
+    node_table<D, false>(t, d_off.p, nullptr, &ar, &p.ref);
+    gather_entities<D, false>(t, p.geom, vals.p, d_off.p, &ar);
+  }
+  OTG_HIP_CHECK(hipMemcpyAsync(p.h_sets, t->d_sets, sizeof(SetParams) * nsets, hipMemcpyDeviceToHost, ctx->stream));
+  if (aux)
+    OTG_HIP_CHECK(hipMemcpyAsync(p.h_sets + nsets, aux->d_sets, sizeof(SetParams) * nsets, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(p.h_words, p.d_words, kPlanWords * 4, hipMemcpyDeviceToHost, ctx->stream));
+}
+
+// Device bytes the captured build takes (temporaries are not reused inside an arena).
+static size_t build_plan_bytes(const BuildPlan &p, uint32_t nsets) {
+  const uint64_t n = p.n, D = (uint64_t)p.dim;
+  const uint64_t tiles = n / 4096 + 2;
+  const auto table = [&](const TableSizes &ts) {  // one node table: per-position passes and node buffers
+    return n * 24 + tiles * 64 + (ts.cap_seg + 2) * 12 + ts.cap_nodes * (8 + 4 + 12 + 29 + 16 * D) +
+           (ts.cap_nodes / 4096 + 2) * 2048 + (1u << 20);
+  };
+  size_t b = n * (4 + 8 + 12) + tiles * 2048 + n * (8 * D + 4) + nsets * 16 + (4u << 20);
+  b += table(p.ref);
+  if (p.fine) b += n * 8 + table(p.fine_sz) + p.fine_sz.cap_nodes * 8 + p.cap_ref_nodes * (29 + 16 * D);
+  return b + b / 8;
+}
+
+static uint64_t plan_cap(uint64_t count) { return count + count / 8 + 4096; }
+
+template <int D>
+static void build_plan_capture(BuildPlan &p, const Tree &eager, const uint64_t *offsets, uint32_t nsets,
+                               const double *boxes, const uint32_t *depths, const uint32_t *caps) {
+  Ctx *ctx = p.ctx;
+  // the plan's tree: the same sets as the plain build, arrays in the arena
+  Tree *t = p.t.get();
+  prepare_sets<D, false>(t, nullptr, offsets, nsets, boxes, depths, caps);
+  t->n = p.n;
+  t->plan_arrays = true;
+  const Tree *ef = eager.knn_tree;
+  p.fine = ef != nullptr;
+  p.ref.cap_seg = plan_cap(p.fine ? 0 : eager.seg_count);
+  p.ref.cap_nodes = plan_cap(eager.node_count);
+  p.cap_ref_nodes = p.ref.cap_nodes;
+  if (p.fine) {
+    Tree *aux = new_finer_tree(t, finer_depth(D, t->layout.sbits));
+    set_layout<D, false>(aux);
+    aux->plan_arrays = true;
+    aux->shares_entities = true;
+    t->knn_tree = aux;
+    t->fine_order = true;
+    p.fine_sz.cap_seg = plan_cap(ef->seg_count);
+    p.fine_sz.cap_nodes = plan_cap(ef->node_count);
+    for (uint32_t q = 0; q < nsets; ++q)
+      if (aux->sets[q].cap != t->sets[q].cap)
+        throw Error{OTG_ERR_UNSUPPORTED, "build plans need the finer index at the tree's capacity"};
+  }
+  OTG_HIP_CHECK(hipHostMalloc((void **)&p.h_words, kPlanWords * 4, hipHostMallocDefault));
+  OTG_HIP_CHECK(hipHostMalloc((void **)&p.h_init, sizeof(SetParams) * 2 * nsets, hipHostMallocDefault));
+  OTG_HIP_CHECK(hipHostMalloc((void **)&p.h_sets, sizeof(SetParams) * 2 * nsets, hipHostMallocDefault));
+  OTG_HIP_CHECK(hipHostMalloc((void **)&p.h_off, (nsets + 1) * 8, hipHostMallocDefault));
+  std::memset(p.h_words, 0, kPlanWords * 4);
+  std::memcpy(p.h_init, t->sets.data(), sizeof(SetParams) * nsets);
+  if (p.fine) std::memcpy(p.h_init + nsets, t->knn_tree->sets.data(), sizeof(SetParams) * nsets);
+  std::memcpy(p.h_off, offsets, (nsets + 1) * 8);
+  for (size_t bytes = build_plan_bytes(p, nsets);; bytes *= 2) {
+    p.mem = (char *)ctx->alloc(bytes);
+    Arena ar{p.mem, bytes, 0};
+    // the words first: no tree array starts at the arena's (allocator-owned) base
+    p.d_words = (uint32_t *)ar.take(kPlanWords * 4);
+    p.ref.counts = p.d_words + kPlanRefCounts;
+    p.ref.overflow = p.d_words + kPlanOverflow;
+    p.fine_sz.counts = p.d_words + kPlanFineCounts;
+    p.fine_sz.overflow = p.d_words + kPlanOverflow;
+    t->d_sets = (SetParams *)ar.take(sizeof(SetParams) * nsets);
+    if (p.fine) t->knn_tree->d_sets = (SetParams *)ar.take(sizeof(SetParams) * nsets);
+    ctx->sync();
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    OTG_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    bool short_arena = false;
+    try {
+      build_plan_body<D>(p, ar);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(ctx->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      ctx->profiling = prof;
+      short_arena = ar.exhausted;
+      if (!short_arena) throw;
+    }
+    if (short_arena) {
+      ctx->free(p.mem);
+      p.mem = nullptr;
+      continue;
+    }
+    ctx->profiling = prof;
+    OTG_HIP_CHECK(hipStreamEndCapture(ctx->stream, &p.graph));
+    break;
+  }
+  OTG_HIP_CHECK(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+}
+
+BuildPlan *build_plan_create(Ctx *ctx, int dim, const double *geom, const uint64_t *offsets, uint32_t nsets,
+                             const double *boxes, const uint32_t *depths, const uint32_t *caps) {
+  if (!boxes) throw Error{OTG_ERR_INVALID_ARG, "build plans take every set's box"};
+  if (dim < 2 || dim > 6) throw Error{OTG_ERR_INVALID_ARG, "dimension must be 2..6"};
+  if (offsets[nsets] == 0) throw Error{OTG_ERR_INVALID_ARG, "plan of an empty forest"};
+  std::unique_ptr<BuildPlan> p(new BuildPlan());
+  p->ctx = ctx;
+  p->geom = geom;
+  p->n = offsets[nsets];
+  p->dim = dim;
+  p->t.reset(new ::otg_tree());
+  p->t->ctx = ctx;
+  p->t->kind = OTG_KIND_POINT;
+  p->t->dim = dim;
+  // a plain build of the current contents fixes the plan's shape and capacities
+  Tree eager;
+  eager.ctx = ctx;
+  eager.kind = OTG_KIND_POINT;
+  eager.dim = dim;
+  build_forest(&eager, geom, OTG_MEM_DEVICE, offsets, nsets, boxes, depths, caps);
+  switch (dim) {
+#define OTG_CASE(DD)                                                             \
+  case DD:                                                                       \
+    build_plan_capture<DD>(*p, eager, offsets, nsets, boxes, depths, caps);     \
+    break;
+    OTG_CASE(2)
+    OTG_CASE(3)
+    OTG_CASE(4)
+    OTG_CASE(5)
+    OTG_CASE(6)
+#undef OTG_CASE
+  }
+  publish_views(p->t.get());
+  build_plan_run(p.get());
+  return p.release();
+}
+
+void build_plan_run(BuildPlan *p) {
+  Ctx *ctx = p->ctx;
+  OTG_HIP_CHECK(hipGraphLaunch(p->exec, ctx->stream));
+  ctx->sync();
+  build_plan_finish(p);
+}
+
+// After a replay (alone or inside a larger graph) has completed: the host side of the
+// tree follows the device.
+void build_plan_finish(BuildPlan *p) {
+  Tree *t = p->t.get();
+  const uint32_t nsets = (uint32_t)t->sets.size();
+  const uint32_t *w = p->h_words;
+  std::memcpy(t->sets.data(), p->h_sets, sizeof(SetParams) * nsets);
+  t->node_count = w[kPlanRefCounts + 3];
+  const int D = p->dim;
+  t->device_bytes = p->n * (D * 8 + 4 + 8) + t->node_count * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+  if (Tree *aux = t->knn_tree) {
+    std::memcpy(aux->sets.data(), p->h_sets + nsets, sizeof(SetParams) * nsets);
+    aux->node_count = w[kPlanFineCounts + 3];
+    aux->device_bytes = p->n * 8 + aux->node_count * (sizeof(NodeInfo) + 4 + 1 + 8 + 2 * D * 8);
+    t->device_bytes += aux->device_bytes;
+  }
+  // the reference-order storage of the last contents is stale
+  std::lock_guard<std::mutex> g(t->ref_mu);
+  t->ctx->free(t->d_ids_ref);
+  t->ctx->free(t->d_geom_ref);
+  t->ctx->free(t->d_view_ref);
+  t->d_ids_ref = nullptr;
+  t->d_geom_ref = nullptr;
+  t->d_view_ref = nullptr;
+  // a table over its capacity was left empty (no set has nodes): the host side says so too
+  if (w[kPlanErr] & kErrOutOfSpace) throw Error{OTG_ERR_OUT_OF_SPACE, "point outside the given box"};
+  if (w[kPlanOverflow])
+    throw Error{OTG_ERR_CAPACITY, "the new contents need more nodes than the plan holds (make a new plan)"};
+}
+
+const Tree *build_plan_tree(const BuildPlan *p) { return p->t.get(); }
+hipGraph_t build_plan_graph(const BuildPlan *p) { return p->graph; }
+void build_plan_free(BuildPlan *p) { delete p; }
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets, const double *boxes,
                   const uint32_t *depths, const uint32_t *caps) {

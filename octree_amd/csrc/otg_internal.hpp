@@ -118,9 +118,13 @@ struct Error {
 struct Arena {
   char *base = nullptr;
   size_t cap = 0, used = 0;
+  bool exhausted = false;  // a take did not fit
   void *take(size_t bytes) {
     const size_t b = (bytes + 255) & ~size_t(255);
-    if (b > cap - used) throw Error{OTG_ERR_UNSUPPORTED, "the batch needs more scratch memory than the limit"};
+    if (b > cap - used) {
+      exhausted = true;
+      throw Error{OTG_ERR_UNSUPPORTED, "the batch needs more scratch memory than the limit"};
+    }
     void *p = base + used;
     used += b;
     return p;
@@ -177,26 +181,49 @@ inline void launch(Ctx *ctx, const char *name, Kernel k, dim3 grid, dim3 block, 
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// A count the kernels read on the device.  A build captured into a graph knows its
+// sizes only there: buffers and grids are sized by `cap`, and the kernels take
+// min(*p, cap) elements (a count beyond cap is flagged by the caller).  p null: cap
+// is the count.
+struct DevN {
+  uint64_t cap = 0;
+  const uint32_t *p = nullptr;
+  __device__ __forceinline__ uint64_t get() const {
+    if (!p) return cap;
+    const uint64_t v = *p;
+    return v < cap ? v : cap;
+  }
+};
+
 // ---- primitives (otg_primitives.hip) ----
 // Exclusive prefix sum of n uint32 values into out (may alias in), accumulated in 64 bits.
 // *d_total (device, if not null) receives the total, or kScanOverflow when it does not
 // fit 32 bits (out is then not meaningful).
 constexpr uint32_t kScanOverflow = 0xFFFFFFFFu;
-void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total,
-                        Arena *arena = nullptr);
+void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, DevN n, uint32_t *d_total, Arena *arena);
+inline void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total,
+                               Arena *arena = nullptr) {
+  exclusive_scan_u32(ctx, in, out, DevN{n, nullptr}, d_total, arena);
+}
 // a[0..n) = v by a kernel (no memset node when captured into a graph)
 void fill_u32(Ctx *ctx, uint32_t *a, uint64_t n, uint32_t v);
 // Build node events (n > 0 bytes: bit 0 segment start, bits 1-2 internal nodes):
 // idx[p] = exclusive counts before p, segments in the high 32 bits and internal nodes
 // in the low 32; d_totals[0..1] = the two totals.
-void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals);
+void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals,
+                Arena *arena = nullptr);
 // Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
 // result is in keys/vals on return.
-void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits, Arena *arena = nullptr);
+void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN n, uint32_t bits, Arena *arena);
+inline void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits,
+                             Arena *arena = nullptr) {
+  radix_sort_pairs(ctx, keys, vals, DevN{n, nullptr}, bits, arena);
+}
 // Boundary-level scans: for v[0..n) (uint8 levels, 0 = hard boundary) computes
 // left[i] = set of running minima of v[i], v[i-1], ... (inclusive, stops at level 0) and
 // right[i] = same going right, as bit masks.
-void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask);
+void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask,
+                 Arena *arena = nullptr);
 
 // Device-side view of one tree, passed to traversal kernels (an array of them
 // lets one launch serve queries on many trees).
@@ -218,6 +245,7 @@ struct Tree {
   int dim = 0;
   uint64_t n = 0;
   uint64_t node_count = 0;
+  uint64_t seg_count = 0;  // leaf segments of the last plain node-table build (sizes a build plan)
   std::vector<SetParams> sets;
   KeyLayout layout{};
   SetParams *d_sets = nullptr;
@@ -240,6 +268,8 @@ struct Tree {
   // A finer index sorted together with its reference tree shares that tree's entity
   // storage (d_ids, d_geom), which the reference tree owns.
   bool shares_entities = false;
+  // Arrays in a build plan's arena (the plan frees them, not the tree).
+  bool plan_arrays = false;
   // The reference tree of a shared sort keeps equal location keys in the finer
   // index's order instead of by id.  Ray-first (whose bound depends on the order
   // hits are met), the export and the archive use the reference's order (equal keys
@@ -271,6 +301,29 @@ void load_static_archive(Tree *t, const uint8_t *buf, uint64_t size, bool large,
                          int mem);
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets,
                   const double *boxes, const uint32_t *depths, const uint32_t *caps);
+
+// Build plan (point forests): one build captured as a HIP graph with every buffer
+// and capacity fixed when it is made (from a plain build of the contents then), and
+// replayed on new contents of the same device geometry.  run() synchronises and
+// brings the host side of the plan's tree up to date; finish() does that part alone
+// after the plan's graph ran inside a larger graph.
+struct BuildPlan;
+BuildPlan *build_plan_create(Ctx *ctx, int dim, const double *geom, const uint64_t *offsets, uint32_t nsets,
+                             const double *boxes, const uint32_t *depths, const uint32_t *caps);
+void build_plan_run(BuildPlan *p);
+void build_plan_finish(BuildPlan *p);
+const Tree *build_plan_tree(const BuildPlan *p);
+hipGraph_t build_plan_graph(const BuildPlan *p);
+void build_plan_free(BuildPlan *p);
+// A build plan followed by a kNN plan on its tree, replayed as one graph; run()
+// synchronises, then brings the tree's host side up to date and completes the kNN
+// batch (band pass) as each plan's own run would.
+struct KnnPlan;
+struct StepPlan;
+StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp);
+void step_plan_run(StepPlan *st);
+void step_plan_free(StepPlan *st);
+Ctx *step_plan_ctx(StepPlan *st);
 
 struct Result {
   Ctx *ctx = nullptr;

@@ -98,8 +98,9 @@ struct LevelSource {
 };
 
 template <class Op, class Src>
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(Src src, uint64_t n, typename Op::T *partials) {
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(Src src, DevN dn, typename Op::T *partials) {
   using T = typename Op::T;
+  const uint64_t n = dn.get();
   __shared__ T lds[kBlock / 64];
   uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   T acc = Op::identity();
@@ -141,10 +142,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(typename Op::T *partia
 
 // Final pass: exclusive u32 output (exact while the total fits 32 bits); the total
 // is written saturated at 0xFFFFFFFF, which callers treat as overflow.
-__global__ void __launch_bounds__(kBlock) k_scan_final_u32(const uint32_t *in, uint32_t *out, uint64_t n,
+__global__ void __launch_bounds__(kBlock) k_scan_final_u32(const uint32_t *in, uint32_t *out, DevN dn,
                                                             const uint64_t *partials, uint32_t *d_total,
                                                             uint32_t nblocks) {
   __shared__ uint64_t lds[kBlock / 64];
+  const uint64_t n = dn.get();
   uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   uint32_t items[kItems];
   uint64_t acc = 0;
@@ -224,11 +226,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_final_events(EventSource src, u
   }
 }
 
-void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals) {
+void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals, Arena *arena) {
   const uint32_t nb = div_up(n, kTile);
-  Buf<uint64_t> partials(ctx, nb);
+  Buf<uint64_t> partials(ctx, arena, nb);
   launch(ctx, "scan_reduce_events", k_scan_reduce<OpAdd64, EventSource>, dim3(nb), dim3(kBlock), 0,
-         EventSource{events}, n, partials.p);
+         EventSource{events}, DevN{n, nullptr}, partials.p);
   launch(ctx, "scan_partials_u32", k_scan_partials<OpAdd64>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
   launch(ctx, "scan_final_events", k_scan_final_events, dim3(nb), dim3(kBlock), 0, EventSource{events}, idx, n,
          (const uint64_t *)partials.p, d_totals, nb);
@@ -243,12 +245,12 @@ void fill_u32(Ctx *ctx, uint32_t *a, uint64_t n, uint32_t v) {
   if (n) launch(ctx, "fill_u32", k_fill_words, dim3((uint32_t)std::min<uint64_t>(div_up(n, 256), 4096)), dim3(256), 0, a, n, v);
 }
 
-void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total, Arena *arena) {
-  if (n == 0) {
-    if (d_total) OTG_HIP_CHECK(hipMemsetAsync(d_total, 0, 4, ctx->stream));
+void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, DevN n, uint32_t *d_total, Arena *arena) {
+  if (n.cap == 0) {
+    if (d_total) fill_u32(ctx, d_total, 1, 0u);
     return;
   }
-  uint32_t nb = div_up(n, kTile);
+  uint32_t nb = div_up(n.cap, kTile);
   Buf<uint64_t> partials(ctx, arena, nb);
   launch(ctx, "scan_reduce_u32", k_scan_reduce<OpAdd64, U32Source>, dim3(nb), dim3(kBlock), 0, U32Source{in}, n,
          partials.p);
@@ -257,14 +259,14 @@ void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,
          d_total, nb);
 }
 
-void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask) {
+void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask, Arena *arena) {
   if (n == 0) return;
   uint32_t nb = div_up(n, kTile);
-  Buf<LevelState> partials(ctx, nb);
+  Buf<LevelState> partials(ctx, arena, nb);
   for (int dir = 0; dir < 2; ++dir) {
     LevelSource src{v, n, dir == 1};
-    launch(ctx, "scan_reduce_level", k_scan_reduce<OpLevel, LevelSource>, dim3(nb), dim3(kBlock), 0, src, n,
-           partials.p);
+    launch(ctx, "scan_reduce_level", k_scan_reduce<OpLevel, LevelSource>, dim3(nb), dim3(kBlock), 0, src,
+           DevN{n, nullptr}, partials.p);
     launch(ctx, "scan_partials_level", k_scan_partials<OpLevel>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
     launch(ctx, "scan_final_level", k_scan_final_level, dim3(nb), dim3(kBlock), 0, src, n,
            (const LevelState *)partials.p, dir == 0 ? left_mask : right_mask);
@@ -307,9 +309,10 @@ __global__ void __launch_bounds__(kBlock) k_os_init(OsState *__restrict__ st, ui
   for (uint64_t i = t; i < nstatus; i += stride) status0[i] = 0u;
 }
 
-__global__ void __launch_bounds__(kBlock) k_os_hist(const uint64_t *__restrict__ keys, uint64_t n, uint32_t passes,
+__global__ void __launch_bounds__(kBlock) k_os_hist(const uint64_t *__restrict__ keys, DevN dn, uint32_t passes,
                                                      OsState *__restrict__ st) {
   __shared__ uint32_t h[kOsMaxPasses][256];
+  const uint64_t n = dn.get();
   for (uint32_t p = 0; p < passes; ++p) h[p][threadIdx.x] = 0u;
   __syncthreads();
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
@@ -341,7 +344,7 @@ constexpr int kOsTile = kOsBlock * kItems;
 
 __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                           uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                          uint64_t n, uint32_t pass, OsState *__restrict__ st,
+                                                          DevN dn, uint32_t pass, OsState *__restrict__ st,
                                                           uint32_t *__restrict__ status, uint32_t *__restrict__ status_next,
                                                           uint32_t *__restrict__ err) {
   __shared__ uint32_t wcnt[kOsWaves][256];
@@ -359,8 +362,11 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   if (digit_thread)
     for (int w = 0; w < kOsWaves; ++w) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
+  const uint64_t n = dn.get();
   const uint32_t tile_id = s_tile;
   const uint64_t tile0 = (uint64_t)tile_id * kOsTile;
+  // tiles past a device-side count have nothing to do; no tile in front of them waits on them
+  if (tile0 >= n) return;
   const uint64_t tile = tile0 + (uint64_t)wave * (64 * kItems);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint64_t key[kItems];
@@ -485,7 +491,8 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   }
 }
 
-void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits, Arena *arena) {
+void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN dn, uint32_t bits, Arena *arena) {
+  const uint64_t n = dn.cap;  // buffers and grids: the capacity (the count itself when dn.p is null)
   if (n <= 1 || bits == 0) return;
   if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
   const uint32_t passes = std::min<uint32_t>((bits + 7) / 8, kOsMaxPasses);
@@ -498,13 +505,13 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint
   launch(ctx, "radix_init", k_os_init, dim3(std::min<uint32_t>(div_up((uint64_t)256 * ntiles, kBlock), 1024)),
          dim3(kBlock), 0, st.p, stat[0], (uint64_t)256 * ntiles);
   launch(ctx, "radix_hist", k_os_hist, dim3(std::min<uint32_t>(div_up(n, kBlock), 2048)), dim3(kBlock), 0,
-         (const uint64_t *)keys, n, passes, st.p);
+         (const uint64_t *)keys, dn, passes, st.p);
   launch(ctx, "radix_prefix", k_os_prefix, dim3(1), dim3(kBlock), 0, st.p, passes);
   uint64_t *ka = keys, *kb = k2.p;
   uint32_t *va = vals, *vb = v2.p;
   for (uint32_t p = 0; p < passes; ++p) {
     launch(ctx, "radix_scatter", k_os_scatter, dim3(ntiles), dim3(kOsBlock), 0, (const uint64_t *)ka,
-           (const uint32_t *)va, kb, vb, n, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err);
+           (const uint32_t *)va, kb, vb, dn, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err);
     std::swap(ka, kb);
     std::swap(va, vb);
   }

@@ -4022,15 +4022,61 @@ KnnPlan *knn_plan_create(const Tree *tree, uint64_t nq, const double *q, const u
   return p.release();
 }
 
+// After the plan's graph ran (alone or inside a larger graph) and the stream drained:
+// the checks, and the band pass when a tolerance band outgrew a list.
+static void knn_plan_after(KnnPlan *p) {
+  if (p->h_flags[1]) throw Error{OTG_ERR_INVALID_ARG, "a query's set id is not a set of the tree"};
+  if (p->h_flags[2] == kScanOverflow) throw Error{OTG_ERR_UNSUPPORTED, "a batch result of 2^32 - 1 or more ids"};
+  if (p->h_flags[0]) p->complete(*p);
+}
+
 void knn_plan_run(KnnPlan *p) {
   Ctx *ctx = p->ctx;
   p->band_result.reset();
   OTG_HIP_CHECK(hipGraphLaunch(p->exec, ctx->stream));
   ctx->sync();
-  if (p->h_flags[1]) throw Error{OTG_ERR_INVALID_ARG, "a query's set id is not a set of the tree"};
-  if (p->h_flags[2] == kScanOverflow) throw Error{OTG_ERR_UNSUPPORTED, "a batch result of 2^32 - 1 or more ids"};
-  if (p->h_flags[0]) p->complete(*p);
+  knn_plan_after(p);
 }
+
+// A build plan and a kNN plan on its tree as one graph: the build's graph, then the
+// kNN batch's, as child graphs in that order.
+struct StepPlan {
+  BuildPlan *bp = nullptr;
+  KnnPlan *kp = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~StepPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp) {
+  if (kp->t != knn_index(build_plan_tree(bp)))
+    throw Error{OTG_ERR_INVALID_ARG, "the kNN plan must be made on the build plan's tree"};
+  std::unique_ptr<StepPlan> st(new StepPlan());
+  st->bp = bp;
+  st->kp = kp;
+  OTG_HIP_CHECK(hipGraphCreate(&st->graph, 0));
+  hipGraphNode_t nb = nullptr, nk = nullptr;
+  OTG_HIP_CHECK(hipGraphAddChildGraphNode(&nb, st->graph, nullptr, 0, build_plan_graph(bp)));
+  OTG_HIP_CHECK(hipGraphAddChildGraphNode(&nk, st->graph, &nb, 1, kp->graph));
+  OTG_HIP_CHECK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
+  return st.release();
+}
+
+void step_plan_run(StepPlan *st) {
+  Ctx *ctx = st->kp->ctx;
+  st->kp->band_result.reset();
+  OTG_HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
+  ctx->sync();
+  build_plan_finish(st->bp);
+  knn_plan_after(st->kp);
+}
+
+void step_plan_free(StepPlan *st) { delete st; }
+
+Ctx *step_plan_ctx(StepPlan *st) { return st->kp->ctx; }
 
 void knn_plan_result(const KnnPlan *p, const uint64_t **off, const uint32_t **ids, uint64_t *nids) {
   if (p->band_result) {
