@@ -309,6 +309,9 @@ def main():
     plan.free()
     fg.free()
     log(f"kNN batch: eager {e_ms * 1e3:.2f} ms, graph replay {g_ms * 1e3:.2f} ms")
+    sg = step_graph(octree_amd, ctx, d_pts, offs, cap, box, None, d_knn, args.k, 3, max(3, args.steps), set_ids=d_sid)
+    log(f"build + kNN: eager {sg['eager_build_knn_ms']:.2f} ms, one graph {sg['graph_build_knn_ms']:.2f} ms; "
+        f"build alone: eager {sg['eager_build_ms']:.2f} ms, graph {sg['graph_build_ms']:.2f} ms")
 
     # ---- per-kernel times (HIP events on the library stream) for the roofline ----
     ctx.set_profiling(True)
@@ -354,6 +357,7 @@ def main():
                     "p50_step_ms": float(np.percentile([x[3] for x in bt], 50)) * 1e3,
                     "p99_step_ms": float(np.percentile([x[3] for x in bt], 99)) * 1e3},
         "knn_graph": {"eager_ms": e_ms * 1e3, "graph_ms": g_ms * 1e3, "graph_qps": Q / g_ms},
+        "step_graph": sg,
         "kernel_ms": {k: v[0] for k, v in kt.items()},
         "build_roofline": {"bound": "hbm", "algorithmic_bytes": build_bytes, "radix_passes": passes,
                            "key_bits": bits, "nodes": nnodes, "build_ms": b_build * 1e3,
@@ -445,6 +449,41 @@ def _timed(ctx, fn, reps):
     return float(np.median(ts)), ts
 
 
+def step_graph(octree_amd, ctx, d_pts, offs, cap, boxes, depths, d_q, k, dim, reps, set_ids=None):
+    """Build + kNN batch replayed as one HIP graph (otg_build_plan and otg_knn_plan joined
+    as an otg_step_plan) beside the same two calls made eagerly (otg_forest_build, then
+    otg_knn), and the build plan replayed alone.  Both sides start from the points and
+    queries resident in HBM; the graph side rebuilds the tree in place every replay."""
+    bp = octree_amd.BuildPlan(d_pts, offs, cap, boxes, depths, ctx=ctx, dim=dim)
+    try:
+        kp = bp.tree.knn_plan(d_q, k, set_ids=set_ids)
+        sp = bp.step(kp)
+        g_ms, _ = _timed(ctx, lambda: sp.run() and None, reps)
+        gb_ms, _ = _timed(ctx, lambda: bp.run() and None, reps)
+
+        def eager():
+            f = octree_amd.StaticTree.forest(None, cap, depths, boxes, ctx=ctx, dim=dim, kind="point",
+                                             device_geom=(d_pts, offs))
+            r = f.knn_batch(d_q, k, set_ids=set_ids)
+            ctx.synchronize()
+            r.free()
+            f.free()
+
+        e_ms, _ = _timed(ctx, eager, reps)
+
+        def eager_build():
+            f = octree_amd.StaticTree.forest(None, cap, depths, boxes, ctx=ctx, dim=dim, kind="point",
+                                             device_geom=(d_pts, offs))
+            ctx.synchronize()
+            f.free()
+
+        eb_ms, _ = _timed(ctx, eager_build, reps)
+    finally:
+        bp.free()
+    return {"eager_build_knn_ms": e_ms * 1e3, "graph_build_knn_ms": g_ms * 1e3, "eager_build_ms": eb_ms * 1e3,
+            "graph_build_ms": gb_ms * 1e3}
+
+
 def knn_roofline(ctx, tree, d_q, k, D, set_ids=None):
     """Roofline of one kNN launch: minimal touched bytes (otg_knn_footprint: the query
     point 8D B, 4 B per returned id, a node record of 13 + 8D B -- entity segment 8,
@@ -490,6 +529,8 @@ def run_single(args, octree_amd, torch):
         tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
         plan.free()
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
+        out["step_graph"] = step_graph(octree_amd, ctx, d_pts, [0, len(pts)], 16, [box], 10, d_q, 8, 3,
+                                       max(3, reps))
         out["roofline"] = knn_roofline(ctx, tree, d_q, 8, 3)
         # Create<SeqExec>: the same tree numbered as the reference's default (unsorted) build
         seq = octree_amd.StaticOctreePoint(ctx=ctx)
@@ -617,6 +658,7 @@ def run_single(args, octree_amd, torch):
         tg, _ = _timed(ctx, lambda: plan.run() and None, reps)
         plan.free()
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
+        out["step_graph"] = step_graph(octree_amd, ctx, d_p, [0, len(pts)], 32, [box], 8, d_q, 16, 6, max(3, reps))
         out["roofline"] = knn_roofline(ctx, tree, d_q, 16, 6)
         # CPU: oracle build of a 1M-point sample and 20k queries on it
         m = 1_000_000
