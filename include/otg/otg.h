@@ -334,12 +334,32 @@ OTG_API otg_status otg_build_plan_run(otg_build_plan *plan);
 OTG_API otg_status otg_build_plan_tree(const otg_build_plan *plan, const otg_tree **tree);
 OTG_API void otg_build_plan_free(otg_build_plan *plan);
 
-/* A build plan and a kNN plan made on its tree, replayed as one graph: the build,
- * then the kNN batch.  otg_step_plan_run synchronises; afterwards the tree is
- * current and otg_knn_plan_result / otg_knn_plan_copy give the batch's ids, as after
- * otg_build_plan_run followed by otg_knn_plan_run.  Free it before either plan. */
+/* Range plans: one RangeSearch batch (otg_range) of device-resident boxes on a 2-D or
+ * 3-D tree, recorded as a HIP graph with every buffer allocated up front; the id
+ * capacity comes from the batch's result at creation (ids + ids / 8 + 65,536).
+ * otg_range_plan_run replays it on the current contents of the boxes (and set ids)
+ * and synchronises; results equal otg_range's for the same contents.  A result that
+ * outgrows the plan fails the run with OTG_ERR_CAPACITY (no ids written; make a new
+ * plan); a set id outside the tree with OTG_ERR_INVALID_ARG.  No counterpart in the
+ * reference, like the kNN plans.  Free plans before their tree. */
+typedef struct otg_range_plan otg_range_plan;
+OTG_API otg_status otg_range_plan_create(const otg_tree *tree, uint64_t nq, const double *ranges,
+                                         const uint32_t *set_ids, int mode, double tolerance, otg_range_plan **out);
+OTG_API otg_status otg_range_plan_run(otg_range_plan *plan);
+OTG_API otg_status otg_range_plan_result(const otg_range_plan *plan, const uint64_t **offsets, const uint32_t **ids,
+                                         uint64_t *nids);
+/* Host copy of the last run's CSR: offsets[nq + 1], ids[nids]. */
+OTG_API otg_status otg_range_plan_copy(const otg_range_plan *plan, uint64_t *offsets, uint32_t *ids);
+OTG_API void otg_range_plan_free(otg_range_plan *plan);
+
+/* A build plan and query plans made on its tree, replayed as one graph: the build,
+ * then the kNN batch (knn_or_null), then the range batch (range_or_null).
+ * otg_step_plan_run synchronises; afterwards the tree is current and each query
+ * plan's result / copy give its batch's ids, as after otg_build_plan_run followed by
+ * each plan's own run.  Free it before the plans. */
 typedef struct otg_step_plan otg_step_plan;
-OTG_API otg_status otg_step_plan_create(otg_build_plan *build, otg_knn_plan *knn, otg_step_plan **out);
+OTG_API otg_status otg_step_plan_create(otg_build_plan *build, otg_knn_plan *knn_or_null,
+                                        otg_range_plan *range_or_null, otg_step_plan **out);
 OTG_API otg_status otg_step_plan_run(otg_step_plan *step);
 OTG_API void otg_step_plan_free(otg_step_plan *step);
 

@@ -227,6 +227,46 @@ class KnnPlan:
             pass
 
 
+class RangePlan:
+    """A captured RangeSearch batch (otg_range_plan): run() replays it on the current
+    contents of the box tensors it was made from; csr() / device_csr() read the last run."""
+
+    def __init__(self, handle, tree, nq: int, keep):
+        self.handle = handle
+        self.tree = tree
+        self.nq = nq
+        self._keep = keep
+
+    def run(self) -> "RangePlan":
+        _check(_abi.lib().otg_range_plan_run(self.handle), "otg_range_plan_run", self.tree.ctx)
+        return self
+
+    def device_csr(self):
+        off, ids, n = C.c_void_p(), C.c_void_p(), C.c_uint64()
+        _check(_abi.lib().otg_range_plan_result(self.handle, C.byref(off), C.byref(ids), C.byref(n)),
+               "otg_range_plan_result", self.tree.ctx)
+        return off.value, ids.value, n.value
+
+    def csr(self):
+        n = self.device_csr()[2]
+        offsets = np.empty(self.nq + 1, dtype=np.uint64)
+        ids = np.empty(max(n, 1), dtype=np.uint32)
+        _check(_abi.lib().otg_range_plan_copy(self.handle, _ptr(offsets), _ptr(ids)), "otg_range_plan_copy",
+               self.tree.ctx)
+        return offsets, ids[:n]
+
+    def free(self):
+        if getattr(self, "handle", None):
+            _abi.lib().otg_range_plan_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class BuildPlan:
     """A point-forest build captured as a HIP graph (otg_build_plan): made from device
     points and host set offsets (every set's box given), replayed by run() on whatever
@@ -263,12 +303,14 @@ class BuildPlan:
         _check(_abi.lib().otg_build_plan_run(self.handle), "otg_build_plan_run", self.ctx)
         return self
 
-    def step(self, knn_plan: KnnPlan) -> "StepPlan":
-        """The build and a kNN plan made on self.tree, replayed as one graph."""
+    def step(self, knn_plan: Optional[KnnPlan] = None, range_plan: Optional[RangePlan] = None) -> "StepPlan":
+        """The build, then a kNN plan and / or a range plan made on self.tree, replayed as
+        one graph."""
         h = C.c_void_p()
-        _check(_abi.lib().otg_step_plan_create(self.handle, knn_plan.handle, C.byref(h)), "otg_step_plan_create",
-               self.ctx)
-        sp = StepPlan(h, self, knn_plan)
+        _check(_abi.lib().otg_step_plan_create(self.handle, knn_plan.handle if knn_plan else None,
+                                               range_plan.handle if range_plan else None, C.byref(h)),
+               "otg_step_plan_create", self.ctx)
+        sp = StepPlan(h, self, knn_plan, range_plan)
         self._steps.append(sp)
         return sp
 
@@ -288,11 +330,11 @@ class BuildPlan:
 
 
 class StepPlan:
-    """A build plan and a kNN plan on its tree as one graph (otg_step_plan); after run()
-    the tree is current and the kNN plan's csr() holds the batch's ids."""
+    """A build plan and query plans on its tree as one graph (otg_step_plan); after run()
+    the tree is current and each query plan's csr() holds its batch's ids."""
 
-    def __init__(self, handle, build: BuildPlan, knn: KnnPlan):
-        self.handle, self.build, self.knn = handle, build, knn
+    def __init__(self, handle, build: BuildPlan, knn: Optional[KnnPlan], rng: Optional[RangePlan]):
+        self.handle, self.build, self.knn, self.range = handle, build, knn, rng
 
     def run(self) -> "StepPlan":
         _check(_abi.lib().otg_step_plan_run(self.handle), "otg_step_plan_run", self.build.ctx)
@@ -558,6 +600,25 @@ class StaticTree:
                                             C.byref(h))
         _check(st, "otg_knn_plan_create", self.ctx)
         plan = KnnPlan(h, self, nq, (points, set_ids))
+        if not hasattr(self, "_plans"):
+            self._plans = weakref.WeakSet()
+        self._plans.add(plan)
+        return plan
+
+    def range_plan(self, ranges, mode: int = _abi.OTG_RANGE_INSIDE, tolerance: float = BASE_TOLERANCE,
+                   set_ids=None) -> "RangePlan":
+        """otg_range_plan_create: the RangeSearch batch over these device tensors (boxes as
+        min then max, 2 x dim columns), captured once and replayed by RangePlan.run()."""
+        if not _is_device(ranges):
+            raise ValueError("a range plan takes device-resident boxes")
+        rp, nq = _dev(ranges, 2 * self.dim)
+        if set_ids is not None and (not _is_device(set_ids) or not set_ids.is_contiguous()):
+            raise ValueError("set ids of a plan must be a contiguous device tensor")
+        sp = None if set_ids is None else C.c_void_p(set_ids.data_ptr())
+        h = C.c_void_p()
+        st = _abi.lib().otg_range_plan_create(self.handle, nq, rp, sp, int(mode), float(tolerance), C.byref(h))
+        _check(st, "otg_range_plan_create", self.ctx)
+        plan = RangePlan(h, self, nq, (ranges, set_ids))
         if not hasattr(self, "_plans"):
             self._plans = weakref.WeakSet()
         self._plans.add(plan)
@@ -849,7 +910,7 @@ def StaticOrthoTreeBoxND(dim: int, loose: bool = True):
 
 
 __all__ = [
-    "Context", "Result", "KnnPlan", "BuildPlan", "StepPlan", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
+    "Context", "Result", "KnnPlan", "RangePlan", "BuildPlan", "StepPlan", "StaticTree", "Server", "OtgError", "BASE_TOLERANCE", "DBL_MAX", "PAR_EXEC", "SEQ_EXEC", "GPU_EXEC",
     "StaticQuadtreePoint", "StaticQuadtreeBox", "StaticOctreePoint", "StaticOctreeBox", "StaticHexatreePoint",
     "StaticHexatreeBox", "StaticOrthoTreePointND", "StaticOrthoTreeBoxND", "default_context",
 ]

@@ -54,6 +54,8 @@ EXPORTED = [
     "otg_knn_plan_create", "otg_knn_plan_run", "otg_knn_plan_result", "otg_knn_plan_copy", "otg_knn_plan_free",
     "otg_build_plan_create", "otg_build_plan_run", "otg_build_plan_tree", "otg_build_plan_free",
     "otg_step_plan_create", "otg_step_plan_run", "otg_step_plan_free",
+    "otg_range_plan_create", "otg_range_plan_run", "otg_range_plan_result", "otg_range_plan_copy",
+    "otg_range_plan_free",
     "otg_autotune", "otg_ctx_set_tuning", "otg_ctx_get_tuning",
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
@@ -142,7 +144,12 @@ def lib() -> C.CDLL:
         "otg_build_plan_run": (C.c_int, [P]),
         "otg_build_plan_tree": (C.c_int, [P, PP]),
         "otg_build_plan_free": (None, [P]),
-        "otg_step_plan_create": (C.c_int, [P, P, PP]),
+        "otg_step_plan_create": (C.c_int, [P, P, P, PP]),
+        "otg_range_plan_create": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_double, PP]),
+        "otg_range_plan_run": (C.c_int, [P]),
+        "otg_range_plan_result": (C.c_int, [P, PP, PP, C.POINTER(C.c_uint64)]),
+        "otg_range_plan_copy": (C.c_int, [P, P, P]),
+        "otg_range_plan_free": (None, [P]),
         "otg_step_plan_run": (C.c_int, [P]),
         "otg_step_plan_free": (None, [P]),
         "otg_knn_footprint": (C.c_int, [P, C.c_uint64, P, P, C.c_int, C.c_uint32, C.c_double, C.c_double, U64P]),

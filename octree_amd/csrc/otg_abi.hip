@@ -679,15 +679,73 @@ OTG_API void otg_build_plan_free(otg_build_plan *plan) {
   otg::build_plan_free(p);
 }
 
-OTG_API otg_status otg_step_plan_create(otg_build_plan *build, otg_knn_plan *knn, otg_step_plan **out) {
-  if (!build || !knn || !out) return OTG_ERR_INVALID_ARG;
+OTG_API otg_status otg_step_plan_create(otg_build_plan *build, otg_knn_plan *knn_or_null,
+                                        otg_range_plan *range_or_null, otg_step_plan **out) {
+  if (!build || !out) return OTG_ERR_INVALID_ARG;
   *out = nullptr;
   auto *bp = reinterpret_cast<otg::BuildPlan *>(build);
   Ctx *ctx = otg::build_plan_tree(bp)->ctx;
   return guarded(ctx, [&] {
     OTG_HIP_CHECK(hipSetDevice(ctx->device));
-    *out = reinterpret_cast<otg_step_plan *>(otg::step_plan_create(bp, reinterpret_cast<otg::KnnPlan *>(knn)));
+    *out = reinterpret_cast<otg_step_plan *>(otg::step_plan_create(bp, reinterpret_cast<otg::KnnPlan *>(knn_or_null),
+                                                                   reinterpret_cast<otg::RangePlan *>(range_or_null)));
   });
+}
+
+OTG_API otg_status otg_range_plan_create(const otg_tree *t, uint64_t nq, const double *ranges, const uint32_t *set_ids,
+                                         int mode, double tolerance, otg_range_plan **out) {
+  if (!t || !out || !ranges || nq == 0) return OTG_ERR_INVALID_ARG;
+  if (mode != OTG_RANGE_INSIDE && mode != OTG_RANGE_OVERLAP) return OTG_ERR_INVALID_ARG;
+  *out = nullptr;
+  Ctx *ctx = t->ctx;
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    *out = reinterpret_cast<otg_range_plan *>(otg::range_plan_create(t, nq, ranges, set_ids, mode, tolerance));
+  });
+}
+
+OTG_API otg_status otg_range_plan_run(otg_range_plan *plan) {
+  if (!plan) return OTG_ERR_INVALID_ARG;
+  auto *p = reinterpret_cast<otg::RangePlan *>(plan);
+  Ctx *ctx = otg::range_plan_ctx(p);
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    otg::range_plan_run(p);
+  });
+}
+
+OTG_API otg_status otg_range_plan_result(const otg_range_plan *plan, const uint64_t **offsets, const uint32_t **ids,
+                                         uint64_t *nids) {
+  if (!plan || !offsets || !ids || !nids) return OTG_ERR_INVALID_ARG;
+  otg::range_plan_result(reinterpret_cast<const otg::RangePlan *>(plan), offsets, ids, nids);
+  return OTG_OK;
+}
+
+OTG_API otg_status otg_range_plan_copy(const otg_range_plan *plan, uint64_t *offsets, uint32_t *ids) {
+  if (!plan || !offsets) return OTG_ERR_INVALID_ARG;
+  auto *p = reinterpret_cast<const otg::RangePlan *>(plan);
+  Ctx *ctx = otg::range_plan_ctx(const_cast<otg::RangePlan *>(p));
+  return guarded(ctx, [&] {
+    OTG_HIP_CHECK(hipSetDevice(ctx->device));
+    const uint64_t *d_off;
+    const uint32_t *d_ids;
+    uint64_t nids = 0;
+    otg::range_plan_result(p, &d_off, &d_ids, &nids);
+    if (nids && !ids) throw Error{OTG_ERR_INVALID_ARG, "ids buffer"};
+    OTG_HIP_CHECK(hipMemcpyAsync(offsets, d_off, (otg::range_plan_queries(p) + 1) * 8, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    if (nids) OTG_HIP_CHECK(hipMemcpyAsync(ids, d_ids, nids * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+OTG_API void otg_range_plan_free(otg_range_plan *plan) {
+  if (!plan) return;
+  auto *p = reinterpret_cast<otg::RangePlan *>(plan);
+  Ctx *ctx = otg::range_plan_ctx(p);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  otg::range_plan_free(p);
 }
 
 OTG_API otg_status otg_step_plan_run(otg_step_plan *step) {

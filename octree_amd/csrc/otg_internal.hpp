@@ -319,8 +319,9 @@ void build_plan_free(BuildPlan *p);
 // synchronises, then brings the tree's host side up to date and completes the kNN
 // batch (band pass) as each plan's own run would.
 struct KnnPlan;
+struct RangePlan;
 struct StepPlan;
-StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp);
+StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp, RangePlan *rp);
 void step_plan_run(StepPlan *st);
 void step_plan_free(StepPlan *st);
 Ctx *step_plan_ctx(StepPlan *st);
@@ -372,6 +373,16 @@ Ctx *knn_plan_ctx(KnnPlan *p);
 uint64_t knn_plan_queries(const KnnPlan *p);
 Result *query_range(const Tree *t, uint64_t nq, const double *ranges, const uint32_t *sets, int mem, int mode,
                     double tol);
+// Range plan (2-D / 3-D trees): one RangeSearch batch of device-resident boxes captured
+// as a HIP graph, buffers sized by the batch's result at creation; replays on new
+// contents of the same buffers; a result that outgrows the plan fails the run with
+// OTG_ERR_CAPACITY.  The result stays valid until the next run.
+RangePlan *range_plan_create(const Tree *t, uint64_t nq, const double *q, const uint32_t *sets, int mode, double tol);
+void range_plan_run(RangePlan *p);
+void range_plan_result(const RangePlan *p, const uint64_t **offsets, const uint32_t **ids, uint64_t *nids);
+void range_plan_free(RangePlan *p);
+Ctx *range_plan_ctx(RangePlan *p);
+uint64_t range_plan_queries(const RangePlan *p);
 Result *query_ray(const Tree *t, uint64_t nq, const double *rays, const uint32_t *sets, int mem, double tol,
                   double inc, double max_dist);
 // One launch over queries on several trees of one kind and dimension; host inputs.

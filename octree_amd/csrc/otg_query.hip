@@ -1408,15 +1408,21 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
                                                     const uint32_t *__restrict__ list, int overlap_mode, double tol,
                                                     uint32_t *__restrict__ counts,
                                                     const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
-                                                    uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
+                                                    uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies,
+                                                    const uint32_t *__restrict__ nlist,
+                                                    const uint32_t *__restrict__ abort) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
   // one frontier, rewritten in place each round: fewer than 64 nodes of <= 8 children
   __shared__ uint32_t s_front[64 * 8];
   __shared__ uint32_t s_nf[2], s_cnt, s_next;
   __shared__ uint32_t s_a[kRangeStack * 64];
   __shared__ uint8_t s_b[kRangeStack * 64];
-  const uint64_t qi = list[blockIdx.x];
+  if (abort && *abort) return;
   const uint32_t lane = threadIdx.x;
+  // list entries: one per block, or (a device-side count) strided over a bounded grid
+  const uint32_t nl = nlist ? *nlist : gridDim.x;
+  for (uint32_t b = blockIdx.x; b < nl; b += gridDim.x) {
+  const uint64_t qi = list[b];
   const TreeView &tv = views[qview ? qview[qi] : 0u];
   const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
   double rmin[D], rmax[D], rsz[D];
@@ -1512,6 +1518,7 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
   }
   __syncthreads();
   if (!FILL && lane == 0) counts[qi] = s_cnt;
+  }
 }
 
 // Queries whose count walk found at most kRangeInline ids have them all in their
@@ -1521,9 +1528,13 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
 __global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, const uint32_t *__restrict__ cnt,
                                const uint32_t *__restrict__ capped, const uint64_t *__restrict__ off,
                                const uint32_t *__restrict__ inl, uint32_t *__restrict__ out,
-                               uint32_t *__restrict__ heavy) {
+                               uint32_t *__restrict__ heavy, const uint32_t *__restrict__ abort) {
   const uint64_t ti = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (ti >= nq) return;
+  if (abort && *abort) {  // results over the plan's capacity: nothing is written
+    heavy[ti] = 0;
+    return;
+  }
   const uint64_t qi = order ? order[ti] : ti;
   const uint32_t c = cnt[qi];
   const bool h = capped[ti] || c > kRangeInline;  // a capped walk left its row incomplete
@@ -1548,24 +1559,31 @@ __global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, 
 }
 
 // Sorting the larger results: key (rank of the query among them, id) per id.
+// (nh: a device-side count of listed queries, strided over a bounded grid; abort: skip)
 __global__ void k_range_sort_keys(const uint32_t *__restrict__ hlist, uint32_t nheavy, const uint64_t *__restrict__ off,
                                   const uint32_t *__restrict__ hpos, const uint32_t *__restrict__ ids,
-                                  uint64_t *__restrict__ keys) {
-  const uint32_t h = blockIdx.x;
-  if (h >= nheavy) return;
-  const uint32_t qi = hlist[h];
-  const uint64_t o = off[qi], c = off[qi + 1] - o;
-  for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) keys[hpos[h] + t] = ((uint64_t)h << 32) | ids[o + t];
+                                  uint64_t *__restrict__ keys, const uint32_t *__restrict__ nh,
+                                  const uint32_t *__restrict__ abort) {
+  if (abort && *abort) return;
+  const uint32_t n = nh ? *nh : nheavy;
+  for (uint32_t h = blockIdx.x; h < n; h += gridDim.x) {
+    const uint32_t qi = hlist[h];
+    const uint64_t o = off[qi], c = off[qi + 1] - o;
+    for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) keys[hpos[h] + t] = ((uint64_t)h << 32) | ids[o + t];
+  }
 }
 
 __global__ void k_range_sort_back(const uint32_t *__restrict__ hlist, uint32_t nheavy, const uint64_t *__restrict__ off,
                                   const uint32_t *__restrict__ hpos, const uint64_t *__restrict__ keys,
-                                  uint32_t *__restrict__ ids) {
-  const uint32_t h = blockIdx.x;
-  if (h >= nheavy) return;
-  const uint32_t qi = hlist[h];
-  const uint64_t o = off[qi], c = off[qi + 1] - o;
-  for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) ids[o + t] = (uint32_t)keys[hpos[h] + t];
+                                  uint32_t *__restrict__ ids, const uint32_t *__restrict__ nh,
+                                  const uint32_t *__restrict__ abort) {
+  if (abort && *abort) return;
+  const uint32_t n = nh ? *nh : nheavy;
+  for (uint32_t h = blockIdx.x; h < n; h += gridDim.x) {
+    const uint32_t qi = hlist[h];
+    const uint64_t o = off[qi], c = off[qi + 1] - o;
+    for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) ids[o + t] = (uint32_t)keys[hpos[h] + t];
+  }
 }
 
 __global__ void k_iota_u32(uint32_t *__restrict__ a, uint32_t n) {
@@ -1574,9 +1592,10 @@ __global__ void k_iota_u32(uint32_t *__restrict__ a, uint32_t n) {
 }
 
 __global__ void k_range_heavy_counts(const uint32_t *__restrict__ hlist, uint32_t nheavy,
-                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ hc) {
+                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ hc,
+                                     const uint32_t *__restrict__ nh) {
   const uint64_t h = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (h < nheavy) hc[h] = (uint32_t)(off[hlist[h] + 1] - off[hlist[h]]);
+  if (h < (nh ? *nh : nheavy)) hc[h] = (uint32_t)(off[hlist[h] + 1] - off[hlist[h]]);
 }
 
 __global__ void k_range_capped(const uint32_t *__restrict__ order, uint64_t nq, const uint32_t *__restrict__ cnt,
@@ -1592,12 +1611,17 @@ __global__ void k_range_heavy_list(const uint32_t *__restrict__ heavy, const uin
 }
 
 // One wavefront per deferred run of a range result.
+// (ncopy: a device-side run count, strided over a bounded grid; abort: skip)
 __global__ void __launch_bounds__(256) k_range_copy(const RangeCopy *__restrict__ copies, uint32_t n,
-                                                     uint32_t *__restrict__ out) {
-  const uint32_t w = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x & 63;
-  if (w >= n) return;
-  const RangeCopy c = copies[w];
-  for (uint32_t t = lane; t < c.len; t += 64) out[c.dst + t] = c.src ? c.src[t] : t;
+                                                     uint32_t *__restrict__ out, const uint32_t *__restrict__ ncopy,
+                                                     const uint32_t *__restrict__ abort) {
+  if (abort && *abort) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t total = ncopy ? *ncopy : n;
+  for (uint32_t w = blockIdx.x * 4 + threadIdx.x / 64; w < total; w += gridDim.x * 4) {
+    const RangeCopy c = copies[w];
+    for (uint32_t t = lane; t < c.len; t += 64) out[c.dst + t] = c.src ? c.src[t] : t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3214,7 +3238,7 @@ static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n
   }
   Buf<uint32_t> hc(ctx, n), hpos(ctx, n + 1), htot(ctx, 1);
   launch(ctx, "range_heavy_counts", k_range_heavy_counts, dim3(div_up(n, 256)), dim3(256), 0, qlist, n,
-         (const uint64_t *)r->d_offsets, hc.p);
+         (const uint64_t *)r->d_offsets, hc.p, (const uint32_t *)nullptr);
   exclusive_scan_u32(ctx, hc.p, hpos.p, n, htot.p);
   uint32_t nh = 0;
   OTG_HIP_CHECK(hipMemcpyAsync(&nh, htot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -3224,10 +3248,10 @@ static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n
   Buf<uint64_t> keys(ctx, nh);
   Buf<uint32_t> vals(ctx, nh);
   launch(ctx, "range_sort_keys", k_range_sort_keys, dim3(n), dim3(256), 0, qlist, n, (const uint64_t *)r->d_offsets,
-         (const uint32_t *)hpos.p, (const uint32_t *)r->d_ids, keys.p);
+         (const uint32_t *)hpos.p, (const uint32_t *)r->d_ids, keys.p, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   radix_sort_pairs(ctx, keys.p, vals.p, nh, 32 + std::max(1u, bit_width32(n - 1)));
   launch(ctx, "range_sort_back", k_range_sort_back, dim3(n), dim3(256), 0, qlist, n, (const uint64_t *)r->d_offsets,
-         (const uint32_t *)hpos.p, (const uint64_t *)keys.p, r->d_ids);
+         (const uint32_t *)hpos.p, (const uint64_t *)keys.p, r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
 }
 
 template <int D, bool BOX>
@@ -3252,7 +3276,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
     const uint32_t ncap = flagged_list(ctx, capped.p, ord, nq, clist);
     launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(ncap), dim3(64), 0, vs.views, vs.qview, q, qs,
            (const uint32_t *)clist.p, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
-           (RangeCopy *)nullptr);
+           (RangeCopy *)nullptr, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   } else {
     OTG_HIP_CHECK(hipMemsetAsync(capped.p, 0, std::max<uint64_t>(nq, 1) * 4, ctx->stream));
   }
@@ -3279,13 +3303,13 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   Buf<uint32_t> heavy(ctx, std::max<uint64_t>(nq, 1));
   launch(ctx, "range_inline", k_range_inline, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq,
          (const uint32_t *)cnt.p, (const uint32_t *)capped.p, (const uint64_t *)r->d_offsets, (const uint32_t *)inl.p,
-         r->d_ids, heavy.p);
+         r->d_ids, heavy.p, (const uint32_t *)nullptr);
   Buf<uint32_t> hlist;
   const uint32_t nheavy = flagged_list(ctx, heavy.p, ord, nq, hlist);
   if constexpr (kWave)
     launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(nheavy), dim3(64), 0, vs.views, vs.qview, q, qs,
            (const uint32_t *)hlist.p, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
-           ncopy.p, copies.p);
+           ncopy.p, copies.p, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   else
     launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nheavy, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
            qs, (const uint32_t *)hlist.p, (uint64_t)nheavy, mode, tol, (uint32_t *)nullptr,
@@ -3294,7 +3318,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   OTG_HIP_CHECK(hipMemcpyAsync(&hcopy, ncopy.p, 4, hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
   launch(ctx, "range_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
-         r->d_ids);
+         r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   // 6. the larger results sorted by id
   sort_segments(ctx, r, hlist.p, nheavy);
   return r;
@@ -3407,7 +3431,7 @@ static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doubl
                  ncopy.p, copies.p);
       });
   launch(ctx, "planes_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
-         r->d_ids);
+         r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   // the wavefront walks append in no fixed order: every list is returned sorted by id
   sort_segments(ctx, r, nullptr, (uint32_t)nq);
   return r;
@@ -4038,11 +4062,218 @@ void knn_plan_run(KnnPlan *p) {
   knn_plan_after(p);
 }
 
+// Range plan: one RangeSearch batch on a 2-D / 3-D tree captured as a HIP graph with
+// every buffer allocated up front.  The result's id capacity comes from an eager run of
+// the batch at creation (ids + ids / 8 + 65,536); deferred runs hold more than
+// kInlineCopy ids each and the count walks count a capped query's runs twice, so
+// 2 x ids / (kInlineCopy + 1) bounds them.  Counts stay on the device: the capped and
+// heavy lists drive bounded grids that stride over them, and a total beyond a capacity
+// sets an abort word that every writing kernel checks (the run then reports
+// OTG_ERR_CAPACITY and leaves every list empty).
+struct RangePlan {
+  const Tree *t = nullptr;  // the index walked
+  Ctx *ctx = nullptr;
+  uint64_t nq = 0;
+  const double *q = nullptr;
+  const uint32_t *sets = nullptr;
+  int mode = 0;
+  double tol = 0.0;
+  uint64_t cap_ids = 0, cap_copies = 0;
+  char *mem = nullptr;
+  // [0] bad set id, [1] id total, [2] capped queries, [3] heavy queries, [4] runs of the
+  // count walks, [5] abort, [6] ids of heavy queries, [7] running run count
+  uint32_t *flags = nullptr;
+  uint32_t *h_flags = nullptr;
+  uint64_t *offsets = nullptr;
+  uint32_t *ids = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~RangePlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_flags) (void)hipHostFree(h_flags);
+    if (ctx && mem) ctx->free(mem);
+  }
+};
+
+constexpr uint32_t kPlanListGrid = 16384;  // blocks striding over a device-side query list
+
+// flags[4] = the count walks' run count, then the running count restarts for the fill;
+// abort when the ids or the runs outgrow the plan
+__global__ void k_range_plan_check(uint32_t *__restrict__ flags, uint64_t cap_ids, uint64_t cap_copies) {
+  flags[4] = flags[7];
+  flags[7] = 0u;
+  flags[5] = ((uint64_t)flags[1] > cap_ids || (uint64_t)flags[4] > cap_copies) ? 1u : 0u;
+}
+
+template <int D, bool BOX>
+static void range_plan_body(RangePlan &p, Arena &ar) {
+  static_assert(D <= 3, "range plans walk 2-D / 3-D trees");
+  Ctx *ctx = p.ctx;
+  const uint64_t nq = p.nq;
+  ViewSet vs = single_view(p.t);
+  uint32_t *F = p.flags;
+  fill_u32(ctx, F, 8, 0u);
+  const uint32_t *qs = nullptr;
+  if (p.sets) {
+    Buf<uint32_t> clean(&ar, nq);
+    launch(ctx, "check_sets", k_sanitize_sets, dim3(std::min<uint64_t>(div_up(nq, 256), 1024)), dim3(256), 0, p.sets,
+           nq, (uint32_t)p.t->sets.size(), clean.p, F + 0);
+    qs = clean.p;
+  }
+  Buf<uint32_t> order = query_order<D>(ctx, vs, p.q, 2 * D, qs, nq, &ar);
+  const uint32_t *ord = order.p;
+  Buf<uint32_t> cnt(&ar, nq), inl(&ar, nq * kRangeInline), capped(&ar, nq), pos(&ar, nq), clist(&ar, nq);
+  Buf<uint32_t> scan(&ar, nq + 1), heavy(&ar, nq), hpos0(&ar, nq), hlist(&ar, nq), hc(&ar, nq), hpos(&ar, nq);
+  Buf<RangeCopy> copies(&ar, p.cap_copies);
+  Buf<uint64_t> keys(&ar, p.cap_ids);
+  Buf<uint32_t> vals(&ar, p.cap_ids);
+  const uint32_t G = (uint32_t)std::min<uint64_t>(nq, kPlanListGrid);
+  // 1. one-lane count walks; the capped ones counted by a wavefront each
+  launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, p.q, qs,
+         ord, nq, p.mode, p.tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, F + 7, (RangeCopy *)nullptr,
+         inl.p, kRangeVisitCap);
+  launch(ctx, "range_capped", k_range_capped, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq, (const uint32_t *)cnt.p,
+         capped.p);
+  exclusive_scan_u32(ctx, capped.p, pos.p, nq, F + 2, &ar);
+  launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, (const uint32_t *)capped.p,
+         (const uint32_t *)pos.p, nq, ord, clist.p);
+  launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(G), dim3(64), 0, vs.views, vs.qview, p.q, qs,
+         (const uint32_t *)clist.p, p.mode, p.tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, F + 7,
+         (RangeCopy *)nullptr, (const uint32_t *)(F + 2), (const uint32_t *)nullptr);
+  // 2. offsets and the capacity check
+  exclusive_scan_u32(ctx, cnt.p, scan.p, nq, F + 1, &ar);
+  launch(ctx, "offsets_dev", k_offsets_dev, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq,
+         (const uint32_t *)(F + 1), p.offsets);
+  launch(ctx, "range_plan_check", k_range_plan_check, dim3(1), dim3(1), 0, F, p.cap_ids, p.cap_copies);
+  const uint32_t *abort = F + 5;
+  // 3. small results from the inline rows; the heavy ones filled by a wavefront each
+  launch(ctx, "range_inline", k_range_inline, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq, (const uint32_t *)cnt.p,
+         (const uint32_t *)capped.p, (const uint64_t *)p.offsets, (const uint32_t *)inl.p, p.ids, heavy.p, abort);
+  exclusive_scan_u32(ctx, heavy.p, hpos0.p, nq, F + 3, &ar);
+  launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, (const uint32_t *)heavy.p,
+         (const uint32_t *)hpos0.p, nq, ord, hlist.p);
+  launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(G), dim3(64), 0, vs.views, vs.qview, p.q, qs,
+         (const uint32_t *)hlist.p, p.mode, p.tol, (uint32_t *)nullptr, (const uint64_t *)p.offsets, p.ids, F + 7,
+         copies.p, (const uint32_t *)(F + 3), abort);
+  // 4. long contained runs, one wavefront each
+  launch(ctx, "range_copy", k_range_copy, dim3(kPlanListGrid / 4), dim3(256), 0, (const RangeCopy *)copies.p, 0u, p.ids,
+         (const uint32_t *)(F + 7), abort);
+  // 5. the heavy results sorted by id: one radix sort of (rank in the list, id) keys
+  launch(ctx, "range_heavy_counts", k_range_heavy_counts, dim3(div_up(nq, 256)), dim3(256), 0,
+         (const uint32_t *)hlist.p, 0u, (const uint64_t *)p.offsets, hc.p, (const uint32_t *)(F + 3));
+  exclusive_scan_u32(ctx, hc.p, hpos.p, DevN{nq, F + 3}, F + 6, &ar);
+  launch(ctx, "range_sort_keys", k_range_sort_keys, dim3(G), dim3(256), 0, (const uint32_t *)hlist.p, 0u,
+         (const uint64_t *)p.offsets, (const uint32_t *)hpos.p, (const uint32_t *)p.ids, keys.p,
+         (const uint32_t *)(F + 3), abort);
+  radix_sort_pairs(ctx, keys.p, vals.p, DevN{p.cap_ids, F + 6}, 32 + std::max(1u, bit_width32((uint32_t)(nq - 1))), &ar);
+  launch(ctx, "range_sort_back", k_range_sort_back, dim3(G), dim3(256), 0, (const uint32_t *)hlist.p, 0u,
+         (const uint64_t *)p.offsets, (const uint32_t *)hpos.p, (const uint64_t *)keys.p, p.ids,
+         (const uint32_t *)(F + 3), abort);
+  OTG_HIP_CHECK(hipMemcpyAsync(p.h_flags, F, 32, hipMemcpyDeviceToHost, ctx->stream));
+}
+
+template <int D, bool BOX>
+static void range_plan_capture(RangePlan &p) {
+  Ctx *ctx = p.ctx;
+  const uint64_t nq = p.nq;
+  OTG_HIP_CHECK(hipHostMalloc(&p.h_flags, 32, hipHostMallocDefault));
+  std::memset(p.h_flags, 0, 32);
+  const uint64_t tiles = nq / 4096 + 2, id_tiles = p.cap_ids / 4096 + 2;
+  size_t bytes = nq * (4 * 14 + 4 * kRangeInline + 8 + 12 + 12) + tiles * 2048 * 2 + p.cap_ids * (4 + 8 + 4 + 12) +
+                 id_tiles * 2048 + p.cap_copies * sizeof(RangeCopy) + (8u << 20);
+  for (;; bytes *= 2) {
+    p.mem = (char *)ctx->alloc(bytes);
+    Arena ar{p.mem, bytes, 0};
+    p.flags = (uint32_t *)ar.take(32);
+    p.offsets = (uint64_t *)ar.take((nq + 1) * 8);
+    p.ids = (uint32_t *)ar.take(std::max<uint64_t>(p.cap_ids, 1) * 4);
+    ctx->sync();
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    OTG_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    bool short_arena = false;
+    try {
+      range_plan_body<D, BOX>(p, ar);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(ctx->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      ctx->profiling = prof;
+      short_arena = ar.exhausted;
+      if (!short_arena) throw;
+    }
+    if (short_arena) {
+      ctx->free(p.mem);
+      p.mem = nullptr;
+      continue;
+    }
+    ctx->profiling = prof;
+    OTG_HIP_CHECK(hipStreamEndCapture(ctx->stream, &p.graph));
+    break;
+  }
+  OTG_HIP_CHECK(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+}
+
+static void range_plan_after(RangePlan *p) {
+  const uint32_t *f = p->h_flags;
+  if (f[0]) throw Error{OTG_ERR_INVALID_ARG, "a query's set id is not a set of the tree"};
+  if (f[1] == kScanOverflow) throw Error{OTG_ERR_UNSUPPORTED, "a batch result of 2^32 - 1 or more ids"};
+  if (f[5]) throw Error{OTG_ERR_CAPACITY, "the batch's results outgrow the plan (make a new plan)"};
+}
+
+RangePlan *range_plan_create(const Tree *tree, uint64_t nq, const double *q, const uint32_t *sets, int mode,
+                             double tol) {
+  if (nq == 0 || nq >= 0x7FFFFFFFull) throw Error{OTG_ERR_INVALID_ARG, "plan query count"};
+  if (tree->dim > 3) throw Error{OTG_ERR_UNSUPPORTED, "range plans walk 2-D and 3-D trees"};
+  std::unique_ptr<RangePlan> p(new RangePlan());
+  p->t = range_index(tree);
+  p->ctx = tree->ctx;
+  p->nq = nq;
+  p->q = q;
+  p->sets = sets;
+  p->mode = mode;
+  p->tol = tol;
+  if (p->t->n == 0) throw Error{OTG_ERR_INVALID_ARG, "plan on an empty tree"};
+  {  // the batch's current contents size the plan
+    std::unique_ptr<Result> r(query_range(tree, nq, q, sets, OTG_MEM_DEVICE, mode, tol));
+    p->cap_ids = r->nids + r->nids / 8 + 65536;
+  }
+  p->cap_copies = 2 * p->cap_ids / (kInlineCopy + 1) + 1024;
+  const bool box = p->t->kind == OTG_KIND_BOX;
+  if (p->t->dim == 2) {
+    if (box) range_plan_capture<2, true>(*p);
+    else range_plan_capture<2, false>(*p);
+  } else {
+    if (box) range_plan_capture<3, true>(*p);
+    else range_plan_capture<3, false>(*p);
+  }
+  return p.release();
+}
+
+void range_plan_run(RangePlan *p) {
+  Ctx *ctx = p->ctx;
+  OTG_HIP_CHECK(hipGraphLaunch(p->exec, ctx->stream));
+  ctx->sync();
+  range_plan_after(p);
+}
+
+void range_plan_result(const RangePlan *p, const uint64_t **off, const uint32_t **ids, uint64_t *nids) {
+  *off = p->offsets;
+  *ids = p->ids;
+  *nids = p->h_flags[5] ? 0 : p->h_flags[1];
+}
+
+void range_plan_free(RangePlan *p) { delete p; }
+Ctx *range_plan_ctx(RangePlan *p) { return p->ctx; }
+uint64_t range_plan_queries(const RangePlan *p) { return p->nq; }
+
 // A build plan and a kNN plan on its tree as one graph: the build's graph, then the
 // kNN batch's, as child graphs in that order.
 struct StepPlan {
   BuildPlan *bp = nullptr;
-  KnnPlan *kp = nullptr;
+  KnnPlan *kp = nullptr;    // or null
+  RangePlan *rp = nullptr;  // or null
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~StepPlan() {
@@ -4051,32 +4282,40 @@ struct StepPlan {
   }
 };
 
-StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp) {
-  if (kp->t != knn_index(build_plan_tree(bp)))
-    throw Error{OTG_ERR_INVALID_ARG, "the kNN plan must be made on the build plan's tree"};
+StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp, RangePlan *rp) {
+  const Tree *t = build_plan_tree(bp);
+  if (kp && kp->t != knn_index(t)) throw Error{OTG_ERR_INVALID_ARG, "the kNN plan must be made on the build plan's tree"};
+  if (rp && rp->t != range_index(t))
+    throw Error{OTG_ERR_INVALID_ARG, "the range plan must be made on the build plan's tree"};
   std::unique_ptr<StepPlan> st(new StepPlan());
   st->bp = bp;
   st->kp = kp;
+  st->rp = rp;
   OTG_HIP_CHECK(hipGraphCreate(&st->graph, 0));
-  hipGraphNode_t nb = nullptr, nk = nullptr;
-  OTG_HIP_CHECK(hipGraphAddChildGraphNode(&nb, st->graph, nullptr, 0, build_plan_graph(bp)));
-  OTG_HIP_CHECK(hipGraphAddChildGraphNode(&nk, st->graph, &nb, 1, kp->graph));
+  hipGraphNode_t prev = nullptr;
+  for (hipGraph_t g : {build_plan_graph(bp), kp ? kp->graph : nullptr, rp ? rp->graph : nullptr}) {
+    if (!g) continue;
+    hipGraphNode_t node = nullptr;
+    OTG_HIP_CHECK(hipGraphAddChildGraphNode(&node, st->graph, prev ? &prev : nullptr, prev ? 1 : 0, g));
+    prev = node;
+  }
   OTG_HIP_CHECK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
   return st.release();
 }
 
 void step_plan_run(StepPlan *st) {
-  Ctx *ctx = st->kp->ctx;
-  st->kp->band_result.reset();
+  Ctx *ctx = build_plan_tree(st->bp)->ctx;
+  if (st->kp) st->kp->band_result.reset();
   OTG_HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
   ctx->sync();
   build_plan_finish(st->bp);
-  knn_plan_after(st->kp);
+  if (st->kp) knn_plan_after(st->kp);
+  if (st->rp) range_plan_after(st->rp);
 }
 
 void step_plan_free(StepPlan *st) { delete st; }
 
-Ctx *step_plan_ctx(StepPlan *st) { return st->kp->ctx; }
+Ctx *step_plan_ctx(StepPlan *st) { return build_plan_tree(st->bp)->ctx; }
 
 void knn_plan_result(const KnnPlan *p, const uint64_t **off, const uint32_t **ids, uint64_t *nids) {
   if (p->band_result) {

@@ -4,8 +4,8 @@ replay must equal a fresh otg_forest_build of the same contents (itself held to 
 oracle in test_gpu_build and test_gpu_configs): node tables field by field for every
 set, and the kNN / range answers.  New contents that need more nodes than the plan
 holds fail with OTG_ERR_CAPACITY and leave the tree without nodes (queries answer
-empty, nothing faults); the step plan (build, then a kNN batch, as one graph) equals a
-fresh build followed by an eager kNN call."""
+empty, nothing faults); the step plan (build, then a kNN batch and a range batch, as one
+graph) equals a fresh build followed by eager kNN and range calls."""
 import numpy as np
 import pytest
 
@@ -135,7 +135,7 @@ def test_build_plan_capacity_overflow(gpu_ctx):
         plan.free()
 
 
-def test_step_plan_equals_build_then_knn(gpu_ctx):
+def test_step_plan_equals_build_then_queries(gpu_ctx):
     import torch
 
     import octree_amd
@@ -147,23 +147,34 @@ def test_step_plan_equals_build_then_knn(gpu_ctx):
     q = torch.zeros((nq, D), dtype=torch.float64, device="cuda")
     sid = torch.zeros(nq, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
+    rq = torch.zeros((nq, 2 * D), dtype=torch.float64, device="cuda")
+    hq0, hs0, hr0 = _queries(199, nsets, D, nq)  # contents of the kind the replays get
+    q.copy_(torch.from_numpy(hq0))
+    rq.copy_(torch.from_numpy(hr0))
+    sid.copy_(torch.from_numpy(hs0.astype(np.int32)))
+    torch.cuda.synchronize()
     plan = octree_amd.BuildPlan(pts, offs, cap, boxes, None, ctx=gpu_ctx, dim=D)
     try:
         kp = plan.tree.knn_plan(q, k, set_ids=sid)
-        step = plan.step(kp)
+        rp = plan.tree.range_plan(rq, set_ids=sid)
+        step = plan.step(kp, rp)
         for rep in range(3):
-            hq, hs, _ = _queries(200 + rep, nsets, D, nq)
+            hq, hs, hr = _queries(200 + rep, nsets, D, nq)
             if rep:
                 pts.copy_(torch.from_numpy(_refill(300 + rep, offs, D, True)))
             q.copy_(torch.from_numpy(hq))
+            rq.copy_(torch.from_numpy(hr))
             sid.copy_(torch.from_numpy(hs.astype(np.int32)))
             torch.cuda.synchronize()
-            got = step.run().knn.csr()
+            step.run()
+            got, got_r = kp.csr(), rp.csr()
             fresh = octree_amd.StaticTree.forest(None, cap, None, boxes, ctx=gpu_ctx, dim=D, kind="point",
                                                  device_geom=(pts, offs))
             try:
                 want = fresh.knn_batch(hq, k, set_ids=hs).csr()
                 assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), f"step rep {rep}"
+                want_r = fresh.range_batch(hr, set_ids=hs).csr()
+                assert np.array_equal(got_r[0], want_r[0]) and np.array_equal(got_r[1], want_r[1]), f"range rep {rep}"
                 _same_tables(plan.tree, fresh, nsets)
             finally:
                 fresh.free()

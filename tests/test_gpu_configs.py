@@ -3,8 +3,9 @@
 * batched uneven forest: the 256-set, 39.2M-point workload bench.py times, at the
   leaf capacity its autotune picks (64) and at 8: every set's node table, and
   2,000 kNN (k = 16) and 2,000 RangeSearch queries per set out of the full
-  12.8M-query launches, against the oracle; the same build and kNN launch replayed
-  as one graph (build plan + kNN plan) give the same tables and ids;
+  12.8M-query launches, against the oracle; the same build, kNN and range launches
+  replayed as one graph (build plan + kNN plan + range plan) give the same tables and
+  ids;
 * OctreeBox: 4,000,000 boxes, depth 8, capacity 32: node table, the whole
   self-collision pair list, and 100,000 rays first-hit;
 * TreePointND<6>: 8,000,000 points, depth 8, capacity 32: node table and a
@@ -81,10 +82,12 @@ def test_batched_forest_at_bench_caps(gpu_ctx, batched, cap):
     plan = octree_amd.BuildPlan(d_pts, offs, cap, box, None, ctx=gpu_ctx, dim=3)
     try:
         kp = plan.tree.knn_plan(d_knn, 16, set_ids=d_sid)
-        step = plan.step(kp)
+        rp = plan.tree.range_plan(d_rng, set_ids=d_sid)
+        step = plan.step(kp, rp)
         for rep in range(2):
             step.run()
             _csr_equal(kp.csr(), (koff, kids), f"graph kNN cap {cap}, replay {rep}")
+            _csr_equal(rp.csr(), (roff, rids), f"graph range cap {cap}, replay {rep}")
         for s in range(nsets):
             compare_tables_fast(plan.tree.export(s), trees[s].export(), False)
     finally:
