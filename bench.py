@@ -309,9 +309,11 @@ def main():
     plan.free()
     fg.free()
     log(f"kNN batch: eager {e_ms * 1e3:.2f} ms, graph replay {g_ms * 1e3:.2f} ms")
-    sg = step_graph(octree_amd, ctx, d_pts, offs, cap, box, None, d_knn, args.k, 3, max(3, args.steps), set_ids=d_sid)
-    log(f"build + kNN: eager {sg['eager_build_knn_ms']:.2f} ms, one graph {sg['graph_build_knn_ms']:.2f} ms; "
-        f"build alone: eager {sg['eager_build_ms']:.2f} ms, graph {sg['graph_build_ms']:.2f} ms")
+    sg = step_graph(octree_amd, ctx, d_pts, offs, cap, box, None, d_knn, args.k, 3, max(3, args.steps), set_ids=d_sid,
+                    d_range=d_rng)
+    log(f"build + kNN + range: eager {sg['eager_build_knn_range_ms']:.2f} ms, one graph "
+        f"{sg['graph_build_knn_range_ms']:.2f} ms; build alone: eager {sg['eager_build_ms']:.2f} ms, graph "
+        f"{sg['graph_build_ms']:.2f} ms")
 
     # ---- per-kernel times (HIP events on the library stream) for the roofline ----
     ctx.set_profiling(True)
@@ -449,15 +451,17 @@ def _timed(ctx, fn, reps):
     return float(np.median(ts)), ts
 
 
-def step_graph(octree_amd, ctx, d_pts, offs, cap, boxes, depths, d_q, k, dim, reps, set_ids=None):
-    """Build + kNN batch replayed as one HIP graph (otg_build_plan and otg_knn_plan joined
-    as an otg_step_plan) beside the same two calls made eagerly (otg_forest_build, then
-    otg_knn), and the build plan replayed alone.  Both sides start from the points and
-    queries resident in HBM; the graph side rebuilds the tree in place every replay."""
+def step_graph(octree_amd, ctx, d_pts, offs, cap, boxes, depths, d_q, k, dim, reps, set_ids=None, d_range=None):
+    """Build + kNN batch (+ range batch) replayed as one HIP graph (otg_build_plan,
+    otg_knn_plan and otg_range_plan joined as an otg_step_plan) beside the same calls made
+    eagerly (otg_forest_build, otg_knn, otg_range), and the build plan replayed alone.
+    Both sides start from the points and queries resident in HBM; the graph side rebuilds
+    the tree in place every replay."""
     bp = octree_amd.BuildPlan(d_pts, offs, cap, boxes, depths, ctx=ctx, dim=dim)
     try:
         kp = bp.tree.knn_plan(d_q, k, set_ids=set_ids)
-        sp = bp.step(kp)
+        rp = None if d_range is None else bp.tree.range_plan(d_range, set_ids=set_ids)
+        sp = bp.step(kp, rp)
         g_ms, _ = _timed(ctx, lambda: sp.run() and None, reps)
         gb_ms, _ = _timed(ctx, lambda: bp.run() and None, reps)
 
@@ -465,8 +469,11 @@ def step_graph(octree_amd, ctx, d_pts, offs, cap, boxes, depths, d_q, k, dim, re
             f = octree_amd.StaticTree.forest(None, cap, depths, boxes, ctx=ctx, dim=dim, kind="point",
                                              device_geom=(d_pts, offs))
             r = f.knn_batch(d_q, k, set_ids=set_ids)
+            rr = None if d_range is None else f.range_batch(d_range, set_ids=set_ids)
             ctx.synchronize()
             r.free()
+            if rr is not None:
+                rr.free()
             f.free()
 
         e_ms, _ = _timed(ctx, eager, reps)
@@ -480,7 +487,8 @@ def step_graph(octree_amd, ctx, d_pts, offs, cap, boxes, depths, d_q, k, dim, re
         eb_ms, _ = _timed(ctx, eager_build, reps)
     finally:
         bp.free()
-    return {"eager_build_knn_ms": e_ms * 1e3, "graph_build_knn_ms": g_ms * 1e3, "eager_build_ms": eb_ms * 1e3,
+    what = "build_knn" if d_range is None else "build_knn_range"
+    return {f"eager_{what}_ms": e_ms * 1e3, f"graph_{what}_ms": g_ms * 1e3, "eager_build_ms": eb_ms * 1e3,
             "graph_build_ms": gb_ms * 1e3}
 
 
