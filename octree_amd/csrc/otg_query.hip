@@ -199,6 +199,32 @@ struct MergeNet {
   }
 };
 
+// Where a walk leaves its list: `ids`, KCAP slots per query; or, when `direct` is set
+// (a batch with one k for every query) and the list holds exactly k ids, straight into
+// the rows of the batch's CSR (row qi at qi * k), so a batch in which every query has k
+// ids needs no compaction.  A list of another length goes to `ids` and sets
+// *irregular.
+struct KnnOut {
+  uint32_t *ids;
+  uint32_t *direct;
+  uint32_t *irregular;
+};
+
+template <int KCAP>
+__device__ __forceinline__ void knn_write(const KnnOut &o, uint64_t qi, const double (&bd)[KCAP],
+                                          const uint32_t (&bid)[KCAP], uint32_t cnt, uint32_t k) {
+  if (o.direct && cnt == k) {
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t)
+      if ((uint32_t)t < k) o.direct[qi * k + t] = bid[t];
+    return;
+  }
+  if (o.direct) *o.irregular = 1u;
+#pragma unroll
+  for (int t = 0; t < KCAP; ++t)
+    if ((uint32_t)t < cnt) o.ids[qi * KCAP + t] = bid[t];
+}
+
 template <int KCAP>
 __device__ __forceinline__ void knn_init(KnnState<KCAP> &S, double F0) {
   S.cnt = 0;
@@ -442,7 +468,7 @@ constexpr int kLdsStack = kMaxDepth;
 template <int D, int KCAP, bool BOX, typename Stack>
 __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                           const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
-                                          Stack &S, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                          Stack &S, KnnOut out_ids, uint32_t *__restrict__ out_cnt,
                                           uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                                           double *__restrict__ out_F) {
   const uint32_t s = qset ? qset[qi] : 0u;
@@ -499,9 +525,7 @@ __device__ __forceinline__ bool knn_query(const TreeView &tv, uint64_t qi, const
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
   knn_tie_order(bd, bid, L.cnt);
-#pragma unroll
-  for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
+  knn_write(out_ids, qi, bd, bid, L.cnt, k);
   return true;
 }
 
@@ -617,7 +641,7 @@ struct LdsLevelStack {
 template <int D, int KCAP, bool BOX>
 __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                                  const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
-                                                 LdsLevelStack &S, uint32_t *__restrict__ out_ids,
+                                                 LdsLevelStack &S, KnnOut out_ids,
                                                  uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow,
                                                  uint32_t *__restrict__ band, double *__restrict__ out_F) {
   static_assert(D <= 3, "packed child order needs D * 2^D <= 28 bits");
@@ -716,9 +740,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
   knn_tie_order(bd, bid, L.cnt);
-#pragma unroll
-  for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
+  knn_write(out_ids, qi, bd, bid, L.cnt, k);
   return true;
 }
 
@@ -740,7 +762,7 @@ template <int D, int KCAP, bool BOX>
 __device__ __forceinline__ bool knn_query_masks(const TreeView &tv, uint64_t qi, const double *__restrict__ qpts,
                                                 const uint32_t *__restrict__ qset, uint32_t k, double F0, double tol,
                                                 uint32_t *s_cb, uint32_t *s_lo, uint32_t *s_hi,
-                                                uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                                KnnOut out_ids, uint32_t *__restrict__ out_cnt,
                                                 uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                                                 double *__restrict__ out_F) {
   static_assert(D >= 4 && D <= 6, "child masks of 2^D bits need D <= 6");
@@ -822,9 +844,7 @@ __device__ __forceinline__ bool knn_query_masks(const TreeView &tv, uint64_t qi,
   out_cnt[qi] = L.cnt;
   if (out_F) out_F[qi] = L.F;
   knn_tie_order(bd, bid, L.cnt);
-#pragma unroll
-  for (int t = 0; t < KCAP; ++t)
-    if ((uint32_t)t < L.cnt) out_ids[qi * KCAP + t] = bid[t];
+  knn_write(out_ids, qi, bd, bid, L.cnt, k);
   return true;
 }
 
@@ -841,7 +861,7 @@ __device__ __forceinline__ void knn_lane(const TreeView *__restrict__ views, con
                                          const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                                          const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
                                          uint64_t ti, uint32_t k, double F0, double tol,
-                                         uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                         KnnOut out_ids, uint32_t *__restrict__ out_cnt,
                                          uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                                          double *__restrict__ out_F, uint32_t *s_n, uint32_t *s_w) {
   constexpr int ML = MaskLevels<D>::value;
@@ -876,7 +896,7 @@ __global__ void __launch_bounds__(kQB, 3) k_knn(const TreeView *__restrict__ vie
                                               const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                                               const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order,
                                               uint64_t nq, uint32_t k, double F0, double tol,
-                                              uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                                              KnnOut out_ids, uint32_t *__restrict__ out_cnt,
                                               uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                                               double *__restrict__ out_F) {
   OTG_KNN_LDS(D)
@@ -895,7 +915,7 @@ __global__ void __launch_bounds__(kQB, 3)
     k_knn_persistent(const TreeView *__restrict__ views, const uint32_t *__restrict__ qview,
                      const double *__restrict__ qpts, const uint32_t *__restrict__ qset,
                      const uint32_t *__restrict__ qk, const uint32_t *__restrict__ order, uint64_t nq, uint32_t k,
-                     double F0, double tol, uint32_t *__restrict__ out_ids, uint32_t *__restrict__ out_cnt,
+                     double F0, double tol, KnnOut out_ids, uint32_t *__restrict__ out_cnt,
                      uint32_t *__restrict__ overflow, uint32_t *__restrict__ band, double *__restrict__ out_F,
                      uint32_t *__restrict__ tiles) {
   OTG_KNN_LDS(D)
@@ -2902,6 +2922,21 @@ static void check_id_total(uint32_t total, Result *owned = nullptr) {
   }
 }
 
+// CSR offsets of a batch in which every query has k ids
+__global__ void k_offsets_linear(uint64_t nq, uint32_t k, uint64_t *__restrict__ off) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i <= nq) off[i] = i * k;
+}
+
+// rows of the lists that had exactly k ids, back into the KCAP-slot lists
+__global__ void k_direct_to_fixed(const uint32_t *__restrict__ direct, const uint32_t *__restrict__ cnt, uint64_t nq,
+                                  uint32_t k, uint32_t kcap, uint32_t *__restrict__ fixed) {
+  const uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t qi = x / k;
+  if (qi >= nq || cnt[qi] != k) return;
+  fixed[qi * kcap + (x - qi * k)] = direct[x];
+}
+
 // Fixed-stride per-query results -> CSR
 static Result *fixed_to_csr(Ctx *ctx, const uint32_t *ids, const uint32_t *cnt, uint32_t stride, uint64_t nq) {
   Result *r = new Result();
@@ -2970,9 +3005,13 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
                         const uint32_t *qk, uint32_t k, double F0, double tol, uint64_t *footprint = nullptr,
                         Arena *ar = nullptr, CsrOut *co = nullptr) {
   Buf<uint32_t> order = ar ? Buf<uint32_t>() : query_order<D>(ctx, vs, q, D, qs, nq);
-  Buf<uint32_t> ids(ctx, ar, nq * KCAP), cnt(ctx, ar, nq), ovf(ctx, ar, 1), band(ctx, ar, nq);
+  Buf<uint32_t> ids(ctx, ar, nq * KCAP), cnt(ctx, ar, nq), ovf(ctx, ar, 2), band(ctx, ar, nq);
   Buf<double> Fq(ctx, ar, nq);
-  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 4, ctx->stream));  // band count
+  OTG_HIP_CHECK(hipMemsetAsync(ovf.p, 0, 8, ctx->stream));  // band count, irregular list length
+  // one k for the whole batch (the plain call): lists of exactly k ids go straight into
+  // the result's rows
+  Buf<uint32_t> direct = (!ar && !qk) ? Buf<uint32_t>(ctx, nq * k) : Buf<uint32_t>();
+  const KnnOut kout{ids.p, direct.p, ovf.p + 1};
   {
     if (ctx->tuning.persistent) {
       const void *kp = (const void *)k_knn_persistent<D, KCAP, BOX>;
@@ -2980,16 +3019,16 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
       Buf<uint32_t> tiles(ctx, ar, 1);
       launch(ctx, "fill_u32", k_fill_u32, dim3(1), dim3(64), 0, tiles.p, (uint64_t)1, 0u);
       launch(ctx, "knn", k_knn_persistent<D, KCAP, BOX>, dim3(persistent_grid(ctx, kp, kQB, pad, div_up(nq, kQB))),
-             dim3(kQB), pad, vs.views, vs.qview, q, qs, qk, (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p,
+             dim3(kQB), pad, vs.views, vs.qview, q, qs, qk, (const uint32_t *)order.p, nq, k, F0, tol, kout, cnt.p,
              ovf.p, band.p, Fq.p, tiles.p);
     } else {
       const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
       launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), pad, vs.views, vs.qview, q, qs, qk,
-             (const uint32_t *)order.p, nq, k, F0, tol, ids.p, cnt.p, ovf.p, band.p, Fq.p);
+             (const uint32_t *)order.p, nq, k, F0, tol, kout, cnt.p, ovf.p, band.p, Fq.p);
     }
   }
-  uint32_t hovf[1] = {0};
-  OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t hovf[2] = {0, 0};
+  OTG_HIP_CHECK(hipMemcpyAsync(hovf, ovf.p, 8, hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
   if (footprint) {
     Buf<unsigned long long> acc(ctx, 2);
@@ -3003,6 +3042,22 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
     footprint[1] += h[1];
   }
   const uint32_t nb = hovf[0];
+  if (direct.p) {
+    if (nb == 0 && hovf[1] == 0) {  // every query has exactly k ids: the rows are the result
+      Result *r = new Result();
+      r->ctx = ctx;
+      r->nq = nq;
+      r->nids = nq * k;
+      check_id_total(r->nids > 0xFFFFFFFEull ? kScanOverflow : (uint32_t)r->nids, r);
+      r->d_offsets = (uint64_t *)ctx->alloc((nq + 1) * 8);
+      launch(ctx, "offsets_linear", k_offsets_linear, dim3(div_up(nq + 1, 256)), dim3(256), 0, nq, k, r->d_offsets);
+      r->d_ids = direct.release();
+      return r;
+    }
+    // lists of other lengths: the rows join the fixed lists, then the general path
+    launch(ctx, "direct_to_fixed", k_direct_to_fixed, dim3(div_up(nq * k, 256)), dim3(256), 0,
+           (const uint32_t *)direct.p, (const uint32_t *)cnt.p, nq, k, (uint32_t)KCAP, ids.p);
+  }
   if (nb == 0) {
     if (co) {
       csr_into(ctx, ar, ids.p, cnt.p, KCAP, nq, *co);
@@ -3907,12 +3962,12 @@ static void knn_plan_body(KnnPlan &p, Arena &ar) {
     const size_t pad = occupancy_pad(ctx, kp, ctx->tuning.waves_per_simd);
     launch(ctx, "knn", k_knn_persistent<D, KCAP, BOX>, dim3(persistent_grid(ctx, kp, kQB, pad, div_up(nq, kQB))),
            dim3(kQB), pad, vs.views, vs.qview, p.q, qs, (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k,
-           p.F0, p.tol, p.ids_fixed, p.cnt, p.flags, p.band, p.Fq, p.flags + 3);
+           p.F0, p.tol, KnnOut{p.ids_fixed, nullptr, nullptr}, p.cnt, p.flags, p.band, p.Fq, p.flags + 3);
   } else {
     const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
     launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), pad, vs.views, vs.qview, p.q, qs,
-           (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, p.ids_fixed, p.cnt, p.flags,
-           p.band, p.Fq);
+           (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, KnnOut{p.ids_fixed, nullptr, nullptr},
+           p.cnt, p.flags, p.band, p.Fq);
   }
   exclusive_scan_u32(ctx, p.cnt, p.scan, nq, p.flags + 2, &ar);
   launch(ctx, "offsets_dev", k_offsets_dev, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)p.scan, nq,
