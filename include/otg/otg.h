@@ -186,7 +186,9 @@ OTG_API otg_status otg_tree_deserialize(otg_ctx *ctx, int kind, int dim, const u
  * index_capacity = leaf capacity of the finer kNN / range index (0: the tree's),
  * waves_per_simd = occupancy cap of the kNN walks (0: as registers allow),
  * order_queries = 1 to walk batches of >= 16,384 queries in Morton order,
- * persistent = 1 to run the kNN walks as a persistent launch.  Query results never
+ * persistent = 1 to run the kNN walks as a persistent launch, range_records = the
+ * records a RangeSearch wavefront count walk keeps per query for its fill (0: a
+ * batch's record budget over its capped queries, 1024 to 8192).  Query results never
  * depend on them. */
 typedef struct {
   uint32_t leaf_capacity;
@@ -194,7 +196,7 @@ typedef struct {
   uint32_t waves_per_simd;
   uint32_t order_queries;
   uint32_t persistent; /* 1: kNN walks as a persistent launch pulling 64-query tiles across sets */
-  uint32_t reserved;
+  uint32_t range_records; /* kept records per capped RangeSearch query (0: from the batch) */
   double step_ms; /* measured build + queries with these settings (otg_autotune) */
 } otg_tuning;
 

@@ -114,8 +114,8 @@ class Context:
         return {f: getattr(out, f) for f, _ in _abi.Tuning._fields_}
 
     def set_tuning(self, index_capacity: int = 0, waves_per_simd: int = 0, order_queries: int = 1,
-                   persistent: int = 0) -> None:
-        t = _abi.Tuning(0, index_capacity, waves_per_simd, order_queries, persistent, 0, 0.0)
+                   persistent: int = 0, range_records: int = 0) -> None:
+        t = _abi.Tuning(0, index_capacity, waves_per_simd, order_queries, persistent, range_records, 0.0)
         _check(_abi.lib().otg_ctx_set_tuning(self.handle, C.byref(t)), "otg_ctx_set_tuning", self)
 
     def tuning(self) -> dict:

@@ -80,7 +80,7 @@ class SetInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("leaf_capacity", C.c_uint32), ("index_capacity", C.c_uint32), ("waves_per_simd", C.c_uint32),
-                ("order_queries", C.c_uint32), ("persistent", C.c_uint32), ("reserved", C.c_uint32),
+                ("order_queries", C.c_uint32), ("persistent", C.c_uint32), ("range_records", C.c_uint32),
                 ("step_ms", C.c_double)]
 
 

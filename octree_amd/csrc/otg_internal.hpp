@@ -67,6 +67,7 @@ struct Tuning {
   uint32_t waves_per_simd = 0;  // occupancy cap of the kNN walks (0: as registers allow)
   uint32_t order_queries = 1;   // Morton-order batches of >= 16,384 queries before walking
   uint32_t persistent = 0;      // kNN walks as a persistent launch pulling 64-query tiles
+  uint32_t range_records = 0;   // kept records per capped RangeSearch query (0: from the batch)
 };
 
 class Ctx {

@@ -1239,6 +1239,37 @@ struct RangeCopy {
 };
 constexpr uint32_t kInlineCopy = 16;
 
+// What a wavefront count walk keeps for the fill (collect mode): one record per node
+// that adds ids -- a boundary node with the count of its entities that pass, a
+// contained subtree, or the whole set -- at its offset in the query's list.  Listed
+// query b (the b-th capped query) owns `cap` records, handed out by an LDS counter
+// (no global atomics); k_range_kept then writes each record with a wavefront (the
+// entity tests of a boundary node run again, the same tests, so the same ids).  The
+// number of records per query, `cap`, is a batch's record budget over its capped
+// queries, between kKeepMin and kKeepMax (or the context's range_records); records
+// beyond it go to an overflow area shared by the batch (xcap = 1024 queries' slots, at
+// most kKeepOverflow records, handed out by a global counter: only the few largest
+// queries reach it).  A query whose records do not all fit is filled by the fill walk instead
+// (ok[query] stays 0).
+struct RangeRec {
+  uint32_t node;
+  uint32_t local;  // offset in the query's list
+  uint32_t len;
+  uint32_t kind;   // kRecNode, kRecSubtree, kRecSet
+};
+constexpr uint32_t kRecNode = 0, kRecSubtree = 1, kRecSet = 2;
+struct RangeKeep {
+  RangeRec *recs;  // cap per listed query
+  uint32_t *nrec;  // per listed query: records in its own slots
+  uint32_t *ok;    // per query (by query index): kept whole
+  RangeRec *xrecs; // overflow area: xcap records
+  uint32_t *xq;    // the listed query of each overflow record
+  uint32_t *nx;    // overflow records handed out (zeroed before the walk)
+  uint32_t cap, xcap;
+};
+constexpr uint32_t kKeepMin = 1024, kKeepMax = 8192, kKeepOverflow = 1u << 20;
+constexpr uint64_t kKeepBudget = 16ull << 20;  // records per batch (256 MiB)
+
 template <bool FILL>
 __device__ __forceinline__ void range_take(uint32_t *__restrict__ out, uint64_t dst, const uint32_t *src, uint32_t len,
                                            uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies) {
@@ -1430,11 +1461,11 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
                                                     const uint64_t *__restrict__ offsets, uint32_t *__restrict__ out,
                                                     uint32_t *__restrict__ ncopy, RangeCopy *__restrict__ copies,
                                                     const uint32_t *__restrict__ nlist,
-                                                    const uint32_t *__restrict__ abort) {
+                                                    const uint32_t *__restrict__ abort, const RangeKeep keep) {
   static_assert(D <= 3, "frontier sized for up to 8 children per node");
   // one frontier, rewritten in place each round: fewer than 64 nodes of <= 8 children
   __shared__ uint32_t s_front[64 * 8];
-  __shared__ uint32_t s_nf[2], s_cnt, s_next;
+  __shared__ uint32_t s_nf[2], s_cnt, s_next, s_nrec, s_bad;
   __shared__ uint32_t s_a[kRangeStack * 64];
   __shared__ uint8_t s_b[kRangeStack * 64];
   if (abort && *abort) return;
@@ -1468,9 +1499,27 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     for (int d = 0; d < D; ++d) vol *= rmax[d] - rmin[d];
     start = root_in ? 1 : (vol > 0.0 ? 2 : 0);
   }
-  auto emit_run = [&](const uint32_t *src, uint32_t len) {
+  // collect mode (count walk with `keep`): slots of listed query b
+  const bool kept = !FILL && keep.recs;
+  RangeRec *const krecs = kept ? keep.recs + (uint64_t)b * keep.cap : nullptr;
+  auto keep_rec = [&](uint32_t node, uint32_t at, uint32_t len, uint32_t kind) {
+    const uint32_t r = atomicAdd(&s_nrec, 1u);
+    if (r < keep.cap) {
+      krecs[r] = RangeRec{node, at, len, kind};
+    } else {
+      const uint32_t x = atomicAdd(keep.nx, 1u);
+      if (x < keep.xcap) {
+        keep.xrecs[x] = RangeRec{node, at, len, kind};
+        keep.xq[x] = b;
+      } else {
+        s_bad = 1;
+      }
+    }
+  };
+  auto emit_run = [&](const uint32_t *src, uint32_t len, uint32_t node, uint32_t kind) {
     const uint32_t at = atomicAdd(&s_cnt, len);
-    range_take<FILL>(out, o + at, src, len, ncopy, copies);
+    range_take<FILL>(out, o + at, src, len, ncopy, copies);  // counted also when kept: the bound of a fill walk
+    if (kept) keep_rec(node, at, len, kind);
   };
   // node visit: whole subtree, or entities + children (returned through ni)
   auto visit = [&](uint32_t node, NodeInfo &ni) -> bool {
@@ -1479,22 +1528,31 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     if (!overlap) return false;
     ni = tv.info[node];
     if (contained) {
-      emit_run(tv.ids + ni.ebegin, tv.rend[node] - ni.ebegin);
+      emit_run(tv.ids + ni.ebegin, tv.rend[node] - ni.ebegin, node, kRecSubtree);
       return false;
     }
-    for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j)
-      if (range_entity<D, BOX>(tv, j, rmin, rmax, overlap_mode, tol)) {
-        const uint32_t at = atomicAdd(&s_cnt, 1u);
-        if (FILL) out[o + at] = tv.ids[j];
+    if (FILL) {
+      for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j)
+        if (range_entity<D, BOX>(tv, j, rmin, rmax, overlap_mode, tol)) out[o + atomicAdd(&s_cnt, 1u)] = tv.ids[j];
+    } else {
+      uint32_t c = 0;
+      for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j)
+        c += range_entity<D, BOX>(tv, j, rmin, rmax, overlap_mode, tol) ? 1u : 0u;
+      if (c) {
+        const uint32_t at = atomicAdd(&s_cnt, c);
+        if (kept) keep_rec(node, at, c, kRecNode);
       }
+    }
     return ni.ccount != 0;
   };
   if (lane == 0) {
     s_cnt = 0;
+    s_nrec = 0;
+    s_bad = 0;
     s_next = 0;
     s_nf[0] = 0;
     s_nf[1] = 0;
-    if (start == 1) emit_run(nullptr, (uint32_t)sp.ent_cnt);
+    if (start == 1) emit_run(nullptr, (uint32_t)sp.ent_cnt, 0u, kRecSet);
     if (start == 2) {
       s_front[0] = (uint32_t)sp.node_off;
       s_nf[0] = 1;
@@ -1537,7 +1595,81 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     }
   }
   __syncthreads();
-  if (!FILL && lane == 0) counts[qi] = s_cnt;
+  if (!FILL && lane == 0) {
+    counts[qi] = s_cnt;
+    if (kept) {
+      keep.nrec[b] = min(s_nrec, keep.cap);
+      keep.ok[qi] = s_bad ? 0u : 1u;
+    }
+  }
+  }
+}
+
+// One kept record into the CSR, by a wavefront.  A boundary node's entities are tested
+// again 64 at a time; the ones that pass are placed by their rank in the ballot, so the
+// record's ids fill exactly its `len` slots.
+template <int D, bool BOX>
+__device__ __forceinline__ void range_kept_rec(const TreeView &tv, const RangeRec rec, const double (&rmin)[D],
+                                               const double (&rmax)[D], int overlap_mode, double tol,
+                                               uint32_t *__restrict__ dst, uint32_t lane) {
+  if (rec.kind == kRecSet) {
+    for (uint32_t t = lane; t < rec.len; t += 64) dst[t] = t;
+  } else if (rec.kind == kRecSubtree) {
+    const uint32_t *src = tv.ids + tv.info[rec.node].ebegin;
+    for (uint32_t t = lane; t < rec.len; t += 64) dst[t] = src[t];
+  } else {
+    const NodeInfo ni = tv.info[rec.node];
+    uint32_t at = 0;
+    for (uint32_t base = 0; base < ni.ecount; base += 64) {
+      const uint32_t j = ni.ebegin + base + lane;
+      const bool pass = base + lane < ni.ecount && range_entity<D, BOX>(tv, j, rmin, rmax, overlap_mode, tol);
+      const uint64_t m = __ballot(pass);
+      if (pass) dst[at + __popcll(m & ((1ull << lane) - 1))] = tv.ids[j];
+      at += __popcll(m);
+    }
+  }
+}
+
+// Kept records into the CSR: a block per listed query (its own slots) and, strided over
+// the rest of the grid, the overflow records; a wavefront per record.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(256) k_range_kept(const TreeView *__restrict__ views,
+                                                     const uint32_t *__restrict__ qview,
+                                                     const double *__restrict__ ranges,
+                                                     const uint32_t *__restrict__ list, uint32_t nlist,
+                                                     int overlap_mode, double tol, const RangeKeep keep,
+                                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63;
+  auto load = [&](uint32_t b, uint64_t &qi, double (&rmin)[D], double (&rmax)[D]) {
+    qi = list[b];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      rmin[d] = ranges[qi * 2 * D + d];
+      rmax[d] = ranges[qi * 2 * D + D + d];
+    }
+  };
+  double rmin[D], rmax[D];
+  uint64_t qi;
+  if (blockIdx.x < nlist) {
+    const uint32_t b = blockIdx.x;
+    load(b, qi, rmin, rmax);
+    if (!keep.ok[qi]) return;
+    const TreeView &tv = views[qview ? qview[qi] : 0u];
+    const uint64_t o = off[qi];
+    const RangeRec *recs = keep.recs + (uint64_t)b * keep.cap;
+    const uint32_t nrec = keep.nrec[b];
+    for (uint32_t r = threadIdx.x / 64; r < nrec; r += 4)
+      range_kept_rec<D, BOX>(tv, recs[r], rmin, rmax, overlap_mode, tol, out + o + recs[r].local, lane);
+    return;
+  }
+  const uint32_t nx = min(*keep.nx, keep.xcap);
+  const uint32_t waves = (gridDim.x - nlist) * 4;
+  for (uint32_t x = (blockIdx.x - nlist) * 4 + threadIdx.x / 64; x < nx; x += waves) {
+    load(keep.xq[x], qi, rmin, rmax);
+    if (!keep.ok[qi]) continue;
+    const TreeView &tv = views[qview ? qview[qi] : 0u];
+    const RangeRec rec = keep.xrecs[x];
+    range_kept_rec<D, BOX>(tv, rec, rmin, rmax, overlap_mode, tol, out + off[qi] + rec.local, lane);
   }
 }
 
@@ -1548,7 +1680,8 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
 __global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, const uint32_t *__restrict__ cnt,
                                const uint32_t *__restrict__ capped, const uint64_t *__restrict__ off,
                                const uint32_t *__restrict__ inl, uint32_t *__restrict__ out,
-                               uint32_t *__restrict__ heavy, const uint32_t *__restrict__ abort) {
+                               uint32_t *__restrict__ heavy, const uint32_t *__restrict__ abort,
+                               uint32_t *__restrict__ walk, const uint32_t *__restrict__ ok) {
   const uint64_t ti = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (ti >= nq) return;
   if (abort && *abort) {  // results over the plan's capacity: nothing is written
@@ -1559,6 +1692,9 @@ __global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, 
   const uint32_t c = cnt[qi];
   const bool h = capped[ti] || c > kRangeInline;  // a capped walk left its row incomplete
   heavy[ti] = h;
+  // walk (when given, with ok): the heavy queries the fill walk still has to fill --
+  // not the capped ones whose count walk kept their whole lists
+  if (walk) walk[ti] = h && !(capped[ti] && ok[qi]);
   if (h || c == 0) return;
   uint32_t v[kRangeInline];
 #pragma unroll
@@ -3323,15 +3459,32 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
          ord, nq, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p, (RangeCopy *)nullptr,
          inl.p, kWave ? kRangeVisitCap : 0u);
   Buf<uint32_t> capped(ctx, std::max<uint64_t>(nq, 1));
+  // the capped queries' lists kept by their count walk (bounded slots per query; a
+  // query that outgrows them is filled by the fill walk instead)
+  Buf<uint32_t> keep_n, keep_ok, keep_xq, clist;
+  Buf<RangeRec> keep_recs;
+  RangeKeep keep{};
+  uint32_t ncap = 0;
   if constexpr (kWave) {
-    // 2. the capped ones: counted by a wavefront each
+    // 2. the capped ones: counted by a wavefront each, their ids kept
     launch(ctx, "range_capped", k_range_capped, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq, (const uint32_t *)cnt.p,
            capped.p);
-    Buf<uint32_t> clist;
-    const uint32_t ncap = flagged_list(ctx, capped.p, ord, nq, clist);
+    ncap = flagged_list(ctx, capped.p, ord, nq, clist);
+    const uint32_t want = ctx->tuning.range_records;
+    if (ncap && (uint64_t)ncap * (want ? want : kKeepMin) <= kKeepBudget) {
+      const uint32_t kcap = want ? want : (uint32_t)std::min<uint64_t>(kKeepMax, kKeepBudget / ncap);
+      const uint32_t xcap = (uint32_t)std::min<uint64_t>(kKeepOverflow, 1024ull * kcap);
+      keep_recs = Buf<RangeRec>(ctx, (uint64_t)ncap * kcap + xcap);
+      keep_n = Buf<uint32_t>(ctx, ncap + 1);
+      keep_ok = Buf<uint32_t>(ctx, std::max<uint64_t>(nq, 1));
+      keep_xq = Buf<uint32_t>(ctx, xcap);
+      OTG_HIP_CHECK(hipMemsetAsync(keep_n.p + ncap, 0, 4, ctx->stream));
+      keep = RangeKeep{keep_recs.p,  keep_n.p,  keep_ok.p, keep_recs.p + (uint64_t)ncap * kcap,
+                       keep_xq.p, keep_n.p + ncap, kcap,      xcap};
+    }
     launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(ncap), dim3(64), 0, vs.views, vs.qview, q, qs,
            (const uint32_t *)clist.p, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
-           (RangeCopy *)nullptr, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+           (RangeCopy *)nullptr, (const uint32_t *)nullptr, (const uint32_t *)nullptr, keep);
   } else {
     OTG_HIP_CHECK(hipMemsetAsync(capped.p, 0, std::max<uint64_t>(nq, 1) * 4, ctx->stream));
   }
@@ -3354,17 +3507,25 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   OTG_HIP_CHECK(hipMemsetAsync(ncopy.p, 0, 4, ctx->stream));
   launch(ctx, "offsets_u64", k_offsets_u64, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq, ht,
          r->d_offsets);
-  // 4. small complete results from the inline rows; the rest are filled by a second walk
-  Buf<uint32_t> heavy(ctx, std::max<uint64_t>(nq, 1));
+  // 4. small complete results from the inline rows; the kept ones from their slots;
+  // the rest are filled by a second walk
+  const bool collected = keep.recs != nullptr;
+  Buf<uint32_t> heavy(ctx, std::max<uint64_t>(nq, 1)), walk;
+  if (collected) walk = Buf<uint32_t>(ctx, std::max<uint64_t>(nq, 1));
   launch(ctx, "range_inline", k_range_inline, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq,
          (const uint32_t *)cnt.p, (const uint32_t *)capped.p, (const uint64_t *)r->d_offsets, (const uint32_t *)inl.p,
-         r->d_ids, heavy.p, (const uint32_t *)nullptr);
-  Buf<uint32_t> hlist;
+         r->d_ids, heavy.p, (const uint32_t *)nullptr, walk.p, (const uint32_t *)keep.ok);
+  Buf<uint32_t> hlist, wlist;
   const uint32_t nheavy = flagged_list(ctx, heavy.p, ord, nq, hlist);
+  const uint32_t nwalk = collected ? flagged_list(ctx, walk.p, ord, nq, wlist) : nheavy;
+  if (collected)
+    launch(ctx, "range_kept", k_range_kept<D, BOX>, dim3(ncap + 256), dim3(256), 0, vs.views, vs.qview, q,
+           (const uint32_t *)clist.p, ncap, mode, tol, keep, (const uint64_t *)r->d_offsets, r->d_ids);
   if constexpr (kWave)
-    launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(nheavy), dim3(64), 0, vs.views, vs.qview, q, qs,
-           (const uint32_t *)hlist.p, mode, tol, (uint32_t *)nullptr, (const uint64_t *)r->d_offsets, r->d_ids,
-           ncopy.p, copies.p, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+    launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(nwalk), dim3(64), 0, vs.views, vs.qview, q, qs,
+           (const uint32_t *)(collected ? wlist.p : hlist.p), mode, tol, (uint32_t *)nullptr,
+           (const uint64_t *)r->d_offsets, r->d_ids, ncopy.p, copies.p, (const uint32_t *)nullptr,
+           (const uint32_t *)nullptr, RangeKeep{});
   else
     launch(ctx, "range_fill", k_range<D, BOX, true>, dim3(div_up(nheavy, kQB)), dim3(kQB), 0, vs.views, vs.qview, q,
            qs, (const uint32_t *)hlist.p, (uint64_t)nheavy, mode, tol, (uint32_t *)nullptr,
@@ -4195,7 +4356,7 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
          (const uint32_t *)pos.p, nq, ord, clist.p);
   launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(G), dim3(64), 0, vs.views, vs.qview, p.q, qs,
          (const uint32_t *)clist.p, p.mode, p.tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, F + 7,
-         (RangeCopy *)nullptr, (const uint32_t *)(F + 2), (const uint32_t *)nullptr);
+         (RangeCopy *)nullptr, (const uint32_t *)(F + 2), (const uint32_t *)nullptr, RangeKeep{});
   // 2. offsets and the capacity check
   exclusive_scan_u32(ctx, cnt.p, scan.p, nq, F + 1, &ar);
   launch(ctx, "offsets_dev", k_offsets_dev, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq,
@@ -4204,13 +4365,14 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
   const uint32_t *abort = F + 5;
   // 3. small results from the inline rows; the heavy ones filled by a wavefront each
   launch(ctx, "range_inline", k_range_inline, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq, (const uint32_t *)cnt.p,
-         (const uint32_t *)capped.p, (const uint64_t *)p.offsets, (const uint32_t *)inl.p, p.ids, heavy.p, abort);
+         (const uint32_t *)capped.p, (const uint64_t *)p.offsets, (const uint32_t *)inl.p, p.ids, heavy.p, abort,
+         (uint32_t *)nullptr, (const uint32_t *)nullptr);
   exclusive_scan_u32(ctx, heavy.p, hpos0.p, nq, F + 3, &ar);
   launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, (const uint32_t *)heavy.p,
          (const uint32_t *)hpos0.p, nq, ord, hlist.p);
   launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(G), dim3(64), 0, vs.views, vs.qview, p.q, qs,
          (const uint32_t *)hlist.p, p.mode, p.tol, (uint32_t *)nullptr, (const uint64_t *)p.offsets, p.ids, F + 7,
-         copies.p, (const uint32_t *)(F + 3), abort);
+         copies.p, (const uint32_t *)(F + 3), abort, RangeKeep{});
   // 4. long contained runs, one wavefront each
   launch(ctx, "range_copy", k_range_copy, dim3(kPlanListGrid / 4), dim3(256), 0, (const RangeCopy *)copies.p, 0u, p.ids,
          (const uint32_t *)(F + 7), abort);

@@ -45,6 +45,7 @@ OTG_API otg_status otg_ctx_set_tuning(otg_ctx *ctx, const otg_tuning *t) {
   ctx->tuning.waves_per_simd = t->waves_per_simd;
   ctx->tuning.order_queries = t->order_queries;
   ctx->tuning.persistent = t->persistent;
+  ctx->tuning.range_records = t->range_records;
   return OTG_OK;
 }
 
@@ -55,7 +56,7 @@ OTG_API otg_status otg_ctx_get_tuning(const otg_ctx *ctx, otg_tuning *t) {
   t->waves_per_simd = ctx->tuning.waves_per_simd;
   t->order_queries = ctx->tuning.order_queries;
   t->persistent = ctx->tuning.persistent;
-  t->reserved = 0;
+  t->range_records = ctx->tuning.range_records;
   t->step_ms = 0.0;
   return OTG_OK;
 }
@@ -155,7 +156,7 @@ OTG_API otg_status otg_autotune(otg_ctx *ctx, int kind, int dim, uint32_t nsets,
     res.waves_per_simd = qpick.waves_per_simd;
     res.order_queries = qpick.order_queries;
     res.persistent = qpick.persistent;
-    res.reserved = 0;
+    res.range_records = qpick.range_records;
     res.step_ms = best - qdefault + qbest;
     ctx->tuning = qpick;
     {

@@ -58,7 +58,7 @@ def test_tuned_results_equal_untuned():
         assert ctx.autotune("point", 3, offs, pts, box, None, knn, sid, 16, rq, sid) == tune
         # every knob forced away from its default leaves the ids alone
         for setting in (dict(index_capacity=8), dict(waves_per_simd=1), dict(order_queries=0), dict(persistent=1),
-                        dict(index_capacity=64, waves_per_simd=2, order_queries=0, persistent=1)):
+                        dict(range_records=2), dict(index_capacity=64, waves_per_simd=2, order_queries=0, persistent=1)):
             ctx.set_tuning(**setting)
             for cap in (8, 64):
                 got = _run(ctx, sets, box, cap, knn, sid, rq)

@@ -163,6 +163,31 @@ def test_range_vs_oracle(gpu_ctx, kind, D, mode):
         assert gids[goff[qi]:goff[qi + 1]].tolist() == sorted(oids[ooff[qi]:ooff[qi + 1]].tolist()), qi
 
 
+@pytest.mark.parametrize("records", [0, 1, 3, 64])
+def test_range_kept_records(gpu_ctx, records):
+    """Large ranges whose lists the wavefront count walks keep as records: with the
+    default slots, with 1 or 3 slots per query (most records in the shared overflow
+    area, which then runs out: those queries take the fill walk) and with 64."""
+    n, D = 200_000, 3
+    geom = gaussian_mixture(n, D, 5, 0.0, 100.0, 0.5, 6.0, seed=41)
+    rng = np.random.default_rng(42)
+    c = rng.uniform(0, 100, (1500, D))
+    e = rng.uniform(2.0, 30.0, (1500, 1))
+    ranges = np.concatenate([c - e, c + e], axis=1)
+    ranges[0] = np.concatenate([np.full(D, -5.0), np.full(D, 105.0)])  # the whole set
+    o = O.OracleTree(D, "point", geom, None, 8, 8)
+    ooff, oids = o.range(ranges, overlap=False, threads=8)
+    g = _tree(gpu_ctx, "point", D, geom, None, 8, 8)
+    try:
+        gpu_ctx.set_tuning(range_records=records)
+        goff, gids = g.range_batch(ranges, 0).csr()
+    finally:
+        gpu_ctx.set_tuning()
+    assert np.array_equal(goff, ooff)
+    for qi in range(len(ranges)):
+        assert gids[goff[qi]:goff[qi + 1]].tolist() == sorted(oids[ooff[qi]:ooff[qi + 1]].tolist()), qi
+
+
 # ---- ray first -----------------------------------------------------------------------
 def test_ray_kats(gpu_ctx):
     for c in kat()["ray_first_2d"]:
