@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--budget-frac", type=float, default=0.6,
                     help="served workload: HBM budget as a fraction of the all-resident working set")
     ap.add_argument("--profile-only", action="store_true", help="one batched step, no plain/CPU legs")
+    ap.add_argument("--build-only", action="store_true",
+                    help="batched workload: time the forest build alone (warmup + steps builds), for build profiles")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "octree_point_1m", "octree_box_4m", "point6d_8m", "served"],
                     help="batched (default) or one of the single-tree configurations")
@@ -279,6 +281,19 @@ def main():
         cap = int(tune["leaf_capacity"])
         log(f"autotune ({time.perf_counter() - t0:.1f}s): {tune}")
     log(f"leaf capacity: {cap}")
+
+    if args.build_only:
+        times = []
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            f = octree_amd.StaticOctreePoint.forest(None, cap, None, box, ctx=ctx, device_geom=(d_pts, offs))
+            ctx.synchronize()
+            if i >= args.warmup:
+                times.append((time.perf_counter() - t0) * 1e3)
+            f.free()
+        log(f"forest build of {N} points in {nsets} sets at leaf capacity {cap}: median {np.median(times):.3f} ms "
+            f"over {len(times)} builds")
+        return
 
     # ---- batched timed steps ----
     for _ in range(args.warmup):

@@ -139,15 +139,16 @@ __global__ void __launch_bounds__(kB) k_bbox(const double *__restrict__ geom, ui
 constexpr uint32_t kErrOutOfSpace = 1u;
 constexpr uint32_t kErrUnsupported = 2u;
 
+// Items per thread of the encode launch: several points' loads in flight per thread
+// (the loads of one point alone leave each wavefront waiting on memory).
+template <int S>
+constexpr int kPointItems = S <= 3 ? 8 : (S <= 6 ? 4 : 1);
+
 template <int D, bool BOX>
-__global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, uint64_t n,
-                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
-                                                const SetParams *__restrict__ sets, KeyView kv,
-                                                uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                uint32_t *__restrict__ err) {
-  uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = set_of_wave(set_off, nsets, i);
+__device__ __forceinline__ void encode_point(const double *p, uint64_t i, uint32_t s,
+                                             const SetParams *__restrict__ sets, KeyView kv,
+                                             uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                             uint32_t *__restrict__ err) {
   const SetParams &sp = sets[s];
   const uint32_t M = sp.max_depth;
   uint32_t g[D];
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, 
     bool out = false;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      double c = geom[i * D + d] - sp.space_min[d];
+      double c = p[d] - sp.space_min[d];
       if (c < -kBaseTol || c > sp.size[d] + kBaseTol) out = true;
       uint32_t r = 0;
       if (c > 0) {
@@ -178,7 +179,7 @@ __global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, 
     }
   } else {
     // si_grid.h:221-285 GetLooseBoxGridData<2.0>, si_mortongrid.h:78-97
-    const double *b = &geom[i * 2 * D];
+    const double *b = p;
     bool inside = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -221,7 +222,39 @@ __global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, 
     }
   }
   keys[i] = kv.set_base(s) | (code << kv.dbits) | (BOX ? (uint64_t)depth : 0ull);
+  keys[i] = kv.set_base(s) | (code << kv.dbits) | (BOX ? (uint64_t)depth : 0ull);
   vals[i] = (uint32_t)i;
+}
+
+// One block encodes kB * kPointItems consecutive points (item r of thread t: point
+// base + r * kB + t): every item's coordinates are loaded first, then encoded; the set
+// of the block's first point is searched once and stepped forward per item.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, uint64_t n,
+                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                const SetParams *__restrict__ sets, KeyView kv,
+                                                uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                uint32_t *__restrict__ err) {
+  constexpr int S = BOX ? 2 * D : D;
+  constexpr int E = kPointItems<S>;
+  const uint64_t base = (uint64_t)blockIdx.x * (kB * E);
+  if (base >= n) return;
+  double c[E][S];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const uint64_t i = base + (uint64_t)r * kB + threadIdx.x;
+    if (i < n)
+#pragma unroll
+      for (int q = 0; q < S; ++q) c[r][q] = geom[i * S + q];
+  }
+  uint32_t s = set_of_wave(set_off, nsets, base);
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const uint64_t i = base + (uint64_t)r * kB + threadIdx.x;
+    if (i >= n) break;
+    while (s + 1 < nsets && set_off[s + 1] <= i) ++s;
+    encode_point<D, BOX>(c[r], i, s, sets, kv, keys, vals, err);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1036,8 +1069,24 @@ static void encode_sort(Tree *t, const double *dgeom, const uint64_t *d_off, uin
   const uint64_t n = t->n;
   const KeyView kv{t->layout.dbits, t->layout.cbits};
   t->d_keys = (uint64_t *)build_alloc(ctx, ar, n * 8);
-  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, d_off,
-         (uint32_t)t->sets.size(), (const SetParams *)t->d_sets, kv, t->d_keys, vals, err);
+  const uint32_t nsets = (uint32_t)t->sets.size(), low = t->layout.cbits + t->layout.dbits;
+  const dim3 grid(div_up(n, kB * kPointItems<BOX ? 2 * D : D>));
+  // the points arrive grouped by set: sorting inside each set skips the set-id digits
+  // when that saves a pass (C3-like forests: 39 of 47 bits, five passes instead of six)
+  if (nsets > 1 && (low + 7) / 8 < (t->layout.total + 7) / 8 && radix_sets_fit(nsets, low)) {
+    // encode into the buffer that makes the last pass land in t->d_keys and vals
+    const bool odd = ((low + 7) / 8) & 1u;
+    Buf<uint64_t> k2(ctx, ar, n);
+    Buf<uint32_t> v2(ctx, ar, n);
+    uint64_t *ka = odd ? k2.p : t->d_keys, *kb = odd ? t->d_keys : k2.p;
+    uint32_t *va = odd ? v2.p : vals, *vb = odd ? vals : v2.p;
+    launch(ctx, "encode", k_encode<D, BOX>, grid, dim3(kB), 0, dgeom, n, d_off, nsets, (const SetParams *)t->d_sets,
+           kv, ka, va, err);
+    radix_sort_sets(ctx, ka, va, kb, vb, n, d_off, nsets, low, ar);
+    return;
+  }
+  launch(ctx, "encode", k_encode<D, BOX>, grid, dim3(kB), 0, dgeom, n, d_off, nsets, (const SetParams *)t->d_sets, kv,
+         t->d_keys, vals, err);
   radix_sort_pairs(ctx, t->d_keys, vals, n, t->layout.total, ar);
 }
 
@@ -1795,7 +1844,7 @@ static void build_seq_impl(Tree *t, const double *geom_in, int mem, uint64_t n, 
   t->d_keys = (uint64_t *)ctx->alloc(n * 8);
   Buf<uint32_t> order(ctx, n), err(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, ctx->stream));
-  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, n, (const uint64_t *)d_off.p, 1u,
+  launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB * kPointItems<BOX ? 2 * D : D>)), dim3(kB), 0, dgeom, n, (const uint64_t *)d_off.p, 1u,
          (const SetParams *)t->d_sets, kv, t->d_keys, order.p, err.p);
   Buf<uint32_t> ff(ctx, n), bt(ctx, n), kid_cnt(ctx, n), scan(ctx, n + 1), total(ctx, 1);
   Buf<uint64_t> kid_key(ctx, n);
@@ -2233,7 +2282,7 @@ static void load_impl(Tree *t, const uint8_t *buf, uint64_t size, bool large, ui
   if (BOX) {  // location keys against the stored tree box, then node keys
     Buf<uint64_t> keys(ctx, ne);
     Buf<uint32_t> vals(ctx, ne);
-    launch(ctx, "encode", k_encode<D, true>, dim3(div_up(ne, kB)), dim3(kB), 0, dgeom, ne, (const uint64_t *)d_off.p,
+    launch(ctx, "encode", k_encode<D, true>, dim3(div_up(ne, kB * kPointItems<2 * D>)), dim3(kB), 0, dgeom, ne, (const uint64_t *)d_off.p,
            1u, (const SetParams *)t->d_sets, kv, keys.p, vals.p, err.p + 1);
     t->d_keys = (uint64_t *)ctx->alloc(ne * 8);
     launch(ctx, "loaded_keys", k_loaded_keys, dim3(div_up(ne, kB)), dim3(kB), 0, (const uint64_t *)keys.p,

@@ -216,6 +216,15 @@ void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint
 // Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
 // result is in keys/vals on return.
 void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN n, uint32_t bits, Arena *arena);
+// Stable LSD radix sort inside each set: the pairs of set s sit in [d_off[s], d_off[s+1])
+// (device offsets, nsets + 1 of them) and are sorted there on bits [0, bits) -- the bits
+// above (the set id of forest keys) are already in order.  The pairs start in (ka, va);
+// (kb, vb) is the other buffer of the ping-pong, so the result ends in (ka, va) after an
+// even number of passes ((bits + 7) / 8) and in (kb, vb) after an odd one.
+// radix_sets_fit says whether the per-set digit tables are small enough (few sets).
+bool radix_sets_fit(uint32_t nsets, uint32_t bits);
+void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb, uint64_t n,
+                     const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena);
 inline void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits,
                              Arena *arena = nullptr) {
   radix_sort_pairs(ctx, keys, vals, DevN{n, nullptr}, bits, arena);

@@ -340,13 +340,96 @@ __global__ void __launch_bounds__(kBlock) k_os_prefix(OsState *__restrict__ st, 
 // run in the output averages 16 keys (a whole 128-byte line of keys).
 constexpr int kOsBlock = 512;
 constexpr int kOsWaves = kOsBlock / 64;
-constexpr int kOsTile = kOsBlock * kItems;
+constexpr int kOsItems = 8;
+constexpr int kOsTile = kOsBlock * kOsItems;
+
+// Sort within sets (radix_sort_sets): the keys of set s sit in [off[s], off[s+1]) and
+// are sorted there on their low bits only.  Tiles never cross a set boundary (a set's
+// last tile may be short); tile_base[s] is the first tile of set s (tile_base[nsets]
+// the tile count), tile_set[t] the set of tile t, and base[(s * passes + p) * 256 + d]
+// the position where digit d of set s starts in pass p.  A set's first tile publishes
+// its counts as inclusive, so no look-back crosses into another set.
+struct OsSegments {
+  const uint64_t *off;
+  const uint32_t *tile_base;
+  const uint32_t *tile_set;
+  const uint32_t *base;
+  const uint32_t *ntiles;  // tile_base + nsets
+  uint32_t passes;
+};
+
+__global__ void __launch_bounds__(kBlock) k_seg_tile_counts(const uint64_t *__restrict__ off, uint32_t nsets,
+                                                            uint32_t *__restrict__ tiles) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < nsets) tiles[s] = (uint32_t)((off[s + 1] - off[s] + kOsTile - 1) / kOsTile);
+}
+
+// one block per set: tile_set over the set's tiles
+__global__ void __launch_bounds__(kBlock) k_seg_tile_sets(const uint32_t *__restrict__ tile_base,
+                                                          uint32_t *__restrict__ tile_set) {
+  const uint32_t s = blockIdx.x;
+  for (uint32_t t = tile_base[s] + threadIdx.x; t < tile_base[s + 1]; t += kBlock) tile_set[t] = s;
+}
+
+// Digit counts per (set, pass): a block takes kSegHistTiles consecutive tiles, counts
+// in LDS and adds its counts to the set's when the set changes and at the end.
+constexpr uint32_t kSegHistTiles = 4;
+constexpr uint64_t kSegMaxTable = 64ull << 20;  // digit-start table bytes (nsets x passes x 256 words)
+__global__ void __launch_bounds__(kBlock) k_seg_hist(const uint64_t *__restrict__ keys, OsSegments sg,
+                                                     const uint32_t *__restrict__ ntiles, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[kOsMaxPasses][256];
+  const uint32_t nt = *ntiles;
+  const uint32_t t0 = blockIdx.x * kSegHistTiles;
+  if (t0 >= nt) return;
+  const uint32_t t1 = min(t0 + kSegHistTiles, nt);
+  for (uint32_t p = 0; p < sg.passes; ++p) h[p][threadIdx.x] = 0u;
+  __syncthreads();
+  uint32_t cur = sg.tile_set[t0];
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t s = sg.tile_set[t];
+    if (s != cur) {  // block-uniform
+      __syncthreads();  // every count of the previous set is in
+      for (uint32_t p = 0; p < sg.passes; ++p) {
+        const uint32_t c = h[p][threadIdx.x];
+        if (c) atomicAdd(&hist[((uint64_t)cur * sg.passes + p) * 256 + threadIdx.x], c);
+        h[p][threadIdx.x] = 0u;
+      }
+      cur = s;
+      __syncthreads();
+    }
+    const uint64_t b = sg.off[s] + (uint64_t)(t - sg.tile_base[s]) * kOsTile;
+    const uint64_t e = min(b + (uint64_t)kOsTile, sg.off[s + 1]);
+    for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) {
+      const uint64_t k = keys[i];
+      for (uint32_t p = 0; p < sg.passes; ++p) atomicAdd(&h[p][(uint32_t)(k >> (8 * p)) & 255u], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t p = 0; p < sg.passes; ++p) {
+    const uint32_t c = h[p][threadIdx.x];
+    if (c) atomicAdd(&hist[((uint64_t)cur * sg.passes + p) * 256 + threadIdx.x], c);
+  }
+}
+
+// one block per set: counts -> the set's digit starts (in place)
+__global__ void __launch_bounds__(kBlock) k_seg_prefix(const uint64_t *__restrict__ off, uint32_t passes,
+                                                       uint32_t *__restrict__ hist) {
+  __shared__ uint64_t lds[kBlock / 64];
+  const uint32_t s = blockIdx.x;
+  for (uint32_t p = 0; p < passes; ++p) {
+    uint32_t *row = hist + ((uint64_t)s * passes + p) * 256;
+    uint64_t ex, tot;
+    block_scan<OpAdd64>((uint64_t)row[threadIdx.x], &ex, &tot, lds);
+    row[threadIdx.x] = (uint32_t)(off[s] + ex);
+    __syncthreads();
+  }
+}
 
 __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                           uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                           DevN dn, uint32_t pass, OsState *__restrict__ st,
                                                           uint32_t *__restrict__ status, uint32_t *__restrict__ status_next,
-                                                          uint32_t *__restrict__ err) {
+                                                          uint32_t *__restrict__ err, OsSegments sg) {
   __shared__ uint32_t wcnt[kOsWaves][256];
   __shared__ uint32_t base_off[256];
   __shared__ uint32_t tstart[256];
@@ -362,19 +445,36 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   if (digit_thread)
     for (int w = 0; w < kOsWaves; ++w) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t n = dn.get();
   const uint32_t tile_id = s_tile;
-  const uint64_t tile0 = (uint64_t)tile_id * kOsTile;
+  // this tile's keys [tile0, n), whether it starts its sort (no look-back), and where
+  // its digits start
+  uint64_t tile0, n;
+  bool first;
+  const uint32_t *digit_base;
+  if (sg.base) {
+    if (tile_id >= *sg.ntiles) return;  // block-uniform
+    const uint32_t s = sg.tile_set[tile_id];
+    const uint32_t local = tile_id - sg.tile_base[s];
+    tile0 = sg.off[s] + (uint64_t)local * kOsTile;
+    n = min(tile0 + (uint64_t)kOsTile, sg.off[s + 1]);
+    first = local == 0;
+    digit_base = sg.base + ((uint64_t)s * sg.passes + pass) * 256;
+  } else {
+    n = dn.get();
+    tile0 = (uint64_t)tile_id * kOsTile;
+    first = tile_id == 0;
+    digit_base = st->base[pass];
+  }
   // tiles past a device-side count have nothing to do; no tile in front of them waits on them
   if (tile0 >= n) return;
-  const uint64_t tile = tile0 + (uint64_t)wave * (64 * kItems);
+  const uint64_t tile = tile0 + (uint64_t)wave * (64 * kOsItems);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  uint64_t key[kItems];
-  uint32_t val[kItems];
-  uint32_t dig[kItems];
-  uint32_t off[kItems];
+  uint64_t key[kOsItems];
+  uint32_t val[kOsItems];
+  uint32_t dig[kOsItems];
+  uint32_t off[kOsItems];
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
+  for (int r = 0; r < kOsItems; ++r) {
     uint64_t i = tile + (uint64_t)r * 64 + lane;
     bool valid = i < n;
     key[r] = valid ? kin[i] : 0ull;
@@ -410,7 +510,7 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
     const uint32_t d = threadIdx.x;
     uint32_t *mine = &status[(uint64_t)tile_id * 256 + d];
     uint32_t excl = 0;
-    if (tile_id == 0) {
+    if (first) {
       __hip_atomic_store(mine, kOsInc | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       __hip_atomic_store(mine, kOsAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -446,7 +546,7 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
       }
       __hip_atomic_store(mine, kOsInc | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    base_off[d] = st->base[pass][d] + excl;
+    base_off[d] = digit_base[d] + excl;
     status_next[(uint64_t)tile_id * 256 + d] = 0u;
     // tile-local start of each digit (exclusive scan of the tile counts in digit order)
     uint32_t x = run;
@@ -468,7 +568,7 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   // stage the tile in digit order in LDS, then write each digit's run with
   // consecutive threads so the global stores coalesce
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
+  for (int r = 0; r < kOsItems; ++r) {
     const uint64_t i = tile + (uint64_t)r * 64 + lane;
     if (i < n) {
       const uint32_t lp = tstart[dig[r]] + wcnt[wave][dig[r]] + off[r];
@@ -479,7 +579,7 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   __syncthreads();
   const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)kOsTile ? (n - tile0) : (uint64_t)kOsTile);
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
+  for (int r = 0; r < kOsItems; ++r) {
     const uint32_t j = (uint32_t)r * kOsBlock + threadIdx.x;
     if (j < cnt) {
       const uint64_t k = skey[j];
@@ -511,13 +611,49 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN dn, uint32_
   uint32_t *va = vals, *vb = v2.p;
   for (uint32_t p = 0; p < passes; ++p) {
     launch(ctx, "radix_scatter", k_os_scatter, dim3(ntiles), dim3(kOsBlock), 0, (const uint64_t *)ka,
-           (const uint32_t *)va, kb, vb, dn, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err);
+           (const uint32_t *)va, kb, vb, dn, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err, OsSegments{});
     std::swap(ka, kb);
     std::swap(va, vb);
   }
   if (ka != keys) {
     OTG_HIP_CHECK(hipMemcpyAsync(keys, ka, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
     OTG_HIP_CHECK(hipMemcpyAsync(vals, va, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+}
+
+bool radix_sets_fit(uint32_t nsets, uint32_t bits) {
+  const uint32_t passes = (bits + 7) / 8;
+  return nsets > 0 && passes <= kOsMaxPasses && (uint64_t)nsets * passes * 256 * 4 <= kSegMaxTable;
+}
+
+void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb, uint64_t n,
+                     const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena) {
+  if (n == 0 || bits == 0) return;
+  if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
+  if (!radix_sets_fit(nsets, bits)) throw Error{OTG_ERR_INVALID_ARG, "per-set radix tables too large (check radix_sets_fit first)"};
+  const uint32_t passes = (bits + 7) / 8;
+  const uint64_t ntiles = div_up(n, kOsTile) + nsets;  // bound: one short tile per set
+  Buf<OsState> st(ctx, arena, 1);
+  Buf<uint32_t> status(ctx, arena, (size_t)2 * 256 * ntiles);
+  Buf<uint32_t> tcount(ctx, arena, nsets), tbase(ctx, arena, (uint64_t)nsets + 1), tset(ctx, arena, ntiles);
+  Buf<uint32_t> base(ctx, arena, (uint64_t)nsets * passes * 256);
+  uint32_t *stat[2] = {status.p, status.p + (size_t)256 * ntiles};
+  launch(ctx, "seg_tile_counts", k_seg_tile_counts, dim3(div_up(nsets, kBlock)), dim3(kBlock), 0, d_off, nsets,
+         tcount.p);
+  exclusive_scan_u32(ctx, tcount.p, tbase.p, nsets, tbase.p + nsets, arena);
+  launch(ctx, "seg_tile_sets", k_seg_tile_sets, dim3(nsets), dim3(kBlock), 0, (const uint32_t *)tbase.p, tset.p);
+  fill_u32(ctx, base.p, (uint64_t)nsets * passes * 256, 0u);
+  const OsSegments sg{d_off, tbase.p, tset.p, base.p, tbase.p + nsets, passes};
+  launch(ctx, "radix_init", k_os_init, dim3(std::min<uint32_t>(div_up((uint64_t)256 * ntiles, kBlock), 1024)),
+         dim3(kBlock), 0, st.p, stat[0], (uint64_t)256 * ntiles);
+  launch(ctx, "seg_hist", k_seg_hist, dim3(div_up(ntiles, kSegHistTiles)), dim3(kBlock), 0, (const uint64_t *)ka,
+         sg, (const uint32_t *)(tbase.p + nsets), base.p);
+  launch(ctx, "seg_prefix", k_seg_prefix, dim3(nsets), dim3(kBlock), 0, d_off, passes, base.p);
+  for (uint32_t p = 0; p < passes; ++p) {
+    launch(ctx, "radix_scatter", k_os_scatter, dim3(ntiles), dim3(kOsBlock), 0, (const uint64_t *)ka,
+           (const uint32_t *)va, kb, vb, DevN{n, nullptr}, p, st.p, stat[p & 1], stat[(p + 1) & 1], ctx->d_err, sg);
+    std::swap(ka, kb);
+    std::swap(va, vb);
   }
 }
 
