@@ -278,7 +278,7 @@ void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, ui
 //
 // One histogram launch counts every digit position at once; a one-block launch
 // turns the counts into each digit's global start.  Each scatter launch then
-// ranks a 2048-key tile in LDS (wavefront ballots), publishes the tile's 256
+// ranks a tile of keys in LDS (wavefront ballots), publishes the tile's 256
 // digit counts and finds the counts of all earlier tiles by decoupled look-back:
 // a tile's status word per digit holds 30 bits of count and a flag, AGG (this
 // tile's own count) or INC (count of this tile and every earlier one).  Tile ids
@@ -336,11 +336,15 @@ __global__ void __launch_bounds__(kBlock) k_os_prefix(OsState *__restrict__ st, 
   }
 }
 
-// Scatter tile: OB threads (OB / 64 wavefronts) x 8 keys = 4096 keys, so a digit's
-// run in the output averages 16 keys (a whole 128-byte line of keys).
-constexpr int kOsBlock = 512;
+// Scatter tile: 1024 threads (16 wavefronts) x 10 keys = 10,240 keys staged in 138 KiB
+// of LDS (one block per CU), so a digit's run in the output averages 40 keys (2.5
+// 128-byte lines of keys).  Measured on the batched forest build (39.2M keys, five
+// passes): 512 x 8 keys 6.07 ms, 1024 x 8 5.97 ms, 1024 x 10 5.88 ms, 512 x 12 6.68 ms
+// (LDS for one block only); writing straight from registers without the LDS stage
+// was slower (7.4 ms).
+constexpr int kOsBlock = 1024;
 constexpr int kOsWaves = kOsBlock / 64;
-constexpr int kOsItems = 8;
+constexpr int kOsItems = 10;
 constexpr int kOsTile = kOsBlock * kOsItems;
 
 // Sort within sets (radix_sort_sets): the keys of set s sit in [off[s], off[s+1]) and
