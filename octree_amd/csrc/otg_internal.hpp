@@ -342,6 +342,7 @@ struct Result {
   uint64_t nids = 0;
   uint64_t *d_offsets = nullptr;
   uint32_t *d_ids = nullptr;
+  uint64_t capped = 0;  // RangeSearch: queries the wavefront walks counted (sizes range plans)
   // host-resident results (served requests): used when host_only is set
   bool host_only = false;
   std::vector<uint64_t> h_offsets;

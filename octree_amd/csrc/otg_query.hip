@@ -1266,6 +1266,7 @@ struct RangeKeep {
   uint32_t *xq;    // the listed query of each overflow record
   uint32_t *nx;    // overflow records handed out (zeroed before the walk)
   uint32_t cap, xcap;
+  uint32_t nlist;  // listed queries with slots (those after them are not kept)
 };
 constexpr uint32_t kKeepMin = 1024, kKeepMax = 8192, kKeepOverflow = 1u << 20;
 constexpr uint64_t kKeepBudget = 16ull << 20;  // records per batch (256 MiB)
@@ -1500,7 +1501,7 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
     start = root_in ? 1 : (vol > 0.0 ? 2 : 0);
   }
   // collect mode (count walk with `keep`): slots of listed query b
-  const bool kept = !FILL && keep.recs;
+  const bool kept = !FILL && keep.recs && b < keep.nlist;
   RangeRec *const krecs = kept ? keep.recs + (uint64_t)b * keep.cap : nullptr;
   auto keep_rec = [&](uint32_t node, uint32_t at, uint32_t len, uint32_t kind) {
     const uint32_t r = atomicAdd(&s_nrec, 1u);
@@ -1597,9 +1598,9 @@ __global__ void __launch_bounds__(64) k_range_wave(const TreeView *__restrict__ 
   __syncthreads();
   if (!FILL && lane == 0) {
     counts[qi] = s_cnt;
-    if (kept) {
-      keep.nrec[b] = min(s_nrec, keep.cap);
-      keep.ok[qi] = s_bad ? 0u : 1u;
+    if (!FILL && keep.recs) {
+      if (kept) keep.nrec[b] = min(s_nrec, keep.cap);
+      keep.ok[qi] = (kept && !s_bad) ? 1u : 0u;
     }
   }
   }
@@ -1638,7 +1639,12 @@ __global__ void __launch_bounds__(256) k_range_kept(const TreeView *__restrict__
                                                      const double *__restrict__ ranges,
                                                      const uint32_t *__restrict__ list, uint32_t nlist,
                                                      int overlap_mode, double tol, const RangeKeep keep,
-                                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ out) {
+                                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ out,
+                                                     const uint32_t *__restrict__ ncount,
+                                                     const uint32_t *__restrict__ abort) {
+  // ncount (when given): the device-side length of the list; abort: results over a
+  // plan's capacity, nothing is written
+  if (abort && *abort) return;
   const uint32_t lane = threadIdx.x & 63;
   auto load = [&](uint32_t b, uint64_t &qi, double (&rmin)[D], double (&rmax)[D]) {
     qi = list[b];
@@ -1652,6 +1658,7 @@ __global__ void __launch_bounds__(256) k_range_kept(const TreeView *__restrict__
   uint64_t qi;
   if (blockIdx.x < nlist) {
     const uint32_t b = blockIdx.x;
+    if (ncount && b >= *ncount) return;
     load(b, qi, rmin, rmax);
     if (!keep.ok[qi]) return;
     const TreeView &tv = views[qview ? qview[qi] : 0u];
@@ -1692,9 +1699,9 @@ __global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, 
   const uint32_t c = cnt[qi];
   const bool h = capped[ti] || c > kRangeInline;  // a capped walk left its row incomplete
   heavy[ti] = h;
-  // walk (when given, with ok): the heavy queries the fill walk still has to fill --
-  // not the capped ones whose count walk kept their whole lists
-  if (walk) walk[ti] = h && !(capped[ti] && ok[qi]);
+  // walk (when given): the heavy queries the fill walk still has to fill -- not the
+  // capped ones whose count walk kept their whole lists (ok; null: none kept)
+  if (walk) walk[ti] = h && !(capped[ti] && ok && ok[qi]);
   if (h || c == 0) return;
   uint32_t v[kRangeInline];
 #pragma unroll
@@ -3479,8 +3486,8 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
       keep_ok = Buf<uint32_t>(ctx, std::max<uint64_t>(nq, 1));
       keep_xq = Buf<uint32_t>(ctx, xcap);
       OTG_HIP_CHECK(hipMemsetAsync(keep_n.p + ncap, 0, 4, ctx->stream));
-      keep = RangeKeep{keep_recs.p,  keep_n.p,  keep_ok.p, keep_recs.p + (uint64_t)ncap * kcap,
-                       keep_xq.p, keep_n.p + ncap, kcap,      xcap};
+      keep = RangeKeep{keep_recs.p, keep_n.p, keep_ok.p, keep_recs.p + (uint64_t)ncap * kcap,
+                       keep_xq.p,   keep_n.p + ncap, kcap, xcap, ncap};
     }
     launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(ncap), dim3(64), 0, vs.views, vs.qview, q, qs,
            (const uint32_t *)clist.p, mode, tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, ncopy.p,
@@ -3492,6 +3499,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   Result *r = new Result();
   r->ctx = ctx;
   r->nq = nq;
+  r->capped = ncap;
   Buf<uint32_t> scan(ctx, nq + 1), total(ctx, 1);
   exclusive_scan_u32(ctx, cnt.p, scan.p, nq, total.p);
   uint32_t ht = 0, hcopy = 0;
@@ -3520,7 +3528,8 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   const uint32_t nwalk = collected ? flagged_list(ctx, walk.p, ord, nq, wlist) : nheavy;
   if (collected)
     launch(ctx, "range_kept", k_range_kept<D, BOX>, dim3(ncap + 256), dim3(256), 0, vs.views, vs.qview, q,
-           (const uint32_t *)clist.p, ncap, mode, tol, keep, (const uint64_t *)r->d_offsets, r->d_ids);
+           (const uint32_t *)clist.p, ncap, mode, tol, keep, (const uint64_t *)r->d_offsets, r->d_ids,
+           (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   if constexpr (kWave)
     launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(nwalk), dim3(64), 0, vs.views, vs.qview, q, qs,
            (const uint32_t *)(collected ? wlist.p : hlist.p), mode, tol, (uint32_t *)nullptr,
@@ -4077,6 +4086,30 @@ __global__ void k_offsets_dev(const uint32_t *__restrict__ scan, uint64_t n, con
   if (i == n) off[n] = *total;
 }
 
+// The plan's output steps, each a bounded grid that returns at once when its case does
+// not hold (flags: [0] band count, [4] a list of another length than k): rows written
+// straight into the CSR join the fixed lists, which are then compacted.
+__global__ void k_plan_direct_to_fixed(const uint32_t *__restrict__ direct, const uint32_t *__restrict__ cnt,
+                                       uint64_t nq, uint32_t k, uint32_t kcap, uint32_t *__restrict__ fixed,
+                                       const uint32_t *__restrict__ flags) {
+  if (!flags[0] && !flags[4]) return;
+  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < nq * k; x += (uint64_t)gridDim.x * 256) {
+    const uint64_t qi = x / k;
+    if (cnt[qi] == k) fixed[qi * kcap + (x - qi * k)] = direct[x];
+  }
+}
+
+__global__ void k_plan_compact(const uint32_t *__restrict__ src, uint32_t stride, const uint64_t *__restrict__ off,
+                               uint64_t nq, uint32_t *__restrict__ dst, const uint32_t *__restrict__ flags) {
+  if (!flags[4]) return;
+  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < nq * stride; x += (uint64_t)gridDim.x * 256) {
+    const uint64_t qi = x / stride;
+    const uint32_t t = (uint32_t)(x - qi * stride);
+    const uint64_t o = off[qi];
+    if (t < off[qi + 1] - o) dst[o + t] = src[x];
+  }
+}
+
 struct KnnPlan {
   const Tree *t = nullptr;  // the index walked (finer index when the tree has one)
   Ctx *ctx = nullptr;
@@ -4087,7 +4120,9 @@ struct KnnPlan {
   double F0 = 0.0, tol = 0.0;
   Tuning tuning;
   char *mem = nullptr;  // every buffer of the plan, from the context allocator
-  uint32_t *flags = nullptr;    // [0] band count, [1] bad set id, [2] id total, [3] persistent tile counter
+  // [0] band count, [1] bad set id, [2] id total, [3] persistent tile counter, [4] a list
+  // of another length than k (the rows written straight into csr_ids are then moved)
+  uint32_t *flags = nullptr;
   uint32_t *h_flags = nullptr;  // pinned copy, written by the graph's last node
   uint32_t *ids_fixed = nullptr, *cnt = nullptr, *band = nullptr, *sets_clean = nullptr, *scan = nullptr;
   uint32_t *csr_ids = nullptr;
@@ -4110,7 +4145,8 @@ static void knn_plan_body(KnnPlan &p, Arena &ar) {
   Ctx *ctx = p.ctx;
   const uint64_t nq = p.nq;
   ViewSet vs = single_view(p.t);
-  launch(ctx, "fill_u32", k_fill_u32, dim3(1), dim3(64), 0, p.flags, (uint64_t)4, 0u);
+  launch(ctx, "fill_u32", k_fill_u32, dim3(1), dim3(64), 0, p.flags, (uint64_t)8, 0u);
+  const KnnOut out{p.ids_fixed, p.csr_ids, p.flags + 4};
   const uint32_t *qs = nullptr;
   if (p.sets) {
     launch(ctx, "check_sets", k_sanitize_sets, dim3(std::min<uint64_t>(div_up(nq, 256), 1024)), dim3(256), 0, p.sets,
@@ -4123,18 +4159,23 @@ static void knn_plan_body(KnnPlan &p, Arena &ar) {
     const size_t pad = occupancy_pad(ctx, kp, ctx->tuning.waves_per_simd);
     launch(ctx, "knn", k_knn_persistent<D, KCAP, BOX>, dim3(persistent_grid(ctx, kp, kQB, pad, div_up(nq, kQB))),
            dim3(kQB), pad, vs.views, vs.qview, p.q, qs, (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k,
-           p.F0, p.tol, KnnOut{p.ids_fixed, nullptr, nullptr}, p.cnt, p.flags, p.band, p.Fq, p.flags + 3);
+           p.F0, p.tol, out, p.cnt, p.flags, p.band, p.Fq, p.flags + 3);
   } else {
     const size_t pad = occupancy_pad(ctx, (const void *)k_knn<D, KCAP, BOX>, ctx->tuning.waves_per_simd);
     launch(ctx, "knn", k_knn<D, KCAP, BOX>, dim3(div_up(nq, kQB)), dim3(kQB), pad, vs.views, vs.qview, p.q, qs,
-           (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, KnnOut{p.ids_fixed, nullptr, nullptr},
-           p.cnt, p.flags, p.band, p.Fq);
+           (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, out, p.cnt, p.flags, p.band, p.Fq);
   }
+  // every list of k ids went straight into its CSR row (offsets i * k); otherwise the
+  // rows join the fixed lists (also for the band pass) and are compacted
+  const uint32_t G = (uint32_t)std::min<uint64_t>(div_up(nq * p.k, 256), 16384);
+  launch(ctx, "direct_to_fixed", k_plan_direct_to_fixed, dim3(G), dim3(256), 0, (const uint32_t *)p.csr_ids,
+         (const uint32_t *)p.cnt, nq, p.k, (uint32_t)KCAP, p.ids_fixed, (const uint32_t *)p.flags);
   exclusive_scan_u32(ctx, p.cnt, p.scan, nq, p.flags + 2, &ar);
   launch(ctx, "offsets_dev", k_offsets_dev, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)p.scan, nq,
          (const uint32_t *)(p.flags + 2), p.offsets);
-  launch(ctx, "compact_fixed", k_compact_fixed, dim3(div_up(nq * KCAP, 256)), dim3(256), 0,
-         (const uint32_t *)p.ids_fixed, (uint32_t)KCAP, (const uint64_t *)p.offsets, nq, p.csr_ids);
+  launch(ctx, "compact_fixed", k_plan_compact, dim3((uint32_t)std::min<uint64_t>(div_up(nq * KCAP, 256), 16384)),
+         dim3(256), 0, (const uint32_t *)p.ids_fixed, (uint32_t)KCAP, (const uint64_t *)p.offsets, nq, p.csr_ids,
+         (const uint32_t *)p.flags);
   OTG_HIP_CHECK(hipMemcpyAsync(p.h_flags, p.flags, 16, hipMemcpyDeviceToHost, ctx->stream));
 }
 
@@ -4180,7 +4221,7 @@ static void knn_plan_capture(KnnPlan &p) {
   const size_t bytes = nq * (8ull * KCAP + 112) + ntiles * 2 * 256 * 4 + (8u << 20);
   p.mem = (char *)ctx->alloc(bytes);
   Arena fixed{p.mem, bytes, 0};
-  p.flags = (uint32_t *)fixed.take(16);
+  p.flags = (uint32_t *)fixed.take(32);
   p.ids_fixed = (uint32_t *)fixed.take(nq * KCAP * 4);
   p.cnt = (uint32_t *)fixed.take(nq * 4);
   p.band = (uint32_t *)fixed.take(nq * 4);
@@ -4285,7 +4326,10 @@ void knn_plan_run(KnnPlan *p) {
 // 2 x ids / (kInlineCopy + 1) bounds them.  Counts stay on the device: the capped and
 // heavy lists drive bounded grids that stride over them, and a total beyond a capacity
 // sets an abort word that every writing kernel checks (the run then reports
-// OTG_ERR_CAPACITY and leaves every list empty).
+// OTG_ERR_CAPACITY and leaves every list empty).  The wavefront count walks keep their
+// records for the first keep_q capped queries (the creation run's capped count plus a
+// sixteenth and 256), keep_cap records each plus the shared overflow area, as the
+// eager path does; capped queries beyond them are filled by the fill walk.
 struct RangePlan {
   const Tree *t = nullptr;  // the index walked
   Ctx *ctx = nullptr;
@@ -4295,9 +4339,11 @@ struct RangePlan {
   int mode = 0;
   double tol = 0.0;
   uint64_t cap_ids = 0, cap_copies = 0;
+  uint32_t keep_q = 0, keep_cap = 0, keep_x = 0;
   char *mem = nullptr;
   // [0] bad set id, [1] id total, [2] capped queries, [3] heavy queries, [4] runs of the
-  // count walks, [5] abort, [6] ids of heavy queries, [7] running run count
+  // count walks, [5] abort, [6] ids of heavy queries, [7] running run count, [8] queries
+  // the fill walk fills, [9] overflow records handed out
   uint32_t *flags = nullptr;
   uint32_t *h_flags = nullptr;
   uint64_t *offsets = nullptr;
@@ -4313,6 +4359,7 @@ struct RangePlan {
 };
 
 constexpr uint32_t kPlanListGrid = 16384;  // blocks striding over a device-side query list
+constexpr uint32_t kRangePlanFlags = 16;    // device words of a range plan (RangePlan::flags)
 
 // flags[4] = the count walks' run count, then the running count restarts for the fill;
 // abort when the ids or the runs outgrow the plan
@@ -4329,7 +4376,7 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
   const uint64_t nq = p.nq;
   ViewSet vs = single_view(p.t);
   uint32_t *F = p.flags;
-  fill_u32(ctx, F, 8, 0u);
+  fill_u32(ctx, F, kRangePlanFlags, 0u);
   const uint32_t *qs = nullptr;
   if (p.sets) {
     Buf<uint32_t> clean(&ar, nq);
@@ -4344,6 +4391,13 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
   Buf<RangeCopy> copies(&ar, p.cap_copies);
   Buf<uint64_t> keys(&ar, p.cap_ids);
   Buf<uint32_t> vals(&ar, p.cap_ids);
+  Buf<uint32_t> walk(&ar, nq), wpos(&ar, nq), wlist(&ar, nq), keep_ok(&ar, nq), keep_n(&ar, p.keep_q),
+      keep_xq(&ar, p.keep_x);
+  Buf<RangeRec> keep_recs(&ar, (uint64_t)p.keep_q * p.keep_cap + p.keep_x);
+  RangeKeep keep{};
+  if (p.keep_q)
+    keep = RangeKeep{keep_recs.p, keep_n.p,   keep_ok.p, keep_recs.p + (uint64_t)p.keep_q * p.keep_cap,
+                     keep_xq.p,   F + 9,      p.keep_cap, p.keep_x, p.keep_q};
   const uint32_t G = (uint32_t)std::min<uint64_t>(nq, kPlanListGrid);
   // 1. one-lane count walks; the capped ones counted by a wavefront each
   launch(ctx, "range_count", k_range<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview, p.q, qs,
@@ -4356,23 +4410,31 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
          (const uint32_t *)pos.p, nq, ord, clist.p);
   launch(ctx, "range_wave_count", k_range_wave<D, BOX, false>, dim3(G), dim3(64), 0, vs.views, vs.qview, p.q, qs,
          (const uint32_t *)clist.p, p.mode, p.tol, cnt.p, (const uint64_t *)nullptr, (uint32_t *)nullptr, F + 7,
-         (RangeCopy *)nullptr, (const uint32_t *)(F + 2), (const uint32_t *)nullptr, RangeKeep{});
+         (RangeCopy *)nullptr, (const uint32_t *)(F + 2), (const uint32_t *)nullptr, keep);
   // 2. offsets and the capacity check
   exclusive_scan_u32(ctx, cnt.p, scan.p, nq, F + 1, &ar);
   launch(ctx, "offsets_dev", k_offsets_dev, dim3(div_up(nq + 1, 256)), dim3(256), 0, (const uint32_t *)scan.p, nq,
          (const uint32_t *)(F + 1), p.offsets);
   launch(ctx, "range_plan_check", k_range_plan_check, dim3(1), dim3(1), 0, F, p.cap_ids, p.cap_copies);
   const uint32_t *abort = F + 5;
-  // 3. small results from the inline rows; the heavy ones filled by a wavefront each
+  // 3. small results from the inline rows; the kept ones from their records; the rest
+  // of the heavy ones filled by a wavefront each
   launch(ctx, "range_inline", k_range_inline, dim3(div_up(nq, 256)), dim3(256), 0, ord, nq, (const uint32_t *)cnt.p,
          (const uint32_t *)capped.p, (const uint64_t *)p.offsets, (const uint32_t *)inl.p, p.ids, heavy.p, abort,
-         (uint32_t *)nullptr, (const uint32_t *)nullptr);
+         walk.p, (const uint32_t *)(p.keep_q ? keep_ok.p : nullptr));
   exclusive_scan_u32(ctx, heavy.p, hpos0.p, nq, F + 3, &ar);
   launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, (const uint32_t *)heavy.p,
          (const uint32_t *)hpos0.p, nq, ord, hlist.p);
+  exclusive_scan_u32(ctx, walk.p, wpos.p, nq, F + 8, &ar);
+  launch(ctx, "range_heavy_list", k_range_heavy_list, dim3(div_up(nq, 256)), dim3(256), 0, (const uint32_t *)walk.p,
+         (const uint32_t *)wpos.p, nq, ord, wlist.p);
+  if (p.keep_q)
+    launch(ctx, "range_kept", k_range_kept<D, BOX>, dim3(p.keep_q + 256), dim3(256), 0, vs.views, vs.qview, p.q,
+           (const uint32_t *)clist.p, p.keep_q, p.mode, p.tol, keep, (const uint64_t *)p.offsets, p.ids,
+           (const uint32_t *)(F + 2), abort);
   launch(ctx, "range_fill", k_range_wave<D, BOX, true>, dim3(G), dim3(64), 0, vs.views, vs.qview, p.q, qs,
-         (const uint32_t *)hlist.p, p.mode, p.tol, (uint32_t *)nullptr, (const uint64_t *)p.offsets, p.ids, F + 7,
-         copies.p, (const uint32_t *)(F + 3), abort, RangeKeep{});
+         (const uint32_t *)wlist.p, p.mode, p.tol, (uint32_t *)nullptr, (const uint64_t *)p.offsets, p.ids, F + 7,
+         copies.p, (const uint32_t *)(F + 8), abort, RangeKeep{});
   // 4. long contained runs, one wavefront each
   launch(ctx, "range_copy", k_range_copy, dim3(kPlanListGrid / 4), dim3(256), 0, (const RangeCopy *)copies.p, 0u, p.ids,
          (const uint32_t *)(F + 7), abort);
@@ -4387,22 +4449,24 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
   launch(ctx, "range_sort_back", k_range_sort_back, dim3(G), dim3(256), 0, (const uint32_t *)hlist.p, 0u,
          (const uint64_t *)p.offsets, (const uint32_t *)hpos.p, (const uint64_t *)keys.p, p.ids,
          (const uint32_t *)(F + 3), abort);
-  OTG_HIP_CHECK(hipMemcpyAsync(p.h_flags, F, 32, hipMemcpyDeviceToHost, ctx->stream));
+  OTG_HIP_CHECK(hipMemcpyAsync(p.h_flags, F, kRangePlanFlags * 4, hipMemcpyDeviceToHost, ctx->stream));
 }
 
 template <int D, bool BOX>
 static void range_plan_capture(RangePlan &p) {
   Ctx *ctx = p.ctx;
   const uint64_t nq = p.nq;
-  OTG_HIP_CHECK(hipHostMalloc(&p.h_flags, 32, hipHostMallocDefault));
-  std::memset(p.h_flags, 0, 32);
+  OTG_HIP_CHECK(hipHostMalloc(&p.h_flags, kRangePlanFlags * 4, hipHostMallocDefault));
+  std::memset(p.h_flags, 0, kRangePlanFlags * 4);
   const uint64_t tiles = nq / 4096 + 2, id_tiles = p.cap_ids / 4096 + 2;
-  size_t bytes = nq * (4 * 14 + 4 * kRangeInline + 8 + 12 + 12) + tiles * 2048 * 2 + p.cap_ids * (4 + 8 + 4 + 12) +
-                 id_tiles * 2048 + p.cap_copies * sizeof(RangeCopy) + (8u << 20);
+  size_t bytes = nq * (4 * 18 + 4 * kRangeInline + 8 + 12 + 12) + tiles * 2048 * 2 + p.cap_ids * (4 + 8 + 4 + 12) +
+                 id_tiles * 2048 + p.cap_copies * sizeof(RangeCopy) +
+                 ((uint64_t)p.keep_q * p.keep_cap + p.keep_x) * sizeof(RangeRec) + (p.keep_q + p.keep_x) * 4ull +
+                 (8u << 20);
   for (;; bytes *= 2) {
     p.mem = (char *)ctx->alloc(bytes);
     Arena ar{p.mem, bytes, 0};
-    p.flags = (uint32_t *)ar.take(32);
+    p.flags = (uint32_t *)ar.take(kRangePlanFlags * 4);
     p.offsets = (uint64_t *)ar.take((nq + 1) * 8);
     p.ids = (uint32_t *)ar.take(std::max<uint64_t>(p.cap_ids, 1) * 4);
     ctx->sync();
@@ -4452,9 +4516,20 @@ RangePlan *range_plan_create(const Tree *tree, uint64_t nq, const double *q, con
   p->mode = mode;
   p->tol = tol;
   if (p->t->n == 0) throw Error{OTG_ERR_INVALID_ARG, "plan on an empty tree"};
+  uint64_t capped = 0;
   {  // the batch's current contents size the plan
     std::unique_ptr<Result> r(query_range(tree, nq, q, sets, OTG_MEM_DEVICE, mode, tol));
     p->cap_ids = r->nids + r->nids / 8 + 65536;
+    capped = r->capped;
+  }
+  if (capped) {  // record slots as the eager path sizes them, for the creation run's capped count and more
+    const uint64_t kq = std::min<uint64_t>(nq, capped + capped / 16 + 256);
+    const uint32_t want = p->ctx->tuning.range_records;
+    if (kq * (want ? want : kKeepMin) <= kKeepBudget) {
+      p->keep_q = (uint32_t)kq;
+      p->keep_cap = want ? want : (uint32_t)std::min<uint64_t>(kKeepMax, kKeepBudget / kq);
+      p->keep_x = (uint32_t)std::min<uint64_t>(kKeepOverflow, 1024ull * p->keep_cap);
+    }
   }
   p->cap_copies = 2 * p->cap_ids / (kInlineCopy + 1) + 1024;
   const bool box = p->t->kind == OTG_KIND_BOX;

@@ -106,3 +106,44 @@ def test_range_plan_forest_set_ids_and_capacity(gpu_ctx):
     finally:
         plan.free()
         f.free()
+
+
+@pytest.mark.parametrize("records", [0, 2])
+def test_range_plan_kept_records(gpu_ctx, records):
+    """The count walks' records inside the graph: replays with many more capped queries
+    than the plan has record slots for (those are filled by the fill walk), and with 2
+    records per query (most in the overflow area, which runs out)."""
+    import torch
+
+    import octree_amd
+
+    D, nq = 3, 20_000
+    geom = gaussian_mixture(200_000, D, 6, 0.0, 100.0, 0.5, 6.0, seed=77)
+    t = octree_amd.StaticTree(ctx=gpu_ctx, dim=D, kind="point")
+    t.Create(geom, None, None, 8)
+    rng = np.random.default_rng(78)
+
+    def contents(nmed):
+        host = _boxes(rng, nq, D, 0.2, 1.0)
+        host[:50] = [[-1.0] * D + [101.0] * D]             # whole-tree boxes: the plan's id capacity
+        host[50:50 + nmed] = _boxes(rng, nmed, D, 8.0, 14.0)  # capped walks
+        return host
+
+    q = torch.zeros((nq, 2 * D), dtype=torch.float64, device="cuda")
+    q.copy_(torch.from_numpy(contents(10)))
+    torch.cuda.synchronize()
+    try:
+        gpu_ctx.set_tuning(range_records=records)
+        plan = t.range_plan(q, 0)
+        try:
+            for rep, nmed in enumerate((10, 2500, 40)):
+                host = contents(nmed)
+                q.copy_(torch.from_numpy(host))
+                torch.cuda.synchronize()
+                got = plan.run().csr()
+                _eq(got, t.range_batch(host, 0).csr(), f"replay {rep} ({nmed} larger boxes)")
+        finally:
+            plan.free()
+    finally:
+        gpu_ctx.set_tuning()
+        t.free()
