@@ -354,8 +354,9 @@ OTG_API otg_status otg_range_plan_result(const otg_range_plan *plan, const uint6
 OTG_API otg_status otg_range_plan_copy(const otg_range_plan *plan, uint64_t *offsets, uint32_t *ids);
 OTG_API void otg_range_plan_free(otg_range_plan *plan);
 
-/* A build plan and query plans made on its tree, replayed as one graph: the build,
- * then the kNN batch (knn_or_null), then the range batch (range_or_null).
+/* A build plan and query plans made on its tree, replayed together: the build, then
+ * the kNN batch (knn_or_null) and the range batch (range_or_null); with both, the
+ * range batch runs on a second stream of the step plan beside the kNN batch.
  * otg_step_plan_run synchronises; afterwards the tree is current and each query
  * plan's result / copy give its batch's ids, as after otg_build_plan_run followed by
  * each plan's own run.  Free it before the plans. */

@@ -2583,6 +2583,7 @@ void build_plan_finish(BuildPlan *p) {
 
 const Tree *build_plan_tree(const BuildPlan *p) { return p->t.get(); }
 hipGraph_t build_plan_graph(const BuildPlan *p) { return p->graph; }
+hipGraphExec_t build_plan_exec(const BuildPlan *p) { return p->exec; }
 void build_plan_free(BuildPlan *p) { delete p; }
 
 void build_forest(Tree *t, const double *geom, int mem, const uint64_t *offsets, uint32_t nsets, const double *boxes,

@@ -324,6 +324,7 @@ void build_plan_run(BuildPlan *p);
 void build_plan_finish(BuildPlan *p);
 const Tree *build_plan_tree(const BuildPlan *p);
 hipGraph_t build_plan_graph(const BuildPlan *p);
+hipGraphExec_t build_plan_exec(const BuildPlan *p);
 void build_plan_free(BuildPlan *p);
 // A build plan followed by a kNN plan on its tree, replayed as one graph; run()
 // synchronises, then brings the tree's host side up to date and completes the kNN

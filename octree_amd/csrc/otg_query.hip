@@ -4560,17 +4560,23 @@ void range_plan_free(RangePlan *p) { delete p; }
 Ctx *range_plan_ctx(RangePlan *p) { return p->ctx; }
 uint64_t range_plan_queries(const RangePlan *p) { return p->nq; }
 
-// A build plan and a kNN plan on its tree as one graph: the build's graph, then the
-// kNN batch's, as child graphs in that order.
+// A build plan and a kNN and/or range plan on its tree as one graph: the build's graph,
+// then the query batches' graphs as child graphs that depend on it only.
 struct StepPlan {
   BuildPlan *bp = nullptr;
   KnnPlan *kp = nullptr;    // or null
   RangePlan *rp = nullptr;  // or null
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // both batches: the range graph replays on a second stream beside the kNN graph
+  hipStream_t side = nullptr;
+  hipEvent_t built = nullptr, ranged = nullptr;
   ~StepPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (built) (void)hipEventDestroy(built);
+    if (ranged) (void)hipEventDestroy(ranged);
+    if (side) (void)hipStreamDestroy(side);
   }
 };
 
@@ -4584,21 +4590,39 @@ StepPlan *step_plan_create(BuildPlan *bp, KnnPlan *kp, RangePlan *rp) {
   st->kp = kp;
   st->rp = rp;
   OTG_HIP_CHECK(hipGraphCreate(&st->graph, 0));
-  hipGraphNode_t prev = nullptr;
-  for (hipGraph_t g : {build_plan_graph(bp), kp ? kp->graph : nullptr, rp ? rp->graph : nullptr}) {
+  // the build first; the kNN and range batches only read the tree and each has its own
+  // buffers, so they are two branches that may run side by side
+  hipGraphNode_t build = nullptr;
+  OTG_HIP_CHECK(hipGraphAddChildGraphNode(&build, st->graph, nullptr, 0, build_plan_graph(bp)));
+  for (hipGraph_t g : {kp ? kp->graph : nullptr, rp ? rp->graph : nullptr}) {
     if (!g) continue;
     hipGraphNode_t node = nullptr;
-    OTG_HIP_CHECK(hipGraphAddChildGraphNode(&node, st->graph, prev ? &prev : nullptr, prev ? 1 : 0, g));
-    prev = node;
+    OTG_HIP_CHECK(hipGraphAddChildGraphNode(&node, st->graph, &build, 1, g));
   }
   OTG_HIP_CHECK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
+  if (kp && rp) {
+    OTG_HIP_CHECK(hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking));
+    OTG_HIP_CHECK(hipEventCreateWithFlags(&st->built, hipEventDisableTiming));
+    OTG_HIP_CHECK(hipEventCreateWithFlags(&st->ranged, hipEventDisableTiming));
+  }
   return st.release();
 }
 
 void step_plan_run(StepPlan *st) {
   Ctx *ctx = build_plan_tree(st->bp)->ctx;
   if (st->kp) st->kp->band_result.reset();
-  OTG_HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
+  if (st->side) {
+    // build, then kNN on the context's stream and range on the side stream at once
+    OTG_HIP_CHECK(hipGraphLaunch(build_plan_exec(st->bp), ctx->stream));
+    OTG_HIP_CHECK(hipEventRecord(st->built, ctx->stream));
+    OTG_HIP_CHECK(hipStreamWaitEvent(st->side, st->built, 0));
+    OTG_HIP_CHECK(hipGraphLaunch(st->rp->exec, st->side));
+    OTG_HIP_CHECK(hipGraphLaunch(st->kp->exec, ctx->stream));
+    OTG_HIP_CHECK(hipEventRecord(st->ranged, st->side));
+    OTG_HIP_CHECK(hipStreamWaitEvent(ctx->stream, st->ranged, 0));
+  } else {
+    OTG_HIP_CHECK(hipGraphLaunch(st->exec, ctx->stream));
+  }
   ctx->sync();
   build_plan_finish(st->bp);
   if (st->kp) knn_plan_after(st->kp);
