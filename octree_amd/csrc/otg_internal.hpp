@@ -68,6 +68,7 @@ struct Tuning {
   uint32_t order_queries = 1;   // Morton-order batches of >= 16,384 queries before walking
   uint32_t persistent = 0;      // kNN walks as a persistent launch pulling 64-query tiles
   uint32_t range_records = 0;   // kept records per capped RangeSearch query (0: from the batch)
+  uint32_t knn_wave = 1;        // 6-D point kNN with k <= 63: one wavefront per query
 };
 
 class Ctx {

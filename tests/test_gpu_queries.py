@@ -115,6 +115,23 @@ def test_knn_band_longer_than_list(gpu_ctx):
     assert "knn_band_fill" in kt, sorted(kt)
 
 
+def test_knn_6d_limits_ties_and_band(gpu_ctx):
+    """6-D point kNN (one wavefront per query): distance limits and tolerances, equal
+    distances ordered by id, and tolerance bands longer than the list, against the oracle."""
+    pts = uniform_points(20000, 6, seed=61)
+    pts[100:103] = pts[7000]        # three copies: equal distances inside a list
+    pts[200:400] = pts[9000]        # 200 + 1 copies: the band outgrows any list
+    qs = np.concatenate([pts[7000:7001] + 1e-3, pts[9000:9001], uniform_points(1500, 6, seed=62)])
+    g = _tree(gpu_ctx, "point", 6, pts, None, 4, 16)
+    o = O.OracleTree(6, "point", pts, None, 4, 16)
+    for k, mw, tol in ((8, DBL_MAX, BASE_TOL), (1, DBL_MAX, BASE_TOL), (20, DBL_MAX, BASE_TOL), (8, 0.5, BASE_TOL),
+                       (16, 12.0, 0.0), (5, DBL_MAX, 1e-6)):
+        goff, gids = g.knn_batch(qs, k, mw, tol).csr()
+        ooff, oids = o.knn(qs, k, mw, tol, threads=8)
+        assert np.array_equal(goff, ooff) and np.array_equal(gids, oids), (k, mw, tol)
+    assert goff[2] - goff[1] == 201  # the band pass returned every copy
+
+
 # ---- box-tree kNN (MinMaxDistance) -----------------------------------------------------
 def test_box_knn_kats(gpu_ctx):
     for c in kat()["knn_box_2d"]:
