@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy (the known-size kernel that scales PMC byte counters)")
     ap.add_argument("--load", type=float, default=0.7, help="served workload: offered load / measured capacity")
+    ap.add_argument("--batch-queries", type=int, default=None,
+                    help="served workload: the server's batch limit in queries (default: the library's, 2,097,152)")
     ap.add_argument("--budget-frac", type=float, default=0.6,
                     help="served workload: HBM budget as a fraction of the all-resident working set")
     ap.add_argument("--profile-only", action="store_true", help="one batched step, no plain/CPU legs")
@@ -782,7 +784,7 @@ def run_served(args, octree_amd):
     log(f"working set {working / 2**30:.2f} GiB (all 64 trees built in {t_add:.1f}s); budget {budget / 2**30:.2f} GiB")
 
     def serve(load_qps=None):
-        srv = octree_amd.Server(0, hbm_budget=budget, workers=2)
+        srv = octree_amd.Server(0, hbm_budget=budget, workers=2, batch_queries=args.batch_queries)
         for i, (kind, g, cap) in enumerate(trees):
             srv.add_tree(i, kind, 3, g, None, None, cap)
         ids = [None] * len(reqs)
@@ -883,7 +885,7 @@ def run_served(args, octree_amd):
     log(f"plain: {plain}")
 
     # determinism spot check: 40 requests served in the mix equal the direct call's ids
-    srv = octree_amd.Server(0, hbm_budget=budget, workers=2)
+    srv = octree_amd.Server(0, hbm_budget=budget, workers=2, batch_queries=args.batch_queries)
     for i, (kind, g, cap) in enumerate(trees):
         srv.add_tree(i, kind, 3, g, None, None, cap)
     pick = np.random.default_rng(3).choice(len(reqs), 40, replace=False)
@@ -913,7 +915,8 @@ def run_served(args, octree_amd):
                                "1e5-2e6 AABBs log-normal extents), Zipf(1) tree popularity, batches log-uniform "
                                "1-65,536 mixing kNN (k 1-32) and range on all trees, rays on the box trees; HBM budget "
                                f"{args.budget_frac:.0%} of the working set", "requests": len(reqs), "queries": total_q,
-                   "working_set_bytes": working, "budget_bytes": budget, "workers": 2},
+                   "working_set_bytes": working, "budget_bytes": budget, "workers": 2,
+                   "batch_queries": args.batch_queries or 2097152},
         "saturation": sat, "open_loop": dict(op, offered_requests_per_s=offered),
         "plain": plain, "speedup_served_vs_plain": sat["queries_per_s"] / plain["queries_per_s"],
         "determinism_check": {"requests_compared": len(pick), "mismatches": mism},

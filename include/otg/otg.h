@@ -382,6 +382,12 @@ OTG_API void otg_result_free(otg_result *res);
  * otg_ray_first) on the same queries, whatever it was batched with. */
 typedef struct otg_server otg_server;
 OTG_API otg_status otg_server_create(int device, uint64_t hbm_budget_bytes, uint32_t nworkers, otg_server **out);
+/* No counterpart in the reference: the most queries a worker takes into one batch
+ * (default 2,097,152; a single larger request still forms a batch alone).  Smaller
+ * batches shorten the service time a request waits for and lower the tail latency at
+ * a given load, at the price of more launches and less throughput.  Applies to batches
+ * formed after the call. */
+OTG_API otg_status otg_server_set_batch_limit(otg_server *server, uint64_t max_queries);
 /* Waits for running batches; frees every resident tree and unclaimed result. */
 OTG_API void otg_server_destroy(otg_server *server);
 /* Registers the tree under tree_id (keeping a host copy of the entities, so that an evicted

@@ -786,13 +786,21 @@ class Server:
     request's ids equal the direct StaticTree query's on the same queries.
     """
 
-    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, workers: int = 2):
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, workers: int = 2,
+                 batch_queries: Optional[int] = None):
         h = C.c_void_p()
         st = _abi.lib().otg_server_create(device, int(hbm_budget), int(workers), C.byref(h))
         if st != _abi.OTG_OK:
             raise OtgError(st, "otg_server_create")
         self.handle = h
         self._dims = {}
+        if batch_queries is not None:
+            self.set_batch_limit(batch_queries)
+
+    def set_batch_limit(self, max_queries: int) -> None:
+        """The most queries one batch takes (otg_server_set_batch_limit): smaller batches
+        lower the tail latency at a given load and cost throughput."""
+        _check(_abi.lib().otg_server_set_batch_limit(self.handle, int(max_queries)), "otg_server_set_batch_limit")
 
     def add_tree(self, tree_id: int, kind: str, dim: int, entities, max_depth: Optional[int] = None,
                  box_space=None, max_elements: int = 16) -> None:

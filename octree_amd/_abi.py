@@ -60,6 +60,7 @@ EXPORTED = [
     "otg_result_get_info", "otg_result_copy", "otg_result_device", "otg_result_free",
     "otg_server_create", "otg_server_destroy", "otg_server_add_tree", "otg_server_remove_tree",
     "otg_server_has_tree", "otg_server_submit", "otg_server_wait", "otg_server_wait_detail", "otg_server_stats",
+    "otg_server_set_batch_limit",
 ]
 
 
@@ -175,6 +176,7 @@ def lib() -> C.CDLL:
         "otg_server_wait": (C.c_int, [P, C.c_uint64, PP, F64P, F64P]),
         "otg_server_wait_detail": (C.c_int, [P, C.c_uint64, PP, C.POINTER(C.c_double)]),
         "otg_server_stats": (C.c_int, [P, U64P]),
+        "otg_server_set_batch_limit": (C.c_int, [P, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

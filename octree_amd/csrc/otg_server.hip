@@ -738,6 +738,13 @@ OTG_API otg_status otg_server_wait_detail(otg_server *s, uint64_t request_id, ot
   return st;
 }
 
+OTG_API otg_status otg_server_set_batch_limit(otg_server *s, uint64_t max_queries) {
+  if (!s || max_queries == 0) return OTG_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  s->max_batch_queries = max_queries;
+  return OTG_OK;
+}
+
 OTG_API otg_status otg_server_stats(otg_server *s, uint64_t *stats) {
   if (!s || !stats) return OTG_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> g(s->mu);

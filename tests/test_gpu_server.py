@@ -61,7 +61,10 @@ def _submit(srv, tid, q, payload, params):
     return srv.submit_ray(tid, payload)
 
 
-def test_served_equals_direct_in_mix(gpu_ctx):
+@pytest.mark.parametrize("batch_queries", [None, 1000])
+def test_served_equals_direct_in_mix(gpu_ctx, batch_queries):
+    """Requests from four client threads, batched with the default limit or with one of
+    1,000 queries (many small batches; a larger request forms a batch alone)."""
     import octree_amd
 
     trees = {
@@ -69,7 +72,9 @@ def test_served_equals_direct_in_mix(gpu_ctx):
         2: ("box", 3, lognormal_boxes(30000, 3, 0.0, 100.0, seed=2), 6, 12),
         3: ("point", 2, uniform_points(40000, 2, seed=3), 9, 8),
     }
-    srv = octree_amd.Server(0, hbm_budget=8 << 30, workers=2)
+    srv = octree_amd.Server(0, hbm_budget=8 << 30, workers=2, batch_queries=batch_queries)
+    with pytest.raises(octree_amd.OtgError):
+        srv.set_batch_limit(0)
     direct = {}
     for tid, (kind, D, geom, M, cap) in trees.items():
         srv.add_tree(tid, kind, D, geom, M, None, cap)
