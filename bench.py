@@ -417,10 +417,12 @@ def main():
         algo = (24 + 12 + 24 * passes + 8 + 4) * N
     achieved = algo / (avg_ms * 1e-3) / 1e9
     traffic = pmc_traffic(dom, f"batched cap={cap}")
-    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": algo,
-                       "avg_launch_ms": avg_ms,
-                       "traffic_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9}
+    cp = copy_peak_gbs(torch)
+    out["roofline"] = with_copy_peak({"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                                      "algorithmic_bytes": algo, "avg_launch_ms": avg_ms,
+                                      "traffic_gbs": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9}, cp)
+    out["build_roofline"] = with_copy_peak(out["build_roofline"], cp)
 
     # ---- CPU baseline: the oracle (a C port of the reference's algorithm) on the host
     # cores: every set built with the PAR path's parallel encode + sort (serial BFS, as
@@ -509,6 +511,31 @@ def step_graph(octree_amd, ctx, d_pts, offs, cap, boxes, depths, d_q, k, dim, re
             "graph_build_ms": gb_ms * 1e3}
 
 
+def copy_peak_gbs(torch, nbytes=1 << 30, reps=10):
+    """Measured HBM peak beside the spec's 8 TB/s: a device-to-device copy of `nbytes`
+    (read + write counted), best of `reps`, timed with HIP events."""
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.synchronize()
+    return 2 * nbytes / (best * 1e-3) / 1e9
+
+
+def with_copy_peak(roof, copy_gbs):
+    """The roofline object with the measured copy peak and the fraction of it."""
+    return dict(roof, measured_copy_peak_gbs=copy_gbs, frac_of_measured_copy_peak=roof["achieved"] / copy_gbs)
+
+
 def knn_roofline(ctx, tree, d_q, k, D, set_ids=None):
     """Roofline of one kNN launch: minimal touched bytes (otg_knn_footprint: the query
     point 8D B, 4 B per returned id, a node record of 13 + 8D B -- entity segment 8,
@@ -556,7 +583,7 @@ def run_single(args, octree_amd, torch):
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
         out["step_graph"] = step_graph(octree_amd, ctx, d_pts, [0, len(pts)], 16, [box], 10, d_q, 8, 3,
                                        max(3, reps))
-        out["roofline"] = knn_roofline(ctx, tree, d_q, 8, 3)
+        out["roofline"] = with_copy_peak(knn_roofline(ctx, tree, d_q, 8, 3), copy_peak_gbs(torch))
         # Create<SeqExec>: the same tree numbered as the reference's default (unsorted) build
         seq = octree_amd.StaticOctreePoint(ctx=ctx)
         ts, _ = _timed(ctx, lambda: seq.Create(d_pts, 10, box, 16, exec_tag=octree_amd.SEQ_EXEC), reps)
@@ -684,7 +711,7 @@ def run_single(args, octree_amd, torch):
         plan.free()
         out["knn_graph"] = {"eager_ms": tk * 1e3, "graph_ms": tg * 1e3, "graph_qps": len(qs) / tg}
         out["step_graph"] = step_graph(octree_amd, ctx, d_p, [0, len(pts)], 32, [box], 8, d_q, 16, 6, max(3, reps))
-        out["roofline"] = knn_roofline(ctx, tree, d_q, 16, 6)
+        out["roofline"] = with_copy_peak(knn_roofline(ctx, tree, d_q, 16, 6), copy_peak_gbs(torch))
         # CPU: oracle build of a 1M-point sample and 20k queries on it
         m = 1_000_000
         out.update({"value": len(qs) / tk, "unit": "kNN queries/s", "ms_per_step": (tb + tk) * 1e3,
