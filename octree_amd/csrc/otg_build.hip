@@ -1073,7 +1073,7 @@ static void encode_sort(Tree *t, const double *dgeom, const uint64_t *d_off, uin
   const dim3 grid(div_up(n, kB * kPointItems<BOX ? 2 * D : D>));
   // the points arrive grouped by set: sorting inside each set skips the set-id digits
   // when that saves a pass (C3-like forests: 39 of 47 bits, five passes instead of six)
-  if (nsets > 1 && (low + 7) / 8 < (t->layout.total + 7) / 8 && radix_sets_fit(nsets, low)) {
+  if (nsets > 1 && (low + 7) / 8 < (t->layout.total + 7) / 8 && radix_sets_fit(n, nsets, low)) {
     // encode into the buffer that makes the last pass land in t->d_keys and vals
     const bool odd = ((low + 7) / 8) & 1u;
     Buf<uint64_t> k2(ctx, ar, n);

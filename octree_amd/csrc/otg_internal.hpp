@@ -222,8 +222,9 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN n, uint32_t
 // above (the set id of forest keys) are already in order.  The pairs start in (ka, va);
 // (kb, vb) is the other buffer of the ping-pong, so the result ends in (ka, va) after an
 // even number of passes ((bits + 7) / 8) and in (kb, vb) after an odd one.
-// radix_sets_fit says whether the per-set digit tables are small enough (few sets).
-bool radix_sets_fit(uint32_t nsets, uint32_t bits);
+// radix_sets_fit says whether the per-set digit tables are small enough and the sets
+// few enough against the tile count for the per-set sort to pay.
+bool radix_sets_fit(uint64_t n, uint32_t nsets, uint32_t bits);
 void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb, uint64_t n,
                      const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena);
 inline void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits,

@@ -625,16 +625,19 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN dn, uint32_
   }
 }
 
-bool radix_sets_fit(uint32_t nsets, uint32_t bits) {
+bool radix_sets_fit(uint64_t n, uint32_t nsets, uint32_t bits) {
   const uint32_t passes = (bits + 7) / 8;
-  return nsets > 0 && passes <= kOsMaxPasses && (uint64_t)nsets * passes * 256 * 4 <= kSegMaxTable;
+  // every set takes at least one tile: with more sets than the whole-array sort has
+  // tiles, most tiles would be nearly empty
+  return nsets > 0 && nsets <= div_up(n, kOsTile) && passes <= kOsMaxPasses &&
+         (uint64_t)nsets * passes * 256 * 4 <= kSegMaxTable;
 }
 
 void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb, uint64_t n,
                      const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena) {
   if (n == 0 || bits == 0) return;
   if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
-  if (!radix_sets_fit(nsets, bits)) throw Error{OTG_ERR_INVALID_ARG, "per-set radix tables too large (check radix_sets_fit first)"};
+  if (!radix_sets_fit(n, nsets, bits)) throw Error{OTG_ERR_INVALID_ARG, "per-set radix tables too large (check radix_sets_fit first)"};
   const uint32_t passes = (bits + 7) / 8;
   const uint64_t ntiles = div_up(n, kOsTile) + nsets;  // bound: one short tile per set
   Buf<OsState> st(ctx, arena, 1);
