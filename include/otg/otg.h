@@ -188,7 +188,7 @@ OTG_API otg_status otg_tree_deserialize(otg_ctx *ctx, int kind, int dim, const u
  * order_queries = 1 to walk batches of >= 16,384 queries in Morton order,
  * persistent = 1 to run the kNN walks as a persistent launch, range_records = the
  * records a RangeSearch wavefront count walk keeps per query for its fill (0: a
- * batch's record budget over its capped queries, 1024 to 8192).  Query results never
+ * batch's record budget over its capped queries, 256 to 8192).  Query results never
  * depend on them. */
 typedef struct {
   uint32_t leaf_capacity;

@@ -1474,7 +1474,7 @@ struct RangeKeep {
   uint32_t cap, xcap;
   uint32_t nlist;  // listed queries with slots (those after them are not kept)
 };
-constexpr uint32_t kKeepMin = 1024, kKeepMax = 8192, kKeepOverflow = 1u << 20;
+constexpr uint32_t kKeepMin = 256, kKeepMax = 8192, kKeepOverflow = 1u << 20;
 constexpr uint64_t kKeepBudget = 16ull << 20;  // records per batch (256 MiB)
 
 template <bool FILL>
@@ -3623,7 +3623,7 @@ void query_knn_noalloc(const Tree *t, uint64_t nq, const double *pts, const uint
 
 // One-lane walks stop counting after this many node visits; such queries, and those
 // with more than kRangeInline ids, are walked by a wavefront each (2-D / 3-D).
-constexpr uint32_t kRangeVisitCap = 64;
+constexpr uint32_t kRangeVisitCap = 32;
 
 // Positions (in walk order) of the flagged queries, as query indices.
 static uint32_t flagged_list(Ctx *ctx, const uint32_t *flags, const uint32_t *order, uint64_t nq, Buf<uint32_t> &list) {
