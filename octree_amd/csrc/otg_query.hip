@@ -749,7 +749,7 @@ __device__ __forceinline__ bool knn_query_levels(const TreeView &tv, uint64_t qi
 // sit at depths below the tree depth, which is at most 63 / D.
 template <int D>
 struct MaskLevels {
-  static constexpr int value = (D >= 4 && D <= 6) ? 63 / D : 1;
+  static constexpr int value = (D >= 2 && D <= 6) ? 63 / D : 1;
 };
 
 // kNN walk for 4-D..6-D with one LDS entry per level: the first child and a bit
@@ -929,20 +929,31 @@ __global__ void __launch_bounds__(kQB, 3)
 }
 #undef OTG_KNN_LDS
 
-// kNN of 4-D..6-D point trees with one wavefront per query (launched for 6-D: kKnnWave).  A node of such a tree has
-// up to 2^D <= 64 children and its leaves hold many entities, so the lanes test a
-// node's children (one each) or a 64-entity chunk of a leaf together; the candidate
-// list is spread over the lanes (slot t in lane t).  The walk is knn_query_masks's and
-// every decision is the same, made once per query instead of once per lane: the same
-// visit order (nearest child at once, the rest in index order, each re-tested when
-// taken), a chunk's candidates marked against the bound at its start and offered in
-// index order against the bound of that moment, the same insertion, limit and band
-// rule (knn_insert, knn_limit), so the ids are the lane walk's.  Four queries per block.
+// kNN of point trees with one wavefront per query (2-D..6-D; where it is launched:
+// kKnnWave).  A node has up to 2^D <= 64 children, so the lanes test a node's children
+// (one each) or a 64-entity chunk of a leaf together; the candidate list is spread
+// over the lanes (slot t in lane t).  The walk is knn_query_masks's and every decision
+// is the same, made once per query instead of once per lane: the same visit order
+// (nearest child at once, the rest in index order, each re-tested when taken), a
+// chunk's candidates marked against the bound at its start and offered in index order
+// against the bound of that moment, the same insertion, limit and band rule
+// (knn_insert, knn_limit), so the result is the masks walk's (in 2-D / 3-D the level
+// walk meets equally distant siblings in another order, as the masks walk does: the
+// ids are the same).  Four queries per block.
 constexpr int kWaveQueries = 4;
-// used for 6-D (measured on 8M clustered points, 1M queries k = 16: 133.8 -> 120.8 ms);
-// 4-D and 5-D keep the lane walk
+// Used for 6-D (measured on 8M clustered points, 1M queries k = 16: 133.8 -> 120.8 ms)
+// and for 2-D / 3-D batches of fewer than kWaveMaxQueries queries (the batches that
+// also skip the query sort), which one lane per query leaves too few wavefronts to fill
+// the GPU with.  Measured on 1M uniform 3-D points, k = 8, lane walk vs wavefront walk:
+// 1k queries 0.317 vs 0.115 ms, 10k 0.419 vs 0.274, 50k 0.574 vs 0.966, 1M 2.107 vs
+// 14.374.  4-D and 5-D keep the lane walk.
 template <int D, int KCAP, bool BOX>
-constexpr bool kKnnWave = D == 6 && !BOX && KCAP <= 64;
+constexpr bool kKnnWave = (D == 6 || D <= 3) && !BOX && KCAP <= 64;
+constexpr uint64_t kWaveMaxQueries = 16384;
+template <int D>
+static bool knn_wave_batch(uint64_t nq) {
+  return D == 6 || nq < kWaveMaxQueries;
+}
 
 template <int D, int KCAP>
 __global__ void __launch_bounds__(64 * kWaveQueries)
@@ -951,7 +962,7 @@ __global__ void __launch_bounds__(64 * kWaveQueries)
                const uint32_t *__restrict__ order, uint64_t nq, uint32_t k, double F0, double tol, KnnOut out,
                uint32_t *__restrict__ out_cnt, uint32_t *__restrict__ overflow, uint32_t *__restrict__ band,
                double *__restrict__ out_F) {
-  static_assert(D >= 4 && D <= 6 && KCAP <= 64, "one child per lane and one list slot per lane");
+  static_assert(D >= 2 && D <= 6 && KCAP <= 64, "one child per lane and one list slot per lane");
   constexpr int ML = MaskLevels<D>::value;
   __shared__ uint32_t s_cb[kWaveQueries][ML], s_lo[kWaveQueries][ML], s_hi[kWaveQueries][ML];
   const uint32_t lane = threadIdx.x & 63;
@@ -3409,7 +3420,7 @@ static Result *knn_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double *
   const KnnOut kout{ids.p, direct.p, ovf.p + 1};
   bool waved = false;
   if constexpr (kKnnWave<D, KCAP, BOX>) {
-    if (ctx->tuning.knn_wave) {
+    if (ctx->tuning.knn_wave && knn_wave_batch<D>(nq)) {
       launch(ctx, "knn", k_knn_wave<D, KCAP>, dim3(div_up(nq, kWaveQueries)), dim3(64 * kWaveQueries), 0, vs.views,
              vs.qview, q, qs, qk, (const uint32_t *)order.p, nq, k, F0, tol, kout, cnt.p, ovf.p, band.p, Fq.p);
       waved = true;
@@ -4416,7 +4427,7 @@ static void knn_plan_body(KnnPlan &p, Arena &ar) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, p.q, D, qs, nq, &ar);
   bool waved = false;
   if constexpr (kKnnWave<D, KCAP, BOX>) {
-    if (ctx->tuning.knn_wave) {
+    if (ctx->tuning.knn_wave && knn_wave_batch<D>(nq)) {
       launch(ctx, "knn", k_knn_wave<D, KCAP>, dim3(div_up(nq, kWaveQueries)), dim3(64 * kWaveQueries), 0, vs.views,
              vs.qview, p.q, qs, (const uint32_t *)nullptr, (const uint32_t *)order.p, nq, p.k, p.F0, p.tol, out, p.cnt,
              p.flags, p.band, p.Fq);
