@@ -1943,26 +1943,27 @@ __global__ void k_range_inline(const uint32_t *__restrict__ order, uint64_t nq, 
 __global__ void k_range_sort_keys(const uint32_t *__restrict__ hlist, uint32_t nheavy, const uint64_t *__restrict__ off,
                                   const uint32_t *__restrict__ hpos, const uint32_t *__restrict__ ids,
                                   uint64_t *__restrict__ keys, const uint32_t *__restrict__ nh,
-                                  const uint32_t *__restrict__ abort) {
+                                  const uint32_t *__restrict__ abort, uint32_t idbits) {
   if (abort && *abort) return;
   const uint32_t n = nh ? *nh : nheavy;
   for (uint32_t h = blockIdx.x; h < n; h += gridDim.x) {
     const uint32_t qi = hlist[h];
     const uint64_t o = off[qi], c = off[qi + 1] - o;
-    for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) keys[hpos[h] + t] = ((uint64_t)h << 32) | ids[o + t];
+    for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) keys[hpos[h] + t] = ((uint64_t)h << idbits) | ids[o + t];
   }
 }
 
 __global__ void k_range_sort_back(const uint32_t *__restrict__ hlist, uint32_t nheavy, const uint64_t *__restrict__ off,
                                   const uint32_t *__restrict__ hpos, const uint64_t *__restrict__ keys,
                                   uint32_t *__restrict__ ids, const uint32_t *__restrict__ nh,
-                                  const uint32_t *__restrict__ abort) {
+                                  const uint32_t *__restrict__ abort, uint32_t idbits) {
   if (abort && *abort) return;
   const uint32_t n = nh ? *nh : nheavy;
+  const uint64_t mask = idbits >= 32 ? 0xFFFFFFFFull : ((1ull << idbits) - 1);
   for (uint32_t h = blockIdx.x; h < n; h += gridDim.x) {
     const uint32_t qi = hlist[h];
     const uint64_t o = off[qi], c = off[qi + 1] - o;
-    for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) ids[o + t] = (uint32_t)keys[hpos[h] + t];
+    for (uint64_t t = threadIdx.x; t < c; t += blockDim.x) ids[o + t] = (uint32_t)(keys[hpos[h] + t] & mask);
   }
 }
 
@@ -3188,6 +3189,7 @@ struct ViewSet {
   uint32_t nviews = 1;
   uint32_t cbits = 0, sbits = 0;
   uint64_t max_nodes = 0, max_entities = 0;  // largest tree of the launch (bounds of ray walks)
+  uint64_t max_set = 0;                        // largest set: local ids are below it
 };
 
 static uint32_t bit_width32(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
@@ -3623,6 +3625,7 @@ static ViewSet single_view(const Tree *t) {
   vs.sbits = t->layout.sbits;
   vs.max_nodes = t->node_count;
   vs.max_entities = t->n;
+  for (const SetParams &sp : t->sets) vs.max_set = std::max<uint64_t>(vs.max_set, sp.ent_cnt);
   return vs;
 }
 
@@ -3697,7 +3700,12 @@ static uint32_t flagged_list(Ctx *ctx, const uint32_t *flags, const uint32_t *or
 
 // Sorts the ids of the listed queries of a CSR result (qlist null: queries 0..n-1)
 // ascending: one radix sort of (rank in the list, id) keys over all their ids.
-static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n) {
+// Bits of the local ids of a launch's trees (every id is below the largest set's size).
+static uint32_t id_bits(const ViewSet &vs) {
+  return std::max(1u, bit_width32((uint32_t)std::min<uint64_t>(std::max<uint64_t>(vs.max_set, 1) - 1, 0xFFFFFFFFull)));
+}
+
+static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n, uint32_t idbits) {
   if (n == 0) return;
   Buf<uint32_t> all;
   if (!qlist) {
@@ -3717,10 +3725,12 @@ static void sort_segments(Ctx *ctx, Result *r, const uint32_t *qlist, uint32_t n
   Buf<uint64_t> keys(ctx, nh);
   Buf<uint32_t> vals(ctx, nh);
   launch(ctx, "range_sort_keys", k_range_sort_keys, dim3(n), dim3(256), 0, qlist, n, (const uint64_t *)r->d_offsets,
-         (const uint32_t *)hpos.p, (const uint32_t *)r->d_ids, keys.p, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
-  radix_sort_pairs(ctx, keys.p, vals.p, nh, 32 + std::max(1u, bit_width32(n - 1)));
+         (const uint32_t *)hpos.p, (const uint32_t *)r->d_ids, keys.p, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+         idbits);
+  radix_sort_pairs(ctx, keys.p, vals.p, nh, idbits + std::max(1u, bit_width32(n - 1)));
   launch(ctx, "range_sort_back", k_range_sort_back, dim3(n), dim3(256), 0, qlist, n, (const uint64_t *)r->d_offsets,
-         (const uint32_t *)hpos.p, (const uint64_t *)keys.p, r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+         (const uint32_t *)hpos.p, (const uint64_t *)keys.p, r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+         idbits);
 }
 
 template <int D, bool BOX>
@@ -3816,7 +3826,7 @@ static Result *range_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const double
   launch(ctx, "range_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
          r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   // 6. the larger results sorted by id
-  sort_segments(ctx, r, hlist.p, nheavy);
+  sort_segments(ctx, r, hlist.p, nheavy, id_bits(vs));
   return r;
 }
 
@@ -3929,7 +3939,7 @@ static Result *planes_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doubl
   launch(ctx, "planes_copy", k_range_copy, dim3(div_up(hcopy, 4)), dim3(256), 0, (const RangeCopy *)copies.p, hcopy,
          r->d_ids, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
   // the wavefront walks append in no fixed order: every list is returned sorted by id
-  sort_segments(ctx, r, nullptr, (uint32_t)nq);
+  sort_segments(ctx, r, nullptr, (uint32_t)nq, id_bits(vs));
   return r;
 }
 
@@ -4242,6 +4252,7 @@ Result *query_multi(Ctx *ctx, int query, const std::vector<const Tree *> &trees,
     hv.push_back(query == 2 ? reference_view_of(x) : view_of(x));
     vs.max_nodes = std::max<uint64_t>(vs.max_nodes, x->node_count);
     vs.max_entities = std::max<uint64_t>(vs.max_entities, x->n);
+    for (const SetParams &sp : x->sets) vs.max_set = std::max<uint64_t>(vs.max_set, sp.ent_cnt);
     vs.cbits = std::max(vs.cbits, x->layout.cbits);
     vs.sbits = std::max(vs.sbits, x->layout.sbits);
     total_n += x->n;
@@ -4725,11 +4736,12 @@ static void range_plan_body(RangePlan &p, Arena &ar) {
   exclusive_scan_u32(ctx, hc.p, hpos.p, DevN{nq, F + 3}, F + 6, &ar);
   launch(ctx, "range_sort_keys", k_range_sort_keys, dim3(G), dim3(256), 0, (const uint32_t *)hlist.p, 0u,
          (const uint64_t *)p.offsets, (const uint32_t *)hpos.p, (const uint32_t *)p.ids, keys.p,
-         (const uint32_t *)(F + 3), abort);
-  radix_sort_pairs(ctx, keys.p, vals.p, DevN{p.cap_ids, F + 6}, 32 + std::max(1u, bit_width32((uint32_t)(nq - 1))), &ar);
+         (const uint32_t *)(F + 3), abort, id_bits(vs));
+  radix_sort_pairs(ctx, keys.p, vals.p, DevN{p.cap_ids, F + 6},
+                   id_bits(vs) + std::max(1u, bit_width32((uint32_t)(nq - 1))), &ar);
   launch(ctx, "range_sort_back", k_range_sort_back, dim3(G), dim3(256), 0, (const uint32_t *)hlist.p, 0u,
          (const uint64_t *)p.offsets, (const uint32_t *)hpos.p, (const uint64_t *)keys.p, p.ids,
-         (const uint32_t *)(F + 3), abort);
+         (const uint32_t *)(F + 3), abort, id_bits(vs));
   OTG_HIP_CHECK(hipMemcpyAsync(p.h_flags, F, kRangePlanFlags * 4, hipMemcpyDeviceToHost, ctx->stream));
 }
 
