@@ -560,6 +560,7 @@ OTG_API otg_status otg_tree_export(const otg_tree *t, uint32_t set, uint8_t *nod
       OTG_HIP_CHECK(hipMemcpyAsync(entity_ids, ids + eo, ne * 4, hipMemcpyDeviceToHost, st));
     }
     std::vector<uint64_t> keys;
+    if (sorted_keys && ne) otg::ensure_keys(t);
     if (sorted_keys && ne && !t->d_keys)
       throw Error{OTG_ERR_UNSUPPORTED, "a point tree loaded from an archive has no location keys (the archive holds none)"};
     if (sorted_keys && ne) {

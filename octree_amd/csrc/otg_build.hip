@@ -1105,6 +1105,23 @@ __global__ void __launch_bounds__(kB) k_coarse_keys(const uint64_t *__restrict__
   coarse[i] = ckv.set_base(s) | ((fkv.code(k) >> sh) << ckv.dbits);
 }
 
+// The reference tree's keys from its finer index's (same storage order).
+template <int D>
+static void coarse_keys_of(const Tree *t, const Tree *aux) {
+  Ctx *ctx = t->ctx;
+  const uint64_t n = t->n;
+  t->d_keys = (uint64_t *)ctx->alloc(std::max<uint64_t>(n, 1) * 8);
+  launch(ctx, "coarse_keys", k_coarse_keys<D>, dim3(div_up(n, kB)), dim3(kB), 0, (const uint64_t *)aux->d_keys, n,
+         KeyView{aux->layout.dbits, aux->layout.cbits}, (const SetParams *)aux->d_sets,
+         KeyView{t->layout.dbits, t->layout.cbits}, (const SetParams *)t->d_sets, t->d_keys);
+}
+
+template <int D>
+static void lazy_coarse_keys(const Tree *t) {
+  coarse_keys_of<D>(t, t->knn_tree);
+  t->ctx->sync();  // complete before any stream reads them
+}
+
 // 3-7. node table from the sorted keys t->d_keys.  herr: host copy of the encode
 // error bits, whose copy was queued before; checked after the first synchronisation
 // (a plain build).  ts: the capacities and device counts of a captured build.
@@ -1266,17 +1283,20 @@ static void reference_from_finer(Tree *t, const Tree *aux, Arena *ar = nullptr, 
 }
 
 // Sorted SoA geometry and local ids of the entity storage.
+// keys_of: the tree whose sorted keys give each position's set (a shared sort's
+// finer index, whose order is the storage's; null: t).
 template <int D, bool BOX>
 static void gather_entities(Tree *t, const double *dgeom, const uint32_t *vals, const uint64_t *d_off,
-                            Arena *ar = nullptr) {
+                            Arena *ar = nullptr, const Tree *keys_of = nullptr) {
   Ctx *ctx = t->ctx;
   constexpr int S = BOX ? 2 * D : D;
   const uint64_t n = t->n;
-  const KeyView kv{t->layout.dbits, t->layout.cbits};
+  const Tree *kt = keys_of ? keys_of : t;
+  const KeyView kv{kt->layout.dbits, kt->layout.cbits};
   t->d_geom = (double *)build_alloc(ctx, ar, n * S * 8);
   t->d_ids = (uint32_t *)build_alloc(ctx, ar, n * 4);
   launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, vals, n,
-         (const uint64_t *)t->d_keys, kv, d_off, t->d_geom, t->d_ids);
+         (const uint64_t *)kt->d_keys, kv, d_off, t->d_geom, t->d_ids);
 }
 
 template <int D>
@@ -1389,20 +1409,20 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
       upload_sets(aux);
       encode_sort<D, false>(aux, dgeom, d_off.p, vals.p, err.p);
       OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-      t->d_keys = (uint64_t *)ctx->alloc(n * 8);
-      launch(ctx, "coarse_keys", k_coarse_keys<D>, dim3(div_up(n, kB)), dim3(kB), 0, (const uint64_t *)aux->d_keys, n,
-             KeyView{aux->layout.dbits, aux->layout.cbits}, (const SetParams *)aux->d_sets,
-             KeyView{t->layout.dbits, t->layout.cbits}, (const SetParams *)t->d_sets, t->d_keys);
-      node_table<D, false>(aux, d_off.p, &herr);
       bool same_caps = true;
       for (uint32_t q = 0; q < nsets; ++q) same_caps &= aux->sets[q].cap == t->sets[q].cap;
+      // the reference keys: made now when the reference table needs them, else on first
+      // use (export, reference entity order) from the finer keys
+      if (!same_caps) coarse_keys_of<D>(t, aux);
+      node_table<D, false>(aux, d_off.p, &herr);
       if (same_caps) reference_from_finer<D>(t, aux);
       else node_table<D, false>(t, d_off.p, nullptr);
-      gather_entities<D, false>(t, dgeom, vals.p, d_off.p);
+      gather_entities<D, false>(t, dgeom, vals.p, d_off.p, nullptr, aux);
       aux->d_geom = t->d_geom;
       aux->d_ids = t->d_ids;
       aux->shares_entities = true;
       t->fine_order = true;
+      if (same_caps) t->lazy_keys = &lazy_coarse_keys<D>;
     } else {
       encode_sort<D, BOX>(t, dgeom, d_off.p, vals.p, err.p);
       OTG_HIP_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1533,9 +1553,16 @@ static void make_reference_order(const Tree *t) {
   t->d_view_ref = dv;
 }
 
+void ensure_keys(const Tree *t) {
+  if (!t->lazy_keys || t->n == 0) return;
+  std::lock_guard<std::mutex> g(t->ref_mu);
+  if (!t->d_keys) t->lazy_keys(t);
+}
+
 TreeView reference_view_of(const Tree *t) {
   TreeView hv = view_of(t);
   if (!t->fine_order || t->n == 0) return hv;
+  ensure_keys(t);
   std::lock_guard<std::mutex> g(t->ref_mu);
   if (!t->d_view_ref) make_reference_order(t);
   hv.geom = t->d_geom_ref;

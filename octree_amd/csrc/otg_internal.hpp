@@ -263,7 +263,7 @@ struct Tree {
   SetParams *d_sets = nullptr;
   // entity data in storage (sorted) order
   uint32_t *d_ids = nullptr;      // local entity ids
-  uint64_t *d_keys = nullptr;     // sorted keys
+  mutable uint64_t *d_keys = nullptr;  // sorted keys (a shared sort's reference tree: made on first use)
   double *d_geom = nullptr;       // SoA sorted geometry: comp*n + j; boxes: 2*dim comps (min..., max...)
   // nodes
   NodeInfo *d_info = nullptr;
@@ -288,6 +288,8 @@ struct Tree {
   // by id), made on first use and kept: ids, geometry and a device view of them.
   bool fine_order = false;
   mutable std::mutex ref_mu;
+  // makes d_keys from the finer index's keys when first needed (export, reference order)
+  void (*lazy_keys)(const Tree *) = nullptr;
   mutable uint32_t *d_ids_ref = nullptr;
   mutable double *d_geom_ref = nullptr;
   mutable TreeView *d_view_ref = nullptr;
@@ -326,6 +328,8 @@ void build_plan_run(BuildPlan *p);
 void build_plan_finish(BuildPlan *p);
 const Tree *build_plan_tree(const BuildPlan *p);
 hipGraph_t build_plan_graph(const BuildPlan *p);
+// The tree's sorted keys, made now if they are made on first use (Tree::lazy_keys).
+void ensure_keys(const Tree *t);
 hipGraphExec_t build_plan_exec(const BuildPlan *p);
 void build_plan_free(BuildPlan *p);
 // A build plan followed by a kNN plan on its tree, replayed as one graph; run()
