@@ -3174,6 +3174,112 @@ __global__ void __launch_bounds__(kQB) k_ray_all(const TreeView *__restrict__ vi
   });
 }
 
+// RayIntersectedAll with one wavefront per ray (2-D / 3-D), for batches too small to
+// fill the GPU one lane per ray: the frontier expansion and per-lane subtree walks of
+// k_range_wave, visiting the nodes the ray hits and keeping every entity hit within
+// max_dist, appended at the ray's offset in no particular order (the lists are sorted
+// by enter distance, ties by id, afterwards).  The same hits as ray_all_query's walk.
+template <int D, bool BOX, bool FILL>
+__global__ void __launch_bounds__(64) k_ray_all_wave(const TreeView *__restrict__ views,
+                                                      const uint32_t *__restrict__ qview,
+                                                      const double *__restrict__ rays, const uint32_t *__restrict__ qset,
+                                                      const uint32_t *__restrict__ order, uint64_t nq, double tol,
+                                                      double inc, double max_dist, uint32_t *__restrict__ counts,
+                                                      const uint64_t *__restrict__ offsets, double *__restrict__ out_d,
+                                                      uint32_t *__restrict__ out_id) {
+  static_assert(D <= 3, "frontier sized for up to 8 children per node");
+  __shared__ uint32_t s_front[64 * 8];
+  __shared__ uint32_t s_nf[2], s_cnt, s_next;
+  __shared__ uint32_t s_a[kRangeStack * 64];
+  __shared__ uint8_t s_b[kRangeStack * 64];
+  const uint64_t ti = blockIdx.x;
+  if (ti >= nq) return;
+  const uint32_t lane = threadIdx.x;
+  const uint64_t qi = order ? order[ti] : ti;
+  const TreeView &tv = views[qview ? qview[qi] : 0u];
+  const SetParams &sp = tv.sets[qset ? qset[qi] : 0u];
+  RayTester<D> rt;
+  const bool walk = sp.node_cnt != 0 && rt.make(&rays[qi * 2 * D], &rays[qi * 2 * D + D], tol, inc);
+  const uint64_t o = FILL ? offsets[qi] : 0;
+  // node visit: the node's entities hit, and whether to go on into its children
+  auto visit = [&](uint32_t node, NodeInfo &ni) -> bool {
+    double en, ex;
+    if (!rt.hit_node(&tv.nmin[(uint64_t)node * D], &tv.nsize[(uint64_t)node * D], &en, &ex)) return false;
+    ni = gload(tv.info + node);
+    for (uint32_t j = ni.ebegin; j < ni.ebegin + ni.ecount; ++j) {
+      bool hit;
+      if (BOX) {
+        double lo[D], hi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) lo[d] = tv.geom[(uint64_t)d * tv.n + j] - rt.origin[d];
+#pragma unroll
+        for (int d = 0; d < D; ++d) hi[d] = tv.geom[(uint64_t)(D + d) * tv.n + j] - rt.origin[d];
+        hit = rt.box_hit(lo, hi, &en, &ex);
+      } else {
+        double pnt[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) pnt[d] = tv.geom[(uint64_t)d * tv.n + j];
+        hit = rt.hit_point(pnt, &en, &ex);
+      }
+      if (!hit || !(max_dist == 0 || en <= max_dist)) continue;
+      const uint32_t at = atomicAdd(&s_cnt, 1u);
+      if (FILL) {
+        out_d[o + at] = en;
+        out_id[o + at] = tv.ids[j];
+      }
+    }
+    return ni.ccount != 0;
+  };
+  if (lane == 0) {
+    s_cnt = 0;
+    s_next = 0;
+    s_nf[0] = 0;
+    s_nf[1] = 0;
+    if (walk) {
+      s_front[0] = (uint32_t)sp.node_off;
+      s_nf[0] = 1;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  while (s_nf[cur] != 0 && s_nf[cur] < 64) {
+    const uint32_t nf = s_nf[cur];
+    const uint32_t mine = lane < nf ? s_front[lane] : 0u;  // read before the buffer is rewritten
+    if (lane == 0) s_nf[cur ^ 1] = 0;
+    __syncthreads();
+    if (lane < nf) {
+      NodeInfo ni;
+      if (visit(mine, ni)) {
+        const uint32_t at = atomicAdd(&s_nf[cur ^ 1], ni.ccount);
+        for (uint32_t c = 0; c < ni.ccount; ++c) s_front[at + c] = ni.cbegin + c;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const uint32_t nf = s_nf[cur];
+  LdsLevelIter stack{&s_a[lane], &s_b[lane], 64};
+  for (;;) {
+    const uint32_t item = atomicAdd(&s_next, 1u);
+    if (item >= nf) break;
+    int top = 0;
+    uint32_t node = s_front[item];
+    for (;;) {
+      NodeInfo ni;
+      if (visit(node, ni)) {
+        stack.nx(top) = ni.cbegin + ni.ccount - 1;
+        stack.left(top) = (uint8_t)ni.ccount;
+        ++top;
+      }
+      if (top == 0) break;
+      node = stack.nx(top - 1)--;
+      if (--stack.left(top - 1) == 0) --top;
+    }
+  }
+  __syncthreads();
+  if (!FILL && lane == 0) counts[qi] = s_cnt;
+}
+
 // ---------------------------------------------------------------------------
 // collision (boxes): every entity of the left set queries the right set's tree
 // with its own box.  Self collision (SELF: the same set on both sides) keeps
@@ -4202,16 +4308,33 @@ static Result *ray_all_impl(Ctx *ctx, const ViewSet &vs, uint64_t nq, const doub
                             double inc, double max_dist) {
   Buf<uint32_t> order = query_order<D>(ctx, vs, q, 2 * D, qs, nq);
   const uint32_t *ord = order.p;
+  const bool wave = D <= 3 && nq < kRayWaveMaxQueries;  // one wavefront per ray
   Buf<double> dist;
   Result *r = count_fill_csr(
       ctx, nq,
       [&](uint32_t *cnt) {
+        if constexpr (D <= 3) {
+          if (wave) {
+            launch(ctx, "ray_all_count", k_ray_all_wave<D, BOX, false>, dim3((uint32_t)nq), dim3(64), 0, vs.views,
+                   vs.qview, q, qs, ord, nq, tol, inc, max_dist, cnt, (const uint64_t *)nullptr, (double *)nullptr,
+                   (uint32_t *)nullptr);
+            return;
+          }
+        }
         launch(ctx, "ray_all_count", k_ray_all<D, BOX, false>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views,
                vs.qview, q, qs, ord, nq, tol, inc, max_dist, cnt, (const uint64_t *)nullptr, (double *)nullptr,
                (uint32_t *)nullptr);
       },
       [&](Result *res, uint32_t total) {
         dist = Buf<double>(ctx, std::max<uint32_t>(total, 1));
+        if constexpr (D <= 3) {
+          if (wave) {
+            launch(ctx, "ray_all_fill", k_ray_all_wave<D, BOX, true>, dim3((uint32_t)nq), dim3(64), 0, vs.views,
+                   vs.qview, q, qs, ord, nq, tol, inc, max_dist, (uint32_t *)nullptr,
+                   (const uint64_t *)res->d_offsets, dist.p, res->d_ids);
+            return;
+          }
+        }
         launch(ctx, "ray_all_fill", k_ray_all<D, BOX, true>, dim3(div_up(nq, kQB)), dim3(kQB), 0, vs.views, vs.qview,
                q, qs, ord, nq, tol, inc, max_dist, (uint32_t *)nullptr, (const uint64_t *)res->d_offsets, dist.p,
                res->d_ids);
