@@ -493,34 +493,6 @@ __device__ __forceinline__ uint32_t internal_gen(const uint32_t *lm, const uint3
   return (uint32_t)__popc(anc) + 1u;
 }
 
-// events per position p in [0, n): bit 0 segment start, bits 1-2 internal nodes
-// (extra root + internal representative)
-__global__ void __launch_bounds__(kB) k_event_flags(const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
-                                                     const uint32_t *__restrict__ lm, uint64_t n,
-                                                     const uint64_t *__restrict__ keys, KeyView kv,
-                                                     const SetParams *__restrict__ sets, uint8_t *__restrict__ ev) {
-  uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t s = kv.set(keys[p]);
-  const SetParams &sp = sets[s];
-  const bool set_start = (sp.ent_off == p);
-  uint32_t ic = 0;
-  if (set_start && sp.root_splits && sp.min_lcp > 1u) ic = 1;  // extra root above a deeper single child
-  if (!set_start && active[p]) {
-    const uint32_t vp = v[p];
-    if (!((lm[p - 1] >> vp) & 1u)) ++ic;  // leftmost boundary of its node
-  }
-  ev[p] = (uint8_t)(((set_start || active[p]) ? 1u : 0u) | (ic << 1));
-}
-
-// nseg: the segment count (device); seg_list holds cap + 1 entries
-__global__ void __launch_bounds__(kB) k_seg_compact(const uint8_t *__restrict__ ev, const uint64_t *__restrict__ idx,
-                                                     uint64_t n, uint32_t *__restrict__ seg_list, DevN nseg) {
-  uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (p < n && (ev[p] & 1u) && (idx[p] >> 32) < nseg.cap) seg_list[idx[p] >> 32] = (uint32_t)p;
-  if (p == 0) seg_list[nseg.get()] = (uint32_t)n;
-}
-
 // Which segments are nodes (leaves): all except single non-refinable boxes.
 template <bool BOX>
 __global__ void __launch_bounds__(kB) k_seg_nodes(const uint32_t *__restrict__ seg_list, DevN dseg,
@@ -555,32 +527,6 @@ struct NodeKeyLayout {
 
 constexpr uint32_t kTypeRoot = 0u, kTypeInternal = 1u, kTypeLeaf = 2u;
 
-__global__ void __launch_bounds__(kB) k_emit_internal(const uint8_t *__restrict__ ev, const uint64_t *__restrict__ idx,
-                                                       const uint8_t *__restrict__ v, const uint8_t *__restrict__ active,
-                                                       const uint32_t *__restrict__ lm, const uint32_t *__restrict__ rm,
-                                                       uint64_t n, const uint64_t *__restrict__ keys, KeyView kv,
-                                                       const SetParams *__restrict__ sets, NodeKeyLayout nk,
-                                                       uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc,
-                                                       uint64_t cap) {
-  uint64_t p = (uint64_t)blockIdx.x * kB + threadIdx.x;
-  if (p >= n || !(ev[p] >> 1)) return;
-  if ((uint64_t)(uint32_t)idx[p] + (ev[p] >> 1) > cap) return;  // beyond the capacity (flagged by the caller)
-  const uint32_t s = kv.set(keys[p]);
-  const SetParams &sp = sets[s];
-  uint32_t out = (uint32_t)idx[p];
-  const bool set_start = (sp.ent_off == p);
-  if (set_start && sp.root_splits && sp.min_lcp > 1u) {
-    nkeys[out] = nk.make(s, 0, 0);
-    ndesc[out] = (kTypeRoot << 30) | (uint32_t)p;
-    ++out;
-  }
-  if (!set_start && active[p] && !((lm[p - 1] >> v[p]) & 1u)) {
-    uint32_t gen = internal_gen(lm, rm, v, p);
-    nkeys[out] = nk.make(s, gen, 2 * (p - sp.ent_off) - 1);
-    ndesc[out] = (kTypeInternal << 30) | (uint32_t)p;
-  }
-}
-
 __global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__ seg_list, DevN dseg,
                                                      const uint32_t *__restrict__ seg_node, const uint32_t *__restrict__ seg_node_idx,
                                                      const uint32_t *__restrict__ nint, uint64_t cap, const uint8_t *__restrict__ v,
@@ -608,6 +554,150 @@ __global__ void __launch_bounds__(kB) k_emit_leaves(const uint32_t *__restrict__
   }
   nkeys[out] = nk.make(s, gen, 2 * (s0 - sp.ent_off));
   ndesc[out] = (kTypeLeaf << 30) | k;
+}
+
+// Node events at position p in [0, n): bit 0 a segment starts at p, bit 1 an internal
+// node is represented by p (the extra root above a deeper single child at a set's
+// start, else the leftmost boundary of its node).  A set starts where the level is 0,
+// so the key (for the set) is read only there.  The events are never stored: the
+// counting launch and the emitting launch both evaluate them from the levels, the
+// split flags and the left level masks.
+struct EventCalc {
+  const uint8_t *v, *active;
+  const uint32_t *lm;
+  const uint64_t *keys;
+  KeyView kv;
+  const SetParams *sets;
+  __device__ __forceinline__ uint32_t at(uint64_t p, uint32_t vp, uint32_t ap) const {
+    if (vp == 0) {
+      const SetParams &sp = sets[kv.set(keys[p])];
+      return (sp.root_splits && sp.min_lcp > 1u) ? 3u : 1u;
+    }
+    if (!ap) return 0u;
+    return ((lm[p - 1] >> vp) & 1u) ? 1u : 3u;
+  }
+  // events of the 8 positions [b, b + 8) (those at or past n are 0); b is a multiple
+  // of 8, and v and active hold 8 bytes of padding past n
+  __device__ __forceinline__ void eight(uint64_t b, uint64_t n, uint32_t e[8]) const {
+    if (b >= n) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) e[i] = 0u;
+      return;
+    }
+    const uint64_t vw = *reinterpret_cast<const uint64_t *>(v + b);
+    const uint64_t aw = *reinterpret_cast<const uint64_t *>(active + b);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      e[i] = (b + i < n) ? at(b + i, (uint32_t)(vw >> (8 * i)) & 0xFFu, (uint32_t)(aw >> (8 * i)) & 0xFFu) : 0u;
+  }
+};
+// segments in the high 32 bits, internal nodes in the low 32 (each total < 2^31)
+__device__ __forceinline__ uint64_t event_count(uint32_t e) { return ((uint64_t)(e & 1u) << 32) | (e >> 1); }
+
+constexpr uint32_t kEvTile = kB * 8;  // positions per block, 8 per thread
+
+// Exclusive block-wide sum of one value per thread; *total = the block's sum.
+__device__ __forceinline__ uint64_t event_block_scan(uint64_t x, uint64_t *lds /* kB / 64 */, uint64_t *total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t inc = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up((unsigned long long)inc, off, 64);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kB / 64; ++w) {
+    if (w < wave) pre += lds[w];
+    tot += lds[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + inc - x;
+}
+
+__global__ void __launch_bounds__(kB) k_event_count(EventCalc ec, uint64_t n, uint64_t *__restrict__ partials) {
+  __shared__ uint64_t lds[kB / 64];
+  uint32_t e[8];
+  ec.eight((uint64_t)blockIdx.x * kEvTile + (uint64_t)threadIdx.x * 8, n, e);
+  uint64_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc += event_count(e[i]);
+  uint64_t tot;
+  (void)event_block_scan(acc, lds, &tot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// One block: the block sums become exclusive block offsets; totals[0] = segments,
+// totals[1] = internal nodes.
+__global__ void __launch_bounds__(kB) k_event_offsets(uint64_t *__restrict__ partials, uint32_t nb,
+                                                       uint32_t *__restrict__ totals) {
+  __shared__ uint64_t lds[kB / 64];
+  uint64_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += kEvTile) {
+    const uint32_t b = b0 + threadIdx.x * 8;
+    uint64_t x[8], acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      x[i] = (b + i < nb) ? partials[b + i] : 0ull;
+      acc += x[i];
+    }
+    uint64_t tot;
+    uint64_t run = carry + event_block_scan(acc, lds, &tot);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (b + i < nb) partials[b + i] = run;
+      run += x[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    totals[0] = (uint32_t)(carry >> 32);
+    totals[1] = (uint32_t)carry;
+  }
+}
+
+// The segment list (segment k starts at seg_list[k]; seg_list[segments] = n) and the
+// internal nodes (keys and descriptors at their event rank); entries beyond a
+// capacity are dropped (the caller flags the overflow).
+__global__ void __launch_bounds__(kB) k_event_emit(EventCalc ec, uint64_t n, const uint64_t *__restrict__ partials,
+                                                    const uint32_t *__restrict__ rm, NodeKeyLayout nk,
+                                                    uint32_t *__restrict__ seg_list, DevN nseg,
+                                                    uint64_t *__restrict__ nkeys, uint32_t *__restrict__ ndesc,
+                                                    uint64_t cap_nodes) {
+  __shared__ uint64_t lds[kB / 64];
+  const uint64_t b = (uint64_t)blockIdx.x * kEvTile + (uint64_t)threadIdx.x * 8;
+  uint32_t e[8];
+  ec.eight(b, n, e);
+  uint64_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc += event_count(e[i]);
+  uint64_t tot;
+  uint64_t run = partials[blockIdx.x] + event_block_scan(acc, lds, &tot);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (e[i]) {
+      const uint64_t p = b + i;
+      const uint64_t seg = run >> 32;
+      const uint32_t out = (uint32_t)run;
+      if (seg < nseg.cap) seg_list[seg] = (uint32_t)p;
+      if ((e[i] >> 1) && out < cap_nodes) {
+        const uint32_t s = ec.kv.set(ec.keys[p]);
+        const SetParams &sp = ec.sets[s];
+        if (ec.v[p] == 0) {
+          nkeys[out] = nk.make(s, 0, 0);
+          ndesc[out] = (kTypeRoot << 30) | (uint32_t)p;
+        } else {
+          nkeys[out] = nk.make(s, internal_gen(ec.lm, rm, ec.v, p), 2 * (p - sp.ent_off) - 1);
+          ndesc[out] = (kTypeInternal << 30) | (uint32_t)p;
+        }
+      }
+    }
+    run += event_count(e[i]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) seg_list[nseg.get()] = (uint32_t)n;
 }
 
 // counts[3] = nodes = internal nodes (counts[1]) + leaves (counts[2]).  A count beyond
@@ -1144,7 +1234,7 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Are
          t->d_sets, nsets, kv);
 
   // 4. active boundaries
-  Buf<uint8_t> active(ctx, ar, n + 1);
+  Buf<uint8_t> active(ctx, ar, n + 9);  // 8 bytes of padding for word-wise reads
   uint64_t max_cap = 0;
   for (const SetParams &sp : t->sets) max_cap = std::max<uint64_t>(max_cap, sp.cap);
   if (max_cap + 8 < kActHalo)
@@ -1158,19 +1248,19 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Are
   Buf<uint32_t> lm(ctx, ar, n + 1), rm(ctx, ar, n + 1);
   level_scans(ctx, v.p, n + 1, lm.p, rm.p, ar);
 
-  // 6. node events
-  Buf<uint8_t> ev(ctx, ar, n);
-  Buf<uint64_t> eidx(ctx, ar, n);
-  launch(ctx, "event_flags", k_event_flags, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)v.p,
-         (const uint8_t *)active.p, (const uint32_t *)lm.p, n, (const uint64_t *)t->d_keys, kv,
-         (const SetParams *)t->d_sets, ev.p);
+  // 6. node events: counted per block, block offsets and totals, then (capacities
+  // known) the segment list and the internal nodes emitted in one pass
+  const EventCalc ec{v.p, active.p, lm.p, t->d_keys, kv, t->d_sets};
+  const uint32_t nbe = div_up(n, kEvTile);
+  Buf<uint64_t> epart(ctx, ar, std::max<uint32_t>(nbe, 1));
   Buf<uint32_t> own_counts;
   uint32_t *counts = ts ? ts->counts : nullptr;
   if (!counts) {
     own_counts = Buf<uint32_t>(ctx, ar, 4);
     counts = own_counts.p;
   }
-  event_scan(ctx, ev.p, eidx.p, n, counts, ar);
+  launch(ctx, "event_count", k_event_count, dim3(nbe), dim3(kB), 0, ec, n, epart.p);
+  launch(ctx, "event_offsets", k_event_offsets, dim3(1), dim3(kB), 0, epart.p, nbe, counts);
   uint64_t cap_seg, cap_nodes;
   if (!ts) {
     uint32_t ht[2];
@@ -1187,9 +1277,15 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Are
   }
   const DevN dseg{cap_seg, counts};
 
+  NodeKeyLayout nk;
+  nk.cobits = 64 - __builtin_clzll(2 * max_n + 1);
+  nk.gbits = 5;
+  if (nk.cobits + nk.gbits + L.sbits > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "node key overflow"};
   Buf<uint32_t> seg_list(ctx, ar, cap_seg + 1), seg_node(ctx, ar, cap_seg), seg_node_idx(ctx, ar, cap_seg);
-  launch(ctx, "seg_compact", k_seg_compact, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)ev.p,
-         (const uint64_t *)eidx.p, n, seg_list.p, dseg);
+  Buf<uint64_t> nkeys(ctx, ar, cap_nodes);
+  Buf<uint32_t> ndesc(ctx, ar, cap_nodes);
+  launch(ctx, "event_emit", k_event_emit, dim3(nbe), dim3(kB), 0, ec, n, (const uint64_t *)epart.p,
+         (const uint32_t *)rm.p, nk, seg_list.p, dseg, nkeys.p, ndesc.p, cap_nodes);
   launch(ctx, "seg_nodes", k_seg_nodes<BOX>, dim3(div_up(cap_seg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p,
          dseg, (const uint8_t *)v.p, (const uint64_t *)t->d_keys, d_off, nsets, (const SetParams *)t->d_sets, kv,
          seg_node.p);
@@ -1204,17 +1300,6 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Are
   launch(ctx, "node_total", k_node_total, dim3(1), dim3(1), 0, counts, cap_seg, cap_nodes,
          ts ? ts->overflow : (uint32_t *)nullptr);
   const DevN dnodes{cap_nodes, counts + 3};
-
-  NodeKeyLayout nk;
-  nk.cobits = 64 - __builtin_clzll(2 * max_n + 1);
-  nk.gbits = 5;
-  if (nk.cobits + nk.gbits + L.sbits > 64) throw Error{OTG_ERR_KEY_OVERFLOW, "node key overflow"};
-  Buf<uint64_t> nkeys(ctx, ar, cap_nodes);
-  Buf<uint32_t> ndesc(ctx, ar, cap_nodes);
-  launch(ctx, "emit_internal", k_emit_internal, dim3(div_up(n, kB)), dim3(kB), 0, (const uint8_t *)ev.p,
-         (const uint64_t *)eidx.p, (const uint8_t *)v.p, (const uint8_t *)active.p, (const uint32_t *)lm.p,
-         (const uint32_t *)rm.p, n, (const uint64_t *)t->d_keys, kv, (const SetParams *)t->d_sets, nk, nkeys.p,
-         ndesc.p, cap_nodes);
   launch(ctx, "emit_leaves", k_emit_leaves, dim3(div_up(cap_seg, kB)), dim3(kB), 0, (const uint32_t *)seg_list.p, dseg,
          (const uint32_t *)seg_node.p, (const uint32_t *)seg_node_idx.p, (const uint32_t *)(counts + 1), cap_nodes,
          (const uint8_t *)v.p, (const uint32_t *)lm.p, (const uint32_t *)rm.p, (const uint64_t *)t->d_keys, kv,

@@ -209,11 +209,6 @@ inline void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, uint
 }
 // a[0..n) = v by a kernel (no memset node when captured into a graph)
 void fill_u32(Ctx *ctx, uint32_t *a, uint64_t n, uint32_t v);
-// Build node events (n > 0 bytes: bit 0 segment start, bits 1-2 internal nodes):
-// idx[p] = exclusive counts before p, segments in the high 32 bits and internal nodes
-// in the low 32; d_totals[0..1] = the two totals.
-void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals,
-                Arena *arena = nullptr);
 // Stable LSD radix sort of (key, val) pairs on bits [0, bits).  Buffers are swapped internally; the
 // result is in keys/vals on return.
 void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN n, uint32_t bits, Arena *arena);

@@ -189,53 +189,6 @@ __global__ void __launch_bounds__(kBlock) k_scan_final_level(LevelSource src, ui
   }
 }
 
-// Node events of the build: bit 0 = segment start, bits 1-2 = internal nodes here.
-// Both counts are scanned at once as one 64-bit sum (segments in the high half; each
-// total is below 2^31, so the halves never carry into each other).
-struct EventSource {
-  const uint8_t *e;
-  __device__ uint64_t operator()(uint64_t i) const {
-    const uint32_t x = e[i];
-    return ((uint64_t)(x & 1u) << 32) | (x >> 1);
-  }
-};
-
-__global__ void __launch_bounds__(kBlock) k_scan_final_events(EventSource src, uint64_t *out, uint64_t n,
-                                                               const uint64_t *partials, uint32_t *d_totals,
-                                                               uint32_t nblocks) {
-  __shared__ uint64_t lds[kBlock / 64];
-  uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
-  uint64_t items[kItems];
-  uint64_t acc = 0;
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    items[i] = (base + i < n) ? src(base + i) : 0ull;
-    acc += items[i];
-  }
-  uint64_t ex, tot;
-  block_scan<OpAdd64>(acc, &ex, &tot, lds);
-  uint64_t run = partials[blockIdx.x] + ex;
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    if (base + i < n) out[base + i] = run;
-    run += items[i];
-  }
-  if (blockIdx.x == nblocks - 1 && threadIdx.x == kBlock - 1) {
-    d_totals[0] = (uint32_t)(run >> 32);
-    d_totals[1] = (uint32_t)run;
-  }
-}
-
-void event_scan(Ctx *ctx, const uint8_t *events, uint64_t *idx, uint64_t n, uint32_t *d_totals, Arena *arena) {
-  const uint32_t nb = div_up(n, kTile);
-  Buf<uint64_t> partials(ctx, arena, nb);
-  launch(ctx, "scan_reduce_events", k_scan_reduce<OpAdd64, EventSource>, dim3(nb), dim3(kBlock), 0,
-         EventSource{events}, DevN{n, nullptr}, partials.p);
-  launch(ctx, "scan_partials_u32", k_scan_partials<OpAdd64>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
-  launch(ctx, "scan_final_events", k_scan_final_events, dim3(nb), dim3(kBlock), 0, EventSource{events}, idx, n,
-         (const uint64_t *)partials.p, d_totals, nb);
-}
-
 __global__ void k_fill_words(uint32_t *__restrict__ a, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     a[i] = v;
