@@ -402,46 +402,72 @@ __global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys
 }
 
 // The same test with the levels of a block's 2048 boundaries and a halo on either
-// side staged in LDS: the runs are walked there instead of through the cache.  Every
-// set's cap + 8 must fit the halo (the host checks).
+// side staged in LDS, together with the minimum level of every 8-byte word: a walk
+// finishes its own word byte by byte, then skips 64 levels per step on the word minima
+// until a word holds a smaller level, and ends inside that word.  The set of a
+// boundary comes from the set offsets, not from its key.  Every set's cap + 64 must
+// fit the halo (the host checks): a walk that has not yet reached its limit stays
+// inside the window.
 constexpr uint32_t kActTile = 2048;  // boundaries per block, 8 per thread
 constexpr uint32_t kActHalo = 1024;  // bytes of levels staged on either side
+constexpr uint32_t kActWords = (kActTile + 2 * kActHalo) / 8;
 
-__device__ __forceinline__ uint32_t lds_run_right(const uint64_t *sv, uint32_t lp, uint32_t vi, uint32_t limit) {
-  uint32_t c = 0;
+// consecutive levels >= vi from window position lp upwards (at least `limit` when
+// the run reaches that far)
+__device__ __forceinline__ uint32_t lds_run_right(const uint64_t *sv, const uint64_t *smin, uint32_t lp, uint32_t vi,
+                                                  uint32_t limit) {
+  const uint32_t o = lp & 7u;
+  const uint64_t m = bytes_below(sv[lp >> 3], vi) & (~0ull << (8 * o));
+  if (m) return ((uint32_t)__builtin_ctzll(m) >> 3) - o;
+  uint32_t c = 8 - o;
+  uint32_t w = (lp >> 3) + 1;
   while (c < limit) {
-    const uint32_t o = lp & 7u;
-    const uint64_t m = bytes_below(sv[lp >> 3], vi) & (~0ull << (8 * o));
-    if (m) return c + ((uint32_t)__builtin_ctzll(m) >> 3) - o;
-    c += 8 - o;
-    lp = (lp & ~7u) + 8;
+    const uint32_t ow = w & 7u;
+    const uint64_t mm = bytes_below(smin[w >> 3], vi) & (~0ull << (8 * ow));
+    if (mm) {
+      const uint32_t wf = (w & ~7u) + ((uint32_t)__builtin_ctzll(mm) >> 3);
+      return c + 8 * (wf - w) + ((uint32_t)__builtin_ctzll(bytes_below(sv[wf], vi)) >> 3);
+    }
+    c += 8 * (8 - ow);
+    w = (w & ~7u) + 8;
   }
   return c;
 }
 
-__device__ __forceinline__ uint32_t lds_run_left(const uint64_t *sv, uint32_t lp, uint32_t vi, uint32_t limit) {
-  uint32_t c = 0;
-  while (c < limit) {
-    const uint32_t o = lp & 7u;
-    const uint64_t keep = (o == 7) ? ~0ull : ((1ull << (8 * (o + 1))) - 1ull);
-    const uint64_t m = bytes_below(sv[lp >> 3], vi) & keep;
-    if (m) return c + o - ((63u - (uint32_t)__builtin_clzll(m)) >> 3);
-    c += o + 1;
-    lp = (lp & ~7u) - 1;  // stays inside the window: the halo exceeds limit + 7
+// the same from lp downwards
+__device__ __forceinline__ uint32_t lds_run_left(const uint64_t *sv, const uint64_t *smin, uint32_t lp, uint32_t vi,
+                                                 uint32_t limit) {
+  const uint32_t o = lp & 7u;
+  const uint64_t keep = (o == 7) ? ~0ull : ((1ull << (8 * (o + 1))) - 1ull);
+  const uint64_t m = bytes_below(sv[lp >> 3], vi) & keep;
+  if (m) return o - ((63u - (uint32_t)__builtin_clzll(m)) >> 3);
+  uint32_t c = o + 1;
+  int32_t w = (int32_t)(lp >> 3) - 1;
+  while (c < limit) {  // c < limit keeps w >= 0: every level below the window start is counted
+    const uint32_t ow = (uint32_t)w & 7u;
+    const uint64_t keepw = (ow == 7) ? ~0ull : ((1ull << (8 * (ow + 1))) - 1ull);
+    const uint64_t mm = bytes_below(smin[w >> 3], vi) & keepw;
+    if (mm) {
+      const int32_t wf = (w & ~7) + (int32_t)((63u - (uint32_t)__builtin_clzll(mm)) >> 3);
+      const uint64_t mf = bytes_below(sv[wf], vi);
+      return c + 8 * (uint32_t)(w - wf) + 7u - ((63u - (uint32_t)__builtin_clzll(mf)) >> 3);
+    }
+    c += 8 * (ow + 1);
+    w = (w & ~7) - 1;
   }
   return c;
 }
 
 template <bool BOX>
 __global__ void __launch_bounds__(kB) k_active_lds(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ v,
-                                                    uint64_t n, KeyView kv, const SetParams *__restrict__ sets,
-                                                    uint8_t *__restrict__ active) {
-  constexpr uint32_t kWords = (kActTile + 2 * kActHalo) / 8;
-  __shared__ uint64_t sv[kWords];
+                                                    uint64_t n, const uint64_t *__restrict__ set_off, uint32_t nsets,
+                                                    const SetParams *__restrict__ sets, uint8_t *__restrict__ active) {
+  __shared__ uint64_t sv[kActWords];
+  __shared__ uint64_t smin[kActWords / 8];
   const uint64_t i0 = (uint64_t)blockIdx.x * kActTile;
   const int64_t w0 = (int64_t)i0 - (int64_t)kActHalo;  // byte position of sv[0]
   // levels v[0..n] (bytes past n read as 0: every walk stops at v[n] = 0 anyway)
-  for (uint32_t w = threadIdx.x; w < kWords; w += kB) {
+  for (uint32_t w = threadIdx.x; w < kActWords; w += kB) {
     const int64_t b = w0 + 8 * (int64_t)w;
     uint64_t x = 0;
     if (b >= 0 && (uint64_t)b <= n) {
@@ -452,8 +478,16 @@ __global__ void __launch_bounds__(kB) k_active_lds(const uint64_t *__restrict__ 
       }
     }
     sv[w] = x;
+    uint32_t mn = 255u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mn = min(mn, (uint32_t)(x >> (8 * j)) & 0xFFu);
+    reinterpret_cast<uint8_t *>(smin)[w] = (uint8_t)mn;
   }
   __syncthreads();
+  // set of the thread's first boundary (the last set starting at or before it), then
+  // forward as the boundaries grow
+  uint32_t s = (uint32_t)upper_bound_u64(set_off, 0, nsets, i0 + threadIdx.x);
+  s = s ? s - 1 : 0u;
 #pragma unroll
   for (uint32_t j = 0; j < kActTile / kB; ++j) {
     const uint64_t i = i0 + (uint64_t)j * kB + threadIdx.x;
@@ -462,12 +496,12 @@ __global__ void __launch_bounds__(kB) k_active_lds(const uint64_t *__restrict__ 
     const uint32_t lp = (uint32_t)((int64_t)i - w0);
     const uint32_t vi = (i > 0 && i < n) ? (uint32_t)((sv[lp >> 3] >> (8 * (lp & 7u))) & 0xFFu) : 0u;
     if (vi) {
-      const uint32_t s = kv.set(keys[i]);
+      while (s + 1 < nsets && set_off[s + 1] <= i) ++s;
       const SetParams &sp = sets[s];
       if (vi - 1 < sp.max_depth) {
         const uint32_t lim = sp.cap;
-        uint32_t cnt = 2 + lds_run_left(sv, lp - 1, vi, lim);
-        if (cnt <= lim) cnt += lds_run_right(sv, lp + 1, vi, lim);
+        uint32_t cnt = 2 + lds_run_left(sv, smin, lp - 1, vi, lim);
+        if (cnt <= lim) cnt += lds_run_right(sv, smin, lp + 1, vi, lim);
         if (cnt > lim) {
           a = 1;
           if (BOX && keys[i - 1] == keys[i]) {
@@ -1237,9 +1271,9 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Are
   Buf<uint8_t> active(ctx, ar, n + 9);  // 8 bytes of padding for word-wise reads
   uint64_t max_cap = 0;
   for (const SetParams &sp : t->sets) max_cap = std::max<uint64_t>(max_cap, sp.cap);
-  if (max_cap + 8 < kActHalo)
+  if (max_cap + 64 < kActHalo)
     launch(ctx, "active", k_active_lds<BOX>, dim3(div_up(n + 1, kActTile)), dim3(kB), 0, (const uint64_t *)t->d_keys,
-           (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
+           (const uint8_t *)v.p, n, d_off, nsets, (const SetParams *)t->d_sets, active.p);
   else
     launch(ctx, "active", k_active<BOX>, dim3(div_up(n + 1, kB)), dim3(kB), 0, (const uint64_t *)t->d_keys,
            (const uint8_t *)v.p, n, kv, (const SetParams *)t->d_sets, active.p);
