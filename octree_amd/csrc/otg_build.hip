@@ -1280,7 +1280,7 @@ static void node_table(Tree *t, const uint64_t *d_off, const uint32_t *herr, Are
 
   // 5. level scans
   Buf<uint32_t> lm(ctx, ar, n + 1), rm(ctx, ar, n + 1);
-  level_scans(ctx, v.p, n + 1, lm.p, rm.p, ar);
+  level_scans(ctx, v.p, n + 1, lm.p, rm.p, active.p, ar);  // only beside split boundaries: all that is read
 
   // 6. node events: counted per block, block offsets and totals, then (capacities
   // known) the segment list and the internal nodes emitted in one pass

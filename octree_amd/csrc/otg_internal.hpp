@@ -228,9 +228,10 @@ inline void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t 
 }
 // Boundary-level scans: for v[0..n) (uint8 levels, 0 = hard boundary) computes
 // left[i] = set of running minima of v[i], v[i-1], ... (inclusive, stops at level 0) and
-// right[i] = same going right, as bit masks.
+// right[i] = same going right, as bit masks.  need (null: every position): only the
+// positions x with need[x + 1] (left) or need[x - 1] (right) set are written.
 void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask,
-                 Arena *arena = nullptr);
+                 const uint8_t *need = nullptr, Arena *arena = nullptr);
 
 // Device-side view of one tree, passed to traversal kernels (an array of them
 // lets one launch serve queries on many trees).

@@ -87,16 +87,6 @@ struct U32Source {
   const uint32_t *p;
   __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
 };
-struct LevelSource {
-  const uint8_t *v;
-  uint64_t n;
-  bool reverse;
-  __device__ LevelState operator()(uint64_t i) const {
-    uint32_t x = v[reverse ? (n - 1 - i) : i];
-    return LevelState{x, 1u << x};
-  }
-};
-
 template <class Op, class Src>
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(Src src, DevN dn, typename Op::T *partials) {
   using T = typename Op::T;
@@ -113,9 +103,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(Src src, DevN dn, typena
 }
 
 // Exclusive scan of up to any number of partials with a single block (carry loop).
+// Block b scans the b-th array of n partials (partials + b * n).
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_scan_partials(typename Op::T *partials, uint64_t n) {
   using T = typename Op::T;
+  partials += (uint64_t)blockIdx.x * n;
   __shared__ T lds[kBlock / 64];
   T carry = Op::identity();
   for (uint64_t base = 0; base < n; base += kTile) {
@@ -167,28 +159,6 @@ __global__ void __launch_bounds__(kBlock) k_scan_final_u32(const uint32_t *in, u
     *d_total = run > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)run;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_final_level(LevelSource src, uint64_t n,
-                                                              const LevelState *partials, uint32_t *mask_out) {
-  __shared__ LevelState lds[kBlock / 64];
-  uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
-  LevelState items[kItems];
-  LevelState acc = OpLevel::identity();
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    items[i] = (base + i < n) ? src(base + i) : OpLevel::identity();
-    acc = OpLevel::combine(acc, items[i]);
-  }
-  LevelState ex, tot;
-  block_scan<OpLevel>(acc, &ex, &tot, lds);
-  LevelState run = OpLevel::combine(partials[blockIdx.x], ex);
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    run = OpLevel::combine(run, items[i]);  // inclusive
-    uint64_t idx = base + i;
-    if (idx < n) mask_out[src.reverse ? (n - 1 - idx) : idx] = run.mask;
-  }
-}
-
 __global__ void k_fill_words(uint32_t *__restrict__ a, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     a[i] = v;
@@ -212,18 +182,94 @@ void exclusive_scan_u32(Ctx *ctx, const uint32_t *in, uint32_t *out, DevN n, uin
          d_total, nb);
 }
 
-void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask, Arena *arena) {
-  if (n == 0) return;
-  uint32_t nb = div_up(n, kTile);
-  Buf<LevelState> partials(ctx, arena, nb);
-  for (int dir = 0; dir < 2; ++dir) {
-    LevelSource src{v, n, dir == 1};
-    launch(ctx, "scan_reduce_level", k_scan_reduce<OpLevel, LevelSource>, dim3(nb), dim3(kBlock), 0, src,
-           DevN{n, nullptr}, partials.p);
-    launch(ctx, "scan_partials_level", k_scan_partials<OpLevel>, dim3(1), dim3(kBlock), 0, partials.p, (uint64_t)nb);
-    launch(ctx, "scan_final_level", k_scan_final_level, dim3(nb), dim3(kBlock), 0, src, n,
-           (const LevelState *)partials.p, dir == 0 ? left_mask : right_mask);
+// Both boundary-level scans at once.  Tile b covers positions [b * kTile, min((b + 1) *
+// kTile, n)); the left scan walks a tile from its start, the right scan from its end,
+// so the right scan's tile states are stored from the last tile down (pr[nb - 1 - b])
+// and scanned in that order.
+__device__ __forceinline__ LevelState level_item(uint32_t x) { return LevelState{x, 1u << x}; }
+
+__global__ void __launch_bounds__(kBlock) k_level_reduce(const uint8_t *__restrict__ v, uint64_t n,
+                                                          LevelState *__restrict__ pl, LevelState *__restrict__ pr,
+                                                          uint32_t nb) {
+  __shared__ LevelState lds[kBlock / 64];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  const uint64_t len = n - t0 < (uint64_t)kTile ? n - t0 : (uint64_t)kTile;
+  LevelState fa = OpLevel::identity(), ra = OpLevel::identity();
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t j = (uint64_t)threadIdx.x * kItems + i;
+    if (j < len) {
+      fa = OpLevel::combine(fa, level_item(v[t0 + j]));
+      ra = OpLevel::combine(ra, level_item(v[t0 + len - 1 - j]));
+    }
   }
+  LevelState ex, tot;
+  block_scan<OpLevel>(fa, &ex, &tot, lds);
+  if (threadIdx.x == 0) pl[blockIdx.x] = tot;
+  block_scan<OpLevel>(ra, &ex, &tot, lds);
+  if (threadIdx.x == 0) pr[nb - 1 - blockIdx.x] = tot;
+}
+
+// Masks from the exclusive tile prefixes.  With `need` (n bytes and more: the split
+// flags), left[x] is written only where need[x + 1] is set and right[x] only where
+// need[x - 1] is: the positions beside a split boundary, the only ones the node
+// emission reads.
+__global__ void __launch_bounds__(kBlock) k_level_final(const uint8_t *__restrict__ v, uint64_t n,
+                                                         const LevelState *__restrict__ pl,
+                                                         const LevelState *__restrict__ pr, uint32_t nb,
+                                                         const uint8_t *__restrict__ need, uint32_t *__restrict__ left,
+                                                         uint32_t *__restrict__ right) {
+  __shared__ LevelState lds[kBlock / 64];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  const uint64_t len = n - t0 < (uint64_t)kTile ? n - t0 : (uint64_t)kTile;
+  const uint64_t j0 = (uint64_t)threadIdx.x * kItems;
+  LevelState items[kItems];
+  {
+    LevelState acc = OpLevel::identity();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      items[i] = (j0 + i < len) ? level_item(v[t0 + j0 + i]) : OpLevel::identity();
+      acc = OpLevel::combine(acc, items[i]);
+    }
+    LevelState ex, tot;
+    block_scan<OpLevel>(acc, &ex, &tot, lds);
+    LevelState run = OpLevel::combine(pl[blockIdx.x], ex);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      run = OpLevel::combine(run, items[i]);  // inclusive
+      const uint64_t x = t0 + j0 + i;
+      if (j0 + i < len && (!need || (x + 1 < n && need[x + 1]))) left[x] = run.mask;
+    }
+  }
+  {
+    LevelState acc = OpLevel::identity();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      items[i] = (j0 + i < len) ? level_item(v[t0 + len - 1 - j0 - i]) : OpLevel::identity();
+      acc = OpLevel::combine(acc, items[i]);
+    }
+    LevelState ex, tot;
+    block_scan<OpLevel>(acc, &ex, &tot, lds);
+    LevelState run = OpLevel::combine(pr[nb - 1 - blockIdx.x], ex);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      run = OpLevel::combine(run, items[i]);
+      const uint64_t x = t0 + len - 1 - j0 - i;
+      if (j0 + i < len && (!need || (x > 0 && need[x - 1]))) right[x] = run.mask;
+    }
+  }
+}
+
+void level_scans(Ctx *ctx, const uint8_t *v, uint64_t n, uint32_t *left_mask, uint32_t *right_mask,
+                 const uint8_t *need, Arena *arena) {
+  if (n == 0) return;
+  const uint32_t nb = div_up(n, kTile);
+  Buf<LevelState> partials(ctx, arena, 2 * (uint64_t)nb);
+  LevelState *pl = partials.p, *pr = partials.p + nb;
+  launch(ctx, "level_reduce", k_level_reduce, dim3(nb), dim3(kBlock), 0, v, n, pl, pr, nb);
+  launch(ctx, "scan_partials_level", k_scan_partials<OpLevel>, dim3(2), dim3(kBlock), 0, partials.p, (uint64_t)nb);
+  launch(ctx, "level_final", k_level_final, dim3(nb), dim3(kBlock), 0, v, n, (const LevelState *)pl,
+         (const LevelState *)pr, nb, need, left_mask, right_mask);
 }
 
 // ---------------------------------------------------------------------------
