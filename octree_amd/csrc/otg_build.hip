@@ -326,6 +326,31 @@ __device__ __forceinline__ uint64_t upper_bound_u64(const uint64_t *a, uint64_t 
   return lo;
 }
 
+// lower_bound_u64 by galloping: probes 1, 2, 4, ... positions from the low end (fwd)
+// or from the high end (back) of [lo, hi), then a binary search inside the last step.
+// The same index as lower_bound_u64, found with reads near the end the answer is
+// expected at.
+__device__ __forceinline__ uint64_t lower_bound_fwd(const uint64_t *a, uint64_t lo, uint64_t hi, uint64_t x) {
+  uint64_t step = 1;
+  while (lo < hi) {  // a[..lo) < x
+    const uint64_t c = (hi - lo > step) ? lo + step - 1 : hi - 1;
+    if (a[c] >= x) return lower_bound_u64(a, lo, c, x);
+    lo = c + 1;
+    step <<= 1;
+  }
+  return hi;
+}
+__device__ __forceinline__ uint64_t lower_bound_back(const uint64_t *a, uint64_t lo, uint64_t hi, uint64_t x) {
+  uint64_t step = 1;
+  while (hi > lo) {  // a[hi..) >= x
+    const uint64_t c = (hi - lo > step) ? hi - step : lo;
+    if (a[c] < x) return lower_bound_u64(a, c + 1, hi, x);
+    hi = c;
+    step <<= 1;
+  }
+  return lo;
+}
+
 // Bytes of x below n, as their high bits (every byte and n below 128): the per-byte
 // subtraction (x | 0x80) - n never borrows across bytes, so each byte is exact.
 __device__ __forceinline__ uint64_t bytes_below(uint64_t x, uint32_t n) {
@@ -391,8 +416,8 @@ __global__ void __launch_bounds__(kB) k_active(const uint64_t *__restrict__ keys
           // run of one repeated key at depth L: a leaf holding only
           // non-refinable entities unless the run extends beyond the key block
           const uint64_t sb = sp.ent_off, se = sp.ent_off + sp.ent_cnt;
-          uint64_t lo = lower_bound_u64(keys, sb, i, keys[i]);
-          uint64_t hi = upper_bound_u64(keys, i, se, keys[i]);
+          uint64_t lo = lower_bound_back(keys, sb, i, keys[i]);
+          uint64_t hi = keys[i] == ~0ull ? se : lower_bound_fwd(keys, i, se, keys[i] + 1);  // upper bound
           if (v[lo] < vi && v[hi] < vi) a = 0;
         }
       }
@@ -506,8 +531,8 @@ __global__ void __launch_bounds__(kB) k_active_lds(const uint64_t *__restrict__ 
           a = 1;
           if (BOX && keys[i - 1] == keys[i]) {
             const uint64_t sb = sp.ent_off, se = sp.ent_off + sp.ent_cnt;
-            uint64_t lo = lower_bound_u64(keys, sb, i, keys[i]);
-            uint64_t hi = upper_bound_u64(keys, i, se, keys[i]);
+            uint64_t lo = lower_bound_back(keys, sb, i, keys[i]);
+            uint64_t hi = keys[i] == ~0ull ? se : lower_bound_fwd(keys, i, se, keys[i] + 1);
             if (v[lo] < vi && v[hi] < vi) a = 0;
           }
         }
@@ -825,12 +850,12 @@ __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nk
     const uint64_t lo_code = (r >= 64) ? 0ull : (P << r);
     const uint64_t hi_code = (P + 1) << r;
     const uint64_t lo_key = kv.set_base(s) | (lo_code << kv.dbits) | (BOX ? (uint64_t)L : 0ull);
-    a = lower_bound_u64(keys, sb, p, lo_key);
+    a = lower_bound_back(keys, sb, p, lo_key);  // most nodes are small: search near p
     if (hi_code == (1ull << (D * M))) {
       b = se;  // the prefix is the last one of the set
     } else {
       const uint64_t hi_key = kv.set_base(s) + (hi_code << kv.dbits);
-      b = lower_bound_u64(keys, p, se, hi_key);
+      b = lower_bound_fwd(keys, p, se, hi_key);
     }
     if (BOX && keys[a] == lo_key) nr = (uint32_t)(upper_bound_u64(keys, a, b, lo_key) - a);
     depth = L;
@@ -856,7 +881,7 @@ __global__ void __launch_bounds__(kB) k_finalize(const uint64_t *__restrict__ nk
     const uint64_t lo = nk.make(s, gen + 1, 2 * (a - sb));
     const uint64_t hi = nk.make(s, gen + 1, 2 * (b - sb));
     const uint64_t c0 = lower_bound_u64(nkeys, x + 1, nnodes, lo);
-    const uint64_t c1 = lower_bound_u64(nkeys, c0, nnodes, hi);
+    const uint64_t c1 = lower_bound_fwd(nkeys, c0, nnodes, hi);  // at most 2^D children
     ni.cbegin = (uint32_t)c0;
     ni.ccount = (uint32_t)(c1 - c0);
   } else {
