@@ -148,7 +148,7 @@ template <int D, bool BOX>
 __device__ __forceinline__ void encode_point(const double *p, uint64_t i, uint32_t s,
                                              const SetParams *__restrict__ sets, KeyView kv,
                                              uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                             uint32_t *__restrict__ err) {
+                                             uint32_t *__restrict__ err, uint32_t ibits, uint64_t s_off) {
   const SetParams &sp = sets[s];
   const uint32_t M = sp.max_depth;
   uint32_t g[D];
@@ -221,9 +221,13 @@ __device__ __forceinline__ void encode_point(const double *p, uint64_t i, uint32
       depth = M - level;
     }
   }
-  keys[i] = kv.set_base(s) | (code << kv.dbits) | (BOX ? (uint64_t)depth : 0ull);
-  keys[i] = kv.set_base(s) | (code << kv.dbits) | (BOX ? (uint64_t)depth : 0ull);
-  vals[i] = (uint32_t)i;
+  const uint64_t low = (code << kv.dbits) | (BOX ? (uint64_t)depth : 0ull);
+  if (ibits) {
+    keys[i] = (low << ibits) | (i - s_off);  // packed for the per-set sort
+  } else {
+    keys[i] = kv.set_base(s) | low;
+    vals[i] = (uint32_t)i;
+  }
 }
 
 // One block encodes kB * kPointItems consecutive points (item r of thread t: point
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, 
                                                 const uint64_t *__restrict__ set_off, uint32_t nsets,
                                                 const SetParams *__restrict__ sets, KeyView kv,
                                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                uint32_t *__restrict__ err) {
+                                                uint32_t *__restrict__ err, uint32_t ibits) {
   constexpr int S = BOX ? 2 * D : D;
   constexpr int E = kPointItems<S>;
   const uint64_t base = (uint64_t)blockIdx.x * (kB * E);
@@ -253,7 +257,7 @@ __global__ void __launch_bounds__(kB) k_encode(const double *__restrict__ geom, 
     const uint64_t i = base + (uint64_t)r * kB + threadIdx.x;
     if (i >= n) break;
     while (s + 1 < nsets && set_off[s + 1] <= i) ++s;
-    encode_point<D, BOX>(c[r], i, s, sets, kv, keys, vals, err);
+    encode_point<D, BOX>(c[r], i, s, sets, kv, keys, vals, err, ibits, set_off[s]);
   }
 }
 
@@ -1229,13 +1233,21 @@ static void encode_sort(Tree *t, const double *dgeom, const uint64_t *d_off, uin
     Buf<uint32_t> v2(ctx, ar, n);
     uint64_t *ka = odd ? k2.p : t->d_keys, *kb = odd ? t->d_keys : k2.p;
     uint32_t *va = odd ? v2.p : vals, *vb = odd ? vals : v2.p;
+    // the index inside the set rides below the key bits when both fit one word: 8 bytes
+    // per point and pass instead of 12, values written by the last pass only
+    uint64_t max_n = 1;
+    for (const SetParams &sp : t->sets) max_n = std::max<uint64_t>(max_n, sp.ent_cnt);
+    uint32_t ibits = 1;
+    while (ibits < 32 && (max_n - 1) >> ibits) ++ibits;
+    if (low + ibits > 64) ibits = 0;
+    const uint32_t set_shift = t->layout.cbits + t->layout.dbits;
     launch(ctx, "encode", k_encode<D, BOX>, grid, dim3(kB), 0, dgeom, n, d_off, nsets, (const SetParams *)t->d_sets,
-           kv, ka, va, err);
-    radix_sort_sets(ctx, ka, va, kb, vb, n, d_off, nsets, low, ar);
+           kv, ka, va, err, ibits);
+    radix_sort_sets(ctx, ka, va, kb, vb, n, d_off, nsets, low, ar, ibits, set_shift >= 64 ? 64u : set_shift);
     return;
   }
   launch(ctx, "encode", k_encode<D, BOX>, grid, dim3(kB), 0, dgeom, n, d_off, nsets, (const SetParams *)t->d_sets, kv,
-         t->d_keys, vals, err);
+         t->d_keys, vals, err, 0u);
   radix_sort_pairs(ctx, t->d_keys, vals, n, t->layout.total, ar);
 }
 
@@ -2016,7 +2028,7 @@ static void build_seq_impl(Tree *t, const double *geom_in, int mem, uint64_t n, 
   Buf<uint32_t> order(ctx, n), err(ctx, 1);
   OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, ctx->stream));
   launch(ctx, "encode", k_encode<D, BOX>, dim3(div_up(n, kB * kPointItems<BOX ? 2 * D : D>)), dim3(kB), 0, dgeom, n, (const uint64_t *)d_off.p, 1u,
-         (const SetParams *)t->d_sets, kv, t->d_keys, order.p, err.p);
+         (const SetParams *)t->d_sets, kv, t->d_keys, order.p, err.p, 0u);
   Buf<uint32_t> ff(ctx, n), bt(ctx, n), kid_cnt(ctx, n), scan(ctx, n + 1), total(ctx, 1);
   Buf<uint64_t> kid_key(ctx, n);
   Buf<uint8_t> kid_depth(ctx, n);
@@ -2454,7 +2466,7 @@ static void load_impl(Tree *t, const uint8_t *buf, uint64_t size, bool large, ui
     Buf<uint64_t> keys(ctx, ne);
     Buf<uint32_t> vals(ctx, ne);
     launch(ctx, "encode", k_encode<D, true>, dim3(div_up(ne, kB * kPointItems<2 * D>)), dim3(kB), 0, dgeom, ne, (const uint64_t *)d_off.p,
-           1u, (const SetParams *)t->d_sets, kv, keys.p, vals.p, err.p + 1);
+           1u, (const SetParams *)t->d_sets, kv, keys.p, vals.p, err.p + 1, 0u);
     t->d_keys = (uint64_t *)ctx->alloc(ne * 8);
     launch(ctx, "loaded_keys", k_loaded_keys, dim3(div_up(ne, kB)), dim3(kB), 0, (const uint64_t *)keys.p,
            (const uint32_t *)t->d_ids, ne, t->d_keys);

@@ -220,8 +220,12 @@ void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, DevN n, uint32_t
 // radix_sets_fit says whether the per-set digit tables are small enough and the sets
 // few enough against the tile count for the per-set sort to pay.
 bool radix_sets_fit(uint64_t n, uint32_t nsets, uint32_t bits);
+// Packed mode (ibits > 0, bits + ibits <= 64): ka holds words (key bits << ibits | index
+// of the point inside its set), va is not read, and the last pass writes the keys
+// (set << set_shift | key bits; set_shift 64: none) and the global indices.
 void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb, uint64_t n,
-                     const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena);
+                     const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena, uint32_t ibits = 0,
+                     uint32_t set_shift = 64);
 inline void radix_sort_pairs(Ctx *ctx, uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t bits,
                              Arena *arena = nullptr) {
   radix_sort_pairs(ctx, keys, vals, DevN{n, nullptr}, bits, arena);

@@ -359,6 +359,10 @@ struct OsSegments {
   const uint32_t *base;
   const uint32_t *ntiles;  // tile_base + nsets
   uint32_t passes;
+  // packed words (ibits > 0): sort bits above the low ibits, which hold the index inside
+  // the set; the last pass writes (set << set_shift | word >> ibits, set start + index)
+  uint32_t ibits;
+  uint32_t set_shift;  // 64: no set bits
 };
 
 __global__ void __launch_bounds__(kBlock) k_seg_tile_counts(const uint64_t *__restrict__ off, uint32_t nsets,
@@ -404,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_seg_hist(const uint64_t *__restrict_
     const uint64_t e = min(b + (uint64_t)kOsTile, sg.off[s + 1]);
     for (uint64_t i = b + threadIdx.x; i < e; i += kBlock) {
       const uint64_t k = keys[i];
-      for (uint32_t p = 0; p < sg.passes; ++p) atomicAdd(&h[p][(uint32_t)(k >> (8 * p)) & 255u], 1u);
+      for (uint32_t p = 0; p < sg.passes; ++p) atomicAdd(&h[p][(uint32_t)(k >> (sg.ibits + 8 * p)) & 255u], 1u);
     }
   }
   __syncthreads();
@@ -442,7 +446,9 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   __shared__ uint32_t sval[kOsTile];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const uint32_t shift = 8 * pass;
+  const uint32_t shift = 8 * pass + sg.ibits;
+  const bool packed = sg.ibits != 0;  // (key, index) words, no values until the last pass
+  const bool unpack = packed && pass + 1 == sg.passes;
   const bool digit_thread = threadIdx.x < 256;  // threads 0..255 own one digit each
   if (threadIdx.x == 0) s_tile = atomicAdd(&st->tile_counter[pass], 1u);
   if (digit_thread)
@@ -454,9 +460,11 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
   uint64_t tile0, n;
   bool first;
   const uint32_t *digit_base;
+  uint32_t seg = 0;
   if (sg.base) {
     if (tile_id >= *sg.ntiles) return;  // block-uniform
     const uint32_t s = sg.tile_set[tile_id];
+    seg = s;
     const uint32_t local = tile_id - sg.tile_base[s];
     tile0 = sg.off[s] + (uint64_t)local * kOsTile;
     n = min(tile0 + (uint64_t)kOsTile, sg.off[s + 1]);
@@ -481,7 +489,7 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
     uint64_t i = tile + (uint64_t)r * 64 + lane;
     bool valid = i < n;
     key[r] = valid ? kin[i] : 0ull;
-    val[r] = valid ? vin[i] : 0u;
+    val[r] = (valid && !packed) ? vin[i] : 0u;
     uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
     dig[r] = d;
     uint64_t peers = __ballot(valid);
@@ -576,7 +584,7 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
     if (i < n) {
       const uint32_t lp = tstart[dig[r]] + wcnt[wave][dig[r]] + off[r];
       skey[lp] = key[r];
-      sval[lp] = val[r];
+      if (!packed) sval[lp] = val[r];
     }
   }
   __syncthreads();
@@ -588,8 +596,16 @@ __global__ void __launch_bounds__(kOsBlock) k_os_scatter(const uint64_t *__restr
       const uint64_t k = skey[j];
       const uint32_t d = (uint32_t)(k >> shift) & 255u;
       const uint32_t pos = base_off[d] + (j - tstart[d]);
-      kout[pos] = k;
-      vout[pos] = sval[j];
+      if (!packed) {
+        kout[pos] = k;
+        vout[pos] = sval[j];
+      } else if (!unpack) {
+        kout[pos] = k;
+      } else {
+        const uint64_t set_key = sg.set_shift >= 64 ? 0ull : (uint64_t)seg << sg.set_shift;
+        kout[pos] = set_key | (k >> sg.ibits);
+        vout[pos] = (uint32_t)(sg.off[seg] + (k & ((1ull << sg.ibits) - 1ull)));
+      }
     }
   }
 }
@@ -633,7 +649,8 @@ bool radix_sets_fit(uint64_t n, uint32_t nsets, uint32_t bits) {
 }
 
 void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_t *vb, uint64_t n,
-                     const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena) {
+                     const uint64_t *d_off, uint32_t nsets, uint32_t bits, Arena *arena, uint32_t ibits,
+                     uint32_t set_shift) {
   if (n == 0 || bits == 0) return;
   if (n > kOsCount) throw Error{OTG_ERR_UNSUPPORTED, "radix sort of 2^30 or more keys"};
   if (!radix_sets_fit(n, nsets, bits)) throw Error{OTG_ERR_INVALID_ARG, "per-set radix tables too large (check radix_sets_fit first)"};
@@ -649,7 +666,8 @@ void radix_sort_sets(Ctx *ctx, uint64_t *ka, uint32_t *va, uint64_t *kb, uint32_
   exclusive_scan_u32(ctx, tcount.p, tbase.p, nsets, tbase.p + nsets, arena);
   launch(ctx, "seg_tile_sets", k_seg_tile_sets, dim3(nsets), dim3(kBlock), 0, (const uint32_t *)tbase.p, tset.p);
   fill_u32(ctx, base.p, (uint64_t)nsets * passes * 256, 0u);
-  const OsSegments sg{d_off, tbase.p, tset.p, base.p, tbase.p + nsets, passes};
+  if (ibits && (ibits >= 32 || bits + ibits > 64)) throw Error{OTG_ERR_INVALID_ARG, "packed radix words exceed 64 bits"};
+  const OsSegments sg{d_off, tbase.p, tset.p, base.p, tbase.p + nsets, passes, ibits, set_shift};
   launch(ctx, "radix_init", k_os_init, dim3(std::min<uint32_t>(div_up((uint64_t)256 * ntiles, kBlock), 1024)),
          dim3(kBlock), 0, st.p, stat[0], (uint64_t)256 * ntiles);
   launch(ctx, "seg_hist", k_seg_hist, dim3(div_up(ntiles, kSegHistTiles)), dim3(kBlock), 0, (const uint64_t *)ka,
