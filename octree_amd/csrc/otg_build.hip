@@ -982,8 +982,7 @@ __global__ void k_ref_sets(const SetParams *__restrict__ fsets, SetParams *__res
 // sorted SoA geometry and local ids
 template <int D, bool BOX>
 __global__ void __launch_bounds__(kB) k_gather(const double *__restrict__ geom, const uint32_t *__restrict__ vals, uint64_t n,
-                                                const uint64_t *__restrict__ keys, KeyView kv,
-                                                const uint64_t *__restrict__ set_off,
+                                                const uint64_t *__restrict__ set_off, uint32_t nsets,
                                                 double *__restrict__ sorted, uint32_t *__restrict__ ids) {
   uint64_t j = (uint64_t)blockIdx.x * kB + threadIdx.x;
   if (j >= n) return;
@@ -991,7 +990,8 @@ __global__ void __launch_bounds__(kB) k_gather(const double *__restrict__ geom, 
   uint32_t id = vals[j];
 #pragma unroll
   for (int c = 0; c < S; ++c) sorted[(uint64_t)c * n + j] = geom[(uint64_t)id * S + c];
-  ids[j] = id - (uint32_t)set_off[kv.set(keys[j])];
+  // the sets keep their offsets through the sort: position j is in the set of index j
+  ids[j] = id - (uint32_t)set_off[set_of_wave(set_off, nsets, j)];
 }
 
 // ---------------------------------------------------------------------------
@@ -1443,16 +1443,14 @@ static void reference_from_finer(Tree *t, const Tree *aux, Arena *ar = nullptr, 
 // finer index, whose order is the storage's; null: t).
 template <int D, bool BOX>
 static void gather_entities(Tree *t, const double *dgeom, const uint32_t *vals, const uint64_t *d_off,
-                            Arena *ar = nullptr, const Tree *keys_of = nullptr) {
+                            Arena *ar = nullptr) {
   Ctx *ctx = t->ctx;
   constexpr int S = BOX ? 2 * D : D;
   const uint64_t n = t->n;
-  const Tree *kt = keys_of ? keys_of : t;
-  const KeyView kv{kt->layout.dbits, kt->layout.cbits};
   t->d_geom = (double *)build_alloc(ctx, ar, n * S * 8);
   t->d_ids = (uint32_t *)build_alloc(ctx, ar, n * 4);
-  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, vals, n,
-         (const uint64_t *)kt->d_keys, kv, d_off, t->d_geom, t->d_ids);
+  launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, vals, n, d_off,
+         (uint32_t)t->sets.size(), t->d_geom, t->d_ids);
 }
 
 template <int D>
@@ -1573,7 +1571,7 @@ static void build_impl(Tree *t, const double *geom_in, int mem, const uint64_t *
       node_table<D, false>(aux, d_off.p, &herr);
       if (same_caps) reference_from_finer<D>(t, aux);
       else node_table<D, false>(t, d_off.p, nullptr);
-      gather_entities<D, false>(t, dgeom, vals.p, d_off.p, nullptr, aux);
+      gather_entities<D, false>(t, dgeom, vals.p, d_off.p);
       aux->d_geom = t->d_geom;
       aux->d_ids = t->d_ids;
       aux->shares_entities = true;
@@ -2091,7 +2089,7 @@ static void build_seq_impl(Tree *t, const double *geom_in, int mem, uint64_t n, 
   t->d_geom = (double *)ctx->alloc(n * S * 8);
   t->d_ids = (uint32_t *)ctx->alloc(n * 4);
   launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(n, kB)), dim3(kB), 0, dgeom, (const uint32_t *)order.p, n,
-         (const uint64_t *)t->d_keys, kv, (const uint64_t *)d_off.p, t->d_geom, t->d_ids);
+         (const uint64_t *)d_off.p, 1u, t->d_geom, t->d_ids);
   if (BOX) build_mbr<D>(t);
   SeqItem r0;
   OTG_HIP_CHECK(hipMemcpyAsync(&r0, items.p, sizeof(SeqItem), hipMemcpyDeviceToHost, ctx->stream));
@@ -2449,15 +2447,13 @@ static void load_impl(Tree *t, const uint8_t *buf, uint64_t size, bool large, ui
     dgeom = dgeom_own.p;
   }
   const KeyView kv{t->layout.dbits, t->layout.cbits};
-  Buf<uint64_t> zero_key(ctx, ne);  // k_gather reads set ids from keys: one set, zero keys will do
-  OTG_HIP_CHECK(hipMemsetAsync(zero_key.p, 0, ne * 8, st));
   Buf<uint64_t> d_off(ctx, 2);
   const uint64_t offs[2] = {0, ne};
   OTG_HIP_CHECK(hipMemcpyAsync(d_off.p, offs, 16, hipMemcpyHostToDevice, st));
   Buf<uint32_t> ids_copy(ctx, ne);
   OTG_HIP_CHECK(hipMemcpyAsync(ids_copy.p, t->d_ids, ne * 4, hipMemcpyDeviceToDevice, st));
   launch(ctx, "gather", k_gather<D, BOX>, dim3(div_up(ne, kB)), dim3(kB), 0, dgeom, (const uint32_t *)ids_copy.p, ne,
-         (const uint64_t *)zero_key.p, kv, (const uint64_t *)d_off.p, t->d_geom, t->d_ids);
+         (const uint64_t *)d_off.p, 1u, t->d_geom, t->d_ids);
   Buf<uint32_t> err(ctx, 2);
   OTG_HIP_CHECK(hipMemsetAsync(err.p, 0, 8, st));
   launch(ctx, "loaded_check", k_loaded_check<D, BOX>, dim3(div_up(nn, kB)), dim3(kB), 0, (const NodeInfo *)t->d_info,
