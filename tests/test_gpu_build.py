@@ -105,6 +105,27 @@ def test_c2_size_boxes(gpu_ctx):
     _parity(gpu_ctx, "box", 3, b, None, 8, 32)
 
 
+@pytest.mark.parametrize("depth", [10, 18])
+def test_forest_parity_sort_words(gpu_ctx, depth):
+    """Forest whose sort runs inside each set: at depth 10 the 30 location bits and the
+    16-bit index inside a set share one word; at depth 18 (54 bits) they do not and the
+    sort moves (key, index) pairs.  Both tables equal one oracle build per set."""
+    import octree_amd
+
+    D = 3
+    # six sets: 3 set-id bits push the whole key past the location bits' last byte, so
+    # the sort runs inside the sets; more sort tiles than sets
+    sizes = [40000, 35000, 41000, 38000, 30000, 36000]
+    sets = [gaussian_mixture(n, D, 8, 0.0, 1000.0, 2.0, 20.0, seed=40 + i) for i, n in enumerate(sizes)]
+    caps = np.array([4, 16, 8, 1, 32, 2], dtype=np.uint32)
+    depths = np.full(len(sizes), depth, dtype=np.uint32)
+    boxes = np.tile([[0.0] * D + [1000.0] * D], (len(sizes), 1))
+    f = octree_amd.StaticOctreePoint.forest(sets, caps, depths, boxes, ctx=gpu_ctx)
+    for s, geom in enumerate(sets):
+        o = O.OracleTree(D, "point", geom, boxes[s], depth, int(caps[s]))
+        compare_tables(f.export(s), o.export(), False)
+
+
 @pytest.mark.parametrize("kind", ["point", "box"])
 def test_forest_parity(gpu_ctx, kind):
     """Batched build of uneven sets equals one oracle build per set."""
